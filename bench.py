@@ -1,0 +1,230 @@
+#!/usr/bin/env python3
+"""Benchmark of the ntt-hip engine on one MI355X.
+
+A step is one bit-exact 2^22 x 2^22-limb multiply (transform length
+N = 2^24) through the library's device-resident entry point, operands and
+product resident in HBM (configs: "Large single multiply", seeded uniform
+limbs, top limb nonzero). The timed region is exactly K steps bracketed by
+synchronisation; the product of the last step is checked bit for bit
+against the golden SHA-256 produced by the reference's own ntt-cpu.
+
+The JSON line carries:
+  value / unit        multiplies per second (whole job, one GPU)
+  ms_per_step         ms per multiply
+  roofline            the dominant kernel's algorithmic bytes per launch over
+                      its mean launch time (hipEvents around each launch on
+                      the stream it runs on, in a second timed pass of K steps)
+  roofline_e2e        B_mul = 96 N + 8 (na + nb) bytes over the whole
+                      multiply time, against the 8 TB/s HBM peak
+  cpu_baseline        the reference's ntt-cpu (oracle/_ref, built from the
+                      reference sources) or its restatement, single thread,
+                      on one 2^22-limb multiply
+  extra               the 60,000-limb multiply and BBP positions 1..65536
+
+    python bench.py [--steps K] [--warmup W] [--no-cpu-baseline]
+"""
+from __future__ import annotations
+
+import argparse
+import hashlib
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "cuda-pi-chudnovsky-bbp_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+HBM_PEAK = 8.0e12
+METRIC = "bit-exact 2^22-limb NTT mul ms (% HBM roofline); batched vs plain mul/s"
+
+
+def alg_bytes(kernel: str, na: int, nb: int, N: int) -> float:
+    """Algorithmic HBM bytes one launch of `kernel` must move for one multiply."""
+    nl = na + nb
+    return {
+        "k_col_fwd": 4 * na + 8 * N,          # limbs in, N-point spectrum out (per operand)
+        "k_row_mul": 24 * N,                  # two spectra in, one out
+        "k_row_sqr": 16 * N,
+        "k_col_inv": 16 * N,
+        "k_carry_reduce": 16 * nl,            # 2 nl coefficients in
+        "k_carry_apply": 16 * nl + 4 * nl,    # coefficients in, limbs out
+    }.get(kernel, 0.0)
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(a, b):
+    import oracle
+    kind = "reference" if oracle.ref_available() else "port"
+    fn = oracle.ref_mul_ntt if kind == "reference" else oracle.mul_ntt
+    t = time.perf_counter()
+    fn(a, b)
+    dt = time.perf_counter() - t
+    return {"value": 1.0 / dt, "unit": "mul/s", "cores": 1, "kind": kind,
+            "sample": f"one 2^22 x 2^22-limb multiply, single-threaded ntt-cpu, {dt:.2f} s on {cpu_model()}"}
+
+
+def read_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_c2_mul.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.split()
+        if len(parts) >= 3 and parts[0] == kernel and not line.startswith("#"):
+            try:
+                return float(parts[1]) + float(parts[2])
+            except ValueError:
+                return None
+    return None
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extra", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(a single multiply or BBP job runs on one device); run with --gpus 1.", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+    import pdh
+    import workloads as W
+
+    if not pdh.device_available():
+        print("no usable gfx950 device: " + pdh.lib().pdh_last_error().decode(), file=sys.stderr)
+        return 1
+    golden = json.load(open(os.path.join(ROOT, "tests", "golden", "golden.json")))
+    g = golden["c2_mul"]
+    na = nb = W.C2_LIMBS
+    N = pdh.ntt_length(na, nb)
+    a, b = W.operand(g["seed_a"], na), W.operand(g["seed_b"], nb)
+    dev = torch.device("cuda:0")
+    da = torch.from_numpy(a.view(np.int32)).to(dev)
+    db = torch.from_numpy(b.view(np.int32)).to(dev)
+    out = torch.zeros(na + nb, dtype=torch.int32, device=dev)
+    ln = torch.zeros(1, dtype=torch.int64, device=dev)
+    stream = torch.cuda.current_stream()
+    sp = stream.cuda_stream
+
+    def step():
+        pdh.mul_dev(da.data_ptr(), na, db.data_ptr(), nb, out.data_ptr(), ln.data_ptr(), sp)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+
+    # timed region: exactly K steps
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(args.steps):
+        step()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    total_ms = e0.elapsed_time(e1)
+    ms = total_ms / args.steps
+
+    n = int(ln.item())
+    got = out[:n].cpu().numpy().view(np.uint32)
+    exact = n == g["len"] and hashlib.sha256(got.astype("<u4").tobytes()).hexdigest() == g["sha256"]
+
+    # second pass with per-kernel events for the roofline
+    pdh.timing_enable(True)
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    times = pdh.timing_read()
+    pdh.timing_enable(False)
+    kernels = {}
+    for name, (tms, cnt) in times.items():
+        avg = tms / cnt
+        B = alg_bytes(name, na, nb, N)
+        kernels[name] = {"avg_us": round(avg * 1e3, 2), "launches_per_mul": cnt / args.steps,
+                         "alg_bytes": B, "frac": round(B / (avg * 1e-3) / HBM_PEAK, 4) if B else None}
+    dom = max(times, key=lambda k: times[k][0])
+    dom_avg_s = times[dom][0] / times[dom][1] * 1e-3
+    dom_B = alg_bytes(dom, na, nb, N)
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": dom_B / dom_avg_s / 1e9, "peak": HBM_PEAK / 1e9,
+                "unit": "GB/s", "frac": dom_B / dom_avg_s / HBM_PEAK, "traffic": read_traffic(dom),
+                "alg_bytes_per_launch": dom_B}
+    B_mul = 96 * N + 8 * (na + nb)
+    roofline_e2e = {"bound": "hbm", "alg_bytes": B_mul, "achieved": B_mul / (ms * 1e-3) / 1e9,
+                    "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": B_mul / (ms * 1e-3) / HBM_PEAK}
+
+    extra = {}
+    if not args.no_extra:
+        # 60,000-limb multiply (the reference's CPU-runnable case), resident
+        c1 = golden["c1_mul"]
+        a1, b1 = W.operand(c1["seed_a"], c1["limbs"]), W.operand(c1["seed_b"], c1["limbs"])
+        d1a = torch.from_numpy(a1.view(np.int32)).to(dev)
+        d1b = torch.from_numpy(b1.view(np.int32)).to(dev)
+        o1 = torch.zeros(len(a1) + len(b1), dtype=torch.int32, device=dev)
+        for _ in range(3):
+            pdh.mul_dev(d1a.data_ptr(), len(a1), d1b.data_ptr(), len(b1), o1.data_ptr(), ln.data_ptr(), sp)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        reps = 50
+        for _ in range(reps):
+            pdh.mul_dev(d1a.data_ptr(), len(a1), d1b.data_ptr(), len(b1), o1.data_ptr(), ln.data_ptr(), sp)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        n1 = int(ln.item())
+        g1 = o1[:n1].cpu().numpy().view(np.uint32)
+        extra["c1_60000_limbs"] = {
+            "ms_per_mul": e0.elapsed_time(e1) / reps,
+            "bit_exact": n1 == c1["len"] and hashlib.sha256(g1.astype("<u4").tobytes()).hexdigest() == c1["sha256"]}
+        # BBP positions 1..65536
+        import oracle
+        lo, hi = W.C4_RANGE
+        cnt = hi - lo + 1
+        vals = torch.zeros(cnt, dtype=torch.uint8, device=dev)
+        pos = torch.arange(lo, hi + 1, dtype=torch.int64, device=dev)
+        pdh.bbp_positions_dev(pos.data_ptr(), 64, vals.data_ptr(), sp)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        pdh.bbp_positions_dev(pos.data_ptr(), cnt, vals.data_ptr(), sp)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        bms = e0.elapsed_time(e1)
+        want = open(os.path.join(ROOT, "tests", "golden", golden["bbp_range"]["file"])).read().strip()
+        M = sum(oracle.bbp_mulmod_count(p) for p in range(lo, hi + 1))
+        extra["bbp_1_65536"] = {"ms": bms, "mulmod_per_s": M / (bms * 1e-3), "mulmods": M,
+                                "bit_exact": pdh.to_hex(vals.cpu().numpy()) == want}
+
+    cpu = None if args.no_cpu_baseline else cpu_baseline(a, b)
+    line = {
+        "metric": METRIC, "value": 1e3 / ms, "unit": "mul/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u64 (GF(2^64-2^32+1))", "data": "synthetic (seeded uniform limbs)",
+        "config": {"workload": "2^22 x 2^22-limb multiply, N = 2^24, operands resident in HBM",
+                   "na": na, "nb": nb, "ntt_length": N, "parallelism": "single GPU"},
+        "bit_exact": exact, "wall_s_timed": wall,
+        "roofline": roofline, "roofline_e2e": roofline_e2e, "kernels": kernels,
+        "cpu_baseline": cpu, "extra": extra,
+    }
+    print(json.dumps(line))
+    return 0 if exact else 3
+
+
+if __name__ == "__main__":
+    sys.exit(main())

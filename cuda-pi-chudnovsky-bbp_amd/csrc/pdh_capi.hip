@@ -1,0 +1,400 @@
+// ============================================================================
+//  pdh_capi.hip -- the C ABI (include/pdh_capi.h): device selection, twiddle
+//  tables, workspace, host<->device staging and error reporting.
+//
+//  Memory: the library keeps one grow-only workspace per process on the
+//  current device (two N-point u64 spectra, the carry scratch, and staging
+//  for the host-buffer entry points) plus the twiddle tables, so repeated
+//  calls allocate nothing once the largest size has been seen. The
+//  reference allocates and frees 24N bytes on every call
+//  (src/ntt_cuda.cu:198-201,237).
+//
+//  Calls are serialised by a mutex. Device-resident calls on different
+//  streams share the one workspace, so they must not overlap in time on the
+//  device; the host entry points synchronise before returning.
+// ============================================================================
+#include "pdh_capi.h"
+#include "gl_field.h"
+#include "pdh_internal.h"
+
+#include <mutex>
+#include <string>
+#include <vector>
+#include <map>
+#include <numeric>
+#include <algorithm>
+#include <cstring>
+#include <cstdio>
+
+namespace pdh {
+namespace {
+
+thread_local std::string t_err;
+
+int fail(int code, const std::string& msg) { t_err = msg; return code; }
+int fail_hip(hipError_t e, const char* what) {
+    return fail(PDH_E_HIP, std::string("[hip] ") + what + " failed: " + hipGetErrorString(e));
+}
+
+#define HIPCALL(expr) do { hipError_t _e = (expr); if (_e != hipSuccess) return fail_hip(_e, #expr); } while (0)
+
+struct Outer { u64* hi = nullptr; u64* lo = nullptr; u64* hi_inv = nullptr; u64* lo_inv = nullptr; int h = 0; };
+
+struct Ctx {
+    std::mutex mu;
+    int state = 0;          // 0 unknown, 1 ready, -1 no device
+    std::string why;
+    u64* twL = nullptr;     // 2^14 entries: twL[2^L + x] = w_{2^L}^x
+    std::map<int, Outer> outer;
+    u64* wsA = nullptr; u64* wsB = nullptr; size_t ws_words = 0;
+    u64* carry = nullptr; size_t carry_bytes = 0;
+    u32* st_a = nullptr; u32* st_b = nullptr; u32* st_out = nullptr; size_t st_limbs = 0;
+    u64* st_len = nullptr;
+    uint64_t* st_pos = nullptr; uint8_t* st_val = nullptr; size_t st_bbp = 0;
+    hipStream_t stream = nullptr;
+};
+
+Ctx& ctx() { static Ctx c; return c; }
+
+int init_locked(Ctx& c) {
+    if (c.state == 1) return PDH_OK;
+    if (c.state == -1) return fail(PDH_E_NODEVICE, c.why);
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev <= 0) {
+        c.state = -1;
+        c.why = "[hip] no HIP device available";
+        return fail(PDH_E_NODEVICE, c.why);
+    }
+    int dev = 0;
+    HIPCALL(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HIPCALL(hipGetDeviceProperties(&prop, dev));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        c.state = -1;
+        c.why = std::string("[hip] device is ") + prop.gcnArchName + ", this library is built for gfx950";
+        return fail(PDH_E_NODEVICE, c.why);
+    }
+    HIPCALL(hipStreamCreate(&c.stream));
+    // twL[2^L + x] = w_{2^L}^x, L = 1..13
+    std::vector<u64> tw((size_t)1 << 14, 0);
+    for (int L = 1; L <= 13; ++L) {
+        const u64 w = gl_root_host(L);
+        u64 v = 1;
+        for (size_t x = 0; x < ((size_t)1 << L); ++x) { tw[((size_t)1 << L) + x] = v; v = gl_mul(v, w); }
+    }
+    HIPCALL(hipMalloc(&c.twL, tw.size() * sizeof(u64)));
+    HIPCALL(hipMemcpy(c.twL, tw.data(), tw.size() * sizeof(u64), hipMemcpyHostToDevice));
+    c.state = 1;
+    return PDH_OK;
+}
+
+int outer_for(Ctx& c, int n, OuterTables* ot) {
+    auto it = c.outer.find(n);
+    if (it == c.outer.end()) {
+        Outer o;
+        o.h = (n + 1) / 2;
+        const size_t nlo = (size_t)1 << o.h, nhi = (size_t)1 << (n - o.h);
+        const u64 w = gl_root_host(n), wi = gl_inv(w);
+        const u64 ninv = gl_inv((u64)1 << n);
+        std::vector<u64> lo(nlo), lo_inv(nlo), hi(nhi), hi_inv(nhi);
+        u64 v = 1, vi = 1;
+        for (size_t x = 0; x < nlo; ++x) { lo[x] = v; lo_inv[x] = vi; v = gl_mul(v, w); vi = gl_mul(vi, wi); }
+        const u64 step = v, step_i = vi;              // w^(2^h), w^(-2^h)
+        v = 1; vi = ninv;
+        for (size_t y = 0; y < nhi; ++y) { hi[y] = v; hi_inv[y] = vi; v = gl_mul(v, step); vi = gl_mul(vi, step_i); }
+        HIPCALL(hipMalloc(&o.lo, nlo * 8));
+        HIPCALL(hipMalloc(&o.lo_inv, nlo * 8));
+        HIPCALL(hipMalloc(&o.hi, nhi * 8));
+        HIPCALL(hipMalloc(&o.hi_inv, nhi * 8));
+        HIPCALL(hipMemcpy(o.lo, lo.data(), nlo * 8, hipMemcpyHostToDevice));
+        HIPCALL(hipMemcpy(o.lo_inv, lo_inv.data(), nlo * 8, hipMemcpyHostToDevice));
+        HIPCALL(hipMemcpy(o.hi, hi.data(), nhi * 8, hipMemcpyHostToDevice));
+        HIPCALL(hipMemcpy(o.hi_inv, hi_inv.data(), nhi * 8, hipMemcpyHostToDevice));
+        it = c.outer.emplace(n, o).first;
+    }
+    ot->hi = it->second.hi; ot->lo = it->second.lo;
+    ot->hi_inv = it->second.hi_inv; ot->lo_inv = it->second.lo_inv; ot->h = it->second.h;
+    return PDH_OK;
+}
+
+template <class P>
+int grow(P** p, size_t* have, size_t need, size_t elem) {
+    if (*have >= need) return PDH_OK;
+    if (*p) HIPCALL(hipFree(*p));
+    *p = nullptr;
+    *have = 0;
+    HIPCALL(hipMalloc((void**)p, need * elem));
+    *have = need;
+    return PDH_OK;
+}
+
+int ensure_ws(Ctx& c, size_t N, size_t nlimbs) {
+    if (c.ws_words < N) {
+        if (c.wsA) HIPCALL(hipFree(c.wsA));
+        if (c.wsB) HIPCALL(hipFree(c.wsB));
+        c.wsA = c.wsB = nullptr;
+        c.ws_words = 0;
+        HIPCALL(hipMalloc(&c.wsA, N * 8));
+        HIPCALL(hipMalloc(&c.wsB, N * 8));
+        c.ws_words = N;
+    }
+    return grow(&c.carry, &c.carry_bytes, carry_ws_bytes(nlimbs), 1);
+}
+
+int log2_exact(size_t N) { int l = 0; while (((size_t)1 << l) < N) ++l; return l; }
+
+int mul_dev_locked(Ctx& c, const u32* a, size_t na, const u32* b, size_t nb, bool square,
+                   u32* out, u64* out_len, hipStream_t s) {
+    if (na == 0 || (!square && nb == 0)) {
+        hipError_t e = launch_zero_len(out_len, s);
+        if (e != hipSuccess) return fail_hip(e, "zero-length launch");
+        return PDH_OK;
+    }
+    if (square) { b = a; nb = na; }
+    const size_t N = pdh_ntt_length(na, nb);
+    const int n = log2_exact(N);
+    if (n > MAX_LOG) return fail(PDH_E_TOO_LARGE, "transform length 2^" + std::to_string(n) + " exceeds 2^24");
+    int rc = ensure_ws(c, N, na + nb);
+    if (rc) return rc;
+    OuterTables ot{nullptr, nullptr, nullptr, nullptr, 0};
+    const NttShape sh = ntt_shape(n);
+    if (sh.L1 > 0 && (rc = outer_for(c, n, &ot))) return rc;
+    hipError_t e = launch_mul(a, na, b, nb, square, out, out_len, c.wsA, c.wsB, c.carry, c.twL, ot, sh, s);
+    if (e != hipSuccess) return fail_hip(e, "multiply launch");
+    return PDH_OK;
+}
+
+int mul_host(const u32* a, size_t na, const u32* b, size_t nb, bool square,
+             u32* out, size_t out_cap, size_t* out_len) {
+    if (!out_len || (na && !a) || (!square && nb && !b)) return fail(PDH_E_ARG, "null pointer argument");
+    if (square) nb = na;
+    if (na == 0 || nb == 0) { *out_len = 0; return PDH_OK; }
+    if (!out || out_cap < na + nb) return fail(PDH_E_CAPACITY, "output capacity below na + nb limbs");
+    Ctx& c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    int rc = init_locked(c);
+    if (rc) return rc;
+    const size_t nl = na + nb;
+    if (c.st_limbs < nl) {
+        if (c.st_a) HIPCALL(hipFree(c.st_a));
+        if (c.st_b) HIPCALL(hipFree(c.st_b));
+        if (c.st_out) HIPCALL(hipFree(c.st_out));
+        c.st_a = c.st_b = c.st_out = nullptr;
+        c.st_limbs = 0;
+        HIPCALL(hipMalloc(&c.st_a, nl * 4));
+        HIPCALL(hipMalloc(&c.st_b, nl * 4));
+        HIPCALL(hipMalloc(&c.st_out, nl * 4));
+        c.st_limbs = nl;
+    }
+    if (!c.st_len) HIPCALL(hipMalloc(&c.st_len, 8));
+    HIPCALL(hipMemcpyAsync(c.st_a, a, na * 4, hipMemcpyHostToDevice, c.stream));
+    if (!square) HIPCALL(hipMemcpyAsync(c.st_b, b, nb * 4, hipMemcpyHostToDevice, c.stream));
+    rc = mul_dev_locked(c, c.st_a, na, square ? c.st_a : c.st_b, nb, square, c.st_out, c.st_len, c.stream);
+    if (rc) return rc;
+    u64 len = 0;
+    HIPCALL(hipMemcpyAsync(&len, c.st_len, 8, hipMemcpyDeviceToHost, c.stream));
+    HIPCALL(hipStreamSynchronize(c.stream));
+    if (len > nl) return fail(PDH_E_HIP, "device returned an invalid product length");
+    if (len) HIPCALL(hipMemcpy(out, c.st_out, len * 4, hipMemcpyDeviceToHost));
+    *out_len = (size_t)len;
+    return PDH_OK;
+}
+
+constexpr uint64_t BBP_POS_MAX = (uint64_t)1 << 28;   // 8(pos-1)+6 < 2^31
+
+int bbp_host(const uint64_t* positions, uint64_t start, size_t count, uint8_t* values) {
+    if (count == 0) return PDH_OK;
+    if (!values) return fail(PDH_E_ARG, "null output pointer");
+    std::vector<uint64_t> sorted;
+    std::vector<size_t> order;
+    if (positions) {
+        for (size_t i = 0; i < count; ++i)
+            if (positions[i] < 1 || positions[i] > BBP_POS_MAX)
+                return fail(PDH_E_ARG, "BBP position outside [1, 2^28]");
+        // Similar positions share a wavefront: sort, compute, scatter back.
+        order.resize(count);
+        std::iota(order.begin(), order.end(), (size_t)0);
+        std::sort(order.begin(), order.end(), [&](size_t x, size_t y) { return positions[x] < positions[y]; });
+        sorted.resize(count);
+        for (size_t i = 0; i < count; ++i) sorted[i] = positions[order[i]];
+    } else if (start < 1 || start - 1 + count > BBP_POS_MAX) {
+        return fail(PDH_E_ARG, "BBP position outside [1, 2^28]");
+    }
+    Ctx& c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    int rc = init_locked(c);
+    if (rc) return rc;
+    if (c.st_bbp < count) {
+        if (c.st_pos) HIPCALL(hipFree(c.st_pos));
+        if (c.st_val) HIPCALL(hipFree(c.st_val));
+        c.st_pos = nullptr; c.st_val = nullptr; c.st_bbp = 0;
+        HIPCALL(hipMalloc(&c.st_pos, count * 8));
+        HIPCALL(hipMalloc(&c.st_val, count));
+        c.st_bbp = count;
+    }
+    if (positions) HIPCALL(hipMemcpyAsync(c.st_pos, sorted.data(), count * 8, hipMemcpyHostToDevice, c.stream));
+    hipError_t e = launch_bbp(positions ? c.st_pos : nullptr, start, count, c.st_val, c.stream);
+    if (e != hipSuccess) return fail_hip(e, "bbp launch");
+    std::vector<uint8_t> tmp(count);
+    HIPCALL(hipMemcpyAsync(tmp.data(), c.st_val, count, hipMemcpyDeviceToHost, c.stream));
+    HIPCALL(hipStreamSynchronize(c.stream));
+    if (positions) for (size_t i = 0; i < count; ++i) values[order[i]] = tmp[i];
+    else std::memcpy(values, tmp.data(), count);
+    return PDH_OK;
+}
+
+// ---- optional per-kernel event timing -----------------------------------------
+struct TimingRec { const char* name; hipEvent_t a, b; };
+struct Timing {
+    std::mutex mu;
+    bool on = false;
+    std::vector<TimingRec> recs;
+    std::vector<hipEvent_t> pool;
+    hipEvent_t get() {
+        if (!pool.empty()) { hipEvent_t e = pool.back(); pool.pop_back(); return e; }
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        return e;
+    }
+    void recycle() {
+        for (auto& r : recs) { pool.push_back(r.a); pool.push_back(r.b); }
+        recs.clear();
+    }
+};
+Timing& timing() { static Timing t; return t; }
+
+} // namespace
+
+hipEvent_t timing_begin(hipStream_t s) {
+    Timing& t = timing();
+    std::lock_guard<std::mutex> g(t.mu);
+    if (!t.on) return nullptr;
+    hipEvent_t e = t.get();
+    if (e) (void)hipEventRecord(e, s);
+    return e;
+}
+
+void timing_end(const char* name, hipEvent_t ev0, hipStream_t s) {
+    if (!ev0) return;
+    Timing& t = timing();
+    std::lock_guard<std::mutex> g(t.mu);
+    hipEvent_t e = t.get();
+    if (!e) { t.pool.push_back(ev0); return; }
+    (void)hipEventRecord(e, s);
+    t.recs.push_back(TimingRec{name, ev0, e});
+}
+
+} // namespace pdh
+
+using namespace pdh;
+
+extern "C" {
+
+const char* pdh_last_error(void) { return t_err.c_str(); }
+
+int pdh_device_available(void) {
+    Ctx& c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    return init_locked(c) == PDH_OK ? 1 : 0;
+}
+
+size_t pdh_ntt_length(size_t na, size_t nb) {
+    size_t n = 1, want = 2 * na + 2 * nb;
+    while (n < want) n <<= 1;
+    return n;
+}
+
+int pdh_mul(const uint32_t* a, size_t na, const uint32_t* b, size_t nb,
+            uint32_t* out, size_t out_cap, size_t* out_len) {
+    return mul_host(a, na, b, nb, false, out, out_cap, out_len);
+}
+
+int pdh_sqr(const uint32_t* a, size_t na, uint32_t* out, size_t out_cap, size_t* out_len) {
+    return mul_host(a, na, a, na, true, out, out_cap, out_len);
+}
+
+int pdh_mul_dev(const uint32_t* d_a, size_t na, const uint32_t* d_b, size_t nb,
+                uint32_t* d_out, uint64_t* d_out_len, void* stream) {
+    if (!d_out_len || (na && nb && (!d_a || !d_b || !d_out))) return fail(PDH_E_ARG, "null pointer argument");
+    Ctx& c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    int rc = init_locked(c);
+    if (rc) return rc;
+    return mul_dev_locked(c, d_a, na, d_b, nb, false, d_out, d_out_len, (hipStream_t)stream);
+}
+
+int pdh_sqr_dev(const uint32_t* d_a, size_t na, uint32_t* d_out, uint64_t* d_out_len, void* stream) {
+    if (!d_out_len || (na && (!d_a || !d_out))) return fail(PDH_E_ARG, "null pointer argument");
+    Ctx& c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    int rc = init_locked(c);
+    if (rc) return rc;
+    return mul_dev_locked(c, d_a, na, d_a, na, true, d_out, d_out_len, (hipStream_t)stream);
+}
+
+int pdh_bbp_range(uint64_t start, uint32_t count, uint8_t* values) {
+    return bbp_host(nullptr, start, count, values);
+}
+
+int pdh_bbp_positions(const uint64_t* positions, size_t n, uint8_t* values) {
+    if (n && !positions) return fail(PDH_E_ARG, "null positions pointer");
+    return bbp_host(positions, 0, n, values);
+}
+
+int pdh_bbp_positions_dev(const uint64_t* d_positions, size_t n, uint8_t* d_values, void* stream) {
+    if (n == 0) return PDH_OK;
+    if (!d_positions || !d_values) return fail(PDH_E_ARG, "null pointer argument");
+    Ctx& c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    int rc = init_locked(c);
+    if (rc) return rc;
+    hipError_t e = launch_bbp(d_positions, 0, n, d_values, (hipStream_t)stream);
+    if (e != hipSuccess) return fail_hip(e, "bbp launch");
+    return PDH_OK;
+}
+
+uint64_t pdh_launch_count(void) { return g_launches.load(); }
+
+int pdh_timing_enable(int on) {
+    Timing& t = timing();
+    std::lock_guard<std::mutex> g(t.mu);
+    t.recycle();
+    t.on = on != 0;
+    return PDH_OK;
+}
+
+int pdh_timing_read(char* names, size_t names_cap, double* total_ms, uint64_t* counts,
+                    int max_kernels, int* n_kernels) {
+    if (!names || !total_ms || !counts || !n_kernels || max_kernels <= 0) return fail(PDH_E_ARG, "null argument");
+    Timing& t = timing();
+    std::lock_guard<std::mutex> g(t.mu);
+    std::vector<std::string> order;
+    std::map<std::string, std::pair<double, uint64_t>> acc;
+    for (auto& r : t.recs) {
+        HIPCALL(hipEventSynchronize(r.b));
+        float ms = 0.f;
+        HIPCALL(hipEventElapsedTime(&ms, r.a, r.b));
+        auto it = acc.find(r.name);
+        if (it == acc.end()) { order.push_back(r.name); it = acc.emplace(r.name, std::make_pair(0.0, (uint64_t)0)).first; }
+        it->second.first += ms;
+        it->second.second += 1;
+    }
+    t.recycle();
+    std::string joined;
+    int k = 0;
+    for (auto& nm : order) {
+        if (k >= max_kernels) break;
+        total_ms[k] = acc[nm].first;
+        counts[k] = acc[nm].second;
+        joined += nm;
+        joined += '\n';
+        ++k;
+    }
+    if (joined.size() + 1 > names_cap) return fail(PDH_E_CAPACITY, "names buffer too small");
+    std::memcpy(names, joined.c_str(), joined.size() + 1);
+    *n_kernels = k;
+    return PDH_OK;
+}
+
+} // extern "C"

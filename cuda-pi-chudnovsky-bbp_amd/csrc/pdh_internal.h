@@ -1,0 +1,64 @@
+// ============================================================================
+//  pdh_internal.h -- declarations shared by the kernel translation units and
+//  the C-ABI layer. Not part of the public interface.
+// ============================================================================
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstddef>
+#include <atomic>
+
+namespace pdh {
+
+// Split of a 2^n-point transform into a column pass over the top L1 index
+// bits (T columns per workgroup) and a row pass over the low L2 bits.
+// n <= SMALL_MAX_LOG runs as one workgroup per multiply instead.
+struct NttShape {
+    int n;       // log2 N
+    int L1;      // column sub-transform bits (0 for the one-workgroup path)
+    int L2;      // row sub-transform bits
+    int T;       // columns per column-pass workgroup
+};
+
+constexpr int SMALL_MAX_LOG = 13;   // 2 x 2^13 u64 (+1/8 padding) = 144 KB LDS
+constexpr int MAX_LOG       = 24;
+constexpr int ROW_MAX_LOG   = 13;
+constexpr int COL_MAX_LOG   = 11;
+
+NttShape ntt_shape(int n);
+
+// Device tables for one device: twL[2^L + x] = w_{2^L}^x for L <= 13, and
+// the per-n outer tables (w_N^e = hi[e >> h] * lo[e & (2^h-1)], forward; the
+// inverse pair carries N^-1 in hi).
+struct OuterTables {
+    const uint64_t* hi;
+    const uint64_t* lo;
+    const uint64_t* hi_inv;
+    const uint64_t* lo_inv;
+    int h;
+};
+
+// Launchers (ntt_kernels.hip). All enqueue on `s`, return hipError_t.
+hipError_t launch_mul(const uint32_t* a, size_t na, const uint32_t* b, size_t nb,
+                      bool square, uint32_t* out, uint64_t* out_len,
+                      uint64_t* wsA, uint64_t* wsB, uint64_t* carry_ws,
+                      const uint64_t* twL, const OuterTables& ot, const NttShape& sh,
+                      hipStream_t s);
+hipError_t launch_zero_len(uint64_t* out_len, hipStream_t s);
+
+// Bytes of carry scratch launch_mul needs for a product of `nlimbs` limbs.
+size_t carry_ws_bytes(size_t nlimbs);
+
+// BBP (bbp_kernels.hip).
+hipError_t launch_bbp(const uint64_t* d_pos, uint64_t start, size_t count, uint8_t* d_out,
+                      hipStream_t s);
+
+extern std::atomic<uint64_t> g_launches;
+
+// Optional per-kernel timing (pdh_timing_enable): when on, each launch is
+// bracketed by hipEvents on its own stream and the durations are summed by
+// kernel name when read. timing_begin returns nullptr when timing is off.
+hipEvent_t timing_begin(hipStream_t s);
+void timing_end(const char* name, hipEvent_t ev0, hipStream_t s);
+
+} // namespace pdh

@@ -1,0 +1,184 @@
+"""ctypes binding of the ntt-hip engine's C ABI (include/pdh_capi.h).
+
+Python mirror of the reference's multiply and BBP entry points:
+
+* ``mul_ntt_hip(a, b)`` -- as ``mul_ntt_cuda(a, b)`` (reference
+  include/bignum.h:200-201): little-endian uint32 limbs in, normalized
+  product limbs out.
+* ``bbp_hex_digits_hip(start, count)`` -- as ``bbp_hex_digits_cuda``
+  (reference include/bbp.h:34): uppercase hex string for 1-based positions.
+* ``bbp_hex_digit(pos)`` -- as ``bbp_hex_digit`` (reference include/bbp.h:27).
+
+Unlike the reference, nothing falls back to the CPU: a missing library or
+device raises ``PdhError``. The library is loaded from this directory and
+must have been built (``make -C`` this directory, or
+``__graft_entry__.build()``).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpdh.so")
+
+PDH_OK = 0
+PDH_E_NODEVICE = -1
+PDH_E_HIP = -2
+PDH_E_ARG = -3
+PDH_E_CAPACITY = -4
+PDH_E_TOO_LARGE = -5
+
+EXPORTED = (
+    "pdh_last_error", "pdh_device_available", "pdh_ntt_length", "pdh_mul", "pdh_sqr",
+    "pdh_mul_dev", "pdh_sqr_dev", "pdh_bbp_range", "pdh_bbp_positions",
+    "pdh_bbp_positions_dev", "pdh_launch_count", "pdh_timing_enable", "pdh_timing_read",
+)
+
+
+class PdhError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"pdh error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+_sz = ctypes.c_size_t
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+_vp = ctypes.c_void_p
+
+
+def lib() -> ctypes.CDLL:
+    """Load libpdh.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise PdhError(PDH_E_NODEVICE, f"{LIB_PATH} not built")
+        L = ctypes.CDLL(LIB_PATH)
+        L.pdh_last_error.restype = ctypes.c_char_p
+        L.pdh_device_available.restype = ctypes.c_int
+        L.pdh_ntt_length.argtypes = [_sz, _sz]
+        L.pdh_ntt_length.restype = _sz
+        L.pdh_mul.argtypes = [_u32p, _sz, _u32p, _sz, _u32p, _sz, ctypes.POINTER(_sz)]
+        L.pdh_sqr.argtypes = [_u32p, _sz, _u32p, _sz, ctypes.POINTER(_sz)]
+        L.pdh_mul_dev.argtypes = [_vp, _sz, _vp, _sz, _vp, _vp, _vp]
+        L.pdh_sqr_dev.argtypes = [_vp, _sz, _vp, _vp, _vp]
+        L.pdh_bbp_range.argtypes = [ctypes.c_uint64, ctypes.c_uint32, _u8p]
+        L.pdh_bbp_positions.argtypes = [_u64p, _sz, _u8p]
+        L.pdh_bbp_positions_dev.argtypes = [_vp, _sz, _vp, _vp]
+        L.pdh_launch_count.restype = ctypes.c_uint64
+        L.pdh_timing_enable.argtypes = [ctypes.c_int]
+        L.pdh_timing_enable.restype = ctypes.c_int
+        L.pdh_timing_read.argtypes = [ctypes.c_char_p, _sz, ctypes.POINTER(ctypes.c_double), _u64p,
+                                      ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+        L.pdh_timing_read.restype = ctypes.c_int
+        for name in ("pdh_mul", "pdh_sqr", "pdh_mul_dev", "pdh_sqr_dev", "pdh_bbp_range",
+                     "pdh_bbp_positions", "pdh_bbp_positions_dev"):
+            getattr(L, name).restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _check(rc: int) -> None:
+    if rc != PDH_OK:
+        raise PdhError(rc, lib().pdh_last_error().decode(errors="replace"))
+
+
+def device_available() -> bool:
+    return bool(lib().pdh_device_available())
+
+
+def launch_count() -> int:
+    return int(lib().pdh_launch_count())
+
+
+def timing_enable(on: bool) -> None:
+    _check(lib().pdh_timing_enable(1 if on else 0))
+
+
+def timing_read() -> dict:
+    """{kernel name: (total ms, launches)} recorded since timing_enable(True)."""
+    names = ctypes.create_string_buffer(4096)
+    ms = (ctypes.c_double * 64)()
+    cnt = (ctypes.c_uint64 * 64)()
+    k = ctypes.c_int(0)
+    _check(lib().pdh_timing_read(names, 4096, ms, cnt, 64, ctypes.byref(k)))
+    keys = names.value.decode().split("\n")[: k.value]
+    return {name: (ms[i], int(cnt[i])) for i, name in enumerate(keys)}
+
+
+def ntt_length(na: int, nb: int) -> int:
+    return int(lib().pdh_ntt_length(na, nb))
+
+
+def _limbs(x) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(x, dtype=np.uint32))
+
+
+def _ptr(arr: np.ndarray, ctype):
+    return arr.ctypes.data_as(ctypes.POINTER(ctype))
+
+
+def mul_ntt_hip(a, b) -> np.ndarray:
+    """Normalized product of two little-endian uint32 limb arrays."""
+    a, b = _limbs(a), _limbs(b)
+    out = np.zeros(max(len(a) + len(b), 1), dtype=np.uint32)
+    n = _sz(0)
+    _check(lib().pdh_mul(_ptr(a, ctypes.c_uint32), len(a), _ptr(b, ctypes.c_uint32), len(b),
+                         _ptr(out, ctypes.c_uint32), len(out), ctypes.byref(n)))
+    return out[: n.value].copy()
+
+
+def sqr_ntt_hip(a) -> np.ndarray:
+    a = _limbs(a)
+    out = np.zeros(max(2 * len(a), 1), dtype=np.uint32)
+    n = _sz(0)
+    _check(lib().pdh_sqr(_ptr(a, ctypes.c_uint32), len(a), _ptr(out, ctypes.c_uint32), len(out),
+                         ctypes.byref(n)))
+    return out[: n.value].copy()
+
+
+def mul_dev(d_a: int, na: int, d_b: int, nb: int, d_out: int, d_len: int, stream: int = 0) -> None:
+    """Device-resident multiply on raw device pointers (e.g. torch data_ptr())."""
+    _check(lib().pdh_mul_dev(d_a, na, d_b, nb, d_out, d_len, stream or None))
+
+
+def sqr_dev(d_a: int, na: int, d_out: int, d_len: int, stream: int = 0) -> None:
+    _check(lib().pdh_sqr_dev(d_a, na, d_out, d_len, stream or None))
+
+
+def bbp_values_range(start: int, count: int) -> np.ndarray:
+    out = np.zeros(max(count, 1), dtype=np.uint8)
+    _check(lib().pdh_bbp_range(start, count, _ptr(out, ctypes.c_uint8)))
+    return out[:count]
+
+
+def bbp_values_positions(positions) -> np.ndarray:
+    pos = np.ascontiguousarray(np.asarray(positions, dtype=np.uint64))
+    out = np.zeros(max(len(pos), 1), dtype=np.uint8)
+    _check(lib().pdh_bbp_positions(_ptr(pos, ctypes.c_uint64), len(pos), _ptr(out, ctypes.c_uint8)))
+    return out[: len(pos)]
+
+
+def bbp_positions_dev(d_pos: int, n: int, d_out: int, stream: int = 0) -> None:
+    _check(lib().pdh_bbp_positions_dev(d_pos, n, d_out, stream or None))
+
+
+_HEX = np.frombuffer(b"0123456789ABCDEF", dtype=np.uint8)
+
+
+def to_hex(values: np.ndarray) -> str:
+    """Digit values 0..15 -> uppercase hex (reference src/bbp_cpu.cpp:20-22)."""
+    return _HEX[np.asarray(values, dtype=np.uint8)].tobytes().decode()
+
+
+def bbp_hex_digits_hip(start: int, count: int) -> str:
+    return to_hex(bbp_values_range(start, count)) if count else ""
+
+
+def bbp_hex_digit(position: int) -> str:
+    return bbp_hex_digits_hip(position, 1)

@@ -1,0 +1,111 @@
+/* ===========================================================================
+ *  pdh_capi.h -- C ABI of the MI355X (gfx950) ntt-hip multiply + BBP engine.
+ *
+ *  The only interface host code sees of HIP: plain pointers, sizes and int
+ *  status codes; no C++ types, no exceptions, no HIP types in signatures
+ *  (streams are passed as void*, a hipStream_t or NULL for the null stream).
+ *
+ *  Big integers are little-endian base-2^32 limb arrays, as the reference's
+ *  BigInt::mag (include/bignum.h:108). Products are written normalized
+ *  (no most-significant zero limbs; length 0 for zero), as mul_ntt_cpu
+ *  returns them (src/ntt_cpu.cpp:111-129, include/ntt.h:89).
+ *
+ *  Every function returns PDH_OK (0) or a negative PDH_E* code; the text of
+ *  the last error on the calling thread is in pdh_last_error(). There is no
+ *  CPU fallback: a missing device or a failed HIP call is reported, never
+ *  silently replaced by a host computation.
+ * ========================================================================= */
+#ifndef PDH_CAPI_H
+#define PDH_CAPI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PDH_OK            0
+#define PDH_E_NODEVICE   -1   /* no usable gfx950 device                    */
+#define PDH_E_HIP        -2   /* a HIP runtime call or kernel launch failed */
+#define PDH_E_ARG        -3   /* bad argument (null pointer, zero position) */
+#define PDH_E_CAPACITY   -4   /* output buffer smaller than required        */
+#define PDH_E_TOO_LARGE  -5   /* transform length beyond the supported 2^24 */
+
+/* Text of the last error on this thread ("" if none). */
+const char* pdh_last_error(void);
+
+/* 1 if a gfx950 device is usable, else 0.
+ * Replaces: bool cuda_available()  (reference include/bignum.h:205,
+ * src/ntt_cuda.cu:165-173). */
+int pdh_device_available(void);
+
+/* Transform length N = next_pow2(2*na + 2*nb) the multiply uses
+ * (reference include/ntt.h:47-50). */
+size_t pdh_ntt_length(size_t na, size_t nb);
+
+/* out = a * b, host buffers. out_cap >= na + nb limbs; *out_len receives the
+ * normalized length. na == 0 or nb == 0 gives *out_len = 0.
+ * Replaces: std::vector<uint32_t> mul_ntt_cuda(const std::vector<uint32_t>&,
+ * const std::vector<uint32_t>&)  (reference include/bignum.h:200-201,
+ * src/ntt_cuda.cu:179-243). */
+int pdh_mul(const uint32_t* a, size_t na, const uint32_t* b, size_t nb,
+            uint32_t* out, size_t out_cap, size_t* out_len);
+
+/* out = a * a with one forward transform, host buffers, out_cap >= 2*na.
+ * Replaces: mul_ntt_cuda(a, a) (the reference has no squaring path; same
+ * product, include/bignum.h:200-201). */
+int pdh_sqr(const uint32_t* a, size_t na, uint32_t* out, size_t out_cap, size_t* out_len);
+
+/* Device-resident multiply: d_a, d_b, d_out are device pointers; d_out holds
+ * na + nb limbs (the top limb may be zero); *d_out_len (device uint64_t)
+ * receives the normalized length. Enqueued on `stream`, asynchronous.
+ * Same product as pdh_mul; the operands stay in HBM (no PCIe traffic).
+ * Replaces: the device part of mul_ntt_cuda (reference src/ntt_cuda.cu:
+ * 198-233) without its per-call cudaMalloc/H2D/D2H/host carry. */
+int pdh_mul_dev(const uint32_t* d_a, size_t na, const uint32_t* d_b, size_t nb,
+                uint32_t* d_out, uint64_t* d_out_len, void* stream);
+
+/* Device-resident square, d_out holds 2*na limbs. */
+int pdh_sqr_dev(const uint32_t* d_a, size_t na, uint32_t* d_out, uint64_t* d_out_len,
+                void* stream);
+
+/* Hex digit values (0..15) at 1-based positions start .. start+count-1, host
+ * buffer. Position 1 is the first hex digit after the point (value 2).
+ * Replaces: std::string bbp_hex_digits_cuda(uint64_t start, uint32_t count)
+ * (reference include/bbp.h:34, src/bbp_cuda.cu:40-70); the caller maps
+ * values to '0'-'9','A'-'F' as src/bbp_cpu.cpp:20-22 does. */
+int pdh_bbp_range(uint64_t start, uint32_t count, uint8_t* values);
+
+/* Digit values at arbitrary 1-based positions (host buffers). New: the
+ * reference only offers contiguous blocks (bbp.h:27-34) or one position
+ * (char bbp_hex_digit(uint64_t), bbp.h:27, src/bbp_cpu.cpp:24-26). */
+int pdh_bbp_positions(const uint64_t* positions, size_t n, uint8_t* values);
+
+/* Device-resident variant: d_positions and d_values are device pointers;
+ * enqueued on `stream`. Positions must lie in [1, 2^28] (the kernel's exact
+ * FP64 modular arithmetic needs 8k + j < 2^31); the host entry points reject
+ * others with PDH_E_ARG, and this one writes 0xFF for them. */
+int pdh_bbp_positions_dev(const uint64_t* d_positions, size_t n, uint8_t* d_values,
+                          void* stream);
+
+/* Number of kernels this library has launched in this process. Lets a test
+ * assert that the device path really ran (the reference's self-test could
+ * not tell its CPU fallback from the GPU, tests/self_test.cu:66). */
+uint64_t pdh_launch_count(void);
+
+/* Per-kernel device timing for benchmarks and profiling (no counterpart in
+ * the reference, which times whole calls with std::chrono,
+ * src/chudnovsky.cpp:108-157). pdh_timing_enable(1) clears the record and
+ * brackets every later launch with hipEvents on its stream; 0 stops.
+ * pdh_timing_read waits for the recorded events, writes up to max_kernels
+ * kernel names ('\n'-separated, NUL-terminated) with their summed durations
+ * in ms and launch counts, and clears the record. */
+int pdh_timing_enable(int on);
+int pdh_timing_read(char* names, size_t names_cap, double* total_ms, uint64_t* counts,
+                    int max_kernels, int* n_kernels);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PDH_CAPI_H */
