@@ -90,6 +90,62 @@ def read_traffic(kernel: str):
     return None
 
 
+def bench_batch(pdh, torch, W, stream, reps=3):
+    """4,096 independent pairs, limb counts log-uniform over [64, 65536]:
+    batched plan replayed as one hipGraph vs the plain path (one pdh_mul_dev
+    launch sequence per multiply, serial on one stream); outputs compared."""
+    import numpy as np
+    pairs = W.batch_pairs(3, W.C3_PAIRS, W.C3_LO, W.C3_HI)
+    na = np.array([len(x) for x, _ in pairs], dtype=np.int64)
+    nb = np.array([len(y) for _, y in pairs], dtype=np.int64)
+    a_off = np.concatenate([[0], np.cumsum(na)[:-1]])
+    b_off = np.concatenate([[0], np.cumsum(nb)[:-1]])
+    o_off = np.concatenate([[0], np.cumsum(na + nb)[:-1]])
+    A = torch.from_numpy(np.concatenate([x for x, _ in pairs]).view(np.int32)).cuda()
+    B = torch.from_numpy(np.concatenate([y for _, y in pairs]).view(np.int32)).cuda()
+    out = torch.zeros(int((na + nb).sum()), dtype=torch.int32, device="cuda")
+    lens = torch.zeros(len(pairs), dtype=torch.int64, device="cuda")
+    pa, pb, po, pl = A.data_ptr(), B.data_ptr(), out.data_ptr(), lens.data_ptr()
+    descs = [(pa + 4 * int(a_off[i]), int(na[i]), pb + 4 * int(b_off[i]), int(nb[i]),
+              po + 4 * int(o_off[i]), pl + 8 * i) for i in range(len(pairs))]
+    stream = torch.cuda.Stream()          # graph capture needs a non-null stream
+    sp = stream.cuda_stream
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    def plain():
+        for d in descs:
+            pdh.mul_dev(*d, sp)
+    plain()
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(reps):
+        plain()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    t_plain = e0.elapsed_time(e1) / reps
+    ref_out, ref_lens = out.clone(), lens.clone()
+    plan = pdh.Batch(descs)
+    plan.run(sp)
+    plan.capture(sp)
+    plan.run(sp)
+    torch.cuda.synchronize()
+    out.zero_()
+    lens.zero_()
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(reps):
+        plan.run(sp)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    t_batch = e0.elapsed_time(e1) / reps
+    same = bool(torch.equal(out, ref_out) and torch.equal(lens, ref_lens))
+    plan.close()
+    n = len(pairs)
+    return {"pairs": n, "plain_ms": t_plain, "batched_ms": t_batch, "plain_mul_per_s": n / (t_plain * 1e-3),
+            "batched_mul_per_s": n / (t_batch * 1e-3), "speedup": t_plain / t_batch,
+            "batched_equals_plain": same}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -210,6 +266,8 @@ def main() -> int:
         M = sum(oracle.bbp_mulmod_count(p) for p in range(lo, hi + 1))
         extra["bbp_1_65536"] = {"ms": bms, "mulmod_per_s": M / (bms * 1e-3), "mulmods": M,
                                 "bit_exact": pdh.to_hex(vals.cpu().numpy()) == want}
+
+        extra["c3_batch_4096"] = bench_batch(pdh, torch, W, stream, reps=3)
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(a, b)
     line = {

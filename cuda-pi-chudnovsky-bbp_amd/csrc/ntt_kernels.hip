@@ -96,12 +96,16 @@ __device__ __forceinline__ void lds_level(u64* const* arr, int L, int M, int T,
         const int rest = u >> tlog;
         const int j = rest % m;
         const int base = (rest / m) * M + j;
+        // the last level (M == Q) has j == 0: all its twiddles are 1
+        const bool tw = M > Q;
         u64 w[Q];
         w[0] = 1;
+        if (tw) {
 #pragma unroll
-        for (int c = 1; c < Q; ++c) {
-            int e = (twscale * j * c) & mask;
-            w[c] = twL[INV ? ((-e) & mask) : e];
+            for (int c = 1; c < Q; ++c) {
+                int e = (twscale * j * c) & mask;
+                w[c] = twL[INV ? ((-e) & mask) : e];
+            }
         }
 #pragma unroll
         for (int a = 0; a < NARR; ++a) {
@@ -110,11 +114,15 @@ __device__ __forceinline__ void lds_level(u64* const* arr, int L, int M, int T,
             for (int t = 0; t < Q; ++t) x[t] = arr[a][lpad((base + t * m) * T + col)];
             if (!INV) {
                 dftq<Q, false>(x);
+                if (tw) {
 #pragma unroll
-                for (int c = 1; c < Q; ++c) x[c] = gl_mul(x[c], w[c]);
+                    for (int c = 1; c < Q; ++c) x[c] = gl_mul(x[c], w[c]);
+                }
             } else {
+                if (tw) {
 #pragma unroll
-                for (int c = 1; c < Q; ++c) x[c] = gl_mul(x[c], w[c]);
+                    for (int c = 1; c < Q; ++c) x[c] = gl_mul(x[c], w[c]);
+                }
                 dftq<Q, true>(x);
             }
 #pragma unroll
@@ -160,26 +168,31 @@ __device__ __forceinline__ u64 digit16(const u32* __restrict__ limbs, size_t nli
 }
 
 // ---- column pass, forward ------------------------------------------------------
-// Workgroup w handles columns b = w*T .. w*T+T-1 of the 2^L1 x 2^L2 view
-// (element (i, b) at i*2^L2 + b).
-template <int T, bool FROM_LIMBS>
-__global__ __launch_bounds__(256) void k_col_fwd(const void* __restrict__ src, size_t nlimbs,
-                                                 u64* __restrict__ dst, int n, int L1,
-                                                 const u64* __restrict__ twL,
+// Launch = a class of multiplies sharing n. Workgroup -> (job, operand, tile):
+// tile t handles columns b = t*T .. t*T+T-1 of the 2^L1 x 2^L2 view (element
+// (i, b) at i*2^L2 + b) of that operand. `jobs == nullptr` means the single
+// job `one` passed by value.
+template <int T>
+__global__ __launch_bounds__(1024) void k_col_fwd(const MulJob* __restrict__ jobs, MulJob one, int ops,
+                                                 int n, int L1, const u64* __restrict__ twL,
                                                  const u64* __restrict__ ohi,
                                                  const u64* __restrict__ olo, int h) {
     extern __shared__ u64 lds[];
     const int L2 = n - L1;
     const size_t S = (size_t)1 << L2;
     const int R = 1 << L1;
-    const size_t b0 = (size_t)blockIdx.x * T;
+    const unsigned tiles = (unsigned)(S / T);
+    const unsigned job = blockIdx.x / (ops * tiles);
+    const unsigned rem = blockIdx.x % (ops * tiles);
+    const int op = (int)(rem / tiles);
+    const size_t b0 = (size_t)(rem % tiles) * T;
+    const MulJob& J = jobs ? jobs[job] : one;
+    const u32* __restrict__ src = op ? J.b : J.a;
+    const size_t nl = op ? J.nb : J.na;
+    u64* __restrict__ dst = op ? J.wsB : J.wsA;
     for (int e = threadIdx.x; e < R * T; e += blockDim.x) {
         const int col = e & (T - 1), i = e / T;
-        const size_t g = (size_t)i * S + b0 + col;
-        u64 v;
-        if constexpr (FROM_LIMBS) v = digit16((const u32*)src, nlimbs, g);
-        else v = ((const u64*)src)[g];
-        lds[lpad(e)] = v;
+        lds[lpad(e)] = digit16(src, nl, (size_t)i * S + b0 + col);
     }
     __syncthreads();
     u64* arr[1] = {lds};
@@ -195,7 +208,7 @@ __global__ __launch_bounds__(256) void k_col_fwd(const void* __restrict__ src, s
 
 // ---- column pass, inverse (times N^-1, folded into the inverse tables) ----------
 template <int T>
-__global__ __launch_bounds__(256) void k_col_inv(u64* __restrict__ buf, int n, int L1,
+__global__ __launch_bounds__(1024) void k_col_inv(const MulJob* __restrict__ jobs, MulJob one, int n, int L1,
                                                  const u64* __restrict__ twL,
                                                  const u64* __restrict__ ohi,
                                                  const u64* __restrict__ olo, int h) {
@@ -203,7 +216,10 @@ __global__ __launch_bounds__(256) void k_col_inv(u64* __restrict__ buf, int n, i
     const int L2 = n - L1;
     const size_t S = (size_t)1 << L2;
     const int R = 1 << L1;
-    const size_t b0 = (size_t)blockIdx.x * T;
+    const unsigned tiles = (unsigned)(S / T);
+    const MulJob& J = jobs ? jobs[blockIdx.x / tiles] : one;
+    u64* __restrict__ buf = J.wsA;
+    const size_t b0 = (size_t)(blockIdx.x % tiles) * T;
     const u64 mask = ((u64)1 << h) - 1;
     for (int e = threadIdx.x; e < R * T; e += blockDim.x) {
         const int col = e & (T - 1), i = e / T;
@@ -222,13 +238,16 @@ __global__ __launch_bounds__(256) void k_col_inv(u64* __restrict__ buf, int n, i
 
 // ---- row pass: forward A (and B), pointwise, inverse; result into A ----------------
 template <bool SQUARE>
-__global__ __launch_bounds__(256) void k_row_mul(u64* __restrict__ A, const u64* __restrict__ B,
-                                                 int L2, const u64* __restrict__ twL) {
+__global__ __launch_bounds__(1024) void k_row_mul(const MulJob* __restrict__ jobs, MulJob one, int L1, int L2,
+                                                 const u64* __restrict__ twL) {
     extern __shared__ u64 lds[];
     const int R = 1 << L2;
     u64* la = lds;
     u64* lb = lds + lds_words(R);
-    const size_t base = (size_t)blockIdx.x * R;
+    const MulJob& J = jobs ? jobs[blockIdx.x >> L1] : one;
+    u64* __restrict__ A = J.wsA;
+    const u64* __restrict__ B = J.wsB;
+    const size_t base = (size_t)(blockIdx.x & ((1u << L1) - 1)) * R;
     for (int e = threadIdx.x; e < R; e += blockDim.x) {
         la[lpad(e)] = A[base + e];
         if constexpr (!SQUARE) lb[lpad(e)] = B[base + e];
@@ -249,51 +268,24 @@ __global__ __launch_bounds__(256) void k_row_mul(u64* __restrict__ A, const u64*
     for (int e = threadIdx.x; e < R; e += blockDim.x) A[base + e] = la[lpad(e)];
 }
 
-// ---- whole multiply in one workgroup (N = 2^n <= 2^13) -----------------------------
-template <bool SQUARE>
-__global__ __launch_bounds__(256) void k_small_mul(const u32* __restrict__ a, size_t na,
-                                                   const u32* __restrict__ b, size_t nb,
-                                                   u64* __restrict__ coef, int n,
-                                                   const u64* __restrict__ twL, u64 ninv) {
-    extern __shared__ u64 lds[];
-    const int R = 1 << n;
-    u64* la = lds;
-    u64* lb = lds + lds_words(R);
-    for (int e = threadIdx.x; e < R; e += blockDim.x) {
-        la[lpad(e)] = digit16(a, na, e);
-        if constexpr (!SQUARE) lb[lpad(e)] = digit16(b, nb, e);
-    }
-    __syncthreads();
-    if constexpr (SQUARE) {
-        u64* arr[1] = {la};
-        lds_dft<false, 1>(arr, n, 1, twL + R);
-        for (int e = threadIdx.x; e < R; e += blockDim.x) { u64 v = la[lpad(e)]; la[lpad(e)] = gl_mul(v, v); }
-    } else {
-        u64* arr[2] = {la, lb};
-        lds_dft<false, 2>(arr, n, 1, twL + R);
-        for (int e = threadIdx.x; e < R; e += blockDim.x) la[lpad(e)] = gl_mul(la[lpad(e)], lb[lpad(e)]);
-    }
-    __syncthreads();
-    u64* arr1[1] = {la};
-    lds_dft<true, 1>(arr1, n, 1, twL + R);
-    for (int e = threadIdx.x; e < R; e += blockDim.x) coef[e] = gl_mul(la[lpad(e)], ninv);
-}
-
 // ---- carry propagation --------------------------------------------------------------
-// Chunk tau holds coefficients 16*tau .. 16*tau+15 -> limbs 8*tau .. 8*tau+7.
-// With carry-in 0 it yields 256 bits D and carry-out C (< 2^41); with carry-in
-// c the carry-out is C + [c >= 2^256 - D], and 2^256 - D fits in 64 bits only
-// when limbs 2..7 of D are all ones. So each chunk is the function
-// f(c) = c >= thr ? C+1 : C, and compositions keep that form.
+// A chunk of k limbs (2k coefficients) with carry-in 0 yields 32k bits D and a
+// carry-out C (< 2^41); with carry-in c the carry-out is C + [c >= 2^(32k) - D].
+// For k >= 2, 2^(32k) - D fits in 64 bits only when limbs 2..k-1 of D are all
+// ones (then it is 2^64 - D mod 2^64, or never if that is 2^64); for k = 1 it
+// is 2^32 - D. So each chunk is the function f(c) = c >= thr ? C+1 : C, and
+// compositions keep that form: the carry is a parallel scan of these triples.
 struct CarryFn { u64 thr, v0, v1; };
 __device__ __forceinline__ u64 cf_apply(const CarryFn& f, u64 c) { return c >= f.thr ? f.v1 : f.v0; }
 // (later o earlier)
 __device__ __forceinline__ CarryFn cf_compose(const CarryFn& later, const CarryFn& earlier) {
     return CarryFn{earlier.thr, cf_apply(later, earlier.v0), cf_apply(later, earlier.v1)};
 }
+// Padding entries (after the last real chunk) are the constant-0 function.
+__device__ __forceinline__ CarryFn cf_pad() { return CarryFn{~(u64)0, 0, 0}; }
 
 constexpr int CARRY_NT = 256;
-constexpr int CARRY_CH = 16;      // coefficients per thread
+constexpr int CARRY_CH = 16;      // coefficients per thread in the multi-workgroup carry
 
 __device__ __forceinline__ void load_chunk(const u64* __restrict__ coef, size_t ncoef, size_t tau,
                                            u64* c) {
@@ -331,7 +323,8 @@ __device__ __forceinline__ CarryFn shfl_up_fn(const CarryFn& f, int d) {
     return CarryFn{__shfl_up(f.thr, d), __shfl_up(f.v0, d), __shfl_up(f.v1, d)};
 }
 
-// Inclusive scan of carry functions over the workgroup (thread 0 earliest).
+// Inclusive scan of carry functions over the workgroup (thread 0 earliest);
+// wsum holds one entry per wave (at most 16).
 __device__ CarryFn block_scan(CarryFn f, CarryFn* wsum) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
@@ -350,76 +343,198 @@ __device__ CarryFn block_scan(CarryFn f, CarryFn* wsum) {
     return f;
 }
 
-__global__ __launch_bounds__(CARRY_NT) void k_carry_reduce(const u64* __restrict__ coef, size_t ncoef,
-                                                           size_t nchunks, CarryFn* __restrict__ agg,
-                                                           u64* __restrict__ out_len) {
+__device__ __forceinline__ u64 wave_max(u64 v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) { u64 o = __shfl_xor(v, d); v = o > v ? o : v; }
+    return v;
+}
+
+// Per-job carry scratch: agg[nblk_max] then cin[nblk_max].
+struct CarryJob {
+    const u64* coef; size_t nchunks, nlimbs, nblk;
+    CarryFn* agg; u64* cin;
+};
+__device__ __forceinline__ CarryJob carry_job(const MulJob& J, size_t nblk_max) {
+    CarryJob c;
+    c.coef = J.wsA;
+    c.nlimbs = J.na + J.nb;
+    c.nchunks = (c.nlimbs + CARRY_CH / 2 - 1) / (CARRY_CH / 2);
+    c.nblk = (c.nchunks + CARRY_NT - 1) / CARRY_NT;
+    c.agg = (CarryFn*)J.carry;
+    c.cin = (u64*)(c.agg + nblk_max);
+    return c;
+}
+
+__global__ __launch_bounds__(CARRY_NT) void k_carry_reduce(const MulJob* __restrict__ jobs, MulJob one,
+                                                           size_t ncoef, size_t nblk_max) {
     __shared__ CarryFn wsum[CARRY_NT / 64];
-    const size_t tau = (size_t)blockIdx.x * CARRY_NT + threadIdx.x;
-    CarryFn f{~(u64)0, 0, 0};
-    if (tau < nchunks) {
+    const MulJob& J = jobs ? jobs[blockIdx.x / nblk_max] : one;
+    const CarryJob C = carry_job(J, nblk_max);
+    const size_t blk = blockIdx.x % nblk_max;
+    if (blk >= C.nblk) return;                       // whole workgroup leaves together
+    const size_t tau = blk * CARRY_NT + threadIdx.x;
+    CarryFn f = cf_pad();
+    if (tau < C.nchunks) {
         u64 c[CARRY_CH];
-        load_chunk(coef, ncoef, tau, c);
+        load_chunk(C.coef, ncoef, tau, c);
         f = chunk_fn(c);
     }
     f = block_scan(f, wsum);
-    if (threadIdx.x == CARRY_NT - 1) agg[blockIdx.x] = f;
-    if (blockIdx.x == 0 && threadIdx.x == 0) *out_len = 0;
+    if (threadIdx.x == CARRY_NT - 1) C.agg[blk] = f;
+    if (blk == 0 && threadIdx.x == 0) *J.out_len = 0;
 }
 
-// One workgroup of 1024 threads: carry-in of every carry block, in tiles.
-__global__ __launch_bounds__(1024) void k_carry_blocks(const CarryFn* __restrict__ agg, size_t nblocks,
-                                                       u64* __restrict__ cin) {
+// One workgroup of 1024 threads per job: carry-in of every carry block, in tiles.
+__global__ __launch_bounds__(1024) void k_carry_blocks(const MulJob* __restrict__ jobs, MulJob one,
+                                                       size_t nblk_max) {
     __shared__ CarryFn wsum[1024 / 64];
     __shared__ u64 tile_out[1024];
+    const MulJob& J = jobs ? jobs[blockIdx.x] : one;
+    const CarryJob C = carry_job(J, nblk_max);
     u64 carry = 0;
-    for (size_t t0 = 0; t0 < nblocks; t0 += 1024) {
+    for (size_t t0 = 0; t0 < C.nblk; t0 += 1024) {
         const size_t w = t0 + threadIdx.x;
-        CarryFn f = (w < nblocks) ? agg[w] : CarryFn{~(u64)0, 0, 0};
+        CarryFn f = (w < C.nblk) ? C.agg[w] : cf_pad();
         f = block_scan(f, wsum);
-        const u64 outc = cf_apply(f, carry);
-        tile_out[threadIdx.x] = outc;
+        tile_out[threadIdx.x] = cf_apply(f, carry);
         __syncthreads();
-        if (w < nblocks) cin[w] = threadIdx.x == 0 ? carry : tile_out[threadIdx.x - 1];
+        if (w < C.nblk) C.cin[w] = threadIdx.x == 0 ? carry : tile_out[threadIdx.x - 1];
         carry = tile_out[1023];
         __syncthreads();
     }
 }
 
-__global__ __launch_bounds__(CARRY_NT) void k_carry_apply(const u64* __restrict__ coef, size_t ncoef,
-                                                          size_t nchunks, size_t nlimbs,
-                                                          const u64* __restrict__ cin,
-                                                          u32* __restrict__ out, u64* __restrict__ out_len) {
+__global__ __launch_bounds__(CARRY_NT) void k_carry_apply(const MulJob* __restrict__ jobs, MulJob one,
+                                                          size_t ncoef, size_t nblk_max) {
     __shared__ CarryFn wsum[CARRY_NT / 64];
     __shared__ u64 outs[CARRY_NT];
-    const size_t tau = (size_t)blockIdx.x * CARRY_NT + threadIdx.x;
+    __shared__ u64 wmax[CARRY_NT / 64];
+    const MulJob& J = jobs ? jobs[blockIdx.x / nblk_max] : one;
+    const CarryJob C = carry_job(J, nblk_max);
+    // Highest limbs first: the block holding the top nonzero limb usually
+    // runs early, and later blocks then skip the atomic on out_len.
+    const size_t rb = blockIdx.x % nblk_max;
+    if (rb >= C.nblk) return;
+    const size_t blk = C.nblk - 1 - rb;
+    const size_t tau = blk * CARRY_NT + threadIdx.x;
     u64 c[CARRY_CH];
-    CarryFn f{~(u64)0, 0, 0};
-    if (tau < nchunks) {
-        load_chunk(coef, ncoef, tau, c);
+    CarryFn f = cf_pad();
+    if (tau < C.nchunks) {
+        load_chunk(C.coef, ncoef, tau, c);
         f = chunk_fn(c);
     }
-    const u64 bc = cin[blockIdx.x];
+    const u64 bc = C.cin[blk];
     CarryFn inc = block_scan(f, wsum);
     outs[threadIdx.x] = cf_apply(inc, bc);
     __syncthreads();
     const u64 my_in = threadIdx.x == 0 ? bc : outs[threadIdx.x - 1];
     u64 top = 0;
-    if (tau < nchunks) {
+    if (tau < C.nchunks) {
         u32 limbs[CARRY_CH / 2];
         run_carry(c, my_in, limbs);
         const size_t l0 = tau * (CARRY_CH / 2);
 #pragma unroll
         for (int q = 0; q < CARRY_CH / 2; ++q) {
-            if (l0 + q < nlimbs) {
-                out[l0 + q] = limbs[q];
+            if (l0 + q < C.nlimbs) {
+                J.out[l0 + q] = limbs[q];
                 if (limbs[q]) top = l0 + q + 1;
             }
         }
     }
-    // wave max, one atomic per wave
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) { u64 o = __shfl_xor(top, d); top = o > top ? o : top; }
-    if ((threadIdx.x & 63) == 0 && top) atomicMax((unsigned long long*)out_len, (unsigned long long)top);
+    top = wave_max(top);
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = top;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        u64 m = 0;
+        for (int w = 0; w < CARRY_NT / 64; ++w) m = wmax[w] > m ? wmax[w] : m;
+        if (m && m > __atomic_load_n((unsigned long long*)J.out_len, __ATOMIC_RELAXED))
+            atomicMax((unsigned long long*)J.out_len, (unsigned long long)m);
+    }
+}
+
+// Carry of one product whose 2^n coefficients (scaled) sit in LDS: each of the
+// blockDim.x threads takes k = ceil(nlimbs / blockDim.x) consecutive limbs.
+__device__ void block_carry(const u64* la, size_t nlimbs, u32* __restrict__ out, u64* __restrict__ out_len,
+                            CarryFn* wsum, u64* outs) {
+    const size_t k = (nlimbs + blockDim.x - 1) / blockDim.x;
+    const size_t l0 = (size_t)threadIdx.x * k;
+    const size_t l1 = l0 + k < nlimbs ? l0 + k : nlimbs;
+    CarryFn f = cf_pad();
+    if (l0 < nlimbs) {
+        u64 carry = 0, lo = 0;
+        bool ones = true;
+        for (size_t l = l0; l < l1; ++l) {
+            u64 cur = carry + la[lpad((int)(2 * l))];
+            u32 lo16 = (u32)(cur & 0xFFFFu);
+            carry = cur >> 16;
+            u64 cur2 = carry + la[lpad((int)(2 * l + 1))];
+            carry = cur2 >> 16;
+            u32 limb = lo16 | ((u32)(cur2 & 0xFFFFu) << 16);
+            if (l - l0 < 2) lo |= (u64)limb << (32 * (l - l0));
+            else ones &= limb == 0xFFFFFFFFu;
+        }
+        u64 thr;
+        if (l1 - l0 == 1) thr = ((u64)1 << 32) - lo;
+        else thr = (ones && lo != 0) ? (u64)(0 - lo) : ~(u64)0;
+        f = CarryFn{thr, carry, carry + 1};
+    }
+    CarryFn inc = block_scan(f, wsum);
+    outs[threadIdx.x] = cf_apply(inc, 0);
+    __syncthreads();
+    u64 carry = threadIdx.x == 0 ? 0 : outs[threadIdx.x - 1];
+    u64 top = 0;
+    for (size_t l = l0; l < l1; ++l) {
+        u64 cur = carry + la[lpad((int)(2 * l))];
+        u32 lo16 = (u32)(cur & 0xFFFFu);
+        carry = cur >> 16;
+        u64 cur2 = carry + la[lpad((int)(2 * l + 1))];
+        carry = cur2 >> 16;
+        u32 limb = lo16 | ((u32)(cur2 & 0xFFFFu) << 16);
+        out[l] = limb;
+        if (limb) top = l + 1;
+    }
+    top = wave_max(top);
+    __syncthreads();                                  // outs[] is reused below
+    if ((threadIdx.x & 63) == 0) outs[threadIdx.x >> 6] = top;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        u64 m = 0;
+        for (int w = 0; w < (int)(blockDim.x / 64); ++w) m = outs[w] > m ? outs[w] : m;
+        *out_len = m;
+    }
+}
+
+// ---- whole multiply in one workgroup (N = 2^n <= 2^13), carry included -------------
+template <bool SQUARE, int NT>
+__global__ __launch_bounds__(NT) void k_small_mul(const MulJob* __restrict__ jobs, MulJob one, int n,
+                                                  const u64* __restrict__ twL, u64 ninv) {
+    extern __shared__ u64 lds[];
+    __shared__ CarryFn wsum[NT / 64];
+    __shared__ u64 outs[NT];
+    const MulJob& J = jobs ? jobs[blockIdx.x] : one;
+    const int R = 1 << n;
+    u64* la = lds;
+    u64* lb = lds + lds_words(R);
+    for (int e = threadIdx.x; e < R; e += blockDim.x) {
+        la[lpad(e)] = digit16(J.a, J.na, e);
+        if constexpr (!SQUARE) lb[lpad(e)] = digit16(J.b, J.nb, e);
+    }
+    __syncthreads();
+    if constexpr (SQUARE) {
+        u64* arr[1] = {la};
+        lds_dft<false, 1>(arr, n, 1, twL + R);
+        for (int e = threadIdx.x; e < R; e += blockDim.x) { u64 v = la[lpad(e)]; la[lpad(e)] = gl_mul(v, v); }
+    } else {
+        u64* arr[2] = {la, lb};
+        lds_dft<false, 2>(arr, n, 1, twL + R);
+        for (int e = threadIdx.x; e < R; e += blockDim.x) la[lpad(e)] = gl_mul(la[lpad(e)], lb[lpad(e)]);
+    }
+    __syncthreads();
+    u64* arr1[1] = {la};
+    lds_dft<true, 1>(arr1, n, 1, twL + R);
+    for (int e = threadIdx.x; e < R; e += blockDim.x) la[lpad(e)] = gl_mul(la[lpad(e)], ninv);
+    __syncthreads();
+    block_carry(la, J.na + J.nb, J.out, J.out_len, wsum, outs);
 }
 
 __global__ void k_zero_len(u64* out_len) { *out_len = 0; }
@@ -439,12 +554,6 @@ NttShape ntt_shape(int n) {
     return s;
 }
 
-size_t carry_ws_bytes(size_t nlimbs) {
-    size_t nchunks = (nlimbs + CARRY_CH / 2 - 1) / (CARRY_CH / 2);
-    size_t nblk = (nchunks + CARRY_NT - 1) / CARRY_NT;
-    return nblk * sizeof(CarryFn) + nblk * sizeof(u64) + 256;
-}
-
 template <class K>
 static hipError_t set_lds(K kernel, size_t bytes) {
     return hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
@@ -456,35 +565,44 @@ static hipError_t set_lds(K kernel, size_t bytes) {
 #define PDH_LAUNCH(name, s, ...) do { hipEvent_t _t0 = timing_begin(s); hipLaunchKernelGGL(__VA_ARGS__); \
     PDH_LAUNCHED(); timing_end(name, _t0, s); } while (0)
 
+// Dynamic-LDS limits are raised once, before any launch (and never while a
+// caller may be capturing a graph).
+hipError_t init_kernel_attrs() {
+    constexpr size_t MAXLDS = 160 * 1024 - 4096;   // static __shared__ of k_small_mul stays inside 160 KB
+    PDH_TRY(set_lds(k_col_fwd<8>, MAXLDS));  PDH_TRY(set_lds(k_col_fwd<16>, MAXLDS));
+    PDH_TRY(set_lds(k_col_fwd<32>, MAXLDS)); PDH_TRY(set_lds(k_col_fwd<64>, MAXLDS));
+    PDH_TRY(set_lds(k_col_inv<8>, MAXLDS));  PDH_TRY(set_lds(k_col_inv<16>, MAXLDS));
+    PDH_TRY(set_lds(k_col_inv<32>, MAXLDS)); PDH_TRY(set_lds(k_col_inv<64>, MAXLDS));
+    PDH_TRY(set_lds(k_row_mul<true>, MAXLDS)); PDH_TRY(set_lds(k_row_mul<false>, MAXLDS));
+    PDH_TRY(set_lds(k_small_mul<true, 256>, MAXLDS)); PDH_TRY(set_lds(k_small_mul<false, 256>, MAXLDS));
+    PDH_TRY(set_lds(k_small_mul<true, 1024>, MAXLDS)); PDH_TRY(set_lds(k_small_mul<false, 1024>, MAXLDS));
+    return hipSuccess;
+}
+
 template <int T>
-static hipError_t launch_cols(const u32* a, size_t na, const u32* b, size_t nb, bool square,
-                              u64* wsA, u64* wsB, const u64* twL, const OuterTables& ot,
-                              const NttShape& sh, hipStream_t s, bool inverse) {
+static hipError_t launch_cols(const MulJob* d_jobs, const MulJob& one, unsigned njobs, int ops,
+                              const u64* twL, const OuterTables& ot, const NttShape& sh, hipStream_t s,
+                              bool inverse) {
     const size_t lds = (size_t)lds_words((1 << sh.L1) * T) * 8;
-    const dim3 grid((unsigned)(((size_t)1 << sh.L2) / T));
+    const unsigned tiles = (unsigned)(((size_t)1 << sh.L2) / T);
     if (!inverse) {
-        PDH_TRY(set_lds(k_col_fwd<T, true>, lds));
-        PDH_LAUNCH("k_col_fwd", s, (k_col_fwd<T, true>), grid, dim3(256), lds, s, (const void*)a, na, wsA,
+        PDH_LAUNCH("k_col_fwd", s, (k_col_fwd<T>), dim3(njobs * ops * tiles), dim3(1024), lds, s, d_jobs, one, ops,
                    sh.n, sh.L1, twL, ot.hi, ot.lo, ot.h);
-        if (!square)
-            PDH_LAUNCH("k_col_fwd", s, (k_col_fwd<T, true>), grid, dim3(256), lds, s, (const void*)b, nb, wsB,
-                       sh.n, sh.L1, twL, ot.hi, ot.lo, ot.h);
     } else {
-        PDH_TRY(set_lds(k_col_inv<T>, lds));
-        PDH_LAUNCH("k_col_inv", s, (k_col_inv<T>), grid, dim3(256), lds, s, wsA, sh.n, sh.L1, twL,
-                   ot.hi_inv, ot.lo_inv, ot.h);
+        PDH_LAUNCH("k_col_inv", s, (k_col_inv<T>), dim3(njobs * tiles), dim3(1024), lds, s, d_jobs, one,
+                   sh.n, sh.L1, twL, ot.hi_inv, ot.lo_inv, ot.h);
     }
     return hipSuccess;
 }
 
-static hipError_t launch_cols_T(const u32* a, size_t na, const u32* b, size_t nb, bool square,
-                                u64* wsA, u64* wsB, const u64* twL, const OuterTables& ot,
-                                const NttShape& sh, hipStream_t s, bool inverse) {
+static hipError_t launch_cols_T(const MulJob* d_jobs, const MulJob& one, unsigned njobs, int ops,
+                                const u64* twL, const OuterTables& ot, const NttShape& sh, hipStream_t s,
+                                bool inverse) {
     switch (sh.T) {
-        case 8:  return launch_cols<8>(a, na, b, nb, square, wsA, wsB, twL, ot, sh, s, inverse);
-        case 16: return launch_cols<16>(a, na, b, nb, square, wsA, wsB, twL, ot, sh, s, inverse);
-        case 32: return launch_cols<32>(a, na, b, nb, square, wsA, wsB, twL, ot, sh, s, inverse);
-        default: return launch_cols<64>(a, na, b, nb, square, wsA, wsB, twL, ot, sh, s, inverse);
+        case 8:  return launch_cols<8>(d_jobs, one, njobs, ops, twL, ot, sh, s, inverse);
+        case 16: return launch_cols<16>(d_jobs, one, njobs, ops, twL, ot, sh, s, inverse);
+        case 32: return launch_cols<32>(d_jobs, one, njobs, ops, twL, ot, sh, s, inverse);
+        default: return launch_cols<64>(d_jobs, one, njobs, ops, twL, ot, sh, s, inverse);
     }
 }
 
@@ -493,51 +611,56 @@ hipError_t launch_zero_len(uint64_t* out_len, hipStream_t s) {
     return hipSuccess;
 }
 
-static hipError_t launch_carry(const u64* coef, size_t ncoef, size_t nlimbs, u32* out, u64* out_len,
-                               u64* carry_ws, hipStream_t s) {
+size_t carry_nblk(size_t nlimbs) {
     const size_t nchunks = (nlimbs + CARRY_CH / 2 - 1) / (CARRY_CH / 2);
-    const size_t nblk = (nchunks + CARRY_NT - 1) / CARRY_NT;
-    CarryFn* agg = (CarryFn*)carry_ws;
-    u64* cin = (u64*)((char*)carry_ws + nblk * sizeof(CarryFn));
-    PDH_LAUNCH("k_carry_reduce", s, k_carry_reduce, dim3((unsigned)nblk), dim3(CARRY_NT), 0, s, coef, ncoef,
-               nchunks, agg, out_len);
-    PDH_LAUNCH("k_carry_blocks", s, k_carry_blocks, dim3(1), dim3(1024), 0, s, (const CarryFn*)agg, nblk, cin);
-    PDH_LAUNCH("k_carry_apply", s, k_carry_apply, dim3((unsigned)nblk), dim3(CARRY_NT), 0, s, coef, ncoef,
-               nchunks, nlimbs, (const u64*)cin, out, out_len);
+    return (nchunks + CARRY_NT - 1) / CARRY_NT;
+}
+
+size_t carry_ws_bytes(size_t nlimbs) {
+    const size_t nblk = carry_nblk(nlimbs);
+    return nblk * sizeof(CarryFn) + nblk * sizeof(u64) + 256;
+}
+
+// All jobs of a launch share n. For n > SMALL_MAX_LOG each job needs wsA, wsB of
+// 2^n words and carry scratch of carry_ws_bytes(2^(n-1)) (or of its own
+// nlimbs when it is launched alone: nblk_max is the per-job carry block count).
+hipError_t launch_class(const MulJob* d_jobs, const MulJob& one, unsigned njobs, int n, bool square,
+                        size_t nblk_max, const u64* twL, const OuterTables& ot, hipStream_t s) {
+    const NttShape sh = ntt_shape(n);
+    const size_t N = (size_t)1 << n;
+    if (sh.L1 == 0) {
+        const size_t lds = (size_t)(square ? 1 : 2) * lds_words((int)N) * 8;
+        const u64 ninv = gl_inv((u64)N);
+        // one workgroup per job: 1024 threads once the tile needs most of the LDS
+        if (n >= 12) {
+            if (square) PDH_LAUNCH("k_small_mul", s, (k_small_mul<true, 1024>), dim3(njobs), dim3(1024), lds, s, d_jobs, one, n, twL, ninv);
+            else PDH_LAUNCH("k_small_mul", s, (k_small_mul<false, 1024>), dim3(njobs), dim3(1024), lds, s, d_jobs, one, n, twL, ninv);
+        } else {
+            if (square) PDH_LAUNCH("k_small_mul", s, (k_small_mul<true, 256>), dim3(njobs), dim3(256), lds, s, d_jobs, one, n, twL, ninv);
+            else PDH_LAUNCH("k_small_mul", s, (k_small_mul<false, 256>), dim3(njobs), dim3(256), lds, s, d_jobs, one, n, twL, ninv);
+        }
+        return hipSuccess;
+    }
+    PDH_TRY(launch_cols_T(d_jobs, one, njobs, square ? 1 : 2, twL, ot, sh, s, false));
+    const size_t lds = (size_t)(square ? 1 : 2) * lds_words(1 << sh.L2) * 8;
+    const dim3 grid((unsigned)(njobs << sh.L1));
+    if (square)
+        PDH_LAUNCH("k_row_sqr", s, k_row_mul<true>, grid, dim3(1024), lds, s, d_jobs, one, sh.L1, sh.L2, twL);
+    else
+        PDH_LAUNCH("k_row_mul", s, k_row_mul<false>, grid, dim3(1024), lds, s, d_jobs, one, sh.L1, sh.L2, twL);
+    PDH_TRY(launch_cols_T(d_jobs, one, njobs, 1, twL, ot, sh, s, true));
+    const dim3 cgrid((unsigned)(njobs * nblk_max));
+    PDH_LAUNCH("k_carry_reduce", s, k_carry_reduce, cgrid, dim3(CARRY_NT), 0, s, d_jobs, one, N, nblk_max);
+    PDH_LAUNCH("k_carry_blocks", s, k_carry_blocks, dim3(njobs), dim3(1024), 0, s, d_jobs, one, nblk_max);
+    PDH_LAUNCH("k_carry_apply", s, k_carry_apply, cgrid, dim3(CARRY_NT), 0, s, d_jobs, one, N, nblk_max);
     return hipSuccess;
 }
 
 hipError_t launch_mul(const uint32_t* a, size_t na, const uint32_t* b, size_t nb, bool square,
                       uint32_t* out, uint64_t* out_len, uint64_t* wsA, uint64_t* wsB, uint64_t* carry_ws,
                       const uint64_t* twL, const OuterTables& ot, const NttShape& sh, hipStream_t s) {
-    const size_t N = (size_t)1 << sh.n;
-    const size_t nlimbs = na + nb;
-    if (sh.L1 == 0) {
-        const size_t lds = (size_t)(square ? 1 : 2) * lds_words((int)N) * 8;
-        const u64 ninv = gl_inv((u64)N);
-        if (square) {
-            PDH_TRY(set_lds(k_small_mul<true>, lds));
-            PDH_LAUNCH("k_small_mul", s, k_small_mul<true>, dim3(1), dim3(256), lds, s, a, na, a, na, wsA, sh.n,
-                       twL, ninv);
-        } else {
-            PDH_TRY(set_lds(k_small_mul<false>, lds));
-            PDH_LAUNCH("k_small_mul", s, k_small_mul<false>, dim3(1), dim3(256), lds, s, a, na, b, nb, wsA, sh.n,
-                       twL, ninv);
-        }
-    } else {
-        PDH_TRY(launch_cols_T(a, na, b, nb, square, wsA, wsB, twL, ot, sh, s, false));
-        const size_t lds = (size_t)(square ? 1 : 2) * lds_words(1 << sh.L2) * 8;
-        const dim3 grid((unsigned)((size_t)1 << sh.L1));
-        if (square) {
-            PDH_TRY(set_lds(k_row_mul<true>, lds));
-            PDH_LAUNCH("k_row_sqr", s, k_row_mul<true>, grid, dim3(256), lds, s, wsA, (const u64*)wsA, sh.L2, twL);
-        } else {
-            PDH_TRY(set_lds(k_row_mul<false>, lds));
-            PDH_LAUNCH("k_row_mul", s, k_row_mul<false>, grid, dim3(256), lds, s, wsA, (const u64*)wsB, sh.L2, twL);
-        }
-        PDH_TRY(launch_cols_T(a, na, b, nb, square, wsA, wsB, twL, ot, sh, s, true));
-    }
-    return launch_carry(wsA, N, nlimbs, out, out_len, carry_ws, s);
+    MulJob one{a, square ? a : b, out, out_len, na, square ? na : nb, wsA, wsB, carry_ws};
+    return launch_class(nullptr, one, 1, sh.n, square, carry_nblk(na + nb), twL, ot, s);
 }
 
 } // namespace pdh

@@ -75,6 +75,7 @@ int init_locked(Ctx& c) {
         c.why = std::string("[hip] device is ") + prop.gcnArchName + ", this library is built for gfx950";
         return fail(PDH_E_NODEVICE, c.why);
     }
+    HIPCALL(init_kernel_attrs());
     HIPCALL(hipStreamCreate(&c.stream));
     // twL[2^L + x] = w_{2^L}^x, L = 1..13
     std::vector<u64> tw((size_t)1 << 14, 0);
@@ -355,6 +356,192 @@ int pdh_bbp_positions_dev(const uint64_t* d_positions, size_t n, uint8_t* d_valu
 }
 
 uint64_t pdh_launch_count(void) { return g_launches.load(); }
+
+// ---- batched multiplies --------------------------------------------------------------
+struct pdh_batch {
+    struct Class { int n; unsigned njobs; MulJob* d_jobs; size_t nblk_max; OuterTables ot; };
+    std::vector<Class> classes;
+    std::vector<uint64_t*> zero_lens;     // jobs with an empty operand
+    char* arena = nullptr;
+    MulJob* d_jobs = nullptr;
+    std::vector<hipStream_t> streams;
+    std::vector<hipEvent_t> events;       // fork event + one join event per stream
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+int batch_enqueue(pdh_batch* p, hipStream_t s) {
+    Ctx& c = ctx();
+    for (uint64_t* z : p->zero_lens) {
+        hipError_t e = launch_zero_len(z, s);
+        if (e != hipSuccess) return fail_hip(e, "zero-length launch");
+    }
+    if (p->classes.empty()) return PDH_OK;
+    // fork: every internal stream waits for work already on s
+    HIPCALL(hipEventRecord(p->events[0], s));
+    for (size_t i = 0; i < p->streams.size(); ++i) HIPCALL(hipStreamWaitEvent(p->streams[i], p->events[0], 0));
+    for (size_t k = 0; k < p->classes.size(); ++k) {
+        const pdh_batch::Class& cl = p->classes[k];
+        hipStream_t si = p->streams[k % p->streams.size()];
+        MulJob none{};
+        hipError_t e = launch_class(cl.d_jobs, none, cl.njobs, cl.n, false, cl.nblk_max, c.twL, cl.ot, si);
+        if (e != hipSuccess) return fail_hip(e, "batch launch");
+    }
+    // join
+    for (size_t i = 0; i < p->streams.size(); ++i) {
+        HIPCALL(hipEventRecord(p->events[1 + i], p->streams[i]));
+        HIPCALL(hipStreamWaitEvent(s, p->events[1 + i], 0));
+    }
+    return PDH_OK;
+}
+
+void batch_free(pdh_batch* p) {
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    (void)hipDeviceSynchronize();       // a replayed graph ran on the caller's stream
+    for (auto st : p->streams) (void)hipStreamDestroy(st);
+    for (auto ev : p->events) (void)hipEventDestroy(ev);
+    if (p->arena) (void)hipFree(p->arena);
+    if (p->d_jobs) (void)hipFree(p->d_jobs);
+    delete p;
+}
+
+} // namespace
+
+int pdh_batch_create(const pdh_mul_desc* descs, size_t n, pdh_batch** plan) {
+    if (!plan || (n && !descs)) return fail(PDH_E_ARG, "null argument");
+    *plan = nullptr;
+    Ctx& c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    int rc = init_locked(c);
+    if (rc) return rc;
+    pdh_batch* p = new pdh_batch();
+    std::map<int, std::vector<size_t>> by_n;
+    for (size_t i = 0; i < n; ++i) {
+        const pdh_mul_desc& d = descs[i];
+        if (!d.out_len) { batch_free(p); return fail(PDH_E_ARG, "null out_len in descriptor"); }
+        if (d.na == 0 || d.nb == 0) { p->zero_lens.push_back(d.out_len); continue; }
+        if (!d.a || !d.b || !d.out) { batch_free(p); return fail(PDH_E_ARG, "null pointer in descriptor"); }
+        const int ln = log2_exact(pdh_ntt_length(d.na, d.nb));
+        if (ln > MAX_LOG) { batch_free(p); return fail(PDH_E_TOO_LARGE, "transform length beyond 2^24 in batch"); }
+        by_n[ln].push_back(i);
+    }
+    // workspace arena: per job of class n > SMALL_MAX_LOG two spectra + carry scratch
+    auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    size_t arena = 0, njobs_total = 0;
+    for (auto& kv : by_n) {
+        njobs_total += kv.second.size();
+        if (kv.first > SMALL_MAX_LOG)
+            arena += kv.second.size() * (2 * align(((size_t)8) << kv.first) + align(carry_ws_bytes((size_t)1 << (kv.first - 1))));
+    }
+    if (arena) {
+        hipError_t e = hipMalloc(&p->arena, arena);
+        if (e != hipSuccess) { batch_free(p); return fail_hip(e, "batch workspace allocation"); }
+    }
+    std::vector<MulJob> jobs;
+    jobs.reserve(njobs_total);
+    size_t off = 0;
+    for (auto& kv : by_n) {
+        const int ln = kv.first;
+        for (size_t i : kv.second) {
+            const pdh_mul_desc& d = descs[i];
+            MulJob J{d.a, d.b, d.out, d.out_len, d.na, d.nb, nullptr, nullptr, nullptr};
+            if (ln > SMALL_MAX_LOG) {
+                J.wsA = (uint64_t*)(p->arena + off); off += align((size_t)8 << ln);
+                J.wsB = (uint64_t*)(p->arena + off); off += align((size_t)8 << ln);
+                J.carry = p->arena + off; off += align(carry_ws_bytes((size_t)1 << (ln - 1)));
+            }
+            jobs.push_back(J);
+        }
+    }
+    if (njobs_total) {
+        hipError_t e = hipMalloc(&p->d_jobs, njobs_total * sizeof(MulJob));
+        if (e == hipSuccess) e = hipMemcpy(p->d_jobs, jobs.data(), njobs_total * sizeof(MulJob), hipMemcpyHostToDevice);
+        if (e != hipSuccess) { batch_free(p); return fail_hip(e, "batch job table upload"); }
+    }
+    size_t first = 0;
+    for (auto& kv : by_n) {
+        pdh_batch::Class cl;
+        cl.n = kv.first;
+        cl.njobs = (unsigned)kv.second.size();
+        cl.d_jobs = p->d_jobs + first;
+        cl.nblk_max = carry_nblk((size_t)1 << (kv.first - 1));
+        cl.ot = OuterTables{nullptr, nullptr, nullptr, nullptr, 0};
+        if (kv.first > SMALL_MAX_LOG && (rc = outer_for(c, kv.first, &cl.ot))) { batch_free(p); return rc; }
+        first += kv.second.size();
+        p->classes.push_back(cl);
+    }
+    // largest classes first: they have the longest critical path
+    std::reverse(p->classes.begin(), p->classes.end());
+    const size_t nstreams = std::min<size_t>(4, std::max<size_t>(1, p->classes.size()));
+    for (size_t i = 0; i < nstreams; ++i) {
+        hipStream_t st;
+        hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+        if (e != hipSuccess) { batch_free(p); return fail_hip(e, "batch stream"); }
+        p->streams.push_back(st);
+    }
+    for (size_t i = 0; i < nstreams + 1; ++i) {
+        hipEvent_t ev;
+        hipError_t e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+        if (e != hipSuccess) { batch_free(p); return fail_hip(e, "batch event"); }
+        p->events.push_back(ev);
+    }
+    *plan = p;
+    return PDH_OK;
+}
+
+int pdh_batch_run(pdh_batch* p, void* stream) {
+    if (!p) return fail(PDH_E_ARG, "null plan");
+    Ctx& c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    if (p->exec) {
+        HIPCALL(hipGraphLaunch(p->exec, (hipStream_t)stream));
+        g_launches.fetch_add(1, std::memory_order_relaxed);
+        return PDH_OK;
+    }
+    return batch_enqueue(p, (hipStream_t)stream);
+}
+
+int pdh_batch_capture(pdh_batch* p, void* stream) {
+    if (!p) return fail(PDH_E_ARG, "null plan");
+    if (!stream) return fail(PDH_E_ARG, "graph capture needs a non-null stream");
+    Ctx& c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    if (p->exec) return PDH_OK;
+    hipStream_t s = (hipStream_t)stream;
+    HIPCALL(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    int rc = batch_enqueue(p, s);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(s, &graph);
+    if (rc) { if (graph) (void)hipGraphDestroy(graph); return rc; }
+    if (e != hipSuccess) return fail_hip(e, "hipStreamEndCapture");
+    p->graph = graph;
+    HIPCALL(hipGraphInstantiate(&p->exec, graph, nullptr, nullptr, 0));
+    return PDH_OK;
+}
+
+int pdh_batch_destroy(pdh_batch* p) {
+    if (!p) return PDH_OK;
+    Ctx& c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    batch_free(p);
+    return PDH_OK;
+}
+
+int pdh_mul_batch_dev(const pdh_mul_desc* descs, size_t n, void* stream) {
+    pdh_batch* p = nullptr;
+    int rc = pdh_batch_create(descs, n, &p);
+    if (rc) return rc;
+    rc = pdh_batch_run(p, stream);
+    if (rc == PDH_OK) {
+        hipError_t e = hipStreamSynchronize((hipStream_t)stream);
+        if (e != hipSuccess) rc = fail_hip(e, "batch synchronize");
+    }
+    pdh_batch_destroy(p);
+    return rc;
+}
 
 int pdh_timing_enable(int on) {
     Timing& t = timing();

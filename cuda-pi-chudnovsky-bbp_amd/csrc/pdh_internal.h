@@ -38,7 +38,24 @@ struct OuterTables {
     int h;
 };
 
+// One multiply as the kernels see it (device-visible when batched).
+struct MulJob {
+    const uint32_t* a;
+    const uint32_t* b;
+    uint32_t* out;          // na + nb limbs
+    uint64_t* out_len;      // normalized length (device)
+    uint64_t na, nb;
+    uint64_t* wsA;          // 2^n words (product coefficients end here)
+    uint64_t* wsB;          // 2^n words (unused by the one-workgroup path)
+    void* carry;            // carry scratch (unused by the one-workgroup path)
+};
+
 // Launchers (ntt_kernels.hip). All enqueue on `s`, return hipError_t.
+hipError_t init_kernel_attrs();
+// Every job in a launch shares n; jobs == nullptr launches the single job `one`.
+hipError_t launch_class(const MulJob* d_jobs, const MulJob& one, unsigned njobs, int n, bool square,
+                        size_t nblk_max, const uint64_t* twL, const OuterTables& ot, hipStream_t s);
+size_t carry_nblk(size_t nlimbs);
 hipError_t launch_mul(const uint32_t* a, size_t na, const uint32_t* b, size_t nb,
                       bool square, uint32_t* out, uint64_t* out_len,
                       uint64_t* wsA, uint64_t* wsB, uint64_t* carry_ws,

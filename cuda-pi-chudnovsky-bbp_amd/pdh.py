@@ -35,7 +35,14 @@ EXPORTED = (
     "pdh_last_error", "pdh_device_available", "pdh_ntt_length", "pdh_mul", "pdh_sqr",
     "pdh_mul_dev", "pdh_sqr_dev", "pdh_bbp_range", "pdh_bbp_positions",
     "pdh_bbp_positions_dev", "pdh_launch_count", "pdh_timing_enable", "pdh_timing_read",
+    "pdh_batch_create", "pdh_batch_run", "pdh_batch_capture", "pdh_batch_destroy", "pdh_mul_batch_dev",
 )
+
+
+class MulDesc(ctypes.Structure):
+    """pdh_mul_desc: device pointers of one multiply in a batch."""
+    _fields_ = [("a", ctypes.c_void_p), ("na", ctypes.c_uint64), ("b", ctypes.c_void_p),
+                ("nb", ctypes.c_uint64), ("out", ctypes.c_void_p), ("out_len", ctypes.c_void_p)]
 
 
 class PdhError(RuntimeError):
@@ -58,6 +65,14 @@ def lib() -> ctypes.CDLL:
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise PdhError(PDH_E_NODEVICE, f"{LIB_PATH} not built")
+        # PyTorch-ROCm ships its own libamdhip64.so.7 / libhsa-runtime64.so.1.
+        # Loading torch first makes libpdh.so bind to that same runtime (same
+        # sonames), so device pointers and streams are shared and the process
+        # never holds two HIP runtimes.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         L.pdh_last_error.restype = ctypes.c_char_p
         L.pdh_device_available.restype = ctypes.c_int
@@ -76,8 +91,14 @@ def lib() -> ctypes.CDLL:
         L.pdh_timing_read.argtypes = [ctypes.c_char_p, _sz, ctypes.POINTER(ctypes.c_double), _u64p,
                                       ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
         L.pdh_timing_read.restype = ctypes.c_int
+        L.pdh_batch_create.argtypes = [ctypes.POINTER(MulDesc), _sz, ctypes.POINTER(_vp)]
+        L.pdh_batch_run.argtypes = [_vp, _vp]
+        L.pdh_batch_capture.argtypes = [_vp, _vp]
+        L.pdh_batch_destroy.argtypes = [_vp]
+        L.pdh_mul_batch_dev.argtypes = [ctypes.POINTER(MulDesc), _sz, _vp]
         for name in ("pdh_mul", "pdh_sqr", "pdh_mul_dev", "pdh_sqr_dev", "pdh_bbp_range",
-                     "pdh_bbp_positions", "pdh_bbp_positions_dev"):
+                     "pdh_bbp_positions", "pdh_bbp_positions_dev", "pdh_batch_create", "pdh_batch_run",
+                     "pdh_batch_capture", "pdh_batch_destroy", "pdh_mul_batch_dev"):
             getattr(L, name).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -149,6 +170,46 @@ def mul_dev(d_a: int, na: int, d_b: int, nb: int, d_out: int, d_len: int, stream
 
 def sqr_dev(d_a: int, na: int, d_out: int, d_len: int, stream: int = 0) -> None:
     _check(lib().pdh_sqr_dev(d_a, na, d_out, d_len, stream or None))
+
+
+class Batch:
+    """A planned batch of independent device-resident multiplies.
+
+    ``descs``: iterable of (a_ptr, na, b_ptr, nb, out_ptr, out_len_ptr) device
+    pointers. ``run(stream)`` enqueues every multiply; after ``capture`` each
+    run replays one hipGraph."""
+
+    def __init__(self, descs):
+        arr = (MulDesc * max(len(descs), 1))()
+        for i, d in enumerate(descs):
+            arr[i] = MulDesc(*d)
+        h = _vp()
+        _check(lib().pdh_batch_create(arr, len(descs), ctypes.byref(h)))
+        self.h = h
+
+    def run(self, stream: int = 0) -> None:
+        _check(lib().pdh_batch_run(self.h, stream or None))
+
+    def capture(self, stream: int) -> None:
+        _check(lib().pdh_batch_capture(self.h, stream))
+
+    def close(self) -> None:
+        if self.h:
+            _check(lib().pdh_batch_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def mul_batch_dev(descs, stream: int = 0) -> None:
+    arr = (MulDesc * max(len(descs), 1))()
+    for i, d in enumerate(descs):
+        arr[i] = MulDesc(*d)
+    _check(lib().pdh_mul_batch_dev(arr, len(descs), stream or None))
 
 
 def bbp_values_range(start: int, count: int) -> np.ndarray:

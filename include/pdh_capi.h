@@ -70,6 +70,37 @@ int pdh_mul_dev(const uint32_t* d_a, size_t na, const uint32_t* d_b, size_t nb,
 int pdh_sqr_dev(const uint32_t* d_a, size_t na, uint32_t* d_out, uint64_t* d_out_len,
                 void* stream);
 
+/* ---- batched multiplies ----------------------------------------------------
+ * Many independent multiplies of uneven sizes (e.g. one level of the
+ * reference's binary-splitting tree, src/chudnovsky.cpp:64-102, whose P, Q, T
+ * products the reference issues one mul_dispatch call at a time,
+ * src/bignum.cpp:204-232). All pointers in a descriptor are device pointers;
+ * out holds na + nb limbs and *out_len (device) receives the normalized
+ * length, exactly as pdh_mul_dev would write them. */
+typedef struct pdh_mul_desc {
+    const uint32_t* a; uint64_t na;
+    const uint32_t* b; uint64_t nb;
+    uint32_t* out;
+    uint64_t* out_len;
+} pdh_mul_desc;
+
+typedef struct pdh_batch pdh_batch;   /* opaque plan */
+
+/* Plan a batch once: group by transform length, allocate the batch's own
+ * workspace, upload the job tables. The descriptors are copied; the device
+ * buffers they point to must stay valid while the plan is run. */
+int pdh_batch_create(const pdh_mul_desc* descs, size_t n, pdh_batch** plan);
+/* Enqueue every multiply of the plan on `stream` (one launch sequence per
+ * transform-length class, classes on parallel internal streams joined back
+ * to `stream`). After pdh_batch_capture the run replays one hipGraph. */
+int pdh_batch_run(pdh_batch* plan, void* stream);
+/* Capture the plan's launch sequence into a hipGraph (once). */
+int pdh_batch_capture(pdh_batch* plan, void* stream);
+/* Waits for the plan's work on the device, then frees it. */
+int pdh_batch_destroy(pdh_batch* plan);
+/* One-shot: create, run on `stream`, wait, destroy. */
+int pdh_mul_batch_dev(const pdh_mul_desc* descs, size_t n, void* stream);
+
 /* Hex digit values (0..15) at 1-based positions start .. start+count-1, host
  * buffer. Position 1 is the first hex digit after the point (value 2).
  * Replaces: std::string bbp_hex_digits_cuda(uint64_t start, uint32_t count)
