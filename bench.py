@@ -45,7 +45,7 @@ def alg_bytes(kernel: str, na: int, nb: int, N: int) -> float:
     """Algorithmic HBM bytes one launch of `kernel` must move for one multiply."""
     nl = na + nb
     return {
-        "k_col_fwd": 4 * na + 8 * N,          # limbs in, N-point spectrum out (per operand)
+        "k_col_fwd": 4 * nl + 16 * N,         # both operands' limbs in, two N-point spectra out
         "k_row_mul": 24 * N,                  # two spectra in, one out
         "k_row_sqr": 16 * N,
         "k_col_inv": 16 * N,
@@ -153,6 +153,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run one 128 MiB device-to-device copy (torch) so a --pmc profile can "
+                         "scale FETCH_SIZE/WRITE_SIZE by a kernel of known traffic")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -182,6 +185,13 @@ def main() -> int:
 
     def step():
         pdh.mul_dev(da.data_ptr(), na, db.data_ptr(), nb, out.data_ptr(), ln.data_ptr(), sp)
+
+    if args.calibrate_copy:
+        src = torch.ones(1 << 24, dtype=torch.int64, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     for _ in range(args.warmup):
         step()

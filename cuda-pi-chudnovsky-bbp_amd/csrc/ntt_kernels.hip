@@ -568,7 +568,10 @@ static hipError_t set_lds(K kernel, size_t bytes) {
 // Dynamic-LDS limits are raised once, before any launch (and never while a
 // caller may be capturing a graph).
 hipError_t init_kernel_attrs() {
-    constexpr size_t MAXLDS = 160 * 1024 - 4096;   // static __shared__ of k_small_mul stays inside 160 KB
+    // 144 KB: the largest dynamic tile any launch uses (2 x 2^13 words + 1/8
+    // padding); static __shared__ plus dynamic must stay within the 160 KB per
+    // workgroup, which the runtime checks here.
+    constexpr size_t MAXLDS = 147456;
     PDH_TRY(set_lds(k_col_fwd<8>, MAXLDS));  PDH_TRY(set_lds(k_col_fwd<16>, MAXLDS));
     PDH_TRY(set_lds(k_col_fwd<32>, MAXLDS)); PDH_TRY(set_lds(k_col_fwd<64>, MAXLDS));
     PDH_TRY(set_lds(k_col_inv<8>, MAXLDS));  PDH_TRY(set_lds(k_col_inv<16>, MAXLDS));
