@@ -6,8 +6,8 @@
 //  function is __host__ __device__ so the host builds the twiddle tables with
 //  the exact arithmetic the kernels use.
 //
-//  Reduction uses 2^64 == 2^32 - 1 and 2^96 == -1 (mod p); on gfx950 the
-//  64x64 product lowers to v_mad_u64_u32 / v_mul_hi_u32 partial products.
+//  Reduction uses 2^64 == 2^32 - 1 and 2^96 == -1 (mod p) and no multiply;
+//  on gfx950 the 64x64 product is four v_mad_u64_u32 partial products.
 //
 //  Root system. The library does not use the reference's roots 7^((p-1)/N):
 //  any primitive N-th root gives the same (unique) integer product. It uses
@@ -28,14 +28,6 @@ typedef uint32_t u32;
 constexpr u64 GL_P   = 0xFFFFFFFF00000001ULL;
 constexpr u64 GL_EPS = 0xFFFFFFFFULL;          // 2^64 mod p
 
-__host__ __device__ __forceinline__ u64 gl_mulhi(u64 a, u64 b) {
-#if defined(__HIP_DEVICE_COMPILE__)
-    return __umul64hi(a, b);
-#else
-    return (u64)(((unsigned __int128)a * b) >> 64);
-#endif
-}
-
 // a + b for canonical a, b. s + EPS wraps exactly when s >= p; a carry out of
 // a + b means the true sum is s + 2^64 == s + EPS, which cannot wrap again.
 __host__ __device__ __forceinline__ u64 gl_add(u64 a, u64 b) {
@@ -53,23 +45,42 @@ __host__ __device__ __forceinline__ u64 gl_sub(u64 a, u64 b) {
 
 __host__ __device__ __forceinline__ u64 gl_neg(u64 a) { return a ? GL_P - a : 0; }
 
-// (hi:lo) mod p: lo - hi_hi + hi_lo * EPS, one carry fold, one conditional
-// subtract (the second carry fold of a general 128-bit input cannot happen
-// after the first: t2 < hi_lo * EPS < 2^64 - 2^33).
-__host__ __device__ __forceinline__ u64 gl_reduce(u64 lo, u64 hi) {
-    u32 hh = (u32)(hi >> 32);
-    u32 hl = (u32)hi;
-    u64 t0 = lo - (u64)hh;
-    if (lo < (u64)hh) t0 -= GL_EPS;
-    u64 t1 = ((u64)hl << 32) - (u64)hl;
-    u64 t2 = t0 + t1;
-    if (t2 < t0) t2 += GL_EPS;
-    u64 t3 = t2 + GL_EPS;
-    return (t3 < t2) ? t3 : t2;
+// (hh:hl:lo) mod p for a 128-bit value lo + hl*2^64 + hh*2^96, using
+// 2^64 == 2^32 - 1 and 2^96 == -1: the value is congruent to
+//   s1 - s2 + c1*EPS,  s1 = lo + hl*2^32 (mod 2^64, carry c1),  s2 = hl + hh,
+// and with t = s1 - s2 (mod 2^64, borrow b) it is t + (c1 - b)*EPS, which
+// stays inside [0, 2^64): if c1 = 1 and b = 0 then s1 < p so t + EPS < 2^64;
+// if b = 1 and c1 = 0 then t >= 2^64 - 2^33 > EPS. One conditional subtract
+// of p finishes. No multiply: hl*EPS is never formed.
+__host__ __device__ __forceinline__ u64 gl_reduce3(u64 lo, u32 hl, u32 hh) {
+    const u32 lo0 = (u32)lo, lo1 = (u32)(lo >> 32);
+    const u32 s1hi = lo1 + hl;
+    const bool c1 = s1hi < hl;
+    const u64 s1 = ((u64)s1hi << 32) | lo0;
+    const u64 s2 = (u64)hl + hh;
+    const u64 t = s1 - s2;
+    const bool b = s1 < s2;
+    u64 r = t;
+    if (c1 && !b) r = t + GL_EPS;
+    if (b && !c1) r = t - GL_EPS;
+    const u64 r2 = r + GL_EPS;             // wraps exactly when r >= p
+    return (r2 < r) ? r2 : r;
 }
 
+__host__ __device__ __forceinline__ u64 gl_reduce(u64 lo, u64 hi) {
+    return gl_reduce3(lo, (u32)hi, (u32)(hi >> 32));
+}
+
+// 64 x 64 -> 128 from four 32 x 32 + 64 multiply-adds (v_mad_u64_u32 on
+// gfx950), then the multiply-free fold above.
 __host__ __device__ __forceinline__ u64 gl_mul(u64 a, u64 b) {
-    return gl_reduce(a * b, gl_mulhi(a, b));
+    const u32 a0 = (u32)a, a1 = (u32)(a >> 32), b0 = (u32)b, b1 = (u32)(b >> 32);
+    const u64 p00 = (u64)a0 * b0;
+    const u64 p01 = (u64)a0 * b1 + (p00 >> 32);
+    const u64 p10 = (u64)a1 * b0 + (u32)p01;
+    const u64 p11 = (u64)a1 * b1 + (p01 >> 32) + (p10 >> 32);   // < 2^64: no overflow
+    const u64 lo = ((u64)(u32)p10 << 32) | (u32)p00;
+    return gl_reduce3(lo, (u32)p11, (u32)(p11 >> 32));
 }
 
 __host__ __device__ inline u64 gl_pow(u64 b, u64 e) {
@@ -93,7 +104,8 @@ __host__ __device__ __forceinline__ u64 gl_mulp2(u64 x) {
     } else if constexpr (K >= 96) {
         return gl_neg(gl_mulp2<K - 96>(x));
     } else if constexpr (K < 64) {
-        return gl_reduce(x << K, x >> (64 - K));
+        const u64 hi = x >> (64 - K);
+        return gl_reduce3(x << K, (u32)hi, (u32)(hi >> 32));
     } else {
         constexpr int j = K - 64;
         u64 L = x << j;

@@ -71,39 +71,69 @@ __device__ __forceinline__ void dft8(u64* x) {
     x[1] = b0; x[3] = b1; x[5] = b2; x[7] = b3;
 }
 
+// X_{c1 + 4 c2} = DFT4_b( w16^(b c1) DFT4_a(x_{b + 4a})_c1 )_c2 with
+// w16 = 2^12: eight DFT4s and nine shift twiddles.
+template <bool INV>
+__device__ __forceinline__ void dft16(u64* x) {
+#pragma unroll
+    for (int b = 0; b < 4; ++b) dft4<INV>(x[b], x[b + 4], x[b + 8], x[b + 12]);
+    // x[b + 4 c1] now holds y[b][c1]
+    if (!INV) {
+        x[5] = gl_mulp2<12>(x[5]);  x[9] = gl_mulp2<24>(x[9]);   x[13] = gl_mulp2<36>(x[13]);
+        x[6] = gl_mulp2<24>(x[6]);  x[10] = gl_mulp2<48>(x[10]); x[14] = gl_mulp2<72>(x[14]);
+        x[7] = gl_mulp2<36>(x[7]);  x[11] = gl_mulp2<72>(x[11]); x[15] = gl_mulp2<108>(x[15]);
+    } else {   // w16^-k = 2^(192 - 12k)
+        x[5] = gl_mulp2<180>(x[5]); x[9] = gl_mulp2<168>(x[9]);  x[13] = gl_mulp2<156>(x[13]);
+        x[6] = gl_mulp2<168>(x[6]); x[10] = gl_mulp2<144>(x[10]); x[14] = gl_mulp2<120>(x[14]);
+        x[7] = gl_mulp2<156>(x[7]); x[11] = gl_mulp2<120>(x[11]); x[15] = gl_mulp2<84>(x[15]);
+    }
+#pragma unroll
+    for (int c1 = 0; c1 < 4; ++c1) dft4<INV>(x[4 * c1], x[4 * c1 + 1], x[4 * c1 + 2], x[4 * c1 + 3]);
+    // x[4 c1 + c2] holds X_{c1 + 4 c2}: transpose the 4x4 in registers
+    u64 t;
+    t = x[1]; x[1] = x[4]; x[4] = t;   t = x[2]; x[2] = x[8]; x[8] = t;
+    t = x[3]; x[3] = x[12]; x[12] = t; t = x[6]; x[6] = x[9]; x[9] = t;
+    t = x[7]; x[7] = x[13]; x[13] = t; t = x[11]; x[11] = x[14]; x[14] = t;
+}
+
 template <int Q, bool INV>
 __device__ __forceinline__ void dftq(u64* x) {
     if constexpr (Q == 2) dft2<INV>(x);
     else if constexpr (Q == 4) dft4a<INV>(x);
-    else dft8<INV>(x);
+    else if constexpr (Q == 8) dft8<INV>(x);
+    else dft16<INV>(x);
 }
+
+template <int T> struct QLogT { static constexpr int v = T == 1 ? 0 : T == 2 ? 1 : T == 4 ? 2 : T == 8 ? 3 : T == 16 ? 4 : T == 32 ? 5 : 6; };
+template <int Q> struct QLog { static constexpr int v = Q == 2 ? 1 : Q == 4 ? 2 : Q == 8 ? 3 : 4; };
 
 // ---- one radix-Q level of an in-place 2^L-point sub-transform in LDS ---------
 // NARR arrays, each holding T interleaved columns: element (i, col) of array
-// a sits at arr[a][lpad(i*T + col)]. Sub-block size M (M = Q*m): butterfly
-// (beta, j) reads i = beta*M + j + t*m, t < Q. Forward: DFT_Q then output c
-// times w_M^(j*c). Inverse: times w_M^(-j*c), then inverse DFT_Q.
+// a sits at arr[a][lpad(i*T + col)]. Sub-block size M = 2^logM = Q*m:
+// butterfly (beta, j) reads i = beta*M + j + t*m, t < Q. Forward: DFT_Q, then
+// output c times w_M^(j*c). Inverse: times w_M^(-j*c), then inverse DFT_Q.
+// The arrays share each butterfly's twiddles and are processed one after the
+// other to keep register pressure at one array's Q values.
 template <int Q, bool INV, int NARR>
-__device__ __forceinline__ void lds_level(u64* const* arr, int L, int M, int T,
+__device__ __forceinline__ void lds_level(u64* const* arr, int L, int logM, int tlog,
                                           const u64* __restrict__ twL) {
-    const int m = M / Q;
-    const int nbf = ((1 << L) / Q) * T;
-    const int twscale = (1 << L) / M;
+    constexpr int LQ = QLog<Q>::v;
+    const int logm = logM - LQ;
+    const int nbf = 1 << (L - LQ + tlog);
     const int mask = (1 << L) - 1;
-    const int tlog = __ffs(T) - 1;
+    const int tmask = (1 << tlog) - 1;
+    const bool tw = logm > 0;                  // the last level (M == Q) has j == 0
     for (int u = threadIdx.x; u < nbf; u += blockDim.x) {
-        const int col = u & (T - 1);
+        const int col = u & tmask;
         const int rest = u >> tlog;
-        const int j = rest % m;
-        const int base = (rest / m) * M + j;
-        // the last level (M == Q) has j == 0: all its twiddles are 1
-        const bool tw = M > Q;
+        const int j = rest & ((1 << logm) - 1);
+        const int base = ((rest >> logm) << logM) + j;
         u64 w[Q];
-        w[0] = 1;
         if (tw) {
+            const int step = j << (L - logM);  // w_M^j = w_{2^L}^(j 2^(L-logM))
 #pragma unroll
             for (int c = 1; c < Q; ++c) {
-                int e = (twscale * j * c) & mask;
+                const int e = (step * c) & mask;
                 w[c] = twL[INV ? ((-e) & mask) : e];
             }
         }
@@ -111,7 +141,7 @@ __device__ __forceinline__ void lds_level(u64* const* arr, int L, int M, int T,
         for (int a = 0; a < NARR; ++a) {
             u64 x[Q];
 #pragma unroll
-            for (int t = 0; t < Q; ++t) x[t] = arr[a][lpad((base + t * m) * T + col)];
+            for (int t = 0; t < Q; ++t) x[t] = arr[a][lpad(((base + (t << logm)) << tlog) + col)];
             if (!INV) {
                 dftq<Q, false>(x);
                 if (tw) {
@@ -126,39 +156,49 @@ __device__ __forceinline__ void lds_level(u64* const* arr, int L, int M, int T,
                 dftq<Q, true>(x);
             }
 #pragma unroll
-            for (int t = 0; t < Q; ++t) arr[a][lpad((base + t * m) * T + col)] = x[t];
+            for (int t = 0; t < Q; ++t) arr[a][lpad(((base + (t << logm)) << tlog) + col)] = x[t];
         }
     }
 }
 
-// Full 2^L-point sub-transform: forward = radix 2^(L mod 3) level then radix-8
-// levels (M = 2^L ... 8); inverse = the same levels in reverse order.
-// Callers synchronise before; this returns after a final barrier.
-template <bool INV, int NARR>
-__device__ void lds_dft(u64* const* arr, int L, int T, const u64* __restrict__ twL) {
-    const int r0 = L % 3;
+// Full 2^L-point sub-transform with radix 2^LR (LR = 3 or 4): forward = a
+// radix-2^(L mod LR) level, then radix-2^LR levels (M = 2^L ... 2^LR);
+// inverse = the same levels in reverse order. Callers synchronise before;
+// this returns after a final barrier.
+template <int Q, bool INV, int NARR>
+__device__ __forceinline__ void lds_level_q(u64* const* arr, int L, int logM, int tlog, const u64* twL) {
+    lds_level<Q, INV, NARR>(arr, L, logM, tlog, twL);
+    __syncthreads();
+}
+
+template <int LR, bool INV, int NARR>
+__device__ void lds_dft(u64* const* arr, int L, int tlog, const u64* __restrict__ twL) {
+    constexpr int RAD = 1 << LR;
+    const int r0 = L % LR;
     if (!INV) {
-        int M = 1 << L;
-        if (r0 == 1) { lds_level<2, false, NARR>(arr, L, M, T, twL); M >>= 1; __syncthreads(); }
-        if (r0 == 2) { lds_level<4, false, NARR>(arr, L, M, T, twL); M >>= 2; __syncthreads(); }
-        for (; M >= 8; M >>= 3) { lds_level<8, false, NARR>(arr, L, M, T, twL); __syncthreads(); }
+        int logM = L;
+        if (r0 == 1) { lds_level_q<2, false, NARR>(arr, L, logM, tlog, twL); logM -= 1; }
+        if (r0 == 2) { lds_level_q<4, false, NARR>(arr, L, logM, tlog, twL); logM -= 2; }
+        if (r0 == 3) { lds_level_q<8, false, NARR>(arr, L, logM, tlog, twL); logM -= 3; }
+        for (; logM >= LR; logM -= LR) lds_level_q<RAD, false, NARR>(arr, L, logM, tlog, twL);
     } else {
-        const int Mtop = (1 << L) >> r0;
-        for (int M = 8; M <= Mtop; M <<= 3) { lds_level<8, true, NARR>(arr, L, M, T, twL); __syncthreads(); }
-        if (r0 == 1) { lds_level<2, true, NARR>(arr, L, 1 << L, T, twL); __syncthreads(); }
-        if (r0 == 2) { lds_level<4, true, NARR>(arr, L, 1 << L, T, twL); __syncthreads(); }
+        for (int logM = LR; logM <= L - r0; logM += LR) lds_level_q<RAD, true, NARR>(arr, L, logM, tlog, twL);
+        if (r0 == 1) lds_level_q<2, true, NARR>(arr, L, L, tlog, twL);
+        if (r0 == 2) lds_level_q<4, true, NARR>(arr, L, L, tlog, twL);
+        if (r0 == 3) lds_level_q<8, true, NARR>(arr, L, L, tlog, twL);
     }
 }
 
 // Frequency held at position i after the forward sub-transform: the position's
-// digits (widths r0, 3, 3, ... from the top) are the frequency's digits from
-// the bottom.
+// digits (widths L mod LR, LR, LR, ... from the top) are the frequency's
+// digits from the bottom.
+template <int LR>
 __device__ __forceinline__ u64 freq_of_pos(int i, int L) {
-    const int r0 = L % 3;
+    const int r0 = L % LR;
     int left = L, sh = 0;
     u64 c = 0;
     if (r0) { left -= r0; c = (u64)((i >> left) & ((1 << r0) - 1)); sh = r0; }
-    while (left > 0) { left -= 3; c |= (u64)((i >> left) & 7) << sh; sh += 3; }
+    while (left > 0) { left -= LR; c |= (u64)((i >> left) & ((1 << LR) - 1)) << sh; sh += LR; }
     return c;
 }
 
@@ -172,11 +212,9 @@ __device__ __forceinline__ u64 digit16(const u32* __restrict__ limbs, size_t nli
 // tile t handles columns b = t*T .. t*T+T-1 of the 2^L1 x 2^L2 view (element
 // (i, b) at i*2^L2 + b) of that operand. `jobs == nullptr` means the single
 // job `one` passed by value.
-template <int T>
+template <int T, int LR>
 __global__ __launch_bounds__(1024) void k_col_fwd(const MulJob* __restrict__ jobs, MulJob one, int ops,
-                                                 int n, int L1, const u64* __restrict__ twL,
-                                                 const u64* __restrict__ ohi,
-                                                 const u64* __restrict__ olo, int h) {
+                                                 int n, int L1, const u64* __restrict__ twL) {
     extern __shared__ u64 lds[];
     const int L2 = n - L1;
     const size_t S = (size_t)1 << L2;
@@ -196,22 +234,17 @@ __global__ __launch_bounds__(1024) void k_col_fwd(const MulJob* __restrict__ job
     }
     __syncthreads();
     u64* arr[1] = {lds};
-    lds_dft<false, 1>(arr, L1, T, twL + R);
-    const u64 mask = ((u64)1 << h) - 1;
+    lds_dft<LR, false, 1>(arr, L1, QLogT<T>::v, twL + R);
     for (int e = threadIdx.x; e < R * T; e += blockDim.x) {
         const int col = e & (T - 1), i = e / T;
-        const u64 ex = (b0 + col) * freq_of_pos(i, L1);
-        const u64 w = gl_mul(ohi[ex >> h], olo[ex & mask]);
-        dst[(size_t)i * S + b0 + col] = gl_mul(lds[lpad(e)], w);
+        dst[(size_t)i * S + b0 + col] = lds[lpad(e)];
     }
 }
 
-// ---- column pass, inverse (times N^-1, folded into the inverse tables) ----------
-template <int T>
+// ---- column pass, inverse ----------------------------------------------------------
+template <int T, int LR>
 __global__ __launch_bounds__(1024) void k_col_inv(const MulJob* __restrict__ jobs, MulJob one, int n, int L1,
-                                                 const u64* __restrict__ twL,
-                                                 const u64* __restrict__ ohi,
-                                                 const u64* __restrict__ olo, int h) {
+                                                 const u64* __restrict__ twL) {
     extern __shared__ u64 lds[];
     const int L2 = n - L1;
     const size_t S = (size_t)1 << L2;
@@ -220,26 +253,44 @@ __global__ __launch_bounds__(1024) void k_col_inv(const MulJob* __restrict__ job
     const MulJob& J = jobs ? jobs[blockIdx.x / tiles] : one;
     u64* __restrict__ buf = J.wsA;
     const size_t b0 = (size_t)(blockIdx.x % tiles) * T;
-    const u64 mask = ((u64)1 << h) - 1;
     for (int e = threadIdx.x; e < R * T; e += blockDim.x) {
         const int col = e & (T - 1), i = e / T;
-        const u64 ex = (b0 + col) * freq_of_pos(i, L1);
-        const u64 w = gl_mul(ohi[ex >> h], olo[ex & mask]);
-        lds[lpad(e)] = gl_mul(buf[(size_t)i * S + b0 + col], w);
+        lds[lpad(e)] = buf[(size_t)i * S + b0 + col];
     }
     __syncthreads();
     u64* arr[1] = {lds};
-    lds_dft<true, 1>(arr, L1, T, twL + R);
+    lds_dft<LR, true, 1>(arr, L1, QLogT<T>::v, twL + R);
     for (int e = threadIdx.x; e < R * T; e += blockDim.x) {
         const int col = e & (T - 1), i = e / T;
         buf[(size_t)i * S + b0 + col] = lds[lpad(e)];
     }
 }
 
-// ---- row pass: forward A (and B), pointwise, inverse; result into A ----------------
-template <bool SQUARE>
+// ---- row pass: outer twiddle, forward A (and B), pointwise, inverse, inverse
+// outer twiddle times N^-1; result into A ----------------------------------------------
+// Row r of the 2^L1 x 2^L2 view holds column-pass frequency c = freq(r); its
+// element b is multiplied by w_N^(b c) on the way in and by w_N^(-b c) N^-1 on
+// the way out. Thread t handles b = t + k*blockDim, so its twiddles form the
+// chain w_N^(c t) * (w_N^(c blockDim))^k: two table lookups per thread.
+__device__ __forceinline__ u64 freq_rt(int i, int L, int lr) {
+    const int r0 = L % lr;
+    int left = L, sh = 0;
+    u64 c = 0;
+    if (r0) { left -= r0; c = (u64)((i >> left) & ((1 << r0) - 1)); sh = r0; }
+    while (left > 0) { left -= lr; c |= (u64)((i >> left) & ((1 << lr) - 1)) << sh; sh += lr; }
+    return c;
+}
+
+__device__ __forceinline__ u64 tw_lookup(const u64* __restrict__ hi, const u64* __restrict__ lo, int h, u64 e) {
+    return gl_mul(hi[e >> h], lo[e & (((u64)1 << h) - 1)]);
+}
+
+template <bool SQUARE, int LR>
 __global__ __launch_bounds__(1024) void k_row_mul(const MulJob* __restrict__ jobs, MulJob one, int L1, int L2,
-                                                 const u64* __restrict__ twL) {
+                                                 int lr_col, const u64* __restrict__ twL,
+                                                 const u64* __restrict__ ohi, const u64* __restrict__ olo,
+                                                 const u64* __restrict__ ohi_inv, const u64* __restrict__ olo_inv,
+                                                 int h) {
     extern __shared__ u64 lds[];
     const int R = 1 << L2;
     u64* la = lds;
@@ -247,25 +298,41 @@ __global__ __launch_bounds__(1024) void k_row_mul(const MulJob* __restrict__ job
     const MulJob& J = jobs ? jobs[blockIdx.x >> L1] : one;
     u64* __restrict__ A = J.wsA;
     const u64* __restrict__ B = J.wsB;
-    const size_t base = (size_t)(blockIdx.x & ((1u << L1) - 1)) * R;
-    for (int e = threadIdx.x; e < R; e += blockDim.x) {
-        la[lpad(e)] = A[base + e];
-        if constexpr (!SQUARE) lb[lpad(e)] = B[base + e];
+    const int row = (int)(blockIdx.x & ((1u << L1) - 1));
+    const size_t base = (size_t)row * R;
+    const u64 c = freq_rt(row, L1, lr_col);
+    const u64 N = (u64)1 << (L1 + L2);
+    {
+        // exponents mod N (w_N has order N; blockDim may exceed the row length)
+        u64 w = tw_lookup(ohi, olo, h, (c * threadIdx.x) & (N - 1));
+        const u64 step = tw_lookup(ohi, olo, h, (c * blockDim.x) & (N - 1));
+        for (int e = threadIdx.x; e < R; e += blockDim.x) {
+            la[lpad(e)] = gl_mul(A[base + e], w);
+            if constexpr (!SQUARE) lb[lpad(e)] = gl_mul(B[base + e], w);
+            w = gl_mul(w, step);
+        }
     }
     __syncthreads();
     if constexpr (SQUARE) {
         u64* arr[1] = {la};
-        lds_dft<false, 1>(arr, L2, 1, twL + R);
+        lds_dft<LR, false, 1>(arr, L2, 0, twL + R);
         for (int e = threadIdx.x; e < R; e += blockDim.x) { u64 v = la[lpad(e)]; la[lpad(e)] = gl_mul(v, v); }
     } else {
         u64* arr[2] = {la, lb};
-        lds_dft<false, 2>(arr, L2, 1, twL + R);
+        lds_dft<LR, false, 2>(arr, L2, 0, twL + R);
         for (int e = threadIdx.x; e < R; e += blockDim.x) la[lpad(e)] = gl_mul(la[lpad(e)], lb[lpad(e)]);
     }
     __syncthreads();
     u64* arr1[1] = {la};
-    lds_dft<true, 1>(arr1, L2, 1, twL + R);
-    for (int e = threadIdx.x; e < R; e += blockDim.x) A[base + e] = la[lpad(e)];
+    lds_dft<LR, true, 1>(arr1, L2, 0, twL + R);
+    {
+        u64 w = tw_lookup(ohi_inv, olo_inv, h, (c * threadIdx.x) & (N - 1));     // includes N^-1
+        const u64 step = tw_lookup(ohi, olo, h, (N - c * blockDim.x) & (N - 1)); // w_N^(-c blockDim)
+        for (int e = threadIdx.x; e < R; e += blockDim.x) {
+            A[base + e] = gl_mul(la[lpad(e)], w);
+            w = gl_mul(w, step);
+        }
+    }
 }
 
 // ---- carry propagation --------------------------------------------------------------
@@ -505,7 +572,7 @@ __device__ void block_carry(const u64* la, size_t nlimbs, u32* __restrict__ out,
 }
 
 // ---- whole multiply in one workgroup (N = 2^n <= 2^13), carry included -------------
-template <bool SQUARE, int NT>
+template <bool SQUARE, int NT, int LR>
 __global__ __launch_bounds__(NT) void k_small_mul(const MulJob* __restrict__ jobs, MulJob one, int n,
                                                   const u64* __restrict__ twL, u64 ninv) {
     extern __shared__ u64 lds[];
@@ -522,16 +589,16 @@ __global__ __launch_bounds__(NT) void k_small_mul(const MulJob* __restrict__ job
     __syncthreads();
     if constexpr (SQUARE) {
         u64* arr[1] = {la};
-        lds_dft<false, 1>(arr, n, 1, twL + R);
+        lds_dft<LR, false, 1>(arr, n, 0, twL + R);
         for (int e = threadIdx.x; e < R; e += blockDim.x) { u64 v = la[lpad(e)]; la[lpad(e)] = gl_mul(v, v); }
     } else {
         u64* arr[2] = {la, lb};
-        lds_dft<false, 2>(arr, n, 1, twL + R);
+        lds_dft<LR, false, 2>(arr, n, 0, twL + R);
         for (int e = threadIdx.x; e < R; e += blockDim.x) la[lpad(e)] = gl_mul(la[lpad(e)], lb[lpad(e)]);
     }
     __syncthreads();
     u64* arr1[1] = {la};
-    lds_dft<true, 1>(arr1, n, 1, twL + R);
+    lds_dft<LR, true, 1>(arr1, n, 0, twL + R);
     for (int e = threadIdx.x; e < R; e += blockDim.x) la[lpad(e)] = gl_mul(la[lpad(e)], ninv);
     __syncthreads();
     block_carry(la, J.na + J.nb, J.out, J.out_len, wsum, outs);
@@ -540,18 +607,47 @@ __global__ __launch_bounds__(NT) void k_small_mul(const MulJob* __restrict__ job
 __global__ void k_zero_len(u64* out_len) { *out_len = 0; }
 
 // ---- host launchers -------------------------------------------------------------------
-NttShape ntt_shape(int n) {
-    NttShape s{n, 0, n, 1};
+// ---- shapes: the default split per n, overridable by pdh_tune_set / autotune ----
+// Defaults measured on MI355X with the shape sweep (the autotuner's search
+// space): {L1, T} for n = 14..24, radix 8 for both passes.
+static NttShape default_shape(int n) {
+    NttShape s{n, 0, n, 1, 3, 3};
     if (n <= SMALL_MAX_LOG) return s;
-    int L1 = n / 2 < COL_MAX_LOG ? n / 2 : COL_MAX_LOG;
-    int L2 = n - L1;
-    if (L2 > ROW_MAX_LOG) { L2 = ROW_MAX_LOG; L1 = n - L2; }
-    s.L1 = L1; s.L2 = L2;
-    // widest column tile that fits 144 KB of LDS and the row length
-    int T = 64;
-    while (T > 8 && ((size_t)lds_words((1 << L1) * T) * 8 > 147456 || T > (1 << L2))) T >>= 1;
-    s.T = T;
+    static const int tab[MAX_LOG + 1][2] = {
+        {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0},
+        {5, 16}, {6, 8}, {7, 4}, {8, 8}, {8, 8}, {8, 16}, {8, 16}, {9, 16}, {9, 32}, {10, 16}, {11, 8}};
+    s.L1 = tab[n][0];
+    s.L2 = n - s.L1;
+    s.T = tab[n][1];
     return s;
+}
+
+static NttShape g_shapes[MAX_LOG + 1];
+static bool g_shapes_init = false;
+
+NttShape ntt_shape(int n) {
+    if (!g_shapes_init) {
+        for (int k = 0; k <= MAX_LOG; ++k) g_shapes[k] = default_shape(k);
+        g_shapes_init = true;
+    }
+    return g_shapes[n];
+}
+
+bool shape_valid(const NttShape& s) {
+    if (s.n < 1 || s.n > MAX_LOG) return false;
+    if (s.lr_col < 3 || s.lr_col > 4 || s.lr_row < 3 || s.lr_row > 4) return false;
+    if (s.L1 == 0) return s.n <= SMALL_MAX_LOG && s.L2 == s.n;
+    if (s.L1 < 1 || s.L2 < 1 || s.L1 + s.L2 != s.n || s.L2 > ROW_MAX_LOG) return false;
+    if (s.T != 4 && s.T != 8 && s.T != 16 && s.T != 32 && s.T != 64) return false;
+    if (s.T > (1 << s.L2)) return false;
+    return (size_t)lds_words((1 << s.L1) * s.T) * 8 <= 147456;
+}
+
+bool set_shape(const NttShape& s) {
+    if (!shape_valid(s)) return false;
+    ntt_shape(s.n);
+    g_shapes[s.n] = s;
+    return true;
 }
 
 template <class K>
@@ -565,6 +661,9 @@ static hipError_t set_lds(K kernel, size_t bytes) {
 #define PDH_LAUNCH(name, s, ...) do { hipEvent_t _t0 = timing_begin(s); hipLaunchKernelGGL(__VA_ARGS__); \
     PDH_LAUNCHED(); timing_end(name, _t0, s); } while (0)
 
+// Every (tile, radix) instantiation the launcher can pick.
+#define PDH_FOR_COL(X) X(4, 3) X(4, 4) X(8, 3) X(8, 4) X(16, 3) X(16, 4) X(32, 3) X(32, 4) X(64, 3) X(64, 4)
+
 // Dynamic-LDS limits are raised once, before any launch (and never while a
 // caller may be capturing a graph).
 hipError_t init_kernel_attrs() {
@@ -572,28 +671,30 @@ hipError_t init_kernel_attrs() {
     // padding); static __shared__ plus dynamic must stay within the 160 KB per
     // workgroup, which the runtime checks here.
     constexpr size_t MAXLDS = 147456;
-    PDH_TRY(set_lds(k_col_fwd<8>, MAXLDS));  PDH_TRY(set_lds(k_col_fwd<16>, MAXLDS));
-    PDH_TRY(set_lds(k_col_fwd<32>, MAXLDS)); PDH_TRY(set_lds(k_col_fwd<64>, MAXLDS));
-    PDH_TRY(set_lds(k_col_inv<8>, MAXLDS));  PDH_TRY(set_lds(k_col_inv<16>, MAXLDS));
-    PDH_TRY(set_lds(k_col_inv<32>, MAXLDS)); PDH_TRY(set_lds(k_col_inv<64>, MAXLDS));
-    PDH_TRY(set_lds(k_row_mul<true>, MAXLDS)); PDH_TRY(set_lds(k_row_mul<false>, MAXLDS));
-    PDH_TRY(set_lds(k_small_mul<true, 256>, MAXLDS)); PDH_TRY(set_lds(k_small_mul<false, 256>, MAXLDS));
-    PDH_TRY(set_lds(k_small_mul<true, 1024>, MAXLDS)); PDH_TRY(set_lds(k_small_mul<false, 1024>, MAXLDS));
+#define PDH_SET_COL(TT, RR) PDH_TRY(set_lds(k_col_fwd<TT, RR>, MAXLDS)); PDH_TRY(set_lds(k_col_inv<TT, RR>, MAXLDS));
+    PDH_FOR_COL(PDH_SET_COL)
+#undef PDH_SET_COL
+    PDH_TRY(set_lds(k_row_mul<true, 3>, MAXLDS)); PDH_TRY(set_lds(k_row_mul<false, 3>, MAXLDS));
+    PDH_TRY(set_lds(k_row_mul<true, 4>, MAXLDS)); PDH_TRY(set_lds(k_row_mul<false, 4>, MAXLDS));
+    PDH_TRY(set_lds((k_small_mul<true, 256, 3>), MAXLDS)); PDH_TRY(set_lds((k_small_mul<false, 256, 3>), MAXLDS));
+    PDH_TRY(set_lds((k_small_mul<true, 1024, 3>), MAXLDS)); PDH_TRY(set_lds((k_small_mul<false, 1024, 3>), MAXLDS));
+    PDH_TRY(set_lds((k_small_mul<true, 256, 4>), MAXLDS)); PDH_TRY(set_lds((k_small_mul<false, 256, 4>), MAXLDS));
+    PDH_TRY(set_lds((k_small_mul<true, 1024, 4>), MAXLDS)); PDH_TRY(set_lds((k_small_mul<false, 1024, 4>), MAXLDS));
     return hipSuccess;
 }
 
-template <int T>
+template <int T, int LR>
 static hipError_t launch_cols(const MulJob* d_jobs, const MulJob& one, unsigned njobs, int ops,
                               const u64* twL, const OuterTables& ot, const NttShape& sh, hipStream_t s,
                               bool inverse) {
     const size_t lds = (size_t)lds_words((1 << sh.L1) * T) * 8;
     const unsigned tiles = (unsigned)(((size_t)1 << sh.L2) / T);
     if (!inverse) {
-        PDH_LAUNCH("k_col_fwd", s, (k_col_fwd<T>), dim3(njobs * ops * tiles), dim3(1024), lds, s, d_jobs, one, ops,
-                   sh.n, sh.L1, twL, ot.hi, ot.lo, ot.h);
+        PDH_LAUNCH("k_col_fwd", s, (k_col_fwd<T, LR>), dim3(njobs * ops * tiles), dim3(1024), lds, s, d_jobs, one,
+                   ops, sh.n, sh.L1, twL);
     } else {
-        PDH_LAUNCH("k_col_inv", s, (k_col_inv<T>), dim3(njobs * tiles), dim3(1024), lds, s, d_jobs, one,
-                   sh.n, sh.L1, twL, ot.hi_inv, ot.lo_inv, ot.h);
+        PDH_LAUNCH("k_col_inv", s, (k_col_inv<T, LR>), dim3(njobs * tiles), dim3(1024), lds, s, d_jobs, one,
+                   sh.n, sh.L1, twL);
     }
     return hipSuccess;
 }
@@ -601,12 +702,10 @@ static hipError_t launch_cols(const MulJob* d_jobs, const MulJob& one, unsigned 
 static hipError_t launch_cols_T(const MulJob* d_jobs, const MulJob& one, unsigned njobs, int ops,
                                 const u64* twL, const OuterTables& ot, const NttShape& sh, hipStream_t s,
                                 bool inverse) {
-    switch (sh.T) {
-        case 8:  return launch_cols<8>(d_jobs, one, njobs, ops, twL, ot, sh, s, inverse);
-        case 16: return launch_cols<16>(d_jobs, one, njobs, ops, twL, ot, sh, s, inverse);
-        case 32: return launch_cols<32>(d_jobs, one, njobs, ops, twL, ot, sh, s, inverse);
-        default: return launch_cols<64>(d_jobs, one, njobs, ops, twL, ot, sh, s, inverse);
-    }
+#define PDH_CASE_COL(TT, RR) if (sh.T == TT && sh.lr_col == RR) return launch_cols<TT, RR>(d_jobs, one, njobs, ops, twL, ot, sh, s, inverse);
+    PDH_FOR_COL(PDH_CASE_COL)
+#undef PDH_CASE_COL
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_zero_len(uint64_t* out_len, hipStream_t s) {
@@ -624,6 +723,14 @@ size_t carry_ws_bytes(size_t nlimbs) {
     return nblk * sizeof(CarryFn) + nblk * sizeof(u64) + 256;
 }
 
+template <bool SQ, int NT>
+static hipError_t launch_small(const MulJob* d_jobs, const MulJob& one, unsigned njobs, int n, int lr, size_t lds,
+                               const u64* twL, u64 ninv, hipStream_t s) {
+    if (lr == 3) PDH_LAUNCH("k_small_mul", s, (k_small_mul<SQ, NT, 3>), dim3(njobs), dim3(NT), lds, s, d_jobs, one, n, twL, ninv);
+    else PDH_LAUNCH("k_small_mul", s, (k_small_mul<SQ, NT, 4>), dim3(njobs), dim3(NT), lds, s, d_jobs, one, n, twL, ninv);
+    return hipSuccess;
+}
+
 // All jobs of a launch share n. For n > SMALL_MAX_LOG each job needs wsA, wsB of
 // 2^n words and carry scratch of carry_ws_bytes(2^(n-1)) (or of its own
 // nlimbs when it is launched alone: nblk_max is the per-job carry block count).
@@ -635,22 +742,21 @@ hipError_t launch_class(const MulJob* d_jobs, const MulJob& one, unsigned njobs,
         const size_t lds = (size_t)(square ? 1 : 2) * lds_words((int)N) * 8;
         const u64 ninv = gl_inv((u64)N);
         // one workgroup per job: 1024 threads once the tile needs most of the LDS
-        if (n >= 12) {
-            if (square) PDH_LAUNCH("k_small_mul", s, (k_small_mul<true, 1024>), dim3(njobs), dim3(1024), lds, s, d_jobs, one, n, twL, ninv);
-            else PDH_LAUNCH("k_small_mul", s, (k_small_mul<false, 1024>), dim3(njobs), dim3(1024), lds, s, d_jobs, one, n, twL, ninv);
-        } else {
-            if (square) PDH_LAUNCH("k_small_mul", s, (k_small_mul<true, 256>), dim3(njobs), dim3(256), lds, s, d_jobs, one, n, twL, ninv);
-            else PDH_LAUNCH("k_small_mul", s, (k_small_mul<false, 256>), dim3(njobs), dim3(256), lds, s, d_jobs, one, n, twL, ninv);
-        }
-        return hipSuccess;
+        if (n >= 12) return square ? launch_small<true, 1024>(d_jobs, one, njobs, n, sh.lr_row, lds, twL, ninv, s)
+                                   : launch_small<false, 1024>(d_jobs, one, njobs, n, sh.lr_row, lds, twL, ninv, s);
+        return square ? launch_small<true, 256>(d_jobs, one, njobs, n, sh.lr_row, lds, twL, ninv, s)
+                      : launch_small<false, 256>(d_jobs, one, njobs, n, sh.lr_row, lds, twL, ninv, s);
     }
     PDH_TRY(launch_cols_T(d_jobs, one, njobs, square ? 1 : 2, twL, ot, sh, s, false));
     const size_t lds = (size_t)(square ? 1 : 2) * lds_words(1 << sh.L2) * 8;
     const dim3 grid((unsigned)(njobs << sh.L1));
-    if (square)
-        PDH_LAUNCH("k_row_sqr", s, k_row_mul<true>, grid, dim3(1024), lds, s, d_jobs, one, sh.L1, sh.L2, twL);
-    else
-        PDH_LAUNCH("k_row_mul", s, k_row_mul<false>, grid, dim3(1024), lds, s, d_jobs, one, sh.L1, sh.L2, twL);
+    if (sh.lr_row == 3) {
+        if (square) PDH_LAUNCH("k_row_sqr", s, (k_row_mul<true, 3>), grid, dim3(1024), lds, s, d_jobs, one, sh.L1, sh.L2, sh.lr_col, twL, ot.hi, ot.lo, ot.hi_inv, ot.lo_inv, ot.h);
+        else PDH_LAUNCH("k_row_mul", s, (k_row_mul<false, 3>), grid, dim3(1024), lds, s, d_jobs, one, sh.L1, sh.L2, sh.lr_col, twL, ot.hi, ot.lo, ot.hi_inv, ot.lo_inv, ot.h);
+    } else {
+        if (square) PDH_LAUNCH("k_row_sqr", s, (k_row_mul<true, 4>), grid, dim3(1024), lds, s, d_jobs, one, sh.L1, sh.L2, sh.lr_col, twL, ot.hi, ot.lo, ot.hi_inv, ot.lo_inv, ot.h);
+        else PDH_LAUNCH("k_row_mul", s, (k_row_mul<false, 4>), grid, dim3(1024), lds, s, d_jobs, one, sh.L1, sh.L2, sh.lr_col, twL, ot.hi, ot.lo, ot.hi_inv, ot.lo_inv, ot.h);
+    }
     PDH_TRY(launch_cols_T(d_jobs, one, njobs, 1, twL, ot, sh, s, true));
     const dim3 cgrid((unsigned)(njobs * nblk_max));
     PDH_LAUNCH("k_carry_reduce", s, k_carry_reduce, cgrid, dim3(CARRY_NT), 0, s, d_jobs, one, N, nblk_max);

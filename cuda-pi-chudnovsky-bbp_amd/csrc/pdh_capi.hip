@@ -543,6 +543,94 @@ int pdh_mul_batch_dev(const pdh_mul_desc* descs, size_t n, void* stream) {
     return rc;
 }
 
+int pdh_tune_get(int n, int* L1, int* T, int* lr_col, int* lr_row) {
+    if (n < 1 || n > MAX_LOG || !L1 || !T || !lr_col || !lr_row) return fail(PDH_E_ARG, "bad argument");
+    Ctx& c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    NttShape sh = ntt_shape(n);
+    *L1 = sh.L1; *T = sh.T; *lr_col = sh.lr_col; *lr_row = sh.lr_row;
+    return PDH_OK;
+}
+
+int pdh_tune_set(int n, int L1, int T, int lr_col, int lr_row) {
+    Ctx& c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    NttShape sh{n, L1, n - L1, T, lr_col, lr_row};
+    if (L1 == 0) { sh.L2 = n; sh.T = 1; sh.lr_col = 3; }
+    if (!set_shape(sh)) return fail(PDH_E_ARG, "shape cannot be launched");
+    return PDH_OK;
+}
+
+int pdh_autotune(int n_lo, int n_hi, int reps, void* stream) {
+    if (n_lo < 2 || n_hi > MAX_LOG || n_lo > n_hi || reps < 1) return fail(PDH_E_ARG, "bad autotune range");
+    Ctx& c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    int rc = init_locked(c);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    const size_t maxlimbs = (size_t)1 << (n_hi - 2);
+    u32 *a = nullptr, *b = nullptr, *out = nullptr;
+    u64* len = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto cleanup = [&]() {
+        if (a) (void)hipFree(a);
+        if (b) (void)hipFree(b);
+        if (out) (void)hipFree(out);
+        if (len) (void)hipFree(len);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    };
+    hipError_t e = hipMalloc(&a, maxlimbs * 4);
+    if (e == hipSuccess) e = hipMalloc(&b, maxlimbs * 4);
+    if (e == hipSuccess) e = hipMalloc(&out, maxlimbs * 8);
+    if (e == hipSuccess) e = hipMalloc(&len, 8);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) {
+        std::vector<u32> h(maxlimbs);
+        uint64_t x = 0x9e3779b97f4a7c15ULL;
+        for (auto& v : h) { x ^= x << 13; x ^= x >> 7; x ^= x << 17; v = (u32)x | 1u; }
+        e = hipMemcpy(a, h.data(), maxlimbs * 4, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(b, h.data(), maxlimbs * 4, hipMemcpyHostToDevice);
+    }
+    if (e != hipSuccess) { cleanup(); return fail_hip(e, "autotune setup"); }
+    for (int n = n_lo; n <= n_hi; ++n) {
+        const size_t na = (size_t)1 << (n - 2);
+        std::vector<NttShape> cands;
+        const NttShape def = ntt_shape(n);
+        if (n <= SMALL_MAX_LOG) {
+            for (int lr = 3; lr <= 4; ++lr) cands.push_back(NttShape{n, 0, n, 1, 3, lr});
+        } else {
+            for (int L1 = def.L1 - 2; L1 <= def.L1 + 2; ++L1)
+                for (int T : {4, 8, 16, 32, 64})
+                    for (int lc = 3; lc <= 4; ++lc)
+                        for (int lr = 3; lr <= 4; ++lr) {
+                            NttShape sh{n, L1, n - L1, T, lc, lr};
+                            if (shape_valid(sh)) cands.push_back(sh);
+                        }
+        }
+        NttShape best = def;
+        float best_ms = 1e30f;
+        for (const NttShape& sh : cands) {
+            set_shape(sh);
+            rc = mul_dev_locked(c, a, na, b, na, false, out, len, s);          // warm-up
+            if (rc) { set_shape(def); cleanup(); return rc; }
+            e = hipEventRecord(e0, s);
+            for (int r = 0; r < reps && e == hipSuccess; ++r)
+                if ((rc = mul_dev_locked(c, a, na, b, na, false, out, len, s))) { set_shape(def); cleanup(); return rc; }
+            if (e == hipSuccess) e = hipEventRecord(e1, s);
+            if (e == hipSuccess) e = hipEventSynchronize(e1);
+            float ms = 0.f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+            if (e != hipSuccess) { set_shape(def); cleanup(); return fail_hip(e, "autotune timing"); }
+            if (ms < best_ms) { best_ms = ms; best = sh; }
+        }
+        set_shape(best);
+    }
+    cleanup();
+    return PDH_OK;
+}
+
 int pdh_timing_enable(int on) {
     Timing& t = timing();
     std::lock_guard<std::mutex> g(t.mu);

@@ -18,6 +18,8 @@ struct NttShape {
     int L1;      // column sub-transform bits (0 for the one-workgroup path)
     int L2;      // row sub-transform bits
     int T;       // columns per column-pass workgroup
+    int lr_col;  // log2 of the column pass radix (3 or 4)
+    int lr_row;  // log2 of the row pass (and one-workgroup path) radix
 };
 
 constexpr int SMALL_MAX_LOG = 13;   // 2 x 2^13 u64 (+1/8 padding) = 144 KB LDS
@@ -26,6 +28,8 @@ constexpr int ROW_MAX_LOG   = 13;
 constexpr int COL_MAX_LOG   = 11;
 
 NttShape ntt_shape(int n);
+bool shape_valid(const NttShape& s);
+bool set_shape(const NttShape& s);      // false if the shape is not launchable
 
 // Device tables for one device: twL[2^L + x] = w_{2^L}^x for L <= 13, and
 // the per-n outer tables (w_N^e = hi[e >> h] * lo[e & (2^h-1)], forward; the

@@ -36,6 +36,7 @@ EXPORTED = (
     "pdh_mul_dev", "pdh_sqr_dev", "pdh_bbp_range", "pdh_bbp_positions",
     "pdh_bbp_positions_dev", "pdh_launch_count", "pdh_timing_enable", "pdh_timing_read",
     "pdh_batch_create", "pdh_batch_run", "pdh_batch_capture", "pdh_batch_destroy", "pdh_mul_batch_dev",
+    "pdh_tune_get", "pdh_tune_set", "pdh_autotune",
 )
 
 
@@ -91,6 +92,12 @@ def lib() -> ctypes.CDLL:
         L.pdh_timing_read.argtypes = [ctypes.c_char_p, _sz, ctypes.POINTER(ctypes.c_double), _u64p,
                                       ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
         L.pdh_timing_read.restype = ctypes.c_int
+        _ip = ctypes.POINTER(ctypes.c_int)
+        L.pdh_tune_get.argtypes = [ctypes.c_int, _ip, _ip, _ip, _ip]
+        L.pdh_tune_set.argtypes = [ctypes.c_int] * 5
+        L.pdh_autotune.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp]
+        for name in ("pdh_tune_get", "pdh_tune_set", "pdh_autotune"):
+            getattr(L, name).restype = ctypes.c_int
         L.pdh_batch_create.argtypes = [ctypes.POINTER(MulDesc), _sz, ctypes.POINTER(_vp)]
         L.pdh_batch_run.argtypes = [_vp, _vp]
         L.pdh_batch_capture.argtypes = [_vp, _vp]
@@ -130,6 +137,21 @@ def timing_read() -> dict:
     _check(lib().pdh_timing_read(names, 4096, ms, cnt, 64, ctypes.byref(k)))
     keys = names.value.decode().split("\n")[: k.value]
     return {name: (ms[i], int(cnt[i])) for i, name in enumerate(keys)}
+
+
+def tune_get(n: int) -> tuple:
+    """(L1, T, lr_col, lr_row) of the shape used for a 2^n-point transform."""
+    v = [ctypes.c_int() for _ in range(4)]
+    _check(lib().pdh_tune_get(n, *[ctypes.byref(x) for x in v]))
+    return tuple(x.value for x in v)
+
+
+def tune_set(n: int, L1: int, T: int, lr_col: int, lr_row: int) -> None:
+    _check(lib().pdh_tune_set(n, L1, T, lr_col, lr_row))
+
+
+def autotune(n_lo: int, n_hi: int, reps: int = 3, stream: int = 0) -> None:
+    _check(lib().pdh_autotune(n_lo, n_hi, reps, stream or None))
 
 
 def ntt_length(na: int, nb: int) -> int:

@@ -101,6 +101,21 @@ int pdh_batch_destroy(pdh_batch* plan);
 /* One-shot: create, run on `stream`, wait, destroy. */
 int pdh_mul_batch_dev(const pdh_mul_desc* descs, size_t n, void* stream);
 
+/* ---- transform shape tuning ----------------------------------------------
+ * A 2^n-point transform (n > 13) runs as a column pass over the top L1 index
+ * bits (T adjacent columns per workgroup, radix 2^lr_col butterflies) and a
+ * fused row pass over the low n - L1 bits (radix 2^lr_row); n <= 13 runs in
+ * one workgroup (radix 2^lr_row). Every valid shape gives the same product;
+ * only speed differs. No counterpart in the reference (its radix-2 transform
+ * has no parameters, src/ntt_cuda.cu:136-159). */
+int pdh_tune_get(int n, int* L1, int* T, int* lr_col, int* lr_row);
+/* PDH_E_ARG if the shape cannot be launched. */
+int pdh_tune_set(int n, int L1, int T, int lr_col, int lr_row);
+/* Time every candidate shape for each n in [n_lo, n_hi] on `stream` (reps
+ * timed runs each, on seeded operands in scratch buffers) and keep the
+ * fastest. Allocates and frees its scratch; the work is synchronous. */
+int pdh_autotune(int n_lo, int n_hi, int reps, void* stream);
+
 /* Hex digit values (0..15) at 1-based positions start .. start+count-1, host
  * buffer. Position 1 is the first hex digit after the point (value 2).
  * Replaces: std::string bbp_hex_digits_cuda(uint64_t start, uint32_t count)
