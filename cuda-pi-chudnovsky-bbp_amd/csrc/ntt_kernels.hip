@@ -172,7 +172,7 @@ __device__ __forceinline__ void lds_level_q(u64* const* arr, int L, int logM, in
 }
 
 template <int LR, bool INV, int NARR>
-__device__ void lds_dft(u64* const* arr, int L, int tlog, const u64* __restrict__ twL) {
+__device__ __forceinline__ void lds_dft(u64* const* arr, int L, int tlog, const u64* __restrict__ twL) {
     constexpr int RAD = 1 << LR;
     const int r0 = L % LR;
     if (!INV) {
@@ -392,7 +392,7 @@ __device__ __forceinline__ CarryFn shfl_up_fn(const CarryFn& f, int d) {
 
 // Inclusive scan of carry functions over the workgroup (thread 0 earliest);
 // wsum holds one entry per wave (at most 16).
-__device__ CarryFn block_scan(CarryFn f, CarryFn* wsum) {
+__device__ __forceinline__ CarryFn block_scan(CarryFn f, CarryFn* wsum) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
@@ -521,7 +521,7 @@ __global__ __launch_bounds__(CARRY_NT) void k_carry_apply(const MulJob* __restri
 
 // Carry of one product whose 2^n coefficients (scaled) sit in LDS: each of the
 // blockDim.x threads takes k = ceil(nlimbs / blockDim.x) consecutive limbs.
-__device__ void block_carry(const u64* la, size_t nlimbs, u32* __restrict__ out, u64* __restrict__ out_len,
+__device__ __forceinline__ void block_carry(const u64* la, size_t nlimbs, u32* __restrict__ out, u64* __restrict__ out_len,
                             CarryFn* wsum, u64* outs) {
     const size_t k = (nlimbs + blockDim.x - 1) / blockDim.x;
     const size_t l0 = (size_t)threadIdx.x * k;

@@ -543,6 +543,51 @@ int pdh_mul_batch_dev(const pdh_mul_desc* descs, size_t n, void* stream) {
     return rc;
 }
 
+int pdh_dev_alloc(size_t bytes, void** ptr) {
+    if (!ptr) return fail(PDH_E_ARG, "null pointer argument");
+    *ptr = nullptr;
+    Ctx& c = ctx();
+    { std::lock_guard<std::mutex> g(c.mu); int rc = init_locked(c); if (rc) return rc; }
+    if (bytes == 0) return PDH_OK;
+    HIPCALL(hipMalloc(ptr, bytes));
+    return PDH_OK;
+}
+int pdh_dev_free(void* ptr) {
+    if (ptr) HIPCALL(hipFree(ptr));
+    return PDH_OK;
+}
+int pdh_copy_htod(void* dst, const void* src, size_t bytes, void* stream) {
+    if (bytes == 0) return PDH_OK;
+    if (!dst || !src) return fail(PDH_E_ARG, "null pointer argument");
+    if (stream) HIPCALL(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, (hipStream_t)stream));
+    else HIPCALL(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return PDH_OK;
+}
+int pdh_copy_dtoh(void* dst, const void* src, size_t bytes, void* stream) {
+    if (bytes == 0) return PDH_OK;
+    if (!dst || !src) return fail(PDH_E_ARG, "null pointer argument");
+    if (stream) HIPCALL(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream));
+    else HIPCALL(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return PDH_OK;
+}
+int pdh_stream_create(void** stream) {
+    if (!stream) return fail(PDH_E_ARG, "null pointer argument");
+    Ctx& c = ctx();
+    { std::lock_guard<std::mutex> g(c.mu); int rc = init_locked(c); if (rc) return rc; }
+    hipStream_t s;
+    HIPCALL(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *stream = s;
+    return PDH_OK;
+}
+int pdh_stream_destroy(void* stream) {
+    if (stream) HIPCALL(hipStreamDestroy((hipStream_t)stream));
+    return PDH_OK;
+}
+int pdh_stream_sync(void* stream) {
+    HIPCALL(hipStreamSynchronize((hipStream_t)stream));
+    return PDH_OK;
+}
+
 int pdh_tune_get(int n, int* L1, int* T, int* lr_col, int* lr_row) {
     if (n < 1 || n > MAX_LOG || !L1 || !T || !lr_col || !lr_row) return fail(PDH_E_ARG, "bad argument");
     Ctx& c = ctx();

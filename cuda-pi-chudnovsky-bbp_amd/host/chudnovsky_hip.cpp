@@ -1,0 +1,569 @@
+// ============================================================================
+//  chudnovsky_hip.cpp -- Chudnovsky pi with every multiply on the ntt-hip seam.
+//
+//  Same digits as the reference's compute_pi_chudnovsky (src/chudnovsky.cpp:
+//  107-164): guard 16, prec = digits + 16, N = floor(prec / 14.1816...) + 2
+//  terms (:54-59, :118-120), P, Q, T over [0, N) (:64-102), then
+//      pi_scaled = floor( 426880 * Q * isqrt(10005 * 10^(2 prec)) / T )
+//  (:130-140) and the first `digits` fractional digits of its decimal string
+//  (:142-152). Every value on that path is an exact integer with a unique
+//  definition, so a different algorithm yields the same digits. What
+//  differs from the reference (the next rows of its own roadmap):
+//
+//  * Binary splitting runs bottom-up, level by level, with all of a level's
+//    P, Q, T products in one batched device launch sequence (pdh_mul_batch_dev).
+//    P(0,N), Q(0,N), T(0,N) do not depend on where the range is split.
+//  * Division is a Newton reciprocal with an exact floor correction (the
+//    reference forwards div_fast to Knuth, src/bignum_div.cpp:170-178); isqrt
+//    is Newton with divisions at doubling precision and an exact correction
+//    (reference: Newton over Knuth divisions, :189-204).
+//  * to_decimal is divide-and-conquer on powers of ten with batched
+//    divisions per level (reference: O(d^2) repeated /10^9, :239-267).
+//
+//  Multiplies go through a Multiplier (device by default). A caller may pass
+//  its own through pdh_pi_chudnovsky_cb -- the CPU tests do, with the oracle's
+//  ntt-cpu -- which lets the algorithm be checked on a machine with no GPU.
+// ============================================================================
+#include "pidigits_hip.h"
+#include "pdh_capi.h"
+#include "pdh_pi.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <string>
+#include <utility>
+#include <vector>
+
+namespace pidigits {
+namespace chud {
+
+typedef std::vector<uint32_t> Mag;
+typedef unsigned __int128 u128;
+
+struct Multiplier {
+    std::function<Mag(const Mag&, const Mag&)> mul;
+    std::function<Mag(const Mag&)> sqr;
+    std::function<std::vector<Mag>(const std::vector<std::pair<const Mag*, const Mag*>>&)> mul_many;
+};
+
+// ---- magnitude helpers (little-endian base 2^32, normalized) -----------------
+static void trim(Mag& a) { while (!a.empty() && a.back() == 0) a.pop_back(); }
+
+static size_t bitlen(const Mag& a) {
+    if (a.empty()) return 0;
+    return 32 * (a.size() - 1) + (32 - __builtin_clz(a.back()));
+}
+
+static int cmp(const Mag& a, const Mag& b) {
+    if (a.size() != b.size()) return a.size() < b.size() ? -1 : 1;
+    for (size_t i = a.size(); i-- > 0;)
+        if (a[i] != b[i]) return a[i] < b[i] ? -1 : 1;
+    return 0;
+}
+
+static Mag add(const Mag& a, const Mag& b) {
+    const Mag& x = a.size() >= b.size() ? a : b;
+    const Mag& y = a.size() >= b.size() ? b : a;
+    Mag r(x.size() + 1);
+    uint64_t c = 0;
+    for (size_t i = 0; i < x.size(); ++i) {
+        c += (uint64_t)x[i] + (i < y.size() ? y[i] : 0);
+        r[i] = (uint32_t)c;
+        c >>= 32;
+    }
+    r[x.size()] = (uint32_t)c;
+    trim(r);
+    return r;
+}
+
+// a - b, requires a >= b
+static Mag sub(const Mag& a, const Mag& b) {
+    Mag r(a.size());
+    int64_t br = 0;
+    for (size_t i = 0; i < a.size(); ++i) {
+        int64_t d = (int64_t)a[i] - (i < b.size() ? b[i] : 0) - br;
+        br = d < 0;
+        r[i] = (uint32_t)(d + (br << 32));
+    }
+    trim(r);
+    return r;
+}
+
+static Mag shl(const Mag& a, size_t bits) {
+    if (a.empty()) return a;
+    const size_t w = bits / 32, s = bits % 32;
+    Mag r(a.size() + w + 1, 0);
+    for (size_t i = 0; i < a.size(); ++i) {
+        r[i + w] |= a[i] << s;
+        if (s) r[i + w + 1] |= a[i] >> (32 - s);
+    }
+    trim(r);
+    return r;
+}
+
+static Mag shr(const Mag& a, size_t bits) {
+    const size_t w = bits / 32, s = bits % 32;
+    if (w >= a.size()) return {};
+    Mag r(a.size() - w);
+    for (size_t i = 0; i < r.size(); ++i) {
+        uint64_t v = a[i + w];
+        if (s && i + w + 1 < a.size()) v |= (uint64_t)a[i + w + 1] << 32;
+        r[i] = (uint32_t)(v >> s);
+    }
+    trim(r);
+    return r;
+}
+
+static Mag from_u128(u128 v) {
+    Mag r;
+    while (v) { r.push_back((uint32_t)v); v >>= 32; }
+    return r;
+}
+static u128 to_u128(const Mag& a) {
+    u128 v = 0;
+    for (size_t i = a.size(); i-- > 0;) v = (v << 32) | a[i];
+    return v;
+}
+
+static Mag mul_u32(const Mag& a, uint32_t m) {
+    if (a.empty() || m == 0) return {};
+    Mag r(a.size() + 1);
+    uint64_t c = 0;
+    for (size_t i = 0; i < a.size(); ++i) {
+        c += (uint64_t)a[i] * m;
+        r[i] = (uint32_t)c;
+        c >>= 32;
+    }
+    r[a.size()] = (uint32_t)c;
+    trim(r);
+    return r;
+}
+
+static Mag pow2(size_t e) { Mag r(e / 32 + 1, 0); r[e / 32] = 1u << (e % 32); return r; }
+
+static void inc(Mag& a) {
+    for (size_t i = 0; i < a.size(); ++i) if (++a[i] != 0) return;
+    a.push_back(1);
+}
+static void dec(Mag& a) {            // a > 0
+    for (size_t i = 0; i < a.size(); ++i) if (a[i]-- != 0) break;
+    trim(a);
+}
+
+// ---- signed values (binary splitting's T) --------------------------------------
+struct SInt { Mag m; int s = 0; };
+static SInt sadd(const SInt& a, const SInt& b) {
+    if (a.s == 0) return b;
+    if (b.s == 0) return a;
+    if (a.s == b.s) return SInt{add(a.m, b.m), a.s};
+    int c = cmp(a.m, b.m);
+    if (c == 0) return SInt{};
+    return c > 0 ? SInt{sub(a.m, b.m), a.s} : SInt{sub(b.m, a.m), b.s};
+}
+
+// ---- Newton reciprocal: about floor(2^(2L) / d), L = bitlen(d) -----------------
+// The top h bits of d give V_h ~ 2^(2h)/d_h; V0 = V_h << (L-h) has relative
+// error ~2^(1-h), and one Newton step V1 = V0 + V0 (2^(2L) - d V0) / 2^(2L)
+// squares it. With h = L/2 + 2 the result is within a few units; callers
+// correct exactly.
+static Mag recip(const Mag& d, const Multiplier& M) {
+    const size_t L = bitlen(d);
+    if (L <= 62) {
+        u128 v = ((u128)1 << (2 * L)) / (u128)to_u128(d);
+        return from_u128(v);
+    }
+    const size_t h = L / 2 + 2;
+    Mag V0 = shl(recip(shr(d, L - h), M), L - h);
+    Mag P = M.mul(d, V0);
+    Mag two = pow2(2 * L);
+    bool neg = cmp(P, two) > 0;
+    Mag e = neg ? sub(P, two) : sub(two, P);
+    Mag corr = shr(M.mul(V0, e), 2 * L);
+    return neg ? sub(V0, corr) : add(V0, corr);
+}
+
+// ---- exact floor division --------------------------------------------------------
+// q ~ a / d from a reciprocal of d scaled to P = k + 32 bits (k = quotient
+// bits): dp = d * 2^(P - nd) (truncated when nd > P), V ~ 2^(2P) / dp, so
+// a / d ~ a V / 2^(2P - (P - nd)). The low bits of a beyond what k + P bits of
+// precision need are dropped first. The estimate is within a few units; the
+// remainder r = a - q d then moves q to the exact floor.
+static void fix_quotient(const Mag& a, const Mag& d, const Mag& qd, Mag& q, Mag* rem) {
+    Mag r;
+    if (cmp(qd, a) > 0) {
+        Mag deficit = sub(qd, a);                 // a - q d < 0
+        while (!deficit.empty()) {
+            dec(q);
+            if (cmp(deficit, d) <= 0) { r = sub(d, deficit); deficit.clear(); }
+            else deficit = sub(deficit, d);
+        }
+    } else {
+        r = sub(a, qd);
+        while (cmp(r, d) >= 0) { r = sub(r, d); inc(q); }
+    }
+    if (rem) *rem = std::move(r);
+}
+
+static Mag div_floor(const Mag& a, const Mag& d, const Multiplier& M, Mag* rem) {
+    if (cmp(a, d) < 0) { if (rem) *rem = a; return {}; }
+    const size_t na = bitlen(a), nd = bitlen(d);
+    const size_t k = na - nd + 1;
+    const size_t P = k + 32;
+    const long sd = (long)P - (long)nd;
+    Mag dp = sd >= 0 ? shl(d, (size_t)sd) : shr(d, (size_t)(-sd));
+    Mag V = recip(dp, M);
+    const size_t ta = na > k + P + 2 ? na - (k + P + 2) : 0;
+    Mag q = shr(M.mul(shr(a, ta), V), (size_t)((long)(2 * P) - sd - (long)ta));
+    fix_quotient(a, d, M.mul(q, d), q, rem);
+    return q;
+}
+
+// ---- exact floor square root -------------------------------------------------------
+static Mag isqrt(const Mag& s, const Multiplier& M) {
+    const size_t L = bitlen(s);
+    if (L <= 100) {
+        u128 v = to_u128(s);
+        u128 x = (u128)std::sqrt((long double)v);
+        while (x * x > v) --x;
+        while ((x + 1) * (x + 1) <= v) ++x;
+        return from_u128(x);
+    }
+    const size_t t = (L - 64) / 4;
+    Mag x = add(shl(isqrt(shr(s, 2 * t), M), t), pow2(t));   // at or above sqrt(s)
+    x = shr(add(x, div_floor(s, x, M, nullptr)), 1);          // one Newton step
+    while (cmp(M.sqr(x), s) > 0) dec(x);
+    for (;;) {
+        Mag x1 = x; inc(x1);
+        if (cmp(M.sqr(x1), s) > 0) break;
+        x = x1;
+    }
+    return x;
+}
+
+static Mag pow10(uint64_t e, const Multiplier& M) {
+    Mag result{1}, base{10};
+    while (e) {
+        if (e & 1) result = M.mul(result, base);
+        e >>= 1;
+        if (e) base = M.sqr(base);
+    }
+    return result;
+}
+
+// ---- divide-and-conquer decimal conversion ---------------------------------------
+// Widths are W = B * 2^K digits; level l splits each node of width B*2^(K-l)
+// by 10^(B*2^(K-l-1)) with one reciprocal per level and the level's
+// quotient/remainder products batched. Leaves of B digits convert on the host
+// by repeated division by 10^9 (no big multiplies).
+static void leaf_digits(Mag x, size_t width, char* out) {
+    std::vector<uint32_t> chunks;                // base 10^9, least significant first
+    while (!x.empty()) {
+        uint64_t r = 0;
+        for (size_t i = x.size(); i-- > 0;) {
+            uint64_t cur = (r << 32) | x[i];
+            x[i] = (uint32_t)(cur / 1000000000u);
+            r = cur % 1000000000u;
+        }
+        trim(x);
+        chunks.push_back((uint32_t)r);
+    }
+    size_t pos = width;
+    for (uint32_t c : chunks) {
+        for (int k = 0; k < 9 && pos > 0; ++k) { out[--pos] = (char)('0' + c % 10); c /= 10; }
+    }
+    while (pos > 0) out[--pos] = '0';
+}
+
+static std::string to_decimal(const Mag& x, const Multiplier& M) {
+    if (x.empty()) return "0";
+    const size_t need = (size_t)((double)bitlen(x) * 0.30102999566398120) + 2;
+    int K = 0;
+    while ((need >> K) > 2304) ++K;
+    const size_t B = ((need + ((size_t)1 << K) - 1) / ((size_t)1 << K) + 8) / 9 * 9;   // leaf width, multiple of 9
+    const size_t W = B << K;
+    std::string out(W, '0');
+    std::vector<Mag> nodes{x};
+    for (int lvl = 0; lvl < K; ++lvl) {
+        const size_t half = (B << (K - lvl)) / 2;
+        Mag D = pow10(half, M);
+        const size_t Ld = bitlen(D);
+        // V ~ 2^(2 (Ld+32)) / (D 2^32): q ~ x V / 2^(2 Ld + 32), x < 10^(2 half)
+        Mag V = recip(shl(D, 32), M);
+        std::vector<Mag> xs(nodes.size());
+        std::vector<std::pair<const Mag*, const Mag*>> pairs;
+        for (size_t i = 0; i < nodes.size(); ++i) xs[i] = nodes[i];
+        for (size_t i = 0; i < nodes.size(); ++i) pairs.push_back({&xs[i], &V});
+        std::vector<Mag> prods = M.mul_many(pairs);
+        std::vector<Mag> qs(nodes.size());
+        for (size_t i = 0; i < nodes.size(); ++i) qs[i] = shr(prods[i], 2 * Ld + 32);
+        pairs.clear();
+        for (size_t i = 0; i < nodes.size(); ++i) pairs.push_back({&qs[i], &D});
+        std::vector<Mag> qds = M.mul_many(pairs);
+        std::vector<Mag> next;
+        next.reserve(2 * nodes.size());
+        for (size_t i = 0; i < nodes.size(); ++i) {
+            Mag q = qs[i], r;
+            fix_quotient(nodes[i], D, qds[i], q, &r);
+            next.push_back(std::move(q));
+            next.push_back(std::move(r));
+        }
+        nodes.swap(next);
+    }
+    for (size_t i = 0; i < nodes.size(); ++i) leaf_digits(nodes[i], B, &out[i * B]);
+    size_t nz = out.find_first_not_of('0');
+    return nz == std::string::npos ? "0" : out.substr(nz);
+}
+
+// ---- binary splitting, bottom-up with batched levels ---------------------------------
+struct PQT { Mag P, Q; SInt T; };
+
+static const uint64_t A_CONST = 13591409ULL;
+static const uint64_t B_CONST = 545140134ULL;
+static const uint64_t C3_OVER_24 = 10939058860032000ULL;
+
+static PQT leaf(uint64_t k) {
+    PQT r;
+    if (k == 0) { r.P = {1}; r.Q = {1}; r.T = SInt{from_u128(A_CONST), 1}; return r; }
+    u128 p = (u128)(6 * k - 5) * (2 * k - 1) * (6 * k - 1);
+    u128 q = (u128)k * k * k * C3_OVER_24;
+    u128 lin = (u128)A_CONST + (u128)B_CONST * k;
+    Mag pm = from_u128(p);
+    r.P = pm;
+    r.Q = from_u128(q);
+    // T = P * (A + B k) may exceed 128 bits: multiply by the 64-bit factor in two halves
+    Mag t = mul_u32(pm, (uint32_t)lin);
+    Mag t2 = shl(mul_u32(pm, (uint32_t)(lin >> 32)), 32);
+    r.T = SInt{add(t, t2), (k & 1) ? -1 : 1};
+    return r;
+}
+
+static PQT binary_split(uint64_t N, const Multiplier& M) {
+    std::vector<PQT> lvl(N);
+    for (uint64_t k = 0; k < N; ++k) lvl[k] = leaf(k);
+    while (lvl.size() > 1) {
+        const size_t pairs = lvl.size() / 2;
+        const bool root = lvl.size() == 2;
+        std::vector<std::pair<const Mag*, const Mag*>> jobs;
+        jobs.reserve(4 * pairs);
+        for (size_t i = 0; i < pairs; ++i) {
+            const PQT& L = lvl[2 * i];
+            const PQT& R = lvl[2 * i + 1];
+            jobs.push_back({&L.Q, &R.Q});
+            jobs.push_back({&L.T.m, &R.Q});
+            jobs.push_back({&L.P, &R.T.m});
+            if (!root) jobs.push_back({&L.P, &R.P});      // the root's P is never used
+        }
+        std::vector<Mag> prod = M.mul_many(jobs);
+        std::vector<PQT> next;
+        next.reserve(pairs + 1);
+        const size_t per = root ? 3 : 4;
+        for (size_t i = 0; i < pairs; ++i) {
+            const PQT& L = lvl[2 * i];
+            const PQT& R = lvl[2 * i + 1];
+            PQT r;
+            r.Q = std::move(prod[per * i]);
+            SInt a{std::move(prod[per * i + 1]), L.T.s};
+            SInt b{std::move(prod[per * i + 2]), R.T.s};
+            if (a.m.empty()) a.s = 0;
+            if (b.m.empty()) b.s = 0;
+            r.T = sadd(a, b);
+            if (!root) r.P = std::move(prod[per * i + 3]);
+            next.push_back(std::move(r));
+        }
+        if (lvl.size() & 1) next.push_back(std::move(lvl.back()));
+        lvl.swap(next);
+    }
+    return std::move(lvl[0]);
+}
+
+uint64_t terms_for_digits(uint64_t digits) {              // src/chudnovsky.cpp:54-59
+    const double DIGITS_PER_TERM = 14.181647462725477;
+    uint64_t n = (uint64_t)((double)digits / DIGITS_PER_TERM) + 2;
+    return n < 2 ? 2 : n;
+}
+
+std::string compute_pi(uint64_t digits, const Multiplier& M, double* phase_ms) {
+    using clk = std::chrono::steady_clock;
+    auto ms = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
+    const uint64_t guard = 16, prec = digits + guard;
+    const uint64_t N = terms_for_digits(prec);
+    auto t0 = clk::now();
+    PQT r = binary_split(N, M);
+    auto t1 = clk::now();
+    Mag one = pow10(prec, M);
+    Mag radicand = mul_u32(M.sqr(one), 10005u);
+    auto t2 = clk::now();
+    Mag sqrtC = isqrt(radicand, M);
+    auto t3 = clk::now();
+    Mag numerator = M.mul(mul_u32(r.Q, 426880u), sqrtC);
+    Mag pi_scaled = div_floor(numerator, r.T.m, M, nullptr);   // T > 0 for the full range
+    auto t4 = clk::now();
+    std::string s = to_decimal(pi_scaled, M);
+    auto t5 = clk::now();
+    if (phase_ms) {
+        phase_ms[0] = ms(t0, t1); phase_ms[1] = ms(t1, t2); phase_ms[2] = ms(t2, t3);
+        phase_ms[3] = ms(t3, t4); phase_ms[4] = ms(t4, t5); phase_ms[5] = (double)N;
+    }
+    std::string out;
+    if (digits == 0) return "3";
+    out.reserve(digits + 2);
+    out.push_back(s[0]);
+    out.push_back('.');
+    out.append(s, 1, (size_t)digits);
+    return out;
+}
+
+// ---- the device multiplier -------------------------------------------------------------
+static void check(int rc) {
+    if (rc != PDH_OK) throw HipError(rc, std::string("ntt-hip: ") + pdh_last_error());
+}
+
+// Products beyond one transform (na + nb > 2^23 limbs) are assembled from
+// blocks of at most 2^21 limbs each.
+static Mag dev_mul(const Mag& a, const Mag& b) {
+    if (a.empty() || b.empty()) return {};
+    const size_t LIM = (size_t)1 << 23, BLK = (size_t)1 << 21;
+    if (a.size() + b.size() <= LIM) return mul_ntt_hip(a, b);
+    Mag acc(a.size() + b.size() + 1, 0);
+    for (size_t i = 0; i < a.size(); i += BLK) {
+        Mag ai(a.begin() + i, a.begin() + std::min(a.size(), i + BLK));
+        trim(ai);
+        for (size_t j = 0; j < b.size(); j += BLK) {
+            Mag bj(b.begin() + j, b.begin() + std::min(b.size(), j + BLK));
+            trim(bj);
+            Mag p = mul_ntt_hip(ai, bj);
+            uint64_t c = 0;
+            size_t k = 0;
+            for (; k < p.size(); ++k) { c += (uint64_t)acc[i + j + k] + p[k]; acc[i + j + k] = (uint32_t)c; c >>= 32; }
+            for (size_t t = i + j + k; c; ++t) { c += acc[t]; acc[t] = (uint32_t)c; c >>= 32; }
+        }
+    }
+    trim(acc);
+    return acc;
+}
+
+static Mag dev_sqr(const Mag& a) {
+    if (a.empty()) return {};
+    if (2 * a.size() <= ((size_t)1 << 23)) return sqr_ntt_hip(a);
+    return dev_mul(a, a);
+}
+
+// One batched launch sequence for every pair that fits a transform.
+static std::vector<Mag> dev_mul_many(const std::vector<std::pair<const Mag*, const Mag*>>& pairs) {
+    std::vector<Mag> out(pairs.size());
+    std::vector<size_t> idx;
+    size_t la = 0, lb = 0, lo = 0;
+    for (size_t i = 0; i < pairs.size(); ++i) {
+        const Mag& a = *pairs[i].first;
+        const Mag& b = *pairs[i].second;
+        if (a.empty() || b.empty()) continue;
+        if (a.size() + b.size() > ((size_t)1 << 23)) { out[i] = dev_mul(a, b); continue; }
+        idx.push_back(i);
+        la += a.size(); lb += b.size(); lo += a.size() + b.size();
+    }
+    if (idx.empty()) return out;
+    std::vector<uint32_t> A(la), B(lb);
+    std::vector<size_t> ao(idx.size()), bo(idx.size()), oo(idx.size());
+    size_t pa = 0, pb = 0, po = 0;
+    for (size_t k = 0; k < idx.size(); ++k) {
+        const Mag& a = *pairs[idx[k]].first;
+        const Mag& b = *pairs[idx[k]].second;
+        ao[k] = pa; bo[k] = pb; oo[k] = po;
+        std::memcpy(&A[pa], a.data(), a.size() * 4);
+        std::memcpy(&B[pb], b.data(), b.size() * 4);
+        pa += a.size(); pb += b.size(); po += a.size() + b.size();
+    }
+    void *dA = nullptr, *dB = nullptr, *dO = nullptr, *dL = nullptr, *st = nullptr;
+    auto release = [&]() { pdh_dev_free(dA); pdh_dev_free(dB); pdh_dev_free(dO); pdh_dev_free(dL); pdh_stream_destroy(st); };
+    try {
+        check(pdh_stream_create(&st));
+        check(pdh_dev_alloc(la * 4, &dA));
+        check(pdh_dev_alloc(lb * 4, &dB));
+        check(pdh_dev_alloc(lo * 4, &dO));
+        check(pdh_dev_alloc(idx.size() * 8, &dL));
+        check(pdh_copy_htod(dA, A.data(), la * 4, st));
+        check(pdh_copy_htod(dB, B.data(), lb * 4, st));
+        std::vector<pdh_mul_desc> d(idx.size());
+        for (size_t k = 0; k < idx.size(); ++k) {
+            d[k].a = (const uint32_t*)dA + ao[k]; d[k].na = pairs[idx[k]].first->size();
+            d[k].b = (const uint32_t*)dB + bo[k]; d[k].nb = pairs[idx[k]].second->size();
+            d[k].out = (uint32_t*)dO + oo[k]; d[k].out_len = (uint64_t*)dL + k;
+        }
+        check(pdh_mul_batch_dev(d.data(), d.size(), st));
+        std::vector<uint32_t> O(lo);
+        std::vector<uint64_t> lens(idx.size());
+        check(pdh_copy_dtoh(O.data(), dO, lo * 4, st));
+        check(pdh_copy_dtoh(lens.data(), dL, idx.size() * 8, st));
+        check(pdh_stream_sync(st));
+        for (size_t k = 0; k < idx.size(); ++k) out[idx[k]].assign(O.begin() + oo[k], O.begin() + oo[k] + lens[k]);
+    } catch (...) {
+        release();
+        throw;
+    }
+    release();
+    return out;
+}
+
+Multiplier device_multiplier() {
+    Multiplier M;
+    M.mul = dev_mul;
+    M.sqr = dev_sqr;
+    M.mul_many = dev_mul_many;
+    return M;
+}
+
+} // namespace chud
+
+std::string compute_pi_chudnovsky_hip(uint64_t digits, double* phase_ms) {
+    return chud::compute_pi(digits, chud::device_multiplier(), phase_ms);
+}
+
+} // namespace pidigits
+
+// ---- C ABI ------------------------------------------------------------------------------
+extern "C" {
+
+static int emit(const std::string& s, char* out, size_t cap) {
+    if (s.size() + 1 > cap) return PDH_E_CAPACITY;
+    std::memcpy(out, s.c_str(), s.size() + 1);
+    return PDH_OK;
+}
+
+int pdh_pi_chudnovsky(uint64_t digits, char* out, size_t cap, double* phase_ms) {
+    try {
+        return emit(pidigits::compute_pi_chudnovsky_hip(digits, phase_ms), out, cap);
+    } catch (const pidigits::HipError& e) {
+        return e.code;
+    }
+}
+
+int pdh_pi_chudnovsky_cb(uint64_t digits, pdh_host_mul_fn mul, char* out, size_t cap, double* phase_ms) {
+    using namespace pidigits::chud;
+    Multiplier M;
+    M.mul = [mul](const Mag& a, const Mag& b) -> Mag {
+        if (a.empty() || b.empty()) return {};
+        Mag r(a.size() + b.size());
+        size_t n = 0;
+        if (mul(a.data(), a.size(), b.data(), b.size(), r.data(), r.size(), &n) != 0) throw pidigits::HipError(PDH_E_HIP, "callback multiply failed");
+        r.resize(n);
+        return r;
+    };
+    M.sqr = [M](const Mag& a) { return M.mul(a, a); };
+    M.mul_many = [M](const std::vector<std::pair<const Mag*, const Mag*>>& p) {
+        std::vector<Mag> r(p.size());
+        for (size_t i = 0; i < p.size(); ++i) r[i] = M.mul(*p[i].first, *p[i].second);
+        return r;
+    };
+    try {
+        return emit(compute_pi(digits, M, phase_ms), out, cap);
+    } catch (const pidigits::HipError& e) {
+        return e.code;
+    }
+}
+
+uint64_t pdh_chudnovsky_terms(uint64_t digits) { return pidigits::chud::terms_for_digits(digits + 16); }
+
+} // extern "C"

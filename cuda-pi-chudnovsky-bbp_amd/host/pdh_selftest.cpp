@@ -54,7 +54,14 @@ int main() {
         CHECK(sqr_ntt_hip(a) == schoolbook(a, a), "ntt-hip square == schoolbook");
     }
     CHECK(pdh_launch_count() > launches, "device kernels ran");
-    std::printf("[2] BBP hex digits\n");
+    std::printf("[2] Chudnovsky pi, every multiply on the device\n");
+    {
+        static const char* ref100 = "1415926535897932384626433832795028841971693993751058209749445923078164062862089986280348253421170679";
+        std::string p = compute_pi_chudnovsky_hip(8000);
+        CHECK(p.size() == 8002 && p.compare(0, 2, "3.") == 0, "8000-digit result has the right shape");
+        CHECK(p.compare(2, 100, ref100) == 0, "first 100 digits match");
+    }
+    std::printf("[3] BBP hex digits\n");
     CHECK(bbp_hex_digits_hip(1, 64) == "243F6A8885A308D313198A2E03707344A4093822299F31D0082EFA98EC4E6C89",
           "BBP 1..64 == reference");
     CHECK(bbp_hex_digits_hip(1000000, 10) == "26C65E52CB", "BBP checkpoint @1e6");

@@ -45,4 +45,9 @@ char bbp_hex_digit_hip(uint64_t position);
 // Digit values (0..15) at arbitrary positions.
 std::vector<uint8_t> bbp_values_hip(const std::vector<uint64_t>& positions);
 
+// "3." + the first `digits` decimal digits of pi, every multiply on the
+// device (reference: compute_pi_chudnovsky, src/chudnovsky.cpp:107-164).
+// phase_ms, if given, receives 6 values (see pdh_pi.h).
+std::string compute_pi_chudnovsky_hip(uint64_t digits, double* phase_ms = nullptr);
+
 } // namespace pidigits

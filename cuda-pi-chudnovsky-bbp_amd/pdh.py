@@ -36,7 +36,8 @@ EXPORTED = (
     "pdh_mul_dev", "pdh_sqr_dev", "pdh_bbp_range", "pdh_bbp_positions",
     "pdh_bbp_positions_dev", "pdh_launch_count", "pdh_timing_enable", "pdh_timing_read",
     "pdh_batch_create", "pdh_batch_run", "pdh_batch_capture", "pdh_batch_destroy", "pdh_mul_batch_dev",
-    "pdh_tune_get", "pdh_tune_set", "pdh_autotune",
+    "pdh_tune_get", "pdh_tune_set", "pdh_autotune", "pdh_dev_alloc", "pdh_dev_free", "pdh_copy_htod",
+    "pdh_copy_dtoh", "pdh_stream_create", "pdh_stream_destroy", "pdh_stream_sync",
 )
 
 

@@ -70,6 +70,17 @@ int pdh_mul_dev(const uint32_t* d_a, size_t na, const uint32_t* d_b, size_t nb,
 int pdh_sqr_dev(const uint32_t* d_a, size_t na, uint32_t* d_out, uint64_t* d_out_len,
                 void* stream);
 
+/* ---- device memory and streams for C/C++ callers ----------------------------
+ * Thin wrappers so host code that builds batches or keeps operands resident
+ * never includes HIP headers. Copies with a stream are asynchronous. */
+int pdh_dev_alloc(size_t bytes, void** ptr);
+int pdh_dev_free(void* ptr);
+int pdh_copy_htod(void* dst, const void* src, size_t bytes, void* stream);
+int pdh_copy_dtoh(void* dst, const void* src, size_t bytes, void* stream);
+int pdh_stream_create(void** stream);
+int pdh_stream_destroy(void* stream);
+int pdh_stream_sync(void* stream);
+
 /* ---- batched multiplies ----------------------------------------------------
  * Many independent multiplies of uneven sizes (e.g. one level of the
  * reference's binary-splitting tree, src/chudnovsky.cpp:64-102, whose P, Q, T
