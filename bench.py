@@ -146,6 +146,79 @@ def bench_batch(pdh, torch, W, stream, reps=3):
             "batched_equals_plain": same}
 
 
+def serve_trace(W, n_req, rate, seed=55):
+    """Poisson arrivals; 70% multiplies (limbs log-uniform [2^10, 2^22]), 20%
+    Chudnovsky (digits log-uniform [1e4, 1e7]), 10% BBP (16-256 hex digits at
+    a start uniform in [1e5, 1e8])."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, size=n_req))
+    reqs = []
+    for i in range(n_req):
+        u = rng.random()
+        if u < 0.7:
+            na, nb = (int(2 ** (10 + 12 * rng.random())) for _ in range(2))
+            reqs.append((float(t[i]), "mul", (W.operand(seed * 10007 + 2 * i, na), W.operand(seed * 10007 + 2 * i + 1, nb))))
+        elif u < 0.9:
+            reqs.append((float(t[i]), "pi", int(10 ** (4 + 3 * rng.random()))))
+        else:
+            reqs.append((float(t[i]), "bbp", (int(rng.integers(10**5, 10**8 + 1)), int(rng.integers(16, 257)))))
+    return reqs
+
+
+def bench_serve(pdh, W, n_req=40, rate=2.0):
+    """Continuous-batching server vs a serial plain ntt-hip loop on the same
+    trace; every server result must equal the serial one."""
+    import threading
+    import numpy as np
+    reqs = serve_trace(W, n_req, rate)
+
+    def run_serial():
+        out, lat = [], []
+        t0 = time.perf_counter()
+        for arr, kind, arg in reqs:
+            now = time.perf_counter() - t0
+            if now < arr:
+                time.sleep(arr - now)
+            if kind == "mul":
+                r = pdh.mul_ntt_hip(*arg)
+            elif kind == "pi":
+                r = pdh.pi_chudnovsky(arg)[0]
+            else:
+                r = pdh.bbp_values_range(*arg)
+            out.append(r)
+            lat.append((time.perf_counter() - t0 - arr) * 1e3)
+        return out, lat, time.perf_counter() - t0
+
+    def run_server():
+        srv = pdh.Server(200 << 30, 64)
+        tickets = [None] * len(reqs)
+        t0 = time.perf_counter()
+        for i, (arr, kind, arg) in enumerate(reqs):
+            now = time.perf_counter() - t0
+            if now < arr:
+                time.sleep(arr - now)
+            tickets[i] = (srv.submit_mul(*arg) if kind == "mul" else
+                          srv.submit_pi(arg) if kind == "pi" else srv.submit_bbp(*arg))
+        lat = [srv.wait(t) for t in tickets]
+        span = time.perf_counter() - t0
+        out = [srv.result(t) for t in tickets]
+        srv.close()
+        return out, lat, span
+
+    s_out, s_lat, s_span = run_server()
+    p_out, p_lat, p_span = run_serial()
+    same = all((np.array_equal(a, b) if not isinstance(a, str) else a == b) for a, b in zip(s_out, p_out))
+
+    def stats(lat, span):
+        lat = np.array(lat)
+        return {"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                "mean_ms": float(lat.mean()), "makespan_s": span, "req_per_s": len(lat) / span}
+    kinds = {k: sum(1 for _, kk, _ in reqs if kk == k) for k in ("mul", "pi", "bbp")}
+    return {"requests": len(reqs), "rate_per_s": rate, "mix": kinds, "server": stats(s_lat, s_span),
+            "serial_plain": stats(p_lat, p_span), "bit_exact_vs_serial": same}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -153,6 +226,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving trace (about a minute)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 128 MiB device-to-device copy (torch) so a --pmc profile can "
                          "scale FETCH_SIZE/WRITE_SIZE by a kernel of known traffic")
@@ -276,8 +350,38 @@ def main() -> int:
         M = sum(oracle.bbp_mulmod_count(p) for p in range(lo, hi + 1))
         extra["bbp_1_65536"] = {"ms": bms, "mulmod_per_s": M / (bms * 1e-3), "mulmods": M,
                                 "bit_exact": pdh.to_hex(vals.cpu().numpy()) == want}
+        # BBP at the 64 golden positions sampled uniformly in [1e6, 1e8]
+        gs = golden["bbp_sampled"]
+        spos = np.array(sorted(gs["positions"]), dtype=np.uint64)
+        order = np.argsort(np.array(gs["positions"], dtype=np.uint64), kind="stable")
+        dpos = torch.from_numpy(spos.view(np.int64)).to(dev)
+        svals = torch.zeros(len(spos), dtype=torch.uint8, device=dev)
+        e0.record(stream)
+        pdh.bbp_positions_dev(dpos.data_ptr(), len(spos), svals.data_ptr(), sp)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        sms = e0.elapsed_time(e1)
+        got = pdh.to_hex(svals.cpu().numpy())
+        want_sorted = "".join(gs["hex"][i] for i in order)
+        Ms = sum(oracle.bbp_mulmod_count(int(p)) for p in spos)
+        extra["bbp_sampled_64"] = {"ms": sms, "mulmod_per_s": Ms / (sms * 1e-3), "mulmods": Ms,
+                                   "positions": "64 uniform in [1e6, 1e8], seed 401",
+                                   "bit_exact": got == want_sorted}
 
         extra["c3_batch_4096"] = bench_batch(pdh, torch, W, stream, reps=3)
+        # Chudnovsky with every multiply on the device
+        g7 = golden.get("pi_dec_sha256", {})
+        for digits in (10**6, 10**7):
+            t = time.perf_counter()
+            s_pi, ph = pdh.pi_chudnovsky(digits)
+            dt = time.perf_counter() - t
+            ref = g7.get(str(digits))
+            extra[f"chudnovsky_{digits}"] = {
+                "s": dt, "phases_ms": {"split": ph[0], "pow10": ph[1], "sqrt": ph[2], "divide": ph[3], "format": ph[4]},
+                "terms": int(ph[5]),
+                "bit_exact": (hashlib.sha256(s_pi[2:].encode()).hexdigest() == ref["sha256"]) if ref else None}
+        if not args.no_serve:
+            extra["c5_serve"] = bench_serve(pdh, W)
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(a, b)
     line = {

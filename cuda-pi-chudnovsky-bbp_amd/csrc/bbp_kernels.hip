@@ -31,11 +31,28 @@
 
 namespace pdh {
 
+// 1/m to ~2^-80 relative: the hardware estimate (v_rcp_f64) refined by two
+// Newton steps. The quotient estimates below only need 2^-40.
+__device__ __forceinline__ double recip_f64(double m) {
+    double r = __builtin_amdgcn_rcp(m);
+    double e = fma(-m, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-m, r, 1.0);
+    return fma(r, e, r);
+}
+
 __device__ __forceinline__ double modpow16_exact(uint64_t e, double m, double minv) {
     double r = 1.0;
     if (e) {
-        const int top = 63 - __clzll((long long)e);
-        for (int bit = top; bit >= 0; --bit) {
+        int bit = 63 - __clzll((long long)e);
+        if (bit >= 2) {
+            // the top three exponent bits t in [4, 7]: 16^t = 2^(4t) is exact
+            const int t = (int)(e >> (bit - 2));
+            r = ldexp(1.0, 4 * t);
+            r = fma(-floor(r * minv), m, r);      // exact: |result| < 2m
+            bit -= 3;
+        }
+        for (; bit >= 0; --bit) {
             const double p = r * r;
             const double err = fma(r, r, -p);
             const double q = __builtin_rint(p * minv);
@@ -56,7 +73,7 @@ __device__ double bbp_series_dev(uint32_t j, uint64_t d) {
     for (uint64_t k = 0; k <= d; ++k) {
         const uint64_t mi = 8ull * k + j;
         const double m = (double)mi;
-        const double r = (mi == 1) ? 0.0 : modpow16_exact(d - k, m, 1.0 / m);
+        const double r = (mi == 1) ? 0.0 : modpow16_exact(d - k, m, recip_f64(m));
         sum += r / m;
         sum -= floor(sum);
     }
@@ -98,6 +115,139 @@ __global__ __launch_bounds__(256) void k_bbp(const uint64_t* __restrict__ pos, u
         if (s < 0.0) s += 1.0;
         out[pi] = (uint8_t)(int)(16.0 * s);
     }
+}
+
+// ---- few positions: chunked two-phase path ---------------------------------------
+// With few positions, one lane per series leaves most of the GPU idle while
+// each lane walks d + 1 terms. Here the head sum of every series s = 4 p + q
+// is split into chunks of CH terms: k_bbp_terms computes the chunk's terms
+// (double)r/(double)m in parallel (one lane per term), k_bbp_fold then adds
+// them to the series' running sum in ascending k with sum -= floor(sum)
+// after each add -- the reference's operation sequence, unchanged. Only the
+// exact integer residues are computed in parallel.
+struct BbpSeries { uint64_t d; double sum; uint32_t j; uint32_t valid; };
+
+__global__ __launch_bounds__(256) void k_bbp_init(const uint64_t* __restrict__ pos, uint64_t start, size_t count,
+                                                  BbpSeries* __restrict__ st) {
+    const size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= 4 * count) return;
+    const size_t pi = s >> 2;
+    const int q = (int)(s & 3);
+    const uint64_t position = pos ? pos[pi] : start + pi;
+    const bool valid = position >= 1 && position <= ((uint64_t)1 << 28);
+    st[s] = BbpSeries{valid ? position - 1 : 0, 0.0, q == 0 ? 1u : (uint32_t)(q + 3), valid ? 1u : 0u};
+}
+
+__global__ __launch_bounds__(256) void k_bbp_terms(const BbpSeries* __restrict__ st, size_t nseries, uint64_t k0,
+                                                   int log_ch, double* __restrict__ buf) {
+    const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t s = g >> log_ch;
+    if (s >= nseries) return;
+    const BbpSeries S = st[s];
+    const uint64_t k = k0 + (g & (((size_t)1 << log_ch) - 1));
+    if (!S.valid || k > S.d) return;
+    const uint64_t mi = 8ull * k + S.j;
+    const double m = (double)mi;
+    const double r = (mi == 1) ? 0.0 : modpow16_exact(S.d - k, m, recip_f64(m));
+    buf[g] = r / m;
+}
+
+__global__ __launch_bounds__(64) void k_bbp_fold(BbpSeries* __restrict__ st, size_t nseries, uint64_t k0,
+                                                 int log_ch, const double* __restrict__ buf) {
+    const size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nseries) return;
+    BbpSeries S = st[s];
+    if (!S.valid || k0 > S.d) return;
+    const uint64_t ch = (uint64_t)1 << log_ch;
+    const uint64_t n = (S.d + 1 - k0) < ch ? (S.d + 1 - k0) : ch;
+    const double* __restrict__ t = buf + (s << log_ch);
+    double sum = S.sum;
+    for (uint64_t i = 0; i < n; ++i) {
+        sum += t[i];
+        sum -= floor(sum);
+    }
+    st[s].sum = sum;
+}
+
+__global__ __launch_bounds__(256) void k_bbp_finish(const BbpSeries* __restrict__ st, size_t count,
+                                                    uint8_t* __restrict__ out) {
+    const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t pi = gid >> 2;
+    const bool active = pi < count;
+    double S = 0.0;
+    BbpSeries me{0, 0.0, 1, 0};
+    if (active) {
+        me = st[gid];
+        double sum = me.sum;
+        double p16 = 1.0 / 16.0;                 // tail, bbp_kernel.h:112-120
+        uint64_t k = me.d + 1;
+        while (p16 > 1e-18) {
+            sum += p16 / (double)(8ull * k + me.j);
+            p16 /= 16.0;
+            ++k;
+        }
+        sum -= floor(sum);
+        S = sum;
+    }
+    const int lane = threadIdx.x & 63;
+    const int g = lane & ~3;
+    const double s1 = __shfl(S, g + 0);
+    const double s4 = __shfl(S, g + 1);
+    const double s5 = __shfl(S, g + 2);
+    const double s6 = __shfl(S, g + 3);
+    if (active && (gid & 3) == 0) {
+        if (!me.valid) { out[pi] = 0xFF; return; }
+        double s = 4.0 * s1 - 2.0 * s4 - 1.0 * s5 - 1.0 * s6;
+        s -= floor(s);
+        if (s < 0.0) s += 1.0;
+        out[pi] = (uint8_t)(int)(16.0 * s);
+    }
+}
+
+__global__ void k_bbp_dmax(const uint64_t* __restrict__ pos, size_t count, unsigned long long* __restrict__ dmax) {
+    unsigned long long m = 0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x) {
+        const uint64_t p = pos[i];
+        if (p >= 1 && p <= ((uint64_t)1 << 28) && p - 1 > m) m = p - 1;
+    }
+    atomicMax(dmax, m);
+}
+
+size_t bbp_chunked_ws_bytes(size_t count) {
+    const size_t S = 4 * count;
+    int log_ch = 15;
+    while (log_ch > 8 && (S << log_ch) > ((size_t)1 << 24)) --log_ch;
+    return S * sizeof(BbpSeries) + (S << log_ch) * sizeof(double) + 256;
+}
+
+#define BBP_LAUNCH(name, s, ...) do { hipEvent_t _t0 = timing_begin(s); hipLaunchKernelGGL(__VA_ARGS__); \
+    g_launches.fetch_add(1, std::memory_order_relaxed); hipError_t _e = hipGetLastError(); \
+    timing_end(name, _t0, s); if (_e != hipSuccess) return _e; } while (0)
+
+// dmax: the largest d = position - 1 among valid positions (known to the host).
+hipError_t launch_bbp_chunked(const uint64_t* d_pos, uint64_t start, size_t count, uint64_t dmax, uint8_t* d_out,
+                              void* ws, hipStream_t s) {
+    const size_t S = 4 * count;
+    int log_ch = 15;
+    while (log_ch > 8 && (S << log_ch) > ((size_t)1 << 24)) --log_ch;
+    BbpSeries* st = (BbpSeries*)ws;
+    double* buf = (double*)((char*)ws + ((S * sizeof(BbpSeries) + 255) & ~(size_t)255));
+    BBP_LAUNCH("k_bbp_init", s, k_bbp_init, dim3((unsigned)((S + 255) / 256)), dim3(256), 0, s, d_pos, start, count, st);
+    const size_t lanes = S << log_ch;
+    for (uint64_t k0 = 0; k0 <= dmax; k0 += (uint64_t)1 << log_ch) {
+        BBP_LAUNCH("k_bbp_terms", s, k_bbp_terms, dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0, s,
+                   (const BbpSeries*)st, S, k0, log_ch, buf);
+        BBP_LAUNCH("k_bbp_fold", s, k_bbp_fold, dim3((unsigned)((S + 63) / 64)), dim3(64), 0, s, st, S, k0, log_ch,
+                   (const double*)buf);
+    }
+    BBP_LAUNCH("k_bbp_finish", s, k_bbp_finish, dim3((unsigned)((S + 255) / 256)), dim3(256), 0, s,
+               (const BbpSeries*)st, count, d_out);
+    return hipSuccess;
+}
+
+hipError_t launch_bbp_dmax(const uint64_t* d_pos, size_t count, uint64_t* d_dmax, hipStream_t s) {
+    BBP_LAUNCH("k_bbp_dmax", s, k_bbp_dmax, dim3(64), dim3(256), 0, s, d_pos, count, (unsigned long long*)d_dmax);
+    return hipSuccess;
 }
 
 hipError_t launch_bbp(const uint64_t* d_pos, uint64_t start, size_t count, uint8_t* d_out,

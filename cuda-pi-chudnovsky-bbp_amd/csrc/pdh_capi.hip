@@ -51,6 +51,8 @@ struct Ctx {
     u32* st_a = nullptr; u32* st_b = nullptr; u32* st_out = nullptr; size_t st_limbs = 0;
     u64* st_len = nullptr;
     uint64_t* st_pos = nullptr; uint8_t* st_val = nullptr; size_t st_bbp = 0;
+    void* bbp_ws = nullptr; size_t bbp_ws_bytes = 0;
+    uint64_t* bbp_dmax = nullptr;
     hipStream_t stream = nullptr;
 };
 
@@ -235,7 +237,15 @@ int bbp_host(const uint64_t* positions, uint64_t start, size_t count, uint8_t* v
         c.st_bbp = count;
     }
     if (positions) HIPCALL(hipMemcpyAsync(c.st_pos, sorted.data(), count * 8, hipMemcpyHostToDevice, c.stream));
-    hipError_t e = launch_bbp(positions ? c.st_pos : nullptr, start, count, c.st_val, c.stream);
+    hipError_t e;
+    if (count < BBP_CHUNKED_MAX) {
+        const uint64_t dmax = positions ? sorted.back() - 1 : start + count - 2;
+        int rc2 = grow((char**)&c.bbp_ws, &c.bbp_ws_bytes, bbp_chunked_ws_bytes(count), 1);
+        if (rc2) return rc2;
+        e = launch_bbp_chunked(positions ? c.st_pos : nullptr, start, count, dmax, c.st_val, c.bbp_ws, c.stream);
+    } else {
+        e = launch_bbp(positions ? c.st_pos : nullptr, start, count, c.st_val, c.stream);
+    }
     if (e != hipSuccess) return fail_hip(e, "bbp launch");
     std::vector<uint8_t> tmp(count);
     HIPCALL(hipMemcpyAsync(tmp.data(), c.st_val, count, hipMemcpyDeviceToHost, c.stream));
@@ -350,7 +360,23 @@ int pdh_bbp_positions_dev(const uint64_t* d_positions, size_t n, uint8_t* d_valu
     std::lock_guard<std::mutex> g(c.mu);
     int rc = init_locked(c);
     if (rc) return rc;
-    hipError_t e = launch_bbp(d_positions, 0, n, d_values, (hipStream_t)stream);
+    hipStream_t st = (hipStream_t)stream;
+    hipError_t e;
+    if (n < BBP_CHUNKED_MAX) {
+        // the chunk loop needs the largest position on the host
+        if (!c.bbp_dmax) HIPCALL(hipMalloc(&c.bbp_dmax, 8));
+        e = launch_zero_len(c.bbp_dmax, st);
+        if (e == hipSuccess) e = launch_bbp_dmax(d_positions, n, c.bbp_dmax, st);
+        uint64_t dmax = 0;
+        if (e == hipSuccess) e = hipMemcpyAsync(&dmax, c.bbp_dmax, 8, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) return fail_hip(e, "bbp position scan");
+        rc = grow((char**)&c.bbp_ws, &c.bbp_ws_bytes, bbp_chunked_ws_bytes(n), 1);
+        if (rc) return rc;
+        e = launch_bbp_chunked(d_positions, 0, n, dmax, d_values, c.bbp_ws, st);
+    } else {
+        e = launch_bbp(d_positions, 0, n, d_values, st);
+    }
     if (e != hipSuccess) return fail_hip(e, "bbp launch");
     return PDH_OK;
 }
@@ -368,6 +394,8 @@ struct pdh_batch {
     std::vector<hipEvent_t> events;       // fork event + one join event per stream
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    hipStream_t last = nullptr;           // stream of the latest run
+    bool ran = false;
 };
 
 namespace {
@@ -400,7 +428,7 @@ int batch_enqueue(pdh_batch* p, hipStream_t s) {
 void batch_free(pdh_batch* p) {
     if (p->exec) (void)hipGraphExecDestroy(p->exec);
     if (p->graph) (void)hipGraphDestroy(p->graph);
-    (void)hipDeviceSynchronize();       // a replayed graph ran on the caller's stream
+    if (p->ran) (void)hipStreamSynchronize(p->last);   // the caller's stream joins every branch
     for (auto st : p->streams) (void)hipStreamDestroy(st);
     for (auto ev : p->events) (void)hipEventDestroy(ev);
     if (p->arena) (void)hipFree(p->arena);
@@ -496,6 +524,8 @@ int pdh_batch_run(pdh_batch* p, void* stream) {
     if (!p) return fail(PDH_E_ARG, "null plan");
     Ctx& c = ctx();
     std::lock_guard<std::mutex> g(c.mu);
+    p->last = (hipStream_t)stream;
+    p->ran = true;
     if (p->exec) {
         HIPCALL(hipGraphLaunch(p->exec, (hipStream_t)stream));
         g_launches.fetch_add(1, std::memory_order_relaxed);

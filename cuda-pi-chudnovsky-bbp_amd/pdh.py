@@ -266,3 +266,95 @@ def bbp_hex_digits_hip(start: int, count: int) -> str:
 
 def bbp_hex_digit(position: int) -> str:
     return bbp_hex_digits_hip(position, 1)
+
+
+# ---- libpidigits_hip.so: Chudnovsky and the serving loop (include/pdh_pi.h) ----
+PI_LIB_PATH = os.path.join(HERE, "libpidigits_hip.so")
+PI_EXPORTED = ("pdh_pi_chudnovsky", "pdh_pi_chudnovsky_cb", "pdh_chudnovsky_terms", "pdh_server_create",
+               "pdh_server_submit_mul", "pdh_server_submit_pi", "pdh_server_submit_bbp", "pdh_server_wait",
+               "pdh_server_result", "pdh_server_destroy")
+_pilib = None
+
+
+def pilib() -> ctypes.CDLL:
+    global _pilib
+    if _pilib is None:
+        lib()                                   # torch first, then libpdh.so, then this
+        if not os.path.exists(PI_LIB_PATH):
+            raise PdhError(PDH_E_NODEVICE, f"{PI_LIB_PATH} not built")
+        L = ctypes.CDLL(PI_LIB_PATH)
+        L.pdh_pi_chudnovsky.argtypes = [ctypes.c_uint64, ctypes.c_char_p, _sz, ctypes.POINTER(ctypes.c_double)]
+        L.pdh_chudnovsky_terms.argtypes = [ctypes.c_uint64]
+        L.pdh_chudnovsky_terms.restype = ctypes.c_uint64
+        L.pdh_server_create.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(_vp)]
+        L.pdh_server_submit_mul.argtypes = [_vp, _u32p, _sz, _u32p, _sz, _u64p]
+        L.pdh_server_submit_pi.argtypes = [_vp, ctypes.c_uint64, _u64p]
+        L.pdh_server_submit_bbp.argtypes = [_vp, ctypes.c_uint64, ctypes.c_uint32, _u64p]
+        L.pdh_server_wait.argtypes = [_vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double)]
+        L.pdh_server_result.argtypes = [_vp, ctypes.c_uint64, _vp, _sz, ctypes.POINTER(_sz)]
+        L.pdh_server_destroy.argtypes = [_vp]
+        for name in ("pdh_pi_chudnovsky", "pdh_server_create", "pdh_server_submit_mul", "pdh_server_submit_pi",
+                     "pdh_server_submit_bbp", "pdh_server_wait", "pdh_server_result", "pdh_server_destroy"):
+            getattr(L, name).restype = ctypes.c_int
+        _pilib = L
+    return _pilib
+
+
+def pi_chudnovsky(digits: int):
+    """("3.1415...", phase ms [split, 10^prec, sqrt, divide, format, terms])."""
+    buf = ctypes.create_string_buffer(digits + 8)
+    ph = (ctypes.c_double * 6)()
+    _check(pilib().pdh_pi_chudnovsky(digits, buf, len(buf), ph))
+    return buf.value.decode(), list(ph)
+
+
+class Server:
+    """Continuous-batching request loop on the device (pdh_server_*)."""
+
+    def __init__(self, hbm_budget_bytes: int = 200 << 30, max_inflight: int = 64):
+        h = _vp()
+        _check(pilib().pdh_server_create(hbm_budget_bytes, max_inflight, ctypes.byref(h)))
+        self.h = h
+        self.kind = {}
+
+    def submit_mul(self, a, b) -> int:
+        a, b = _limbs(a), _limbs(b)
+        t = ctypes.c_uint64()
+        _check(pilib().pdh_server_submit_mul(self.h, _ptr(a, ctypes.c_uint32), len(a), _ptr(b, ctypes.c_uint32),
+                                             len(b), ctypes.byref(t)))
+        self.kind[t.value] = ("mul", len(a) + len(b))
+        return t.value
+
+    def submit_pi(self, digits: int) -> int:
+        t = ctypes.c_uint64()
+        _check(pilib().pdh_server_submit_pi(self.h, digits, ctypes.byref(t)))
+        self.kind[t.value] = ("pi", digits + 2)
+        return t.value
+
+    def submit_bbp(self, start: int, count: int) -> int:
+        t = ctypes.c_uint64()
+        _check(pilib().pdh_server_submit_bbp(self.h, start, count, ctypes.byref(t)))
+        self.kind[t.value] = ("bbp", count)
+        return t.value
+
+    def wait(self, ticket: int) -> float:
+        ms = ctypes.c_double()
+        _check(pilib().pdh_server_wait(self.h, ticket, ctypes.byref(ms)))
+        return ms.value
+
+    def result(self, ticket: int):
+        kind, n = self.kind.pop(ticket)
+        ln = _sz()
+        if kind == "mul":
+            out = np.zeros(max(n, 1), dtype=np.uint32)
+            _check(pilib().pdh_server_result(self.h, ticket, out.ctypes.data, n, ctypes.byref(ln)))
+            return out[: ln.value].copy()
+        buf = ctypes.create_string_buffer(max(n, 1))
+        _check(pilib().pdh_server_result(self.h, ticket, ctypes.cast(buf, _vp), n, ctypes.byref(ln)))
+        raw = buf.raw[: ln.value]
+        return raw.decode() if kind == "pi" else np.frombuffer(raw, dtype=np.uint8).copy()
+
+    def close(self) -> None:
+        if self.h:
+            _check(pilib().pdh_server_destroy(self.h))
+            self.h = None

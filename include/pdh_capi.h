@@ -140,7 +140,9 @@ int pdh_bbp_range(uint64_t start, uint32_t count, uint8_t* values);
 int pdh_bbp_positions(const uint64_t* positions, size_t n, uint8_t* values);
 
 /* Device-resident variant: d_positions and d_values are device pointers;
- * enqueued on `stream`. Positions must lie in [1, 2^28] (the kernel's exact
+ * enqueued on `stream`. Below 8192 positions the work is split into chunks
+ * of terms computed in parallel and folded in order per series, which first
+ * needs the largest position: that case synchronises `stream` once. Positions must lie in [1, 2^28] (the kernel's exact
  * FP64 modular arithmetic needs 8k + j < 2^31); the host entry points reject
  * others with PDH_E_ARG, and this one writes 0xFF for them. */
 int pdh_bbp_positions_dev(const uint64_t* d_positions, size_t n, uint8_t* d_values,

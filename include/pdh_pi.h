@@ -34,6 +34,27 @@ int pdh_pi_chudnovsky_cb(uint64_t digits, pdh_host_mul_fn mul, char* out, size_t
  * (src/chudnovsky.cpp:54-59, 118-120). */
 uint64_t pdh_chudnovsky_terms(uint64_t digits);
 
+/* ---- serving: a continuous-batching request loop on one GPU ----------------
+ * No counterpart in the reference (one computation per process run,
+ * src/main.cpp:93-179). Requests wait in arrival order until the HBM budget
+ * and the in-flight limit admit them; admitted requests run while earlier
+ * ones are still on the device (multiplies admitted together share one
+ * batched plan; BBP requests run in order on their own stream; Chudnovsky
+ * requests run on a host worker whose multiplies go to the device). Results
+ * stay in device memory (multiplies) until fetched. */
+typedef struct pdh_server pdh_server;
+int pdh_server_create(uint64_t hbm_budget_bytes, int max_inflight, pdh_server** srv);
+int pdh_server_submit_mul(pdh_server* s, const uint32_t* a, size_t na, const uint32_t* b, size_t nb, uint64_t* id);
+int pdh_server_submit_pi(pdh_server* s, uint64_t digits, uint64_t* id);
+int pdh_server_submit_bbp(pdh_server* s, uint64_t start, uint32_t count, uint64_t* id);
+/* Blocks until request `id` has finished; latency = finish - submission. */
+int pdh_server_wait(pdh_server* s, uint64_t id, double* latency_ms);
+/* Copies the result (multiply: normalized u32 limbs; pi: "3.14..." chars;
+ * BBP: digit values 0..15), *len in elements, and releases the request. */
+int pdh_server_result(pdh_server* s, uint64_t id, void* out, size_t cap, size_t* len);
+/* Finishes the requests already running, then frees the server. */
+int pdh_server_destroy(pdh_server* s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -12,10 +12,11 @@ import pytest
 from conftest import ROOT
 
 HEADER = os.path.join(ROOT, "include", "pdh_capi.h")
+PI_HEADER = os.path.join(ROOT, "include", "pdh_pi.h")
 
 
-def declared():
-    text = open(HEADER).read()
+def declared(header=HEADER):
+    text = open(header).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return sorted(set(re.findall(r"\b(pdh_[a-z_0-9]+)\s*\(", text)))
 
@@ -28,6 +29,15 @@ def test_header_declarations_are_exported():
     for n in names:
         assert hasattr(L, n), n
     assert set(names) == set(pdh.EXPORTED)
+
+
+def test_pi_header_declarations_are_exported():
+    import pdh
+    names = [n for n in declared(PI_HEADER) if n != "pdh_host_mul_fn"]
+    L = pdh.pilib()
+    for n in names:
+        assert hasattr(L, n), n
+    assert set(names) == set(pdh.PI_EXPORTED)
 
 
 def test_ntt_length_is_the_reference_rule():
