@@ -117,18 +117,41 @@ __global__ __launch_bounds__(256) void k_bbp(const uint64_t* __restrict__ pos, u
     }
 }
 
-// ---- few positions: chunked two-phase path ---------------------------------------
+// ---- few positions ------------------------------------------------------------------
 // With few positions, one lane per series leaves most of the GPU idle while
-// each lane walks d + 1 terms. Here the head sum of every series s = 4 p + q
-// is split into chunks of CH terms: k_bbp_terms computes the chunk's terms
-// (double)r/(double)m in parallel (one lane per term), k_bbp_fold then adds
-// them to the series' running sum in ascending k with sum -= floor(sum)
-// after each add -- the reference's operation sequence, unchanged. Only the
-// exact integer residues are computed in parallel.
+// each lane walks d + 1 terms, so the terms of every series are computed in
+// parallel (one lane per term). The reference's sum is an ordered FP64 fold,
+// which is inherently sequential; two paths handle it.
+//
+// Fixed-point path (default). Each head term r/m is also taken exactly to 64
+// fractional bits, F = floor(r 2^64 / m), and summed with wrapping u64 adds
+// (mod 2^64 is mod 1), in any order: an atomic reduction. The 14 tail terms
+// 16^-i/m (the reference's loop runs while p16 > 1e-18, i.e. i = 1..14) are
+// floor(2^(64-4i)/m). T = 4F1 - 2F4 - F5 - F6 (mod 2^64) then lies within
+// Mu(d) of 2^64 times the reference's FP64 value s (mod 2^64), where, in
+// units of 2^-64:
+//   head, per term: |fl(r/m) - r/m| <= 2^-54 and the add into [0, 2) rounds
+//     by <= 2^-53; sum -= floor(sum) is exact: 3 * 2^10 per term, d+1 terms;
+//   tail: 14 adds (<= 2^-53) and divides (<= 2^-58): below 15 * 2^11;
+//   combination 4 s1 - 2 s4 - s5 - s6: three roundings below 4, <= 3 * 2^12;
+//   so |2^64 s - 2^64 s_exact| <= 8 (3072 (d+1) + 30720) + 12288, and the
+//   floors of the fixed point lose below d+15 units per series, at most
+//   4 (d+15) after the combination: Mu(d) = 24584 (d+1) + 262144 bounds all.
+// When T is farther than 2 Mu(d) from every multiple of 2^60, the
+// reference's digit floor(16 frac(s)) equals T >> 60 (both values lie
+// strictly inside the same sixteenth of the circle). Otherwise the position
+// is marked undecided (0xFE) and recomputed by the ordered path below. With
+// d = 10^8 that happens with probability about 4e-6 per position.
+//
+// Ordered path (fallback, or forced by pdh_bbp_set_exact). The head sum is
+// split into chunks of CH terms: k_bbp_terms computes each chunk's terms
+// (double)r/(double)m in parallel, k_bbp_fold then adds them to the series'
+// running sum in ascending k with sum -= floor(sum) after each add -- the
+// reference's operation sequence, unchanged.
 struct BbpSeries { uint64_t d; double sum; uint32_t j; uint32_t valid; };
 
 __global__ __launch_bounds__(256) void k_bbp_init(const uint64_t* __restrict__ pos, uint64_t start, size_t count,
-                                                  BbpSeries* __restrict__ st) {
+                                                  BbpSeries* __restrict__ st, uint64_t* __restrict__ acc) {
     const size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= 4 * count) return;
     const size_t pi = s >> 2;
@@ -136,15 +159,89 @@ __global__ __launch_bounds__(256) void k_bbp_init(const uint64_t* __restrict__ p
     const uint64_t position = pos ? pos[pi] : start + pi;
     const bool valid = position >= 1 && position <= ((uint64_t)1 << 28);
     st[s] = BbpSeries{valid ? position - 1 : 0, 0.0, q == 0 ? 1u : (uint32_t)(q + 3), valid ? 1u : 0u};
+    if (acc) acc[s] = 0;
+}
+
+// floor(r 2^32 / m) and the remainder, for integers 0 <= r < m < 2^31; c32 =
+// 2^32 / m to ~2^-50 relative, so the estimate is within 2^-18 of the true
+// quotient (below 2^32 - 2) and one correction step suffices.
+__device__ __forceinline__ uint32_t div_step(uint32_t r, uint32_t m, double c32, uint32_t& rem) {
+    uint32_t q = (uint32_t)((double)r * c32);
+    const int64_t x = (int64_t)(((uint64_t)r << 32) - (uint64_t)q * m);
+    int64_t t = x;
+    if (t < 0) { --q; t += m; } else if (t >= (int64_t)m) { ++q; t -= m; }
+    rem = (uint32_t)t;
+    return q;
+}
+
+// floor(r 2^64 / m) for integers 0 <= r < m < 2^31.
+__device__ __forceinline__ uint64_t frac64(uint32_t r, uint32_t m, double minv) {
+    const double c32 = minv * 4294967296.0;
+    uint32_t rem;
+    const uint32_t hi = div_step(r, m, c32, rem);
+    const uint32_t lo = div_step(rem, m, c32, rem);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Virtual block v: series v / bps, terms [kb * 256 * KT, (kb + 1) * 256 * KT).
+// Lane t takes k = base + i * 256 + t, so a wavefront's exponents d - k are
+// consecutive and its square-and-multiply loops have the same length.
+__global__ __launch_bounds__(256) void k_bbp_fx_terms(const BbpSeries* __restrict__ st, unsigned bps, int KT,
+                                                      uint64_t* __restrict__ acc) {
+    const unsigned s = blockIdx.x / bps;
+    const uint64_t kb = blockIdx.x - (uint64_t)s * bps;
+    const BbpSeries S = st[s];
+    const uint64_t base = kb * 256u * (uint64_t)KT;
+    if (!S.valid || base > S.d) return;
+    uint64_t local = 0;
+    for (int i = 0; i < KT; ++i) {
+        const uint64_t k = base + (uint64_t)i * 256u + threadIdx.x;
+        if (k > S.d) break;
+        const uint64_t mi = 8ull * k + S.j;
+        if (mi == 1) continue;                        // 16^d mod 1 = 0
+        const double m = (double)mi;
+        const double minv = recip_f64(m);
+        const double r = modpow16_exact(S.d - k, m, minv);
+        local += frac64((uint32_t)r, (uint32_t)mi, minv);
+    }
+    for (int o = 32; o > 0; o >>= 1) local += __shfl_xor(local, o);
+    __shared__ uint64_t part[4];
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = local;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        atomicAdd((unsigned long long*)&acc[s], (unsigned long long)(part[0] + part[1] + part[2] + part[3]));
+}
+
+__global__ __launch_bounds__(256) void k_bbp_fx_finish(const BbpSeries* __restrict__ st, size_t count,
+                                                       const uint64_t* __restrict__ acc,
+                                                       uint8_t* __restrict__ out, int odd_undecided) {
+    const size_t pi = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (pi >= count) return;
+    const BbpSeries* P = st + 4 * pi;
+    if (!P[0].valid) { out[pi] = 0xFF; return; }
+    const uint64_t d = P[0].d;
+    uint64_t F[4];
+    for (int q = 0; q < 4; ++q) {
+        uint64_t f = acc[4 * pi + q];
+        for (int i = 1; i <= 14; ++i) f += ((uint64_t)1 << (64 - 4 * i)) / (8ull * (d + i) + P[q].j);
+        F[q] = f;
+    }
+    const uint64_t T = 4 * F[0] - 2 * F[1] - F[2] - F[3];
+    const uint64_t cell = (uint64_t)1 << 60;
+    const uint64_t x = T & (cell - 1);
+    const uint64_t dist = x < cell - x ? x : cell - x;
+    const uint64_t mu = 24584ull * (d + 1) + 262144ull;
+    const bool decided = dist > 2 * mu && !(odd_undecided && (pi & 1));
+    out[pi] = decided ? (uint8_t)(T >> 60) : (uint8_t)0xFE;
 }
 
 __global__ __launch_bounds__(256) void k_bbp_terms(const BbpSeries* __restrict__ st, size_t nseries, uint64_t k0,
-                                                   int log_ch, double* __restrict__ buf) {
+                                                   size_t lanes, double* __restrict__ buf) {
     const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const size_t s = g >> log_ch;
-    if (s >= nseries) return;
+    if (g >= lanes) return;
+    const size_t i = g / nseries, s = g - i * nseries;        // term-major: buf[i * nseries + s]
     const BbpSeries S = st[s];
-    const uint64_t k = k0 + (g & (((size_t)1 << log_ch) - 1));
+    const uint64_t k = k0 + i;
     if (!S.valid || k > S.d) return;
     const uint64_t mi = 8ull * k + S.j;
     const double m = (double)mi;
@@ -153,19 +250,21 @@ __global__ __launch_bounds__(256) void k_bbp_terms(const BbpSeries* __restrict__
 }
 
 __global__ __launch_bounds__(64) void k_bbp_fold(BbpSeries* __restrict__ st, size_t nseries, uint64_t k0,
-                                                 int log_ch, const double* __restrict__ buf) {
+                                                 uint64_t ch, const double* __restrict__ buf) {
     const size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nseries) return;
     BbpSeries S = st[s];
     if (!S.valid || k0 > S.d) return;
-    const uint64_t ch = (uint64_t)1 << log_ch;
     const uint64_t n = (S.d + 1 - k0) < ch ? (S.d + 1 - k0) : ch;
-    const double* __restrict__ t = buf + (s << log_ch);
+    const double* __restrict__ t = buf + s;
     double sum = S.sum;
-    for (uint64_t i = 0; i < n; ++i) {
-        sum += t[i];
-        sum -= floor(sum);
+    uint64_t i = 0;
+    for (; i + 8 <= n; i += 8) {                      // eight loads in flight, then the ordered adds
+        double v[8];
+        for (int u = 0; u < 8; ++u) v[u] = t[(i + u) * nseries];
+        for (int u = 0; u < 8; ++u) { sum += v[u]; sum -= floor(sum); }
     }
+    for (; i < n; ++i) { sum += t[i * nseries]; sum -= floor(sum); }
     st[s].sum = sum;
 }
 
@@ -213,11 +312,18 @@ __global__ void k_bbp_dmax(const uint64_t* __restrict__ pos, size_t count, unsig
     atomicMax(dmax, m);
 }
 
-size_t bbp_chunked_ws_bytes(size_t count) {
-    const size_t S = 4 * count;
+static int chunk_log(size_t S) {
     int log_ch = 15;
     while (log_ch > 8 && (S << log_ch) > ((size_t)1 << 24)) --log_ch;
-    return S * sizeof(BbpSeries) + (S << log_ch) * sizeof(double) + 256;
+    return log_ch;
+}
+
+size_t bbp_few_ws_bytes(size_t count) {
+    const size_t S = 4 * count;
+    const size_t head = (S * sizeof(BbpSeries) + 255) & ~(size_t)255;
+    const size_t acc = (S * 8 + 255) & ~(size_t)255;
+    const size_t buf = (S << chunk_log(S)) * sizeof(double);
+    return head + (acc > buf ? acc : buf);
 }
 
 #define BBP_LAUNCH(name, s, ...) do { hipEvent_t _t0 = timing_begin(s); hipLaunchKernelGGL(__VA_ARGS__); \
@@ -225,19 +331,38 @@ size_t bbp_chunked_ws_bytes(size_t count) {
     timing_end(name, _t0, s); if (_e != hipSuccess) return _e; } while (0)
 
 // dmax: the largest d = position - 1 among valid positions (known to the host).
+hipError_t launch_bbp_fx(const uint64_t* d_pos, uint64_t start, size_t count, uint64_t dmax, uint8_t* d_out,
+                         void* ws, int odd_undecided, hipStream_t s) {
+    const size_t S = 4 * count;
+    BbpSeries* st = (BbpSeries*)ws;
+    uint64_t* acc = (uint64_t*)((char*)ws + ((S * sizeof(BbpSeries) + 255) & ~(size_t)255));
+    BBP_LAUNCH("k_bbp_init", s, k_bbp_init, dim3((unsigned)((S + 255) / 256)), dim3(256), 0, s, d_pos, start, count,
+               st, acc);
+    // terms per lane: at least 32, more when the grid would pass 2^23 blocks
+    int KT = 32;
+    uint64_t bps = (dmax + 1 + 256ull * KT - 1) / (256ull * KT);
+    while (bps * S > ((uint64_t)1 << 23)) { KT *= 2; bps = (dmax + 1 + 256ull * KT - 1) / (256ull * KT); }
+    BBP_LAUNCH("k_bbp_fx_terms", s, k_bbp_fx_terms, dim3((unsigned)(bps * S)), dim3(256), 0, s,
+               (const BbpSeries*)st, (unsigned)bps, KT, acc);
+    BBP_LAUNCH("k_bbp_fx_finish", s, k_bbp_fx_finish, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s,
+               (const BbpSeries*)st, count, (const uint64_t*)acc, d_out, odd_undecided);
+    return hipSuccess;
+}
+
 hipError_t launch_bbp_chunked(const uint64_t* d_pos, uint64_t start, size_t count, uint64_t dmax, uint8_t* d_out,
                               void* ws, hipStream_t s) {
     const size_t S = 4 * count;
-    int log_ch = 15;
-    while (log_ch > 8 && (S << log_ch) > ((size_t)1 << 24)) --log_ch;
+    const int log_ch = chunk_log(S);
     BbpSeries* st = (BbpSeries*)ws;
     double* buf = (double*)((char*)ws + ((S * sizeof(BbpSeries) + 255) & ~(size_t)255));
-    BBP_LAUNCH("k_bbp_init", s, k_bbp_init, dim3((unsigned)((S + 255) / 256)), dim3(256), 0, s, d_pos, start, count, st);
+    BBP_LAUNCH("k_bbp_init", s, k_bbp_init, dim3((unsigned)((S + 255) / 256)), dim3(256), 0, s, d_pos, start, count,
+               st, (uint64_t*)nullptr);
+    const uint64_t ch = (uint64_t)1 << log_ch;
     const size_t lanes = S << log_ch;
-    for (uint64_t k0 = 0; k0 <= dmax; k0 += (uint64_t)1 << log_ch) {
+    for (uint64_t k0 = 0; k0 <= dmax; k0 += ch) {
         BBP_LAUNCH("k_bbp_terms", s, k_bbp_terms, dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0, s,
-                   (const BbpSeries*)st, S, k0, log_ch, buf);
-        BBP_LAUNCH("k_bbp_fold", s, k_bbp_fold, dim3((unsigned)((S + 63) / 64)), dim3(64), 0, s, st, S, k0, log_ch,
+                   (const BbpSeries*)st, S, k0, lanes, buf);
+        BBP_LAUNCH("k_bbp_fold", s, k_bbp_fold, dim3((unsigned)((S + 63) / 64)), dim3(64), 0, s, st, S, k0, ch,
                    (const double*)buf);
     }
     BBP_LAUNCH("k_bbp_finish", s, k_bbp_finish, dim3((unsigned)((S + 255) / 256)), dim3(256), 0, s,

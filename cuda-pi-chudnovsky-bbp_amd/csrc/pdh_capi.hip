@@ -53,6 +53,10 @@ struct Ctx {
     uint64_t* st_pos = nullptr; uint8_t* st_val = nullptr; size_t st_bbp = 0;
     void* bbp_ws = nullptr; size_t bbp_ws_bytes = 0;
     uint64_t* bbp_dmax = nullptr;
+    uint64_t* fix_pos = nullptr; size_t fix_pos_n = 0;
+    uint8_t* fix_val = nullptr; size_t fix_val_n = 0;
+    int bbp_exact = 0;               // pdh_bbp_set_exact mode
+    uint64_t bbp_exact_count = 0;    // positions computed by the ordered path
     hipStream_t stream = nullptr;
 };
 
@@ -206,6 +210,61 @@ int mul_host(const u32* a, size_t na, const u32* b, size_t nb, bool square,
 
 constexpr uint64_t BBP_POS_MAX = (uint64_t)1 << 28;   // 8(pos-1)+6 < 2^31
 
+// Few positions (count < BBP_CHUNKED_MAX): the fixed-point path, then the
+// ordered FP64 path for every position it leaves undecided (or for all of
+// them when pdh_bbp_set_exact is on). d_pos (or start for a contiguous
+// range) and d_out are device buffers; h_pos, when given, holds the same
+// positions on the host. The values end in d_out and in h_vals. Synchronises
+// `st`.
+int bbp_few_locked(Ctx& c, const uint64_t* d_pos, const uint64_t* h_pos, uint64_t start, size_t count,
+                   uint64_t dmax, uint8_t* d_out, uint8_t* h_vals, hipStream_t st) {
+    int rc = grow((char**)&c.bbp_ws, &c.bbp_ws_bytes, bbp_few_ws_bytes(count), 1);
+    if (rc) return rc;
+    if (c.bbp_exact == 1) {
+        hipError_t e = launch_bbp_chunked(d_pos, start, count, dmax, d_out, c.bbp_ws, st);
+        if (e != hipSuccess) return fail_hip(e, "bbp launch");
+        HIPCALL(hipMemcpyAsync(h_vals, d_out, count, hipMemcpyDeviceToHost, st));
+        HIPCALL(hipStreamSynchronize(st));
+        c.bbp_exact_count += count;
+        return PDH_OK;
+    }
+    hipError_t e = launch_bbp_fx(d_pos, start, count, dmax, d_out, c.bbp_ws, c.bbp_exact == 2, st);
+    if (e != hipSuccess) return fail_hip(e, "bbp launch");
+    HIPCALL(hipMemcpyAsync(h_vals, d_out, count, hipMemcpyDeviceToHost, st));
+    HIPCALL(hipStreamSynchronize(st));
+    std::vector<size_t> und;
+    for (size_t i = 0; i < count; ++i)
+        if (h_vals[i] == 0xFE) und.push_back(i);
+    if (und.empty()) return PDH_OK;
+    std::vector<uint64_t> all;
+    if (d_pos && !h_pos) {
+        all.resize(count);
+        HIPCALL(hipMemcpy(all.data(), d_pos, count * 8, hipMemcpyDeviceToHost));
+        h_pos = all.data();
+    }
+    std::vector<uint64_t> upos(und.size());
+    uint64_t umax = 0;
+    for (size_t i = 0; i < und.size(); ++i) {
+        upos[i] = h_pos ? h_pos[und[i]] : start + und[i];
+        umax = std::max(umax, upos[i] - 1);
+    }
+    const size_t nu = und.size();
+    if ((rc = grow((char**)&c.fix_pos, &c.fix_pos_n, nu, 8))) return rc;
+    if ((rc = grow((char**)&c.fix_val, &c.fix_val_n, nu, 1))) return rc;
+    if ((rc = grow((char**)&c.bbp_ws, &c.bbp_ws_bytes, bbp_few_ws_bytes(nu), 1))) return rc;
+    HIPCALL(hipMemcpyAsync(c.fix_pos, upos.data(), nu * 8, hipMemcpyHostToDevice, st));
+    e = launch_bbp_chunked(c.fix_pos, 0, nu, umax, c.fix_val, c.bbp_ws, st);
+    if (e != hipSuccess) return fail_hip(e, "bbp launch");
+    std::vector<uint8_t> fixed(nu);
+    HIPCALL(hipMemcpyAsync(fixed.data(), c.fix_val, nu, hipMemcpyDeviceToHost, st));
+    HIPCALL(hipStreamSynchronize(st));
+    for (size_t i = 0; i < nu; ++i) h_vals[und[i]] = fixed[i];
+    HIPCALL(hipMemcpyAsync(d_out, h_vals, count, hipMemcpyHostToDevice, st));
+    HIPCALL(hipStreamSynchronize(st));
+    c.bbp_exact_count += nu;
+    return PDH_OK;
+}
+
 int bbp_host(const uint64_t* positions, uint64_t start, size_t count, uint8_t* values) {
     if (count == 0) return PDH_OK;
     if (!values) return fail(PDH_E_ARG, "null output pointer");
@@ -237,19 +296,18 @@ int bbp_host(const uint64_t* positions, uint64_t start, size_t count, uint8_t* v
         c.st_bbp = count;
     }
     if (positions) HIPCALL(hipMemcpyAsync(c.st_pos, sorted.data(), count * 8, hipMemcpyHostToDevice, c.stream));
-    hipError_t e;
+    std::vector<uint8_t> tmp(count);
     if (count < BBP_CHUNKED_MAX) {
         const uint64_t dmax = positions ? sorted.back() - 1 : start + count - 2;
-        int rc2 = grow((char**)&c.bbp_ws, &c.bbp_ws_bytes, bbp_chunked_ws_bytes(count), 1);
-        if (rc2) return rc2;
-        e = launch_bbp_chunked(positions ? c.st_pos : nullptr, start, count, dmax, c.st_val, c.bbp_ws, c.stream);
+        rc = bbp_few_locked(c, positions ? c.st_pos : nullptr, positions ? sorted.data() : nullptr, start, count,
+                            dmax, c.st_val, tmp.data(), c.stream);
+        if (rc) return rc;
     } else {
-        e = launch_bbp(positions ? c.st_pos : nullptr, start, count, c.st_val, c.stream);
+        hipError_t e = launch_bbp(positions ? c.st_pos : nullptr, start, count, c.st_val, c.stream);
+        if (e != hipSuccess) return fail_hip(e, "bbp launch");
+        HIPCALL(hipMemcpyAsync(tmp.data(), c.st_val, count, hipMemcpyDeviceToHost, c.stream));
+        HIPCALL(hipStreamSynchronize(c.stream));
     }
-    if (e != hipSuccess) return fail_hip(e, "bbp launch");
-    std::vector<uint8_t> tmp(count);
-    HIPCALL(hipMemcpyAsync(tmp.data(), c.st_val, count, hipMemcpyDeviceToHost, c.stream));
-    HIPCALL(hipStreamSynchronize(c.stream));
     if (positions) for (size_t i = 0; i < count; ++i) values[order[i]] = tmp[i];
     else std::memcpy(values, tmp.data(), count);
     return PDH_OK;
@@ -363,7 +421,7 @@ int pdh_bbp_positions_dev(const uint64_t* d_positions, size_t n, uint8_t* d_valu
     hipStream_t st = (hipStream_t)stream;
     hipError_t e;
     if (n < BBP_CHUNKED_MAX) {
-        // the chunk loop needs the largest position on the host
+        // both paths size their grids by the largest position
         if (!c.bbp_dmax) HIPCALL(hipMalloc(&c.bbp_dmax, 8));
         e = launch_zero_len(c.bbp_dmax, st);
         if (e == hipSuccess) e = launch_bbp_dmax(d_positions, n, c.bbp_dmax, st);
@@ -371,14 +429,26 @@ int pdh_bbp_positions_dev(const uint64_t* d_positions, size_t n, uint8_t* d_valu
         if (e == hipSuccess) e = hipMemcpyAsync(&dmax, c.bbp_dmax, 8, hipMemcpyDeviceToHost, st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);
         if (e != hipSuccess) return fail_hip(e, "bbp position scan");
-        rc = grow((char**)&c.bbp_ws, &c.bbp_ws_bytes, bbp_chunked_ws_bytes(n), 1);
-        if (rc) return rc;
-        e = launch_bbp_chunked(d_positions, 0, n, dmax, d_values, c.bbp_ws, st);
-    } else {
-        e = launch_bbp(d_positions, 0, n, d_values, st);
+        std::vector<uint8_t> tmp(n);
+        return bbp_few_locked(c, d_positions, nullptr, 0, n, dmax, d_values, tmp.data(), st);
     }
+    e = launch_bbp(d_positions, 0, n, d_values, st);
     if (e != hipSuccess) return fail_hip(e, "bbp launch");
     return PDH_OK;
+}
+
+int pdh_bbp_set_exact(int mode) {
+    if (mode < 0 || mode > 2) return fail(PDH_E_ARG, "pdh_bbp_set_exact: mode must be 0, 1 or 2");
+    Ctx& c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    c.bbp_exact = mode;
+    return PDH_OK;
+}
+
+uint64_t pdh_bbp_exact_count(void) {
+    Ctx& c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    return c.bbp_exact_count;
 }
 
 uint64_t pdh_launch_count(void) { return g_launches.load(); }

@@ -71,17 +71,23 @@ hipError_t launch_zero_len(uint64_t* out_len, hipStream_t s);
 size_t carry_ws_bytes(size_t nlimbs);
 
 // BBP (bbp_kernels.hip). launch_bbp: one lane per series, for many
-// positions. launch_bbp_chunked: terms of every series in parallel and the
-// ordered fold per series, for few positions (workspace bbp_chunked_ws_bytes,
-// dmax = largest position - 1). launch_bbp_dmax: atomic max of position - 1
-// into *d_dmax (zeroed by the caller).
+// positions. For few positions (workspace bbp_few_ws_bytes, dmax = largest
+// position - 1): launch_bbp_fx computes every term in parallel as an exact
+// 64-bit fixed-point fraction and writes the digit, or 0xFE where the
+// reference's FP64 rounding could decide it differently; launch_bbp_chunked
+// computes terms in parallel and folds them per series in the reference's
+// FP64 order (odd_undecided, for tests, also marks every odd index
+// undecided). launch_bbp_dmax: atomic max of position - 1 into *d_dmax
+// (zeroed by the caller).
 hipError_t launch_bbp(const uint64_t* d_pos, uint64_t start, size_t count, uint8_t* d_out,
                       hipStream_t s);
+hipError_t launch_bbp_fx(const uint64_t* d_pos, uint64_t start, size_t count, uint64_t dmax,
+                         uint8_t* d_out, void* ws, int odd_undecided, hipStream_t s);
 hipError_t launch_bbp_chunked(const uint64_t* d_pos, uint64_t start, size_t count, uint64_t dmax,
                               uint8_t* d_out, void* ws, hipStream_t s);
 hipError_t launch_bbp_dmax(const uint64_t* d_pos, size_t count, uint64_t* d_dmax, hipStream_t s);
-size_t bbp_chunked_ws_bytes(size_t count);
-constexpr size_t BBP_CHUNKED_MAX = 8192;      // positions below which the chunked path runs
+size_t bbp_few_ws_bytes(size_t count);
+constexpr size_t BBP_CHUNKED_MAX = 8192;      // positions below which the few-positions paths run
 
 extern std::atomic<uint64_t> g_launches;
 

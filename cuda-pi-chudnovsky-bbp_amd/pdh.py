@@ -38,6 +38,7 @@ EXPORTED = (
     "pdh_batch_create", "pdh_batch_run", "pdh_batch_capture", "pdh_batch_destroy", "pdh_mul_batch_dev",
     "pdh_tune_get", "pdh_tune_set", "pdh_autotune", "pdh_dev_alloc", "pdh_dev_free", "pdh_copy_htod",
     "pdh_copy_dtoh", "pdh_stream_create", "pdh_stream_destroy", "pdh_stream_sync",
+    "pdh_bbp_set_exact", "pdh_bbp_exact_count",
 )
 
 
@@ -88,6 +89,9 @@ def lib() -> ctypes.CDLL:
         L.pdh_bbp_positions.argtypes = [_u64p, _sz, _u8p]
         L.pdh_bbp_positions_dev.argtypes = [_vp, _sz, _vp, _vp]
         L.pdh_launch_count.restype = ctypes.c_uint64
+        L.pdh_bbp_set_exact.argtypes = [ctypes.c_int]
+        L.pdh_bbp_set_exact.restype = ctypes.c_int
+        L.pdh_bbp_exact_count.restype = ctypes.c_uint64
         L.pdh_timing_enable.argtypes = [ctypes.c_int]
         L.pdh_timing_enable.restype = ctypes.c_int
         L.pdh_timing_read.argtypes = [ctypes.c_char_p, _sz, ctypes.POINTER(ctypes.c_double), _u64p,
@@ -250,6 +254,18 @@ def bbp_values_positions(positions) -> np.ndarray:
 
 def bbp_positions_dev(d_pos: int, n: int, d_out: int, stream: int = 0) -> None:
     _check(lib().pdh_bbp_positions_dev(d_pos, n, d_out, stream or None))
+
+
+def bbp_set_exact(mode: int) -> None:
+    """Calls below 8192 positions: 0 fixed point with fallback (default),
+    1 FP64-order path only, 2 fixed point with odd indices sent to the
+    fallback too (testing)."""
+    _check(lib().pdh_bbp_set_exact(int(mode)))
+
+
+def bbp_exact_count() -> int:
+    """Positions computed so far by the FP64-order path (forced or fallback)."""
+    return int(lib().pdh_bbp_exact_count())
 
 
 _HEX = np.frombuffer(b"0123456789ABCDEF", dtype=np.uint8)

@@ -47,3 +47,30 @@ def batch_pairs(seed: int, count: int, lo: int = C3_LO, hi: int = C3_HI):
 def sampled_positions(seed: int, count: int, lo: int = C4_LO, hi: int = C4_HI) -> np.ndarray:
     rng = np.random.default_rng(seed)
     return rng.integers(lo, hi + 1, size=count, dtype=np.uint64)
+
+
+def bbp_mulmod_count(positions) -> int:
+    """Exact number of bbp_mulmod calls the reference makes for these 1-based
+    positions: per series j in {1,4,5,6} and k = 0..d (d = pos - 1),
+    bbp_modpow16(d - k, 8k + j) squares once per bit of the exponent and
+    multiplies once per set bit (include/bbp_kernel.h:86-96); the j = 1,
+    k = 0 term has modulus 1 and makes none. Closed form over the exponent
+    range [0, d] per bit position."""
+    d = np.atleast_1d(np.asarray(positions, dtype=np.uint64)) - np.uint64(1)
+    x = d + np.uint64(1)                          # exponents e in [0, x)
+    total = 0
+    for b in range(63):
+        period, half = np.uint64(1 << (b + 1)), np.uint64(1 << b)
+        full, rem = x // period, x % period
+        ones = full * half + np.where(rem > half, rem - half, np.uint64(0))   # popcount prefix
+        lo = np.uint64(1 << b)                    # e in [2^b, 2^(b+1)) has bit length b + 1
+        hi = np.uint64(1 << (b + 1))
+        span = np.clip(x, lo, hi) - lo
+        total += int(ones.sum(dtype=np.uint64)) + (b + 1) * int(span.sum(dtype=np.uint64))
+    bits = np.zeros_like(d)
+    ones_d = np.zeros_like(d)
+    for b in range(64):
+        sh = d >> np.uint64(b)
+        bits += (sh > 0).astype(np.uint64)
+        ones_d += sh & np.uint64(1)
+    return 4 * total - int((bits + ones_d).sum(dtype=np.uint64))

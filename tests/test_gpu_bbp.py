@@ -62,3 +62,56 @@ def test_bad_positions_raise(pdh):
         pdh.bbp_values_range(0, 4)
     with pytest.raises(pdh.PdhError):
         pdh.bbp_values_positions([(1 << 28) + 1])
+
+
+@pytest.fixture
+def exact_mode(pdh):
+    yield pdh.bbp_set_exact
+    pdh.bbp_set_exact(0)
+
+
+def test_sampled_positions_fixed_point_decides_all(pdh, golden):
+    """The 64 sampled positions are all decided by the fixed-point path (no
+    FP64-order fallback), through the host and the device entry points."""
+    import torch
+    g = golden["bbp_sampled"]
+    before = pdh.bbp_exact_count()
+    pos = np.array(g["positions"], dtype=np.uint64)
+    assert pdh.to_hex(pdh.bbp_values_positions(pos)) == g["hex"]
+    dpos = torch.from_numpy(pos.view(np.int64)).cuda()
+    dval = torch.zeros(len(pos), dtype=torch.uint8, device="cuda")
+    pdh.bbp_positions_dev(dpos.data_ptr(), len(pos), dval.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    assert pdh.to_hex(dval.cpu().numpy()) == g["hex"]
+    assert pdh.bbp_exact_count() == before
+
+
+def test_ordered_path_forced(pdh, golden, exact_mode):
+    """Mode 1: the FP64-order path alone gives the reference's digits."""
+    exact_mode(1)
+    before = pdh.bbp_exact_count()
+    assert pdh.bbp_hex_digits_hip(1, 64) == golden["pi_hex_64"]
+    assert pdh.bbp_hex_digits_hip(1_000_000, 10) == golden["bailey_1e6"]
+    assert pdh.bbp_hex_digits_hip(500, 64) == oracle.bbp_hex_range(500, 64, threads=8)
+    assert pdh.bbp_exact_count() == before + 138
+
+
+def test_fallback_plumbing(pdh, golden, exact_mode):
+    """Mode 2: every odd index (sorted order) is recomputed by the fallback
+    and scattered back; host and device entry points give the same digits."""
+    import torch
+    exact_mode(2)
+    before = pdh.bbp_exact_count()
+    assert pdh.bbp_hex_digits_hip(1, 64) == golden["pi_hex_64"]
+    assert pdh.bbp_exact_count() == before + 32
+    rng = np.random.default_rng(17)
+    pos = rng.integers(1, 200_000, size=301).astype(np.uint64)
+    want = oracle.bbp_values_positions(pos, threads=8)
+    assert np.array_equal(pdh.bbp_values_positions(pos), want)
+    dpos = torch.from_numpy(pos.view(np.int64)).cuda()
+    dval = torch.zeros(len(pos), dtype=torch.uint8, device="cuda")
+    pdh.bbp_positions_dev(dpos.data_ptr(), len(pos), dval.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    assert np.array_equal(dval.cpu().numpy(), want)
+    with pytest.raises(pdh.PdhError):
+        pdh.bbp_set_exact(3)
