@@ -13,8 +13,10 @@
 //                multiply and the inverse first pass in one HBM round trip.
 //    k_col_inv   inverse outer twiddle * N^-1, DIT column sub-transforms.
 //    k_small_mul whole multiply in one workgroup for N <= 2^13.
-//    k_carry_*   base-2^16 carry (include/ntt.h:70-91) as a parallel scan of
-//                carry functions f(c) = (c >= thr) ? v1 : v0.
+//                Its workgroups also run the base-2^16 carry (include/ntt.h:
+//                70-91) inside each row segment of T coefficients.
+//    k_carry_*   the carry across segments as a parallel scan of carry
+//                functions f(c) = (c >= thr) ? v1 : v0.
 //
 //  No bit-reversal pass: the forward transform is decimation in frequency
 //  (natural order in, digit-reversed spectrum out) and the inverse is its
@@ -202,6 +204,15 @@ __device__ __forceinline__ u64 freq_of_pos(int i, int L) {
     return c;
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2). The column passes read and write T-word row segments, so neighbouring
+// tiles share HBM lines; renumbering gives XCD x the contiguous range
+// [x G/8, (x+1) G/8) of the grid, so the tiles sharing a line run at the same
+// time on the same L2 instead of each XCD fetching whole lines for a part.
+__device__ __forceinline__ unsigned xcd_tile_order(unsigned b, unsigned G) {
+    return (G & 7) ? b : (b & 7) * (G >> 3) + (b >> 3);
+}
+
 __device__ __forceinline__ u64 digit16(const u32* __restrict__ limbs, size_t nlimbs, size_t g) {
     size_t li = g >> 1;
     return li < nlimbs ? (u64)((limbs[li] >> (16 * (g & 1))) & 0xFFFFu) : 0;
@@ -220,11 +231,12 @@ __global__ __launch_bounds__(1024) void k_col_fwd(const MulJob* __restrict__ job
     const size_t S = (size_t)1 << L2;
     const int R = 1 << L1;
     const unsigned tiles = (unsigned)(S / T);
-    const unsigned job = blockIdx.x / (ops * tiles);
-    const unsigned rem = blockIdx.x % (ops * tiles);
+    const unsigned bid = xcd_tile_order(blockIdx.x, gridDim.x);
+    const unsigned job = bid / (ops * tiles);
+    const unsigned rem = bid % (ops * tiles);
     const int op = (int)(rem / tiles);
     const size_t b0 = (size_t)(rem % tiles) * T;
-    const MulJob& J = jobs ? jobs[job] : one;
+    const MulJob J = jobs ? jobs[job] : one;
     const u32* __restrict__ src = op ? J.b : J.a;
     const size_t nl = op ? J.nb : J.na;
     u64* __restrict__ dst = op ? J.wsB : J.wsA;
@@ -242,29 +254,6 @@ __global__ __launch_bounds__(1024) void k_col_fwd(const MulJob* __restrict__ job
 }
 
 // ---- column pass, inverse ----------------------------------------------------------
-template <int T, int LR>
-__global__ __launch_bounds__(1024) void k_col_inv(const MulJob* __restrict__ jobs, MulJob one, int n, int L1,
-                                                 const u64* __restrict__ twL) {
-    extern __shared__ u64 lds[];
-    const int L2 = n - L1;
-    const size_t S = (size_t)1 << L2;
-    const int R = 1 << L1;
-    const unsigned tiles = (unsigned)(S / T);
-    const MulJob& J = jobs ? jobs[blockIdx.x / tiles] : one;
-    u64* __restrict__ buf = J.wsA;
-    const size_t b0 = (size_t)(blockIdx.x % tiles) * T;
-    for (int e = threadIdx.x; e < R * T; e += blockDim.x) {
-        const int col = e & (T - 1), i = e / T;
-        lds[lpad(e)] = buf[(size_t)i * S + b0 + col];
-    }
-    __syncthreads();
-    u64* arr[1] = {lds};
-    lds_dft<LR, true, 1>(arr, L1, QLogT<T>::v, twL + R);
-    for (int e = threadIdx.x; e < R * T; e += blockDim.x) {
-        const int col = e & (T - 1), i = e / T;
-        buf[(size_t)i * S + b0 + col] = lds[lpad(e)];
-    }
-}
 
 // ---- row pass: outer twiddle, forward A (and B), pointwise, inverse, inverse
 // outer twiddle times N^-1; result into A ----------------------------------------------
@@ -295,7 +284,7 @@ __global__ __launch_bounds__(1024) void k_row_mul(const MulJob* __restrict__ job
     const int R = 1 << L2;
     u64* la = lds;
     u64* lb = lds + lds_words(R);
-    const MulJob& J = jobs ? jobs[blockIdx.x >> L1] : one;
+    const MulJob J = jobs ? jobs[blockIdx.x >> L1] : one;
     u64* __restrict__ A = J.wsA;
     const u64* __restrict__ B = J.wsB;
     const int row = (int)(blockIdx.x & ((1u << L1) - 1));
@@ -352,39 +341,16 @@ __device__ __forceinline__ CarryFn cf_compose(const CarryFn& later, const CarryF
 __device__ __forceinline__ CarryFn cf_pad() { return CarryFn{~(u64)0, 0, 0}; }
 
 constexpr int CARRY_NT = 256;
-constexpr int CARRY_CH = 16;      // coefficients per thread in the multi-workgroup carry
-
-__device__ __forceinline__ void load_chunk(const u64* __restrict__ coef, size_t ncoef, size_t tau,
-                                           u64* c) {
-    const size_t g0 = tau * CARRY_CH;
-#pragma unroll
-    for (int q = 0; q < CARRY_CH; ++q) c[q] = (g0 + q < ncoef) ? coef[g0 + q] : 0;
-}
-
-__device__ __forceinline__ u64 run_carry(const u64* c, u64 carry, u32* limbs) {
-#pragma unroll
-    for (int q = 0; q < CARRY_CH / 2; ++q) {
-        u64 cur = carry + c[2 * q];
-        u32 lo16 = (u32)(cur & 0xFFFFu);
-        carry = cur >> 16;
-        u64 cur2 = carry + c[2 * q + 1];
-        u32 hi16 = (u32)(cur2 & 0xFFFFu);
-        carry = cur2 >> 16;
-        limbs[q] = lo16 | (hi16 << 16);
-    }
-    return carry;
-}
-
-__device__ __forceinline__ CarryFn chunk_fn(const u64* c) {
-    u32 limbs[CARRY_CH / 2];
-    u64 C = run_carry(c, 0, limbs);
-    bool ones = true;
-#pragma unroll
-    for (int q = 2; q < CARRY_CH / 2; ++q) ones &= (limbs[q] == 0xFFFFFFFFu);
-    u64 lo = (u64)limbs[0] | ((u64)limbs[1] << 32);
-    u64 thr = (ones && lo != 0) ? (u64)(0 - lo) : ~(u64)0;
-    return CarryFn{thr, C, C + 1};
-}
+// Multi-workgroup carry: each thread takes CPT = max(1, 16 / KL) consecutive
+// chunks (KL limbs each), a workgroup CARRY_NT * CPT chunks.
+template <int T> struct CarryGeom {
+    static constexpr int KL = T / 2;
+    static constexpr int CPT = KL >= 16 ? 1 : 16 / KL;
+    static constexpr int LIMBS = CARRY_NT * CPT * KL;          // limbs per workgroup
+};
+// LDS index of a workgroup's limb w: one pad word per 16 keeps the per-thread
+// (16-limb stride) accesses on distinct banks.
+__device__ __forceinline__ int cpad(int w) { return w + (w >> 4); }
 
 __device__ __forceinline__ CarryFn shfl_up_fn(const CarryFn& f, int d) {
     return CarryFn{__shfl_up(f.thr, d), __shfl_up(f.v0, d), __shfl_up(f.v1, d)};
@@ -416,35 +382,106 @@ __device__ __forceinline__ u64 wave_max(u64 v) {
     return v;
 }
 
-// Per-job carry scratch: agg[nblk_max] then cin[nblk_max].
+// ---- multi-workgroup carry ---------------------------------------------------------
+// The inverse column pass holds, per workgroup, T adjacent coefficients of
+// every row: a chunk of T coefficients = KL = T/2 limbs. It runs each chunk's
+// carry with carry-in 0 and writes the chunk's digits D (KL limbs, straight
+// into the product buffer) and its carry-out C into the carry scratch; the
+// coefficients themselves never return to HBM. D and C determine the chunk's
+// carry function (thr from D's limbs, v0 = C, v1 = C + 1), so two scans over
+// (D, C) finish the carry: k_carry_reduce composes each workgroup's chunks,
+// k_carry_blocks scans the workgroup aggregates, k_carry_apply adds each
+// chunk's carry-in to D in place. Limbs at or above na + nb are zero in D
+// (the product has na + nb limbs), so they are neither stored nor read.
+//
+// Scratch per job: C[ceil(nl/2)] (enough for T >= 4), then agg[nblk_max]
+// CarryFn, then cin[nblk_max].
 struct CarryJob {
-    const u64* coef; size_t nchunks, nlimbs, nblk;
-    CarryFn* agg; u64* cin;
+    u32* out; size_t nlimbs, nchunks, nblk;
+    u64* C; CarryFn* agg; u64* cin;
 };
+template <int T>
 __device__ __forceinline__ CarryJob carry_job(const MulJob& J, size_t nblk_max) {
+    using G = CarryGeom<T>;
     CarryJob c;
-    c.coef = J.wsA;
+    c.out = J.out;
     c.nlimbs = J.na + J.nb;
-    c.nchunks = (c.nlimbs + CARRY_CH / 2 - 1) / (CARRY_CH / 2);
-    c.nblk = (c.nchunks + CARRY_NT - 1) / CARRY_NT;
-    c.agg = (CarryFn*)J.carry;
+    c.nchunks = (2 * c.nlimbs + T - 1) / T;
+    c.nblk = (c.nchunks + CARRY_NT * G::CPT - 1) / (CARRY_NT * G::CPT);
+    c.C = (u64*)J.carry;
+    c.agg = (CarryFn*)(c.C + ((c.nlimbs + 1) / 2 + 3) / 4 * 4);
     c.cin = (u64*)(c.agg + nblk_max);
     return c;
 }
 
+// Coalesced copy of the workgroup's limb range [L0, L0 + LIMBS) into LDS
+// (zeros at and above nlimbs), and back.
+template <int T>
+__device__ __forceinline__ void stage_in(const CarryJob& c, size_t L0, u32* sD) {
+    constexpr int LB = CarryGeom<T>::LIMBS;
+    const u32* src = c.out + L0;
+    if (L0 + LB <= c.nlimbs && ((uintptr_t)src & 15) == 0) {
+        for (int v = threadIdx.x; v < LB / 4; v += CARRY_NT) {
+            const uint4 q = ((const uint4*)src)[v];
+            sD[cpad(4 * v)] = q.x; sD[cpad(4 * v + 1)] = q.y; sD[cpad(4 * v + 2)] = q.z; sD[cpad(4 * v + 3)] = q.w;
+        }
+    } else {
+        for (int w = threadIdx.x; w < LB; w += CARRY_NT) sD[cpad(w)] = (L0 + w < c.nlimbs) ? src[w] : 0u;
+    }
+}
+template <int T>
+__device__ __forceinline__ void stage_out(const CarryJob& c, size_t L0, const u32* sD) {
+    constexpr int LB = CarryGeom<T>::LIMBS;
+    u32* dst = c.out + L0;
+    if (L0 + LB <= c.nlimbs && ((uintptr_t)dst & 15) == 0) {
+        for (int v = threadIdx.x; v < LB / 4; v += CARRY_NT)
+            ((uint4*)dst)[v] = uint4{sD[cpad(4 * v)], sD[cpad(4 * v + 1)], sD[cpad(4 * v + 2)], sD[cpad(4 * v + 3)]};
+    } else {
+        for (int w = threadIdx.x; w < LB; w += CARRY_NT)
+            if (L0 + w < c.nlimbs) dst[w] = sD[cpad(w)];
+    }
+}
+
+// f(c) = C + [D + c >= 2^(32 KL)]: the threshold 2^(32 KL) - D fits in 64
+// bits only when limbs 2..KL-1 of D are all ones (KL >= 2).
+template <int KL>
+__device__ __forceinline__ CarryFn chunk_fn(const u32* D, u64 C) {
+    bool ones = true;
+#pragma unroll
+    for (int q = 2; q < KL; ++q) ones &= D[q] == 0xFFFFFFFFu;
+    const u64 lo = (u64)D[0] | ((u64)D[1] << 32);
+    return CarryFn{(ones && lo != 0) ? (u64)(0 - lo) : ~(u64)0, C, C + 1};
+}
+
+template <int T>
 __global__ __launch_bounds__(CARRY_NT) void k_carry_reduce(const MulJob* __restrict__ jobs, MulJob one,
-                                                           size_t ncoef, size_t nblk_max) {
+                                                           size_t nblk_max) {
+    using G = CarryGeom<T>;
+    constexpr int KL = G::KL, CPT = G::CPT;
     __shared__ CarryFn wsum[CARRY_NT / 64];
-    const MulJob& J = jobs ? jobs[blockIdx.x / nblk_max] : one;
-    const CarryJob C = carry_job(J, nblk_max);
+    __shared__ u32 sD[G::LIMBS + G::LIMBS / 16];
+    const MulJob J = jobs ? jobs[blockIdx.x / nblk_max] : one;
+    const CarryJob C = carry_job<T>(J, nblk_max);
     const size_t blk = blockIdx.x % nblk_max;
     if (blk >= C.nblk) return;                       // whole workgroup leaves together
-    const size_t tau = blk * CARRY_NT + threadIdx.x;
+    stage_in<T>(C, blk * G::LIMBS, sD);
+    __syncthreads();
+    const size_t ch0 = (blk * CARRY_NT + threadIdx.x) * CPT;
     CarryFn f = cf_pad();
-    if (tau < C.nchunks) {
-        u64 c[CARRY_CH];
-        load_chunk(C.coef, ncoef, tau, c);
-        f = chunk_fn(c);
+    bool first = true;
+#pragma unroll
+    for (int u = 0; u < CPT; ++u) {
+        const size_t ch = ch0 + u;
+        if (ch < C.nchunks) {
+            u32 D[KL];
+#pragma unroll
+            for (int q = 0; q < KL; ++q) D[q] = sD[cpad((threadIdx.x * CPT + u) * KL + q)];
+            const CarryFn g = chunk_fn<KL>(D, C.C[ch]);
+            f = first ? g : cf_compose(g, f);
+            first = false;
+        } else if (!first) {
+            f = cf_compose(cf_pad(), f);
+        }
     }
     f = block_scan(f, wsum);
     if (threadIdx.x == CARRY_NT - 1) C.agg[blk] = f;
@@ -452,12 +489,13 @@ __global__ __launch_bounds__(CARRY_NT) void k_carry_reduce(const MulJob* __restr
 }
 
 // One workgroup of 1024 threads per job: carry-in of every carry block, in tiles.
+template <int T>
 __global__ __launch_bounds__(1024) void k_carry_blocks(const MulJob* __restrict__ jobs, MulJob one,
                                                        size_t nblk_max) {
     __shared__ CarryFn wsum[1024 / 64];
     __shared__ u64 tile_out[1024];
-    const MulJob& J = jobs ? jobs[blockIdx.x] : one;
-    const CarryJob C = carry_job(J, nblk_max);
+    const MulJob J = jobs ? jobs[blockIdx.x] : one;
+    const CarryJob C = carry_job<T>(J, nblk_max);
     u64 carry = 0;
     for (size_t t0 = 0; t0 < C.nblk; t0 += 1024) {
         const size_t w = t0 + threadIdx.x;
@@ -471,43 +509,77 @@ __global__ __launch_bounds__(1024) void k_carry_blocks(const MulJob* __restrict_
     }
 }
 
+template <int T>
 __global__ __launch_bounds__(CARRY_NT) void k_carry_apply(const MulJob* __restrict__ jobs, MulJob one,
-                                                          size_t ncoef, size_t nblk_max) {
+                                                          size_t nblk_max) {
+    using G = CarryGeom<T>;
+    constexpr int KL = G::KL, CPT = G::CPT;
     __shared__ CarryFn wsum[CARRY_NT / 64];
     __shared__ u64 outs[CARRY_NT];
     __shared__ u64 wmax[CARRY_NT / 64];
-    const MulJob& J = jobs ? jobs[blockIdx.x / nblk_max] : one;
-    const CarryJob C = carry_job(J, nblk_max);
+    __shared__ u32 sD[G::LIMBS + G::LIMBS / 16];
+    const MulJob J = jobs ? jobs[blockIdx.x / nblk_max] : one;
+    const CarryJob C = carry_job<T>(J, nblk_max);
     // Highest limbs first: the block holding the top nonzero limb usually
     // runs early, and later blocks then skip the atomic on out_len.
     const size_t rb = blockIdx.x % nblk_max;
     if (rb >= C.nblk) return;
     const size_t blk = C.nblk - 1 - rb;
-    const size_t tau = blk * CARRY_NT + threadIdx.x;
-    u64 c[CARRY_CH];
+    const size_t L0 = blk * G::LIMBS;
+    stage_in<T>(C, L0, sD);
+    __syncthreads();
+    const size_t ch0 = (blk * CARRY_NT + threadIdx.x) * CPT;
+    u64 Cc[CPT];
     CarryFn f = cf_pad();
-    if (tau < C.nchunks) {
-        load_chunk(C.coef, ncoef, tau, c);
-        f = chunk_fn(c);
+    bool first = true;
+#pragma unroll
+    for (int u = 0; u < CPT; ++u) {
+        const size_t ch = ch0 + u;
+        Cc[u] = 0;
+        if (ch < C.nchunks) {
+            u32 D[KL];
+#pragma unroll
+            for (int q = 0; q < KL; ++q) D[q] = sD[cpad((threadIdx.x * CPT + u) * KL + q)];
+            Cc[u] = C.C[ch];
+            const CarryFn g = chunk_fn<KL>(D, Cc[u]);
+            f = first ? g : cf_compose(g, f);
+            first = false;
+        } else if (!first) {
+            f = cf_compose(cf_pad(), f);
+        }
     }
     const u64 bc = C.cin[blk];
     CarryFn inc = block_scan(f, wsum);
     outs[threadIdx.x] = cf_apply(inc, bc);
     __syncthreads();
-    const u64 my_in = threadIdx.x == 0 ? bc : outs[threadIdx.x - 1];
+    u64 cin = threadIdx.x == 0 ? bc : outs[threadIdx.x - 1];
     u64 top = 0;
-    if (tau < C.nchunks) {
-        u32 limbs[CARRY_CH / 2];
-        run_carry(c, my_in, limbs);
-        const size_t l0 = tau * (CARRY_CH / 2);
 #pragma unroll
-        for (int q = 0; q < CARRY_CH / 2; ++q) {
-            if (l0 + q < C.nlimbs) {
-                J.out[l0 + q] = limbs[q];
-                if (limbs[q]) top = l0 + q + 1;
-            }
+    for (int u = 0; u < CPT; ++u) {
+        const size_t ch = ch0 + u;
+        if (ch >= C.nchunks) break;
+        const int w0 = (threadIdx.x * CPT + u) * KL;
+        u32 D[KL];
+#pragma unroll
+        for (int q = 0; q < KL; ++q) D[q] = sD[cpad(w0 + q)];
+        const u64 next = cf_apply(chunk_fn<KL>(D, Cc[u]), cin);
+        // D + cin over KL limbs (the overflow is next's +1)
+        const u64 lo = (u64)D[0] | ((u64)D[1] << 32);
+        const u64 s = lo + cin;
+        u32 carry = s < lo ? 1u : 0u;
+        D[0] = (u32)s; D[1] = (u32)(s >> 32);
+#pragma unroll
+        for (int q = 2; q < KL; ++q) { const u32 t = D[q] + carry; carry = (t < carry) ? 1u : 0u; D[q] = t; }
+        const size_t l0 = ch * KL;
+#pragma unroll
+        for (int q = 0; q < KL; ++q) {
+            sD[cpad(w0 + q)] = D[q];
+            if (D[q] && l0 + q < C.nlimbs) top = l0 + q + 1;
         }
+        cin = next;
     }
+    __syncthreads();
+    stage_out<T>(C, L0, sD);
     top = wave_max(top);
     if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = top;
     __syncthreads();
@@ -516,6 +588,68 @@ __global__ __launch_bounds__(CARRY_NT) void k_carry_apply(const MulJob* __restri
         for (int w = 0; w < CARRY_NT / 64; ++w) m = wmax[w] > m ? wmax[w] : m;
         if (m && m > __atomic_load_n((unsigned long long*)J.out_len, __ATOMIC_RELAXED))
             atomicMax((unsigned long long*)J.out_len, (unsigned long long)m);
+    }
+}
+
+// Inverse column pass with the first half of the carry: after the DIT column
+// sub-transforms every row segment of T coefficients is a carry chunk; its
+// digits D go to the product buffer and its carry-out to the carry scratch.
+template <int T, int LR>
+__global__ __launch_bounds__(1024) void k_col_inv(const MulJob* __restrict__ jobs, MulJob one, int n, int L1,
+                                                 const u64* __restrict__ twL, size_t nblk_max) {
+    constexpr int KL = T / 2;
+    extern __shared__ u64 lds[];
+    const int L2 = n - L1;
+    const size_t S = (size_t)1 << L2;
+    const int R = 1 << L1;
+    const unsigned tiles = (unsigned)(S / T);
+    const unsigned bid = xcd_tile_order(blockIdx.x, gridDim.x);
+    const MulJob J = jobs ? jobs[bid / tiles] : one;
+    const u64* __restrict__ buf = J.wsA;
+    const size_t b0 = (size_t)(bid % tiles) * T;
+    for (int e = threadIdx.x; e < R * T; e += blockDim.x) {
+        const int col = e & (T - 1), i = e / T;
+        lds[lpad(e)] = buf[(size_t)i * S + b0 + col];
+    }
+    __syncthreads();
+    u64* arr[1] = {lds};
+    lds_dft<LR, true, 1>(arr, L1, QLogT<T>::v, twL + R);
+    const CarryJob C = carry_job<T>(J, nblk_max);
+    for (int i = threadIdx.x; i < R; i += blockDim.x) {
+        const size_t ch = ((size_t)i * S + b0) / T;
+        if (ch >= C.nchunks) continue;
+        const size_t l0 = ch * KL;
+        u64 carry = 0;
+        if constexpr (KL % 4 == 0 && KL <= 8) {
+            u32 D[KL];
+#pragma unroll
+            for (int q = 0; q < KL; ++q) {
+                const u64 c0 = carry + lds[lpad(i * T + 2 * q)];
+                carry = c0 >> 16;
+                const u64 c1 = carry + lds[lpad(i * T + 2 * q + 1)];
+                carry = c1 >> 16;
+                D[q] = (u32)(c0 & 0xFFFFu) | ((u32)(c1 & 0xFFFFu) << 16);
+            }
+            u32* dst = C.out + l0;
+            if (l0 + KL <= C.nlimbs && ((uintptr_t)dst & 15) == 0) {
+#pragma unroll
+                for (int v = 0; v < KL / 4; ++v) ((uint4*)dst)[v] = uint4{D[4 * v], D[4 * v + 1], D[4 * v + 2], D[4 * v + 3]};
+            } else {
+#pragma unroll
+                for (int q = 0; q < KL; ++q)
+                    if (l0 + q < C.nlimbs) dst[q] = D[q];
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < KL; ++q) {
+                const u64 c0 = carry + lds[lpad(i * T + 2 * q)];
+                carry = c0 >> 16;
+                const u64 c1 = carry + lds[lpad(i * T + 2 * q + 1)];
+                carry = c1 >> 16;
+                if (l0 + q < C.nlimbs) C.out[l0 + q] = (u32)(c0 & 0xFFFFu) | ((u32)(c1 & 0xFFFFu) << 16);
+            }
+        }
+        C.C[ch] = carry;
     }
 }
 
@@ -578,7 +712,7 @@ __global__ __launch_bounds__(NT) void k_small_mul(const MulJob* __restrict__ job
     extern __shared__ u64 lds[];
     __shared__ CarryFn wsum[NT / 64];
     __shared__ u64 outs[NT];
-    const MulJob& J = jobs ? jobs[blockIdx.x] : one;
+    const MulJob J = jobs ? jobs[blockIdx.x] : one;
     const int R = 1 << n;
     u64* la = lds;
     u64* lb = lds + lds_words(R);
@@ -686,7 +820,7 @@ hipError_t init_kernel_attrs() {
 template <int T, int LR>
 static hipError_t launch_cols(const MulJob* d_jobs, const MulJob& one, unsigned njobs, int ops,
                               const u64* twL, const OuterTables& ot, const NttShape& sh, hipStream_t s,
-                              bool inverse) {
+                              bool inverse, size_t nblk_max) {
     const size_t lds = (size_t)lds_words((1 << sh.L1) * T) * 8;
     const unsigned tiles = (unsigned)(((size_t)1 << sh.L2) / T);
     if (!inverse) {
@@ -694,15 +828,19 @@ static hipError_t launch_cols(const MulJob* d_jobs, const MulJob& one, unsigned 
                    ops, sh.n, sh.L1, twL);
     } else {
         PDH_LAUNCH("k_col_inv", s, (k_col_inv<T, LR>), dim3(njobs * tiles), dim3(1024), lds, s, d_jobs, one,
-                   sh.n, sh.L1, twL);
+                   sh.n, sh.L1, twL, nblk_max);
+        const dim3 cgrid((unsigned)(njobs * nblk_max));
+        PDH_LAUNCH("k_carry_reduce", s, k_carry_reduce<T>, cgrid, dim3(CARRY_NT), 0, s, d_jobs, one, nblk_max);
+        PDH_LAUNCH("k_carry_blocks", s, k_carry_blocks<T>, dim3(njobs), dim3(1024), 0, s, d_jobs, one, nblk_max);
+        PDH_LAUNCH("k_carry_apply", s, k_carry_apply<T>, cgrid, dim3(CARRY_NT), 0, s, d_jobs, one, nblk_max);
     }
     return hipSuccess;
 }
 
 static hipError_t launch_cols_T(const MulJob* d_jobs, const MulJob& one, unsigned njobs, int ops,
                                 const u64* twL, const OuterTables& ot, const NttShape& sh, hipStream_t s,
-                                bool inverse) {
-#define PDH_CASE_COL(TT, RR) if (sh.T == TT && sh.lr_col == RR) return launch_cols<TT, RR>(d_jobs, one, njobs, ops, twL, ot, sh, s, inverse);
+                                bool inverse, size_t nblk_max) {
+#define PDH_CASE_COL(TT, RR) if (sh.T == TT && sh.lr_col == RR) return launch_cols<TT, RR>(d_jobs, one, njobs, ops, twL, ot, sh, s, inverse, nblk_max);
     PDH_FOR_COL(PDH_CASE_COL)
 #undef PDH_CASE_COL
     return hipErrorInvalidValue;
@@ -713,14 +851,14 @@ hipError_t launch_zero_len(uint64_t* out_len, hipStream_t s) {
     return hipSuccess;
 }
 
+// Carry blocks: every shape's workgroup covers at least CARRY_NT * 16 limbs.
 size_t carry_nblk(size_t nlimbs) {
-    const size_t nchunks = (nlimbs + CARRY_CH / 2 - 1) / (CARRY_CH / 2);
-    return (nchunks + CARRY_NT - 1) / CARRY_NT;
+    return (nlimbs + CARRY_NT * 16 - 1) / (CARRY_NT * 16);
 }
 
 size_t carry_ws_bytes(size_t nlimbs) {
     const size_t nblk = carry_nblk(nlimbs);
-    return nblk * sizeof(CarryFn) + nblk * sizeof(u64) + 256;
+    return ((nlimbs + 1) / 2 + 3) / 4 * 4 * sizeof(u64) + nblk * sizeof(CarryFn) + nblk * sizeof(u64) + 256;
 }
 
 template <bool SQ, int NT>
@@ -747,7 +885,7 @@ hipError_t launch_class(const MulJob* d_jobs, const MulJob& one, unsigned njobs,
         return square ? launch_small<true, 256>(d_jobs, one, njobs, n, sh.lr_row, lds, twL, ninv, s)
                       : launch_small<false, 256>(d_jobs, one, njobs, n, sh.lr_row, lds, twL, ninv, s);
     }
-    PDH_TRY(launch_cols_T(d_jobs, one, njobs, square ? 1 : 2, twL, ot, sh, s, false));
+    PDH_TRY(launch_cols_T(d_jobs, one, njobs, square ? 1 : 2, twL, ot, sh, s, false, nblk_max));
     const size_t lds = (size_t)(square ? 1 : 2) * lds_words(1 << sh.L2) * 8;
     const dim3 grid((unsigned)(njobs << sh.L1));
     if (sh.lr_row == 3) {
@@ -757,12 +895,7 @@ hipError_t launch_class(const MulJob* d_jobs, const MulJob& one, unsigned njobs,
         if (square) PDH_LAUNCH("k_row_sqr", s, (k_row_mul<true, 4>), grid, dim3(1024), lds, s, d_jobs, one, sh.L1, sh.L2, sh.lr_col, twL, ot.hi, ot.lo, ot.hi_inv, ot.lo_inv, ot.h);
         else PDH_LAUNCH("k_row_mul", s, (k_row_mul<false, 4>), grid, dim3(1024), lds, s, d_jobs, one, sh.L1, sh.L2, sh.lr_col, twL, ot.hi, ot.lo, ot.hi_inv, ot.lo_inv, ot.h);
     }
-    PDH_TRY(launch_cols_T(d_jobs, one, njobs, 1, twL, ot, sh, s, true));
-    const dim3 cgrid((unsigned)(njobs * nblk_max));
-    PDH_LAUNCH("k_carry_reduce", s, k_carry_reduce, cgrid, dim3(CARRY_NT), 0, s, d_jobs, one, N, nblk_max);
-    PDH_LAUNCH("k_carry_blocks", s, k_carry_blocks, dim3(njobs), dim3(1024), 0, s, d_jobs, one, nblk_max);
-    PDH_LAUNCH("k_carry_apply", s, k_carry_apply, cgrid, dim3(CARRY_NT), 0, s, d_jobs, one, N, nblk_max);
-    return hipSuccess;
+    return launch_cols_T(d_jobs, one, njobs, 1, twL, ot, sh, s, true, nblk_max);
 }
 
 hipError_t launch_mul(const uint32_t* a, size_t na, const uint32_t* b, size_t nb, bool square,
