@@ -41,16 +41,26 @@ HBM_PEAK = 8.0e12
 METRIC = "bit-exact 2^22-limb NTT mul ms (% HBM roofline); batched vs plain mul/s"
 
 
-def alg_bytes(kernel: str, na: int, nb: int, N: int) -> float:
-    """Algorithmic HBM bytes one launch of `kernel` must move for one multiply."""
+def alg_bytes(kernel: str, na: int, nb: int, N: int, T: int, eng: int) -> float:
+    """Algorithmic HBM bytes one launch of `kernel` must move for one multiply.
+    Engine 0 (Goldilocks, k_*): two N-point u64 spectra, carry chunks of T
+    coefficients = T/2 limbs. Engine 1 (three primes, k3_*): three residue
+    arrays of N/2 u32 points per operand, carry chunks of T limbs. The carry
+    scan kernels are shared."""
     nl = na + nb
+    N3 = N // 2
+    cb = 8 * -(-nl // (T if eng == 1 else T // 2))   # one 8-byte carry-out per chunk
     return {
         "k_col_fwd": 4 * nl + 16 * N,         # both operands' limbs in, two N-point spectra out
         "k_row_mul": 24 * N,                  # two spectra in, one out
         "k_row_sqr": 16 * N,
-        "k_col_inv": 16 * N,
-        "k_carry_reduce": 16 * nl,            # 2 nl coefficients in
-        "k_carry_apply": 16 * nl + 4 * nl,    # coefficients in, limbs out
+        "k_col_inv": 8 * N + 4 * nl + cb,     # spectrum in; chunk digits (limbs) and carry-outs out
+        "k3_col_fwd": 4 * nl + 2 * 3 * 4 * N3,    # limbs in, six residue spectra out
+        "k3_row_mul": 3 * 12 * N3,                # per prime two spectra in, one out
+        "k3_row_sqr": 3 * 8 * N3,
+        "k3_col_inv": 3 * 4 * N3 + 4 * nl + cb,   # three spectra in; digits and carry-outs out
+        "k_carry_reduce": 4 * nl + cb,        # digits and carry-outs in
+        "k_carry_apply": 8 * nl + cb,         # digits and carry-outs in, limbs out
     }.get(kernel, 0.0)
 
 
@@ -248,6 +258,8 @@ def main() -> int:
     g = golden["c2_mul"]
     na = nb = W.C2_LIMBS
     N = pdh.ntt_length(na, nb)
+    T = pdh.tune_get(N.bit_length() - 1)[1]
+    eng = pdh.tune_engine(N.bit_length() - 1)
     a, b = W.operand(g["seed_a"], na), W.operand(g["seed_b"], nb)
     dev = torch.device("cuda:0")
     da = torch.from_numpy(a.view(np.int32)).to(dev)
@@ -301,12 +313,12 @@ def main() -> int:
     kernels = {}
     for name, (tms, cnt) in times.items():
         avg = tms / cnt
-        B = alg_bytes(name, na, nb, N)
+        B = alg_bytes(name, na, nb, N, T, eng)
         kernels[name] = {"avg_us": round(avg * 1e3, 2), "launches_per_mul": cnt / args.steps,
                          "alg_bytes": B, "frac": round(B / (avg * 1e-3) / HBM_PEAK, 4) if B else None}
     dom = max(times, key=lambda k: times[k][0])
     dom_avg_s = times[dom][0] / times[dom][1] * 1e-3
-    dom_B = alg_bytes(dom, na, nb, N)
+    dom_B = alg_bytes(dom, na, nb, N, T, eng)
     roofline = {"bound": "hbm", "kernel": dom, "achieved": dom_B / dom_avg_s / 1e9, "peak": HBM_PEAK / 1e9,
                 "unit": "GB/s", "frac": dom_B / dom_avg_s / HBM_PEAK, "traffic": read_traffic(dom),
                 "alg_bytes_per_launch": dom_B}
@@ -393,9 +405,11 @@ def main() -> int:
     line = {
         "metric": METRIC, "value": 1e3 / ms, "unit": "mul/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "u64 (GF(2^64-2^32+1))", "data": "synthetic (seeded uniform limbs)",
+        "vs_baseline": None, "dtype": "u32 (residues mod three primes < 2^31, CRT)" if eng == 1 else "u64 (GF(2^64-2^32+1))", "data": "synthetic (seeded uniform limbs)",
         "config": {"workload": "2^22 x 2^22-limb multiply, N = 2^24, operands resident in HBM",
-                   "na": na, "nb": nb, "ntt_length": N, "parallelism": "single GPU"},
+                   "na": na, "nb": nb, "ntt_length": N, "parallelism": "single GPU",
+                   "engine": "three primes, 2^23-point transforms" if eng == 1 else "Goldilocks, 2^24-point transforms",
+                   "shape": dict(zip(("L1", "T", "lr_col", "lr_row"), pdh.tune_get(N.bit_length() - 1)))},
         "bit_exact": exact, "wall_s_timed": wall,
         "roofline": roofline, "roofline_e2e": roofline_e2e, "kernels": kernels,
         "cpu_baseline": cpu, "extra": extra,

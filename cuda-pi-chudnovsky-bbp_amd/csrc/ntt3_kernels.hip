@@ -1,0 +1,444 @@
+// ============================================================================
+//  ntt3_kernels.hip -- three-prime NTT multiply for gfx950 (products with
+//  N = 2^n > 2^13 in the reference's 16-bit digit count).
+//
+//  The product is a unique integer, so any exact convolution method gives the
+//  reference's result (src/ntt_cuda.cu:179-243, include/ntt.h:47-91). This
+//  engine convolves 32-bit digits (the limbs themselves) modulo three primes
+//  below 2^31 and recombines each coefficient by CRT:
+//
+//      p0 = 15 * 2^27 + 1,  p1 = 63 * 2^25 + 1,  p2 = 127 * 2^24 + 1
+//
+//  A coefficient of a length-2^m cyclic convolution of 32-bit digits is below
+//  2^m * 2^64 <= 2^87 < p0 p1 p2 (2^92.9) for m <= 23, so it is recovered
+//  exactly. The transforms are half as long as with 16-bit digits, and a
+//  residue butterfly is 6 VALU ops (add, sub each: add, sub p, unsigned min)
+//  and a Montgomery multiply (R = 2^32) 5 ops (two v_mad_u64_u32, one
+//  v_mul_lo_u32, sub, min), against 12 and 27 for a Goldilocks butterfly and
+//  multiply.
+//
+//  Same four-step structure as the Goldilocks engine: column DIF per operand
+//  and prime (k3_col_fwd), a fused row kernel per prime (outer twiddle,
+//  forward A and B, pointwise, inverse, inverse twiddle with the scale), and
+//  k3_col_inv, which runs the inverse columns of all three primes of a tile,
+//  the CRT and each row segment's carry with carry-in 0, feeding the shared
+//  carry scan. Values are canonical residues in [0, p); twiddles are stored
+//  in Montgomery form (w R mod p) so that mm(x, wR) = x w.
+// ============================================================================
+#include "ntt_common.h"
+
+namespace pdh {
+
+// ---- host constants ---------------------------------------------------------------
+static const uint32_t kP3[3] = {2013265921u, 2113929217u, 2130706433u};   // 15*2^27+1, 63*2^25+1, 127*2^24+1
+static const uint32_t kG3[3] = {31u, 5u, 3u};                              // generators of the unit groups
+static const int kK3[3] = {27, 25, 24};                                    // 2-adicity of p - 1
+
+uint32_t r3_mul(uint32_t a, uint32_t b, uint32_t p) { return (uint32_t)((uint64_t)a * b % p); }
+uint32_t r3_pow(uint32_t a, uint64_t e, uint32_t p) {
+    uint32_t r = 1;
+    for (; e; e >>= 1, a = r3_mul(a, a, p)) if (e & 1) r = r3_mul(r, a, p);
+    return r;
+}
+uint32_t r3_to_mont(uint32_t a, uint32_t p) { return (uint32_t)(((uint64_t)a << 32) % p); }
+static uint32_t r3_inv(uint32_t a, uint32_t p) { return r3_pow(a, p - 2, p); }
+
+// Compatible root system: w_{2^L} = rho^(2^(K - L)), rho = g^((p - 1) / 2^K).
+uint32_t r3_root(int pi, int L) {
+    const uint32_t p = kP3[pi];
+    const uint32_t rho = r3_pow(kG3[pi], (p - 1) >> kK3[pi], p);
+    return r3_pow(rho, (uint64_t)1 << (kK3[pi] - L), p);
+}
+
+const Primes3& primes3() {
+    static const Primes3 PS = [] {
+        Primes3 s{};
+        for (int i = 0; i < 3; ++i) {
+            const uint32_t p = kP3[i];
+            uint32_t inv = 1;                                  // p^-1 mod 2^32 by Newton
+            for (int k = 0; k < 5; ++k) inv *= 2u - p * inv;
+            s.p[i].p = p;
+            s.p[i].ninv = 0u - inv;
+            const uint32_t w8 = r3_root(i, 3), w8i = r3_inv(w8, p);
+            for (int k = 0; k < 4; ++k) {
+                s.p[i].w8[k] = r3_to_mont(r3_pow(w8, k, p), p);
+                s.p[i].w8i[k] = r3_to_mont(r3_pow(w8i, k, p), p);
+            }
+        }
+        const uint32_t p0 = kP3[0], p1 = kP3[1], p2 = kP3[2];
+        s.inv_p0_m1 = r3_to_mont(r3_inv(p0 % p1, p1), p1);
+        s.p0_m2 = r3_to_mont(p0 % p2, p2);
+        s.inv_p0p1_m2 = r3_to_mont(r3_inv(r3_mul(p0 % p2, p1 % p2, p2), p2), p2);
+        s.p0p1 = (uint64_t)p0 * p1;
+        return s;
+    }();
+    return PS;
+}
+
+// ---- residue arithmetic -----------------------------------------------------------
+__device__ __forceinline__ u32 umin32(u32 a, u32 b) { return a < b ? a : b; }
+__device__ __forceinline__ u32 r_add(u32 a, u32 b, u32 p) { const u32 s = a + b; return umin32(s, s - p); }
+__device__ __forceinline__ u32 r_sub(u32 a, u32 b, u32 p) { const u32 d = a - b; return umin32(d, d + p); }
+// a * b * 2^-32 mod p for a, b < p < 2^31: t + m p < 2^64 and (t + m p) / 2^32 < 2p.
+__device__ __forceinline__ u32 mm(u32 a, u32 b, u32 p, u32 ninv) {
+    const u64 t = (u64)a * b;
+    const u32 m = (u32)t * ninv;
+    const u32 r = (u32)(((u64)m * p + t) >> 32);
+    return umin32(r, r - p);
+}
+
+// ---- radix-q DFTs on residues, natural order in and out ---------------------------
+template <bool INV>
+__device__ __forceinline__ void r_dft2(u32* x, const Prime3& P) {
+    const u32 a = r_add(x[0], x[1], P.p), b = r_sub(x[0], x[1], P.p);
+    x[0] = a; x[1] = b;
+}
+
+template <bool INV>
+__device__ __forceinline__ void r_dft4(u32& x0, u32& x1, u32& x2, u32& x3, const Prime3& P) {
+    const u32 c0 = r_add(x0, x2, P.p), d0 = r_sub(x0, x2, P.p);
+    const u32 c1 = r_add(x1, x3, P.p);
+    const u32 d1 = mm(r_sub(x1, x3, P.p), INV ? P.w8i[2] : P.w8[2], P.p, P.ninv);   // w4^{-1}, w4
+    x0 = r_add(c0, c1, P.p); x2 = r_sub(c0, c1, P.p);
+    x1 = r_add(d0, d1, P.p); x3 = r_sub(d0, d1, P.p);
+}
+
+// Y_{2k} = DFT4(x_t + x_{t+4})_k, Y_{2k+1} = DFT4((x_t - x_{t+4}) w8^t)_k.
+template <bool INV>
+__device__ __forceinline__ void r_dft8(u32* x, const Prime3& P) {
+    const u32* w = INV ? P.w8i : P.w8;
+    u32 a0 = r_add(x[0], x[4], P.p), b0 = r_sub(x[0], x[4], P.p);
+    u32 a1 = r_add(x[1], x[5], P.p), b1 = mm(r_sub(x[1], x[5], P.p), w[1], P.p, P.ninv);
+    u32 a2 = r_add(x[2], x[6], P.p), b2 = mm(r_sub(x[2], x[6], P.p), w[2], P.p, P.ninv);
+    u32 a3 = r_add(x[3], x[7], P.p), b3 = mm(r_sub(x[3], x[7], P.p), w[3], P.p, P.ninv);
+    r_dft4<INV>(a0, a1, a2, a3, P);
+    r_dft4<INV>(b0, b1, b2, b3, P);
+    x[0] = a0; x[2] = a1; x[4] = a2; x[6] = a3;
+    x[1] = b0; x[3] = b1; x[5] = b2; x[7] = b3;
+}
+
+template <int Q, bool INV>
+__device__ __forceinline__ void r_dftq(u32* x, const Prime3& P) {
+    if constexpr (Q == 2) r_dft2<INV>(x, P);
+    else if constexpr (Q == 4) r_dft4<INV>(x[0], x[1], x[2], x[3], P);
+    else r_dft8<INV>(x, P);
+}
+
+// One radix-Q level of an in-place 2^L-point sub-transform of NARR residue
+// arrays in LDS (same index map as the Goldilocks lds_level): butterfly
+// (beta, j) of sub-block size M = 2^logM = Q m reads i = beta M + j + t m.
+// Forward: DFT_Q then output c times w_M^(j c); inverse: the mirror.
+// tw[2^L + x] = w_{2^L}^x R mod p.
+template <int Q, bool INV, int NARR>
+__device__ __forceinline__ void r_lds_level(u32* const* arr, int L, int logM, int tlog,
+                                            const u32* __restrict__ tw, const Prime3& P) {
+    constexpr int LQ = QLog<Q>::v;
+    const int logm = logM - LQ;
+    const int nbf = 1 << (L - LQ + tlog);
+    const int mask = (1 << L) - 1;
+    const int tmask = (1 << tlog) - 1;
+    const bool has_tw = logm > 0;
+    for (int u = threadIdx.x; u < nbf; u += blockDim.x) {
+        const int col = u & tmask;
+        const int rest = u >> tlog;
+        const int j = rest & ((1 << logm) - 1);
+        const int base = ((rest >> logm) << logM) + j;
+        u32 w[Q];
+        if (has_tw) {
+            const int step = j << (L - logM);
+#pragma unroll
+            for (int c = 1; c < Q; ++c) {
+                const int e = (step * c) & mask;
+                w[c] = tw[INV ? ((-e) & mask) : e];
+            }
+        }
+#pragma unroll
+        for (int a = 0; a < NARR; ++a) {
+            u32 x[Q];
+#pragma unroll
+            for (int t = 0; t < Q; ++t) x[t] = arr[a][lpad(((base + (t << logm)) << tlog) + col)];
+            if (!INV) {
+                r_dftq<Q, false>(x, P);
+                if (has_tw) {
+#pragma unroll
+                    for (int c = 1; c < Q; ++c) x[c] = mm(x[c], w[c], P.p, P.ninv);
+                }
+            } else {
+                if (has_tw) {
+#pragma unroll
+                    for (int c = 1; c < Q; ++c) x[c] = mm(x[c], w[c], P.p, P.ninv);
+                }
+                r_dftq<Q, true>(x, P);
+            }
+#pragma unroll
+            for (int t = 0; t < Q; ++t) arr[a][lpad(((base + (t << logm)) << tlog) + col)] = x[t];
+        }
+    }
+    __syncthreads();
+}
+
+// Radix-8 sub-transform of 2^L points: DIF = a radix-2^(L mod 3) level then
+// radix-8 levels; the inverse is the exact mirror. Ends with a barrier.
+template <bool INV, int NARR>
+__device__ __forceinline__ void r_lds_dft(u32* const* arr, int L, int tlog, const u32* __restrict__ tw,
+                                          const Prime3& P) {
+    const int r0 = L % 3;
+    if (!INV) {
+        int logM = L;
+        if (r0 == 1) { r_lds_level<2, false, NARR>(arr, L, logM, tlog, tw, P); logM -= 1; }
+        if (r0 == 2) { r_lds_level<4, false, NARR>(arr, L, logM, tlog, tw, P); logM -= 2; }
+        for (; logM >= 3; logM -= 3) r_lds_level<8, false, NARR>(arr, L, logM, tlog, tw, P);
+    } else {
+        for (int logM = 3; logM <= L - r0; logM += 3) r_lds_level<8, true, NARR>(arr, L, logM, tlog, tw, P);
+        if (r0 == 1) r_lds_level<2, true, NARR>(arr, L, L, tlog, tw, P);
+        if (r0 == 2) r_lds_level<4, true, NARR>(arr, L, L, tlog, tw, P);
+    }
+}
+
+// A limb below 2^32 < 3p reduced to [0, p).
+// The prime of a (wave-uniform) index, selected field by field so the
+// kernel-argument struct is never indexed dynamically.
+__device__ __forceinline__ Prime3 pick3(const Primes3& PS, int pi) {
+    return pi == 0 ? PS.p[0] : (pi == 1 ? PS.p[1] : PS.p[2]);
+}
+
+__device__ __forceinline__ u32 r_reduce_limb(u32 v, u32 p) {
+    v = umin32(v, v - p);
+    return umin32(v, v - p);
+}
+
+// ---- column pass, forward ---------------------------------------------------------
+// Workgroup -> (job, operand, prime, tile); the residue spectra of an operand
+// sit in its workspace as three consecutive arrays of 2^n3 u32.
+template <int T>
+__global__ __launch_bounds__(1024) void k3_col_fwd(const MulJob* __restrict__ jobs, MulJob one, int ops, int n3,
+                                                  int L1, const u32* __restrict__ tw3, Primes3 PS) {
+    extern __shared__ u32 lds3[];
+    const int L2 = n3 - L1;
+    const size_t S = (size_t)1 << L2;
+    const int R = 1 << L1;
+    const unsigned tiles = (unsigned)(S / T);
+    const unsigned bid = xcd_tile_order(blockIdx.x, gridDim.x);
+    const unsigned per_job = (unsigned)ops * 3u * tiles;
+    const unsigned job = bid / per_job;
+    const unsigned rem = bid % per_job;
+    const int op = (int)(rem / (3u * tiles));
+    const int pi = (int)((rem / tiles) % 3u);
+    const size_t b0 = (size_t)(rem % tiles) * T;
+    const MulJob J = jobs ? jobs[job] : one;
+    const Prime3 P = pick3(PS, pi);
+    const u32* __restrict__ src = op ? J.b : J.a;
+    const size_t nl = op ? J.nb : J.na;
+    u32* __restrict__ dst = (u32*)(op ? J.wsB : J.wsA) + ((size_t)pi << n3);
+    for (int e = threadIdx.x; e < R * T; e += blockDim.x) {
+        const int col = e & (T - 1), i = e / T;
+        const size_t g = (size_t)i * S + b0 + col;
+        lds3[lpad(e)] = g < nl ? r_reduce_limb(src[g], P.p) : 0u;
+    }
+    __syncthreads();
+    u32* arr[1] = {lds3};
+    r_lds_dft<false, 1>(arr, L1, QLogT<T>::v, tw3 + (size_t)pi * TW3_STRIDE + R, P);
+    for (int e = threadIdx.x; e < R * T; e += blockDim.x) {
+        const int col = e & (T - 1), i = e / T;
+        dst[(size_t)i * S + b0 + col] = lds3[lpad(e)];
+    }
+}
+
+// ---- row pass ---------------------------------------------------------------------
+// Workgroup -> (job, prime, row). Row r holds column frequency c = freq(r);
+// element b is multiplied by w^(b c) on the way in and by w^(-b c) s on the
+// way out, s = 2^-n3 R mod p (R undoes the pointwise Montgomery factor).
+// Outer tables per prime: hi | lo | hi_inv | lo_inv, w^e R = mm(hi[e >> h], lo[e & (2^h - 1)]).
+__device__ __forceinline__ u32 r_tw(const u32* __restrict__ hi, const u32* __restrict__ lo, int h, u64 e,
+                                    const Prime3& P) {
+    return mm(hi[e >> h], lo[e & (((u64)1 << h) - 1)], P.p, P.ninv);
+}
+
+template <bool SQUARE>
+__global__ __launch_bounds__(1024) void k3_row_mul(const MulJob* __restrict__ jobs, MulJob one, int L1, int L2,
+                                                  int lr_col, const u32* __restrict__ tw3,
+                                                  const u32* __restrict__ o3, int h, Primes3 PS) {
+    extern __shared__ u32 lds3[];
+    const int R = 1 << L2;
+    const int n3 = L1 + L2;
+    u32* la = lds3;
+    u32* lb = lds3 + lds_words(R);
+    const unsigned rows = 1u << L1;
+    const unsigned job = blockIdx.x / (3u * rows);
+    const int pi = (int)((blockIdx.x / rows) % 3u);
+    const int row = (int)(blockIdx.x % rows);
+    const MulJob J = jobs ? jobs[job] : one;
+    const Prime3 P = pick3(PS, pi);
+    u32* __restrict__ A = (u32*)J.wsA + ((size_t)pi << n3);
+    const u32* __restrict__ B = (const u32*)J.wsB + ((size_t)pi << n3);
+    const size_t nhi = (size_t)1 << (n3 - h), nlo = (size_t)1 << h;
+    const u32* hi = o3 + (size_t)pi * 2 * (nhi + nlo);
+    const u32* lo = hi + nhi;
+    const u32* hi_inv = lo + nlo;
+    const u32* lo_inv = hi_inv + nhi;
+    const u32* tw = tw3 + (size_t)pi * TW3_STRIDE + R;
+    const size_t base = (size_t)row * R;
+    const u64 c = freq_rt(row, L1, lr_col);
+    const u64 N = (u64)1 << n3;
+    {
+        u32 w = r_tw(hi, lo, h, (c * threadIdx.x) & (N - 1), P);
+        const u32 step = r_tw(hi, lo, h, (c * blockDim.x) & (N - 1), P);
+        for (int e = threadIdx.x; e < R; e += blockDim.x) {
+            la[lpad(e)] = mm(A[base + e], w, P.p, P.ninv);
+            if constexpr (!SQUARE) lb[lpad(e)] = mm(B[base + e], w, P.p, P.ninv);
+            w = mm(w, step, P.p, P.ninv);
+        }
+    }
+    __syncthreads();
+    if constexpr (SQUARE) {
+        u32* arr[1] = {la};
+        r_lds_dft<false, 1>(arr, L2, 0, tw, P);
+        for (int e = threadIdx.x; e < R; e += blockDim.x) { const u32 v = la[lpad(e)]; la[lpad(e)] = mm(v, v, P.p, P.ninv); }
+    } else {
+        u32* arr[2] = {la, lb};
+        r_lds_dft<false, 2>(arr, L2, 0, tw, P);
+        for (int e = threadIdx.x; e < R; e += blockDim.x) la[lpad(e)] = mm(la[lpad(e)], lb[lpad(e)], P.p, P.ninv);
+    }
+    __syncthreads();
+    u32* arr1[1] = {la};
+    r_lds_dft<true, 1>(arr1, L2, 0, tw, P);
+    {
+        u32 w = r_tw(hi_inv, lo_inv, h, (c * threadIdx.x) & (N - 1), P);
+        const u32 step = r_tw(hi, lo, h, (N - c * blockDim.x) & (N - 1), P);
+        for (int e = threadIdx.x; e < R; e += blockDim.x) {
+            A[base + e] = mm(la[lpad(e)], w, P.p, P.ninv);
+            w = mm(w, step, P.p, P.ninv);
+        }
+    }
+}
+
+// ---- inverse column pass, CRT and the in-segment carry ------------------------------
+// Garner: x0 = r0, x1 = (r1 - x0) p0^-1 mod p1, x2 = (r2 - x0 - x1 p0)(p0 p1)^-1
+// mod p2, X = x0 + x1 p0 + x2 p0 p1 < p0 p1 p2, returned as lo32 + up * 2^32.
+__device__ __forceinline__ void crt3(u32 r0, u32 r1, u32 r2, const Primes3& PS, u32& lo32, u64& up) {
+    const Prime3& P1 = PS.p[1];
+    const Prime3& P2 = PS.p[2];
+    const u32 x0 = r0;                                              // < p0 < p1 < p2
+    const u32 x1 = mm(r_sub(r1, x0, P1.p), PS.inv_p0_m1, P1.p, P1.ninv);
+    const u32 t = r_sub(r_sub(r2, x0, P2.p), mm(x1, PS.p0_m2, P2.p, P2.ninv), P2.p);
+    const u32 x2 = mm(t, PS.inv_p0p1_m2, P2.p, P2.ninv);
+    const u64 a = (u64)x0 + (u64)x1 * PS.p[0].p;                   // < 2^62
+    const u64 b = a + (u64)x2 * (u32)PS.p0p1;                       // < 2^62 + 2^63
+    lo32 = (u32)b;
+    up = (b >> 32) + (u64)x2 * (u32)(PS.p0p1 >> 32);
+}
+
+template <int T>
+__global__ __launch_bounds__(1024) void k3_col_inv(const MulJob* __restrict__ jobs, MulJob one, int n3, int L1,
+                                                  const u32* __restrict__ tw3, Primes3 PS, size_t nblk_max) {
+    extern __shared__ u32 lds3[];
+    const int L2 = n3 - L1;
+    const size_t S = (size_t)1 << L2;
+    const int R = 1 << L1;
+    const unsigned tiles = (unsigned)(S / T);
+    const unsigned bid = xcd_tile_order(blockIdx.x, gridDim.x);
+    const MulJob J = jobs ? jobs[bid / tiles] : one;
+    const size_t b0 = (size_t)(bid % tiles) * T;
+    const int words = lds_words(R * T);
+    for (int pi = 0; pi < 3; ++pi) {
+        const u32* __restrict__ buf = (const u32*)J.wsA + ((size_t)pi << n3);
+        u32* l = lds3 + pi * words;
+        for (int e = threadIdx.x; e < R * T; e += blockDim.x) {
+            const int col = e & (T - 1), i = e / T;
+            l[lpad(e)] = buf[(size_t)i * S + b0 + col];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int pi = 0; pi < 3; ++pi) {
+        u32* arr[1] = {lds3 + pi * words};
+        r_lds_dft<true, 1>(arr, L1, QLogT<T>::v, tw3 + (size_t)pi * TW3_STRIDE + R, PS.p[pi]);
+    }
+    // chunk = the T coefficients (= limbs) of one row segment; 2T is the
+    // carry geometry's coefficient count for T limbs
+    const CarryJob C = carry_job<2 * T>(J, nblk_max);
+    for (int i = threadIdx.x; i < R; i += blockDim.x) {
+        const size_t k0 = (size_t)i * S + b0;
+        const size_t ch = k0 / T;
+        if (ch >= C.nchunks) continue;
+        u64 carry = 0;
+#pragma unroll
+        for (int q = 0; q < T; ++q) {
+            const int e = lpad(i * T + q);
+            u32 lo32; u64 up;
+            crt3(lds3[e], lds3[words + e], lds3[2 * words + e], PS, lo32, up);
+            const u64 s = (carry & 0xFFFFFFFFull) + lo32;
+            if (k0 + q < C.nlimbs) C.out[k0 + q] = (u32)s;
+            carry = (s >> 32) + (carry >> 32) + up;
+        }
+        C.C[ch] = carry;
+    }
+}
+
+// ---- launcher -----------------------------------------------------------------------
+template <class K>
+static hipError_t set_lds3(K kernel, size_t bytes) {
+    return hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+hipError_t init_kernel_attrs3() {
+    constexpr size_t MAXLDS = 147456;
+    hipError_t e;
+#define PDH3_SET(TT) \
+    if ((e = set_lds3(k3_col_fwd<TT>, MAXLDS)) != hipSuccess) return e; \
+    if ((e = set_lds3(k3_col_inv<TT>, MAXLDS)) != hipSuccess) return e;
+    PDH3_SET(4) PDH3_SET(8) PDH3_SET(16) PDH3_SET(32)
+#undef PDH3_SET
+    if ((e = set_lds3(k3_row_mul<true>, MAXLDS)) != hipSuccess) return e;
+    return set_lds3(k3_row_mul<false>, MAXLDS);
+}
+
+#define PDH3_LAUNCH(name, s, ...) do { hipEvent_t _t0 = timing_begin(s); hipLaunchKernelGGL(__VA_ARGS__); \
+    g_launches.fetch_add(1, std::memory_order_relaxed); hipError_t _e = hipGetLastError(); \
+    timing_end(name, _t0, s); if (_e != hipSuccess) return _e; } while (0)
+
+template <int T>
+static hipError_t launch_cols3(const MulJob* d_jobs, const MulJob& one, unsigned njobs, int ops, int n3,
+                               const NttShape& sh, const OuterTables& ot, hipStream_t s, bool inverse,
+                               size_t nblk_max) {
+    const Primes3& PS = primes3();
+    const unsigned tiles = (unsigned)(((size_t)1 << (n3 - sh.L1)) / T);
+    const int words = lds_words((1 << sh.L1) * T);
+    if (!inverse) {
+        PDH3_LAUNCH("k3_col_fwd", s, k3_col_fwd<T>, dim3(njobs * ops * 3 * tiles), dim3(1024), (size_t)words * 4, s,
+                    d_jobs, one, ops, n3, sh.L1, ot.tw3, PS);
+        return hipSuccess;
+    }
+    PDH3_LAUNCH("k3_col_inv", s, k3_col_inv<T>, dim3(njobs * tiles), dim3(1024), (size_t)3 * words * 4, s,
+                d_jobs, one, n3, sh.L1, ot.tw3, PS, nblk_max);
+    return launch_carry_scan(T, d_jobs, one, njobs, nblk_max, s);
+}
+
+static hipError_t launch_cols3_T(const MulJob* d_jobs, const MulJob& one, unsigned njobs, int ops, int n3,
+                                 const NttShape& sh, const OuterTables& ot, hipStream_t s, bool inverse,
+                                 size_t nblk_max) {
+    switch (sh.T) {
+    case 4: return launch_cols3<4>(d_jobs, one, njobs, ops, n3, sh, ot, s, inverse, nblk_max);
+    case 8: return launch_cols3<8>(d_jobs, one, njobs, ops, n3, sh, ot, s, inverse, nblk_max);
+    case 16: return launch_cols3<16>(d_jobs, one, njobs, ops, n3, sh, ot, s, inverse, nblk_max);
+    case 32: return launch_cols3<32>(d_jobs, one, njobs, ops, n3, sh, ot, s, inverse, nblk_max);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_class3(const MulJob* d_jobs, const MulJob& one, unsigned njobs, const NttShape& sh, bool square,
+                         size_t nblk_max, const OuterTables& ot, hipStream_t s) {
+    const int n3 = sh.n - 1;
+    hipError_t e = launch_cols3_T(d_jobs, one, njobs, square ? 1 : 2, n3, sh, ot, s, false, nblk_max);
+    if (e != hipSuccess) return e;
+    const size_t lds = (size_t)(square ? 1 : 2) * lds_words(1 << sh.L2) * 4;
+    const dim3 grid((unsigned)(njobs * 3u) << sh.L1);
+    const Primes3& PS = primes3();
+    if (square) PDH3_LAUNCH("k3_row_sqr", s, k3_row_mul<true>, grid, dim3(1024), lds, s, d_jobs, one, sh.L1, sh.L2,
+                            sh.lr_col, ot.tw3, ot.o3, ot.h3, PS);
+    else PDH3_LAUNCH("k3_row_mul", s, k3_row_mul<false>, grid, dim3(1024), lds, s, d_jobs, one, sh.L1, sh.L2,
+                     sh.lr_col, ot.tw3, ot.o3, ot.h3, PS);
+    return launch_cols3_T(d_jobs, one, njobs, 1, n3, sh, ot, s, true, nblk_max);
+}
+
+} // namespace pdh

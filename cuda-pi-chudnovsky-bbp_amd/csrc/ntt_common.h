@@ -1,0 +1,181 @@
+// ============================================================================
+//  ntt_common.h -- device helpers shared by the Goldilocks and three-prime
+//  transforms: LDS padding, digit-reversed frequency maps, XCD-aware tile
+//  order, and the chunked carry (carry functions, per-job carry scratch,
+//  LDS staging) that both engines' inverse column passes feed.
+// ============================================================================
+#pragma once
+#include "gl_field.h"
+#include "pdh_internal.h"
+
+namespace pdh {
+
+__device__ __forceinline__ int lpad(int e) { return e + (e >> 3); }
+__host__ __device__ constexpr int lds_words(int e) { return e + (e >> 3); }
+
+template <int T> struct QLogT { static constexpr int v = T == 1 ? 0 : T == 2 ? 1 : T == 4 ? 2 : T == 8 ? 3 : T == 16 ? 4 : T == 32 ? 5 : 6; };
+template <int Q> struct QLog { static constexpr int v = Q == 2 ? 1 : Q == 4 ? 2 : Q == 8 ? 3 : 4; };
+
+// Frequency held at position i after the forward sub-transform: the position's
+// digits (widths L mod LR, LR, LR, ... from the top) are the frequency's
+// digits from the bottom.
+template <int LR>
+__device__ __forceinline__ u64 freq_of_pos(int i, int L) {
+    const int r0 = L % LR;
+    int left = L, sh = 0;
+    u64 c = 0;
+    if (r0) { left -= r0; c = (u64)((i >> left) & ((1 << r0) - 1)); sh = r0; }
+    while (left > 0) { left -= LR; c |= (u64)((i >> left) & ((1 << LR) - 1)) << sh; sh += LR; }
+    return c;
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2). The column passes read and write T-word row segments, so neighbouring
+// tiles share HBM lines; renumbering gives XCD x the contiguous range
+// [x G/8, (x+1) G/8) of the grid, so the tiles sharing a line run at the same
+// time on the same L2 instead of each XCD fetching whole lines for a part.
+__device__ __forceinline__ unsigned xcd_tile_order(unsigned b, unsigned G) {
+    return (G & 7) ? b : (b & 7) * (G >> 3) + (b >> 3);
+}
+
+
+__device__ __forceinline__ u64 freq_rt(int i, int L, int lr) {
+    const int r0 = L % lr;
+    int left = L, sh = 0;
+    u64 c = 0;
+    if (r0) { left -= r0; c = (u64)((i >> left) & ((1 << r0) - 1)); sh = r0; }
+    while (left > 0) { left -= lr; c |= (u64)((i >> left) & ((1 << lr) - 1)) << sh; sh += lr; }
+    return c;
+}
+
+// ---- carry propagation --------------------------------------------------------------
+// A chunk of k limbs (2k coefficients) with carry-in 0 yields 32k bits D and a
+// carry-out C (< 2^41); with carry-in c the carry-out is C + [c >= 2^(32k) - D].
+// For k >= 2, 2^(32k) - D fits in 64 bits only when limbs 2..k-1 of D are all
+// ones (then it is 2^64 - D mod 2^64, or never if that is 2^64); for k = 1 it
+// is 2^32 - D. So each chunk is the function f(c) = c >= thr ? C+1 : C, and
+// compositions keep that form: the carry is a parallel scan of these triples.
+struct CarryFn { u64 thr, v0, v1; };
+__device__ __forceinline__ u64 cf_apply(const CarryFn& f, u64 c) { return c >= f.thr ? f.v1 : f.v0; }
+// (later o earlier)
+__device__ __forceinline__ CarryFn cf_compose(const CarryFn& later, const CarryFn& earlier) {
+    return CarryFn{earlier.thr, cf_apply(later, earlier.v0), cf_apply(later, earlier.v1)};
+}
+// Padding entries (after the last real chunk) are the constant-0 function.
+__device__ __forceinline__ CarryFn cf_pad() { return CarryFn{~(u64)0, 0, 0}; }
+
+constexpr int CARRY_NT = 256;
+// Multi-workgroup carry: each thread takes CPT = max(1, 16 / KL) consecutive
+// chunks (KL limbs each), a workgroup CARRY_NT * CPT chunks.
+template <int T> struct CarryGeom {
+    static constexpr int KL = T / 2;
+    static constexpr int CPT = KL >= 16 ? 1 : 16 / KL;
+    static constexpr int LIMBS = CARRY_NT * CPT * KL;          // limbs per workgroup
+};
+// LDS index of a workgroup's limb w: one pad word per 16 keeps the per-thread
+// (16-limb stride) accesses on distinct banks.
+__device__ __forceinline__ int cpad(int w) { return w + (w >> 4); }
+
+__device__ __forceinline__ CarryFn shfl_up_fn(const CarryFn& f, int d) {
+    return CarryFn{__shfl_up(f.thr, d), __shfl_up(f.v0, d), __shfl_up(f.v1, d)};
+}
+
+// Inclusive scan of carry functions over the workgroup (thread 0 earliest);
+// wsum holds one entry per wave (at most 16).
+__device__ __forceinline__ CarryFn block_scan(CarryFn f, CarryFn* wsum) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        CarryFn g = shfl_up_fn(f, d);
+        if (lane >= d) f = cf_compose(f, g);
+    }
+    if (lane == 63) wsum[wid] = f;
+    __syncthreads();
+    if (wid > 0) {
+        CarryFn pre = wsum[0];
+        for (int w = 1; w < wid; ++w) pre = cf_compose(wsum[w], pre);
+        f = cf_compose(f, pre);
+    }
+    __syncthreads();
+    return f;
+}
+
+__device__ __forceinline__ u64 wave_max(u64 v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) { u64 o = __shfl_xor(v, d); v = o > v ? o : v; }
+    return v;
+}
+
+// ---- multi-workgroup carry ---------------------------------------------------------
+// The inverse column pass holds, per workgroup, T adjacent coefficients of
+// every row: a chunk of T coefficients = KL = T/2 limbs. It runs each chunk's
+// carry with carry-in 0 and writes the chunk's digits D (KL limbs, straight
+// into the product buffer) and its carry-out C into the carry scratch; the
+// coefficients themselves never return to HBM. D and C determine the chunk's
+// carry function (thr from D's limbs, v0 = C, v1 = C + 1), so two scans over
+// (D, C) finish the carry: k_carry_reduce composes each workgroup's chunks,
+// k_carry_blocks scans the workgroup aggregates, k_carry_apply adds each
+// chunk's carry-in to D in place. Limbs at or above na + nb are zero in D
+// (the product has na + nb limbs), so they are neither stored nor read.
+//
+// Scratch per job: C[ceil(nl/2)] (enough for T >= 4), then agg[nblk_max]
+// CarryFn, then cin[nblk_max].
+struct CarryJob {
+    u32* out; size_t nlimbs, nchunks, nblk;
+    u64* C; CarryFn* agg; u64* cin;
+};
+template <int T>
+__device__ __forceinline__ CarryJob carry_job(const MulJob& J, size_t nblk_max) {
+    using G = CarryGeom<T>;
+    CarryJob c;
+    c.out = J.out;
+    c.nlimbs = J.na + J.nb;
+    c.nchunks = (2 * c.nlimbs + T - 1) / T;
+    c.nblk = (c.nchunks + CARRY_NT * G::CPT - 1) / (CARRY_NT * G::CPT);
+    c.C = (u64*)J.carry;
+    c.agg = (CarryFn*)(c.C + ((c.nlimbs + 1) / 2 + 3) / 4 * 4);
+    c.cin = (u64*)(c.agg + nblk_max);
+    return c;
+}
+
+// Coalesced copy of the workgroup's limb range [L0, L0 + LIMBS) into LDS
+// (zeros at and above nlimbs), and back.
+template <int T>
+__device__ __forceinline__ void stage_in(const CarryJob& c, size_t L0, u32* sD) {
+    constexpr int LB = CarryGeom<T>::LIMBS;
+    const u32* src = c.out + L0;
+    if (L0 + LB <= c.nlimbs && ((uintptr_t)src & 15) == 0) {
+        for (int v = threadIdx.x; v < LB / 4; v += CARRY_NT) {
+            const uint4 q = ((const uint4*)src)[v];
+            sD[cpad(4 * v)] = q.x; sD[cpad(4 * v + 1)] = q.y; sD[cpad(4 * v + 2)] = q.z; sD[cpad(4 * v + 3)] = q.w;
+        }
+    } else {
+        for (int w = threadIdx.x; w < LB; w += CARRY_NT) sD[cpad(w)] = (L0 + w < c.nlimbs) ? src[w] : 0u;
+    }
+}
+template <int T>
+__device__ __forceinline__ void stage_out(const CarryJob& c, size_t L0, const u32* sD) {
+    constexpr int LB = CarryGeom<T>::LIMBS;
+    u32* dst = c.out + L0;
+    if (L0 + LB <= c.nlimbs && ((uintptr_t)dst & 15) == 0) {
+        for (int v = threadIdx.x; v < LB / 4; v += CARRY_NT)
+            ((uint4*)dst)[v] = uint4{sD[cpad(4 * v)], sD[cpad(4 * v + 1)], sD[cpad(4 * v + 2)], sD[cpad(4 * v + 3)]};
+    } else {
+        for (int w = threadIdx.x; w < LB; w += CARRY_NT)
+            if (L0 + w < c.nlimbs) dst[w] = sD[cpad(w)];
+    }
+}
+
+// f(c) = C + [D + c >= 2^(32 KL)]: the threshold 2^(32 KL) - D fits in 64
+// bits only when limbs 2..KL-1 of D are all ones (KL >= 2).
+template <int KL>
+__device__ __forceinline__ CarryFn chunk_fn(const u32* D, u64 C) {
+    bool ones = true;
+#pragma unroll
+    for (int q = 2; q < KL; ++q) ones &= D[q] == 0xFFFFFFFFu;
+    const u64 lo = (u64)D[0] | ((u64)D[1] << 32);
+    return CarryFn{(ones && lo != 0) ? (u64)(0 - lo) : ~(u64)0, C, C + 1};
+}
+
+
+} // namespace pdh

@@ -38,13 +38,15 @@ int fail_hip(hipError_t e, const char* what) {
 
 #define HIPCALL(expr) do { hipError_t _e = (expr); if (_e != hipSuccess) return fail_hip(_e, #expr); } while (0)
 
-struct Outer { u64* hi = nullptr; u64* lo = nullptr; u64* hi_inv = nullptr; u64* lo_inv = nullptr; int h = 0; };
+struct Outer { u64* hi = nullptr; u64* lo = nullptr; u64* hi_inv = nullptr; u64* lo_inv = nullptr; int h = 0;
+               u32* o3 = nullptr; int h3 = 0; };
 
 struct Ctx {
     std::mutex mu;
     int state = 0;          // 0 unknown, 1 ready, -1 no device
     std::string why;
     u64* twL = nullptr;     // 2^14 entries: twL[2^L + x] = w_{2^L}^x
+    u32* tw3 = nullptr;     // three primes x 2^15 entries: w_{2^L}^x R mod p
     std::map<int, Outer> outer;
     u64* wsA = nullptr; u64* wsB = nullptr; size_t ws_words = 0;
     u64* carry = nullptr; size_t carry_bytes = 0;
@@ -82,6 +84,7 @@ int init_locked(Ctx& c) {
         return fail(PDH_E_NODEVICE, c.why);
     }
     HIPCALL(init_kernel_attrs());
+    HIPCALL(init_kernel_attrs3());
     HIPCALL(hipStreamCreate(&c.stream));
     // twL[2^L + x] = w_{2^L}^x, L = 1..13
     std::vector<u64> tw((size_t)1 << 14, 0);
@@ -92,6 +95,21 @@ int init_locked(Ctx& c) {
     }
     HIPCALL(hipMalloc(&c.twL, tw.size() * sizeof(u64)));
     HIPCALL(hipMemcpy(c.twL, tw.data(), tw.size() * sizeof(u64), hipMemcpyHostToDevice));
+    // tw3[pi * 2^15 + 2^L + x] = w_{2^L}^x R mod p_pi, L = 1..14 (Montgomery form)
+    std::vector<u32> t3(3 * TW3_STRIDE, 0);
+    for (int pi = 0; pi < 3; ++pi) {
+        const u32 p = primes3().p[pi].p;
+        for (int L = 1; L <= 14; ++L) {
+            const u32 w = r3_root(pi, L);
+            u32 v = 1;
+            for (size_t x = 0; x < ((size_t)1 << L); ++x) {
+                t3[pi * TW3_STRIDE + ((size_t)1 << L) + x] = r3_to_mont(v, p);
+                v = r3_mul(v, w, p);
+            }
+        }
+    }
+    HIPCALL(hipMalloc(&c.tw3, t3.size() * sizeof(u32)));
+    HIPCALL(hipMemcpy(c.tw3, t3.data(), t3.size() * sizeof(u32), hipMemcpyHostToDevice));
     c.state = 1;
     return PDH_OK;
 }
@@ -118,10 +136,40 @@ int outer_for(Ctx& c, int n, OuterTables* ot) {
         HIPCALL(hipMemcpy(o.lo_inv, lo_inv.data(), nlo * 8, hipMemcpyHostToDevice));
         HIPCALL(hipMemcpy(o.hi, hi.data(), nhi * 8, hipMemcpyHostToDevice));
         HIPCALL(hipMemcpy(o.hi_inv, hi_inv.data(), nhi * 8, hipMemcpyHostToDevice));
+        // three primes over 2^(n-1) points, per prime: hi | lo | hi_inv | lo_inv
+        // with w^e R = mm(hi[e >> h3], lo[e & (2^h3 - 1)]) and the inverse pair
+        // carrying s = 2^-(n-1) R (undoing the pointwise Montgomery factor)
+        const int n3 = n - 1;
+        o.h3 = (n3 + 1) / 2;
+        const size_t l3 = (size_t)1 << o.h3, h3n = (size_t)1 << (n3 - o.h3);
+        std::vector<u32> t((size_t)3 * 2 * (l3 + h3n));
+        for (int pi = 0; pi < 3; ++pi) {
+            const u32 p = primes3().p[pi].p;
+            const u32 w = r3_root(pi, n3), wi = r3_pow(w, p - 2, p);
+            const u32 sc = r3_mul(r3_pow(r3_pow(2, n3, p), p - 2, p), r3_to_mont(1, p), p);   // 2^-n3 R
+            u32* hi3 = t.data() + (size_t)pi * 2 * (l3 + h3n);
+            u32* lo3 = hi3 + h3n;
+            u32* hi3i = lo3 + l3;
+            u32* lo3i = hi3i + h3n;
+            u32 v = 1, vi = 1;
+            for (size_t x = 0; x < l3; ++x) {
+                lo3[x] = r3_to_mont(v, p); lo3i[x] = r3_to_mont(vi, p);
+                v = r3_mul(v, w, p); vi = r3_mul(vi, wi, p);
+            }
+            const u32 st = v, sti = vi;                   // w^(2^h3), w^(-2^h3)
+            v = 1; vi = sc;
+            for (size_t y = 0; y < h3n; ++y) {
+                hi3[y] = r3_to_mont(v, p); hi3i[y] = r3_to_mont(vi, p);
+                v = r3_mul(v, st, p); vi = r3_mul(vi, sti, p);
+            }
+        }
+        HIPCALL(hipMalloc(&o.o3, t.size() * 4));
+        HIPCALL(hipMemcpy(o.o3, t.data(), t.size() * 4, hipMemcpyHostToDevice));
         it = c.outer.emplace(n, o).first;
     }
     ot->hi = it->second.hi; ot->lo = it->second.lo;
     ot->hi_inv = it->second.hi_inv; ot->lo_inv = it->second.lo_inv; ot->h = it->second.h;
+    ot->tw3 = c.tw3; ot->o3 = it->second.o3; ot->h3 = it->second.h3;
     return PDH_OK;
 }
 
@@ -164,7 +212,7 @@ int mul_dev_locked(Ctx& c, const u32* a, size_t na, const u32* b, size_t nb, boo
     if (n > MAX_LOG) return fail(PDH_E_TOO_LARGE, "transform length 2^" + std::to_string(n) + " exceeds 2^24");
     int rc = ensure_ws(c, N, na + nb);
     if (rc) return rc;
-    OuterTables ot{nullptr, nullptr, nullptr, nullptr, 0};
+    OuterTables ot{nullptr, nullptr, nullptr, nullptr, 0, nullptr, nullptr, 0};
     const NttShape sh = ntt_shape(n);
     if (sh.L1 > 0 && (rc = outer_for(c, n, &ot))) return rc;
     hipError_t e = launch_mul(a, na, b, nb, square, out, out_len, c.wsA, c.wsB, c.carry, c.twL, ot, sh, s);
@@ -566,7 +614,7 @@ int pdh_batch_create(const pdh_mul_desc* descs, size_t n, pdh_batch** plan) {
         cl.njobs = (unsigned)kv.second.size();
         cl.d_jobs = p->d_jobs + first;
         cl.nblk_max = carry_nblk((size_t)1 << (kv.first - 1));
-        cl.ot = OuterTables{nullptr, nullptr, nullptr, nullptr, 0};
+        cl.ot = OuterTables{nullptr, nullptr, nullptr, nullptr, 0, nullptr, nullptr, 0};
         if (kv.first > SMALL_MAX_LOG && (rc = outer_for(c, kv.first, &cl.ot))) { batch_free(p); return rc; }
         first += kv.second.size();
         p->classes.push_back(cl);
@@ -698,11 +746,25 @@ int pdh_tune_get(int n, int* L1, int* T, int* lr_col, int* lr_row) {
 }
 
 int pdh_tune_set(int n, int L1, int T, int lr_col, int lr_row) {
+    if (n < 1 || n > MAX_LOG) return fail(PDH_E_ARG, "bad argument");
     Ctx& c = ctx();
     std::lock_guard<std::mutex> g(c.mu);
-    NttShape sh{n, L1, n - L1, T, lr_col, lr_row};
+    const int eng = ntt_shape(n).eng;
+    NttShape sh{n, L1, n - eng - L1, T, lr_col, lr_row, eng};
     if (L1 == 0) { sh.L2 = n; sh.T = 1; sh.lr_col = 3; }
     if (!set_shape(sh)) return fail(PDH_E_ARG, "shape cannot be launched");
+    return PDH_OK;
+}
+
+int pdh_tune_engine(int n, int engine, int* current) {
+    if (n < 1 || n > MAX_LOG || engine < -1 || engine > 1) return fail(PDH_E_ARG, "bad argument");
+    Ctx& c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    if (engine >= 0 && engine != ntt_shape(n).eng) {
+        if (n <= SMALL_MAX_LOG) return fail(PDH_E_ARG, "products with N <= 2^13 run on one workgroup (engine 0)");
+        set_shape(default_shape_for(n, engine));
+    }
+    if (current) *current = ntt_shape(n).eng;
     return PDH_OK;
 }
 
@@ -750,7 +812,7 @@ int pdh_autotune(int n_lo, int n_hi, int reps, void* stream) {
                 for (int T : {4, 8, 16, 32, 64})
                     for (int lc = 3; lc <= 4; ++lc)
                         for (int lr = 3; lr <= 4; ++lr) {
-                            NttShape sh{n, L1, n - L1, T, lc, lr};
+                            NttShape sh{n, L1, n - def.eng - L1, T, lc, lr, def.eng};
                             if (shape_valid(sh)) cands.push_back(sh);
                         }
         }

@@ -10,16 +10,20 @@
 
 namespace pdh {
 
-// Split of a 2^n-point transform into a column pass over the top L1 index
-// bits (T columns per workgroup) and a row pass over the low L2 bits.
-// n <= SMALL_MAX_LOG runs as one workgroup per multiply instead.
+// How a product with N = 2^n (16-bit digit count, include/ntt.h:47-50) is
+// transformed. n <= SMALL_MAX_LOG runs as one workgroup per multiply
+// (Goldilocks). Above that, engine 0 is the Goldilocks four-step over 2^n
+// points (L1 + L2 = n) and engine 1 the three-prime four-step over 32-bit
+// digits, 2^(n-1) points (L1 + L2 = n - 1): a column pass over the top L1
+// index bits (T columns per workgroup) and a row pass over the low L2 bits.
 struct NttShape {
     int n;       // log2 N
     int L1;      // column sub-transform bits (0 for the one-workgroup path)
     int L2;      // row sub-transform bits
     int T;       // columns per column-pass workgroup
-    int lr_col;  // log2 of the column pass radix (3 or 4)
+    int lr_col;  // log2 of the column pass radix (3 or 4; 3 for engine 1)
     int lr_row;  // log2 of the row pass (and one-workgroup path) radix
+    int eng;     // 0 Goldilocks, 1 three primes
 };
 
 constexpr int SMALL_MAX_LOG = 13;   // 2 x 2^13 u64 (+1/8 padding) = 144 KB LDS
@@ -30,6 +34,7 @@ constexpr int COL_MAX_LOG   = 11;
 NttShape ntt_shape(int n);
 bool shape_valid(const NttShape& s);
 bool set_shape(const NttShape& s);      // false if the shape is not launchable
+NttShape default_shape_for(int n, int eng);
 
 // Device tables for one device: twL[2^L + x] = w_{2^L}^x for L <= 13, and
 // the per-n outer tables (w_N^e = hi[e >> h] * lo[e & (2^h-1)], forward; the
@@ -40,7 +45,33 @@ struct OuterTables {
     const uint64_t* hi_inv;
     const uint64_t* lo_inv;
     int h;
+    // three-prime engine: tw3[pi * TW3_STRIDE + 2^L + x] = w_{2^L}^x R mod p_pi
+    // (L <= 14), and per n the outer tables of all three primes (o3, h3)
+    const uint32_t* tw3;
+    const uint32_t* o3;
+    int h3;
 };
+
+// The three primes, their Montgomery constants (R = 2^32) and radix-8
+// twiddles, and the CRT constants; see ntt3_kernels.hip.
+struct Prime3 {
+    uint32_t p, ninv;        // ninv = -p^-1 mod 2^32
+    uint32_t w8[4], w8i[4];  // w8^k R mod p and w8^-k R mod p, k = 0..3
+};
+struct Primes3 {
+    Prime3 p[3];
+    uint32_t inv_p0_m1;      // p0^-1 R mod p1
+    uint32_t p0_m2;          // p0 R mod p2
+    uint32_t inv_p0p1_m2;    // (p0 p1)^-1 R mod p2
+    uint64_t p0p1;
+};
+constexpr size_t TW3_STRIDE = (size_t)1 << 15;
+const Primes3& primes3();
+// w_{2^L} for prime pi (L <= 24), normal form, and the host's modular helpers.
+uint32_t r3_root(int pi, int L);
+uint32_t r3_mul(uint32_t a, uint32_t b, uint32_t p);
+uint32_t r3_pow(uint32_t a, uint64_t e, uint32_t p);
+uint32_t r3_to_mont(uint32_t a, uint32_t p);
 
 // One multiply as the kernels see it (device-visible when batched).
 struct MulJob {
@@ -54,8 +85,15 @@ struct MulJob {
     void* carry;            // carry scratch (unused by the one-workgroup path)
 };
 
-// Launchers (ntt_kernels.hip). All enqueue on `s`, return hipError_t.
+// Launchers (ntt_kernels.hip, ntt3_kernels.hip). All enqueue on `s`, return hipError_t.
 hipError_t init_kernel_attrs();
+hipError_t init_kernel_attrs3();
+hipError_t launch_class3(const MulJob* d_jobs, const MulJob& one, unsigned njobs, const NttShape& sh, bool square,
+                         size_t nblk_max, const OuterTables& ot, hipStream_t s);
+// The carry scan over chunks of `kl` limbs (digits and carry-outs already in
+// the product buffer and the carry scratch).
+hipError_t launch_carry_scan(int kl, const MulJob* d_jobs, const MulJob& one, unsigned njobs, size_t nblk_max,
+                             hipStream_t s);
 // Every job in a launch shares n; jobs == nullptr launches the single job `one`.
 hipError_t launch_class(const MulJob* d_jobs, const MulJob& one, unsigned njobs, int n, bool square,
                         size_t nblk_max, const uint64_t* twL, const OuterTables& ot, hipStream_t s);

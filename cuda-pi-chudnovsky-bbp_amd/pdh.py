@@ -38,7 +38,7 @@ EXPORTED = (
     "pdh_batch_create", "pdh_batch_run", "pdh_batch_capture", "pdh_batch_destroy", "pdh_mul_batch_dev",
     "pdh_tune_get", "pdh_tune_set", "pdh_autotune", "pdh_dev_alloc", "pdh_dev_free", "pdh_copy_htod",
     "pdh_copy_dtoh", "pdh_stream_create", "pdh_stream_destroy", "pdh_stream_sync",
-    "pdh_bbp_set_exact", "pdh_bbp_exact_count",
+    "pdh_bbp_set_exact", "pdh_bbp_exact_count", "pdh_tune_engine",
 )
 
 
@@ -101,7 +101,8 @@ def lib() -> ctypes.CDLL:
         L.pdh_tune_get.argtypes = [ctypes.c_int, _ip, _ip, _ip, _ip]
         L.pdh_tune_set.argtypes = [ctypes.c_int] * 5
         L.pdh_autotune.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp]
-        for name in ("pdh_tune_get", "pdh_tune_set", "pdh_autotune"):
+        L.pdh_tune_engine.argtypes = [ctypes.c_int, ctypes.c_int, _ip]
+        for name in ("pdh_tune_get", "pdh_tune_set", "pdh_autotune", "pdh_tune_engine"):
             getattr(L, name).restype = ctypes.c_int
         L.pdh_batch_create.argtypes = [ctypes.POINTER(MulDesc), _sz, ctypes.POINTER(_vp)]
         L.pdh_batch_run.argtypes = [_vp, _vp]
@@ -149,6 +150,14 @@ def tune_get(n: int) -> tuple:
     v = [ctypes.c_int() for _ in range(4)]
     _check(lib().pdh_tune_get(n, *[ctypes.byref(x) for x in v]))
     return tuple(x.value for x in v)
+
+
+def tune_engine(n: int, engine: int = -1) -> int:
+    """Engine for products with N = 2^n > 2^13 (0 Goldilocks, 1 three primes);
+    engine = -1 only reads. Returns the engine in effect."""
+    cur = ctypes.c_int()
+    _check(lib().pdh_tune_engine(n, engine, ctypes.byref(cur)))
+    return cur.value
 
 
 def tune_set(n: int, L1: int, T: int, lr_col: int, lr_row: int) -> None:

@@ -122,6 +122,13 @@ int pdh_mul_batch_dev(const pdh_mul_desc* descs, size_t n, void* stream);
 int pdh_tune_get(int n, int* L1, int* T, int* lr_col, int* lr_row);
 /* PDH_E_ARG if the shape cannot be launched. */
 int pdh_tune_set(int n, int L1, int T, int lr_col, int lr_row);
+/* Engine for products with N = 2^n > 2^13: 0 = Goldilocks over 2^n points
+ * (16-bit digits, the reference's field), 1 = three primes below 2^31 over
+ * 2^(n-1) points (32-bit digits, CRT); the default is 1. engine = -1 only
+ * reads. Switching resets that n to the engine's default shape, after which
+ * pdh_tune_get/set use that engine's split (L1 + L2 = n - engine). Every
+ * engine and shape gives the same product bits. */
+int pdh_tune_engine(int n, int engine, int* current);
 /* Time every candidate shape for each n in [n_lo, n_hi] on `stream` (reps
  * timed runs each, on seeded operands in scratch buffers) and keep the
  * fastest. Allocates and frees its scratch; the work is synchronous. */
