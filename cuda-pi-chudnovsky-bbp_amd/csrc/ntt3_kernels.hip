@@ -124,18 +124,31 @@ __device__ __forceinline__ void r_dftq(u32* x, const Prime3& P) {
     else r_dft8<INV>(x, P);
 }
 
+
+// Level twiddles, per prime: for each radix Q in {2, 4, 8}, direction and
+// sub-block size M = 2^logM (logM <= 14), the block at
+// ((qi * 2 + INV) << 15) + 2^logM holds w_M^(+-j c) R mod p at j Q + c
+// (j < M / Q, c < Q). A butterfly's Q - 1 twiddles are then contiguous and
+// load from one address; w_M^(j c) equals the w_{2^L}^((j 2^(L - logM)) c)
+// of the full sub-transform, so the table does not depend on L.
+constexpr int TW3_BLOCK = 1 << 15;
+template <int Q, bool INV>
+__device__ __forceinline__ const u32* tw_level(const u32* tw_prime, int logM) {
+    constexpr int qi = QLog<Q>::v - 1;
+    return tw_prime + (size_t)((qi * 2 + (INV ? 1 : 0)) * TW3_BLOCK) + (1 << logM);
+}
+
 // One radix-Q level of an in-place 2^L-point sub-transform of NARR residue
 // arrays in LDS (same index map as the Goldilocks lds_level): butterfly
 // (beta, j) of sub-block size M = 2^logM = Q m reads i = beta M + j + t m.
 // Forward: DFT_Q then output c times w_M^(j c); inverse: the mirror.
-// tw[2^L + x] = w_{2^L}^x R mod p.
+// tw is the prime's level-twiddle table.
 template <int Q, bool INV, int NARR>
 __device__ __forceinline__ void r_lds_level(u32* const* arr, int L, int logM, int tlog,
                                             const u32* __restrict__ tw, const Prime3& P) {
     constexpr int LQ = QLog<Q>::v;
     const int logm = logM - LQ;
     const int nbf = 1 << (L - LQ + tlog);
-    const int mask = (1 << L) - 1;
     const int tmask = (1 << tlog) - 1;
     const bool has_tw = logm > 0;
     for (int u = threadIdx.x; u < nbf; u += blockDim.x) {
@@ -145,18 +158,16 @@ __device__ __forceinline__ void r_lds_level(u32* const* arr, int L, int logM, in
         const int base = ((rest >> logm) << logM) + j;
         u32 w[Q];
         if (has_tw) {
-            const int step = j << (L - logM);
+            const u32* __restrict__ tl = tw_level<Q, INV>(tw, logM) + j * Q;
 #pragma unroll
-            for (int c = 1; c < Q; ++c) {
-                const int e = (step * c) & mask;
-                w[c] = tw[INV ? ((-e) & mask) : e];
-            }
+            for (int c = 1; c < Q; ++c) w[c] = tl[c];
         }
+        const int a0 = lpad((base << tlog) + col);
 #pragma unroll
         for (int a = 0; a < NARR; ++a) {
             u32 x[Q];
 #pragma unroll
-            for (int t = 0; t < Q; ++t) x[t] = arr[a][lpad(((base + (t << logm)) << tlog) + col)];
+            for (int t = 0; t < Q; ++t) x[t] = arr[a][a0 + pad_step(t << (logm + tlog))];
             if (!INV) {
                 r_dftq<Q, false>(x, P);
                 if (has_tw) {
@@ -171,7 +182,7 @@ __device__ __forceinline__ void r_lds_level(u32* const* arr, int L, int logM, in
                 r_dftq<Q, true>(x, P);
             }
 #pragma unroll
-            for (int t = 0; t < Q; ++t) arr[a][lpad(((base + (t << logm)) << tlog) + col)] = x[t];
+            for (int t = 0; t < Q; ++t) arr[a][a0 + pad_step(t << (logm + tlog))] = x[t];
         }
     }
     __syncthreads();
@@ -195,6 +206,70 @@ __device__ __forceinline__ void r_lds_dft(u32* const* arr, int L, int tlog, cons
     }
 }
 
+// The same level applied to the three residue arrays of one tile (one per
+// prime) in one pass: the butterfly index math and the twiddle exponents are
+// shared, each prime reads its own table (stride TW3_STRIDE) at the same
+// index.
+template <int Q, bool INV>
+__device__ __forceinline__ void r_lds_level3(u32* const* arr, int L, int logM, int tlog,
+                                             const u32* __restrict__ tw0, const Primes3& PS) {
+    constexpr int LQ = QLog<Q>::v;
+    const int logm = logM - LQ;
+    const int nbf = 1 << (L - LQ + tlog);
+    const int tmask = (1 << tlog) - 1;
+    const bool has_tw = logm > 0;
+    for (int u = threadIdx.x; u < nbf; u += blockDim.x) {
+        const int col = u & tmask;
+        const int rest = u >> tlog;
+        const int j = rest & ((1 << logm) - 1);
+        const int base = ((rest >> logm) << logM) + j;
+        int idx[Q];
+        const int a0 = lpad((base << tlog) + col);
+#pragma unroll
+        for (int t = 0; t < Q; ++t) idx[t] = a0 + pad_step(t << (logm + tlog));
+#pragma unroll
+        for (int pi = 0; pi < 3; ++pi) {
+            const Prime3& P = PS.p[pi];
+            const u32* __restrict__ tl = tw_level<Q, INV>(tw0 + (size_t)pi * TW3_STRIDE, logM) + j * Q;
+            u32 x[Q];
+#pragma unroll
+            for (int t = 0; t < Q; ++t) x[t] = arr[pi][idx[t]];
+            if (!INV) {
+                r_dftq<Q, false>(x, P);
+                if (has_tw) {
+#pragma unroll
+                    for (int c = 1; c < Q; ++c) x[c] = mm(x[c], tl[c], P.p, P.ninv);
+                }
+            } else {
+                if (has_tw) {
+#pragma unroll
+                    for (int c = 1; c < Q; ++c) x[c] = mm(x[c], tl[c], P.p, P.ninv);
+                }
+                r_dftq<Q, true>(x, P);
+            }
+#pragma unroll
+            for (int t = 0; t < Q; ++t) arr[pi][idx[t]] = x[t];
+        }
+    }
+    __syncthreads();
+}
+
+template <bool INV>
+__device__ __forceinline__ void r_lds_dft3(u32* const* arr, int L, int tlog, const u32* __restrict__ tw0,
+                                           const Primes3& PS) {
+    const int r0 = L % 3;
+    if (!INV) {
+        int logM = L;
+        if (r0 == 1) { r_lds_level3<2, false>(arr, L, logM, tlog, tw0, PS); logM -= 1; }
+        if (r0 == 2) { r_lds_level3<4, false>(arr, L, logM, tlog, tw0, PS); logM -= 2; }
+        for (; logM >= 3; logM -= 3) r_lds_level3<8, false>(arr, L, logM, tlog, tw0, PS);
+    } else {
+        for (int logM = 3; logM <= L - r0; logM += 3) r_lds_level3<8, true>(arr, L, logM, tlog, tw0, PS);
+        if (r0 == 1) r_lds_level3<2, true>(arr, L, L, tlog, tw0, PS);
+        if (r0 == 2) r_lds_level3<4, true>(arr, L, L, tlog, tw0, PS);
+    }
+}
+
 // A limb below 2^32 < 3p reduced to [0, p).
 // The prime of a (wave-uniform) index, selected field by field so the
 // kernel-argument struct is never indexed dynamically.
@@ -208,8 +283,10 @@ __device__ __forceinline__ u32 r_reduce_limb(u32 v, u32 p) {
 }
 
 // ---- column pass, forward ---------------------------------------------------------
-// Workgroup -> (job, operand, prime, tile); the residue spectra of an operand
-// sit in its workspace as three consecutive arrays of 2^n3 u32.
+// Workgroup -> (job, operand, tile): the tile's limbs are read once and
+// reduced modulo each prime into three LDS arrays, transformed together. The
+// residue spectra of an operand sit in its workspace as three consecutive
+// arrays of 2^n3 u32.
 template <int T>
 __global__ __launch_bounds__(1024) void k3_col_fwd(const MulJob* __restrict__ jobs, MulJob one, int ops, int n3,
                                                   int L1, const u32* __restrict__ tw3, Primes3 PS) {
@@ -219,28 +296,32 @@ __global__ __launch_bounds__(1024) void k3_col_fwd(const MulJob* __restrict__ jo
     const int R = 1 << L1;
     const unsigned tiles = (unsigned)(S / T);
     const unsigned bid = xcd_tile_order(blockIdx.x, gridDim.x);
-    const unsigned per_job = (unsigned)ops * 3u * tiles;
-    const unsigned job = bid / per_job;
-    const unsigned rem = bid % per_job;
-    const int op = (int)(rem / (3u * tiles));
-    const int pi = (int)((rem / tiles) % 3u);
+    const unsigned job = bid / ((unsigned)ops * tiles);
+    const unsigned rem = bid % ((unsigned)ops * tiles);
+    const int op = (int)(rem / tiles);
     const size_t b0 = (size_t)(rem % tiles) * T;
     const MulJob J = jobs ? jobs[job] : one;
-    const Prime3 P = pick3(PS, pi);
     const u32* __restrict__ src = op ? J.b : J.a;
     const size_t nl = op ? J.nb : J.na;
-    u32* __restrict__ dst = (u32*)(op ? J.wsB : J.wsA) + ((size_t)pi << n3);
+    u32* __restrict__ dst = (u32*)(op ? J.wsB : J.wsA);
+    const int words = lds_words(R * T);
     for (int e = threadIdx.x; e < R * T; e += blockDim.x) {
         const int col = e & (T - 1), i = e / T;
         const size_t g = (size_t)i * S + b0 + col;
-        lds3[lpad(e)] = g < nl ? r_reduce_limb(src[g], P.p) : 0u;
+        const u32 v = g < nl ? src[g] : 0u;
+        lds3[lpad(e)] = r_reduce_limb(v, PS.p[0].p);
+        lds3[words + lpad(e)] = r_reduce_limb(v, PS.p[1].p);
+        lds3[2 * words + lpad(e)] = r_reduce_limb(v, PS.p[2].p);
     }
     __syncthreads();
-    u32* arr[1] = {lds3};
-    r_lds_dft<false, 1>(arr, L1, QLogT<T>::v, tw3 + (size_t)pi * TW3_STRIDE + R, P);
+    u32* arr[3] = {lds3, lds3 + words, lds3 + 2 * words};
+    r_lds_dft3<false>(arr, L1, QLogT<T>::v, tw3, PS);
     for (int e = threadIdx.x; e < R * T; e += blockDim.x) {
         const int col = e & (T - 1), i = e / T;
-        dst[(size_t)i * S + b0 + col] = lds3[lpad(e)];
+        const size_t g = (size_t)i * S + b0 + col;
+        dst[g] = lds3[lpad(e)];
+        dst[((size_t)1 << n3) + g] = lds3[words + lpad(e)];
+        dst[((size_t)2 << n3) + g] = lds3[2 * words + lpad(e)];
     }
 }
 
@@ -276,7 +357,7 @@ __global__ __launch_bounds__(1024) void k3_row_mul(const MulJob* __restrict__ jo
     const u32* lo = hi + nhi;
     const u32* hi_inv = lo + nlo;
     const u32* lo_inv = hi_inv + nhi;
-    const u32* tw = tw3 + (size_t)pi * TW3_STRIDE + R;
+    const u32* tw = tw3 + (size_t)pi * TW3_STRIDE;
     const size_t base = (size_t)row * R;
     const u64 c = freq_rt(row, L1, lr_col);
     const u64 N = (u64)1 << n3;
@@ -349,11 +430,8 @@ __global__ __launch_bounds__(1024) void k3_col_inv(const MulJob* __restrict__ jo
         }
     }
     __syncthreads();
-#pragma unroll
-    for (int pi = 0; pi < 3; ++pi) {
-        u32* arr[1] = {lds3 + pi * words};
-        r_lds_dft<true, 1>(arr, L1, QLogT<T>::v, tw3 + (size_t)pi * TW3_STRIDE + R, PS.p[pi]);
-    }
+    u32* arr[3] = {lds3, lds3 + words, lds3 + 2 * words};
+    r_lds_dft3<true>(arr, L1, QLogT<T>::v, tw3, PS);
     // chunk = the T coefficients (= limbs) of one row segment; 2T is the
     // carry geometry's coefficient count for T limbs
     const CarryJob C = carry_job<2 * T>(J, nblk_max);
@@ -405,7 +483,7 @@ static hipError_t launch_cols3(const MulJob* d_jobs, const MulJob& one, unsigned
     const unsigned tiles = (unsigned)(((size_t)1 << (n3 - sh.L1)) / T);
     const int words = lds_words((1 << sh.L1) * T);
     if (!inverse) {
-        PDH3_LAUNCH("k3_col_fwd", s, k3_col_fwd<T>, dim3(njobs * ops * 3 * tiles), dim3(1024), (size_t)words * 4, s,
+        PDH3_LAUNCH("k3_col_fwd", s, k3_col_fwd<T>, dim3(njobs * ops * tiles), dim3(1024), (size_t)3 * words * 4, s,
                     d_jobs, one, ops, n3, sh.L1, ot.tw3, PS);
         return hipSuccess;
     }

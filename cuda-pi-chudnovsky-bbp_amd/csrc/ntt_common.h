@@ -13,6 +13,15 @@ namespace pdh {
 __device__ __forceinline__ int lpad(int e) { return e + (e >> 3); }
 __host__ __device__ constexpr int lds_words(int e) { return e + (e >> 3); }
 
+// LDS offset between element e0 of a butterfly and element e0 + d, d = t D
+// with D = 2^(logm + tlog) the butterfly stride: lpad(e0 + d) - lpad(e0) =
+// d + (d >> 3). For D >= 8 this is exact since D is a multiple of 8; for
+// D < 8 the butterfly starts at e0 = (beta M + j) T + col with beta M T a
+// multiple of Q D >= 8 and j T + col < D, so e0 mod 8 < D and adding the
+// multiple d of D (D | 8) crosses a multiple of 8 exactly when d does. The
+// offset is wave-uniform: one add per element instead of recomputing lpad.
+__device__ __forceinline__ int pad_step(int d) { return d + (d >> 3); }
+
 template <int T> struct QLogT { static constexpr int v = T == 1 ? 0 : T == 2 ? 1 : T == 4 ? 2 : T == 8 ? 3 : T == 16 ? 4 : T == 32 ? 5 : 6; };
 template <int Q> struct QLog { static constexpr int v = Q == 2 ? 1 : Q == 4 ? 2 : Q == 8 ? 3 : 4; };
 

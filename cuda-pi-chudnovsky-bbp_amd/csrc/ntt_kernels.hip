@@ -136,11 +136,12 @@ __device__ __forceinline__ void lds_level(u64* const* arr, int L, int logM, int 
                 w[c] = twL[INV ? ((-e) & mask) : e];
             }
         }
+        const int a0 = lpad((base << tlog) + col);
 #pragma unroll
         for (int a = 0; a < NARR; ++a) {
             u64 x[Q];
 #pragma unroll
-            for (int t = 0; t < Q; ++t) x[t] = arr[a][lpad(((base + (t << logm)) << tlog) + col)];
+            for (int t = 0; t < Q; ++t) x[t] = arr[a][a0 + pad_step(t << (logm + tlog))];
             if (!INV) {
                 dftq<Q, false>(x);
                 if (tw) {
@@ -155,7 +156,7 @@ __device__ __forceinline__ void lds_level(u64* const* arr, int L, int logM, int 
                 dftq<Q, true>(x);
             }
 #pragma unroll
-            for (int t = 0; t < Q; ++t) arr[a][lpad(((base + (t << logm)) << tlog) + col)] = x[t];
+            for (int t = 0; t < Q; ++t) arr[a][a0 + pad_step(t << (logm + tlog))] = x[t];
         }
     }
 }
@@ -594,15 +595,15 @@ static NttShape default_shape_eng(int n, int eng) {
         s.T = tab[n][1];
         return s;
     }
-    // three primes over 2^(n-1) points: rows of up to 2^13, and a column tile
-    // whose three residue arrays fit the LDS (2^L1 * T <= 2^13)
-    const int n3 = n - 1;
+    // three primes over 2^(n-1) points: {L1, T} for n = 14..24, measured on
+    // MI355X with the shape sweep (rows of 2^13..2^14 points at the top)
+    static const int tab3[MAX_LOG + 1][2] = {
+        {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0},
+        {4, 8}, {5, 8}, {6, 8}, {6, 8}, {6, 8}, {6, 16}, {7, 16}, {8, 16}, {8, 16}, {9, 8}, {9, 8}};
     s.eng = 1;
-    s.L1 = n3 - 13 > 4 ? n3 - 13 : 4;
-    s.L2 = n3 - s.L1;
-    int T = 32;
-    while (T > 4 && ((1 << s.L1) * T > 8192 || T > (1 << s.L2))) T >>= 1;
-    s.T = T;
+    s.L1 = tab3[n][0];
+    s.L2 = n - 1 - s.L1;
+    s.T = tab3[n][1];
     return s;
 }
 static NttShape default_shape(int n) { return default_shape_eng(n, n > SMALL_MAX_LOG ? 1 : 0); }
@@ -624,9 +625,11 @@ bool shape_valid(const NttShape& s) {
     if (s.L1 == 0) return s.n <= SMALL_MAX_LOG && s.L2 == s.n && s.eng == 0;
     if (s.n <= SMALL_MAX_LOG) return false;
     if (s.T != 4 && s.T != 8 && s.T != 16 && s.T != 32 && s.T != 64) return false;
-    if (s.T > (1 << s.L2) || s.L1 < 1 || s.L2 < 3 || s.L2 > ROW_MAX_LOG) return false;
-    if (s.eng == 0) return s.L1 + s.L2 == s.n && (size_t)lds_words((1 << s.L1) * s.T) * 8 <= 147456;
-    if (s.eng != 1 || s.lr_col != 3 || s.lr_row != 3 || s.T > 32) return false;
+    if (s.T > (1 << s.L2) || s.L1 < 1 || s.L2 < 3) return false;
+    if (s.eng == 0)
+        return s.L2 <= ROW_MAX_LOG && s.L1 + s.L2 == s.n && (size_t)lds_words((1 << s.L1) * s.T) * 8 <= 147456;
+    // three primes: u32 rows of up to 2^14 points (two arrays fill the 144 KB)
+    if (s.eng != 1 || s.lr_col != 3 || s.lr_row != 3 || s.T > 32 || s.L2 > ROW_MAX_LOG + 1) return false;
     return s.L1 + s.L2 == s.n - 1 && (size_t)3 * lds_words((1 << s.L1) * s.T) * 4 <= 147456;
 }
 

@@ -95,16 +95,29 @@ int init_locked(Ctx& c) {
     }
     HIPCALL(hipMalloc(&c.twL, tw.size() * sizeof(u64)));
     HIPCALL(hipMemcpy(c.twL, tw.data(), tw.size() * sizeof(u64), hipMemcpyHostToDevice));
-    // tw3[pi * 2^15 + 2^L + x] = w_{2^L}^x R mod p_pi, L = 1..14 (Montgomery form)
+    // level twiddles of the three-prime engine, per prime: block (qi, dir) of
+    // 2^15 words for radix Q = 2^(qi+1), then at 2^logM + j Q + c the value
+    // w_{2^logM}^(+-j c) R mod p (Montgomery form), j < 2^logM / Q, c < Q
     std::vector<u32> t3(3 * TW3_STRIDE, 0);
     for (int pi = 0; pi < 3; ++pi) {
         const u32 p = primes3().p[pi].p;
-        for (int L = 1; L <= 14; ++L) {
-            const u32 w = r3_root(pi, L);
-            u32 v = 1;
-            for (size_t x = 0; x < ((size_t)1 << L); ++x) {
-                t3[pi * TW3_STRIDE + ((size_t)1 << L) + x] = r3_to_mont(v, p);
-                v = r3_mul(v, w, p);
+        for (int qi = 0; qi < 3; ++qi) {
+            const int Q = 2 << qi;
+            for (int dir = 0; dir < 2; ++dir) {
+                u32* blk = t3.data() + pi * TW3_STRIDE + (size_t)(qi * 2 + dir) * ((size_t)1 << 15);
+                for (int logM = qi + 1; logM <= 14; ++logM) {
+                    const u32 w0 = r3_root(pi, logM);
+                    const u32 w = dir ? r3_pow(w0, p - 2, p) : w0;
+                    const size_t nj = ((size_t)1 << logM) / Q;
+                    for (size_t j = 0; j < nj; ++j) {
+                        const u32 wj = r3_pow(w, j, p);
+                        u32 v = 1;
+                        for (int c = 0; c < Q; ++c) {
+                            blk[((size_t)1 << logM) + j * Q + c] = r3_to_mont(v, p);
+                            v = r3_mul(v, wj, p);
+                        }
+                    }
+                }
             }
         }
     }

@@ -45,8 +45,9 @@ struct OuterTables {
     const uint64_t* hi_inv;
     const uint64_t* lo_inv;
     int h;
-    // three-prime engine: tw3[pi * TW3_STRIDE + 2^L + x] = w_{2^L}^x R mod p_pi
-    // (L <= 14), and per n the outer tables of all three primes (o3, h3)
+    // three-prime engine: per prime (stride TW3_STRIDE) the level twiddles
+    // w_M^(+-j c) R mod p, j-major per radix, direction and M = 2^logM
+    // (ntt3_kernels.hip), and per n the outer tables of all three primes (o3, h3)
     const uint32_t* tw3;
     const uint32_t* o3;
     int h3;
@@ -65,7 +66,7 @@ struct Primes3 {
     uint32_t inv_p0p1_m2;    // (p0 p1)^-1 R mod p2
     uint64_t p0p1;
 };
-constexpr size_t TW3_STRIDE = (size_t)1 << 15;
+constexpr size_t TW3_STRIDE = (size_t)6 << 15;    // 3 radices x 2 directions x 2^15
 const Primes3& primes3();
 // w_{2^L} for prime pi (L <= 24), normal form, and the host's modular helpers.
 uint32_t r3_root(int pi, int L);

@@ -71,7 +71,7 @@ def traffic(fetch_db: str, write_db: str, calib_bytes: float, cmd: str) -> None:
           f"FETCH_SIZE {cf:.1f}, WRITE_SIZE {cw:.1f}; scale factors {sf:.1f} and {sw:.1f} B per unit")
     print(f"# {'kernel':<30} {'fetch_bytes':>14} {'write_bytes':>14}")
     for k in sorted(set(f) & set(w)):
-        if k.startswith("k_"):
+        if k.startswith(("k_", "k3_")):
             base = k.split("<")[0]
             print(f"{base:<32} {f[k]['FETCH_SIZE'] * sf:>14.0f} {w[k]['WRITE_SIZE'] * sw:>14.0f}")
 

@@ -28,7 +28,7 @@ def shapes(n, eng):
     pts = n - eng
     for L1 in range(1, pts - 2):
         L2 = pts - L1
-        if L2 < 3 or L2 > 13:
+        if L2 < 3 or L2 > (13 if eng == 0 else 14):
             continue
         for T in ((4, 8, 16, 32, 64) if eng == 0 else (4, 8, 16, 32)):
             if T > (1 << L2):
@@ -93,6 +93,8 @@ def test_invalid_shape_rejected(pdh, restore_shapes):
         pdh.tune_set(24, 10, 64, 3, 3)        # three residue tiles beyond the LDS
     with pytest.raises(pdh.PdhError):
         pdh.tune_set(24, 10, 8, 4, 3)         # engine 1 is radix 8 only
+    with pytest.raises(pdh.PdhError):
+        pdh.tune_set(24, 8, 8, 3, 3)          # engine 1 rows longer than 2^14
     with pytest.raises(pdh.PdhError):
         pdh.tune_engine(13, 1)                # one-workgroup sizes stay on engine 0
 
