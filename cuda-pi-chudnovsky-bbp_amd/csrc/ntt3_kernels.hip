@@ -5,17 +5,23 @@
 //  The product is a unique integer, so any exact convolution method gives the
 //  reference's result (src/ntt_cuda.cu:179-243, include/ntt.h:47-91). This
 //  engine convolves 32-bit digits (the limbs themselves) modulo three primes
-//  below 2^31 and recombines each coefficient by CRT:
+//  below 2^30 and recombines each coefficient by CRT:
 //
-//      p0 = 15 * 2^27 + 1,  p1 = 63 * 2^25 + 1,  p2 = 127 * 2^24 + 1
+//      p0 = 105 * 2^23 + 1,  p1 = 107 * 2^23 + 1,  p2 = 119 * 2^23 + 1
 //
-//  A coefficient of a length-2^m cyclic convolution of 32-bit digits is below
-//  2^m * 2^64 <= 2^87 < p0 p1 p2 (2^92.9) for m <= 23, so it is recovered
-//  exactly. The transforms are half as long as with 16-bit digits, and a
-//  residue butterfly is 6 VALU ops (add, sub each: add, sub p, unsigned min)
-//  and a Montgomery multiply (R = 2^32) 5 ops (two v_mad_u64_u32, one
-//  v_mul_lo_u32, sub, min), against 12 and 27 for a Goldilocks butterfly and
-//  multiply.
+//  A coefficient of a length-2^m cyclic convolution of 32-bit digits is at
+//  most min(na, nb) (2^32 - 1)^2 < 2^86 for m <= 23 (na + nb <= 2^m), below
+//  p0 p1 p2 (2^89.35), so it is recovered exactly. The transforms are half as
+//  long as with 16-bit digits.
+//
+//  Lazy residues (Harvey's butterflies): inside the transforms values live in
+//  [0, 2p), sums and differences in [0, 4p) (4p < 2^32), and a Montgomery
+//  product (R = 2^32) of a b < 4p^2 is (a b + m p) / 2^32 < p (4p / 2^32 + 1)
+//  < 2p without a final correction: 3 VALU ops (two v_mad_u64_u32, one
+//  v_mul_lo_u32). A twiddled butterfly output costs sub + add + 3, a reduction
+//  [0, 4p) -> [0, 2p) 2 ops (sub 2p, unsigned min). A radix-8 butterfly with
+//  its seven twiddles is 96 ops (Goldilocks: about 450). Canonical residues
+//  are only formed for the CRT.
 //
 //  Same four-step structure as the Goldilocks engine: column DIF per operand
 //  and prime (k3_col_fwd), a fused row kernel per prime (outer twiddle,
@@ -30,9 +36,9 @@
 namespace pdh {
 
 // ---- host constants ---------------------------------------------------------------
-static const uint32_t kP3[3] = {2013265921u, 2113929217u, 2130706433u};   // 15*2^27+1, 63*2^25+1, 127*2^24+1
-static const uint32_t kG3[3] = {31u, 5u, 3u};                              // generators of the unit groups
-static const int kK3[3] = {27, 25, 24};                                    // 2-adicity of p - 1
+static const uint32_t kP3[3] = {880803841u, 897581057u, 998244353u};      // 105, 107, 119 * 2^23 + 1 (ascending)
+static const uint32_t kG3[3] = {26u, 3u, 3u};                              // generators of the unit groups
+static const int kK3[3] = {23, 23, 23};                                    // 2-adicity of p - 1
 
 uint32_t r3_mul(uint32_t a, uint32_t b, uint32_t p) { return (uint32_t)((uint64_t)a * b % p); }
 uint32_t r3_pow(uint32_t a, uint64_t e, uint32_t p) {
@@ -77,7 +83,6 @@ const Primes3& primes3() {
 
 // ---- residue arithmetic -----------------------------------------------------------
 __device__ __forceinline__ u32 umin32(u32 a, u32 b) { return a < b ? a : b; }
-__device__ __forceinline__ u32 r_add(u32 a, u32 b, u32 p) { const u32 s = a + b; return umin32(s, s - p); }
 __device__ __forceinline__ u32 r_sub(u32 a, u32 b, u32 p) { const u32 d = a - b; return umin32(d, d + p); }
 // a * b * 2^-32 mod p for a, b < p < 2^31: t + m p < 2^64 and (t + m p) / 2^32 < 2p.
 __device__ __forceinline__ u32 mm(u32 a, u32 b, u32 p, u32 ninv) {
@@ -87,43 +92,82 @@ __device__ __forceinline__ u32 mm(u32 a, u32 b, u32 p, u32 ninv) {
     return umin32(r, r - p);
 }
 
-// ---- radix-q DFTs on residues, natural order in and out ---------------------------
-template <bool INV>
-__device__ __forceinline__ void r_dft2(u32* x, const Prime3& P) {
-    const u32 a = r_add(x[0], x[1], P.p), b = r_sub(x[0], x[1], P.p);
-    x[0] = a; x[1] = b;
+// Lazy forms: red2 takes [0, 4p) to [0, 2p); mml is the Montgomery product
+// without its final correction, valid for a b < 4p^2, result in [0, 2p).
+__device__ __forceinline__ u32 red2(u32 x, u32 p2) { return umin32(x, x - p2); }
+__device__ __forceinline__ u32 mml(u32 a, u32 b, u32 p, u32 ninv) {
+    const u64 t = (u64)a * b;
+    const u32 m = (u32)t * ninv;
+    return (u32)(((u64)m * p + t) >> 32);
 }
 
+// ---- radix-q DFTs on lazy residues, natural order in and out ----------------------
+// Inputs in [0, 2p), outputs in [0, 4p).
 template <bool INV>
-__device__ __forceinline__ void r_dft4(u32& x0, u32& x1, u32& x2, u32& x3, const Prime3& P) {
-    const u32 c0 = r_add(x0, x2, P.p), d0 = r_sub(x0, x2, P.p);
-    const u32 c1 = r_add(x1, x3, P.p);
-    const u32 d1 = mm(r_sub(x1, x3, P.p), INV ? P.w8i[2] : P.w8[2], P.p, P.ninv);   // w4^{-1}, w4
-    x0 = r_add(c0, c1, P.p); x2 = r_sub(c0, c1, P.p);
-    x1 = r_add(d0, d1, P.p); x3 = r_sub(d0, d1, P.p);
+__device__ __forceinline__ void l_dft4(u32& x0, u32& x1, u32& x2, u32& x3, const Prime3& P) {
+    const u32 p2 = P.p << 1;
+    const u32 c0 = red2(x0 + x2, p2), d0 = red2(x0 + p2 - x2, p2);
+    const u32 c1 = red2(x1 + x3, p2);
+    const u32 d1 = mml(x1 + p2 - x3, INV ? P.w8i[2] : P.w8[2], P.p, P.ninv);   // w4^{-1}, w4
+    x0 = c0 + c1; x2 = c0 + p2 - c1;
+    x1 = d0 + d1; x3 = d0 + p2 - d1;
 }
 
 // Y_{2k} = DFT4(x_t + x_{t+4})_k, Y_{2k+1} = DFT4((x_t - x_{t+4}) w8^t)_k.
 template <bool INV>
-__device__ __forceinline__ void r_dft8(u32* x, const Prime3& P) {
+__device__ __forceinline__ void l_dft8(u32* x, const Prime3& P) {
     const u32* w = INV ? P.w8i : P.w8;
-    u32 a0 = r_add(x[0], x[4], P.p), b0 = r_sub(x[0], x[4], P.p);
-    u32 a1 = r_add(x[1], x[5], P.p), b1 = mm(r_sub(x[1], x[5], P.p), w[1], P.p, P.ninv);
-    u32 a2 = r_add(x[2], x[6], P.p), b2 = mm(r_sub(x[2], x[6], P.p), w[2], P.p, P.ninv);
-    u32 a3 = r_add(x[3], x[7], P.p), b3 = mm(r_sub(x[3], x[7], P.p), w[3], P.p, P.ninv);
-    r_dft4<INV>(a0, a1, a2, a3, P);
-    r_dft4<INV>(b0, b1, b2, b3, P);
+    const u32 p2 = P.p << 1;
+    u32 a0 = red2(x[0] + x[4], p2), b0 = red2(x[0] + p2 - x[4], p2);
+    u32 a1 = red2(x[1] + x[5], p2), b1 = mml(x[1] + p2 - x[5], w[1], P.p, P.ninv);
+    u32 a2 = red2(x[2] + x[6], p2), b2 = mml(x[2] + p2 - x[6], w[2], P.p, P.ninv);
+    u32 a3 = red2(x[3] + x[7], p2), b3 = mml(x[3] + p2 - x[7], w[3], P.p, P.ninv);
+    l_dft4<INV>(a0, a1, a2, a3, P);
+    l_dft4<INV>(b0, b1, b2, b3, P);
     x[0] = a0; x[2] = a1; x[4] = a2; x[6] = a3;
     x[1] = b0; x[3] = b1; x[5] = b2; x[7] = b3;
 }
 
 template <int Q, bool INV>
-__device__ __forceinline__ void r_dftq(u32* x, const Prime3& P) {
-    if constexpr (Q == 2) r_dft2<INV>(x, P);
-    else if constexpr (Q == 4) r_dft4<INV>(x[0], x[1], x[2], x[3], P);
-    else r_dft8<INV>(x, P);
+__device__ __forceinline__ void l_dftq(u32* x, const Prime3& P) {
+    if constexpr (Q == 2) {
+        const u32 a = x[0] + x[1], b = x[0] + (P.p << 1) - x[1];
+        x[0] = a; x[1] = b;
+    } else if constexpr (Q == 4) {
+        l_dft4<INV>(x[0], x[1], x[2], x[3], P);
+    } else {
+        l_dft8<INV>(x, P);
+    }
 }
 
+
+// One lazy butterfly of a level, values in [0, 2p) in and out. Forward: DFT_Q
+// (outputs in [0, 4p)), then each output c >= 1 times its twiddle (mml gives
+// [0, 2p)) and output 0 reduced; without twiddles every output is reduced.
+// Inverse: inputs c >= 1 times their twiddles first, then DFT_Q and reduce.
+template <int Q, bool INV>
+__device__ __forceinline__ void lazy_butterfly(u32* x, const u32* w, bool has_tw, const Prime3& P) {
+    const u32 p2 = P.p << 1;
+    if (!INV) {
+        l_dftq<Q, false>(x, P);
+        if (has_tw) {
+            x[0] = red2(x[0], p2);
+#pragma unroll
+            for (int c = 1; c < Q; ++c) x[c] = mml(x[c], w[c], P.p, P.ninv);
+        } else {
+#pragma unroll
+            for (int c = 0; c < Q; ++c) x[c] = red2(x[c], p2);
+        }
+    } else {
+        if (has_tw) {
+#pragma unroll
+            for (int c = 1; c < Q; ++c) x[c] = mml(x[c], w[c], P.p, P.ninv);
+        }
+        l_dftq<Q, true>(x, P);
+#pragma unroll
+        for (int c = 0; c < Q; ++c) x[c] = red2(x[c], p2);
+    }
+}
 
 // Level twiddles, per prime: for each radix Q in {2, 4, 8}, direction and
 // sub-block size M = 2^logM (logM <= 14), the block at
@@ -168,19 +212,7 @@ __device__ __forceinline__ void r_lds_level(u32* const* arr, int L, int logM, in
             u32 x[Q];
 #pragma unroll
             for (int t = 0; t < Q; ++t) x[t] = arr[a][a0 + pad_step(t << (logm + tlog))];
-            if (!INV) {
-                r_dftq<Q, false>(x, P);
-                if (has_tw) {
-#pragma unroll
-                    for (int c = 1; c < Q; ++c) x[c] = mm(x[c], w[c], P.p, P.ninv);
-                }
-            } else {
-                if (has_tw) {
-#pragma unroll
-                    for (int c = 1; c < Q; ++c) x[c] = mm(x[c], w[c], P.p, P.ninv);
-                }
-                r_dftq<Q, true>(x, P);
-            }
+            lazy_butterfly<Q, INV>(x, w, has_tw, P);
 #pragma unroll
             for (int t = 0; t < Q; ++t) arr[a][a0 + pad_step(t << (logm + tlog))] = x[t];
         }
@@ -234,19 +266,12 @@ __device__ __forceinline__ void r_lds_level3(u32* const* arr, int L, int logM, i
             u32 x[Q];
 #pragma unroll
             for (int t = 0; t < Q; ++t) x[t] = arr[pi][idx[t]];
-            if (!INV) {
-                r_dftq<Q, false>(x, P);
-                if (has_tw) {
+            u32 w[Q];
+            if (has_tw) {
 #pragma unroll
-                    for (int c = 1; c < Q; ++c) x[c] = mm(x[c], tl[c], P.p, P.ninv);
-                }
-            } else {
-                if (has_tw) {
-#pragma unroll
-                    for (int c = 1; c < Q; ++c) x[c] = mm(x[c], tl[c], P.p, P.ninv);
-                }
-                r_dftq<Q, true>(x, P);
+                for (int c = 1; c < Q; ++c) w[c] = tl[c];
             }
+            lazy_butterfly<Q, INV>(x, w, has_tw, P);
 #pragma unroll
             for (int t = 0; t < Q; ++t) arr[pi][idx[t]] = x[t];
         }
@@ -270,16 +295,17 @@ __device__ __forceinline__ void r_lds_dft3(u32* const* arr, int L, int tlog, con
     }
 }
 
-// A limb below 2^32 < 3p reduced to [0, p).
 // The prime of a (wave-uniform) index, selected field by field so the
 // kernel-argument struct is never indexed dynamically.
 __device__ __forceinline__ Prime3 pick3(const Primes3& PS, int pi) {
     return pi == 0 ? PS.p[0] : (pi == 1 ? PS.p[1] : PS.p[2]);
 }
 
+// A limb below 2^32 < 5p brought into [0, 2p): [0, 5p) -> [0, 3p) -> [0, 2p).
 __device__ __forceinline__ u32 r_reduce_limb(u32 v, u32 p) {
-    v = umin32(v, v - p);
-    return umin32(v, v - p);
+    const u32 p2 = p << 1;
+    v = umin32(v, v - p2);
+    return umin32(v, v - p2);
 }
 
 // ---- column pass, forward ---------------------------------------------------------
@@ -365,20 +391,20 @@ __global__ __launch_bounds__(1024) void k3_row_mul(const MulJob* __restrict__ jo
         u32 w = r_tw(hi, lo, h, (c * threadIdx.x) & (N - 1), P);
         const u32 step = r_tw(hi, lo, h, (c * blockDim.x) & (N - 1), P);
         for (int e = threadIdx.x; e < R; e += blockDim.x) {
-            la[lpad(e)] = mm(A[base + e], w, P.p, P.ninv);
-            if constexpr (!SQUARE) lb[lpad(e)] = mm(B[base + e], w, P.p, P.ninv);
-            w = mm(w, step, P.p, P.ninv);
+            la[lpad(e)] = mml(A[base + e], w, P.p, P.ninv);
+            if constexpr (!SQUARE) lb[lpad(e)] = mml(B[base + e], w, P.p, P.ninv);
+            w = mml(w, step, P.p, P.ninv);
         }
     }
     __syncthreads();
     if constexpr (SQUARE) {
         u32* arr[1] = {la};
         r_lds_dft<false, 1>(arr, L2, 0, tw, P);
-        for (int e = threadIdx.x; e < R; e += blockDim.x) { const u32 v = la[lpad(e)]; la[lpad(e)] = mm(v, v, P.p, P.ninv); }
+        for (int e = threadIdx.x; e < R; e += blockDim.x) { const u32 v = la[lpad(e)]; la[lpad(e)] = mml(v, v, P.p, P.ninv); }
     } else {
         u32* arr[2] = {la, lb};
         r_lds_dft<false, 2>(arr, L2, 0, tw, P);
-        for (int e = threadIdx.x; e < R; e += blockDim.x) la[lpad(e)] = mm(la[lpad(e)], lb[lpad(e)], P.p, P.ninv);
+        for (int e = threadIdx.x; e < R; e += blockDim.x) la[lpad(e)] = mml(la[lpad(e)], lb[lpad(e)], P.p, P.ninv);
     }
     __syncthreads();
     u32* arr1[1] = {la};
@@ -387,8 +413,8 @@ __global__ __launch_bounds__(1024) void k3_row_mul(const MulJob* __restrict__ jo
         u32 w = r_tw(hi_inv, lo_inv, h, (c * threadIdx.x) & (N - 1), P);
         const u32 step = r_tw(hi, lo, h, (N - c * blockDim.x) & (N - 1), P);
         for (int e = threadIdx.x; e < R; e += blockDim.x) {
-            A[base + e] = mm(la[lpad(e)], w, P.p, P.ninv);
-            w = mm(w, step, P.p, P.ninv);
+            A[base + e] = mml(la[lpad(e)], w, P.p, P.ninv);
+            w = mml(w, step, P.p, P.ninv);
         }
     }
 }
@@ -444,7 +470,8 @@ __global__ __launch_bounds__(1024) void k3_col_inv(const MulJob* __restrict__ jo
         for (int q = 0; q < T; ++q) {
             const int e = lpad(i * T + q);
             u32 lo32; u64 up;
-            crt3(lds3[e], lds3[words + e], lds3[2 * words + e], PS, lo32, up);
+            crt3(umin32(lds3[e], lds3[e] - PS.p[0].p), umin32(lds3[words + e], lds3[words + e] - PS.p[1].p),
+                 umin32(lds3[2 * words + e], lds3[2 * words + e] - PS.p[2].p), PS, lo32, up);
             const u64 s = (carry & 0xFFFFFFFFull) + lo32;
             if (k0 + q < C.nlimbs) C.out[k0 + q] = (u32)s;
             carry = (s >> 32) + (carry >> 32) + up;
