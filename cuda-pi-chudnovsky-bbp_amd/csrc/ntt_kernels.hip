@@ -599,7 +599,7 @@ static NttShape default_shape_eng(int n, int eng) {
     // MI355X with the shape sweep (rows of 2^13..2^14 points at the top)
     static const int tab3[MAX_LOG + 1][2] = {
         {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0},
-        {4, 8}, {5, 8}, {6, 8}, {6, 8}, {6, 8}, {6, 16}, {7, 16}, {8, 16}, {8, 16}, {9, 8}, {9, 8}};
+        {4, 8}, {5, 8}, {6, 8}, {6, 8}, {6, 8}, {6, 16}, {7, 16}, {8, 16}, {8, 16}, {9, 8}, {9, 16}};
     s.eng = 1;
     s.L1 = tab3[n][0];
     s.L2 = n - 1 - s.L1;
