@@ -100,12 +100,15 @@ def read_traffic(kernel: str):
     return None
 
 
-def bench_batch(pdh, torch, W, stream, reps=3):
-    """4,096 independent pairs, limb counts log-uniform over [64, 65536]:
-    batched plan replayed as one hipGraph vs the plain path (one pdh_mul_dev
-    launch sequence per multiply, serial on one stream); outputs compared."""
+def bench_batch(pdh, torch, W, stream, reps=3, pairs=None, reps_plain=None):
+    """Independent pairs (default: 4,096 with limb counts log-uniform over
+    [64, 65536]): batched plan replayed as one hipGraph vs the plain path (one
+    pdh_mul_dev launch sequence per multiply, serial on one stream); outputs
+    compared."""
     import numpy as np
-    pairs = W.batch_pairs(3, W.C3_PAIRS, W.C3_LO, W.C3_HI)
+    if pairs is None:
+        pairs = W.batch_pairs(3, W.C3_PAIRS, W.C3_LO, W.C3_HI)
+    reps_plain = reps if reps_plain is None else reps_plain
     na = np.array([len(x) for x, _ in pairs], dtype=np.int64)
     nb = np.array([len(y) for _, y in pairs], dtype=np.int64)
     a_off = np.concatenate([[0], np.cumsum(na)[:-1]])
@@ -128,11 +131,11 @@ def bench_batch(pdh, torch, W, stream, reps=3):
     plain()
     torch.cuda.synchronize()
     e0.record(stream)
-    for _ in range(reps):
+    for _ in range(reps_plain):
         plain()
     e1.record(stream)
     torch.cuda.synchronize()
-    t_plain = e0.elapsed_time(e1) / reps
+    t_plain = e0.elapsed_time(e1) / reps_plain
     ref_out, ref_lens = out.clone(), lens.clone()
     plan = pdh.Batch(descs)
     plan.run(sp)
@@ -236,7 +239,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
-    ap.add_argument("--no-serve", action="store_true", help="skip the serving trace (about a minute)")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving traces (about a minute)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 128 MiB-in, 128 MiB-out negate kernel (torch) so a --pmc profile can "
                          "scale FETCH_SIZE/WRITE_SIZE by a kernel of known traffic")
@@ -387,6 +390,11 @@ def main() -> int:
                                    "bit_exact": got == want_sorted}
 
         extra["c3_batch_4096"] = bench_batch(pdh, torch, W, stream, reps=3)
+        # the binary-splitting products of a 10^7-digit run at tree depths 8..14
+        bs = W.bs_level_pairs(7, int(pdh.pilib().pdh_chudnovsky_terms(10**7)))
+        r = bench_batch(pdh, torch, W, stream, reps=3, pairs=bs, reps_plain=1)
+        r["levels"] = "depths 8-14 of the 10^7-digit tree, 130,048 products of 10..4,862 limbs"
+        extra["c3_bs_levels_1e7"] = r
         # Chudnovsky with every multiply on the device
         g7 = golden.get("pi_dec_sha256", {})
         for digits in (10**6, 10**7):
@@ -400,6 +408,9 @@ def main() -> int:
                 "bit_exact": (hashlib.sha256(s_pi[2:].encode()).hexdigest() == ref["sha256"]) if ref else None}
         if not args.no_serve:
             extra["c5_serve"] = bench_serve(pdh, W)
+            # the same mix arriving ten times faster: requests queue, the
+            # server overlaps and groups them, the serial loop cannot
+            extra["c5_serve_burst"] = bench_serve(pdh, W, n_req=40, rate=20.0)
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(a, b)
     line = {

@@ -635,11 +635,28 @@ bool shape_valid(const NttShape& s) {
 
 NttShape default_shape_for(int n, int eng) { return default_shape_eng(n, eng); }
 
+static bool g_custom[MAX_LOG + 1];   // shape set by pdh_tune_set / pdh_tune_engine / autotune
+
 bool set_shape(const NttShape& s) {
     if (!shape_valid(s)) return false;
     ntt_shape(s.n);
     g_shapes[s.n] = s;
+    g_custom[s.n] = true;
     return true;
+}
+
+// A class of many independent products already fills the GPU, so for the
+// three-prime engine at n = 14..18 short columns and wide tiles (fewer
+// levels, fewer workgroups) beat the single-product table: the 4,096-pair
+// batch (limbs 64..65,536) takes 10.9 ms with L1 = 4, T = 32 against
+// 15.5 ms with the single-product shapes. A shape set explicitly for n is
+// used as it is.
+NttShape batch_shape(int n, unsigned njobs) {
+    const NttShape s = ntt_shape(n);
+    if (g_custom[n] || njobs < 16 || s.eng != 1 || n > 18) return s;
+    NttShape t = s;
+    t.L1 = 4; t.L2 = n - 1 - 4; t.T = 32;
+    return shape_valid(t) ? t : s;
 }
 
 template <class K>
@@ -751,8 +768,9 @@ static hipError_t launch_small(const MulJob* d_jobs, const MulJob& one, unsigned
 // 2^n words and carry scratch of carry_ws_bytes(2^(n-1)) (or of its own
 // nlimbs when it is launched alone: nblk_max is the per-job carry block count).
 hipError_t launch_class(const MulJob* d_jobs, const MulJob& one, unsigned njobs, int n, bool square,
-                        size_t nblk_max, const u64* twL, const OuterTables& ot, hipStream_t s) {
-    const NttShape sh = ntt_shape(n);
+                        size_t nblk_max, const u64* twL, const OuterTables& ot, hipStream_t s,
+                        const NttShape* shape) {
+    const NttShape sh = shape ? *shape : ntt_shape(n);
     const size_t N = (size_t)1 << n;
     if (sh.L1 == 0) {
         const size_t lds = (size_t)(square ? 1 : 2) * lds_words((int)N) * 8;
@@ -781,7 +799,7 @@ hipError_t launch_mul(const uint32_t* a, size_t na, const uint32_t* b, size_t nb
                       uint32_t* out, uint64_t* out_len, uint64_t* wsA, uint64_t* wsB, uint64_t* carry_ws,
                       const uint64_t* twL, const OuterTables& ot, const NttShape& sh, hipStream_t s) {
     MulJob one{a, square ? a : b, out, out_len, na, square ? na : nb, wsA, wsB, carry_ws};
-    return launch_class(nullptr, one, 1, sh.n, square, carry_nblk(na + nb), twL, ot, s);
+    return launch_class(nullptr, one, 1, sh.n, square, carry_nblk(na + nb), twL, ot, s, &sh);
 }
 
 } // namespace pdh

@@ -516,7 +516,7 @@ uint64_t pdh_launch_count(void) { return g_launches.load(); }
 
 // ---- batched multiplies --------------------------------------------------------------
 struct pdh_batch {
-    struct Class { int n; unsigned njobs; MulJob* d_jobs; size_t nblk_max; OuterTables ot; };
+    struct Class { int n; unsigned njobs; MulJob* d_jobs; size_t nblk_max; OuterTables ot; NttShape shape; };
     std::vector<Class> classes;
     std::vector<uint64_t*> zero_lens;     // jobs with an empty operand
     char* arena = nullptr;
@@ -545,7 +545,7 @@ int batch_enqueue(pdh_batch* p, hipStream_t s) {
         const pdh_batch::Class& cl = p->classes[k];
         hipStream_t si = p->streams[k % p->streams.size()];
         MulJob none{};
-        hipError_t e = launch_class(cl.d_jobs, none, cl.njobs, cl.n, false, cl.nblk_max, c.twL, cl.ot, si);
+        hipError_t e = launch_class(cl.d_jobs, none, cl.njobs, cl.n, false, cl.nblk_max, c.twL, cl.ot, si, &cl.shape);
         if (e != hipSuccess) return fail_hip(e, "batch launch");
     }
     // join
@@ -627,6 +627,7 @@ int pdh_batch_create(const pdh_mul_desc* descs, size_t n, pdh_batch** plan) {
         cl.njobs = (unsigned)kv.second.size();
         cl.d_jobs = p->d_jobs + first;
         cl.nblk_max = carry_nblk((size_t)1 << (kv.first - 1));
+        cl.shape = batch_shape(kv.first, cl.njobs);
         cl.ot = OuterTables{nullptr, nullptr, nullptr, nullptr, 0, nullptr, nullptr, 0};
         if (kv.first > SMALL_MAX_LOG && (rc = outer_for(c, kv.first, &cl.ot))) { batch_free(p); return rc; }
         first += kv.second.size();

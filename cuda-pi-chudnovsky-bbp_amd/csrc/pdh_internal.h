@@ -35,6 +35,8 @@ NttShape ntt_shape(int n);
 bool shape_valid(const NttShape& s);
 bool set_shape(const NttShape& s);      // false if the shape is not launchable
 NttShape default_shape_for(int n, int eng);
+// Shape for a launch of njobs independent products of size class n.
+NttShape batch_shape(int n, unsigned njobs);
 
 // Device tables for one device: twL[2^L + x] = w_{2^L}^x for L <= 13, and
 // the per-n outer tables (w_N^e = hi[e >> h] * lo[e & (2^h-1)], forward; the
@@ -97,7 +99,8 @@ hipError_t launch_carry_scan(int kl, const MulJob* d_jobs, const MulJob& one, un
                              hipStream_t s);
 // Every job in a launch shares n; jobs == nullptr launches the single job `one`.
 hipError_t launch_class(const MulJob* d_jobs, const MulJob& one, unsigned njobs, int n, bool square,
-                        size_t nblk_max, const uint64_t* twL, const OuterTables& ot, hipStream_t s);
+                        size_t nblk_max, const uint64_t* twL, const OuterTables& ot, hipStream_t s,
+                        const NttShape* shape = nullptr);
 size_t carry_nblk(size_t nlimbs);
 hipError_t launch_mul(const uint32_t* a, size_t na, const uint32_t* b, size_t nb,
                       bool square, uint32_t* out, uint64_t* out_len,

@@ -74,3 +74,45 @@ def bbp_mulmod_count(positions) -> int:
         bits += (sh > 0).astype(np.uint64)
         ones_d += sh & np.uint64(1)
     return 4 * total - int((bits + ones_d).sum(dtype=np.uint64))
+
+
+def bs_level_sizes(terms: int, depths=range(8, 15)):
+    """Operand limb counts of the binary-splitting products that build the
+    tree nodes at the given depths for a Chudnovsky run of `terms` terms
+    (src/chudnovsky.cpp:64-102; depth 0 is the root, a node [a, b) splits at
+    (a + b) / 2). Each node multiplies its children: Q_L Q_R, T_L Q_R,
+    P_L T_R and P_L P_R. Child sizes are the sums of their leaves' bit
+    lengths, P(k) = (6k-5)(2k-1)(6k-1), Q(k) = k^3 C^3/24, T(k) = (A + Bk) P(k)."""
+    k = np.arange(terms, dtype=np.float64)
+    kk = np.maximum(k, 1.0)
+    p_bits = np.where(k == 0, 1.0, np.log2(6 * kk - 5) + np.log2(2 * kk - 1) + np.log2(6 * kk - 1))
+    q_bits = np.where(k == 0, 1.0, 3 * np.log2(kk) + np.log2(10939058860032000.0))
+    t_bits = p_bits + np.log2(13591409.0 + 545140134.0 * k)
+    cum = {n: np.concatenate([[0.0], np.cumsum(b)]) for n, b in (("P", p_bits), ("Q", q_bits), ("T", t_bits))}
+
+    def limbs(kind, a, b):
+        return max(1, int(np.ceil((cum[kind][b] - cum[kind][a]) / 32.0)))
+
+    out = []
+    for d in depths:
+        edges = [0, terms]
+        for _ in range(d):
+            edges = sorted(set(edges + [(x + y) // 2 for x, y in zip(edges[:-1], edges[1:])]))
+        for a, b in zip(edges[:-1], edges[1:]):
+            m = (a + b) // 2
+            out += [(limbs("Q", a, m), limbs("Q", m, b)), (limbs("T", a, m), limbs("Q", m, b)),
+                    (limbs("P", a, m), limbs("T", m, b)), (limbs("P", a, m), limbs("P", m, b))]
+    return out
+
+
+def bs_level_pairs(seed: int, terms: int, depths=range(8, 15)):
+    """Seeded random operands (top limbs nonzero) with the limb counts of
+    bs_level_sizes, drawn from one generator."""
+    sizes = bs_level_sizes(terms, depths)
+    rng = np.random.default_rng(seed)
+    flat = rng.integers(1, 1 << 32, size=sum(a + b for a, b in sizes), dtype=np.uint32)
+    out, o = [], 0
+    for na, nb in sizes:
+        out.append((flat[o:o + na], flat[o + na:o + na + nb]))
+        o += na + nb
+    return out
