@@ -498,6 +498,21 @@ int pdh_bbp_positions_dev(const uint64_t* d_positions, size_t n, uint8_t* d_valu
     return PDH_OK;
 }
 
+int pdh_bs_groups(uint64_t n_terms, int group_log, uint32_t* d_P, uint32_t* d_Q, uint32_t* d_T, uint64_t* d_len,
+                  int32_t* d_sign, void* stream) {
+    if (group_log < 1 || group_log > 6) return fail(PDH_E_ARG, "group_log must lie in [1, 6]");
+    if (n_terms > ((uint64_t)1 << 28)) return fail(PDH_E_ARG, "more than 2^28 terms");
+    if (n_terms == 0) return PDH_OK;
+    if (!d_P || !d_Q || !d_T || !d_len || !d_sign) return fail(PDH_E_ARG, "null pointer argument");
+    Ctx& c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    int rc = init_locked(c);
+    if (rc) return rc;
+    hipError_t e = launch_bs_groups(n_terms, group_log, d_P, d_Q, d_T, d_len, d_sign, (hipStream_t)stream);
+    if (e != hipSuccess) return fail_hip(e, "binary-splitting groups");
+    return PDH_OK;
+}
+
 int pdh_bbp_set_exact(int mode) {
     if (mode < 0 || mode > 2) return fail(PDH_E_ARG, "pdh_bbp_set_exact: mode must be 0, 1 or 2");
     Ctx& c = ctx();

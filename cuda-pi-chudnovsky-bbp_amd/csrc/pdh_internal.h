@@ -131,6 +131,11 @@ hipError_t launch_bbp_dmax(const uint64_t* d_pos, size_t count, uint64_t* d_dmax
 size_t bbp_few_ws_bytes(size_t count);
 constexpr size_t BBP_CHUNKED_MAX = 8192;      // positions below which the few-positions paths run
 
+// Chudnovsky binary splitting (bs_kernels.hip): P, Q, T of every group of
+// 2^G consecutive terms of [0, n_terms) into slots of 3, 5, 6 x 2^G limbs.
+hipError_t launch_bs_groups(uint64_t n_terms, int G, uint32_t* dP, uint32_t* dQ, uint32_t* dT, uint64_t* dlen,
+                            int32_t* dsgn, hipStream_t s);
+
 extern std::atomic<uint64_t> g_launches;
 
 // Optional per-kernel timing (pdh_timing_enable): when on, each launch is

@@ -12,7 +12,9 @@
 //
 //  * Binary splitting runs bottom-up, level by level, with all of a level's
 //    P, Q, T products in one batched device launch sequence (pdh_mul_batch_dev).
-//    P(0,N), Q(0,N), T(0,N) do not depend on where the range is split.
+//    The leaves and the first six levels (groups of 64 terms) are computed
+//    by one device kernel (pdh_bs_groups). P(0,N), Q(0,N), T(0,N) do not
+//    depend on where the range is split.
 //  * Division is a Newton reciprocal with an exact floor correction (the
 //    reference forwards div_fast to Knuth, src/bignum_div.cpp:170-178); isqrt
 //    is Newton with divisions at doubling precision and an exact correction
@@ -34,6 +36,7 @@
 #include <cstring>
 #include <functional>
 #include <string>
+#include <thread>
 #include <utility>
 #include <vector>
 
@@ -43,10 +46,16 @@ namespace chud {
 typedef std::vector<uint32_t> Mag;
 typedef unsigned __int128 u128;
 
+struct SInt { Mag m; int s = 0; };            // sign s in {-1, 0, 1}, magnitude m
+struct PQT { Mag P, Q; SInt T; };
+
 struct Multiplier {
     std::function<Mag(const Mag&, const Mag&)> mul;
     std::function<Mag(const Mag&)> sqr;
     std::function<std::vector<Mag>(const std::vector<std::pair<const Mag*, const Mag*>>&)> mul_many;
+    // optional: P, Q, T of consecutive groups of terms covering [0, N) (the
+    // bottom levels of the tree computed elsewhere); empty = from the leaves
+    std::function<std::vector<PQT>(uint64_t N)> groups;
 };
 
 // ---- magnitude helpers (little-endian base 2^32, normalized) -----------------
@@ -154,7 +163,6 @@ static void dec(Mag& a) {            // a > 0
 }
 
 // ---- signed values (binary splitting's T) --------------------------------------
-struct SInt { Mag m; int s = 0; };
 static SInt sadd(const SInt& a, const SInt& b) {
     if (a.s == 0) return b;
     if (b.s == 0) return a;
@@ -312,13 +320,20 @@ static std::string to_decimal(const Mag& x, const Multiplier& M) {
         }
         nodes.swap(next);
     }
-    for (size_t i = 0; i < nodes.size(); ++i) leaf_digits(nodes[i], B, &out[i * B]);
+    // the leaves are independent: convert them on several host threads
+    const size_t nth = std::max<size_t>(1, std::min<size_t>({(size_t)std::thread::hardware_concurrency(), (size_t)16,
+                                                            nodes.size() / 64 + 1}));
+    std::vector<std::thread> pool;
+    for (size_t t = 0; t < nth; ++t)
+        pool.emplace_back([&, t]() {
+            for (size_t i = t; i < nodes.size(); i += nth) leaf_digits(nodes[i], B, &out[i * B]);
+        });
+    for (auto& th : pool) th.join();
     size_t nz = out.find_first_not_of('0');
     return nz == std::string::npos ? "0" : out.substr(nz);
 }
 
 // ---- binary splitting, bottom-up with batched levels ---------------------------------
-struct PQT { Mag P, Q; SInt T; };
 
 static const uint64_t A_CONST = 13591409ULL;
 static const uint64_t B_CONST = 545140134ULL;
@@ -341,8 +356,13 @@ static PQT leaf(uint64_t k) {
 }
 
 static PQT binary_split(uint64_t N, const Multiplier& M) {
-    std::vector<PQT> lvl(N);
-    for (uint64_t k = 0; k < N; ++k) lvl[k] = leaf(k);
+    std::vector<PQT> lvl;
+    if (M.groups) {
+        lvl = M.groups(N);
+    } else {
+        lvl.resize(N);
+        for (uint64_t k = 0; k < N; ++k) lvl[k] = leaf(k);
+    }
     while (lvl.size() > 1) {
         const size_t pairs = lvl.size() / 2;
         const bool root = lvl.size() == 2;
@@ -507,11 +527,53 @@ static std::vector<Mag> dev_mul_many(const std::vector<std::pair<const Mag*, con
     return out;
 }
 
+// Leaves and the first six levels on the device (pdh_bs_groups): one node per
+// 64 consecutive terms.
+static std::vector<PQT> dev_groups(uint64_t N) {
+    const int G = 6;
+    const uint64_t groups = (N + (1u << G) - 1) >> G;
+    const size_t cp = (size_t)3 << G, cq = (size_t)5 << G, ct = (size_t)6 << G;
+    void *dP = nullptr, *dQ = nullptr, *dT = nullptr, *dL = nullptr, *dS = nullptr, *st = nullptr;
+    auto release = [&]() { pdh_dev_free(dP); pdh_dev_free(dQ); pdh_dev_free(dT); pdh_dev_free(dL);
+                           pdh_dev_free(dS); pdh_stream_destroy(st); };
+    std::vector<PQT> out(groups);
+    try {
+        check(pdh_stream_create(&st));
+        check(pdh_dev_alloc(groups * cp * 4, &dP));
+        check(pdh_dev_alloc(groups * cq * 4, &dQ));
+        check(pdh_dev_alloc(groups * ct * 4, &dT));
+        check(pdh_dev_alloc(groups * 3 * 8, &dL));
+        check(pdh_dev_alloc(groups * 4, &dS));
+        check(pdh_bs_groups(N, G, (uint32_t*)dP, (uint32_t*)dQ, (uint32_t*)dT, (uint64_t*)dL, (int32_t*)dS, st));
+        std::vector<uint32_t> P(groups * cp), Q(groups * cq), T(groups * ct);
+        std::vector<uint64_t> L(groups * 3);
+        std::vector<int32_t> S(groups);
+        check(pdh_copy_dtoh(P.data(), dP, P.size() * 4, st));
+        check(pdh_copy_dtoh(Q.data(), dQ, Q.size() * 4, st));
+        check(pdh_copy_dtoh(T.data(), dT, T.size() * 4, st));
+        check(pdh_copy_dtoh(L.data(), dL, L.size() * 8, st));
+        check(pdh_copy_dtoh(S.data(), dS, S.size() * 4, st));
+        check(pdh_stream_sync(st));
+        for (uint64_t g = 0; g < groups; ++g) {
+            out[g].P.assign(P.begin() + g * cp, P.begin() + g * cp + L[3 * g]);
+            out[g].Q.assign(Q.begin() + g * cq, Q.begin() + g * cq + L[3 * g + 1]);
+            out[g].T.m.assign(T.begin() + g * ct, T.begin() + g * ct + L[3 * g + 2]);
+            out[g].T.s = S[g];
+        }
+    } catch (...) {
+        release();
+        throw;
+    }
+    release();
+    return out;
+}
+
 Multiplier device_multiplier() {
     Multiplier M;
     M.mul = dev_mul;
     M.sqr = dev_sqr;
     M.mul_many = dev_mul_many;
+    M.groups = dev_groups;
     return M;
 }
 

@@ -38,7 +38,7 @@ EXPORTED = (
     "pdh_batch_create", "pdh_batch_run", "pdh_batch_capture", "pdh_batch_destroy", "pdh_mul_batch_dev",
     "pdh_tune_get", "pdh_tune_set", "pdh_autotune", "pdh_dev_alloc", "pdh_dev_free", "pdh_copy_htod",
     "pdh_copy_dtoh", "pdh_stream_create", "pdh_stream_destroy", "pdh_stream_sync",
-    "pdh_bbp_set_exact", "pdh_bbp_exact_count", "pdh_tune_engine",
+    "pdh_bbp_set_exact", "pdh_bbp_exact_count", "pdh_tune_engine", "pdh_bs_groups",
 )
 
 
@@ -89,6 +89,8 @@ def lib() -> ctypes.CDLL:
         L.pdh_bbp_positions.argtypes = [_u64p, _sz, _u8p]
         L.pdh_bbp_positions_dev.argtypes = [_vp, _sz, _vp, _vp]
         L.pdh_launch_count.restype = ctypes.c_uint64
+        L.pdh_bs_groups.argtypes = [ctypes.c_uint64, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp]
+        L.pdh_bs_groups.restype = ctypes.c_int
         L.pdh_bbp_set_exact.argtypes = [ctypes.c_int]
         L.pdh_bbp_set_exact.restype = ctypes.c_int
         L.pdh_bbp_exact_count.restype = ctypes.c_uint64
@@ -263,6 +265,12 @@ def bbp_values_positions(positions) -> np.ndarray:
 
 def bbp_positions_dev(d_pos: int, n: int, d_out: int, stream: int = 0) -> None:
     _check(lib().pdh_bbp_positions_dev(d_pos, n, d_out, stream or None))
+
+
+def bs_groups(n_terms: int, group_log: int, d_P: int, d_Q: int, d_T: int, d_len: int, d_sign: int,
+              stream: int = 0) -> None:
+    """Device binary-splitting groups (pdh_bs_groups); all pointers are device addresses."""
+    _check(lib().pdh_bs_groups(n_terms, group_log, d_P, d_Q, d_T, d_len, d_sign, stream or None))
 
 
 def bbp_set_exact(mode: int) -> None:

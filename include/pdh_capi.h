@@ -158,6 +158,18 @@ int pdh_bbp_positions(const uint64_t* positions, size_t n, uint8_t* values);
 int pdh_bbp_positions_dev(const uint64_t* d_positions, size_t n, uint8_t* d_values,
                           void* stream);
 
+/* The bottom of the Chudnovsky binary-splitting tree on the device (the
+ * reference computes leaves and merges on the host, src/chudnovsky.cpp:
+ * 64-102, leaves :70-83): for every group g of 2^group_log consecutive
+ * terms of [0, n_terms), P, Q and T of the group, written to device buffers
+ * at g * 3 * 2^group_log, g * 5 * 2^group_log and g * 6 * 2^group_log limbs
+ * (normalized little-endian magnitudes), their lengths to d_len[3g..3g+2]
+ * and the sign of T (-1, 0, 1) to d_sign[g]. The values do not depend on
+ * how a range is split, so they are the reference's P(a,b), Q(a,b), T(a,b).
+ * group_log in [1, 6], n_terms <= 2^28. Enqueued on `stream`. */
+int pdh_bs_groups(uint64_t n_terms, int group_log, uint32_t* d_P, uint32_t* d_Q, uint32_t* d_T,
+                  uint64_t* d_len, int32_t* d_sign, void* stream);
+
 /* Testing and diagnostics (no counterpart in the reference).
  * pdh_bbp_set_exact(mode) for calls below 8192 positions: 0 (default) the
  * fixed-point path with the FP64-order fallback; 1 the FP64-order path for

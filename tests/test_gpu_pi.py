@@ -42,3 +42,45 @@ def test_independent_fixture(pdh, golden, digits):
     s, ph = pi_dev(pdh, digits)
     print(digits, "phases ms", [round(x, 1) for x in ph])
     assert hashlib.sha256(s[2:].encode()).hexdigest() == golden["pi_dec_sha256"][key]["sha256"]
+
+
+def _leaf(k):
+    A, B, C = 13591409, 545140134, 10939058860032000
+    if k == 0:
+        return 1, 1, A
+    p = (6 * k - 5) * (2 * k - 1) * (6 * k - 1)
+    return p, k ** 3 * C, (A + B * k) * p * (-1 if k & 1 else 1)
+
+
+@pytest.mark.parametrize("n_terms,G", [(1, 1), (7, 2), (64, 6), (1000, 6), (777, 5), (5000, 6)])
+def test_bs_groups_match_the_definition(pdh, n_terms, G):
+    """pdh_bs_groups: P, Q, T of every group of 2^G terms equal the products
+    and the sum that define them (src/chudnovsky.cpp:64-102), computed here
+    with Python integers, including the partial last group and the sign."""
+    import numpy as np
+    import torch
+    groups = (n_terms + (1 << G) - 1) >> G
+    dP = torch.zeros(groups * (3 << G), dtype=torch.int32, device="cuda")
+    dQ = torch.zeros(groups * (5 << G), dtype=torch.int32, device="cuda")
+    dT = torch.zeros(groups * (6 << G), dtype=torch.int32, device="cuda")
+    dL = torch.zeros(groups * 3, dtype=torch.int64, device="cuda")
+    dS = torch.zeros(groups, dtype=torch.int32, device="cuda")
+    pdh.bs_groups(n_terms, G, dP.data_ptr(), dQ.data_ptr(), dT.data_ptr(), dL.data_ptr(), dS.data_ptr(),
+                  torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    P, Q, T = (x.cpu().numpy().view(np.uint32) for x in (dP, dQ, dT))
+    L, S = dL.cpu().numpy(), dS.cpu().numpy()
+
+    def val(arr, off, n):
+        return sum(int(arr[off + i]) << (32 * i) for i in range(n))
+    for g in range(groups):
+        lo, hi = g << G, min(n_terms, (g + 1) << G)
+        p, q, t = _leaf(lo)
+        for k in range(lo + 1, hi):
+            pk, qk, tk = _leaf(k)
+            p, q, t = p * pk, q * qk, t * qk + p * tk
+        assert val(P, g * (3 << G), L[3 * g]) == p, g
+        assert val(Q, g * (5 << G), L[3 * g + 1]) == q, g
+        assert int(S[g]) * val(T, g * (6 << G), L[3 * g + 2]) == t, g
+    with pytest.raises(pdh.PdhError):
+        pdh.bs_groups(10, 7, dP.data_ptr(), dQ.data_ptr(), dT.data_ptr(), dL.data_ptr(), dS.data_ptr())
