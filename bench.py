@@ -402,10 +402,13 @@ def main() -> int:
             s_pi, ph = pdh.pi_chudnovsky(digits)
             dt = time.perf_counter() - t
             ref = g7.get(str(digits))
-            extra[f"chudnovsky_{digits}"] = {
-                "s": dt, "phases_ms": {"split": ph[0], "pow10": ph[1], "sqrt": ph[2], "divide": ph[3], "format": ph[4]},
-                "terms": int(ph[5]),
-                "bit_exact": (hashlib.sha256(s_pi[2:].encode()).hexdigest() == ref["sha256"]) if ref else None}
+            rec = {"s": dt, "phases_ms": {"split": ph[0], "pow10": ph[1], "sqrt": ph[2], "divide": ph[3], "format": ph[4]},
+                   "terms": int(ph[5]),
+                   "bit_exact": (hashlib.sha256(s_pi[2:].encode()).hexdigest() == ref["sha256"]) if ref else None}
+            if digits > 10**6 and str(10**6) in g7:
+                rec["prefix_1e6_bit_exact"] = (hashlib.sha256(s_pi[2:2 + 10**6].encode()).hexdigest()
+                                               == g7[str(10**6)]["sha256"])
+            extra[f"chudnovsky_{digits}"] = rec
         if not args.no_serve:
             extra["c5_serve"] = bench_serve(pdh, W)
             # the same mix arriving ten times faster: requests queue, the

@@ -637,11 +637,11 @@ NttShape default_shape_for(int n, int eng) { return default_shape_eng(n, eng); }
 
 static bool g_custom[MAX_LOG + 1];   // shape set by pdh_tune_set / pdh_tune_engine / autotune
 
-bool set_shape(const NttShape& s) {
+bool set_shape(const NttShape& s, bool custom) {
     if (!shape_valid(s)) return false;
     ntt_shape(s.n);
     g_shapes[s.n] = s;
-    g_custom[s.n] = true;
+    g_custom[s.n] = custom;
     return true;
 }
 

@@ -791,7 +791,7 @@ int pdh_tune_engine(int n, int engine, int* current) {
     std::lock_guard<std::mutex> g(c.mu);
     if (engine >= 0 && engine != ntt_shape(n).eng) {
         if (n <= SMALL_MAX_LOG) return fail(PDH_E_ARG, "products with N <= 2^13 run on one workgroup (engine 0)");
-        set_shape(default_shape_for(n, engine));
+        set_shape(default_shape_for(n, engine), engine != 1);   // engine 1 is the default above 2^13
     }
     if (current) *current = ntt_shape(n).eng;
     return PDH_OK;

@@ -33,7 +33,9 @@ constexpr int COL_MAX_LOG   = 11;
 
 NttShape ntt_shape(int n);
 bool shape_valid(const NttShape& s);
-bool set_shape(const NttShape& s);      // false if the shape is not launchable
+// false if the shape is not launchable; custom = false marks an engine's
+// default shape (batches then keep their own shape rule)
+bool set_shape(const NttShape& s, bool custom = true);
 NttShape default_shape_for(int n, int eng);
 // Shape for a launch of njobs independent products of size class n.
 NttShape batch_shape(int n, unsigned njobs);

@@ -44,6 +44,28 @@ def test_independent_fixture(pdh, golden, digits):
     assert hashlib.sha256(s[2:].encode()).hexdigest() == golden["pi_dec_sha256"][key]["sha256"]
 
 
+def test_1e7_digits_engines_agree(pdh, golden):
+    """10^7 digits: the first 10^6 equal the independent fixture, and the
+    whole string is the same whether every multiply runs on the three-prime
+    engine or on the reference's Goldilocks NTT (a different exact algorithm
+    for every product); with a 10^7 fixture present, its hash too."""
+    digits = 10 ** 7
+    s1, _ = pi_dev(pdh, digits)
+    assert hashlib.sha256(s1[2:2 + 10 ** 6].encode()).hexdigest() == golden["pi_dec_sha256"]["1000000"]["sha256"]
+    saved = {n: pdh.tune_engine(n) for n in range(14, 25)}
+    try:
+        for n in range(14, 25):
+            pdh.tune_engine(n, 0)
+        s0, _ = pi_dev(pdh, digits)
+    finally:
+        for n, e in saved.items():
+            pdh.tune_engine(n, e)
+    assert s0 == s1
+    ref = golden["pi_dec_sha256"].get(str(digits))
+    if ref:
+        assert hashlib.sha256(s1[2:].encode()).hexdigest() == ref["sha256"]
+
+
 def _leaf(k):
     A, B, C = 13591409, 545140134, 10939058860032000
     if k == 0:
