@@ -70,6 +70,12 @@ const Primes3& primes3() {
                 s.p[i].w8[k] = r3_to_mont(r3_pow(w8, k, p), p);
                 s.p[i].w8i[k] = r3_to_mont(r3_pow(w8i, k, p), p);
             }
+            const uint32_t w16 = r3_root(i, 4), w16i = r3_inv(w16, p);
+            const int e16[3] = {1, 3, 9};
+            for (int k = 0; k < 3; ++k) {
+                s.p[i].w16[k] = r3_to_mont(r3_pow(w16, e16[k], p), p);
+                s.p[i].w16i[k] = r3_to_mont(r3_pow(w16i, e16[k], p), p);
+            }
         }
         const uint32_t p0 = kP3[0], p1 = kP3[1], p2 = kP3[2];
         s.inv_p0_m1 = r3_to_mont(r3_inv(p0 % p1, p1), p1);
@@ -128,6 +134,41 @@ __device__ __forceinline__ void l_dft8(u32* x, const Prime3& P) {
     x[1] = b0; x[3] = b1; x[5] = b2; x[7] = b3;
 }
 
+// 16 = 4 x 4: with t = 4 t1 + t2 and k = k1 + 4 k2,
+// Y_k = sum_t2 w4^(t2 k2) [w16^(t2 k1) sum_t1 w4^(t1 k1) x_{4 t1 + t2}]:
+// four DFT4 over t1, nine twiddles w16^(t2 k1) (t2, k1 >= 1), four DFT4
+// over t2. The middle values are reduced to [0, 2p) by the twiddle product or
+// red2, so each DFT4 sees inputs in [0, 2p).
+template <bool INV>
+__device__ __forceinline__ void l_dft16(u32* x, const Prime3& P) {
+    const u32 p2 = P.p << 1;
+    const u32* w8 = INV ? P.w8i : P.w8;
+    const u32* w16 = INV ? P.w16i : P.w16;
+#pragma unroll
+    for (int t2 = 0; t2 < 4; ++t2) l_dft4<INV>(x[t2], x[t2 + 4], x[t2 + 8], x[t2 + 12], P);
+    // x[t2 + 4 k1] now holds the inner DFT at (t2, k1); exponents t2 k1:
+    // 1 2 3 / 2 4 6 / 3 6 9 -> w16, w8, w16^3 / w8, w4, w8^3 / w16^3, w8^3, w16^9
+    x[0] = red2(x[0], p2); x[1] = red2(x[1], p2); x[2] = red2(x[2], p2); x[3] = red2(x[3], p2);
+    x[4] = red2(x[4], p2); x[8] = red2(x[8], p2); x[12] = red2(x[12], p2);
+    x[5] = mml(x[5], w16[0], P.p, P.ninv);
+    x[6] = mml(x[6], w8[1], P.p, P.ninv);
+    x[7] = mml(x[7], w16[1], P.p, P.ninv);
+    x[9] = mml(x[9], w8[1], P.p, P.ninv);
+    x[10] = mml(x[10], w8[2], P.p, P.ninv);
+    x[11] = mml(x[11], w8[3], P.p, P.ninv);
+    x[13] = mml(x[13], w16[1], P.p, P.ninv);
+    x[14] = mml(x[14], w8[3], P.p, P.ninv);
+    x[15] = mml(x[15], w16[2], P.p, P.ninv);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) l_dft4<INV>(x[4 * k1], x[4 * k1 + 1], x[4 * k1 + 2], x[4 * k1 + 3], P);
+    // x[4 k1 + k2] = Y_{k1 + 4 k2}: transpose to natural order
+    u32 y[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) y[k] = x[4 * (k & 3) + (k >> 2)];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) x[k] = y[k];
+}
+
 template <int Q, bool INV>
 __device__ __forceinline__ void l_dftq(u32* x, const Prime3& P) {
     if constexpr (Q == 2) {
@@ -135,8 +176,10 @@ __device__ __forceinline__ void l_dftq(u32* x, const Prime3& P) {
         x[0] = a; x[1] = b;
     } else if constexpr (Q == 4) {
         l_dft4<INV>(x[0], x[1], x[2], x[3], P);
-    } else {
+    } else if constexpr (Q == 8) {
         l_dft8<INV>(x, P);
+    } else {
+        l_dft16<INV>(x, P);
     }
 }
 
@@ -169,7 +212,7 @@ __device__ __forceinline__ void lazy_butterfly(u32* x, const u32* w, bool has_tw
     }
 }
 
-// Level twiddles, per prime: for each radix Q in {2, 4, 8}, direction and
+// Level twiddles, per prime: for each radix Q in {2, 4, 8, 16}, direction and
 // sub-block size M = 2^logM (logM <= 14), the block at
 // ((qi * 2 + INV) << 15) + 2^logM holds w_M^(+-j c) R mod p at j Q + c
 // (j < M / Q, c < Q). A butterfly's Q - 1 twiddles are then contiguous and
@@ -419,6 +462,179 @@ __global__ __launch_bounds__(1024) void k3_row_mul(const MulJob* __restrict__ jo
     }
 }
 
+// ---- register-resident row pass (radix 16) -------------------------------------------
+// A row of R = 2^L2 elements is spread over R / 16 threads, 16 elements each,
+// held in registers. The transform runs in "sets": in set B a thread holds the
+// 16 positions ((u >> B) << (B + 4)) | (t << B) | (u & (2^B - 1)), t < 16, and
+// applies one radix-16 step at M = 2^(B + 4) (four radix-2 levels) to them.
+// Sets run at B = L2 - 4, L2 - 8, ... and a last set at B = 0, which applies
+// the remaining L2 mod 4 levels (radix 2, 4 or 8, no twiddles) when L2 is not
+// a multiple of 4. Between sets the values pass through LDS (write in the old
+// set's positions, barrier, read in the new set's): one barrier and one LDS
+// round trip per four levels, where level-at-a-time passes take one per three
+// levels plus the loads, stores and pointwise product through LDS. Set
+// L2 - 4 is t R/16 + u, coalesced in global memory, so the outer twiddle is
+// applied on load and store. A and B are transformed together (one barrier
+// per exchange for both), the pointwise product is formed in registers and
+// the inverse runs the sets backwards from where the forward ended.
+template <int L2>
+struct RowPlan {
+    static constexpr int TPR = 1 << (L2 - 4);           // threads per row
+    static constexpr int WG = TPR < 256 ? 256 : TPR;    // threads per workgroup
+    static constexpr int G = WG / TPR;                  // rows per workgroup
+    static constexpr int REM = L2 % 4;                  // levels of the last, partial set
+    static constexpr int NSET = L2 / 4 + (REM ? 1 : 0);
+    // bottom bit of set s: L2 - 4 (s + 1) for the full sets, 0 for the partial one
+    static constexpr int bottom(int s) { return (REM && s == NSET - 1) ? 0 : L2 - 4 * (s + 1); }
+    static constexpr bool partial(int s) { return REM && s == NSET - 1; }
+};
+
+template <int B>
+__device__ __forceinline__ int set_base(int u) {
+    return lpad(((u >> B) << (B + 4)) | (u & ((1 << B) - 1)));
+}
+
+template <int B>
+__device__ __forceinline__ void set_store(u32* __restrict__ l, int a0, const u32* x) {
+#pragma unroll
+    for (int t = 0; t < 16; ++t) l[a0 + pad_step(t << B)] = x[t];
+}
+
+template <int B>
+__device__ __forceinline__ void set_load(const u32* __restrict__ l, int a0, u32* x) {
+#pragma unroll
+    for (int t = 0; t < 16; ++t) x[t] = l[a0 + pad_step(t << B)];
+}
+
+// Twiddles of set S's radix-16 step: w_M^(j c), M = 2^(B + 4), j = u mod 2^B.
+template <int L2, int S, bool INV>
+__device__ __forceinline__ void set_tw(u32* w, int u, const u32* __restrict__ tw) {
+    constexpr int B = RowPlan<L2>::bottom(S);
+    if constexpr (!RowPlan<L2>::partial(S) && B > 0) {
+        const uint4* tl = (const uint4*)(tw_level<16, INV>(tw, B + 4) + (u & ((1 << B) - 1)) * 16);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint4 v = tl[k];
+            w[4 * k] = v.x; w[4 * k + 1] = v.y; w[4 * k + 2] = v.z; w[4 * k + 3] = v.w;
+        }
+    }
+}
+
+// Set S's step: the radix-16 butterfly, or for the partial set the
+// 2^(4 - REM) butterflies of radix 2^REM over the low bits.
+template <int L2, int S, bool INV>
+__device__ __forceinline__ void set_step(u32* x, const u32* w, const Prime3& P) {
+    using RP = RowPlan<L2>;
+    if constexpr (RP::partial(S)) {
+        constexpr int Q = 1 << RP::REM;
+#pragma unroll
+        for (int hq = 0; hq < 16 / Q; ++hq) lazy_butterfly<Q, INV>(x + hq * Q, w, false, P);
+    } else {
+        lazy_butterfly<16, INV>(x, w, RP::bottom(S) > 0, P);
+    }
+}
+
+// Forward of NOP operands (1 or 2, arrays l[0], l[1]): x holds set 0 on entry,
+// the spectrum in the last set's positions on exit.
+template <int L2, int NOP, int S = 0>
+__device__ __forceinline__ void row_fwd(u32 (*x)[16], int u, u32* const* l, const u32* __restrict__ tw,
+                                        const Prime3& P) {
+    using RP = RowPlan<L2>;
+    {
+        u32 w[16];
+        set_tw<L2, S, false>(w, u, tw);
+#pragma unroll
+        for (int k = 0; k < NOP; ++k) set_step<L2, S, false>(x[k], w, P);
+    }
+    if constexpr (S + 1 < RP::NSET) {
+        const int a0 = set_base<RP::bottom(S)>(u), a1 = set_base<RP::bottom(S + 1)>(u);
+#pragma unroll
+        for (int k = 0; k < NOP; ++k) set_store<RP::bottom(S)>(l[k], a0, x[k]);
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < NOP; ++k) set_load<RP::bottom(S + 1)>(l[k], a1, x[k]);
+        row_fwd<L2, NOP, S + 1>(x, u, l, tw, P);
+    }
+}
+
+// Inverse: x holds the last set on entry, set 0 (natural order) on exit.
+template <int L2, int S = RowPlan<L2>::NSET - 1>
+__device__ __forceinline__ void row_inv(u32* x, int u, u32* __restrict__ l, const u32* __restrict__ tw,
+                                        const Prime3& P) {
+    using RP = RowPlan<L2>;
+    {
+        u32 w[16];
+        set_tw<L2, S, true>(w, u, tw);
+        set_step<L2, S, true>(x, w, P);
+    }
+    if constexpr (S > 0) {
+        set_store<RP::bottom(S)>(l, set_base<RP::bottom(S)>(u), x);
+        __syncthreads();
+        set_load<RP::bottom(S - 1)>(l, set_base<RP::bottom(S - 1)>(u), x);
+        row_inv<L2, S - 1>(x, u, l, tw, P);
+    }
+}
+
+template <int L2, bool SQUARE>
+__global__ __launch_bounds__(RowPlan<L2>::WG) void k3_row_reg(const MulJob* __restrict__ jobs, MulJob one, int L1,
+                                                             int lr_col, const u32* __restrict__ tw3,
+                                                             const u32* __restrict__ o3, int h, Primes3 PS) {
+    using RP = RowPlan<L2>;
+    constexpr int NOP = SQUARE ? 1 : 2;
+    constexpr int R = 1 << L2;
+    constexpr int TPR = RP::TPR;
+    extern __shared__ u32 lds3[];
+    const int n3 = L1 + L2;
+    const int g = threadIdx.x / TPR;
+    const int u = threadIdx.x % TPR;
+    u32* l[2] = {lds3 + g * lds_words(R), lds3 + (RP::G + g) * lds_words(R)};
+    const unsigned wgs = (1u << L1) / RP::G;                       // workgroups per prime
+    const unsigned job = blockIdx.x / (3u * wgs);
+    const int pi = (int)((blockIdx.x / wgs) % 3u);
+    const int row = (int)(blockIdx.x % wgs) * RP::G + g;
+    const MulJob J = jobs ? jobs[job] : one;
+    const Prime3 P = pick3(PS, pi);
+    u32* __restrict__ A = (u32*)J.wsA + ((size_t)pi << n3) + (size_t)row * R;
+    const u32* __restrict__ Bv = (const u32*)J.wsB + ((size_t)pi << n3) + (size_t)row * R;
+    const size_t nhi = (size_t)1 << (n3 - h), nlo = (size_t)1 << h;
+    const u32* hi = o3 + (size_t)pi * 2 * (nhi + nlo);
+    const u32* lo = hi + nhi;
+    const u32* hi_inv = lo + nlo;
+    const u32* lo_inv = hi_inv + nhi;
+    const u32* tw = tw3 + (size_t)pi * TW3_STRIDE;
+    const u64 c = freq_rt(row, L1, lr_col);
+    const u64 N = (u64)1 << n3;
+    u32 x[NOP][16];
+    // set 0 from global memory with the outer twiddle w^(b c), b = t R/16 + u
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+        x[0][t] = A[t * TPR + u];
+        if constexpr (!SQUARE) x[1][t] = Bv[t * TPR + u];
+    }
+    {
+        u32 w = r_tw(hi, lo, h, (c * (u64)u) & (N - 1), P);
+        const u32 step = r_tw(hi, lo, h, (c * (u64)TPR) & (N - 1), P);
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+#pragma unroll
+            for (int k = 0; k < NOP; ++k) x[k][t] = mml(x[k][t], w, P.p, P.ninv);
+            w = mml(w, step, P.p, P.ninv);
+        }
+    }
+    row_fwd<L2, NOP>(x, u, l, tw, P);
+#pragma unroll
+    for (int t = 0; t < 16; ++t) x[0][t] = mml(x[0][t], x[NOP - 1][t], P.p, P.ninv);
+    // the inverse's first exchange writes the positions this thread read last
+    row_inv<L2>(x[0], u, l[0], tw, P);
+    u32 w = r_tw(hi_inv, lo_inv, h, (c * (u64)u) & (N - 1), P);
+    const u32 step = r_tw(hi, lo, h, (N - c * (u64)TPR) & (N - 1), P);
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+        A[t * TPR + u] = mml(x[0][t], w, P.p, P.ninv);
+        w = mml(w, step, P.p, P.ninv);
+    }
+}
+
 // ---- inverse column pass, CRT and the in-segment carry ------------------------------
 // Garner: x0 = r0, x1 = (r1 - x0) p0^-1 mod p1, x2 = (r2 - x0 - x1 p0)(p0 p1)^-1
 // mod p2, X = x0 + x1 p0 + x2 p0 p1 < p0 p1 p2, returned as lo32 + up * 2^32.
@@ -481,6 +697,13 @@ __global__ __launch_bounds__(1024) void k3_col_inv(const MulJob* __restrict__ jo
 }
 
 // ---- launcher -----------------------------------------------------------------------
+// Row lengths of the register-resident row kernel and its LDS per workgroup.
+#define PDH3_FOR_ROW(X) X(8) X(9) X(10) X(11) X(12) X(13) X(14)
+static size_t row_reg_lds(int L2, bool square) {
+    const int tpr = 1 << (L2 - 4), wg = tpr < 256 ? 256 : tpr;
+    return (size_t)(square ? 1 : 2) * (wg / tpr) * lds_words(1 << L2) * 4;
+}
+
 template <class K>
 static hipError_t set_lds3(K kernel, size_t bytes) {
     return hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
@@ -495,7 +718,13 @@ hipError_t init_kernel_attrs3() {
     PDH3_SET(4) PDH3_SET(8) PDH3_SET(16) PDH3_SET(32)
 #undef PDH3_SET
     if ((e = set_lds3(k3_row_mul<true>, MAXLDS)) != hipSuccess) return e;
-    return set_lds3(k3_row_mul<false>, MAXLDS);
+    if ((e = set_lds3(k3_row_mul<false>, MAXLDS)) != hipSuccess) return e;
+#define PDH3_SET_ROW(LL) \
+    if ((e = set_lds3(k3_row_reg<LL, true>, row_reg_lds(LL, true))) != hipSuccess) return e; \
+    if ((e = set_lds3(k3_row_reg<LL, false>, row_reg_lds(LL, false))) != hipSuccess) return e;
+    PDH3_FOR_ROW(PDH3_SET_ROW)
+#undef PDH3_SET_ROW
+    return hipSuccess;
 }
 
 #define PDH3_LAUNCH(name, s, ...) do { hipEvent_t _t0 = timing_begin(s); hipLaunchKernelGGL(__VA_ARGS__); \
@@ -539,10 +768,29 @@ hipError_t launch_class3(const MulJob* d_jobs, const MulJob& one, unsigned njobs
     const size_t lds = (size_t)(square ? 1 : 2) * lds_words(1 << sh.L2) * 4;
     const dim3 grid((unsigned)(njobs * 3u) << sh.L1);
     const Primes3& PS = primes3();
-    if (square) PDH3_LAUNCH("k3_row_sqr", s, k3_row_mul<true>, grid, dim3(1024), lds, s, d_jobs, one, sh.L1, sh.L2,
-                            sh.lr_col, ot.tw3, ot.o3, ot.h3, PS);
-    else PDH3_LAUNCH("k3_row_mul", s, k3_row_mul<false>, grid, dim3(1024), lds, s, d_jobs, one, sh.L1, sh.L2,
-                     sh.lr_col, ot.tw3, ot.o3, ot.h3, PS);
+    if (sh.lr_row == 4) {
+        switch (sh.L2) {
+#define PDH3_ROW_CASE(LL) \
+        case LL: { \
+            using RP = RowPlan<LL>; \
+            const dim3 g((unsigned)(njobs * 3u) * ((1u << sh.L1) / RP::G)); \
+            if (square) PDH3_LAUNCH("k3_row_reg_sqr", s, (k3_row_reg<LL, true>), g, dim3(RP::WG), row_reg_lds(LL, true), s, \
+                                    d_jobs, one, sh.L1, sh.lr_col, ot.tw3, ot.o3, ot.h3, PS); \
+            else PDH3_LAUNCH("k3_row_reg", s, (k3_row_reg<LL, false>), g, dim3(RP::WG), row_reg_lds(LL, false), s, \
+                             d_jobs, one, sh.L1, sh.lr_col, ot.tw3, ot.o3, ot.h3, PS); \
+            break; \
+        }
+        PDH3_FOR_ROW(PDH3_ROW_CASE)
+#undef PDH3_ROW_CASE
+        default: return hipErrorInvalidValue;
+        }
+    } else if (square) {
+        PDH3_LAUNCH("k3_row_sqr", s, k3_row_mul<true>, grid, dim3(1024), lds, s, d_jobs, one, sh.L1, sh.L2,
+                    sh.lr_col, ot.tw3, ot.o3, ot.h3, PS);
+    } else {
+        PDH3_LAUNCH("k3_row_mul", s, k3_row_mul<false>, grid, dim3(1024), lds, s, d_jobs, one, sh.L1, sh.L2,
+                    sh.lr_col, ot.tw3, ot.o3, ot.h3, PS);
+    }
     return launch_cols3_T(d_jobs, one, njobs, 1, n3, sh, ot, s, true, nblk_max);
 }
 

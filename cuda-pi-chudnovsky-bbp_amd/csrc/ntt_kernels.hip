@@ -582,7 +582,7 @@ __global__ void k_zero_len(u64* out_len) { *out_len = 0; }
 // ---- host launchers -------------------------------------------------------------------
 // ---- shapes: the default split per n, overridable by pdh_tune_set / autotune ----
 // Defaults measured on MI355X with the shape sweep (the autotuner's search
-// space): {L1, T} for n = 14..24, radix 8 for both passes.
+// space): {L1, T} for n = 14..24; Goldilocks radix 8 for both passes.
 static NttShape default_shape_eng(int n, int eng) {
     NttShape s{n, 0, n, 1, 3, 3, 0};
     if (n <= SMALL_MAX_LOG) return s;
@@ -596,11 +596,13 @@ static NttShape default_shape_eng(int n, int eng) {
         return s;
     }
     // three primes over 2^(n-1) points: {L1, T} for n = 14..24, measured on
-    // MI355X with the shape sweep (rows of 2^13..2^14 points at the top)
+    // MI355X with the shape sweep (rows of 2^13..2^14 points at the top);
+    // rows as register-resident radix-16 sets
     static const int tab3[MAX_LOG + 1][2] = {
         {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0},
-        {4, 8}, {5, 8}, {6, 8}, {6, 8}, {6, 8}, {6, 16}, {7, 16}, {8, 16}, {8, 16}, {9, 8}, {9, 16}};
+        {4, 8}, {5, 8}, {6, 8}, {6, 8}, {6, 8}, {6, 16}, {7, 16}, {8, 16}, {8, 16}, {9, 8}, {9, 8}};
     s.eng = 1;
+    s.lr_row = 4;
     s.L1 = tab3[n][0];
     s.L2 = n - 1 - s.L1;
     s.T = tab3[n][1];
@@ -628,8 +630,11 @@ bool shape_valid(const NttShape& s) {
     if (s.T > (1 << s.L2) || s.L1 < 1 || s.L2 < 3) return false;
     if (s.eng == 0)
         return s.L2 <= ROW_MAX_LOG && s.L1 + s.L2 == s.n && (size_t)lds_words((1 << s.L1) * s.T) * 8 <= 147456;
-    // three primes: u32 rows of up to 2^14 points (two arrays fill the 144 KB)
-    if (s.eng != 1 || s.lr_col != 3 || s.lr_row != 3 || s.T > 32 || s.L2 > ROW_MAX_LOG + 1) return false;
+    // three primes: u32 rows of up to 2^14 points (two arrays fill the 144 KB);
+    // radix-8 columns; rows either radix-8 levels through LDS (lr_row 3) or
+    // register-resident radix-16 sets (lr_row 4, rows of 2^8 points or more)
+    if (s.eng != 1 || s.lr_col != 3 || s.T > 32 || s.L2 > ROW_MAX_LOG + 1) return false;
+    if (s.lr_row == 4 && s.L2 < 8) return false;
     return s.L1 + s.L2 == s.n - 1 && (size_t)3 * lds_words((1 << s.L1) * s.T) * 4 <= 147456;
 }
 

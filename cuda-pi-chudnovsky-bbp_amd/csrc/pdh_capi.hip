@@ -101,7 +101,7 @@ int init_locked(Ctx& c) {
     std::vector<u32> t3(3 * TW3_STRIDE, 0);
     for (int pi = 0; pi < 3; ++pi) {
         const u32 p = primes3().p[pi].p;
-        for (int qi = 0; qi < 3; ++qi) {
+        for (int qi = 0; qi < 4; ++qi) {
             const int Q = 2 << qi;
             for (int dir = 0; dir < 2; ++dir) {
                 u32* blk = t3.data() + pi * TW3_STRIDE + (size_t)(qi * 2 + dir) * ((size_t)1 << 15);

@@ -62,6 +62,7 @@ struct OuterTables {
 struct Prime3 {
     uint32_t p, ninv;        // ninv = -p^-1 mod 2^32
     uint32_t w8[4], w8i[4];  // w8^k R mod p and w8^-k R mod p, k = 0..3
+    uint32_t w16[3], w16i[3];  // w16^k R mod p and w16^-k R mod p, k = 1, 3, 9
 };
 struct Primes3 {
     Prime3 p[3];
@@ -70,7 +71,7 @@ struct Primes3 {
     uint32_t inv_p0p1_m2;    // (p0 p1)^-1 R mod p2
     uint64_t p0p1;
 };
-constexpr size_t TW3_STRIDE = (size_t)6 << 15;    // 3 radices x 2 directions x 2^15
+constexpr size_t TW3_STRIDE = (size_t)8 << 15;    // 4 radices x 2 directions x 2^15
 const Primes3& primes3();
 // w_{2^L} for prime pi (L <= 24), normal form, and the host's modular helpers.
 uint32_t r3_root(int pi, int L);

@@ -3,7 +3,9 @@ radix) gives the same, bit-exact product; the autotuner only changes speed.
 
 Products with N = 2^n > 2^13 run on one of two engines: 0, the Goldilocks
 four-step over 2^n points (the reference's field and digit size), and 1, the
-three-prime four-step over 2^(n-1) points of 32-bit digits (the default)."""
+three-prime four-step over 2^(n-1) points of 32-bit digits (the default),
+whose rows run either as radix-8 levels through LDS (lr_row 3) or as
+register-resident radix-16 sets (lr_row 4)."""
 from __future__ import annotations
 
 import hashlib
@@ -37,7 +39,10 @@ def shapes(n, eng):
                 continue
             if eng == 1 and 3 * lds_words((1 << L1) * T) * 4 > 147456:
                 continue
-            radices = [(lc, lr) for lc in (3, 4) for lr in (3, 4)] if eng == 0 else [(3, 3)]
+            if eng == 0:
+                radices = [(lc, lr) for lc in (3, 4) for lr in (3, 4)]
+            else:
+                radices = [(3, 3)] + ([(3, 4)] if L2 >= 8 else [])
             for lc, lr in radices:
                 out.append((L1, T, lc, lr))
     return out
@@ -92,9 +97,11 @@ def test_invalid_shape_rejected(pdh, restore_shapes):
     with pytest.raises(pdh.PdhError):
         pdh.tune_set(24, 10, 64, 3, 3)        # three residue tiles beyond the LDS
     with pytest.raises(pdh.PdhError):
-        pdh.tune_set(24, 10, 8, 4, 3)         # engine 1 is radix 8 only
+        pdh.tune_set(24, 10, 8, 4, 3)         # engine 1 columns are radix 8 only
     with pytest.raises(pdh.PdhError):
         pdh.tune_set(24, 8, 8, 3, 3)          # engine 1 rows longer than 2^14
+    with pytest.raises(pdh.PdhError):
+        pdh.tune_set(14, 6, 4, 3, 4)          # radix-16 row sets need rows of 2^8 or more
     with pytest.raises(pdh.PdhError):
         pdh.tune_engine(13, 1)                # one-workgroup sizes stay on engine 0
 
