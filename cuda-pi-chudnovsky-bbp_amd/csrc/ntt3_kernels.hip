@@ -489,21 +489,24 @@ struct RowPlan {
     static constexpr bool partial(int s) { return REM && s == NSET - 1; }
 };
 
-template <int B>
-__device__ __forceinline__ int set_base(int u) {
-    return lpad(((u >> B) << (B + 4)) | (u & ((1 << B) - 1)));
+// Positions of set B of radix 2^LQ in a sub-transform whose element i sits at
+// LDS word lpad((i << LOGT) | col): i(t) = ((u >> B) << (B + LQ)) | (t << B) |
+// (u mod 2^B). The offsets of t > 0 from t = 0 are compile-time constants.
+template <int B, int LQ, int LOGT>
+__device__ __forceinline__ int set_base(int u, int col) {
+    return lpad(((((u >> B) << (B + LQ)) | (u & ((1 << B) - 1))) << LOGT) | col);
 }
 
-template <int B>
+template <int B, int LQ, int LOGT>
 __device__ __forceinline__ void set_store(u32* __restrict__ l, int a0, const u32* x) {
 #pragma unroll
-    for (int t = 0; t < 16; ++t) l[a0 + pad_step(t << B)] = x[t];
+    for (int t = 0; t < (1 << LQ); ++t) l[a0 + pad_step((t << B) << LOGT)] = x[t];
 }
 
-template <int B>
+template <int B, int LQ, int LOGT>
 __device__ __forceinline__ void set_load(const u32* __restrict__ l, int a0, u32* x) {
 #pragma unroll
-    for (int t = 0; t < 16; ++t) x[t] = l[a0 + pad_step(t << B)];
+    for (int t = 0; t < (1 << LQ); ++t) x[t] = l[a0 + pad_step((t << B) << LOGT)];
 }
 
 // Twiddles of set S's radix-16 step: w_M^(j c), M = 2^(B + 4), j = u mod 2^B.
@@ -547,12 +550,12 @@ __device__ __forceinline__ void row_fwd(u32 (*x)[16], int u, u32* const* l, cons
         for (int k = 0; k < NOP; ++k) set_step<L2, S, false>(x[k], w, P);
     }
     if constexpr (S + 1 < RP::NSET) {
-        const int a0 = set_base<RP::bottom(S)>(u), a1 = set_base<RP::bottom(S + 1)>(u);
+        const int a0 = set_base<RP::bottom(S), 4, 0>(u, 0), a1 = set_base<RP::bottom(S + 1), 4, 0>(u, 0);
 #pragma unroll
-        for (int k = 0; k < NOP; ++k) set_store<RP::bottom(S)>(l[k], a0, x[k]);
+        for (int k = 0; k < NOP; ++k) set_store<RP::bottom(S), 4, 0>(l[k], a0, x[k]);
         __syncthreads();
 #pragma unroll
-        for (int k = 0; k < NOP; ++k) set_load<RP::bottom(S + 1)>(l[k], a1, x[k]);
+        for (int k = 0; k < NOP; ++k) set_load<RP::bottom(S + 1), 4, 0>(l[k], a1, x[k]);
         row_fwd<L2, NOP, S + 1>(x, u, l, tw, P);
     }
 }
@@ -568,9 +571,9 @@ __device__ __forceinline__ void row_inv(u32* x, int u, u32* __restrict__ l, cons
         set_step<L2, S, true>(x, w, P);
     }
     if constexpr (S > 0) {
-        set_store<RP::bottom(S)>(l, set_base<RP::bottom(S)>(u), x);
+        set_store<RP::bottom(S), 4, 0>(l, set_base<RP::bottom(S), 4, 0>(u, 0), x);
         __syncthreads();
-        set_load<RP::bottom(S - 1)>(l, set_base<RP::bottom(S - 1)>(u), x);
+        set_load<RP::bottom(S - 1), 4, 0>(l, set_base<RP::bottom(S - 1), 4, 0>(u, 0), x);
         row_inv<L2, S - 1>(x, u, l, tw, P);
     }
 }
@@ -609,7 +612,7 @@ __global__ __launch_bounds__(RowPlan<L2>::WG) void k3_row_reg(const MulJob* __re
 #pragma unroll
     for (int t = 0; t < 16; ++t) {
         x[0][t] = A[t * TPR + u];
-        if constexpr (!SQUARE) x[1][t] = Bv[t * TPR + u];
+        if constexpr (NOP == 2) x[1][t] = Bv[t * TPR + u];
     }
     {
         u32 w = r_tw(hi, lo, h, (c * (u64)u) & (N - 1), P);
@@ -696,9 +699,202 @@ __global__ __launch_bounds__(1024) void k3_col_inv(const MulJob* __restrict__ jo
     }
 }
 
+// ---- register-resident column passes (radix-8 sets) ---------------------------------
+// A tile of T columns x R = 2^L1 rows of all three primes; a thread holds 8
+// rows of one column for each prime (24 residues), WG = R/8 * T threads. Sets
+// as in the row kernel but of radix 8 with the partial set first: set 0 holds
+// rows t R/8 + u and applies the top L1 mod 3 levels (radix 2 or 4, twiddled)
+// when L1 is not a multiple of 3, then radix-8 sets at bottoms
+// L1 - L1 mod 3 - 3 s down to 0. That is the level order of the LDS-level
+// column pass, so the frequency order (and the row kernel's c) is the same.
+// Set 0 is read from (forward) or ends in (inverse) rows t R/8 + u, the last
+// set rows 8u + t; both go to and from global memory directly, and the LDS
+// only carries the exchanges between sets (and, inverse, the CRT values to
+// the per-row carry).
+template <int L1, int T>
+struct ColPlan {
+    static constexpr int LOGT = QLogT<T>::v;
+    static constexpr int WG = (1 << (L1 - 3)) * T;
+    static constexpr int REM = L1 % 3;
+    static constexpr int NSET = L1 / 3 + (REM ? 1 : 0);
+    static constexpr bool partial(int s) { return REM && s == 0; }
+    static constexpr int bottom(int s) { return REM ? (s == 0 ? L1 - 3 : L1 - REM - 3 * s) : L1 - 3 * (s + 1); }
+};
+
+// Tiles the register column kernels are built for: at least one wave, the
+// three residue tiles within the LDS limit.
+constexpr bool col_reg_pair(int L1, int T) {
+    const int lt = T == 4 ? 2 : T == 8 ? 3 : T == 16 ? 4 : T == 32 ? 5 : 99;
+    return L1 >= 3 && L1 + lt >= 9 && L1 + lt <= 13;
+}
+
+template <int L1, int T, int S, bool INV>
+__device__ __forceinline__ void col_step(u32* x, int u, const u32* __restrict__ tw, const Prime3& P) {
+    using CP = ColPlan<L1, T>;
+    if constexpr (CP::partial(S)) {
+        // radix Q = 2^REM over the top bits: t = q 2^(3 - REM) + tlo, so the
+        // butterfly of j = tlo R/8 + u takes x[q TL + tlo], q < Q
+        constexpr int Q = 1 << CP::REM, TL = 8 / Q;
+#pragma unroll
+        for (int tlo = 0; tlo < TL; ++tlo) {
+            const int j = tlo * (1 << (L1 - 3)) + u;
+            const u32* __restrict__ tl = tw_level<Q, INV>(tw, L1) + j * Q;
+            u32 y[Q], w[Q];
+#pragma unroll
+            for (int q = 0; q < Q; ++q) y[q] = x[q * TL + tlo];
+#pragma unroll
+            for (int c = 1; c < Q; ++c) w[c] = tl[c];
+            lazy_butterfly<Q, INV>(y, w, true, P);
+#pragma unroll
+            for (int q = 0; q < Q; ++q) x[q * TL + tlo] = y[q];
+        }
+    } else {
+        constexpr int B = CP::bottom(S);
+        u32 w[8];
+        if constexpr (B > 0) {
+            const uint4* tl = (const uint4*)(tw_level<8, INV>(tw, B + 3) + (u & ((1 << B) - 1)) * 8);
+            const uint4 v0 = tl[0], v1 = tl[1];
+            w[1] = v0.y; w[2] = v0.z; w[3] = v0.w; w[4] = v1.x; w[5] = v1.y; w[6] = v1.z; w[7] = v1.w;
+        }
+        lazy_butterfly<8, INV>(x, w, B > 0, P);
+    }
+}
+
+template <int L1, int T, int S = 0>
+__device__ __forceinline__ void col_fwd3(u32 (*x)[8], int u, int col, u32* const* l, const u32* __restrict__ tw0,
+                                         const Primes3& PS) {
+    using CP = ColPlan<L1, T>;
+#pragma unroll
+    for (int p = 0; p < 3; ++p) col_step<L1, T, S, false>(x[p], u, tw0 + (size_t)p * TW3_STRIDE, PS.p[p]);
+    if constexpr (S + 1 < CP::NSET) {
+        constexpr int B0 = CP::bottom(S), B1 = CP::bottom(S + 1);
+        const int a0 = set_base<B0, 3, CP::LOGT>(u, col), a1 = set_base<B1, 3, CP::LOGT>(u, col);
+#pragma unroll
+        for (int p = 0; p < 3; ++p) set_store<B0, 3, CP::LOGT>(l[p], a0, x[p]);
+        __syncthreads();
+#pragma unroll
+        for (int p = 0; p < 3; ++p) set_load<B1, 3, CP::LOGT>(l[p], a1, x[p]);
+        col_fwd3<L1, T, S + 1>(x, u, col, l, tw0, PS);
+    }
+}
+
+template <int L1, int T, int S = ColPlan<L1, T>::NSET - 1>
+__device__ __forceinline__ void col_inv3(u32 (*x)[8], int u, int col, u32* const* l, const u32* __restrict__ tw0,
+                                         const Primes3& PS) {
+    using CP = ColPlan<L1, T>;
+#pragma unroll
+    for (int p = 0; p < 3; ++p) col_step<L1, T, S, true>(x[p], u, tw0 + (size_t)p * TW3_STRIDE, PS.p[p]);
+    if constexpr (S > 0) {
+        constexpr int B0 = CP::bottom(S), B1 = CP::bottom(S - 1);
+        const int a0 = set_base<B0, 3, CP::LOGT>(u, col), a1 = set_base<B1, 3, CP::LOGT>(u, col);
+#pragma unroll
+        for (int p = 0; p < 3; ++p) set_store<B0, 3, CP::LOGT>(l[p], a0, x[p]);
+        __syncthreads();
+#pragma unroll
+        for (int p = 0; p < 3; ++p) set_load<B1, 3, CP::LOGT>(l[p], a1, x[p]);
+        col_inv3<L1, T, S - 1>(x, u, col, l, tw0, PS);
+    }
+}
+
+template <int L1, int T>
+__global__ __launch_bounds__((ColPlan<L1, T>::WG)) void k3_col_fwd_reg(const MulJob* __restrict__ jobs, MulJob one,
+                                                                    int ops, int n3, const u32* __restrict__ tw3,
+                                                                    Primes3 PS) {
+    using CP = ColPlan<L1, T>;
+    extern __shared__ u32 lds3[];
+    constexpr int R = 1 << L1;
+    constexpr int words = lds_words(R * T);
+    const size_t S = (size_t)1 << (n3 - L1);
+    const unsigned tiles = (unsigned)(S / T);
+    const unsigned bid = xcd_tile_order(blockIdx.x, gridDim.x);
+    const unsigned job = bid / ((unsigned)ops * tiles);
+    const unsigned rem = bid % ((unsigned)ops * tiles);
+    const int op = (int)(rem / tiles);
+    const size_t b0 = (size_t)(rem % tiles) * T;
+    const MulJob J = jobs ? jobs[job] : one;
+    const u32* __restrict__ src = op ? J.b : J.a;
+    const size_t nl = op ? J.nb : J.na;
+    u32* __restrict__ dst = (u32*)(op ? J.wsB : J.wsA);
+    const int col = threadIdx.x & (T - 1), u = threadIdx.x >> CP::LOGT;
+    u32 x[3][8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+        const size_t g = (size_t)(t * (R / 8) + u) * S + b0 + col;
+        const u32 v = g < nl ? src[g] : 0u;
+#pragma unroll
+        for (int p = 0; p < 3; ++p) x[p][t] = r_reduce_limb(v, PS.p[p].p);
+    }
+    u32* l[3] = {lds3, lds3 + words, lds3 + 2 * words};
+    col_fwd3<L1, T>(x, u, col, l, tw3, PS);
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+        const size_t g = (size_t)((u << 3) | t) * S + b0 + col;
+#pragma unroll
+        for (int p = 0; p < 3; ++p) dst[((size_t)p << n3) + g] = x[p][t];
+    }
+}
+
+template <int L1, int T>
+__global__ __launch_bounds__((ColPlan<L1, T>::WG)) void k3_col_inv_reg(const MulJob* __restrict__ jobs, MulJob one,
+                                                                    int n3, const u32* __restrict__ tw3, Primes3 PS,
+                                                                    size_t nblk_max) {
+    using CP = ColPlan<L1, T>;
+    extern __shared__ u32 lds3[];
+    constexpr int R = 1 << L1;
+    constexpr int words = lds_words(R * T);
+    const size_t S = (size_t)1 << (n3 - L1);
+    const unsigned tiles = (unsigned)(S / T);
+    const unsigned bid = xcd_tile_order(blockIdx.x, gridDim.x);
+    const MulJob J = jobs ? jobs[bid / tiles] : one;
+    const size_t b0 = (size_t)(bid % tiles) * T;
+    const int col = threadIdx.x & (T - 1), u = threadIdx.x >> CP::LOGT;
+    u32 x[3][8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+        const size_t g = (size_t)((u << 3) | t) * S + b0 + col;
+#pragma unroll
+        for (int p = 0; p < 3; ++p) x[p][t] = ((const u32*)J.wsA)[((size_t)p << n3) + g];
+    }
+    u32* l[3] = {lds3, lds3 + words, lds3 + 2 * words};
+    col_inv3<L1, T>(x, u, col, l, tw3, PS);
+    // CRT of each coefficient of set 0 into (lo32, up) over the three arrays
+    // at the words this thread read last
+    {
+        constexpr int B0 = CP::bottom(0);
+        const int a0 = set_base<B0, 3, CP::LOGT>(u, col);
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const int e = a0 + pad_step((t << B0) << CP::LOGT);
+            u32 lo32; u64 up;
+            crt3(umin32(x[0][t], x[0][t] - PS.p[0].p), umin32(x[1][t], x[1][t] - PS.p[1].p),
+                 umin32(x[2][t], x[2][t] - PS.p[2].p), PS, lo32, up);
+            l[0][e] = lo32; l[1][e] = (u32)up; l[2][e] = (u32)(up >> 32);
+        }
+    }
+    __syncthreads();
+    // chunk = the T coefficients (= limbs) of one row segment, carry-in 0
+    const CarryJob C = carry_job<2 * T>(J, nblk_max);
+    for (int i = threadIdx.x; i < R; i += blockDim.x) {
+        const size_t k0 = (size_t)i * S + b0;
+        const size_t ch = k0 / T;
+        if (ch >= C.nchunks) continue;
+        u64 carry = 0;
+#pragma unroll
+        for (int q = 0; q < T; ++q) {
+            const int e = lpad(i * T + q);
+            const u64 up = (u64)l[1][e] | ((u64)l[2][e] << 32);
+            const u64 s = (carry & 0xFFFFFFFFull) + l[0][e];
+            if (k0 + q < C.nlimbs) C.out[k0 + q] = (u32)s;
+            carry = (s >> 32) + (carry >> 32) + up;
+        }
+        C.C[ch] = carry;
+    }
+}
+
 // ---- launcher -----------------------------------------------------------------------
 // Row lengths of the register-resident row kernel and its LDS per workgroup.
 #define PDH3_FOR_ROW(X) X(8) X(9) X(10) X(11) X(12) X(13) X(14)
+#define PDH3_FOR_COL_L1(X) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11)
 static size_t row_reg_lds(int L2, bool square) {
     const int tpr = 1 << (L2 - 4), wg = tpr < 256 ? 256 : tpr;
     return (size_t)(square ? 1 : 2) * (wg / tpr) * lds_words(1 << L2) * 4;
@@ -707,6 +903,20 @@ static size_t row_reg_lds(int L2, bool square) {
 template <class K>
 static hipError_t set_lds3(K kernel, size_t bytes) {
     return hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+template <int T, int LL = 4>
+static hipError_t set_col_reg_attrs() {
+    constexpr size_t MAXLDS = 147456;
+    if constexpr (LL <= 11) {
+        if constexpr (col_reg_pair(LL, T)) {
+            hipError_t e;
+            if ((e = set_lds3(k3_col_fwd_reg<LL, T>, MAXLDS)) != hipSuccess) return e;
+            if ((e = set_lds3(k3_col_inv_reg<LL, T>, MAXLDS)) != hipSuccess) return e;
+        }
+        return set_col_reg_attrs<T, LL + 1>();
+    }
+    return hipSuccess;
 }
 
 hipError_t init_kernel_attrs3() {
@@ -724,7 +934,10 @@ hipError_t init_kernel_attrs3() {
     if ((e = set_lds3(k3_row_reg<LL, false>, row_reg_lds(LL, false))) != hipSuccess) return e;
     PDH3_FOR_ROW(PDH3_SET_ROW)
 #undef PDH3_SET_ROW
-    return hipSuccess;
+    if ((e = set_col_reg_attrs<4>()) != hipSuccess) return e;
+    if ((e = set_col_reg_attrs<8>()) != hipSuccess) return e;
+    if ((e = set_col_reg_attrs<16>()) != hipSuccess) return e;
+    return set_col_reg_attrs<32>();
 }
 
 #define PDH3_LAUNCH(name, s, ...) do { hipEvent_t _t0 = timing_begin(s); hipLaunchKernelGGL(__VA_ARGS__); \
@@ -738,6 +951,26 @@ static hipError_t launch_cols3(const MulJob* d_jobs, const MulJob& one, unsigned
     const Primes3& PS = primes3();
     const unsigned tiles = (unsigned)(((size_t)1 << (n3 - sh.L1)) / T);
     const int words = lds_words((1 << sh.L1) * T);
+    if (sh.lr_row == 4) {
+        bool done = false;
+        switch (sh.L1) {
+#define PDH3_COL_CASE(LL) \
+        case LL: \
+            if constexpr (col_reg_pair(LL, T)) { \
+                constexpr int WG = ColPlan<LL, T>::WG; \
+                if (!inverse) PDH3_LAUNCH("k3_col_fwd_reg", s, (k3_col_fwd_reg<LL, T>), dim3(njobs * ops * tiles), \
+                                          dim3(WG), (size_t)3 * words * 4, s, d_jobs, one, ops, n3, ot.tw3, PS); \
+                else PDH3_LAUNCH("k3_col_inv_reg", s, (k3_col_inv_reg<LL, T>), dim3(njobs * tiles), dim3(WG), \
+                                 (size_t)3 * words * 4, s, d_jobs, one, n3, ot.tw3, PS, nblk_max); \
+                done = true; \
+            } \
+            break;
+        PDH3_FOR_COL_L1(PDH3_COL_CASE)
+#undef PDH3_COL_CASE
+        default: break;
+        }
+        if (done) return inverse ? launch_carry_scan(T, d_jobs, one, njobs, nblk_max, s) : hipSuccess;
+    }
     if (!inverse) {
         PDH3_LAUNCH("k3_col_fwd", s, k3_col_fwd<T>, dim3(njobs * ops * tiles), dim3(1024), (size_t)3 * words * 4, s,
                     d_jobs, one, ops, n3, sh.L1, ot.tw3, PS);

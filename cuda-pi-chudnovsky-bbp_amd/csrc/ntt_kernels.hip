@@ -597,10 +597,11 @@ static NttShape default_shape_eng(int n, int eng) {
     }
     // three primes over 2^(n-1) points: {L1, T} for n = 14..24, measured on
     // MI355X with the shape sweep (rows of 2^13..2^14 points at the top);
-    // rows as register-resident radix-16 sets
+    // lr_row 4: rows as register-resident radix-16 sets and, where the tile
+    // has a register kernel, columns as register-resident radix-8 sets
     static const int tab3[MAX_LOG + 1][2] = {
         {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0},
-        {4, 8}, {5, 8}, {6, 8}, {6, 8}, {6, 8}, {6, 16}, {7, 16}, {8, 16}, {8, 16}, {9, 8}, {9, 8}};
+        {4, 8}, {5, 8}, {6, 8}, {6, 8}, {5, 16}, {6, 16}, {6, 16}, {8, 16}, {8, 16}, {8, 16}, {9, 16}};
     s.eng = 1;
     s.lr_row = 4;
     s.L1 = tab3[n][0];
@@ -631,8 +632,10 @@ bool shape_valid(const NttShape& s) {
     if (s.eng == 0)
         return s.L2 <= ROW_MAX_LOG && s.L1 + s.L2 == s.n && (size_t)lds_words((1 << s.L1) * s.T) * 8 <= 147456;
     // three primes: u32 rows of up to 2^14 points (two arrays fill the 144 KB);
-    // radix-8 columns; rows either radix-8 levels through LDS (lr_row 3) or
-    // register-resident radix-16 sets (lr_row 4, rows of 2^8 points or more)
+    // radix-8 columns. lr_row 3: both passes as radix-8 levels through LDS;
+    // lr_row 4 (rows of 2^8 points or more): register-resident passes, rows
+    // of radix-16 sets and columns of radix-8 sets (tiles of 2^9 to 2^13
+    // elements; other tiles keep the LDS-level column pass)
     if (s.eng != 1 || s.lr_col != 3 || s.T > 32 || s.L2 > ROW_MAX_LOG + 1) return false;
     if (s.lr_row == 4 && s.L2 < 8) return false;
     return s.L1 + s.L2 == s.n - 1 && (size_t)3 * lds_words((1 << s.L1) * s.T) * 4 <= 147456;
