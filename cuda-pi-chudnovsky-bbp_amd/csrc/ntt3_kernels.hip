@@ -872,7 +872,8 @@ __global__ __launch_bounds__((ColPlan<L1, T>::WG)) void k3_col_inv_reg(const Mul
         }
     }
     __syncthreads();
-    // chunk = the T coefficients (= limbs) of one row segment, carry-in 0
+    // chunk = the T coefficients (= limbs) of one row segment, carry-in 0;
+    // the chunk's limbs go back to array 0 in place
     const CarryJob C = carry_job<2 * T>(J, nblk_max);
     for (int i = threadIdx.x; i < R; i += blockDim.x) {
         const size_t k0 = (size_t)i * S + b0;
@@ -884,10 +885,19 @@ __global__ __launch_bounds__((ColPlan<L1, T>::WG)) void k3_col_inv_reg(const Mul
             const int e = lpad(i * T + q);
             const u64 up = (u64)l[1][e] | ((u64)l[2][e] << 32);
             const u64 s = (carry & 0xFFFFFFFFull) + l[0][e];
-            if (k0 + q < C.nlimbs) C.out[k0 + q] = (u32)s;
+            l[0][e] = (u32)s;
             carry = (s >> 32) + (carry >> 32) + up;
         }
         C.C[ch] = carry;
+    }
+    __syncthreads();
+    // limbs out with consecutive lanes on consecutive columns of a row
+    // (T-limb runs), not one row per lane
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int e = k * CP::WG + threadIdx.x;
+        const size_t g = (size_t)(e / T) * S + b0 + (e & (T - 1));
+        if (g < C.nlimbs) C.out[g] = l[0][lpad(e)];
     }
 }
 
