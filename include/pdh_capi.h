@@ -116,9 +116,12 @@ int pdh_mul_batch_dev(const pdh_mul_desc* descs, size_t n, void* stream);
  * A 2^n-point transform (n > 13) runs as a column pass over the top L1 index
  * bits (T adjacent columns per workgroup, radix 2^lr_col butterflies) and a
  * fused row pass over the low n - L1 bits (radix 2^lr_row); n <= 13 runs in
- * one workgroup (radix 2^lr_row). Every valid shape gives the same product;
- * only speed differs. No counterpart in the reference (its radix-2 transform
- * has no parameters, src/ntt_cuda.cu:136-159). */
+ * one workgroup (radix 2^lr_row). On the three-prime engine lr_col is 3 and
+ * lr_row selects the kernels: 4 (default, rows of 2^8 points or more)
+ * register-resident radix-16 rows and radix-8 columns, 3 radix-8 levels
+ * through LDS. Every valid shape gives the same product; only speed differs.
+ * No counterpart in the reference (its radix-2 transform has no parameters,
+ * src/ntt_cuda.cu:136-159). */
 int pdh_tune_get(int n, int* L1, int* T, int* lr_col, int* lr_row);
 /* PDH_E_ARG if the shape cannot be launched. */
 int pdh_tune_set(int n, int L1, int T, int lr_col, int lr_row);
