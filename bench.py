@@ -19,7 +19,9 @@ The JSON line carries:
   cpu_baseline        the reference's ntt-cpu (oracle/_ref, built from the
                       reference sources) or its restatement, single thread,
                       on one 2^22-limb multiply
-  extra               the 60,000-limb multiply and BBP positions 1..65536
+  extra               the C2 square (B_sqr = 64 N + 12 na over its time), the
+                      60,000-limb multiply, batches, BBP, Chudnovsky and the
+                      serving trace
 
     python bench.py [--steps K] [--warmup W] [--no-cpu-baseline]
 """
@@ -335,6 +337,28 @@ def main() -> int:
 
     extra = {}
     if not args.no_extra:
+        # the C2 square: the same operand size, one transform fewer
+        gs = golden["c2_sqr"]
+        asq = W.operand(gs["seed_a"], gs["limbs"])
+        dsq = torch.from_numpy(asq.view(np.int32)).to(dev)
+        osq = torch.zeros(2 * len(asq), dtype=torch.int32, device=dev)
+        for _ in range(max(1, args.warmup)):
+            pdh.sqr_dev(dsq.data_ptr(), len(asq), osq.data_ptr(), ln.data_ptr(), sp)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(args.steps):
+            pdh.sqr_dev(dsq.data_ptr(), len(asq), osq.data_ptr(), ln.data_ptr(), sp)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms_sq = e0.elapsed_time(e1) / args.steps
+        nsq = int(ln.item())
+        gsq = osq[:nsq].cpu().numpy().view(np.uint32)
+        B_sqr = 64 * N + 12 * len(asq)
+        extra["c2_square"] = {"ms": ms_sq, "sqr_per_s": 1e3 / ms_sq, "alg_bytes": B_sqr,
+                              "roofline_frac": B_sqr / (ms_sq * 1e-3) / HBM_PEAK,
+                              "bit_exact": nsq == gs["len"] and hashlib.sha256(
+                                  gsq.astype("<u4").tobytes()).hexdigest() == gs["sha256"]}
+        del dsq, osq
         # 60,000-limb multiply (the reference's CPU-runnable case), resident
         c1 = golden["c1_mul"]
         a1, b1 = W.operand(c1["seed_a"], c1["limbs"]), W.operand(c1["seed_b"], c1["limbs"])

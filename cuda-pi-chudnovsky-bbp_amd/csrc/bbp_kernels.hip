@@ -13,7 +13,8 @@
 //  none of those operations is fused.
 //
 //  The integer r = 16^(d-k) mod m is unique, so it may be computed by any
-//  exact method. Here it is left-to-right square-and-multiply on FP64 values
+//  exact method: below m = 2^30 by integer Montgomery arithmetic (see
+//  modpow16_mont), above by left-to-right square-and-multiply on FP64 values
 //  (v_fma_f64 issues at full rate on gfx950, 64-bit integer modulo does not
 //  exist in hardware): for |r| < 16m, m < 2^31,
 //      p = r*r (rounded), e = fma(r, r, -p) (exact error, r*r = p + e),
@@ -67,13 +68,56 @@ __device__ __forceinline__ double modpow16_exact(uint64_t e, double m, double mi
     return r;
 }
 
+// The same residue by integer Montgomery arithmetic (R = 2^32) for 1 < m <
+// 2^30: m = 2^s mo with mo odd (s <= 2 for m = 8k + j, j in {1, 4, 5, 6}),
+// and for e >= 1, 16^e = 2^s 2^E with E = 4e - s, so 16^e mod m = 2^s (2^E mod
+// mo). 2^E mod mo runs a left-to-right square-and-double ladder on lazy
+// residues in [0, 2 mo): a square r^2 < 4 mo^2 < 2^32 mo reduces to [0, 2 mo)
+// without a final correction, a double goes to [0, 4 mo) and one unsigned
+// min brings it back. Seven VALU per exponent bit against fifteen for the
+// FP64 ladder. minv = 1/m (double) gives the Montgomery one, 2^32 mod mo.
+__device__ __forceinline__ uint32_t modpow16_mont(uint64_t e, uint32_t m, double minv) {
+    if (e == 0) return 1u;
+    const int s = __builtin_ctz(m);
+    const uint32_t mo = m >> s;
+    const uint32_t E = (uint32_t)(4 * e) - (uint32_t)s;
+    uint32_t inv = mo;                                  // mo * mo = 1 mod 8
+    for (int i = 0; i < 4; ++i) inv *= 2u - mo * inv;   // 3 -> 48 correct bits
+    const uint32_t ninv = 0u - inv;
+    // 2^32 mod mo from a floor estimate of 2^32 / mo (within one)
+    const uint64_t q = (uint64_t)(4294967296.0 * (minv * (double)(1u << s)));
+    int64_t rr = (int64_t)(((uint64_t)1 << 32) - q * mo);
+    if (rr < 0) rr += mo;
+    if (rr >= (int64_t)mo) rr -= mo;
+    uint32_t r = (uint32_t)rr;
+    const uint32_t two_mo = mo << 1;
+    for (int b = 31 - __clz(E); b >= 0; --b) {
+        const uint64_t t = (uint64_t)r * r;
+        const uint32_t mq = (uint32_t)t * ninv;
+        r = (uint32_t)(((uint64_t)mq * mo + t) >> 32);
+        r <<= (E >> b) & 1u;
+        const uint32_t d = r - two_mo;
+        r = d < r ? d : r;
+    }
+    // out of Montgomery form: in [0, mo], and mo only when mo = 1
+    const uint32_t mq = r * ninv;
+    uint32_t y = (uint32_t)(((uint64_t)mq * mo + r) >> 32);
+    if (y >= mo) y -= mo;
+    return y << s;
+}
+
+// 16^e mod m for 1 < m < 2^31: Montgomery below 2^30, the FP64 ladder above.
+__device__ __forceinline__ double modpow16(uint64_t e, uint64_t mi, double m, double minv) {
+    return mi < ((uint64_t)1 << 30) ? (double)modpow16_mont(e, (uint32_t)mi, minv) : modpow16_exact(e, m, minv);
+}
+
 // {16^d S_j}: bbp_kernel.h:99-122 with the integer part done exactly above.
 __device__ double bbp_series_dev(uint32_t j, uint64_t d) {
     double sum = 0.0;
     for (uint64_t k = 0; k <= d; ++k) {
         const uint64_t mi = 8ull * k + j;
         const double m = (double)mi;
-        const double r = (mi == 1) ? 0.0 : modpow16_exact(d - k, m, recip_f64(m));
+        const double r = (mi == 1) ? 0.0 : modpow16(d - k, mi, m, recip_f64(m));
         sum += r / m;
         sum -= floor(sum);
     }
@@ -201,7 +245,7 @@ __global__ __launch_bounds__(256) void k_bbp_fx_terms(const BbpSeries* __restric
         if (mi == 1) continue;                        // 16^d mod 1 = 0
         const double m = (double)mi;
         const double minv = recip_f64(m);
-        const double r = modpow16_exact(S.d - k, m, minv);
+        const double r = modpow16(S.d - k, mi, m, minv);
         local += frac64((uint32_t)r, (uint32_t)mi, minv);
     }
     for (int o = 32; o > 0; o >>= 1) local += __shfl_xor(local, o);
@@ -245,7 +289,7 @@ __global__ __launch_bounds__(256) void k_bbp_terms(const BbpSeries* __restrict__
     if (!S.valid || k > S.d) return;
     const uint64_t mi = 8ull * k + S.j;
     const double m = (double)mi;
-    const double r = (mi == 1) ? 0.0 : modpow16_exact(S.d - k, m, recip_f64(m));
+    const double r = (mi == 1) ? 0.0 : modpow16(S.d - k, mi, m, recip_f64(m));
     buf[g] = r / m;
 }
 

@@ -873,22 +873,30 @@ __global__ __launch_bounds__((ColPlan<L1, T>::WG)) void k3_col_inv_reg(const Mul
     }
     __syncthreads();
     // chunk = the T coefficients (= limbs) of one row segment, carry-in 0;
-    // the chunk's limbs go back to array 0 in place
+    // the chunk's limbs go back to array 0 in place. With the carry-out C the
+    // chunk's threshold goes out too (2^(32T) - D when limbs 2..T-1 of D are
+    // all ones, else never; limbs at or above na + nb count as zero), so the
+    // carry scan reads (TH, C) and not the digits.
     const CarryJob C = carry_job<2 * T>(J, nblk_max);
     for (int i = threadIdx.x; i < R; i += blockDim.x) {
         const size_t k0 = (size_t)i * S + b0;
         const size_t ch = k0 / T;
         if (ch >= C.nchunks) continue;
-        u64 carry = 0;
+        u64 carry = 0, lo = 0;
+        bool ones = true;
 #pragma unroll
         for (int q = 0; q < T; ++q) {
             const int e = lpad(i * T + q);
             const u64 up = (u64)l[1][e] | ((u64)l[2][e] << 32);
             const u64 s = (carry & 0xFFFFFFFFull) + l[0][e];
             l[0][e] = (u32)s;
+            const bool in = k0 + q < C.nlimbs;
+            if (q < 2) lo |= (in ? (u64)(u32)s : 0) << (32 * q);
+            else ones &= in && (u32)s == 0xFFFFFFFFu;
             carry = (s >> 32) + (carry >> 32) + up;
         }
         C.C[ch] = carry;
+        C.TH[ch] = (ones && lo != 0) ? (u64)(0 - lo) : ~(u64)0;
     }
     __syncthreads();
     // limbs out with consecutive lanes on consecutive columns of a row
@@ -979,7 +987,7 @@ static hipError_t launch_cols3(const MulJob* d_jobs, const MulJob& one, unsigned
 #undef PDH3_COL_CASE
         default: break;
         }
-        if (done) return inverse ? launch_carry_scan(T, d_jobs, one, njobs, nblk_max, s) : hipSuccess;
+        if (done) return inverse ? launch_carry_scan(T, d_jobs, one, njobs, nblk_max, s, true) : hipSuccess;
     }
     if (!inverse) {
         PDH3_LAUNCH("k3_col_fwd", s, k3_col_fwd<T>, dim3(njobs * ops * tiles), dim3(1024), (size_t)3 * words * 4, s,

@@ -127,11 +127,12 @@ __device__ __forceinline__ u64 wave_max(u64 v) {
 // chunk's carry-in to D in place. Limbs at or above na + nb are zero in D
 // (the product has na + nb limbs), so they are neither stored nor read.
 //
-// Scratch per job: C[ceil(nl/2)] (enough for T >= 4), then agg[nblk_max]
+// Scratch per job: C[ceil(nl/2)] (enough for T >= 4), then TH[ceil(nl/2)]
+// (the chunk thresholds, when the producer writes them), then agg[nblk_max]
 // CarryFn, then cin[nblk_max].
 struct CarryJob {
     u32* out; size_t nlimbs, nchunks, nblk;
-    u64* C; CarryFn* agg; u64* cin;
+    u64* C; u64* TH; CarryFn* agg; u64* cin;
 };
 template <int T>
 __device__ __forceinline__ CarryJob carry_job(const MulJob& J, size_t nblk_max) {
@@ -141,8 +142,10 @@ __device__ __forceinline__ CarryJob carry_job(const MulJob& J, size_t nblk_max) 
     c.nlimbs = J.na + J.nb;
     c.nchunks = (2 * c.nlimbs + T - 1) / T;
     c.nblk = (c.nchunks + CARRY_NT * G::CPT - 1) / (CARRY_NT * G::CPT);
+    const size_t cap = ((c.nlimbs + 1) / 2 + 3) / 4 * 4;
     c.C = (u64*)J.carry;
-    c.agg = (CarryFn*)(c.C + ((c.nlimbs + 1) / 2 + 3) / 4 * 4);
+    c.TH = c.C + cap;
+    c.agg = (CarryFn*)(c.TH + cap);
     c.cin = (u64*)(c.agg + nblk_max);
     return c;
 }

@@ -327,6 +327,40 @@ __global__ __launch_bounds__(CARRY_NT) void k_carry_reduce(const MulJob* __restr
     if (blk == 0 && threadIdx.x == 0) *J.out_len = 0;
 }
 
+// The same reduce when the producer wrote the chunk thresholds: a chunk's
+// function is (TH, C, C + 1), 16 bytes per chunk and no digits. (Folding the
+// aggregate scan into the last workgroup to finish needs device-scope
+// fences, which write back the L2 of each XCD; a separate launch is cheaper.)
+template <int T>
+__global__ __launch_bounds__(CARRY_NT) void k_carry_reduce_th(const MulJob* __restrict__ jobs, MulJob one,
+                                                              size_t nblk_max) {
+    using G = CarryGeom<T>;
+    constexpr int CPT = G::CPT;
+    __shared__ CarryFn wsum[CARRY_NT / 64];
+    const MulJob J = jobs ? jobs[blockIdx.x / nblk_max] : one;
+    const CarryJob C = carry_job<T>(J, nblk_max);
+    const size_t blk = blockIdx.x % nblk_max;
+    if (blk >= C.nblk) return;                       // whole workgroup leaves together
+    const size_t ch0 = (blk * CARRY_NT + threadIdx.x) * CPT;
+    CarryFn f = cf_pad();
+    bool first = true;
+#pragma unroll
+    for (int u = 0; u < CPT; ++u) {
+        const size_t ch = ch0 + u;
+        if (ch < C.nchunks) {
+            const u64 c = C.C[ch];
+            const CarryFn g{C.TH[ch], c, c + 1};
+            f = first ? g : cf_compose(g, f);
+            first = false;
+        } else if (!first) {
+            f = cf_compose(cf_pad(), f);
+        }
+    }
+    f = block_scan(f, wsum);
+    if (threadIdx.x == CARRY_NT - 1) C.agg[blk] = f;
+    if (blk == 0 && threadIdx.x == 0) *J.out_len = 0;
+}
+
 // One workgroup of 1024 threads per job: carry-in of every carry block, in tiles.
 template <int T>
 __global__ __launch_bounds__(1024) void k_carry_blocks(const MulJob* __restrict__ jobs, MulJob one,
@@ -728,9 +762,10 @@ static hipError_t launch_cols_T(const MulJob* d_jobs, const MulJob& one, unsigne
 
 template <int T>
 static hipError_t launch_carry_scan_T(const MulJob* d_jobs, const MulJob& one, unsigned njobs, size_t nblk_max,
-                                      hipStream_t s) {
+                                      hipStream_t s, bool th) {
     const dim3 cgrid((unsigned)(njobs * nblk_max));
-    PDH_LAUNCH("k_carry_reduce", s, k_carry_reduce<T>, cgrid, dim3(CARRY_NT), 0, s, d_jobs, one, nblk_max);
+    if (th) PDH_LAUNCH("k_carry_reduce_th", s, k_carry_reduce_th<T>, cgrid, dim3(CARRY_NT), 0, s, d_jobs, one, nblk_max);
+    else PDH_LAUNCH("k_carry_reduce", s, k_carry_reduce<T>, cgrid, dim3(CARRY_NT), 0, s, d_jobs, one, nblk_max);
     PDH_LAUNCH("k_carry_blocks", s, k_carry_blocks<T>, dim3(njobs), dim3(1024), 0, s, d_jobs, one, nblk_max);
     PDH_LAUNCH("k_carry_apply", s, k_carry_apply<T>, cgrid, dim3(CARRY_NT), 0, s, d_jobs, one, nblk_max);
     return hipSuccess;
@@ -738,13 +773,13 @@ static hipError_t launch_carry_scan_T(const MulJob* d_jobs, const MulJob& one, u
 
 // chunks of kl limbs = 2 kl 16-bit coefficients
 hipError_t launch_carry_scan(int kl, const MulJob* d_jobs, const MulJob& one, unsigned njobs, size_t nblk_max,
-                             hipStream_t s) {
+                             hipStream_t s, bool th) {
     switch (kl) {
-    case 2: return launch_carry_scan_T<4>(d_jobs, one, njobs, nblk_max, s);
-    case 4: return launch_carry_scan_T<8>(d_jobs, one, njobs, nblk_max, s);
-    case 8: return launch_carry_scan_T<16>(d_jobs, one, njobs, nblk_max, s);
-    case 16: return launch_carry_scan_T<32>(d_jobs, one, njobs, nblk_max, s);
-    case 32: return launch_carry_scan_T<64>(d_jobs, one, njobs, nblk_max, s);
+    case 2: return launch_carry_scan_T<4>(d_jobs, one, njobs, nblk_max, s, th);
+    case 4: return launch_carry_scan_T<8>(d_jobs, one, njobs, nblk_max, s, th);
+    case 8: return launch_carry_scan_T<16>(d_jobs, one, njobs, nblk_max, s, th);
+    case 16: return launch_carry_scan_T<32>(d_jobs, one, njobs, nblk_max, s, th);
+    case 32: return launch_carry_scan_T<64>(d_jobs, one, njobs, nblk_max, s, th);
     default: return hipErrorInvalidValue;
     }
 }
@@ -761,7 +796,7 @@ size_t carry_nblk(size_t nlimbs) {
 
 size_t carry_ws_bytes(size_t nlimbs) {
     const size_t nblk = carry_nblk(nlimbs);
-    return ((nlimbs + 1) / 2 + 3) / 4 * 4 * sizeof(u64) + nblk * sizeof(CarryFn) + nblk * sizeof(u64) + 256;
+    return 2 * (((nlimbs + 1) / 2 + 3) / 4 * 4) * sizeof(u64) + nblk * sizeof(CarryFn) + nblk * sizeof(u64) + 256;
 }
 
 template <bool SQ, int NT>

@@ -97,9 +97,10 @@ hipError_t init_kernel_attrs3();
 hipError_t launch_class3(const MulJob* d_jobs, const MulJob& one, unsigned njobs, const NttShape& sh, bool square,
                          size_t nblk_max, const OuterTables& ot, hipStream_t s);
 // The carry scan over chunks of `kl` limbs (digits and carry-outs already in
-// the product buffer and the carry scratch).
+// the product buffer and the carry scratch; with `th`, also the chunk
+// thresholds, so the reduce reads no digits).
 hipError_t launch_carry_scan(int kl, const MulJob* d_jobs, const MulJob& one, unsigned njobs, size_t nblk_max,
-                             hipStream_t s);
+                             hipStream_t s, bool th = false);
 // Every job in a launch shares n; jobs == nullptr launches the single job `one`.
 hipError_t launch_class(const MulJob* d_jobs, const MulJob& one, unsigned njobs, int n, bool square,
                         size_t nblk_max, const uint64_t* twL, const OuterTables& ot, hipStream_t s,
