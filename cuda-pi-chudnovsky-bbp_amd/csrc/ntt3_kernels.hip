@@ -540,41 +540,38 @@ __device__ __forceinline__ void set_step(u32* x, const u32* w, const Prime3& P) 
 // Forward of NOP operands (1 or 2, arrays l[0], l[1]): x holds set 0 on entry,
 // the spectrum in the last set's positions on exit.
 template <int L2, int NOP, int S = 0>
-__device__ __forceinline__ void row_fwd(u32 (*x)[16], int u, u32* const* l, const u32* __restrict__ tw,
+__device__ __forceinline__ void row_fwd(u32 (*x)[16], const u32* w, int u, u32* const* l, const u32* __restrict__ tw,
                                         const Prime3& P) {
     using RP = RowPlan<L2>;
-    {
-        u32 w[16];
-        set_tw<L2, S, false>(w, u, tw);
 #pragma unroll
-        for (int k = 0; k < NOP; ++k) set_step<L2, S, false>(x[k], w, P);
-    }
+    for (int k = 0; k < NOP; ++k) set_step<L2, S, false>(x[k], w, P);
     if constexpr (S + 1 < RP::NSET) {
+        // the next set's twiddles are in flight during the exchange
+        u32 wn[16];
+        set_tw<L2, S + 1, false>(wn, u, tw);
         const int a0 = set_base<RP::bottom(S), 4, 0>(u, 0), a1 = set_base<RP::bottom(S + 1), 4, 0>(u, 0);
 #pragma unroll
         for (int k = 0; k < NOP; ++k) set_store<RP::bottom(S), 4, 0>(l[k], a0, x[k]);
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < NOP; ++k) set_load<RP::bottom(S + 1), 4, 0>(l[k], a1, x[k]);
-        row_fwd<L2, NOP, S + 1>(x, u, l, tw, P);
+        row_fwd<L2, NOP, S + 1>(x, wn, u, l, tw, P);
     }
 }
 
 // Inverse: x holds the last set on entry, set 0 (natural order) on exit.
 template <int L2, int S = RowPlan<L2>::NSET - 1>
-__device__ __forceinline__ void row_inv(u32* x, int u, u32* __restrict__ l, const u32* __restrict__ tw,
-                                        const Prime3& P) {
+__device__ __forceinline__ void row_inv(u32* x, const u32* w, int u, u32* __restrict__ l,
+                                        const u32* __restrict__ tw, const Prime3& P) {
     using RP = RowPlan<L2>;
-    {
-        u32 w[16];
-        set_tw<L2, S, true>(w, u, tw);
-        set_step<L2, S, true>(x, w, P);
-    }
+    set_step<L2, S, true>(x, w, P);
     if constexpr (S > 0) {
+        u32 wn[16];
+        set_tw<L2, S - 1, true>(wn, u, tw);
         set_store<RP::bottom(S), 4, 0>(l, set_base<RP::bottom(S), 4, 0>(u, 0), x);
         __syncthreads();
         set_load<RP::bottom(S - 1), 4, 0>(l, set_base<RP::bottom(S - 1), 4, 0>(u, 0), x);
-        row_inv<L2, S - 1>(x, u, l, tw, P);
+        row_inv<L2, S - 1>(x, wn, u, l, tw, P);
     }
 }
 
@@ -624,11 +621,19 @@ __global__ __launch_bounds__(RowPlan<L2>::WG) void k3_row_reg(const MulJob* __re
             w = mml(w, step, P.p, P.ninv);
         }
     }
-    row_fwd<L2, NOP>(x, u, l, tw, P);
+    {
+        u32 w[16];
+        set_tw<L2, 0, false>(w, u, tw);
+        row_fwd<L2, NOP>(x, w, u, l, tw, P);
+    }
 #pragma unroll
     for (int t = 0; t < 16; ++t) x[0][t] = mml(x[0][t], x[NOP - 1][t], P.p, P.ninv);
     // the inverse's first exchange writes the positions this thread read last
-    row_inv<L2>(x[0], u, l[0], tw, P);
+    {
+        u32 w[16];
+        set_tw<L2, RP::NSET - 1, true>(w, u, tw);
+        row_inv<L2>(x[0], w, u, l[0], tw, P);
+    }
     u32 w = r_tw(hi_inv, lo_inv, h, (c * (u64)u) & (N - 1), P);
     const u32 step = r_tw(hi, lo, h, (N - c * (u64)TPR) & (N - 1), P);
 #pragma unroll
