@@ -122,9 +122,11 @@ __device__ __forceinline__ u64 wave_max(u64 v) {
 // into the product buffer) and its carry-out C into the carry scratch; the
 // coefficients themselves never return to HBM. D and C determine the chunk's
 // carry function (thr from D's limbs, v0 = C, v1 = C + 1), so two scans over
-// (D, C) finish the carry: k_carry_reduce composes each workgroup's chunks,
-// k_carry_blocks scans the workgroup aggregates, k_carry_apply adds each
-// chunk's carry-in to D in place. Limbs at or above na + nb are zero in D
+// (D, C) finish the carry: k_carry_reduce composes each workgroup's chunks
+// (k_carry_reduce_th reads the thresholds TH instead of D when the producer
+// wrote them), k_carry_blocks scans the workgroup aggregates (or, for jobs of
+// at most CARRY_NT workgroups, k_carry_apply composes them itself), and
+// k_carry_apply adds each chunk's carry-in to D in place. Limbs at or above na + nb are zero in D
 // (the product has na + nb limbs), so they are neither stored nor read.
 //
 // Scratch per job: C[ceil(nl/2)] (enough for T >= 4), then TH[ceil(nl/2)]

@@ -384,7 +384,7 @@ __global__ __launch_bounds__(1024) void k_carry_blocks(const MulJob* __restrict_
 
 template <int T>
 __global__ __launch_bounds__(CARRY_NT) void k_carry_apply(const MulJob* __restrict__ jobs, MulJob one,
-                                                          size_t nblk_max) {
+                                                          size_t nblk_max, bool inline_cin) {
     using G = CarryGeom<T>;
     constexpr int KL = G::KL, CPT = G::CPT;
     __shared__ CarryFn wsum[CARRY_NT / 64];
@@ -421,7 +421,19 @@ __global__ __launch_bounds__(CARRY_NT) void k_carry_apply(const MulJob* __restri
             f = cf_compose(cf_pad(), f);
         }
     }
-    const u64 bc = C.cin[blk];
+    // this workgroup's carry-in: from k_carry_blocks, or, when the job has at
+    // most CARRY_NT workgroups (inline_cin), by composing the aggregates of
+    // workgroups 0..blk-1 here (one scan; its entry blk-1 is their composition)
+    u64 bc = 0;
+    if (!inline_cin) {
+        bc = C.cin[blk];
+    } else if (blk > 0) {
+        const CarryFn g = block_scan(threadIdx.x < blk ? C.agg[threadIdx.x] : cf_pad(), wsum);
+        if (threadIdx.x == blk - 1) outs[0] = cf_apply(g, 0);
+        __syncthreads();
+        bc = outs[0];
+        __syncthreads();
+    }
     CarryFn inc = block_scan(f, wsum);
     outs[threadIdx.x] = cf_apply(inc, bc);
     __syncthreads();
@@ -766,8 +778,11 @@ static hipError_t launch_carry_scan_T(const MulJob* d_jobs, const MulJob& one, u
     const dim3 cgrid((unsigned)(njobs * nblk_max));
     if (th) PDH_LAUNCH("k_carry_reduce_th", s, k_carry_reduce_th<T>, cgrid, dim3(CARRY_NT), 0, s, d_jobs, one, nblk_max);
     else PDH_LAUNCH("k_carry_reduce", s, k_carry_reduce<T>, cgrid, dim3(CARRY_NT), 0, s, d_jobs, one, nblk_max);
-    PDH_LAUNCH("k_carry_blocks", s, k_carry_blocks<T>, dim3(njobs), dim3(1024), 0, s, d_jobs, one, nblk_max);
-    PDH_LAUNCH("k_carry_apply", s, k_carry_apply<T>, cgrid, dim3(CARRY_NT), 0, s, d_jobs, one, nblk_max);
+    // a job of at most CARRY_NT carry workgroups needs no separate aggregate scan
+    const bool inline_cin = nblk_max <= (size_t)CARRY_NT;
+    if (!inline_cin)
+        PDH_LAUNCH("k_carry_blocks", s, k_carry_blocks<T>, dim3(njobs), dim3(1024), 0, s, d_jobs, one, nblk_max);
+    PDH_LAUNCH("k_carry_apply", s, k_carry_apply<T>, cgrid, dim3(CARRY_NT), 0, s, d_jobs, one, nblk_max, inline_cin);
     return hipSuccess;
 }
 
