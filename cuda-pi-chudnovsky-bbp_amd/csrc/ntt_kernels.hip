@@ -361,24 +361,38 @@ __global__ __launch_bounds__(CARRY_NT) void k_carry_reduce_th(const MulJob* __re
     if (blk == 0 && threadIdx.x == 0) *J.out_len = 0;
 }
 
-// One workgroup of 1024 threads per job: carry-in of every carry block, in tiles.
+// One workgroup of 1024 threads per job: carry-in of every carry block.
 template <int T>
 __global__ __launch_bounds__(1024) void k_carry_blocks(const MulJob* __restrict__ jobs, MulJob one,
                                                        size_t nblk_max) {
     __shared__ CarryFn wsum[1024 / 64];
-    __shared__ u64 tile_out[1024];
+    __shared__ u64 seg_out[1024];
     const MulJob J = jobs ? jobs[blockIdx.x] : one;
     const CarryJob C = carry_job<T>(J, nblk_max);
-    u64 carry = 0;
-    for (size_t t0 = 0; t0 < C.nblk; t0 += 1024) {
-        const size_t w = t0 + threadIdx.x;
-        CarryFn f = (w < C.nblk) ? C.agg[w] : cf_pad();
-        f = block_scan(f, wsum);
-        tile_out[threadIdx.x] = cf_apply(f, carry);
-        __syncthreads();
-        if (w < C.nblk) C.cin[w] = threadIdx.x == 0 ? carry : tile_out[threadIdx.x - 1];
-        carry = tile_out[1023];
-        __syncthreads();
+    // thread t takes the K contiguous aggregates [t K, t K + K): one scan over
+    // the segments' compositions, then each thread walks its own segment
+    const size_t K = (C.nblk + 1023) / 1024;
+    const size_t w0 = threadIdx.x * K;
+    CarryFn f = cf_pad();
+    bool first = true;
+    for (size_t k = 0; k < K; ++k) {
+        const size_t w = w0 + k;
+        if (w < C.nblk) {
+            f = first ? C.agg[w] : cf_compose(C.agg[w], f);
+            first = false;
+        } else if (!first) {
+            f = cf_compose(cf_pad(), f);
+        }
+    }
+    f = block_scan(f, wsum);
+    seg_out[threadIdx.x] = cf_apply(f, 0);
+    __syncthreads();
+    u64 c = threadIdx.x == 0 ? 0 : seg_out[threadIdx.x - 1];
+    for (size_t k = 0; k < K; ++k) {
+        const size_t w = w0 + k;
+        if (w >= C.nblk) break;
+        C.cin[w] = c;
+        c = cf_apply(C.agg[w], c);
     }
 }
 
