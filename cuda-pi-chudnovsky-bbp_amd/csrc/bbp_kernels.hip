@@ -28,6 +28,7 @@
 //  the four series through cross-lane shuffles in the reference's order.
 // ============================================================================
 #include "pdh_internal.h"
+#include <algorithm>
 #include <cmath>
 
 namespace pdh {
@@ -133,31 +134,43 @@ __device__ double bbp_series_dev(uint32_t j, uint64_t d) {
     return sum;
 }
 
+// A work queue of groups of 16 positions (one wavefront: 4 lanes each),
+// handed out longest first (positions are ascending, so from the last group
+// down): series lengths d + 1 vary widely, and a static split leaves some
+// SIMDs with much more work than others. Every wavefront loops until the
+// queue (a counter zeroed before the launch) runs past the last group.
 __global__ __launch_bounds__(256) void k_bbp(const uint64_t* __restrict__ pos, uint64_t start,
-                                             size_t count, uint8_t* __restrict__ out) {
-    const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const size_t pi = gid >> 2;
-    const int q = (int)(gid & 3);
-    const bool active = pi < count;
-    uint64_t position = active ? (pos ? pos[pi] : start + pi) : 1;
-    // Positions outside [1, 2^28] (possible only through the device entry
-    // point) are not computed: their value is 0xFF.
-    const bool valid = position >= 1 && position <= ((uint64_t)1 << 28);
-    if (!valid) position = 1;
-    const uint32_t j = q == 0 ? 1u : (uint32_t)(q + 3);   // 1, 4, 5, 6
-    const double S = bbp_series_dev(j, position - 1);
+                                             size_t count, uint8_t* __restrict__ out,
+                                             unsigned long long* __restrict__ next) {
+    const size_t ngroups = (count + 15) / 16;
     const int lane = threadIdx.x & 63;
+    const int q = lane & 3;
     const int g = lane & ~3;
-    const double s1 = __shfl(S, g + 0);
-    const double s4 = __shfl(S, g + 1);
-    const double s5 = __shfl(S, g + 2);
-    const double s6 = __shfl(S, g + 3);
-    if (active && q == 0 && !valid) out[pi] = 0xFF;
-    if (active && q == 0 && valid) {
-        double s = 4.0 * s1 - 2.0 * s4 - 1.0 * s5 - 1.0 * s6;
-        s -= floor(s);
-        if (s < 0.0) s += 1.0;
-        out[pi] = (uint8_t)(int)(16.0 * s);
+    for (;;) {
+        unsigned long long idx = 0;
+        if (lane == 0) idx = atomicAdd(next, 1ull);
+        idx = __shfl(idx, 0);
+        if (idx >= ngroups) break;                       // the same on every lane
+        const size_t pi = (ngroups - 1 - idx) * 16 + (size_t)(lane >> 2);
+        const bool active = pi < count;
+        uint64_t position = active ? (pos ? pos[pi] : start + pi) : 1;
+        // Positions outside [1, 2^28] (possible only through the device entry
+        // point) are not computed: their value is 0xFF.
+        const bool valid = position >= 1 && position <= ((uint64_t)1 << 28);
+        if (!valid) position = 1;
+        const uint32_t j = q == 0 ? 1u : (uint32_t)(q + 3);   // 1, 4, 5, 6
+        const double S = bbp_series_dev(j, position - 1);
+        const double s1 = __shfl(S, g + 0);
+        const double s4 = __shfl(S, g + 1);
+        const double s5 = __shfl(S, g + 2);
+        const double s6 = __shfl(S, g + 3);
+        if (active && q == 0 && !valid) out[pi] = 0xFF;
+        if (active && q == 0 && valid) {
+            double s = 4.0 * s1 - 2.0 * s4 - 1.0 * s5 - 1.0 * s6;
+            s -= floor(s);
+            if (s < 0.0) s += 1.0;
+            out[pi] = (uint8_t)(int)(16.0 * s);
+        }
     }
 }
 
@@ -419,13 +432,16 @@ hipError_t launch_bbp_dmax(const uint64_t* d_pos, size_t count, uint64_t* d_dmax
     return hipSuccess;
 }
 
-hipError_t launch_bbp(const uint64_t* d_pos, uint64_t start, size_t count, uint8_t* d_out,
+hipError_t launch_bbp(const uint64_t* d_pos, uint64_t start, size_t count, uint8_t* d_out, uint64_t* d_next,
                       hipStream_t s) {
     if (count == 0) return hipSuccess;
-    const size_t lanes = count * 4;
-    const dim3 grid((unsigned)((lanes + 255) / 256));
+    hipError_t ez = launch_zero_len(d_next, s);
+    if (ez != hipSuccess) return ez;
+    // at most as many wavefronts as groups, and no more than the GPU holds
+    const size_t waves = std::min<size_t>((count + 15) / 16, (size_t)256 * 32);
+    const dim3 grid((unsigned)((waves + 3) / 4));
     hipEvent_t t0 = timing_begin(s);
-    hipLaunchKernelGGL(k_bbp, grid, dim3(256), 0, s, d_pos, start, count, d_out);
+    hipLaunchKernelGGL(k_bbp, grid, dim3(256), 0, s, d_pos, start, count, d_out, (unsigned long long*)d_next);
     g_launches.fetch_add(1, std::memory_order_relaxed);
     hipError_t e = hipGetLastError();
     timing_end("k_bbp", t0, s);

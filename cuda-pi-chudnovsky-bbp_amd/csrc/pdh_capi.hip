@@ -47,6 +47,7 @@ struct Ctx {
     std::string why;
     u64* twL = nullptr;     // 2^14 entries: twL[2^L + x] = w_{2^L}^x
     u32* tw3 = nullptr;     // three primes x 2^15 entries: w_{2^L}^x R mod p
+    u64* bbp_next = nullptr;  // work-queue counter of k_bbp
     std::map<int, Outer> outer;
     u64* wsA = nullptr; u64* wsB = nullptr; size_t ws_words = 0;
     u64* carry = nullptr; size_t carry_bytes = 0;
@@ -121,6 +122,7 @@ int init_locked(Ctx& c) {
             }
         }
     }
+    HIPCALL(hipMalloc(&c.bbp_next, sizeof(u64)));
     HIPCALL(hipMalloc(&c.tw3, t3.size() * sizeof(u32)));
     HIPCALL(hipMemcpy(c.tw3, t3.data(), t3.size() * sizeof(u32), hipMemcpyHostToDevice));
     c.state = 1;
@@ -364,7 +366,7 @@ int bbp_host(const uint64_t* positions, uint64_t start, size_t count, uint8_t* v
                             dmax, c.st_val, tmp.data(), c.stream);
         if (rc) return rc;
     } else {
-        hipError_t e = launch_bbp(positions ? c.st_pos : nullptr, start, count, c.st_val, c.stream);
+        hipError_t e = launch_bbp(positions ? c.st_pos : nullptr, start, count, c.st_val, c.bbp_next, c.stream);
         if (e != hipSuccess) return fail_hip(e, "bbp launch");
         HIPCALL(hipMemcpyAsync(tmp.data(), c.st_val, count, hipMemcpyDeviceToHost, c.stream));
         HIPCALL(hipStreamSynchronize(c.stream));
@@ -493,7 +495,7 @@ int pdh_bbp_positions_dev(const uint64_t* d_positions, size_t n, uint8_t* d_valu
         std::vector<uint8_t> tmp(n);
         return bbp_few_locked(c, d_positions, nullptr, 0, n, dmax, d_values, tmp.data(), st);
     }
-    e = launch_bbp(d_positions, 0, n, d_values, st);
+    e = launch_bbp(d_positions, 0, n, d_values, c.bbp_next, st);
     if (e != hipSuccess) return fail_hip(e, "bbp launch");
     return PDH_OK;
 }

@@ -125,7 +125,7 @@ size_t carry_ws_bytes(size_t nlimbs);
 // FP64 order (odd_undecided, for tests, also marks every odd index
 // undecided). launch_bbp_dmax: atomic max of position - 1 into *d_dmax
 // (zeroed by the caller).
-hipError_t launch_bbp(const uint64_t* d_pos, uint64_t start, size_t count, uint8_t* d_out,
+hipError_t launch_bbp(const uint64_t* d_pos, uint64_t start, size_t count, uint8_t* d_out, uint64_t* d_next,
                       hipStream_t s);
 hipError_t launch_bbp_fx(const uint64_t* d_pos, uint64_t start, size_t count, uint64_t dmax,
                          uint8_t* d_out, void* ws, int odd_undecided, hipStream_t s);
@@ -133,7 +133,7 @@ hipError_t launch_bbp_chunked(const uint64_t* d_pos, uint64_t start, size_t coun
                               uint8_t* d_out, void* ws, hipStream_t s);
 hipError_t launch_bbp_dmax(const uint64_t* d_pos, size_t count, uint64_t* d_dmax, hipStream_t s);
 size_t bbp_few_ws_bytes(size_t count);
-constexpr size_t BBP_CHUNKED_MAX = 8192;      // positions below which the few-positions paths run
+constexpr size_t BBP_CHUNKED_MAX = 49152;     // positions below which the fixed-point path runs (measured crossover)
 
 // Chudnovsky binary splitting (bs_kernels.hip): P, Q, T of every group of
 // 2^G consecutive terms of [0, n_terms) into slots of 3, 5, 6 x 2^G limbs.

@@ -39,6 +39,25 @@ def test_contiguous_1_to_65536(pdh, golden):
     assert not bad, f"{len(bad)} mismatches, first at position {bad[0] + 1}"
 
 
+def test_range_on_the_fixed_point_path(pdh, golden):
+    """Positions 1..40,000 in one call run the fixed-point path (below 49,152
+    positions); the digits equal the reference's."""
+    g = golden["bbp_range"]
+    want = open(os.path.join(GOLDEN_DIR, g["file"])).read().strip()
+    assert pdh.bbp_hex_digits_hip(1, 40000) == want[:40000]
+
+
+def test_work_queue_path_any_order(pdh, golden):
+    """60,000 positions in random order (the lane-per-series path with its
+    work queue) equal the reference's digits at those positions."""
+    g = golden["bbp_range"]
+    want = open(os.path.join(GOLDEN_DIR, g["file"])).read().strip()
+    rng = np.random.default_rng(11)
+    pos = rng.integers(1, g["count"] + 1, size=60000).astype(np.uint64)
+    got = pdh.to_hex(pdh.bbp_values_positions(pos))
+    assert got == "".join(want[int(p) - 1] for p in pos)
+
+
 def test_positions_api_any_order(pdh):
     rng = np.random.default_rng(3)
     pos = rng.integers(1, 3000, size=500).astype(np.uint64)
