@@ -81,15 +81,52 @@ def cpu_model() -> str:
     return platform.processor() or "unknown"
 
 
-def cpu_baseline(a, b):
+CPU_THREADS = 16      # the host-core share of one GPU
+
+
+def cpu_baseline(a, b, W=None, extras=True):
+    """The reference's ntt-cpu (or its restatement) on one 2^22-limb multiply,
+    single thread; with `extras`, also the throughput configurations on
+    CPU_THREADS threads of independent single-threaded jobs: the 4,096-pair
+    batch, BBP over positions 1..8,192 and 32 positions in [10^6, 2*10^6]
+    (mulmod/s, the reference's bbp_mulmod count; bounded samples)."""
     import oracle
     kind = "reference" if oracle.ref_available() else "port"
     fn = oracle.ref_mul_ntt if kind == "reference" else oracle.mul_ntt
     t = time.perf_counter()
     fn(a, b)
     dt = time.perf_counter() - t
-    return {"value": 1.0 / dt, "unit": "mul/s", "cores": 1, "kind": kind,
-            "sample": f"one 2^22 x 2^22-limb multiply, single-threaded ntt-cpu, {dt:.2f} s on {cpu_model()}"}
+    out = {"value": 1.0 / dt, "unit": "mul/s", "cores": 1, "kind": kind,
+           "sample": f"one 2^22 x 2^22-limb multiply, single-threaded ntt-cpu, {dt:.2f} s on {cpu_model()}"}
+    if not extras or W is None:
+        return out
+    import numpy as np
+    th = min(CPU_THREADS, os.cpu_count() or 1)
+    batch = oracle.ref_mul_ntt_batch if kind == "reference" else oracle.mul_ntt_batch
+    pairs = W.batch_pairs(3, W.C3_PAIRS, W.C3_LO, W.C3_HI)
+    t = time.perf_counter()
+    batch(pairs, th)
+    dt = time.perf_counter() - t
+    out["c3_batch_4096"] = {"value": len(pairs) / dt, "unit": "mul/s", "cores": th, "kind": kind,
+                            "sample": f"the 4,096 pairs, {th} threads, {dt:.2f} s"}
+    rng = oracle.ref_bbp_hex_range if kind == "reference" else oracle.bbp_hex_range
+    t = time.perf_counter()
+    rng(1, 8192, th)
+    dt = time.perf_counter() - t
+    M = W.bbp_mulmod_count(np.arange(1, 8193, dtype=np.uint64))
+    out["c4_bbp_1_8192"] = {"value": M / dt, "unit": "mulmod/s", "cores": th, "kind": kind,
+                            "sample": f"positions 1..8,192, {th} threads, {dt:.2f} s"}
+    pos = np.sort(np.random.default_rng(17).integers(10**6, 2 * 10**6, 32)).astype(np.uint64)
+    t = time.perf_counter()
+    if kind == "reference":
+        oracle.ref_bbp_hex_positions(pos, th)
+    else:
+        oracle.bbp_values_positions(pos, th)
+    dt = time.perf_counter() - t
+    M = W.bbp_mulmod_count(pos)
+    out["c4_bbp_sampled"] = {"value": M / dt, "unit": "mulmod/s", "cores": th, "kind": kind,
+                             "sample": f"32 positions uniform in [1e6, 2e6] (seed 17), {th} threads, {dt:.2f} s"}
+    return out
 
 
 def read_traffic(kernel: str):
@@ -417,6 +454,14 @@ def main() -> int:
                                    "fallback_positions": pdh.bbp_exact_count() - fb0,
                                    "extrapolated_2p20_positions_s": M20 / (Ms / (sms * 1e-3)),
                                    "bit_exact": got == want_sorted}
+        # the integer-multiply ceiling: the measured v_mad_u64_u32 issue rate,
+        # at four multiply-class instructions per 32-bit-modulus mulmod
+        rate = pdh.probe_mad_rate()
+        ceiling = rate / 4
+        extra["bbp_ceiling"] = {"mad_u64_u32_lane_ops_per_s": rate, "mulmod_per_s": ceiling,
+                                "basis": "measured v_mad_u64_u32 rate / 4 multiply-class ops per mulmod"}
+        for k in ("bbp_1_65536", "bbp_sampled_64"):
+            extra[k]["ceiling_frac"] = extra[k]["mulmod_per_s"] / ceiling
 
         extra["c3_batch_4096"] = bench_batch(pdh, torch, W, stream, reps=3)
         # the binary-splitting products of a 10^7-digit run at tree depths 8..14
@@ -444,7 +489,7 @@ def main() -> int:
             # server overlaps and groups them, the serial loop cannot
             extra["c5_serve_burst"] = bench_serve(pdh, W, n_req=40, rate=20.0)
 
-    cpu = None if args.no_cpu_baseline else cpu_baseline(a, b)
+    cpu = None if args.no_cpu_baseline else cpu_baseline(a, b, W, extras=not args.no_extra)
     line = {
         "metric": METRIC, "value": 1e3 / ms, "unit": "mul/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",

@@ -395,10 +395,13 @@ hipError_t launch_bbp_fx(const uint64_t* d_pos, uint64_t start, size_t count, ui
     uint64_t* acc = (uint64_t*)((char*)ws + ((S * sizeof(BbpSeries) + 255) & ~(size_t)255));
     BBP_LAUNCH("k_bbp_init", s, k_bbp_init, dim3((unsigned)((S + 255) / 256)), dim3(256), 0, s, d_pos, start, count,
                st, acc);
-    // terms per lane: at least 32, more when the grid would pass 2^23 blocks
+    // terms per lane: at least 32, more while the grid passes 2^17 blocks (and
+    // a series has more than one): few long blocks beat many short ones, most
+    // of them empty for the shorter series (measured on 1..65,536: 2^23-block
+    // cap 102 ms, 2^18 52 ms; 2^17 also halves 1..32,768 to 12.6 ms)
     int KT = 32;
     uint64_t bps = (dmax + 1 + 256ull * KT - 1) / (256ull * KT);
-    while (bps * S > ((uint64_t)1 << 23)) { KT *= 2; bps = (dmax + 1 + 256ull * KT - 1) / (256ull * KT); }
+    while (bps > 1 && bps * S > ((uint64_t)1 << 17)) { KT *= 2; bps = (dmax + 1 + 256ull * KT - 1) / (256ull * KT); }
     BBP_LAUNCH("k_bbp_fx_terms", s, k_bbp_fx_terms, dim3((unsigned)(bps * S)), dim3(256), 0, s,
                (const BbpSeries*)st, (unsigned)bps, KT, acc);
     BBP_LAUNCH("k_bbp_fx_finish", s, k_bbp_fx_finish, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s,
@@ -430,6 +433,53 @@ hipError_t launch_bbp_chunked(const uint64_t* d_pos, uint64_t start, size_t coun
 hipError_t launch_bbp_dmax(const uint64_t* d_pos, size_t count, uint64_t* d_dmax, hipStream_t s) {
     BBP_LAUNCH("k_bbp_dmax", s, k_bbp_dmax, dim3(64), dim3(256), 0, s, d_pos, count, (unsigned long long*)d_dmax);
     return hipSuccess;
+}
+
+// ---- integer multiply issue-rate probe ------------------------------------------------
+// The ceiling the BBP throughput is compared with: eight independent chains of
+// t = lo32(t) * x + t (one v_mad_u64_u32 each) per lane over a fixed grid,
+// every access in bounds (one u64 out per thread).
+__global__ __launch_bounds__(256) void k_probe_mad(uint64_t* __restrict__ out, uint32_t seed, int iters) {
+    const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t x = (seed ^ (uint32_t)gid) | 1u;
+    uint64_t t[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) t[c] = gid * 8 + c;
+    for (int i = 0; i < iters; ++i) {
+#pragma unroll
+        for (int c = 0; c < 8; ++c) t[c] = (uint64_t)(uint32_t)t[c] * x + t[c];
+    }
+    uint64_t acc = 0;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) acc += t[c];
+    out[gid] = acc;
+}
+
+hipError_t probe_mad_rate(double* lane_ops_per_s) {
+    constexpr unsigned blocks = 8192, threads = 256;
+    constexpr int iters = 4096;
+    uint64_t* d = nullptr;
+    hipError_t e = hipMalloc(&d, (size_t)blocks * threads * 8);
+    if (e != hipSuccess) return e;
+    hipEvent_t a = nullptr, b = nullptr;
+    float ms = 0.f;
+    if ((e = hipEventCreate(&a)) == hipSuccess && (e = hipEventCreate(&b)) == hipSuccess) {
+        hipLaunchKernelGGL(k_probe_mad, dim3(blocks), dim3(threads), 0, 0, d, 7u, iters);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipEventRecord(a, 0);
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(k_probe_mad, dim3(blocks), dim3(threads), 0, 0, d, 11u, iters);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipEventRecord(b, 0);
+        if (e == hipSuccess) e = hipEventSynchronize(b);
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+    }
+    if (a) hipEventDestroy(a);
+    if (b) hipEventDestroy(b);
+    hipFree(d);
+    if (e == hipSuccess) *lane_ops_per_s = (double)blocks * threads * iters * 8 / (ms * 1e-3);
+    return e;
 }
 
 hipError_t launch_bbp(const uint64_t* d_pos, uint64_t start, size_t count, uint8_t* d_out, uint64_t* d_next,

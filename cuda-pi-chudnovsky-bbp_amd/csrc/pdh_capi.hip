@@ -273,9 +273,8 @@ int mul_host(const u32* a, size_t na, const u32* b, size_t nb, bool square,
 
 constexpr uint64_t BBP_POS_MAX = (uint64_t)1 << 28;   // 8(pos-1)+6 < 2^31
 
-// Few positions (count < BBP_CHUNKED_MAX): the fixed-point path, then the
-// ordered FP64 path for every position it leaves undecided (or for all of
-// them when pdh_bbp_set_exact is on). d_pos (or start for a contiguous
+// The fixed-point path, then the ordered FP64 path for every position it
+// leaves undecided (or for all of them when pdh_bbp_set_exact is 1). d_pos (or start for a contiguous
 // range) and d_out are device buffers; h_pos, when given, holds the same
 // positions on the host. The values end in d_out and in h_vals. Synchronises
 // `st`.
@@ -360,7 +359,7 @@ int bbp_host(const uint64_t* positions, uint64_t start, size_t count, uint8_t* v
     }
     if (positions) HIPCALL(hipMemcpyAsync(c.st_pos, sorted.data(), count * 8, hipMemcpyHostToDevice, c.stream));
     std::vector<uint8_t> tmp(count);
-    if (count < BBP_CHUNKED_MAX) {
+    if (c.bbp_exact != 3) {
         const uint64_t dmax = positions ? sorted.back() - 1 : start + count - 2;
         rc = bbp_few_locked(c, positions ? c.st_pos : nullptr, positions ? sorted.data() : nullptr, start, count,
                             dmax, c.st_val, tmp.data(), c.stream);
@@ -483,8 +482,8 @@ int pdh_bbp_positions_dev(const uint64_t* d_positions, size_t n, uint8_t* d_valu
     if (rc) return rc;
     hipStream_t st = (hipStream_t)stream;
     hipError_t e;
-    if (n < BBP_CHUNKED_MAX) {
-        // both paths size their grids by the largest position
+    if (c.bbp_exact != 3) {
+        // both few-position paths size their grids by the largest position
         if (!c.bbp_dmax) HIPCALL(hipMalloc(&c.bbp_dmax, 8));
         e = launch_zero_len(c.bbp_dmax, st);
         if (e == hipSuccess) e = launch_bbp_dmax(d_positions, n, c.bbp_dmax, st);
@@ -516,7 +515,7 @@ int pdh_bs_groups(uint64_t n_terms, int group_log, uint32_t* d_P, uint32_t* d_Q,
 }
 
 int pdh_bbp_set_exact(int mode) {
-    if (mode < 0 || mode > 2) return fail(PDH_E_ARG, "pdh_bbp_set_exact: mode must be 0, 1 or 2");
+    if (mode < 0 || mode > 3) return fail(PDH_E_ARG, "pdh_bbp_set_exact: mode must be 0, 1, 2 or 3");
     Ctx& c = ctx();
     std::lock_guard<std::mutex> g(c.mu);
     c.bbp_exact = mode;
@@ -530,6 +529,17 @@ uint64_t pdh_bbp_exact_count(void) {
 }
 
 uint64_t pdh_launch_count(void) { return g_launches.load(); }
+
+int pdh_probe_mad_rate(double* lane_ops_per_s) {
+    if (!lane_ops_per_s) return fail(PDH_E_ARG, "null pointer argument");
+    Ctx& c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    int rc = init_locked(c);
+    if (rc) return rc;
+    hipError_t e = probe_mad_rate(lane_ops_per_s);
+    if (e != hipSuccess) return fail_hip(e, "probe launch");
+    return PDH_OK;
+}
 
 // ---- batched multiplies --------------------------------------------------------------
 struct pdh_batch {

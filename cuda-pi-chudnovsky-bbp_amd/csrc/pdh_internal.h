@@ -132,8 +132,9 @@ hipError_t launch_bbp_fx(const uint64_t* d_pos, uint64_t start, size_t count, ui
 hipError_t launch_bbp_chunked(const uint64_t* d_pos, uint64_t start, size_t count, uint64_t dmax,
                               uint8_t* d_out, void* ws, hipStream_t s);
 hipError_t launch_bbp_dmax(const uint64_t* d_pos, size_t count, uint64_t* d_dmax, hipStream_t s);
+// Measured issue rate of v_mad_u64_u32 (lane operations per second, whole GPU).
+hipError_t probe_mad_rate(double* lane_ops_per_s);
 size_t bbp_few_ws_bytes(size_t count);
-constexpr size_t BBP_CHUNKED_MAX = 49152;     // positions below which the fixed-point path runs (measured crossover)
 
 // Chudnovsky binary splitting (bs_kernels.hip): P, Q, T of every group of
 // 2^G consecutive terms of [0, n_terms) into slots of 3, 5, 6 x 2^G limbs.

@@ -34,7 +34,7 @@ PDH_E_TOO_LARGE = -5
 EXPORTED = (
     "pdh_last_error", "pdh_device_available", "pdh_ntt_length", "pdh_mul", "pdh_sqr",
     "pdh_mul_dev", "pdh_sqr_dev", "pdh_bbp_range", "pdh_bbp_positions",
-    "pdh_bbp_positions_dev", "pdh_launch_count", "pdh_timing_enable", "pdh_timing_read",
+    "pdh_bbp_positions_dev", "pdh_launch_count", "pdh_timing_enable", "pdh_timing_read", "pdh_probe_mad_rate",
     "pdh_batch_create", "pdh_batch_run", "pdh_batch_capture", "pdh_batch_destroy", "pdh_mul_batch_dev",
     "pdh_tune_get", "pdh_tune_set", "pdh_autotune", "pdh_dev_alloc", "pdh_dev_free", "pdh_copy_htod",
     "pdh_copy_dtoh", "pdh_stream_create", "pdh_stream_destroy", "pdh_stream_sync",
@@ -274,10 +274,17 @@ def bs_groups(n_terms: int, group_log: int, d_P: int, d_Q: int, d_T: int, d_len:
 
 
 def bbp_set_exact(mode: int) -> None:
-    """Calls below 8192 positions: 0 fixed point with fallback (default),
-    1 FP64-order path only, 2 fixed point with odd indices sent to the
-    fallback too (testing)."""
+    """0 fixed point with fallback (default), 1 chunked FP64-order path
+    only, 2 fixed point with odd indices sent to the fallback too (testing),
+    3 one lane per series in FP64 order (work-queue kernel)."""
     _check(lib().pdh_bbp_set_exact(int(mode)))
+
+
+def probe_mad_rate() -> float:
+    """Measured v_mad_u64_u32 issue rate of the device (lane ops per second)."""
+    v = ctypes.c_double(0.0)
+    _check(lib().pdh_probe_mad_rate(ctypes.byref(v)))
+    return v.value
 
 
 def bbp_exact_count() -> int:

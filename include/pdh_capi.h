@@ -150,14 +150,11 @@ int pdh_bbp_range(uint64_t start, uint32_t count, uint8_t* values);
 int pdh_bbp_positions(const uint64_t* positions, size_t n, uint8_t* values);
 
 /* Device-resident variant: d_positions and d_values are device pointers;
- * enqueued on `stream`. Below 49,152 positions every term is computed in
- * parallel: first as exact 64-bit fixed-point fractions, which decide the
- * digit whenever it lies provably far from a digit boundary; the rare rest
- * (about 4e-6 per position at 10^8) is recomputed with the reference's FP64
- * fold order. That case synchronises `stream`. From 49,152 positions one
- * lane per series keeps the reference's FP64 order, wavefronts taking groups
- * of positions from a work-queue counter in the library's context, so such
- * calls on different streams must not overlap on the device. Positions must lie in
+ * enqueued on `stream`. Every term is computed in parallel: first as exact
+ * 64-bit fixed-point fractions, which decide the digit whenever it lies
+ * provably far from a digit boundary; the rare rest (about 4e-6 per position
+ * at 10^8) is recomputed with the reference's FP64 fold order. The call
+ * synchronises `stream` once (to find those positions). Positions must lie in
  * [1, 2^28] (the kernel's exact FP64 modular arithmetic needs 8k + j <
  * 2^31); the host entry points reject others with PDH_E_ARG, and this one
  * writes 0xFF for them. */
@@ -177,14 +174,21 @@ int pdh_bs_groups(uint64_t n_terms, int group_log, uint32_t* d_P, uint32_t* d_Q,
                   uint64_t* d_len, int32_t* d_sign, void* stream);
 
 /* Testing and diagnostics (no counterpart in the reference).
- * pdh_bbp_set_exact(mode) for calls below 49,152 positions: 0 (default) the
- * fixed-point path with the FP64-order fallback; 1 the FP64-order path for
- * every position; 2 the fixed-point path with every odd index (in sorted
- * order) sent to the fallback as well, to exercise it. pdh_bbp_exact_count
- * is the number of positions computed by the FP64-order path in this
+ * pdh_bbp_set_exact(mode): 0 (default) the fixed-point path with the
+ * FP64-order fallback; 1 the FP64-order (chunked) path for every position;
+ * 2 the fixed-point path with every odd index (in sorted order) sent to the
+ * fallback as well, to exercise it; 3 one lane per series in the reference's
+ * FP64 order for every position (wavefronts take groups of positions from a
+ * work-queue counter in the library's context, so such calls on different
+ * streams must not overlap on the device). pdh_bbp_exact_count is the
+ * number of positions computed by the chunked FP64-order path in this
  * process. */
 int pdh_bbp_set_exact(int mode);
 uint64_t pdh_bbp_exact_count(void);
+/* The device's measured v_mad_u64_u32 issue rate (lane operations per
+ * second, whole GPU), the integer-multiply ceiling BBP throughput is quoted
+ * against. Synchronous; allocates and frees its own buffer. */
+int pdh_probe_mad_rate(double* lane_ops_per_s);
 
 /* Number of kernels this library has launched in this process. Lets a test
  * assert that the device path really ran (the reference's self-test could

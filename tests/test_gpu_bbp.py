@@ -39,17 +39,16 @@ def test_contiguous_1_to_65536(pdh, golden):
     assert not bad, f"{len(bad)} mismatches, first at position {bad[0] + 1}"
 
 
-def test_range_on_the_fixed_point_path(pdh, golden):
-    """Positions 1..40,000 in one call run the fixed-point path (below 49,152
-    positions); the digits equal the reference's."""
+def test_range_prefix(pdh, golden):
+    """Positions 1..40,000 in one call equal the reference's digits."""
     g = golden["bbp_range"]
     want = open(os.path.join(GOLDEN_DIR, g["file"])).read().strip()
     assert pdh.bbp_hex_digits_hip(1, 40000) == want[:40000]
 
 
-def test_work_queue_path_any_order(pdh, golden):
-    """60,000 positions in random order (the lane-per-series path with its
-    work queue) equal the reference's digits at those positions."""
+def test_any_order_60000(pdh, golden):
+    """60,000 positions in random order equal the reference's digits at
+    those positions."""
     g = golden["bbp_range"]
     want = open(os.path.join(GOLDEN_DIR, g["file"])).read().strip()
     rng = np.random.default_rng(11)
@@ -115,6 +114,24 @@ def test_ordered_path_forced(pdh, golden, exact_mode):
     assert pdh.bbp_exact_count() == before + 138
 
 
+def test_lane_per_series_path(pdh, golden, exact_mode):
+    """Mode 3: one lane per series in the reference's FP64 order, the
+    wavefronts fed by the work queue, for the contiguous range 1..65,536
+    (host entry point) and for random positions (device entry point)."""
+    import torch
+    exact_mode(3)
+    g = golden["bbp_range"]
+    want = open(os.path.join(GOLDEN_DIR, g["file"])).read().strip()
+    assert pdh.bbp_hex_digits_hip(g["start"], g["count"]) == want
+    rng = np.random.default_rng(12)
+    pos = rng.integers(1, g["count"] + 1, size=5000).astype(np.uint64)
+    dpos = torch.from_numpy(pos.view(np.int64)).cuda()
+    dval = torch.zeros(len(pos), dtype=torch.uint8, device="cuda")
+    pdh.bbp_positions_dev(dpos.data_ptr(), len(pos), dval.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    assert pdh.to_hex(dval.cpu().numpy()) == "".join(want[int(p) - 1] for p in pos)
+
+
 def test_fallback_plumbing(pdh, golden, exact_mode):
     """Mode 2: every odd index (sorted order) is recomputed by the fallback
     and scattered back; host and device entry points give the same digits."""
@@ -133,4 +150,4 @@ def test_fallback_plumbing(pdh, golden, exact_mode):
     torch.cuda.synchronize()
     assert np.array_equal(dval.cpu().numpy(), want)
     with pytest.raises(pdh.PdhError):
-        pdh.bbp_set_exact(3)
+        pdh.bbp_set_exact(4)
