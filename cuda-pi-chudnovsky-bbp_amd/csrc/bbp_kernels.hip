@@ -475,9 +475,9 @@ hipError_t probe_mad_rate(double* lane_ops_per_s) {
         if (e == hipSuccess) e = hipEventSynchronize(b);
         if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
     }
-    if (a) hipEventDestroy(a);
-    if (b) hipEventDestroy(b);
-    hipFree(d);
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    (void)hipFree(d);
     if (e == hipSuccess) *lane_ops_per_s = (double)blocks * threads * iters * 8 / (ms * 1e-3);
     return e;
 }

@@ -261,4 +261,205 @@ hipError_t launch_bs_groups(uint64_t n_terms, int G, uint32_t* dP, uint32_t* dQ,
     return e;
 }
 
+// ---- signed merge of T above the groups: T = sa A + sb B ------------------------
+// A = T_L Q_R and B = P_L T_R come from the batched multiplies; their signs are
+// the children's T signs. Every node is cut into tiles of SA_NT x SA_K limbs
+// (one workgroup each; thread k owns SA_K consecutive limbs). A segment of
+// limbs has a carry status for the sum, or a borrow status for the difference
+// X - Y: GEN (carry/borrow out whatever comes in), PROP (out iff in), KILL
+// (never out). For X - Y the borrow status of a segment is its comparison:
+// X_seg < Y_seg GEN, equal PROP, greater KILL, so the tile statuses of A - B
+// also give the order of |A| and |B| (the highest non-PROP tile decides) and
+// flipping GEN and KILL gives the statuses of B - A. The carry into any
+// segment is the status of the nearest lower non-PROP segment (GEN: 1,
+// KILL: 0; none: 0). Pass 1 writes one status per tile; pass 2 composes the
+// tile's carry-in from the lower tiles' statuses, recomputes its own segments,
+// writes T and atomically raises the node's length.
+namespace {
+
+constexpr int SA_NT = 256;
+constexpr int SA_K = BS_ADD_TILE / SA_NT;
+static_assert(SA_NT * SA_K == BS_ADD_TILE, "tile size");
+constexpr uint32_t ST_KILL = 0, ST_PROP = 1, ST_GEN = 2;
+
+__device__ __forceinline__ void sa_load(const uint32_t* a, uint64_t la, const uint32_t* b, uint64_t lb,
+                                        uint64_t base, uint32_t* x, uint32_t* y) {
+#pragma unroll
+    for (int k = 0; k < SA_K; ++k) {
+        const uint64_t i = base + k;
+        x[k] = i < la ? a[i] : 0u;
+        y[k] = i < lb ? b[i] : 0u;
+    }
+}
+
+__device__ __forceinline__ uint32_t seg_status(const uint32_t* x, const uint32_t* y, bool sub) {
+    if (!sub) {
+        uint64_t c = 0;
+        uint32_t all = 0xFFFFFFFFu;
+#pragma unroll
+        for (int k = 0; k < SA_K; ++k) {
+            c += (uint64_t)x[k] + y[k];
+            all &= (uint32_t)c;
+            c >>= 32;
+        }
+        return c ? ST_GEN : (all == 0xFFFFFFFFu ? ST_PROP : ST_KILL);
+    }
+    uint32_t st = ST_PROP;
+#pragma unroll
+    for (int k = SA_K - 1; k >= 0; --k)
+        if (st == ST_PROP && x[k] != y[k]) st = x[k] < y[k] ? ST_GEN : ST_KILL;
+    return st;
+}
+
+// The status of the nearest lower thread of the block whose status is not
+// PROP (PROP if none), and in *agg the block's own status (that of its
+// highest non-PROP thread). wv holds SA_NT / 64 words of LDS.
+__device__ __forceinline__ uint32_t block_status_in(uint32_t st, uint32_t* wv, uint32_t* agg) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint64_t m = __ballot(st != ST_PROP);
+    const uint64_t lower = m & ((1ull << lane) - 1ull);
+    const uint32_t from_lower = __shfl(st, lower ? 63 - __builtin_clzll(lower) : 0);
+    const uint32_t wave_agg = __shfl(st, m ? 63 - __builtin_clzll(m) : 0);
+    if (lane == 0) wv[w] = m ? wave_agg : ST_PROP;
+    __syncthreads();
+    uint32_t in = lower ? from_lower : ST_PROP;
+    for (int v = w - 1; v >= 0 && in == ST_PROP; --v) in = wv[v];
+    uint32_t a = ST_PROP;
+    for (int v = SA_NT / 64 - 1; v >= 0 && a == ST_PROP; --v) a = wv[v];
+    *agg = a;
+    return in;
+}
+
+__device__ __forceinline__ uint32_t block_max_u32(uint32_t v, uint32_t* red) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint32_t u = __shfl_xor(v, o);
+        v = u > v ? u : v;
+    }
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    uint32_t r = 0;
+    for (int i = 0; i < SA_NT / 64; ++i) r = red[i] > r ? red[i] : r;
+    return r;
+}
+
+struct SaView {
+    uint32_t nd, j;
+    uint64_t la, lb;
+    bool sub;
+};
+
+__device__ __forceinline__ SaView sa_view(const BsAddNode& N, uint32_t nd, uint32_t t) {
+    SaView v;
+    v.nd = nd;
+    v.j = t - N.tile0;
+    v.la = N.sa ? *N.la : 0;
+    v.lb = N.sb ? *N.lb : 0;
+    v.sub = N.sa != 0 && N.sb != 0 && N.sa != N.sb;
+    return v;
+}
+
+__global__ __launch_bounds__(SA_NT) void k_bs_add_status(const BsAddNode* __restrict__ nodes,
+                                                         const uint32_t* __restrict__ tile_node,
+                                                         uint8_t* __restrict__ status, uint64_t* __restrict__ len) {
+    __shared__ uint32_t wv[SA_NT / 64];
+    const uint32_t t = blockIdx.x;
+    const uint32_t nd = tile_node[t];
+    const BsAddNode N = nodes[nd];
+    const SaView v = sa_view(N, nd, t);
+    const uint64_t base = (uint64_t)v.j * BS_ADD_TILE + (uint64_t)threadIdx.x * SA_K;
+    uint32_t x[SA_K], y[SA_K];
+    sa_load(N.T, v.la, N.B, v.lb, base, x, y);
+    uint32_t agg;
+    (void)block_status_in(seg_status(x, y, v.sub), wv, &agg);
+    if (threadIdx.x == 0) {
+        status[t] = (uint8_t)agg;
+        if (v.j == 0) len[nd] = 0;
+    }
+}
+
+__global__ __launch_bounds__(SA_NT) void k_bs_add_apply(const BsAddNode* __restrict__ nodes,
+                                                        const uint32_t* __restrict__ tile_node,
+                                                        const uint8_t* __restrict__ status, uint64_t* __restrict__ len,
+                                                        int32_t* __restrict__ sign) {
+    __shared__ uint32_t wv[SA_NT / 64];
+    __shared__ uint32_t red[2][SA_NT / 64];
+    const uint32_t t = blockIdx.x;
+    const uint32_t nd = tile_node[t];
+    const BsAddNode N = nodes[nd];
+    const SaView v = sa_view(N, nd, t);
+    // nearest lower non-PROP tile (carry into this tile) and, for a
+    // difference, the highest non-PROP tile (the order of |A| and |B|);
+    // keys are (tile + 1) * 4 + status, 0 for none
+    uint32_t kin = 0, kall = 0;
+    const uint32_t lim = v.sub ? N.ntiles : v.j;
+    for (uint32_t i = threadIdx.x; i < lim; i += SA_NT) {
+        const uint32_t s = status[N.tile0 + i];
+        if (s != ST_PROP) {
+            const uint32_t key = (i + 1) * 4 + s;
+            if (i < v.j) kin = key > kin ? key : kin;
+            kall = key > kall ? key : kall;
+        }
+    }
+    kin = block_max_u32(kin, red[0]);
+    kall = v.sub ? block_max_u32(kall, red[1]) : 0;
+    const bool swap = v.sub && (kall & 3) == ST_GEN;          // |A| < |B|: T = sb (B - A)
+    uint32_t tin = kin ? (kin & 3) : ST_PROP;
+    if (swap && tin != ST_PROP) tin = ST_GEN + ST_KILL - tin;
+    const uint64_t base = (uint64_t)v.j * BS_ADD_TILE + (uint64_t)threadIdx.x * SA_K;
+    uint32_t x[SA_K], y[SA_K];
+    if (swap) sa_load(N.B, v.lb, N.T, v.la, base, x, y);
+    else sa_load(N.T, v.la, N.B, v.lb, base, x, y);
+    uint32_t agg;
+    const uint32_t in = block_status_in(seg_status(x, y, v.sub), wv, &agg);
+    const uint32_t st_in = in != ST_PROP ? in : tin;
+    uint64_t c = st_in == ST_GEN ? 1u : 0u;
+    uint32_t top = 0;
+#pragma unroll
+    for (int k = 0; k < SA_K; ++k) {
+        uint32_t r;
+        if (!v.sub) {
+            c += (uint64_t)x[k] + y[k];
+            r = (uint32_t)c;
+            c >>= 32;
+        } else {
+            const uint64_t d = (uint64_t)x[k] - y[k] - c;
+            r = (uint32_t)d;
+            c = (d >> 32) & 1u;
+        }
+        const uint64_t i = base + k;
+        if (i < N.cap) {
+            N.T[i] = r;
+            if (r) top = (uint32_t)(k + 1);
+        }
+    }
+    uint64_t hi = top ? base + top : 0;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint64_t u = __shfl_xor(hi, o);
+        hi = u > hi ? u : hi;
+    }
+    if ((threadIdx.x & 63) == 0 && hi) atomicMax((unsigned long long*)&len[nd], (unsigned long long)hi);
+    if (v.j == 0 && threadIdx.x == 0) sign[nd] = !v.sub ? (N.sa ? N.sa : N.sb) : (swap ? N.sb : N.sa);
+}
+
+} // namespace
+
+hipError_t launch_bs_signed_add(const BsAddNode* d_nodes, const uint32_t* d_tile_node, uint32_t ntiles,
+                                uint8_t* d_status, uint64_t* d_len, int32_t* d_sign, hipStream_t s) {
+    if (ntiles == 0) return hipSuccess;
+    hipEvent_t t0 = timing_begin(s);
+    hipLaunchKernelGGL(k_bs_add_status, dim3(ntiles), dim3(SA_NT), 0, s, d_nodes, d_tile_node, d_status, d_len);
+    g_launches.fetch_add(1, std::memory_order_relaxed);
+    hipError_t e = hipGetLastError();
+    timing_end("k_bs_add_status", t0, s);
+    if (e != hipSuccess) return e;
+    t0 = timing_begin(s);
+    hipLaunchKernelGGL(k_bs_add_apply, dim3(ntiles), dim3(SA_NT), 0, s, d_nodes, d_tile_node, d_status, d_len, d_sign);
+    g_launches.fetch_add(1, std::memory_order_relaxed);
+    e = hipGetLastError();
+    timing_end("k_bs_add_apply", t0, s);
+    return e;
+}
+
 } // namespace pdh

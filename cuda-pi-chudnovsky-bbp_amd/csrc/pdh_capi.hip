@@ -732,6 +732,133 @@ int pdh_mul_batch_dev(const pdh_mul_desc* descs, size_t n, void* stream) {
     return rc;
 }
 
+// One merge level of binary splitting in device memory (src/chudnovsky.cpp:
+// 93-100): the pairs' Q, T_L Q_R, P_L T_R and P products as one batch, with
+// T_L Q_R written straight into the output T buffer and P_L T_R into
+// scratch, then the signed sum in place (launch_bs_signed_add). Only the
+// lengths and signs come back to the host.
+int pdh_bs_merge_dev(const pdh_bs_node* in, size_t n_in, pdh_bs_node* out, int skip_p, void* stream) {
+    if (n_in == 0) return PDH_OK;
+    if (!in || !out) return fail(PDH_E_ARG, "null pointer argument");
+    for (size_t i = 0; i < n_in; ++i) {
+        const pdh_bs_node& v = in[i];
+        if (v.nP == 0 || v.nQ == 0 || !v.P || !v.Q || (v.nT && !v.T) || v.sT < -1 || v.sT > 1)
+            return fail(PDH_E_ARG, "pdh_bs_merge_dev: every input needs P, Q > 0, T's buffer and a sign in {-1, 0, 1}");
+    }
+    const size_t pairs = n_in / 2, n_out = (n_in + 1) / 2;
+    for (size_t i = 0; i < n_out; ++i)
+        if (!out[i].Q || !out[i].T || (!skip_p && !out[i].P)) return fail(PDH_E_ARG, "pdh_bs_merge_dev: null output buffer");
+    Ctx& c = ctx();
+    {
+        std::lock_guard<std::mutex> g(c.mu);
+        int rc = init_locked(c);
+        if (rc) return rc;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    auto eff_nT = [](const pdh_bs_node& v) { return v.sT ? v.nT : (uint64_t)0; };
+    auto al256 = [](size_t x) { return (x + 255) & ~(size_t)255; };
+
+    // scratch: B products | product lengths (4 per pair) | T lengths | T signs | add nodes | tile map | statuses
+    std::vector<uint64_t> boff(pairs), capT(pairs);
+    uint64_t bwords = 0, ntiles = 0;
+    for (size_t i = 0; i < pairs; ++i) {
+        const pdh_bs_node& L = in[2 * i];
+        const pdh_bs_node& R = in[2 * i + 1];
+        const uint64_t nA = eff_nT(L) ? eff_nT(L) + R.nQ : 0;
+        const uint64_t nB = eff_nT(R) ? L.nP + eff_nT(R) : 0;
+        boff[i] = bwords;
+        bwords += (nB + 3) & ~(uint64_t)3;
+        capT[i] = std::max(nA, nB) + 1;
+        ntiles += (capT[i] + BS_ADD_TILE - 1) / BS_ADD_TILE;
+    }
+    if (ntiles > 0xFFFFFFFFull) return fail(PDH_E_TOO_LARGE, "pdh_bs_merge_dev: too many tiles");
+    const size_t o_lens = al256(bwords * 4);
+    const size_t o_olen = o_lens + al256(pairs * 4 * 8);
+    const size_t o_osgn = o_olen + pairs * 8;
+    const size_t o_nodes = al256(o_osgn + pairs * 4);
+    const size_t o_tmap = o_nodes + al256(pairs * sizeof(BsAddNode));
+    const size_t o_stat = o_tmap + al256(ntiles * 4);
+    const size_t bytes = o_stat + al256(ntiles);
+    char* ws = nullptr;
+    pdh_batch* plan = nullptr;
+    auto cleanup = [&]() {
+        if (plan) pdh_batch_destroy(plan);
+        if (ws) { (void)hipStreamSynchronize(s); (void)hipFree(ws); }
+    };
+    if (pairs) {
+        hipError_t e = hipMalloc(&ws, bytes);
+        if (e != hipSuccess) return fail_hip(e, "pdh_bs_merge_dev: scratch allocation");
+    }
+    uint32_t* dB = (uint32_t*)ws;
+    uint64_t* dlens = (uint64_t*)(ws + o_lens);
+    uint64_t* dolen = (uint64_t*)(ws + o_olen);
+    int32_t* dosgn = (int32_t*)(ws + o_osgn);
+
+    // products
+    std::vector<pdh_mul_desc> jobs;
+    jobs.reserve(4 * pairs);
+    std::vector<char> stage(o_stat - o_nodes);
+    BsAddNode* an = (BsAddNode*)stage.data();
+    uint32_t* tmap = (uint32_t*)(stage.data() + (o_tmap - o_nodes));
+    uint32_t tile = 0;
+    for (size_t i = 0; i < pairs; ++i) {
+        const pdh_bs_node& L = in[2 * i];
+        const pdh_bs_node& R = in[2 * i + 1];
+        uint64_t* li = dlens + 4 * i;
+        jobs.push_back(pdh_mul_desc{L.Q, L.nQ, R.Q, R.nQ, out[i].Q, li + 0});
+        jobs.push_back(pdh_mul_desc{L.T, eff_nT(L), R.Q, R.nQ, out[i].T, li + 1});
+        jobs.push_back(pdh_mul_desc{L.P, L.nP, R.T, eff_nT(R), dB + boff[i], li + 2});
+        if (!skip_p) jobs.push_back(pdh_mul_desc{L.P, L.nP, R.P, R.nP, out[i].P, li + 3});
+        const uint32_t nt = (uint32_t)((capT[i] + BS_ADD_TILE - 1) / BS_ADD_TILE);
+        an[i] = BsAddNode{out[i].T, dB + boff[i], li + 1, li + 2, capT[i], tile, nt,
+                          eff_nT(L) ? L.sT : 0, eff_nT(R) ? R.sT : 0};
+        for (uint32_t k = 0; k < nt; ++k) tmap[tile + k] = (uint32_t)i;
+        tile += nt;
+    }
+    int rc = PDH_OK;
+    hipError_t e = hipSuccess;
+    if (pairs) {
+        e = hipMemcpyAsync(ws + o_nodes, stage.data(), stage.size(), hipMemcpyHostToDevice, s);
+        if (e != hipSuccess) { cleanup(); return fail_hip(e, "pdh_bs_merge_dev: node upload"); }
+        rc = pdh_batch_create(jobs.data(), jobs.size(), &plan);
+        if (rc == PDH_OK) rc = pdh_batch_run(plan, stream);
+        if (rc != PDH_OK) { std::string m = pdh_last_error(); cleanup(); return fail(rc, m); }
+        {
+            std::lock_guard<std::mutex> g(c.mu);
+            e = launch_bs_signed_add((const BsAddNode*)(ws + o_nodes), (const uint32_t*)(ws + o_tmap), tile,
+                                     (uint8_t*)(ws + o_stat), dolen, dosgn, s);
+        }
+        if (e != hipSuccess) { cleanup(); return fail_hip(e, "pdh_bs_merge_dev: signed add"); }
+    }
+    if (n_in & 1) {
+        const pdh_bs_node& v = in[n_in - 1];
+        pdh_bs_node& o = out[n_out - 1];
+        if (!skip_p) e = hipMemcpyAsync(o.P, v.P, v.nP * 4, hipMemcpyDeviceToDevice, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(o.Q, v.Q, v.nQ * 4, hipMemcpyDeviceToDevice, s);
+        if (e == hipSuccess && eff_nT(v)) e = hipMemcpyAsync(o.T, v.T, eff_nT(v) * 4, hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) { cleanup(); return fail_hip(e, "pdh_bs_merge_dev: copy of the odd node"); }
+        o.nP = skip_p ? 0 : v.nP;
+        o.nQ = v.nQ;
+        o.nT = eff_nT(v);
+        o.sT = o.nT ? v.sT : 0;
+    }
+    std::vector<char> back(o_osgn + pairs * 4 - o_lens);
+    if (pairs) e = hipMemcpyAsync(back.data(), ws + o_lens, back.size(), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) { cleanup(); return fail_hip(e, "pdh_bs_merge_dev: result lengths"); }
+    const uint64_t* hl = (const uint64_t*)back.data();
+    const uint64_t* ho = (const uint64_t*)(back.data() + (o_olen - o_lens));
+    const int32_t* hs = (const int32_t*)(back.data() + (o_osgn - o_lens));
+    for (size_t i = 0; i < pairs; ++i) {
+        out[i].nQ = hl[4 * i];
+        out[i].nP = skip_p ? 0 : hl[4 * i + 3];
+        out[i].nT = ho[i];
+        out[i].sT = ho[i] ? hs[i] : 0;
+    }
+    cleanup();
+    return PDH_OK;
+}
+
 int pdh_dev_alloc(size_t bytes, void** ptr) {
     if (!ptr) return fail(PDH_E_ARG, "null pointer argument");
     *ptr = nullptr;

@@ -141,6 +141,26 @@ size_t bbp_few_ws_bytes(size_t count);
 hipError_t launch_bs_groups(uint64_t n_terms, int G, uint32_t* dP, uint32_t* dQ, uint32_t* dT, uint64_t* dlen,
                             int32_t* dsgn, hipStream_t s);
 
+// Signed merge of T above the groups (bs_kernels.hip): for every node,
+// T = sa A + sb B with A in T's own buffer (device length *la) and B at B
+// (length *lb); sa, sb in {-1, 0, 1} (0: the operand is zero). Nodes are cut
+// into BS_ADD_TILE-limb tiles numbered tile0 .. tile0 + ntiles - 1 in one flat
+// list (d_tile_node maps a tile to its node); T is written below cap limbs,
+// d_len[node] receives T's normalized length and d_sign[node] its sign
+// (meaningless when the length is 0). d_status: one byte per tile.
+constexpr uint32_t BS_ADD_TILE = 4096;
+struct BsAddNode {
+    uint32_t* T;
+    const uint32_t* B;
+    const uint64_t* la;
+    const uint64_t* lb;
+    uint64_t cap;
+    uint32_t tile0, ntiles;
+    int32_t sa, sb;
+};
+hipError_t launch_bs_signed_add(const BsAddNode* d_nodes, const uint32_t* d_tile_node, uint32_t ntiles,
+                                uint8_t* d_status, uint64_t* d_len, int32_t* d_sign, hipStream_t s);
+
 extern std::atomic<uint64_t> g_launches;
 
 // Optional per-kernel timing (pdh_timing_enable): when on, each launch is

@@ -10,11 +10,12 @@
 //  definition, so a different algorithm yields the same digits. What
 //  differs from the reference (the next rows of its own roadmap):
 //
-//  * Binary splitting runs bottom-up, level by level, with all of a level's
-//    P, Q, T products in one batched device launch sequence (pdh_mul_batch_dev).
-//    The leaves and the first six levels (groups of 64 terms) are computed
-//    by one device kernel (pdh_bs_groups). P(0,N), Q(0,N), T(0,N) do not
-//    depend on where the range is split.
+//  * Binary splitting runs bottom-up, level by level, in device memory: the
+//    leaves and the first six levels (groups of 64 terms) in one kernel
+//    (pdh_bs_groups), then one pdh_bs_merge_dev call per level (the level's
+//    P, Q, T products as one batch and the signed T sum on the device); only
+//    lengths and signs reach the host until the root. P(0,N), Q(0,N), T(0,N)
+//    do not depend on where the range is split.
 //  * Division is a Newton reciprocal with an exact floor correction (the
 //    reference forwards div_fast to Knuth, src/bignum_div.cpp:170-178); isqrt
 //    is Newton with divisions at doubling precision and an exact correction
@@ -53,8 +54,9 @@ struct Multiplier {
     std::function<Mag(const Mag&, const Mag&)> mul;
     std::function<Mag(const Mag&)> sqr;
     std::function<std::vector<Mag>(const std::vector<std::pair<const Mag*, const Mag*>>&)> mul_many;
-    // optional: P, Q, T of consecutive groups of terms covering [0, N) (the
-    // bottom levels of the tree computed elsewhere); empty = from the leaves
+    // optional: P, Q, T of consecutive ranges covering [0, N) in order (the
+    // lower levels of the tree computed elsewhere; one node = the root, whose
+    // P may be empty); empty = from the leaves
     std::function<std::vector<PQT>(uint64_t N)> groups;
 };
 
@@ -527,16 +529,24 @@ static std::vector<Mag> dev_mul_many(const std::vector<std::pair<const Mag*, con
     return out;
 }
 
-// Leaves and the first six levels on the device (pdh_bs_groups): one node per
-// 64 consecutive terms.
-static std::vector<PQT> dev_groups(uint64_t N) {
+// The tree on the device. The leaves and the first six levels come from one
+// kernel (pdh_bs_groups, one node per 64 consecutive terms); every level above
+// is one pdh_bs_merge_dev call (batched products and the signed T sum in HBM),
+// each level in a fresh arena, the previous one freed. Only lengths and signs
+// cross PCIe per level. Merging stops where a product would exceed one
+// transform (2^23 limbs); the remaining level is downloaded and the host loop
+// finishes it with blocked products. The returned level has one node when the
+// device reached the root (its P, never used, is left empty).
+static std::vector<PQT> dev_levels(uint64_t N) {
     const int G = 6;
     const uint64_t groups = (N + (1u << G) - 1) >> G;
     const size_t cp = (size_t)3 << G, cq = (size_t)5 << G, ct = (size_t)6 << G;
-    void *dP = nullptr, *dQ = nullptr, *dT = nullptr, *dL = nullptr, *dS = nullptr, *st = nullptr;
+    const uint64_t LIM = (uint64_t)1 << 23;
+    void *dP = nullptr, *dQ = nullptr, *dT = nullptr, *dL = nullptr, *dS = nullptr, *st = nullptr, *arena = nullptr;
     auto release = [&]() { pdh_dev_free(dP); pdh_dev_free(dQ); pdh_dev_free(dT); pdh_dev_free(dL);
-                           pdh_dev_free(dS); pdh_stream_destroy(st); };
-    std::vector<PQT> out(groups);
+                           pdh_dev_free(dS); pdh_dev_free(arena); pdh_stream_destroy(st); };
+    auto r4 = [](uint64_t x) { return (x + 3) & ~(uint64_t)3; };
+    std::vector<PQT> out;
     try {
         check(pdh_stream_create(&st));
         check(pdh_dev_alloc(groups * cp * 4, &dP));
@@ -545,21 +555,64 @@ static std::vector<PQT> dev_groups(uint64_t N) {
         check(pdh_dev_alloc(groups * 3 * 8, &dL));
         check(pdh_dev_alloc(groups * 4, &dS));
         check(pdh_bs_groups(N, G, (uint32_t*)dP, (uint32_t*)dQ, (uint32_t*)dT, (uint64_t*)dL, (int32_t*)dS, st));
-        std::vector<uint32_t> P(groups * cp), Q(groups * cq), T(groups * ct);
         std::vector<uint64_t> L(groups * 3);
         std::vector<int32_t> S(groups);
-        check(pdh_copy_dtoh(P.data(), dP, P.size() * 4, st));
-        check(pdh_copy_dtoh(Q.data(), dQ, Q.size() * 4, st));
-        check(pdh_copy_dtoh(T.data(), dT, T.size() * 4, st));
         check(pdh_copy_dtoh(L.data(), dL, L.size() * 8, st));
         check(pdh_copy_dtoh(S.data(), dS, S.size() * 4, st));
         check(pdh_stream_sync(st));
-        for (uint64_t g = 0; g < groups; ++g) {
-            out[g].P.assign(P.begin() + g * cp, P.begin() + g * cp + L[3 * g]);
-            out[g].Q.assign(Q.begin() + g * cq, Q.begin() + g * cq + L[3 * g + 1]);
-            out[g].T.m.assign(T.begin() + g * ct, T.begin() + g * ct + L[3 * g + 2]);
-            out[g].T.s = S[g];
+        std::vector<pdh_bs_node> lvl(groups);
+        for (uint64_t g = 0; g < groups; ++g)
+            lvl[g] = pdh_bs_node{(uint32_t*)dP + g * cp, L[3 * g], (uint32_t*)dQ + g * cq, L[3 * g + 1],
+                                 (uint32_t*)dT + g * ct, L[3 * g + 2], S[g]};
+        while (lvl.size() > 1) {
+            const size_t pairs = lvl.size() / 2, n_out = (lvl.size() + 1) / 2;
+            const bool root = lvl.size() == 2;
+            bool fits = true;
+            uint64_t words = 0;
+            std::vector<uint64_t> cP(n_out), cQ(n_out), cT(n_out);
+            for (size_t i = 0; i < pairs; ++i) {
+                const pdh_bs_node& A = lvl[2 * i];
+                const pdh_bs_node& B = lvl[2 * i + 1];
+                cP[i] = root ? 0 : A.nP + B.nP;
+                cQ[i] = A.nQ + B.nQ;
+                cT[i] = std::max(A.nT + B.nQ, A.nP + B.nT) + 1;
+                if (cP[i] > LIM || cQ[i] > LIM || cT[i] - 1 > LIM) fits = false;
+            }
+            if (!fits) break;
+            if (lvl.size() & 1) {
+                const pdh_bs_node& v = lvl.back();
+                cP[pairs] = root ? 0 : v.nP; cQ[pairs] = v.nQ; cT[pairs] = std::max<uint64_t>(v.nT, 1);
+            }
+            for (size_t i = 0; i < n_out; ++i) words += r4(cP[i]) + r4(cQ[i]) + r4(cT[i]);
+            void* next = nullptr;
+            check(pdh_dev_alloc(words * 4, &next));
+            std::vector<pdh_bs_node> nl(n_out);
+            uint32_t* w = (uint32_t*)next;
+            for (size_t i = 0; i < n_out; ++i) {
+                nl[i].P = cP[i] ? w : nullptr; w += r4(cP[i]);
+                nl[i].Q = w; w += r4(cQ[i]);
+                nl[i].T = w; w += r4(cT[i]);
+            }
+            const int rc = pdh_bs_merge_dev(lvl.data(), lvl.size(), nl.data(), root ? 1 : 0, st);
+            if (rc != PDH_OK) { pdh_dev_free(next); check(rc); }
+            // the inputs are no longer needed: the groups' buffers or the previous arena
+            if (arena) { pdh_dev_free(arena); arena = nullptr; }
+            else { pdh_dev_free(dP); pdh_dev_free(dQ); pdh_dev_free(dT); dP = dQ = dT = nullptr; }
+            arena = next;
+            lvl.swap(nl);
         }
+        out.resize(lvl.size());
+        for (size_t i = 0; i < lvl.size(); ++i) {
+            const pdh_bs_node& v = lvl[i];
+            out[i].P.resize(v.nP);
+            out[i].Q.resize(v.nQ);
+            out[i].T.m.resize(v.nT);
+            out[i].T.s = v.nT ? v.sT : 0;
+            check(pdh_copy_dtoh(out[i].P.data(), v.P, v.nP * 4, st));
+            check(pdh_copy_dtoh(out[i].Q.data(), v.Q, v.nQ * 4, st));
+            check(pdh_copy_dtoh(out[i].T.m.data(), v.T, v.nT * 4, st));
+        }
+        check(pdh_stream_sync(st));
     } catch (...) {
         release();
         throw;
@@ -573,7 +626,7 @@ Multiplier device_multiplier() {
     M.mul = dev_mul;
     M.sqr = dev_sqr;
     M.mul_many = dev_mul_many;
-    M.groups = dev_groups;
+    M.groups = dev_levels;
     return M;
 }
 

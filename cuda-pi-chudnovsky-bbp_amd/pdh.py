@@ -38,7 +38,7 @@ EXPORTED = (
     "pdh_batch_create", "pdh_batch_run", "pdh_batch_capture", "pdh_batch_destroy", "pdh_mul_batch_dev",
     "pdh_tune_get", "pdh_tune_set", "pdh_autotune", "pdh_dev_alloc", "pdh_dev_free", "pdh_copy_htod",
     "pdh_copy_dtoh", "pdh_stream_create", "pdh_stream_destroy", "pdh_stream_sync",
-    "pdh_bbp_set_exact", "pdh_bbp_exact_count", "pdh_tune_engine", "pdh_bs_groups",
+    "pdh_bbp_set_exact", "pdh_bbp_exact_count", "pdh_tune_engine", "pdh_bs_groups", "pdh_bs_merge_dev",
 )
 
 
@@ -46,6 +46,13 @@ class MulDesc(ctypes.Structure):
     """pdh_mul_desc: device pointers of one multiply in a batch."""
     _fields_ = [("a", ctypes.c_void_p), ("na", ctypes.c_uint64), ("b", ctypes.c_void_p),
                 ("nb", ctypes.c_uint64), ("out", ctypes.c_void_p), ("out_len", ctypes.c_void_p)]
+
+
+class BsNode(ctypes.Structure):
+    """pdh_bs_node: device magnitudes P, Q, |T| with host lengths and the sign of T."""
+    _fields_ = [("P", ctypes.c_void_p), ("nP", ctypes.c_uint64), ("Q", ctypes.c_void_p),
+                ("nQ", ctypes.c_uint64), ("T", ctypes.c_void_p), ("nT", ctypes.c_uint64),
+                ("sT", ctypes.c_int32)]
 
 
 class PdhError(RuntimeError):
@@ -91,6 +98,8 @@ def lib() -> ctypes.CDLL:
         L.pdh_launch_count.restype = ctypes.c_uint64
         L.pdh_bs_groups.argtypes = [ctypes.c_uint64, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp]
         L.pdh_bs_groups.restype = ctypes.c_int
+        L.pdh_bs_merge_dev.argtypes = [ctypes.POINTER(BsNode), _sz, ctypes.POINTER(BsNode), ctypes.c_int, _vp]
+        L.pdh_bs_merge_dev.restype = ctypes.c_int
         L.pdh_bbp_set_exact.argtypes = [ctypes.c_int]
         L.pdh_bbp_set_exact.restype = ctypes.c_int
         L.pdh_bbp_exact_count.restype = ctypes.c_uint64
@@ -271,6 +280,18 @@ def bs_groups(n_terms: int, group_log: int, d_P: int, d_Q: int, d_T: int, d_len:
               stream: int = 0) -> None:
     """Device binary-splitting groups (pdh_bs_groups); all pointers are device addresses."""
     _check(lib().pdh_bs_groups(n_terms, group_log, d_P, d_Q, d_T, d_len, d_sign, stream or None))
+
+
+def bs_merge_dev(nodes, outs, skip_p: bool = False, stream: int = 0) -> list:
+    """One binary-splitting merge level in device memory (pdh_bs_merge_dev).
+
+    nodes: BsNode inputs; outs: BsNode outputs whose P, Q, T device buffers
+    the caller sized as the C header states. Returns the filled outputs."""
+    n_in = len(nodes)
+    arr_in = (BsNode * max(n_in, 1))(*nodes)
+    arr_out = (BsNode * max(len(outs), 1))(*outs)
+    _check(lib().pdh_bs_merge_dev(arr_in, n_in, arr_out, 1 if skip_p else 0, stream or None))
+    return [arr_out[i] for i in range(len(outs))]
 
 
 def bbp_set_exact(mode: int) -> None:

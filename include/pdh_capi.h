@@ -173,6 +173,29 @@ int pdh_bbp_positions_dev(const uint64_t* d_positions, size_t n, uint8_t* d_valu
 int pdh_bs_groups(uint64_t n_terms, int group_log, uint32_t* d_P, uint32_t* d_Q, uint32_t* d_T,
                   uint64_t* d_len, int32_t* d_sign, void* stream);
 
+/* One binary-splitting node: device magnitudes P, Q, |T| (little-endian,
+ * normalized, lengths in limbs on the host) and the sign of T. */
+typedef struct pdh_bs_node {
+    uint32_t* P; uint64_t nP;
+    uint32_t* Q; uint64_t nQ;
+    uint32_t* T; uint64_t nT;
+    int32_t sT;
+} pdh_bs_node;
+
+/* One level of the reference's merge (src/chudnovsky.cpp:93-100,
+ *     P = P_L P_R,  Q = Q_L Q_R,  T = T_L Q_R + P_L T_R)
+ * for the pairs (in[2i], in[2i+1]), entirely in device memory: the products
+ * run as one batch and the signed sum of T on the device (two passes over
+ * 4,096-limb tiles). out[i].P, .Q, .T are device buffers set by the caller
+ * with room for nP_L + nP_R, nQ_L + nQ_R and
+ * max(nT_L + nQ_R, nP_L + nT_R) + 1 limbs; they must not overlap the inputs.
+ * With n_in odd, out[n_in / 2] receives a copy of the last input node.
+ * skip_p != 0 leaves every output P unwritten (nP = 0; the root's P is never
+ * used). On return (the stream is synchronized) out[i].nP, .nQ, .nT, .sT
+ * hold the results. Every input needs nP > 0 and nQ > 0; nT = 0 means T = 0.
+ * Products of more than 2^23 limbs return PDH_E_TOO_LARGE. */
+int pdh_bs_merge_dev(const pdh_bs_node* in, size_t n_in, pdh_bs_node* out, int skip_p, void* stream);
+
 /* Testing and diagnostics (no counterpart in the reference).
  * pdh_bbp_set_exact(mode): 0 (default) the fixed-point path with the
  * FP64-order fallback; 1 the FP64-order (chunked) path for every position;

@@ -73,3 +73,22 @@ def test_argument_errors_before_device():
     a = (ctypes.c_uint32 * 2)(1, 2)
     rc = pdh.lib().pdh_mul(a, 2, a, 2, out, 2, ctypes.byref(n))     # capacity 2 < 4
     assert rc == pdh.PDH_E_CAPACITY
+
+
+def test_bs_merge_argument_errors_before_device():
+    """pdh_bs_merge_dev rejects null arrays, an input without P or Q and a
+    sign outside {-1, 0, 1} before any device work; an empty level is a no-op."""
+    import pdh
+    L = pdh.lib()
+    assert L.pdh_bs_merge_dev(None, 0, None, 0, None) == pdh.PDH_OK
+    assert L.pdh_bs_merge_dev(None, 2, None, 0, None) == pdh.PDH_E_ARG
+    ins = (pdh.BsNode * 2)(pdh.BsNode(8, 1, 8, 0, 8, 1, 1), pdh.BsNode(8, 1, 8, 1, 8, 1, 1))
+    outs = (pdh.BsNode * 1)(pdh.BsNode(8, 0, 8, 0, 8, 0, 0))
+    assert L.pdh_bs_merge_dev(ins, 2, outs, 0, None) == pdh.PDH_E_ARG          # nQ = 0
+    ins[0].nQ = 1
+    ins[1].sT = 2
+    assert L.pdh_bs_merge_dev(ins, 2, outs, 0, None) == pdh.PDH_E_ARG          # sign 2
+    ins[1].sT = -1
+    outs[0].T = None
+    assert L.pdh_bs_merge_dev(ins, 2, outs, 0, None) == pdh.PDH_E_ARG          # no T buffer
+    assert pdh.launch_count() == 0
