@@ -175,24 +175,33 @@ static SInt sadd(const SInt& a, const SInt& b) {
 }
 
 // ---- Newton reciprocal: about floor(2^(2L) / d), L = bitlen(d) -----------------
-// The top h bits of d give V_h ~ 2^(2h)/d_h; V0 = V_h << (L-h) has relative
-// error ~2^(1-h), and one Newton step V1 = V0 + V0 (2^(2L) - d V0) / 2^(2L)
-// squares it. With h = L/2 + 2 the result is within a few units; callers
-// correct exactly.
+// One Newton step V = V0 + V0 (2^(2L) - d V0) / 2^(2L) from V0 = V0h 2^sh
+// (sh <= L - 2) squares V0's relative error. e = 2^(2L) - d V0 is formed at
+// scale 2^sh, e_s = |2^(2L - sh) - d V0h|, and only e_s's bits from 2^k up,
+// k = L - 2 - sh, enter the correction: the dropped part moves V by
+// V0h e_lo 2^(2 sh) / 2^(2L) < 2^(L + 1 - sh + k + 2 sh - 2L) = 1/2.
+static Mag recip_step(const Mag& d, size_t L, const Mag& V0h, size_t sh, const Multiplier& M) {
+    Mag P = M.mul(d, V0h);
+    Mag two = pow2(2 * L - sh);
+    const bool neg = cmp(P, two) > 0;
+    Mag e = neg ? sub(P, two) : sub(two, P);
+    const size_t k = L - 2 - sh;
+    Mag corr = shr(M.mul(V0h, shr(e, k)), L + 2 - sh);
+    Mag V0 = shl(V0h, sh);
+    return neg ? sub(V0, corr) : add(V0, corr);
+}
+
+// The top h = L/2 + 8 bits of d give V_h ~ 2^(2h)/d_h; V0 = V_h 2^(L-h) has
+// relative error ~2^(3-h), and one step takes it to within about two units
+// (the truncations); callers correct exactly.
 static Mag recip(const Mag& d, const Multiplier& M) {
     const size_t L = bitlen(d);
     if (L <= 62) {
         u128 v = ((u128)1 << (2 * L)) / (u128)to_u128(d);
         return from_u128(v);
     }
-    const size_t h = L / 2 + 2;
-    Mag V0 = shl(recip(shr(d, L - h), M), L - h);
-    Mag P = M.mul(d, V0);
-    Mag two = pow2(2 * L);
-    bool neg = cmp(P, two) > 0;
-    Mag e = neg ? sub(P, two) : sub(two, P);
-    Mag corr = shr(M.mul(V0, e), 2 * L);
-    return neg ? sub(V0, corr) : add(V0, corr);
+    const size_t h = L / 2 + 8;
+    return recip_step(d, L, recip(shr(d, L - h), M), L - h, M);
 }
 
 // ---- exact floor division --------------------------------------------------------
@@ -232,6 +241,40 @@ static Mag div_floor(const Mag& a, const Mag& d, const Multiplier& M, Mag* rem) 
 }
 
 // ---- exact floor square root -------------------------------------------------------
+// Y ~ 2^E / sqrt(s), E = bitlen(s) rounded up to even: an (E/2 + 1)-bit
+// integer within a few units, by Newton on the inverse square root with no
+// division. From Y_h for the top E_h ~ E/2 + 32 bits of s (s_h = s >> 2t,
+// E_h = E - 2t), Y0 = Y_h 2^t has relative error ~2^(-E/4 - 13), and
+//     Y = Y0 + Y0 (2^(2E) - s Y0^2) / 2^(2E + 1)
+// squares it. Only what that step can see is formed: s without its low
+// b = E/2 - 8 bits (s_lo Y0^2 < 2^(3E/2 - 6)), and e = 2^(2E) - s Y0^2 from
+// 2^c up, c = 3E/2 - 2 (the dropped part moves Y by less than 1/2).
+static Mag rsqrt_approx(const Mag& s, const Multiplier& M) {
+    const size_t L = bitlen(s);
+    const size_t E = L + (L & 1);
+    if (E <= 100) {
+        long double v = 0;
+        for (size_t i = s.size(); i-- > 0;) v = v * 4294967296.0L + (long double)s[i];
+        return from_u128((u128)(std::ldexp(1.0L, (int)E) / std::sqrt(v)));
+    }
+    size_t Eh = E / 2 + 32;
+    Eh += Eh & 1;
+    const size_t t = (E - Eh) / 2;
+    const Mag Y0h = rsqrt_approx(shr(s, 2 * t), M);
+    const Mag Y0 = shl(Y0h, t);
+    const size_t b = E / 2 - 8, c = 3 * E / 2 - 2;
+    Mag sy2 = M.mul(shr(s, b), M.sqr(Y0));        // s Y0^2 / 2^b, s truncated
+    Mag two = pow2(2 * E - b);
+    const bool neg = cmp(sy2, two) > 0;
+    Mag e = neg ? sub(sy2, two) : sub(two, sy2);
+    Mag corr = shr(M.mul(Y0, shr(e, c - b)), 2 * E + 1 - c);
+    return neg ? sub(Y0, corr) : add(Y0, corr);
+}
+
+// x0 = s Y / 2^(E + G) from Y ~ 2^(E + G) / sqrt(s) (the inverse square root
+// of s 2^(2G), G = 8 guard bits) is within about one unit; then
+// r = s - x^2 moves x to the floor root with r updated by 2x -+ 1 per step
+// (0 <= r <= 2x at the floor root), so only one square is formed.
 static Mag isqrt(const Mag& s, const Multiplier& M) {
     const size_t L = bitlen(s);
     if (L <= 100) {
@@ -241,14 +284,23 @@ static Mag isqrt(const Mag& s, const Multiplier& M) {
         while ((x + 1) * (x + 1) <= v) ++x;
         return from_u128(x);
     }
-    const size_t t = (L - 64) / 4;
-    Mag x = add(shl(isqrt(shr(s, 2 * t), M), t), pow2(t));   // at or above sqrt(s)
-    x = shr(add(x, div_floor(s, x, M, nullptr)), 1);          // one Newton step
-    while (cmp(M.sqr(x), s) > 0) dec(x);
-    for (;;) {
-        Mag x1 = x; inc(x1);
-        if (cmp(M.sqr(x1), s) > 0) break;
-        x = x1;
+    const size_t G = 8;
+    const size_t E = L + (L & 1);
+    Mag Y = rsqrt_approx(shl(s, 2 * G), M);
+    const size_t b = E / 2 - 2;                   // s_lo Y / 2^(E + G) < 1/2
+    Mag x = shr(M.mul(shr(s, b), Y), E + G - b);
+    Mag x2 = M.sqr(x);
+    SInt r = cmp(s, x2) >= 0 ? SInt{sub(s, x2), 1} : SInt{sub(x2, s), -1};
+    if (r.m.empty()) r.s = 0;
+    while (r.s < 0) {                             // s - (x-1)^2 = r + 2x - 1
+        r = sadd(r, SInt{sub(shl(x, 1), Mag{1}), 1});
+        dec(x);
+    }
+    for (;;) {                                    // s - (x+1)^2 = r - 2x - 1
+        Mag two_x = shl(x, 1);
+        if (r.s == 0 || cmp(r.m, two_x) <= 0) break;
+        r = sadd(r, SInt{add(two_x, Mag{1}), -1});
+        inc(x);
     }
     return x;
 }
@@ -265,9 +317,12 @@ static Mag pow10(uint64_t e, const Multiplier& M) {
 
 // ---- divide-and-conquer decimal conversion ---------------------------------------
 // Widths are W = B * 2^K digits; level l splits each node of width B*2^(K-l)
-// by 10^(B*2^(K-l-1)) with one reciprocal per level and the level's
-// quotient/remainder products batched. Leaves of B digits convert on the host
-// by repeated division by 10^9 (no big multiplies).
+// by D_l = 10^(B*2^(K-l-1)) with one reciprocal per level and the level's
+// quotient/remainder products batched. The divisors come bottom-up, D_l =
+// D_{l+1}^2, and so do their reciprocals: 1/D_l = (1/D_{l+1})^2, so V_{l+1}^2
+// carries about half of V_l's bits and one Newton step completes it. Leaves
+// of B digits convert on the host by repeated division by 10^9 (no big
+// multiplies).
 static void leaf_digits(Mag x, size_t width, char* out) {
     std::vector<uint32_t> chunks;                // base 10^9, least significant first
     while (!x.empty()) {
@@ -295,13 +350,27 @@ static std::string to_decimal(const Mag& x, const Multiplier& M) {
     const size_t B = ((need + ((size_t)1 << K) - 1) / ((size_t)1 << K) + 8) / 9 * 9;   // leaf width, multiple of 9
     const size_t W = B << K;
     std::string out(W, '0');
+    // Ds[l] = 10^(B 2^(K-1-l)); Vs[l] ~ 2^(2L) / (Ds[l] 2^32), L = Ld + 32, i.e.
+    // 2^(2 Ld + 32) / Ds[l]. Vs[l+1]^2 / 2^z with z = 4 Ld[l+1] + 32 - 2 Ld[l]
+    // approximates Vs[l] to about L/2 + 14 bits; its top L/2 + 16 bits seed the step.
+    std::vector<Mag> Ds(K), Vs(K);
+    std::vector<size_t> Lds(K);
+    for (int l = K - 1; l >= 0; --l) {
+        Ds[l] = l == K - 1 ? pow10(B, M) : M.sqr(Ds[l + 1]);
+        Lds[l] = bitlen(Ds[l]);
+        const Mag d32 = shl(Ds[l], 32);
+        const size_t L = Lds[l] + 32;
+        if (l == K - 1) { Vs[l] = recip(d32, M); continue; }
+        const size_t z = 4 * Lds[l + 1] + 32 - 2 * Lds[l];
+        const size_t sh = L / 2 - 15;
+        Vs[l] = recip_step(d32, L, shr(M.sqr(Vs[l + 1]), z + sh), sh, M);
+    }
     std::vector<Mag> nodes{x};
     for (int lvl = 0; lvl < K; ++lvl) {
-        const size_t half = (B << (K - lvl)) / 2;
-        Mag D = pow10(half, M);
-        const size_t Ld = bitlen(D);
-        // V ~ 2^(2 (Ld+32)) / (D 2^32): q ~ x V / 2^(2 Ld + 32), x < 10^(2 half)
-        Mag V = recip(shl(D, 32), M);
+        const Mag& D = Ds[lvl];
+        const size_t Ld = Lds[lvl];
+        // q ~ x V / 2^(2 Ld + 32), x < 10^(2 half)
+        const Mag& V = Vs[lvl];
         std::vector<Mag> xs(nodes.size());
         std::vector<std::pair<const Mag*, const Mag*>> pairs;
         for (size_t i = 0; i < nodes.size(); ++i) xs[i] = nodes[i];

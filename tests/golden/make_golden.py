@@ -9,6 +9,14 @@ reference checkout; the GPU tests read only the files it writes.
     python tests/golden/make_golden.py --part mul          # C1/C2 products (~1 min)
     python tests/golden/make_golden.py --part bbp-range    # positions 1..65536 (~2 min, 8 threads)
     python tests/golden/make_golden.py --part bbp-sampled --count 64
+    python tests/golden/make_golden.py --part bbp-sampled-4096   # ~30 min, 8 threads
+
+The 4,096-position set (positions uniform in [10^6, 10^8], seed 401, the
+64-position set is its prefix) would take the reference's bbp_cpu about 36
+core-hours, so bbp_fast.c computes it: the reference's FP64 accumulation in
+the reference's order, with the exact residues 16^(d-k) mod m from a
+vectorized Montgomery ladder. It is first checked against the reference's
+own digits for positions 1..65,536 and the 64 sampled positions.
     python tests/golden/make_golden.py --part pi-data
 """
 from __future__ import annotations
@@ -84,6 +92,53 @@ def part_bbp_sampled(g: dict, count: int, threads: int) -> None:
     print("bbp sampled", count, f"{time.time() - t:.1f}s", flush=True)
 
 
+def build_bbp_fast(tmp: str):
+    import ctypes
+    import subprocess
+    so = os.path.join(tmp, "libbbpfast.so")
+    subprocess.run(["gcc", "-O3", "-march=native", "-ffp-contract=off", "-fPIC", "-shared", "-pthread",
+                    "-o", so, os.path.join(HERE, "bbp_fast.c"), "-lm"], check=True)
+    L = ctypes.CDLL(so)
+    L.bbp_fast_positions.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_int]
+    L.bbp_fast_positions.restype = ctypes.c_int
+
+    def hexdigits(pos, threads):
+        pos = np.ascontiguousarray(pos, dtype=np.uint64)
+        v = np.zeros(len(pos), np.uint8)
+        assert L.bbp_fast_positions(pos.ctypes.data, len(pos), v.ctypes.data, threads) == 0
+        return "".join("0123456789ABCDEF"[x] for x in v)
+    return hexdigits
+
+
+def part_bbp_sampled_4096(g: dict, threads: int) -> None:
+    import tempfile
+    out = os.path.join(HERE, "bbp_sampled_4096.json")
+    partial = out + ".partial"
+    with tempfile.TemporaryDirectory() as tmp:
+        fast = build_bbp_fast(tmp)
+        t = time.time()
+        rng = open(os.path.join(HERE, g["bbp_range"]["file"])).read().strip()
+        assert fast(np.arange(1, 65537, dtype=np.uint64), threads) == rng, "bbp_fast differs on 1..65536"
+        s64 = g["bbp_sampled"]
+        assert fast(s64["positions"], threads) == s64["hex"], "bbp_fast differs on the sampled fixture"
+        print("bbp_fast agrees with the reference's digits", f"{time.time() - t:.1f}s", flush=True)
+        pos = W.sampled_positions(401, 4096)
+        assert [int(x) for x in pos[:64]] == s64["positions"]
+        done = open(partial).read().strip() if os.path.exists(partial) else ""
+        for lo in range(len(done), len(pos), 256):
+            t = time.time()
+            done += fast(pos[lo:lo + 256], threads)
+            with open(partial, "w") as f:
+                f.write(done)
+            print("bbp sampled", len(done), "/", len(pos), f"{time.time() - t:.1f}s", flush=True)
+    assert done[:64] == s64["hex"]
+    with open(out, "w") as f:
+        json.dump({"seed": 401, "count": len(pos), "lo": W.C4_LO, "hi": W.C4_HI, "generator": "bbp_fast.c",
+                   "positions": [int(x) for x in pos], "hex": done}, f)
+        f.write("\n")
+    os.remove(partial)
+
+
 def part_pi_data(g: dict) -> None:
     text = open(os.path.join(REFERENCE_DIR, "include", "pi_reference.h")).read()
 
@@ -100,12 +155,15 @@ def part_pi_data(g: dict) -> None:
 
 def main() -> None:
     ap = argparse.ArgumentParser()
-    ap.add_argument("--part", required=True, choices=["mul", "bbp-range", "bbp-sampled", "pi-data"])
+    ap.add_argument("--part", required=True, choices=["mul", "bbp-range", "bbp-sampled", "bbp-sampled-4096", "pi-data"])
     ap.add_argument("--count", type=int, default=64)
     ap.add_argument("--threads", type=int, default=os.cpu_count() or 1)
     args = ap.parse_args()
-    assert oracle.ref_available(), "build oracle/_ref first: make -C oracle ref"
     g = load()
+    if args.part == "bbp-sampled-4096":
+        part_bbp_sampled_4096(g, args.threads)
+        return
+    assert oracle.ref_available(), "build oracle/_ref first: make -C oracle ref"
     if args.part == "mul":
         part_mul(g)
     elif args.part == "bbp-range":
