@@ -261,9 +261,11 @@ hipError_t launch_bs_groups(uint64_t n_terms, int G, uint32_t* dP, uint32_t* dQ,
     return e;
 }
 
-// ---- signed merge of T above the groups: T = sa A + sb B ------------------------
-// A = T_L Q_R and B = P_L T_R come from the batched multiplies; their signs are
-// the children's T signs. Every node is cut into tiles of SA_NT x SA_K limbs
+// ---- signed sums T = sa A + sb B of many magnitude pairs ---------------------------
+// Used for the merge of T above the groups (A = T_L Q_R and B = P_L T_R from
+// the batched multiplies, with the children's T signs; T overwrites A) and for
+// remainders elsewhere. T may alias A: every limb is read and written by the
+// same thread. Every node is cut into tiles of SA_NT x SA_K limbs
 // (one workgroup each; thread k owns SA_K consecutive limbs). A segment of
 // limbs has a carry status for the sum, or a borrow status for the difference
 // X - Y: GEN (carry/borrow out whatever comes in), PROP (out iff in), KILL
@@ -361,7 +363,7 @@ __device__ __forceinline__ SaView sa_view(const BsAddNode& N, uint32_t nd, uint3
 
 __global__ __launch_bounds__(SA_NT) void k_bs_add_status(const BsAddNode* __restrict__ nodes,
                                                          const uint32_t* __restrict__ tile_node,
-                                                         uint8_t* __restrict__ status, uint64_t* __restrict__ len) {
+                                                         uint8_t* __restrict__ status) {
     __shared__ uint32_t wv[SA_NT / 64];
     const uint32_t t = blockIdx.x;
     const uint32_t nd = tile_node[t];
@@ -369,19 +371,18 @@ __global__ __launch_bounds__(SA_NT) void k_bs_add_status(const BsAddNode* __rest
     const SaView v = sa_view(N, nd, t);
     const uint64_t base = (uint64_t)v.j * BS_ADD_TILE + (uint64_t)threadIdx.x * SA_K;
     uint32_t x[SA_K], y[SA_K];
-    sa_load(N.T, v.la, N.B, v.lb, base, x, y);
+    sa_load(N.A, v.la, N.B, v.lb, base, x, y);
     uint32_t agg;
     (void)block_status_in(seg_status(x, y, v.sub), wv, &agg);
     if (threadIdx.x == 0) {
         status[t] = (uint8_t)agg;
-        if (v.j == 0) len[nd] = 0;
+        if (v.j == 0) *N.len = 0;
     }
 }
 
 __global__ __launch_bounds__(SA_NT) void k_bs_add_apply(const BsAddNode* __restrict__ nodes,
                                                         const uint32_t* __restrict__ tile_node,
-                                                        const uint8_t* __restrict__ status, uint64_t* __restrict__ len,
-                                                        int32_t* __restrict__ sign) {
+                                                        const uint8_t* __restrict__ status) {
     __shared__ uint32_t wv[SA_NT / 64];
     __shared__ uint32_t red[2][SA_NT / 64];
     const uint32_t t = blockIdx.x;
@@ -408,8 +409,8 @@ __global__ __launch_bounds__(SA_NT) void k_bs_add_apply(const BsAddNode* __restr
     if (swap && tin != ST_PROP) tin = ST_GEN + ST_KILL - tin;
     const uint64_t base = (uint64_t)v.j * BS_ADD_TILE + (uint64_t)threadIdx.x * SA_K;
     uint32_t x[SA_K], y[SA_K];
-    if (swap) sa_load(N.B, v.lb, N.T, v.la, base, x, y);
-    else sa_load(N.T, v.la, N.B, v.lb, base, x, y);
+    if (swap) sa_load(N.B, v.lb, N.A, v.la, base, x, y);
+    else sa_load(N.A, v.la, N.B, v.lb, base, x, y);
     uint32_t agg;
     const uint32_t in = block_status_in(seg_status(x, y, v.sub), wv, &agg);
     const uint32_t st_in = in != ST_PROP ? in : tin;
@@ -439,23 +440,23 @@ __global__ __launch_bounds__(SA_NT) void k_bs_add_apply(const BsAddNode* __restr
         const uint64_t u = __shfl_xor(hi, o);
         hi = u > hi ? u : hi;
     }
-    if ((threadIdx.x & 63) == 0 && hi) atomicMax((unsigned long long*)&len[nd], (unsigned long long)hi);
-    if (v.j == 0 && threadIdx.x == 0) sign[nd] = !v.sub ? (N.sa ? N.sa : N.sb) : (swap ? N.sb : N.sa);
+    if ((threadIdx.x & 63) == 0 && hi) atomicMax((unsigned long long*)N.len, (unsigned long long)hi);
+    if (v.j == 0 && threadIdx.x == 0) *N.sign = !v.sub ? (N.sa ? N.sa : N.sb) : (swap ? N.sb : N.sa);
 }
 
 } // namespace
 
 hipError_t launch_bs_signed_add(const BsAddNode* d_nodes, const uint32_t* d_tile_node, uint32_t ntiles,
-                                uint8_t* d_status, uint64_t* d_len, int32_t* d_sign, hipStream_t s) {
+                                uint8_t* d_status, hipStream_t s) {
     if (ntiles == 0) return hipSuccess;
     hipEvent_t t0 = timing_begin(s);
-    hipLaunchKernelGGL(k_bs_add_status, dim3(ntiles), dim3(SA_NT), 0, s, d_nodes, d_tile_node, d_status, d_len);
+    hipLaunchKernelGGL(k_bs_add_status, dim3(ntiles), dim3(SA_NT), 0, s, d_nodes, d_tile_node, d_status);
     g_launches.fetch_add(1, std::memory_order_relaxed);
     hipError_t e = hipGetLastError();
     timing_end("k_bs_add_status", t0, s);
     if (e != hipSuccess) return e;
     t0 = timing_begin(s);
-    hipLaunchKernelGGL(k_bs_add_apply, dim3(ntiles), dim3(SA_NT), 0, s, d_nodes, d_tile_node, d_status, d_len, d_sign);
+    hipLaunchKernelGGL(k_bs_add_apply, dim3(ntiles), dim3(SA_NT), 0, s, d_nodes, d_tile_node, d_status);
     g_launches.fetch_add(1, std::memory_order_relaxed);
     e = hipGetLastError();
     timing_end("k_bs_add_apply", t0, s);

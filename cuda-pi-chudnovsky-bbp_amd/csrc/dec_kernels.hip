@@ -1,0 +1,98 @@
+// ============================================================================
+//  dec_kernels.hip -- device building blocks of divide-and-conquer decimal
+//  conversion (the reference converts with repeated division by 10^9 over the
+//  whole number, src/bignum_div.cpp:239-267): batched right shifts of
+//  magnitudes (the quotient estimates q = x V >> s of every node of a level)
+//  and the decimal digits of the leaves.
+// ============================================================================
+#include "pdh_internal.h"
+
+namespace pdh {
+
+namespace {
+
+constexpr int SH_NT = 256;
+constexpr int SH_K = SHR_TILE / SH_NT;
+static_assert(SH_NT * SH_K == SHR_TILE, "tile size");
+
+// out[j] = limbs j + w and j + w + 1 of a joined and shifted right by s
+// (bits = 32 w + s). The output's top candidate is j = la - w - 1: it is
+// nonzero unless a's top limb is below 2^s, and then limb j - 1 holds that
+// top limb's bits, so the normalized length needs no reduction.
+__global__ __launch_bounds__(SH_NT) void k_shr_batch(const ShrNode* __restrict__ nodes,
+                                                     const uint32_t* __restrict__ tile_node) {
+    const uint32_t t = blockIdx.x;
+    const ShrNode N = nodes[tile_node[t]];
+    const uint64_t la = *N.a_len;
+    const uint64_t w = N.bits >> 5;
+    const uint32_t s = (uint32_t)(N.bits & 31);
+    const uint64_t base = (uint64_t)(t - N.tile0) * SHR_TILE;
+    for (int k = 0; k < SH_K; ++k) {
+        const uint64_t j = base + (uint64_t)k * SH_NT + threadIdx.x;
+        if (j >= N.out_cap) break;
+        const uint64_t i = j + w;
+        const uint32_t lo = i < la ? N.a[i] : 0u;
+        const uint32_t hi = i + 1 < la ? N.a[i + 1] : 0u;
+        const uint32_t v = s ? (lo >> s) | (hi << (32 - s)) : lo;
+        N.out[j] = v;
+        if (la > w && j == la - w - 1) *N.out_len = v ? j + 1 : j;
+    }
+    if (t == N.tile0 && threadIdx.x == 0) {
+        if (la <= w) *N.out_len = 0;
+        else if (la - w - 1 >= N.out_cap) *N.out_len = N.out_cap;      // does not fit: truncated
+    }
+}
+
+// One thread per value: nine digits per round from the right end, each
+// round one pass of division by 10^9 from the top limb down (the value is
+// its own scratch).
+__global__ __launch_bounds__(64) void k_decimal_leaves(uint32_t* __restrict__ x, uint64_t stride,
+                                                       const uint64_t* __restrict__ x_len, uint64_t n,
+                                                       uint32_t width, char* __restrict__ out) {
+    const uint64_t i = (uint64_t)blockIdx.x * 64 + threadIdx.x;
+    if (i >= n) return;
+    uint32_t* v = x + i * stride;
+    uint64_t len = x_len[i];
+    char* o = out + i * width;
+    for (uint32_t pos = width; pos >= 9; pos -= 9) {
+        uint64_t r = 0;
+        for (uint64_t k = len; k-- > 0;) {
+            const uint64_t cur = (r << 32) | v[k];
+            const uint64_t q = cur / 1000000000u;
+            v[k] = (uint32_t)q;
+            r = cur - q * 1000000000u;
+        }
+        while (len && v[len - 1] == 0) --len;
+        uint32_t r32 = (uint32_t)r;
+        for (int d = 1; d <= 9; ++d) {
+            o[pos - d] = (char)('0' + r32 % 10u);
+            r32 /= 10u;
+        }
+    }
+}
+
+} // namespace
+
+hipError_t launch_shr_batch(const ShrNode* d_nodes, const uint32_t* d_tile_node, uint32_t ntiles, hipStream_t s) {
+    if (ntiles == 0) return hipSuccess;
+    hipEvent_t t0 = timing_begin(s);
+    hipLaunchKernelGGL(k_shr_batch, dim3(ntiles), dim3(SH_NT), 0, s, d_nodes, d_tile_node);
+    g_launches.fetch_add(1, std::memory_order_relaxed);
+    hipError_t e = hipGetLastError();
+    timing_end("k_shr_batch", t0, s);
+    return e;
+}
+
+hipError_t launch_decimal_leaves(uint32_t* x, uint64_t stride, const uint64_t* x_len, uint64_t n, uint32_t width,
+                                 char* out, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipEvent_t t0 = timing_begin(s);
+    hipLaunchKernelGGL(k_decimal_leaves, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, x, stride, x_len, n, width,
+                       out);
+    g_launches.fetch_add(1, std::memory_order_relaxed);
+    hipError_t e = hipGetLastError();
+    timing_end("k_decimal_leaves", t0, s);
+    return e;
+}
+
+} // namespace pdh

@@ -810,7 +810,7 @@ int pdh_bs_merge_dev(const pdh_bs_node* in, size_t n_in, pdh_bs_node* out, int s
         jobs.push_back(pdh_mul_desc{L.P, L.nP, R.T, eff_nT(R), dB + boff[i], li + 2});
         if (!skip_p) jobs.push_back(pdh_mul_desc{L.P, L.nP, R.P, R.nP, out[i].P, li + 3});
         const uint32_t nt = (uint32_t)((capT[i] + BS_ADD_TILE - 1) / BS_ADD_TILE);
-        an[i] = BsAddNode{out[i].T, dB + boff[i], li + 1, li + 2, capT[i], tile, nt,
+        an[i] = BsAddNode{out[i].T, out[i].T, dB + boff[i], li + 1, li + 2, capT[i], dolen + i, dosgn + i, tile, nt,
                           eff_nT(L) ? L.sT : 0, eff_nT(R) ? R.sT : 0};
         for (uint32_t k = 0; k < nt; ++k) tmap[tile + k] = (uint32_t)i;
         tile += nt;
@@ -826,7 +826,7 @@ int pdh_bs_merge_dev(const pdh_bs_node* in, size_t n_in, pdh_bs_node* out, int s
         {
             std::lock_guard<std::mutex> g(c.mu);
             e = launch_bs_signed_add((const BsAddNode*)(ws + o_nodes), (const uint32_t*)(ws + o_tmap), tile,
-                                     (uint8_t*)(ws + o_stat), dolen, dosgn, s);
+                                     (uint8_t*)(ws + o_stat), s);
         }
         if (e != hipSuccess) { cleanup(); return fail_hip(e, "pdh_bs_merge_dev: signed add"); }
     }
@@ -857,6 +857,119 @@ int pdh_bs_merge_dev(const pdh_bs_node* in, size_t n_in, pdh_bs_node* out, int s
     }
     cleanup();
     return PDH_OK;
+}
+
+} // extern "C"
+
+namespace {
+
+// Uploads a node table (nodes, then the tile -> node map) into one device
+// allocation, runs `launch` on it, waits for the stream and frees it.
+template <class Node, class Launch>
+int run_tiled(const std::vector<Node>& nodes, const std::vector<uint32_t>& tmap, size_t extra, hipStream_t s,
+              const char* what, Launch launch) {
+    auto al256 = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t o_tmap = al256(nodes.size() * sizeof(Node));
+    const size_t o_extra = o_tmap + al256(tmap.size() * 4);
+    std::vector<char> stage(o_extra);
+    std::memcpy(stage.data(), nodes.data(), nodes.size() * sizeof(Node));
+    std::memcpy(stage.data() + o_tmap, tmap.data(), tmap.size() * 4);
+    char* ws = nullptr;
+    hipError_t e = hipMalloc(&ws, o_extra + al256(extra));
+    if (e != hipSuccess) return fail_hip(e, what);
+    e = hipMemcpyAsync(ws, stage.data(), stage.size(), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) {
+        Ctx& c = ctx();
+        std::lock_guard<std::mutex> g(c.mu);
+        e = launch((const Node*)ws, (const uint32_t*)(ws + o_tmap), (uint32_t)tmap.size(), ws + o_extra);
+    }
+    hipError_t e2 = hipStreamSynchronize(s);
+    (void)hipFree(ws);
+    if (e == hipSuccess) e = e2;
+    return e == hipSuccess ? PDH_OK : fail_hip(e, what);
+}
+
+int dev_ready() {
+    Ctx& c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    return init_locked(c);
+}
+
+} // namespace
+
+extern "C" {
+
+int pdh_sadd_batch_dev(const pdh_sadd_desc* d, size_t n, void* stream) {
+    if (n == 0) return PDH_OK;
+    if (!d) return fail(PDH_E_ARG, "null pointer argument");
+    uint64_t ntiles = 0;
+    for (size_t i = 0; i < n; ++i) {
+        const pdh_sadd_desc& x = d[i];
+        if (x.sa < -1 || x.sa > 1 || x.sb < -1 || x.sb > 1 || (x.sa && (!x.a || !x.a_len)) ||
+            (x.sb && (!x.b || !x.b_len)) || !x.out || x.out_cap == 0 || !x.out_len || !x.out_sign)
+            return fail(PDH_E_ARG, "pdh_sadd_batch_dev: bad descriptor");
+        ntiles += (x.out_cap + BS_ADD_TILE - 1) / BS_ADD_TILE;
+    }
+    if (ntiles > 0xFFFFFFFFull) return fail(PDH_E_TOO_LARGE, "pdh_sadd_batch_dev: too many tiles");
+    int rc = dev_ready();
+    if (rc) return rc;
+    std::vector<BsAddNode> nodes(n);
+    std::vector<uint32_t> tmap;
+    tmap.reserve(ntiles);
+    for (size_t i = 0; i < n; ++i) {
+        const pdh_sadd_desc& x = d[i];
+        const uint32_t nt = (uint32_t)((x.out_cap + BS_ADD_TILE - 1) / BS_ADD_TILE);
+        nodes[i] = BsAddNode{x.out, x.a, x.b, x.a_len, x.b_len, x.out_cap, x.out_len, x.out_sign,
+                             (uint32_t)tmap.size(), nt, x.sa, x.sb};
+        tmap.insert(tmap.end(), nt, (uint32_t)i);
+    }
+    hipStream_t s = (hipStream_t)stream;
+    return run_tiled(nodes, tmap, tmap.size(), s, "pdh_sadd_batch_dev",
+                     [&](const BsAddNode* dn, const uint32_t* dt, uint32_t nt, char* extra) {
+                         return launch_bs_signed_add(dn, dt, nt, (uint8_t*)extra, s);
+                     });
+}
+
+int pdh_shr_batch_dev(const pdh_shr_desc* d, size_t n, void* stream) {
+    if (n == 0) return PDH_OK;
+    if (!d) return fail(PDH_E_ARG, "null pointer argument");
+    uint64_t ntiles = 0;
+    for (size_t i = 0; i < n; ++i) {
+        const pdh_shr_desc& x = d[i];
+        if (!x.a || !x.a_len || !x.out || x.out_cap == 0 || !x.out_len)
+            return fail(PDH_E_ARG, "pdh_shr_batch_dev: bad descriptor");
+        ntiles += (x.out_cap + SHR_TILE - 1) / SHR_TILE;
+    }
+    if (ntiles > 0xFFFFFFFFull) return fail(PDH_E_TOO_LARGE, "pdh_shr_batch_dev: too many tiles");
+    int rc = dev_ready();
+    if (rc) return rc;
+    std::vector<ShrNode> nodes(n);
+    std::vector<uint32_t> tmap;
+    tmap.reserve(ntiles);
+    for (size_t i = 0; i < n; ++i) {
+        const pdh_shr_desc& x = d[i];
+        const uint32_t nt = (uint32_t)((x.out_cap + SHR_TILE - 1) / SHR_TILE);
+        nodes[i] = ShrNode{x.a, x.a_len, x.out, x.out_len, x.out_cap, x.bits, (uint32_t)tmap.size(), nt};
+        tmap.insert(tmap.end(), nt, (uint32_t)i);
+    }
+    hipStream_t s = (hipStream_t)stream;
+    return run_tiled(nodes, tmap, 0, s, "pdh_shr_batch_dev",
+                     [&](const ShrNode* dn, const uint32_t* dt, uint32_t nt, char*) {
+                         return launch_shr_batch(dn, dt, nt, s);
+                     });
+}
+
+int pdh_decimal_leaves_dev(uint32_t* d_x, uint64_t stride, const uint64_t* d_len, uint64_t n, uint32_t width,
+                           char* d_out, void* stream) {
+    if (n == 0) return PDH_OK;
+    if (!d_x || !d_len || !d_out) return fail(PDH_E_ARG, "null pointer argument");
+    if (width == 0 || width % 9 || stride == 0) return fail(PDH_E_ARG, "pdh_decimal_leaves_dev: width must be a positive multiple of 9, stride positive");
+    Ctx& c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    int rc = init_locked(c);
+    if (rc) return rc;
+    hipError_t e = launch_decimal_leaves(d_x, stride, d_len, n, width, d_out, (hipStream_t)stream);
+    return e == hipSuccess ? PDH_OK : fail_hip(e, "decimal leaves launch");
 }
 
 int pdh_dev_alloc(size_t bytes, void** ptr) {

@@ -141,25 +141,48 @@ size_t bbp_few_ws_bytes(size_t count);
 hipError_t launch_bs_groups(uint64_t n_terms, int G, uint32_t* dP, uint32_t* dQ, uint32_t* dT, uint64_t* dlen,
                             int32_t* dsgn, hipStream_t s);
 
-// Signed merge of T above the groups (bs_kernels.hip): for every node,
-// T = sa A + sb B with A in T's own buffer (device length *la) and B at B
-// (length *lb); sa, sb in {-1, 0, 1} (0: the operand is zero). Nodes are cut
-// into BS_ADD_TILE-limb tiles numbered tile0 .. tile0 + ntiles - 1 in one flat
-// list (d_tile_node maps a tile to its node); T is written below cap limbs,
-// d_len[node] receives T's normalized length and d_sign[node] its sign
-// (meaningless when the length is 0). d_status: one byte per tile.
+// Signed sums (bs_kernels.hip): for every node, T = sa A + sb B with A
+// (device length *la) and B (length *lb); T may be A's own buffer; sa, sb in
+// {-1, 0, 1} (0: the operand is zero). Nodes are cut into BS_ADD_TILE-limb
+// tiles numbered tile0 .. tile0 + ntiles - 1 in one flat list (d_tile_node
+// maps a tile to its node); T is written below cap limbs (the sum must fit),
+// *len receives T's normalized length and *sign its sign (meaningless when
+// the length is 0). d_status: one byte per tile.
 constexpr uint32_t BS_ADD_TILE = 4096;
 struct BsAddNode {
     uint32_t* T;
+    const uint32_t* A;
     const uint32_t* B;
     const uint64_t* la;
     const uint64_t* lb;
     uint64_t cap;
+    uint64_t* len;
+    int32_t* sign;
     uint32_t tile0, ntiles;
     int32_t sa, sb;
 };
 hipError_t launch_bs_signed_add(const BsAddNode* d_nodes, const uint32_t* d_tile_node, uint32_t ntiles,
-                                uint8_t* d_status, uint64_t* d_len, int32_t* d_sign, hipStream_t s);
+                                uint8_t* d_status, hipStream_t s);
+
+// Decimal conversion helpers (dec_kernels.hip). Right shifts: out = a >> bits
+// for every node, over a flat list of SHR_TILE-limb tiles of the outputs
+// (limbs below out_cap written; *out_len normalized from *a_len, which must be
+// normalized). Leaves: for value i at x + i stride (normalized length
+// x_len[i], below 10^width, width a multiple of 9), width ASCII digits at
+// out + i width, most significant first; the values are overwritten.
+constexpr uint32_t SHR_TILE = 2048;
+struct ShrNode {
+    const uint32_t* a;
+    const uint64_t* a_len;
+    uint32_t* out;
+    uint64_t* out_len;
+    uint64_t out_cap;
+    uint64_t bits;
+    uint32_t tile0, ntiles;
+};
+hipError_t launch_shr_batch(const ShrNode* d_nodes, const uint32_t* d_tile_node, uint32_t ntiles, hipStream_t s);
+hipError_t launch_decimal_leaves(uint32_t* x, uint64_t stride, const uint64_t* x_len, uint64_t n, uint32_t width,
+                                 char* out, hipStream_t s);
 
 extern std::atomic<uint64_t> g_launches;
 

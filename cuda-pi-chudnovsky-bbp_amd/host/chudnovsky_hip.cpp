@@ -18,10 +18,12 @@
 //    do not depend on where the range is split.
 //  * Division is a Newton reciprocal with an exact floor correction (the
 //    reference forwards div_fast to Knuth, src/bignum_div.cpp:170-178); isqrt
-//    is Newton with divisions at doubling precision and an exact correction
+//    is a division-free Newton inverse square root and an exact correction
 //    (reference: Newton over Knuth divisions, :189-204).
-//  * to_decimal is divide-and-conquer on powers of ten with batched
-//    divisions per level (reference: O(d^2) repeated /10^9, :239-267).
+//  * to_decimal is divide-and-conquer on powers of ten (reference: O(d^2)
+//    repeated /10^9, :239-267): on the device, every level's quotients and
+//    remainders stay in HBM (batched products, shifts and signed sums) and
+//    one kernel writes the leaves' digits.
 //
 //  Multiplies go through a Multiplier (device by default). A caller may pass
 //  its own through pdh_pi_chudnovsky_cb -- the CPU tests do, with the oracle's
@@ -58,6 +60,9 @@ struct Multiplier {
     // lower levels of the tree computed elsewhere; one node = the root, whose
     // P may be empty); empty = from the leaves
     std::function<std::vector<PQT>(uint64_t N)> groups;
+    // optional: the decimal string of x > 0 (the levels run elsewhere); empty =
+    // the host loop below
+    std::function<std::string(const std::vector<uint32_t>& x, const Multiplier& M)> to_decimal;
 };
 
 // ---- magnitude helpers (little-endian base 2^32, normalized) -----------------
@@ -342,35 +347,57 @@ static void leaf_digits(Mag x, size_t width, char* out) {
     while (pos > 0) out[--pos] = '0';
 }
 
-static std::string to_decimal(const Mag& x, const Multiplier& M) {
-    if (x.empty()) return "0";
-    const size_t need = (size_t)((double)bitlen(x) * 0.30102999566398120) + 2;
+struct DecimalPlan {
     int K = 0;
-    while ((need >> K) > 2304) ++K;
-    const size_t B = ((need + ((size_t)1 << K) - 1) / ((size_t)1 << K) + 8) / 9 * 9;   // leaf width, multiple of 9
-    const size_t W = B << K;
-    std::string out(W, '0');
+    size_t B = 0, W = 0;
+    std::vector<Mag> Ds, Vs;
+    std::vector<size_t> Lds;
+};
+
+static DecimalPlan decimal_plan(const Mag& x, const Multiplier& M) {
+    DecimalPlan p;
+    const size_t need = (size_t)((double)bitlen(x) * 0.30102999566398120) + 2;
+    while ((need >> p.K) > 2304) ++p.K;
+    const int K = p.K;
+    p.B = ((need + ((size_t)1 << K) - 1) / ((size_t)1 << K) + 8) / 9 * 9;   // leaf width, multiple of 9
+    p.W = p.B << K;
     // Ds[l] = 10^(B 2^(K-1-l)); Vs[l] ~ 2^(2L) / (Ds[l] 2^32), L = Ld + 32, i.e.
     // 2^(2 Ld + 32) / Ds[l]. Vs[l+1]^2 / 2^z with z = 4 Ld[l+1] + 32 - 2 Ld[l]
     // approximates Vs[l] to about L/2 + 14 bits; its top L/2 + 16 bits seed the step.
-    std::vector<Mag> Ds(K), Vs(K);
-    std::vector<size_t> Lds(K);
+    p.Ds.resize(K);
+    p.Vs.resize(K);
+    p.Lds.resize(K);
     for (int l = K - 1; l >= 0; --l) {
-        Ds[l] = l == K - 1 ? pow10(B, M) : M.sqr(Ds[l + 1]);
-        Lds[l] = bitlen(Ds[l]);
-        const Mag d32 = shl(Ds[l], 32);
-        const size_t L = Lds[l] + 32;
-        if (l == K - 1) { Vs[l] = recip(d32, M); continue; }
-        const size_t z = 4 * Lds[l + 1] + 32 - 2 * Lds[l];
+        p.Ds[l] = l == K - 1 ? pow10(p.B, M) : M.sqr(p.Ds[l + 1]);
+        p.Lds[l] = bitlen(p.Ds[l]);
+        const Mag d32 = shl(p.Ds[l], 32);
+        const size_t L = p.Lds[l] + 32;
+        if (l == K - 1) { p.Vs[l] = recip(d32, M); continue; }
+        const size_t z = 4 * p.Lds[l + 1] + 32 - 2 * p.Lds[l];
         const size_t sh = L / 2 - 15;
-        Vs[l] = recip_step(d32, L, shr(M.sqr(Vs[l + 1]), z + sh), sh, M);
+        p.Vs[l] = recip_step(d32, L, shr(M.sqr(p.Vs[l + 1]), z + sh), sh, M);
     }
+    return p;
+}
+
+static std::string strip_zeros(const std::string& out) {
+    size_t nz = out.find_first_not_of('0');
+    return nz == std::string::npos ? "0" : out.substr(nz);
+}
+
+static std::string to_decimal(const Mag& x, const Multiplier& M) {
+    if (x.empty()) return "0";
+    if (M.to_decimal) return M.to_decimal(x, M);
+    const DecimalPlan p = decimal_plan(x, M);
+    const int K = p.K;
+    const size_t B = p.B;
+    std::string out(p.W, '0');
     std::vector<Mag> nodes{x};
     for (int lvl = 0; lvl < K; ++lvl) {
-        const Mag& D = Ds[lvl];
-        const size_t Ld = Lds[lvl];
+        const Mag& D = p.Ds[lvl];
+        const size_t Ld = p.Lds[lvl];
         // q ~ x V / 2^(2 Ld + 32), x < 10^(2 half)
-        const Mag& V = Vs[lvl];
+        const Mag& V = p.Vs[lvl];
         std::vector<Mag> xs(nodes.size());
         std::vector<std::pair<const Mag*, const Mag*>> pairs;
         for (size_t i = 0; i < nodes.size(); ++i) xs[i] = nodes[i];
@@ -400,8 +427,7 @@ static std::string to_decimal(const Mag& x, const Multiplier& M) {
             for (size_t i = t; i < nodes.size(); i += nth) leaf_digits(nodes[i], B, &out[i * B]);
         });
     for (auto& th : pool) th.join();
-    size_t nz = out.find_first_not_of('0');
-    return nz == std::string::npos ? "0" : out.substr(nz);
+    return strip_zeros(out);
 }
 
 // ---- binary splitting, bottom-up with batched levels ---------------------------------
@@ -690,12 +716,136 @@ static std::vector<PQT> dev_levels(uint64_t N) {
     return out;
 }
 
+// Divide-and-conquer decimal conversion in device memory. Level l holds 2^l
+// nodes of width W_l = B 2^(K-l) digits at a fixed stride C_l limbs (values
+// below 10^W_l, zero-padded, device lengths). Per level, with D = 10^(W_l/2)
+// and its reciprocal V from the plan: P = X V (one batch), Q = P >> (2 Ld + 32)
+// into the next level's even slots, QD = Q D (one batch), R = X - QD into the
+// odd slots and T = |R| - D (signed sums). Q is floor(X / D) when R >= 0 and
+// T < 0; otherwise (rare: the estimate is within a unit) the node is fixed
+// by steps of D on the host. The leaves' digits come from one kernel; only
+// the lengths, signs and the final W characters cross PCIe.
+static std::string dev_to_decimal(const Mag& x, const Multiplier& M) {
+    const DecimalPlan p = decimal_plan(x, M);
+    const int K = p.K;
+    const size_t B = p.B;
+    auto cap_of = [](uint64_t w) { return (uint64_t)((double)w * 3.3219280948873623 / 32.0) + 3; };
+    auto r64 = [](uint64_t v) { return (v + 63) & ~(uint64_t)63; };
+    void *st = nullptr, *cur = nullptr, *nxt = nullptr, *aux = nullptr, *dout = nullptr;
+    auto release = [&]() { pdh_dev_free(cur); pdh_dev_free(nxt); pdh_dev_free(aux); pdh_dev_free(dout);
+                           pdh_stream_destroy(st); };
+    std::string out(p.W, '0');
+    try {
+        check(pdh_stream_create(&st));
+        // a level: n nodes x C limbs, then n lengths
+        uint64_t n = 1, C = cap_of(p.W);
+        check(pdh_dev_alloc(r64(n * C) * 4 + n * 8, &cur));
+        {
+            std::vector<uint32_t> x0(C, 0);
+            std::copy(x.begin(), x.end(), x0.begin());
+            const uint64_t lx = x.size();
+            check(pdh_copy_htod(cur, x0.data(), C * 4, st));
+            check(pdh_copy_htod((uint32_t*)cur + r64(C), &lx, 8, st));
+        }
+        for (int l = 0; l < K; ++l) {
+            uint32_t* X = (uint32_t*)cur;
+            uint64_t* XL = (uint64_t*)(X + r64(n * C));
+            const Mag& D = p.Ds[l];
+            const Mag& V = p.Vs[l];
+            const uint64_t nD = D.size(), nV = V.size(), Cn = cap_of(p.W >> (l + 1));
+            // next level
+            check(pdh_dev_alloc(r64(2 * n * Cn) * 4 + 2 * n * 8, &nxt));
+            uint32_t* Xn = (uint32_t*)nxt;
+            uint64_t* XnL = (uint64_t*)(Xn + r64(2 * n * Cn));
+            // aux: D | V | P (n x (C + nV)) | QD (n x (Cn + nD)) | T (n x Cn) | lengths | signs
+            const uint64_t oV = r64(nD), oP = oV + r64(nV), oQD = oP + r64(n * (C + nV)),
+                           oT = oQD + r64(n * (Cn + nD)), oEnd = oT + r64(n * Cn);
+            check(pdh_dev_alloc(oEnd * 4 + (3 * n + 1) * 8 + 2 * n * 4, &aux));
+            uint32_t* A = (uint32_t*)aux;
+            uint32_t *dD = A, *dV = A + oV, *dP = A + oP, *dQD = A + oQD, *dT = A + oT;
+            uint64_t* PL = (uint64_t*)(A + oEnd);
+            uint64_t *QDL = PL + n, *TL = QDL + n, *DL = TL + n;
+            int32_t* RS = (int32_t*)(DL + 1);
+            int32_t* TS = RS + n;
+            check(pdh_copy_htod(dD, D.data(), nD * 4, st));
+            check(pdh_copy_htod(dV, V.data(), nV * 4, st));
+            check(pdh_copy_htod(DL, &nD, 8, st));
+            const uint64_t sh = 2 * p.Lds[l] + 32;
+            std::vector<pdh_mul_desc> md(n);
+            for (uint64_t i = 0; i < n; ++i)
+                md[i] = pdh_mul_desc{X + i * C, C, dV, nV, dP + i * (C + nV), PL + i};
+            check(pdh_mul_batch_dev(md.data(), n, st));
+            std::vector<pdh_shr_desc> sd(n);
+            for (uint64_t i = 0; i < n; ++i)
+                sd[i] = pdh_shr_desc{dP + i * (C + nV), PL + i, Xn + 2 * i * Cn, Cn, XnL + 2 * i, sh};
+            check(pdh_shr_batch_dev(sd.data(), n, st));
+            for (uint64_t i = 0; i < n; ++i)
+                md[i] = pdh_mul_desc{Xn + 2 * i * Cn, Cn, dD, nD, dQD + i * (Cn + nD), QDL + i};
+            check(pdh_mul_batch_dev(md.data(), n, st));
+            std::vector<pdh_sadd_desc> ad(n);
+            for (uint64_t i = 0; i < n; ++i)
+                ad[i] = pdh_sadd_desc{X + i * C, XL + i, 1, dQD + i * (Cn + nD), QDL + i, -1,
+                                      Xn + (2 * i + 1) * Cn, Cn, XnL + 2 * i + 1, RS + i};
+            check(pdh_sadd_batch_dev(ad.data(), n, st));
+            for (uint64_t i = 0; i < n; ++i)
+                ad[i] = pdh_sadd_desc{Xn + (2 * i + 1) * Cn, XnL + 2 * i + 1, 1, dD, DL, -1, dT + i * Cn, Cn, TL + i,
+                                      TS + i};
+            check(pdh_sadd_batch_dev(ad.data(), n, st));
+            std::vector<int32_t> sg(2 * n);
+            std::vector<uint64_t> ln(2 * n), tl(n);
+            check(pdh_copy_dtoh(sg.data(), RS, 2 * n * 4, st));
+            check(pdh_copy_dtoh(ln.data(), XnL, 2 * n * 8, st));
+            check(pdh_copy_dtoh(tl.data(), TL, n * 8, st));
+            check(pdh_stream_sync(st));
+            for (uint64_t i = 0; i < n; ++i) {
+                // signs are meaningful only for nonzero values: R < 0, or R > 0 and |R| - D >= 0
+                const bool rnz = ln[2 * i + 1] != 0;
+                const bool neg = rnz && sg[i] < 0;
+                const bool ge = rnz && sg[i] > 0 && (tl[i] == 0 || sg[n + i] > 0);
+                if (!neg && !ge) continue;
+                Mag q(ln[2 * i]), rm(ln[2 * i + 1]);
+                check(pdh_copy_dtoh(q.data(), Xn + 2 * i * Cn, q.size() * 4, st));
+                check(pdh_copy_dtoh(rm.data(), Xn + (2 * i + 1) * Cn, rm.size() * 4, st));
+                check(pdh_stream_sync(st));
+                SInt r{rm, sg[i]};
+                while (r.s < 0) { r = sadd(r, SInt{D, 1}); dec(q); }
+                while (r.s > 0 && cmp(r.m, D) >= 0) { r = sadd(r, SInt{D, -1}); inc(q); }
+                Mag qz(Cn, 0), rz(Cn, 0);
+                std::copy(q.begin(), q.end(), qz.begin());
+                std::copy(r.m.begin(), r.m.end(), rz.begin());
+                const uint64_t lens[2] = {q.size(), r.m.size()};
+                check(pdh_copy_htod(Xn + 2 * i * Cn, qz.data(), Cn * 4, st));
+                check(pdh_copy_htod(Xn + (2 * i + 1) * Cn, rz.data(), Cn * 4, st));
+                check(pdh_copy_htod(XnL + 2 * i, lens, 16, st));
+                check(pdh_stream_sync(st));
+            }
+            pdh_dev_free(aux); aux = nullptr;
+            pdh_dev_free(cur);
+            cur = nxt; nxt = nullptr;
+            n *= 2;
+            C = Cn;
+        }
+        uint32_t* X = (uint32_t*)cur;
+        uint64_t* XL = (uint64_t*)(X + r64(n * C));
+        check(pdh_dev_alloc(p.W, &dout));
+        check(pdh_decimal_leaves_dev(X, C, XL, n, (uint32_t)B, (char*)dout, st));
+        check(pdh_copy_dtoh(&out[0], dout, p.W, st));
+        check(pdh_stream_sync(st));
+    } catch (...) {
+        release();
+        throw;
+    }
+    release();
+    return strip_zeros(out);
+}
+
 Multiplier device_multiplier() {
     Multiplier M;
     M.mul = dev_mul;
     M.sqr = dev_sqr;
     M.mul_many = dev_mul_many;
     M.groups = dev_levels;
+    M.to_decimal = dev_to_decimal;
     return M;
 }
 

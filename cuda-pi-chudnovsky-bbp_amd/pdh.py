@@ -39,6 +39,7 @@ EXPORTED = (
     "pdh_tune_get", "pdh_tune_set", "pdh_autotune", "pdh_dev_alloc", "pdh_dev_free", "pdh_copy_htod",
     "pdh_copy_dtoh", "pdh_stream_create", "pdh_stream_destroy", "pdh_stream_sync",
     "pdh_bbp_set_exact", "pdh_bbp_exact_count", "pdh_tune_engine", "pdh_bs_groups", "pdh_bs_merge_dev",
+    "pdh_sadd_batch_dev", "pdh_shr_batch_dev", "pdh_decimal_leaves_dev",
 )
 
 
@@ -53,6 +54,20 @@ class BsNode(ctypes.Structure):
     _fields_ = [("P", ctypes.c_void_p), ("nP", ctypes.c_uint64), ("Q", ctypes.c_void_p),
                 ("nQ", ctypes.c_uint64), ("T", ctypes.c_void_p), ("nT", ctypes.c_uint64),
                 ("sT", ctypes.c_int32)]
+
+
+class SaddDesc(ctypes.Structure):
+    """pdh_sadd_desc: out = sa*a + sb*b on device magnitudes with device lengths."""
+    _fields_ = [("a", ctypes.c_void_p), ("a_len", ctypes.c_void_p), ("sa", ctypes.c_int32),
+                ("b", ctypes.c_void_p), ("b_len", ctypes.c_void_p), ("sb", ctypes.c_int32),
+                ("out", ctypes.c_void_p), ("out_cap", ctypes.c_uint64),
+                ("out_len", ctypes.c_void_p), ("out_sign", ctypes.c_void_p)]
+
+
+class ShrDesc(ctypes.Structure):
+    """pdh_shr_desc: out = a >> bits on device magnitudes with device lengths."""
+    _fields_ = [("a", ctypes.c_void_p), ("a_len", ctypes.c_void_p), ("out", ctypes.c_void_p),
+                ("out_cap", ctypes.c_uint64), ("out_len", ctypes.c_void_p), ("bits", ctypes.c_uint64)]
 
 
 class PdhError(RuntimeError):
@@ -100,6 +115,11 @@ def lib() -> ctypes.CDLL:
         L.pdh_bs_groups.restype = ctypes.c_int
         L.pdh_bs_merge_dev.argtypes = [ctypes.POINTER(BsNode), _sz, ctypes.POINTER(BsNode), ctypes.c_int, _vp]
         L.pdh_bs_merge_dev.restype = ctypes.c_int
+        L.pdh_sadd_batch_dev.argtypes = [ctypes.POINTER(SaddDesc), _sz, _vp]
+        L.pdh_shr_batch_dev.argtypes = [ctypes.POINTER(ShrDesc), _sz, _vp]
+        L.pdh_decimal_leaves_dev.argtypes = [_vp, ctypes.c_uint64, _vp, ctypes.c_uint64, ctypes.c_uint32, _vp, _vp]
+        for name in ("pdh_sadd_batch_dev", "pdh_shr_batch_dev", "pdh_decimal_leaves_dev"):
+            getattr(L, name).restype = ctypes.c_int
         L.pdh_bbp_set_exact.argtypes = [ctypes.c_int]
         L.pdh_bbp_set_exact.restype = ctypes.c_int
         L.pdh_bbp_exact_count.restype = ctypes.c_uint64
@@ -292,6 +312,23 @@ def bs_merge_dev(nodes, outs, skip_p: bool = False, stream: int = 0) -> list:
     arr_out = (BsNode * max(len(outs), 1))(*outs)
     _check(lib().pdh_bs_merge_dev(arr_in, n_in, arr_out, 1 if skip_p else 0, stream or None))
     return [arr_out[i] for i in range(len(outs))]
+
+
+def sadd_batch_dev(descs, stream: int = 0) -> None:
+    """Signed sums of device magnitudes (pdh_sadd_batch_dev)."""
+    arr = (SaddDesc * max(len(descs), 1))(*descs)
+    _check(lib().pdh_sadd_batch_dev(arr, len(descs), stream or None))
+
+
+def shr_batch_dev(descs, stream: int = 0) -> None:
+    """Right shifts of device magnitudes (pdh_shr_batch_dev)."""
+    arr = (ShrDesc * max(len(descs), 1))(*descs)
+    _check(lib().pdh_shr_batch_dev(arr, len(descs), stream or None))
+
+
+def decimal_leaves_dev(d_x: int, stride: int, d_len: int, n: int, width: int, d_out: int, stream: int = 0) -> None:
+    """Decimal digits of n device values below 10^width (pdh_decimal_leaves_dev)."""
+    _check(lib().pdh_decimal_leaves_dev(d_x, stride, d_len, n, width, d_out, stream or None))
 
 
 def bbp_set_exact(mode: int) -> None:

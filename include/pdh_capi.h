@@ -196,6 +196,44 @@ typedef struct pdh_bs_node {
  * Products of more than 2^23 limbs return PDH_E_TOO_LARGE. */
 int pdh_bs_merge_dev(const pdh_bs_node* in, size_t n_in, pdh_bs_node* out, int skip_p, void* stream);
 
+/* ---- device-resident magnitudes ---------------------------------------------
+ * Building blocks of the reference's to_decimal (src/bignum_div.cpp:239-267,
+ * repeated division of the whole number by 10^9) done divide-and-conquer on
+ * device buffers: quotient estimates by shifts, remainders by signed sums,
+ * and the leaves' digits. Lengths are device uint64_t normalized limb counts,
+ * so a level's steps need no host reads of the values. */
+
+/* out = sa*a + sb*b for every descriptor (sa, sb in {-1, 0, 1}; 0 means that
+ * operand is zero and is not read). out may be a itself. Limbs below out_cap
+ * are written and the result must fit in them; *out_len receives its
+ * normalized length and *out_sign its sign (0 for zero). Two passes over
+ * 4,096-limb tiles of every output; the stream is synchronized on return. */
+typedef struct pdh_sadd_desc {
+    const uint32_t* a; const uint64_t* a_len; int32_t sa;
+    const uint32_t* b; const uint64_t* b_len; int32_t sb;
+    uint32_t* out; uint64_t out_cap;
+    uint64_t* out_len; int32_t* out_sign;
+} pdh_sadd_desc;
+int pdh_sadd_batch_dev(const pdh_sadd_desc* descs, size_t n, void* stream);
+
+/* out = floor(a / 2^bits) for every descriptor; *a_len is a's normalized
+ * length (device). Limbs below out_cap (at least 1) are written; the result
+ * must fit. *out_len receives the normalized length. Synchronizes the stream. */
+typedef struct pdh_shr_desc {
+    const uint32_t* a; const uint64_t* a_len;
+    uint32_t* out; uint64_t out_cap;
+    uint64_t* out_len;
+    uint64_t bits;
+} pdh_shr_desc;
+int pdh_shr_batch_dev(const pdh_shr_desc* descs, size_t n, void* stream);
+
+/* Decimal digits of n values below 10^width (value i at d_x + i*stride limbs,
+ * normalized length d_len[i]): width ASCII digits each, most significant
+ * first and zero-padded, at d_out + i*width. width is a positive multiple of
+ * 9. The values are overwritten. Enqueued on `stream`. */
+int pdh_decimal_leaves_dev(uint32_t* d_x, uint64_t stride, const uint64_t* d_len, uint64_t n, uint32_t width,
+                           char* d_out, void* stream);
+
 /* Testing and diagnostics (no counterpart in the reference).
  * pdh_bbp_set_exact(mode): 0 (default) the fixed-point path with the
  * FP64-order fallback; 1 the FP64-order (chunked) path for every position;
