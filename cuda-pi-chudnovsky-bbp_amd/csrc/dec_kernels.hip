@@ -71,6 +71,39 @@ __global__ __launch_bounds__(64) void k_decimal_leaves(uint32_t* __restrict__ x,
     }
 }
 
+// The same with the values staged in LDS (limb-major, thread t's limb k at
+// k * LV_NT + t): the division passes then wait on LDS, not on global memory.
+constexpr int LV_NT = 32;
+constexpr uint64_t LV_MAX_STRIDE = 480;      // 480 x 32 x 4 B = 60 KiB of LDS
+
+__global__ __launch_bounds__(LV_NT) void k_decimal_leaves_lds(const uint32_t* __restrict__ x, uint64_t stride,
+                                                              const uint64_t* __restrict__ x_len, uint64_t n,
+                                                              uint32_t width, char* __restrict__ out) {
+    extern __shared__ uint32_t lv[];
+    const uint64_t i = (uint64_t)blockIdx.x * LV_NT + threadIdx.x;
+    if (i >= n) return;
+    uint64_t len = x_len[i];
+    const uint32_t* src = x + i * stride;
+    for (uint64_t k = 0; k < len; ++k) lv[k * LV_NT + threadIdx.x] = src[k];
+    char* o = out + i * width;
+    for (uint32_t pos = width; pos >= 9; pos -= 9) {
+        uint64_t r = 0;
+        for (uint64_t k = len; k-- > 0;) {
+            uint32_t* cell = &lv[k * LV_NT + threadIdx.x];
+            const uint64_t cur = (r << 32) | *cell;
+            const uint64_t q = cur / 1000000000u;
+            *cell = (uint32_t)q;
+            r = cur - q * 1000000000u;
+        }
+        while (len && lv[(len - 1) * LV_NT + threadIdx.x] == 0) --len;
+        uint32_t r32 = (uint32_t)r;
+        for (int d = 1; d <= 9; ++d) {
+            o[pos - d] = (char)('0' + r32 % 10u);
+            r32 /= 10u;
+        }
+    }
+}
+
 } // namespace
 
 hipError_t launch_shr_batch(const ShrNode* d_nodes, const uint32_t* d_tile_node, uint32_t ntiles, hipStream_t s) {
@@ -87,8 +120,12 @@ hipError_t launch_decimal_leaves(uint32_t* x, uint64_t stride, const uint64_t* x
                                  char* out, hipStream_t s) {
     if (n == 0) return hipSuccess;
     hipEvent_t t0 = timing_begin(s);
-    hipLaunchKernelGGL(k_decimal_leaves, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, x, stride, x_len, n, width,
-                       out);
+    if (stride <= LV_MAX_STRIDE)
+        hipLaunchKernelGGL(k_decimal_leaves_lds, dim3((unsigned)((n + LV_NT - 1) / LV_NT)), dim3(LV_NT),
+                           (size_t)stride * LV_NT * 4, s, x, stride, x_len, n, width, out);
+    else
+        hipLaunchKernelGGL(k_decimal_leaves, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, x, stride, x_len, n,
+                           width, out);
     g_launches.fetch_add(1, std::memory_order_relaxed);
     hipError_t e = hipGetLastError();
     timing_end("k_decimal_leaves", t0, s);

@@ -61,9 +61,72 @@ struct Ctx {
     int bbp_exact = 0;               // pdh_bbp_set_exact mode
     uint64_t bbp_exact_count = 0;    // positions computed by the ordered path
     hipStream_t stream = nullptr;
+    std::vector<hipStream_t> sh_streams;   // internal streams of one-shot batches
+    std::vector<hipEvent_t> sh_events;
 };
 
 Ctx& ctx() { static Ctx c; return c; }
+
+// Device blocks for scratch, batch workspaces and pdh_dev_alloc are kept for
+// reuse instead of being returned with hipFree, which waits for the whole
+// device and costs tens of microseconds per call. A block re-enters the cache
+// only when no work can still use it (internal users free it after
+// synchronizing the stream that used it; pdh_dev_free waits for the device,
+// as hipFree would). An allocation takes the smallest cached block of at
+// least the requested size and at most twice it; at most kCacheMax bytes are
+// kept, and blocks above kBlockMax are never cached.
+struct BlockCache {
+    std::mutex mu;
+    std::multimap<size_t, void*> idle;     // size -> block
+    std::map<void*, size_t> size_of;       // every block the cache handed out
+    size_t idle_bytes = 0;
+};
+BlockCache& bcache() { static BlockCache b; return b; }
+constexpr size_t kCacheMax = (size_t)8 << 30;
+constexpr size_t kBlockMax = (size_t)2 << 30;
+
+void cache_trim(BlockCache& b) {   // caller holds b.mu
+    for (auto& kv : b.idle) { b.size_of.erase(kv.second); (void)hipFree(kv.second); }
+    b.idle.clear();
+    b.idle_bytes = 0;
+}
+
+hipError_t cache_alloc(void** p, size_t bytes) {
+    bytes = (std::max<size_t>(bytes, 1) + 255) & ~(size_t)255;
+    BlockCache& b = bcache();
+    std::lock_guard<std::mutex> g(b.mu);
+    auto it = b.idle.lower_bound(bytes);
+    if (it != b.idle.end() && it->first <= 2 * bytes) {
+        *p = it->second;
+        b.idle_bytes -= it->first;
+        b.idle.erase(it);
+        return hipSuccess;
+    }
+    hipError_t e = hipMalloc(p, bytes);
+    if (e == hipErrorOutOfMemory && !b.idle.empty()) {
+        (void)hipGetLastError();
+        cache_trim(b);
+        e = hipMalloc(p, bytes);
+    }
+    if (e == hipSuccess) b.size_of[*p] = bytes;
+    return e;
+}
+
+void cache_free(void* p) {
+    if (!p) return;
+    BlockCache& b = bcache();
+    std::lock_guard<std::mutex> g(b.mu);
+    auto it = b.size_of.find(p);
+    if (it == b.size_of.end()) { (void)hipFree(p); return; }
+    const size_t sz = it->second;
+    if (sz > kBlockMax || b.idle_bytes + sz > kCacheMax) {
+        b.size_of.erase(it);
+        (void)hipFree(p);
+        return;
+    }
+    b.idle.emplace(sz, p);
+    b.idle_bytes += sz;
+}
 
 int init_locked(Ctx& c) {
     if (c.state == 1) return PDH_OK;
@@ -554,6 +617,7 @@ struct pdh_batch {
     hipGraphExec_t exec = nullptr;
     hipStream_t last = nullptr;           // stream of the latest run
     bool ran = false;
+    bool shared = false;                  // streams and events are the context's (one-shot batches)
 };
 
 namespace {
@@ -587,16 +651,48 @@ void batch_free(pdh_batch* p) {
     if (p->exec) (void)hipGraphExecDestroy(p->exec);
     if (p->graph) (void)hipGraphDestroy(p->graph);
     if (p->ran) (void)hipStreamSynchronize(p->last);   // the caller's stream joins every branch
-    for (auto st : p->streams) (void)hipStreamDestroy(st);
-    for (auto ev : p->events) (void)hipEventDestroy(ev);
-    if (p->arena) (void)hipFree(p->arena);
-    if (p->d_jobs) (void)hipFree(p->d_jobs);
+    if (!p->shared) {
+        for (auto st : p->streams) (void)hipStreamDestroy(st);
+        for (auto ev : p->events) (void)hipEventDestroy(ev);
+    }
+    cache_free(p->arena);
+    cache_free(p->d_jobs);
     delete p;
 }
+
+// One-shot batches (created, run once, waited for and destroyed while the
+// caller holds no other plan) share the context's streams and events: their
+// enqueues are serialised by the context mutex, so reuse is safe.
+int shared_streams(Ctx& c, size_t n, pdh_batch* p) {
+    while (c.sh_streams.size() < 4) {
+        hipStream_t st;
+        HIPCALL(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        c.sh_streams.push_back(st);
+    }
+    while (c.sh_events.size() < 5) {
+        hipEvent_t ev;
+        HIPCALL(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        c.sh_events.push_back(ev);
+    }
+    p->streams.assign(c.sh_streams.begin(), c.sh_streams.begin() + n);
+    p->events.assign(c.sh_events.begin(), c.sh_events.begin() + n + 1);
+    p->shared = true;
+    return PDH_OK;
+}
+
+int batch_create_impl(const pdh_mul_desc* descs, size_t n, pdh_batch** plan, bool shared);
 
 } // namespace
 
 int pdh_batch_create(const pdh_mul_desc* descs, size_t n, pdh_batch** plan) {
+    return batch_create_impl(descs, n, plan, false);
+}
+
+} // extern "C"
+
+namespace {
+
+int batch_create_impl(const pdh_mul_desc* descs, size_t n, pdh_batch** plan, bool shared) {
     if (!plan || (n && !descs)) return fail(PDH_E_ARG, "null argument");
     *plan = nullptr;
     Ctx& c = ctx();
@@ -623,7 +719,7 @@ int pdh_batch_create(const pdh_mul_desc* descs, size_t n, pdh_batch** plan) {
             arena += kv.second.size() * (2 * align(((size_t)8) << kv.first) + align(carry_ws_bytes((size_t)1 << (kv.first - 1))));
     }
     if (arena) {
-        hipError_t e = hipMalloc(&p->arena, arena);
+        hipError_t e = cache_alloc((void**)&p->arena, arena);
         if (e != hipSuccess) { batch_free(p); return fail_hip(e, "batch workspace allocation"); }
     }
     std::vector<MulJob> jobs;
@@ -643,7 +739,7 @@ int pdh_batch_create(const pdh_mul_desc* descs, size_t n, pdh_batch** plan) {
         }
     }
     if (njobs_total) {
-        hipError_t e = hipMalloc(&p->d_jobs, njobs_total * sizeof(MulJob));
+        hipError_t e = cache_alloc((void**)&p->d_jobs, njobs_total * sizeof(MulJob));
         if (e == hipSuccess) e = hipMemcpy(p->d_jobs, jobs.data(), njobs_total * sizeof(MulJob), hipMemcpyHostToDevice);
         if (e != hipSuccess) { batch_free(p); return fail_hip(e, "batch job table upload"); }
     }
@@ -663,6 +759,11 @@ int pdh_batch_create(const pdh_mul_desc* descs, size_t n, pdh_batch** plan) {
     // largest classes first: they have the longest critical path
     std::reverse(p->classes.begin(), p->classes.end());
     const size_t nstreams = std::min<size_t>(4, std::max<size_t>(1, p->classes.size()));
+    if (shared) {
+        if ((rc = shared_streams(c, nstreams, p))) { batch_free(p); return rc; }
+        *plan = p;
+        return PDH_OK;
+    }
     for (size_t i = 0; i < nstreams; ++i) {
         hipStream_t st;
         hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
@@ -678,6 +779,10 @@ int pdh_batch_create(const pdh_mul_desc* descs, size_t n, pdh_batch** plan) {
     *plan = p;
     return PDH_OK;
 }
+
+} // namespace
+
+extern "C" {
 
 int pdh_batch_run(pdh_batch* p, void* stream) {
     if (!p) return fail(PDH_E_ARG, "null plan");
@@ -721,7 +826,7 @@ int pdh_batch_destroy(pdh_batch* p) {
 
 int pdh_mul_batch_dev(const pdh_mul_desc* descs, size_t n, void* stream) {
     pdh_batch* p = nullptr;
-    int rc = pdh_batch_create(descs, n, &p);
+    int rc = batch_create_impl(descs, n, &p, true);
     if (rc) return rc;
     rc = pdh_batch_run(p, stream);
     if (rc == PDH_OK) {
@@ -783,10 +888,10 @@ int pdh_bs_merge_dev(const pdh_bs_node* in, size_t n_in, pdh_bs_node* out, int s
     pdh_batch* plan = nullptr;
     auto cleanup = [&]() {
         if (plan) pdh_batch_destroy(plan);
-        if (ws) { (void)hipStreamSynchronize(s); (void)hipFree(ws); }
+        if (ws) { (void)hipStreamSynchronize(s); cache_free(ws); }
     };
     if (pairs) {
-        hipError_t e = hipMalloc(&ws, bytes);
+        hipError_t e = cache_alloc((void**)&ws, bytes);
         if (e != hipSuccess) return fail_hip(e, "pdh_bs_merge_dev: scratch allocation");
     }
     uint32_t* dB = (uint32_t*)ws;
@@ -820,7 +925,7 @@ int pdh_bs_merge_dev(const pdh_bs_node* in, size_t n_in, pdh_bs_node* out, int s
     if (pairs) {
         e = hipMemcpyAsync(ws + o_nodes, stage.data(), stage.size(), hipMemcpyHostToDevice, s);
         if (e != hipSuccess) { cleanup(); return fail_hip(e, "pdh_bs_merge_dev: node upload"); }
-        rc = pdh_batch_create(jobs.data(), jobs.size(), &plan);
+        rc = batch_create_impl(jobs.data(), jobs.size(), &plan, true);
         if (rc == PDH_OK) rc = pdh_batch_run(plan, stream);
         if (rc != PDH_OK) { std::string m = pdh_last_error(); cleanup(); return fail(rc, m); }
         {
@@ -875,7 +980,7 @@ int run_tiled(const std::vector<Node>& nodes, const std::vector<uint32_t>& tmap,
     std::memcpy(stage.data(), nodes.data(), nodes.size() * sizeof(Node));
     std::memcpy(stage.data() + o_tmap, tmap.data(), tmap.size() * 4);
     char* ws = nullptr;
-    hipError_t e = hipMalloc(&ws, o_extra + al256(extra));
+    hipError_t e = cache_alloc((void**)&ws, o_extra + al256(extra));
     if (e != hipSuccess) return fail_hip(e, what);
     e = hipMemcpyAsync(ws, stage.data(), stage.size(), hipMemcpyHostToDevice, s);
     if (e == hipSuccess) {
@@ -884,7 +989,7 @@ int run_tiled(const std::vector<Node>& nodes, const std::vector<uint32_t>& tmap,
         e = launch((const Node*)ws, (const uint32_t*)(ws + o_tmap), (uint32_t)tmap.size(), ws + o_extra);
     }
     hipError_t e2 = hipStreamSynchronize(s);
-    (void)hipFree(ws);
+    cache_free(ws);
     if (e == hipSuccess) e = e2;
     return e == hipSuccess ? PDH_OK : fail_hip(e, what);
 }
@@ -978,11 +1083,13 @@ int pdh_dev_alloc(size_t bytes, void** ptr) {
     Ctx& c = ctx();
     { std::lock_guard<std::mutex> g(c.mu); int rc = init_locked(c); if (rc) return rc; }
     if (bytes == 0) return PDH_OK;
-    HIPCALL(hipMalloc(ptr, bytes));
+    HIPCALL(cache_alloc(ptr, bytes));
     return PDH_OK;
 }
 int pdh_dev_free(void* ptr) {
-    if (ptr) HIPCALL(hipFree(ptr));
+    if (!ptr) return PDH_OK;
+    HIPCALL(hipDeviceSynchronize());       // as hipFree: no work may still use the block
+    cache_free(ptr);
     return PDH_OK;
 }
 int pdh_copy_htod(void* dst, const void* src, size_t bytes, void* stream) {

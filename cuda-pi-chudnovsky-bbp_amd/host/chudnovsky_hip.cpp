@@ -538,6 +538,15 @@ static void check(int rc) {
     if (rc != PDH_OK) throw HipError(rc, std::string("ntt-hip: ") + pdh_last_error());
 }
 
+// One stream per host thread for the device-resident steps, created on first
+// use and kept (creating and destroying a stream per call costs more than
+// most of the calls).
+static void* work_stream() {
+    static thread_local void* st = nullptr;
+    if (!st) check(pdh_stream_create(&st));
+    return st;
+}
+
 // Products beyond one transform (na + nb > 2^23 limbs) are assembled from
 // blocks of at most 2^21 limbs each.
 static Mag dev_mul(const Mag& a, const Mag& b) {
@@ -594,9 +603,9 @@ static std::vector<Mag> dev_mul_many(const std::vector<std::pair<const Mag*, con
         pa += a.size(); pb += b.size(); po += a.size() + b.size();
     }
     void *dA = nullptr, *dB = nullptr, *dO = nullptr, *dL = nullptr, *st = nullptr;
-    auto release = [&]() { pdh_dev_free(dA); pdh_dev_free(dB); pdh_dev_free(dO); pdh_dev_free(dL); pdh_stream_destroy(st); };
+    auto release = [&]() { pdh_dev_free(dA); pdh_dev_free(dB); pdh_dev_free(dO); pdh_dev_free(dL); };
     try {
-        check(pdh_stream_create(&st));
+        st = work_stream();
         check(pdh_dev_alloc(la * 4, &dA));
         check(pdh_dev_alloc(lb * 4, &dB));
         check(pdh_dev_alloc(lo * 4, &dO));
@@ -639,11 +648,11 @@ static std::vector<PQT> dev_levels(uint64_t N) {
     const uint64_t LIM = (uint64_t)1 << 23;
     void *dP = nullptr, *dQ = nullptr, *dT = nullptr, *dL = nullptr, *dS = nullptr, *st = nullptr, *arena = nullptr;
     auto release = [&]() { pdh_dev_free(dP); pdh_dev_free(dQ); pdh_dev_free(dT); pdh_dev_free(dL);
-                           pdh_dev_free(dS); pdh_dev_free(arena); pdh_stream_destroy(st); };
+                           pdh_dev_free(dS); pdh_dev_free(arena); };
     auto r4 = [](uint64_t x) { return (x + 3) & ~(uint64_t)3; };
     std::vector<PQT> out;
     try {
-        check(pdh_stream_create(&st));
+        st = work_stream();
         check(pdh_dev_alloc(groups * cp * 4, &dP));
         check(pdh_dev_alloc(groups * cq * 4, &dQ));
         check(pdh_dev_alloc(groups * ct * 4, &dT));
@@ -732,11 +741,10 @@ static std::string dev_to_decimal(const Mag& x, const Multiplier& M) {
     auto cap_of = [](uint64_t w) { return (uint64_t)((double)w * 3.3219280948873623 / 32.0) + 3; };
     auto r64 = [](uint64_t v) { return (v + 63) & ~(uint64_t)63; };
     void *st = nullptr, *cur = nullptr, *nxt = nullptr, *aux = nullptr, *dout = nullptr;
-    auto release = [&]() { pdh_dev_free(cur); pdh_dev_free(nxt); pdh_dev_free(aux); pdh_dev_free(dout);
-                           pdh_stream_destroy(st); };
+    auto release = [&]() { pdh_dev_free(cur); pdh_dev_free(nxt); pdh_dev_free(aux); pdh_dev_free(dout); };
     std::string out(p.W, '0');
     try {
-        check(pdh_stream_create(&st));
+        st = work_stream();
         // a level: n nodes x C limbs, then n lengths
         uint64_t n = 1, C = cap_of(p.W);
         check(pdh_dev_alloc(r64(n * C) * 4 + n * 8, &cur));

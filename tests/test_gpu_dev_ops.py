@@ -138,12 +138,14 @@ def test_shr_batch(pdh, seed):
         assert full == w                      # zeros above the length, up to cap
 
 
-@pytest.mark.parametrize("width", [9, 18, 1224, 2304])
-def test_decimal_leaves(pdh, width):
+@pytest.mark.parametrize("width,pad", [(9, 0), (18, 0), (1224, 0), (2304, 0), (1224, 400)])
+def test_decimal_leaves(pdh, width, pad):
+    """Both kernels: values staged in LDS (stride up to 480 limbs) and, with
+    a padded stride, converted in place in global memory."""
     import torch
     rng = random.Random(width)
     vals = [rng.randrange(10 ** width) for _ in range(70)] + [0, 1, 10 ** width - 1, 10 ** (width - 1)]
-    stride = (width * 3322 // 1000) // 32 + 3
+    stride = (width * 3322 // 1000) // 32 + 3 + pad
     X = np.zeros(len(vals) * stride, np.uint32)
     L = np.zeros(len(vals), np.int64)
     for i, v in enumerate(vals):
