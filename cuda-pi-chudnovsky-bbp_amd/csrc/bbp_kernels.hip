@@ -240,6 +240,52 @@ __device__ __forceinline__ uint64_t frac64(uint32_t r, uint32_t m, double minv) 
     return ((uint64_t)hi << 32) | lo;
 }
 
+// floor((16^e mod m) 2^64 / m) for e >= 1 and 1 < m < 2^30, with no division.
+// With m = 2^s mo (mo odd) and r = 16^e mod m = 2^s r_o, r_o = 2^(4e - s) mod
+// mo, the fraction is floor(r_o 2^64 / mo) = F, and F mo = r_o 2^64 - rem
+// with rem = r_o 2^64 mod mo = 2^(4e - s + 64) mod mo. So F = -rem mo^-1 mod
+// 2^64 (exact: F < 2^64), and rem comes from the square-and-double ladder run
+// on E = 4e - s + 64 -- the same bit count as 4e - s but for the last few
+// terms of a series. mo^-1 mod 2^64 is one 64-bit Newton step from the 32-bit inverse.
+//
+// The ladder is modpow16_mont's, for callers whose first active lane holds
+// the wavefront's largest exponent (lanes taking consecutive terms k of one
+// series: e = d - k falls with the lane). Its bit loop runs on that lane's
+// bit count, read once into a scalar register: no per-lane counter or exit
+// test, and the bit is one v_bfe. A lane with a shorter exponent squares the
+// Montgomery one through its leading zero bits, which keeps its residue. The
+// set-up is trimmed too: the inverse starts from (3 mo) xor 2, right to five
+// bits for odd mo, so three Newton steps reach 40; and the Montgomery one
+// 2^32 mod mo is taken in the lazy range [0, 2 mo) from a 32-bit quotient
+// estimate q within one of floor(2^32 / mo): 2^32 - q mo, formed mod 2^32,
+// lies in [-mo, 2 mo), and one add of mo fixes the negative case.
+__device__ __forceinline__ uint64_t frac64_mont_wave(uint64_t e, uint32_t m, double minv) {
+    const int s = __builtin_ctz(m);
+    const uint32_t mo = m >> s;
+    const uint32_t E = (uint32_t)(4 * e) - (uint32_t)s + 64u;
+    uint32_t inv = (3u * mo) ^ 2u;
+    for (int i = 0; i < 3; ++i) inv *= 2u - mo * inv;
+    const uint32_t ninv = 0u - inv;
+    const uint32_t q = (uint32_t)(4294967296.0 * (minv * (double)(1u << s)));
+    uint32_t r = 0u - q * mo;
+    if ((int32_t)r < 0) r += mo;
+    const uint32_t two_mo = mo << 1;
+    const uint32_t Emax = __builtin_amdgcn_readfirstlane(E);
+    for (int b = 31 - __clz(Emax); b >= 0; --b) {
+        const uint64_t t = (uint64_t)r * r;
+        const uint32_t mq = (uint32_t)t * ninv;
+        r = (uint32_t)(((uint64_t)mq * mo + t) >> 32);
+        r <<= (E >> b) & 1u;
+        const uint32_t d = r - two_mo;
+        r = d < r ? d : r;
+    }
+    const uint32_t mq = r * ninv;
+    uint32_t rem = (uint32_t)(((uint64_t)mq * mo + r) >> 32);
+    if (rem >= mo) rem -= mo;
+    const uint64_t inv64 = (uint64_t)inv * (2u - (uint64_t)mo * inv);
+    return (0ull - (uint64_t)rem) * inv64;
+}
+
 // Virtual block v: series v / bps, terms [kb * 256 * KT, (kb + 1) * 256 * KT).
 // Lane t takes k = base + i * 256 + t, so a wavefront's exponents d - k are
 // consecutive and its square-and-multiply loops have the same length.
@@ -258,8 +304,12 @@ __global__ __launch_bounds__(256) void k_bbp_fx_terms(const BbpSeries* __restric
         if (mi == 1) continue;                        // 16^d mod 1 = 0
         const double m = (double)mi;
         const double minv = recip_f64(m);
-        const double r = modpow16(S.d - k, mi, m, minv);
-        local += frac64((uint32_t)r, (uint32_t)mi, minv);
+        if (mi < ((uint64_t)1 << 30) && k < S.d) {
+            local += frac64_mont_wave(S.d - k, (uint32_t)mi, minv);
+        } else {                                      // 16^0 = 1, or m >= 2^30
+            const uint32_t r = k == S.d ? 1u : (uint32_t)modpow16_exact(S.d - k, m, minv);
+            local += frac64(r, (uint32_t)mi, minv);
+        }
     }
     for (int o = 32; o > 0; o >>= 1) local += __shfl_xor(local, o);
     __shared__ uint64_t part[4];
