@@ -73,6 +73,25 @@ def test_sampled_positions(pdh, golden):
     assert got == g["hex"]
 
 
+def test_sampled_4096_positions(pdh):
+    """Config C4's parity subset: 4,096 positions uniform in [10^6, 10^8]
+    (seed 401), position for position against digits computed on the host
+    in the reference's FP64 order (tests/golden/bbp_sampled_4096.json; its
+    generator agrees with the reference's own bbp_cpu on 1..65,536 and on the
+    64-position prefix). Also counts how many positions needed the ordered
+    fallback."""
+    import json
+    with open(os.path.join(GOLDEN_DIR, "bbp_sampled_4096.json")) as f:
+        g = json.load(f)
+    pos = np.array(g["positions"], dtype=np.uint64)
+    assert len(pos) == 4096 and len(g["hex"]) == 4096
+    before = pdh.bbp_exact_count()
+    got = pdh.to_hex(pdh.bbp_values_positions(pos))
+    bad = [i for i in range(len(pos)) if got[i] != g["hex"][i]]
+    assert not bad, [(int(pos[i]), got[i], g["hex"][i]) for i in bad[:8]]
+    print("ordered fallback positions:", pdh.bbp_exact_count() - before)
+
+
 def test_bad_positions_raise(pdh):
     with pytest.raises(pdh.PdhError):
         pdh.bbp_values_positions([0])
