@@ -361,13 +361,14 @@ __device__ __forceinline__ SaView sa_view(const BsAddNode& N, uint32_t nd, uint3
     return v;
 }
 
+// nodes == nullptr: a single node, passed by value in `one`.
 __global__ __launch_bounds__(SA_NT) void k_bs_add_status(const BsAddNode* __restrict__ nodes,
-                                                         const uint32_t* __restrict__ tile_node,
+                                                         const uint32_t* __restrict__ tile_node, BsAddNode one,
                                                          uint8_t* __restrict__ status) {
     __shared__ uint32_t wv[SA_NT / 64];
     const uint32_t t = blockIdx.x;
-    const uint32_t nd = tile_node[t];
-    const BsAddNode N = nodes[nd];
+    const uint32_t nd = nodes ? tile_node[t] : 0u;
+    const BsAddNode N = nodes ? nodes[nd] : one;
     const SaView v = sa_view(N, nd, t);
     const uint64_t base = (uint64_t)v.j * BS_ADD_TILE + (uint64_t)threadIdx.x * SA_K;
     uint32_t x[SA_K], y[SA_K];
@@ -381,13 +382,13 @@ __global__ __launch_bounds__(SA_NT) void k_bs_add_status(const BsAddNode* __rest
 }
 
 __global__ __launch_bounds__(SA_NT) void k_bs_add_apply(const BsAddNode* __restrict__ nodes,
-                                                        const uint32_t* __restrict__ tile_node,
+                                                        const uint32_t* __restrict__ tile_node, BsAddNode one,
                                                         const uint8_t* __restrict__ status) {
     __shared__ uint32_t wv[SA_NT / 64];
     __shared__ uint32_t red[2][SA_NT / 64];
     const uint32_t t = blockIdx.x;
-    const uint32_t nd = tile_node[t];
-    const BsAddNode N = nodes[nd];
+    const uint32_t nd = nodes ? tile_node[t] : 0u;
+    const BsAddNode N = nodes ? nodes[nd] : one;
     const SaView v = sa_view(N, nd, t);
     // nearest lower non-PROP tile (carry into this tile) and, for a
     // difference, the highest non-PROP tile (the order of |A| and |B|);
@@ -447,16 +448,18 @@ __global__ __launch_bounds__(SA_NT) void k_bs_add_apply(const BsAddNode* __restr
 } // namespace
 
 hipError_t launch_bs_signed_add(const BsAddNode* d_nodes, const uint32_t* d_tile_node, uint32_t ntiles,
-                                uint8_t* d_status, hipStream_t s) {
+                                uint8_t* d_status, hipStream_t s, const BsAddNode* one) {
     if (ntiles == 0) return hipSuccess;
+    const BsAddNode single = one ? *one : BsAddNode{};
+    if (one) { d_nodes = nullptr; d_tile_node = nullptr; }
     hipEvent_t t0 = timing_begin(s);
-    hipLaunchKernelGGL(k_bs_add_status, dim3(ntiles), dim3(SA_NT), 0, s, d_nodes, d_tile_node, d_status);
+    hipLaunchKernelGGL(k_bs_add_status, dim3(ntiles), dim3(SA_NT), 0, s, d_nodes, d_tile_node, single, d_status);
     g_launches.fetch_add(1, std::memory_order_relaxed);
     hipError_t e = hipGetLastError();
     timing_end("k_bs_add_status", t0, s);
     if (e != hipSuccess) return e;
     t0 = timing_begin(s);
-    hipLaunchKernelGGL(k_bs_add_apply, dim3(ntiles), dim3(SA_NT), 0, s, d_nodes, d_tile_node, d_status);
+    hipLaunchKernelGGL(k_bs_add_apply, dim3(ntiles), dim3(SA_NT), 0, s, d_nodes, d_tile_node, single, d_status);
     g_launches.fetch_add(1, std::memory_order_relaxed);
     e = hipGetLastError();
     timing_end("k_bs_add_apply", t0, s);

@@ -15,14 +15,17 @@ constexpr int SH_NT = 256;
 constexpr int SH_K = SHR_TILE / SH_NT;
 static_assert(SH_NT * SH_K == SHR_TILE, "tile size");
 
-// out[j] = limbs j + w and j + w + 1 of a joined and shifted right by s
-// (bits = 32 w + s). The output's top candidate is j = la - w - 1: it is
-// nonzero unless a's top limb is below 2^s, and then limb j - 1 holds that
-// top limb's bits, so the normalized length needs no reduction.
-__global__ __launch_bounds__(SH_NT) void k_shr_batch(const ShrNode* __restrict__ nodes,
-                                                     const uint32_t* __restrict__ tile_node) {
+// Right shift (out[j] = limbs j + w and j + w + 1 of a joined and shifted
+// right by s) or left shift (out[j] = limbs j - w and j - w - 1 joined and
+// shifted left by s), bits = 32 w + s. The normalized length follows from
+// a's top limb alone, which the first thread of the node reads: right, the
+// top candidate is la - w - 1, nonzero unless a's top limb is below 2^s (and
+// then limb la - w - 2 holds that limb's bits); left, la + w, nonzero iff the
+// top limb has bits above 32 - s. nodes == nullptr: one node in `one`.
+__global__ __launch_bounds__(SH_NT) void k_shift_batch(const ShiftNode* __restrict__ nodes,
+                                                       const uint32_t* __restrict__ tile_node, ShiftNode one) {
     const uint32_t t = blockIdx.x;
-    const ShrNode N = nodes[tile_node[t]];
+    const ShiftNode N = nodes ? nodes[tile_node[t]] : one;
     const uint64_t la = *N.a_len;
     const uint64_t w = N.bits >> 5;
     const uint32_t s = (uint32_t)(N.bits & 31);
@@ -30,16 +33,27 @@ __global__ __launch_bounds__(SH_NT) void k_shr_batch(const ShrNode* __restrict__
     for (int k = 0; k < SH_K; ++k) {
         const uint64_t j = base + (uint64_t)k * SH_NT + threadIdx.x;
         if (j >= N.out_cap) break;
-        const uint64_t i = j + w;
-        const uint32_t lo = i < la ? N.a[i] : 0u;
-        const uint32_t hi = i + 1 < la ? N.a[i + 1] : 0u;
-        const uint32_t v = s ? (lo >> s) | (hi << (32 - s)) : lo;
+        uint32_t v;
+        if (N.left) {
+            const uint32_t hi = j >= w && j - w < la ? N.a[j - w] : 0u;
+            const uint32_t lo = j >= w + 1 && j - w - 1 < la ? N.a[j - w - 1] : 0u;
+            v = s ? (hi << s) | (lo >> (32 - s)) : hi;
+        } else {
+            const uint64_t i = j + w;
+            const uint32_t lo = i < la ? N.a[i] : 0u;
+            const uint32_t hi = i + 1 < la ? N.a[i + 1] : 0u;
+            v = s ? (lo >> s) | (hi << (32 - s)) : lo;
+        }
         N.out[j] = v;
-        if (la > w && j == la - w - 1) *N.out_len = v ? j + 1 : j;
     }
     if (t == N.tile0 && threadIdx.x == 0) {
-        if (la <= w) *N.out_len = 0;
-        else if (la - w - 1 >= N.out_cap) *N.out_len = N.out_cap;      // does not fit: truncated
+        uint64_t len = 0;
+        if (la) {
+            const uint32_t top = N.a[la - 1];
+            if (N.left) len = la + w + ((s && (top >> (32 - s))) ? 1 : 0);
+            else if (la > w) len = la - w - ((top >> s) ? 0 : 1);
+        }
+        *N.out_len = len < N.out_cap ? len : N.out_cap;      // a result that does not fit is truncated
     }
 }
 
@@ -106,13 +120,16 @@ __global__ __launch_bounds__(LV_NT) void k_decimal_leaves_lds(const uint32_t* __
 
 } // namespace
 
-hipError_t launch_shr_batch(const ShrNode* d_nodes, const uint32_t* d_tile_node, uint32_t ntiles, hipStream_t s) {
+hipError_t launch_shift_batch(const ShiftNode* d_nodes, const uint32_t* d_tile_node, uint32_t ntiles, hipStream_t s,
+                              const ShiftNode* one) {
     if (ntiles == 0) return hipSuccess;
+    const ShiftNode single = one ? *one : ShiftNode{};
+    if (one) { d_nodes = nullptr; d_tile_node = nullptr; }
     hipEvent_t t0 = timing_begin(s);
-    hipLaunchKernelGGL(k_shr_batch, dim3(ntiles), dim3(SH_NT), 0, s, d_nodes, d_tile_node);
+    hipLaunchKernelGGL(k_shift_batch, dim3(ntiles), dim3(SH_NT), 0, s, d_nodes, d_tile_node, single);
     g_launches.fetch_add(1, std::memory_order_relaxed);
     hipError_t e = hipGetLastError();
-    timing_end("k_shr_batch", t0, s);
+    timing_end("k_shift_batch", t0, s);
     return e;
 }
 

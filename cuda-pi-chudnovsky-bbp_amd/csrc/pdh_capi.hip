@@ -61,6 +61,7 @@ struct Ctx {
     int bbp_exact = 0;               // pdh_bbp_set_exact mode
     uint64_t bbp_exact_count = 0;    // positions computed by the ordered path
     hipStream_t stream = nullptr;
+    uint8_t* sadd_status = nullptr; size_t sadd_status_n = 0;   // tile statuses of pdh_sadd_dev
     std::vector<hipStream_t> sh_streams;   // internal streams of one-shot batches
     std::vector<hipEvent_t> sh_events;
 };
@@ -1048,20 +1049,52 @@ int pdh_shr_batch_dev(const pdh_shr_desc* d, size_t n, void* stream) {
     if (ntiles > 0xFFFFFFFFull) return fail(PDH_E_TOO_LARGE, "pdh_shr_batch_dev: too many tiles");
     int rc = dev_ready();
     if (rc) return rc;
-    std::vector<ShrNode> nodes(n);
+    std::vector<ShiftNode> nodes(n);
     std::vector<uint32_t> tmap;
     tmap.reserve(ntiles);
     for (size_t i = 0; i < n; ++i) {
         const pdh_shr_desc& x = d[i];
         const uint32_t nt = (uint32_t)((x.out_cap + SHR_TILE - 1) / SHR_TILE);
-        nodes[i] = ShrNode{x.a, x.a_len, x.out, x.out_len, x.out_cap, x.bits, (uint32_t)tmap.size(), nt};
+        nodes[i] = ShiftNode{x.a, x.a_len, x.out, x.out_len, x.out_cap, x.bits, (uint32_t)tmap.size(), nt, 0u};
         tmap.insert(tmap.end(), nt, (uint32_t)i);
     }
     hipStream_t s = (hipStream_t)stream;
     return run_tiled(nodes, tmap, 0, s, "pdh_shr_batch_dev",
-                     [&](const ShrNode* dn, const uint32_t* dt, uint32_t nt, char*) {
-                         return launch_shr_batch(dn, dt, nt, s);
+                     [&](const ShiftNode* dn, const uint32_t* dt, uint32_t nt, char*) {
+                         return launch_shift_batch(dn, dt, nt, s);
                      });
+}
+
+int pdh_sadd_dev(const uint32_t* a, const uint64_t* a_len, int32_t sa, const uint32_t* b, const uint64_t* b_len,
+                 int32_t sb, uint32_t* out, uint64_t out_cap, uint64_t* out_len, int32_t* out_sign, void* stream) {
+    if (sa < -1 || sa > 1 || sb < -1 || sb > 1 || (sa && (!a || !a_len)) || (sb && (!b || !b_len)) || !out ||
+        out_cap == 0 || !out_len || !out_sign)
+        return fail(PDH_E_ARG, "pdh_sadd_dev: bad argument");
+    const uint64_t nt = (out_cap + BS_ADD_TILE - 1) / BS_ADD_TILE;
+    if (nt > 0xFFFFFFFFull) return fail(PDH_E_TOO_LARGE, "pdh_sadd_dev: too many tiles");
+    Ctx& c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    int rc = init_locked(c);
+    if (rc) return rc;
+    if ((rc = grow(&c.sadd_status, &c.sadd_status_n, nt, 1))) return rc;
+    const BsAddNode one{out, a, b, a_len, b_len, out_cap, out_len, out_sign, 0u, (uint32_t)nt, sa, sb};
+    hipError_t e = launch_bs_signed_add(nullptr, nullptr, (uint32_t)nt, c.sadd_status, (hipStream_t)stream, &one);
+    return e == hipSuccess ? PDH_OK : fail_hip(e, "signed sum launch");
+}
+
+int pdh_shift_dev(const uint32_t* a, const uint64_t* a_len, int64_t bits, uint32_t* out, uint64_t out_cap,
+                  uint64_t* out_len, void* stream) {
+    if (!a || !a_len || !out || out_cap == 0 || !out_len) return fail(PDH_E_ARG, "pdh_shift_dev: bad argument");
+    const uint64_t nt = (out_cap + SHR_TILE - 1) / SHR_TILE;
+    if (nt > 0xFFFFFFFFull) return fail(PDH_E_TOO_LARGE, "pdh_shift_dev: too many tiles");
+    Ctx& c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    int rc = init_locked(c);
+    if (rc) return rc;
+    const uint64_t mag = bits < 0 ? (uint64_t)0 - (uint64_t)bits : (uint64_t)bits;
+    const ShiftNode one{a, a_len, out, out_len, out_cap, mag, 0u, (uint32_t)nt, bits < 0 ? 0u : 1u};
+    hipError_t e = launch_shift_batch(nullptr, nullptr, (uint32_t)nt, (hipStream_t)stream, &one);
+    return e == hipSuccess ? PDH_OK : fail_hip(e, "shift launch");
 }
 
 int pdh_decimal_leaves_dev(uint32_t* d_x, uint64_t stride, const uint64_t* d_len, uint64_t n, uint32_t width,

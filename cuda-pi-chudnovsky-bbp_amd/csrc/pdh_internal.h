@@ -161,17 +161,19 @@ struct BsAddNode {
     uint32_t tile0, ntiles;
     int32_t sa, sb;
 };
+// one != nullptr: a single node passed by value (d_nodes, d_tile_node unused).
 hipError_t launch_bs_signed_add(const BsAddNode* d_nodes, const uint32_t* d_tile_node, uint32_t ntiles,
-                                uint8_t* d_status, hipStream_t s);
+                                uint8_t* d_status, hipStream_t s, const BsAddNode* one = nullptr);
 
-// Decimal conversion helpers (dec_kernels.hip). Right shifts: out = a >> bits
-// for every node, over a flat list of SHR_TILE-limb tiles of the outputs
-// (limbs below out_cap written; *out_len normalized from *a_len, which must be
-// normalized). Leaves: for value i at x + i stride (normalized length
-// x_len[i], below 10^width, width a multiple of 9), width ASCII digits at
-// out + i width, most significant first; the values are overwritten.
+// Decimal conversion helpers (dec_kernels.hip). Shifts: out = a >> bits (or
+// a << bits when left) for every node, over a flat list of SHR_TILE-limb
+// tiles of the outputs (limbs below out_cap written; *out_len normalized from
+// *a_len, which must be normalized). Leaves: for value i at x + i stride
+// (normalized length x_len[i], below 10^width, width a multiple of 9), width
+// ASCII digits at out + i width, most significant first; the values are
+// overwritten.
 constexpr uint32_t SHR_TILE = 2048;
-struct ShrNode {
+struct ShiftNode {
     const uint32_t* a;
     const uint64_t* a_len;
     uint32_t* out;
@@ -179,8 +181,10 @@ struct ShrNode {
     uint64_t out_cap;
     uint64_t bits;
     uint32_t tile0, ntiles;
+    uint32_t left;
 };
-hipError_t launch_shr_batch(const ShrNode* d_nodes, const uint32_t* d_tile_node, uint32_t ntiles, hipStream_t s);
+hipError_t launch_shift_batch(const ShiftNode* d_nodes, const uint32_t* d_tile_node, uint32_t ntiles, hipStream_t s,
+                              const ShiftNode* one = nullptr);
 hipError_t launch_decimal_leaves(uint32_t* x, uint64_t stride, const uint64_t* x_len, uint64_t n, uint32_t width,
                                  char* out, hipStream_t s);
 

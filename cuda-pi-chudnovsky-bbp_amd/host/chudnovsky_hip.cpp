@@ -63,6 +63,10 @@ struct Multiplier {
     // optional: the decimal string of x > 0 (the levels run elsewhere); empty =
     // the host loop below
     std::function<std::string(const std::vector<uint32_t>& x, const Multiplier& M)> to_decimal;
+    // optional: floor(426880 Q sqrt(10005 10^(2 prec)) / T) computed elsewhere
+    // (the square root and the division kept in device memory); empty = the
+    // host backend below
+    std::function<Mag(const PQT& r, uint64_t prec, std::chrono::steady_clock::time_point* ts)> pi_scaled;
 };
 
 // ---- magnitude helpers (little-endian base 2^32, normalized) -----------------
@@ -179,34 +183,78 @@ static SInt sadd(const SInt& a, const SInt& b) {
     return c > 0 ? SInt{sub(a.m, b.m), a.s} : SInt{sub(b.m, a.m), b.s};
 }
 
+// ---- arithmetic backends ------------------------------------------------------------
+// The Newton algorithms below are written once over a backend A with value
+// type A::Num and the operations bits, shl, shr, add, sub (a >= b), cmp, pow2,
+// mul, sqr, zero, inc, dec, mul_u32, from_u128 / to_u128 (small values) and
+// up / down (to and from host magnitudes). HostArith keeps values on the host
+// and multiplies through the Multiplier (the CPU tests run the algorithms
+// this way with the oracle's multiply); DevArith (further down) keeps them in
+// device memory.
+struct HostArith {
+    typedef Mag Num;
+    const Multiplier& M;
+    size_t bits(const Mag& a) { return bitlen(a); }
+    Mag shl(const Mag& a, size_t k) { return chud::shl(a, k); }
+    Mag shr(const Mag& a, size_t k) { return chud::shr(a, k); }
+    Mag add(const Mag& a, const Mag& b) { return chud::add(a, b); }
+    Mag sub(const Mag& a, const Mag& b) { return chud::sub(a, b); }
+    int cmp(const Mag& a, const Mag& b) { return chud::cmp(a, b); }
+    Mag pow2(size_t e) { return chud::pow2(e); }
+    Mag mul(const Mag& a, const Mag& b) { return M.mul(a, b); }
+    Mag sqr(const Mag& a) { return M.sqr(a); }
+    bool zero(const Mag& a) { return a.empty(); }
+    Mag inc(Mag a) { chud::inc(a); return a; }
+    Mag dec(Mag a) { chud::dec(a); return a; }
+    Mag mul_u32(const Mag& a, uint32_t m) { return chud::mul_u32(a, m); }
+    Mag from_u128(u128 v) { return chud::from_u128(v); }
+    u128 to_u128(const Mag& a) { return chud::to_u128(a); }
+    Mag up(const Mag& a) { return a; }
+    Mag down(const Mag& a) { return a; }
+    void sync() {}
+};
+
+template <class A> struct SNum { typename A::Num m; int s = 0; };   // sign s in {-1, 0, 1}
+
+template <class A> static SNum<A> sadd_t(A& o, const SNum<A>& a, const SNum<A>& b) {
+    if (a.s == 0) return b;
+    if (b.s == 0) return a;
+    if (a.s == b.s) return SNum<A>{o.add(a.m, b.m), a.s};
+    const int c = o.cmp(a.m, b.m);
+    if (c == 0) return SNum<A>{o.from_u128(0), 0};
+    return c > 0 ? SNum<A>{o.sub(a.m, b.m), a.s} : SNum<A>{o.sub(b.m, a.m), b.s};
+}
+
 // ---- Newton reciprocal: about floor(2^(2L) / d), L = bitlen(d) -----------------
 // One Newton step V = V0 + V0 (2^(2L) - d V0) / 2^(2L) from V0 = V0h 2^sh
 // (sh <= L - 2) squares V0's relative error. e = 2^(2L) - d V0 is formed at
 // scale 2^sh, e_s = |2^(2L - sh) - d V0h|, and only e_s's bits from 2^k up,
 // k = L - 2 - sh, enter the correction: the dropped part moves V by
 // V0h e_lo 2^(2 sh) / 2^(2L) < 2^(L + 1 - sh + k + 2 sh - 2L) = 1/2.
-static Mag recip_step(const Mag& d, size_t L, const Mag& V0h, size_t sh, const Multiplier& M) {
-    Mag P = M.mul(d, V0h);
-    Mag two = pow2(2 * L - sh);
-    const bool neg = cmp(P, two) > 0;
-    Mag e = neg ? sub(P, two) : sub(two, P);
+template <class A>
+static typename A::Num recip_step(A& o, const typename A::Num& d, size_t L, const typename A::Num& V0h, size_t sh) {
+    typedef typename A::Num N;
+    N P = o.mul(d, V0h);
+    N two = o.pow2(2 * L - sh);
+    const bool neg = o.cmp(P, two) > 0;
+    N e = neg ? o.sub(P, two) : o.sub(two, P);
     const size_t k = L - 2 - sh;
-    Mag corr = shr(M.mul(V0h, shr(e, k)), L + 2 - sh);
-    Mag V0 = shl(V0h, sh);
-    return neg ? sub(V0, corr) : add(V0, corr);
+    N corr = o.shr(o.mul(V0h, o.shr(e, k)), L + 2 - sh);
+    N V0 = o.shl(V0h, sh);
+    return neg ? o.sub(V0, corr) : o.add(V0, corr);
 }
 
 // The top h = L/2 + 8 bits of d give V_h ~ 2^(2h)/d_h; V0 = V_h 2^(L-h) has
 // relative error ~2^(3-h), and one step takes it to within about two units
-// (the truncations); callers correct exactly.
-static Mag recip(const Mag& d, const Multiplier& M) {
-    const size_t L = bitlen(d);
+// (the truncations); callers correct exactly. L may be passed when known.
+template <class A> static typename A::Num recip(A& o, const typename A::Num& d, size_t L = 0) {
+    if (!L) L = o.bits(d);
     if (L <= 62) {
-        u128 v = ((u128)1 << (2 * L)) / (u128)to_u128(d);
-        return from_u128(v);
+        const u128 v = ((u128)1 << (2 * L)) / o.to_u128(d);
+        return o.from_u128(v);
     }
     const size_t h = L / 2 + 8;
-    return recip_step(d, L, recip(shr(d, L - h), M), L - h, M);
+    return recip_step(o, d, L, recip(o, o.shr(d, L - h), h), L - h);
 }
 
 // ---- exact floor division --------------------------------------------------------
@@ -215,33 +263,37 @@ static Mag recip(const Mag& d, const Multiplier& M) {
 // a / d ~ a V / 2^(2P - (P - nd)). The low bits of a beyond what k + P bits of
 // precision need are dropped first. The estimate is within a few units; the
 // remainder r = a - q d then moves q to the exact floor.
-static void fix_quotient(const Mag& a, const Mag& d, const Mag& qd, Mag& q, Mag* rem) {
-    Mag r;
-    if (cmp(qd, a) > 0) {
-        Mag deficit = sub(qd, a);                 // a - q d < 0
-        while (!deficit.empty()) {
-            dec(q);
-            if (cmp(deficit, d) <= 0) { r = sub(d, deficit); deficit.clear(); }
-            else deficit = sub(deficit, d);
+template <class A>
+static void fix_quotient(A& o, const typename A::Num& a, const typename A::Num& d, const typename A::Num& qd,
+                         typename A::Num& q, typename A::Num* rem) {
+    typedef typename A::Num N;
+    N r;
+    if (o.cmp(qd, a) > 0) {
+        N deficit = o.sub(qd, a);                 // a - q d < 0
+        for (;;) {
+            q = o.dec(q);
+            if (o.cmp(deficit, d) <= 0) { r = o.sub(d, deficit); break; }
+            deficit = o.sub(deficit, d);
         }
     } else {
-        r = sub(a, qd);
-        while (cmp(r, d) >= 0) { r = sub(r, d); inc(q); }
+        r = o.sub(a, qd);
+        while (o.cmp(r, d) >= 0) { r = o.sub(r, d); q = o.inc(q); }
     }
-    if (rem) *rem = std::move(r);
+    if (rem) *rem = r;
 }
 
-static Mag div_floor(const Mag& a, const Mag& d, const Multiplier& M, Mag* rem) {
-    if (cmp(a, d) < 0) { if (rem) *rem = a; return {}; }
-    const size_t na = bitlen(a), nd = bitlen(d);
+template <class A>
+static typename A::Num div_floor(A& o, const typename A::Num& a, const typename A::Num& d, typename A::Num* rem) {
+    const size_t na = o.bits(a), nd = o.bits(d);
+    if (na < nd || (na == nd && o.cmp(a, d) < 0)) { if (rem) *rem = a; return o.from_u128(0); }
     const size_t k = na - nd + 1;
     const size_t P = k + 32;
     const long sd = (long)P - (long)nd;
-    Mag dp = sd >= 0 ? shl(d, (size_t)sd) : shr(d, (size_t)(-sd));
-    Mag V = recip(dp, M);
+    typename A::Num dp = sd >= 0 ? o.shl(d, (size_t)sd) : o.shr(d, (size_t)(-sd));
+    typename A::Num V = recip(o, dp, P);
     const size_t ta = na > k + P + 2 ? na - (k + P + 2) : 0;
-    Mag q = shr(M.mul(shr(a, ta), V), (size_t)((long)(2 * P) - sd - (long)ta));
-    fix_quotient(a, d, M.mul(q, d), q, rem);
+    typename A::Num q = o.shr(o.mul(o.shr(a, ta), V), (size_t)((long)(2 * P) - sd - (long)ta));
+    fix_quotient(o, a, d, o.mul(q, d), q, rem);
     return q;
 }
 
@@ -253,69 +305,71 @@ static Mag div_floor(const Mag& a, const Mag& d, const Multiplier& M, Mag* rem) 
 //     Y = Y0 + Y0 (2^(2E) - s Y0^2) / 2^(2E + 1)
 // squares it. Only what that step can see is formed: s without its low
 // b = E/2 - 8 bits (s_lo Y0^2 < 2^(3E/2 - 6)), and e = 2^(2E) - s Y0^2 from
-// 2^c up, c = 3E/2 - 2 (the dropped part moves Y by less than 1/2).
-static Mag rsqrt_approx(const Mag& s, const Multiplier& M) {
-    const size_t L = bitlen(s);
+// 2^c up, c = 3E/2 - 2 (the dropped part moves Y by less than 1/2). L =
+// bitlen(s) is passed down: s_h's is L - 2t.
+template <class A> static typename A::Num rsqrt_approx(A& o, const typename A::Num& s, size_t L) {
+    typedef typename A::Num N;
     const size_t E = L + (L & 1);
     if (E <= 100) {
-        long double v = 0;
-        for (size_t i = s.size(); i-- > 0;) v = v * 4294967296.0L + (long double)s[i];
-        return from_u128((u128)(std::ldexp(1.0L, (int)E) / std::sqrt(v)));
+        const long double v = (long double)o.to_u128(s);
+        return o.from_u128((u128)(std::ldexp(1.0L, (int)E) / std::sqrt(v)));
     }
     size_t Eh = E / 2 + 32;
     Eh += Eh & 1;
     const size_t t = (E - Eh) / 2;
-    const Mag Y0h = rsqrt_approx(shr(s, 2 * t), M);
-    const Mag Y0 = shl(Y0h, t);
+    const N Y0h = rsqrt_approx(o, o.shr(s, 2 * t), L - 2 * t);
+    const N Y0 = o.shl(Y0h, t);
     const size_t b = E / 2 - 8, c = 3 * E / 2 - 2;
-    Mag sy2 = M.mul(shr(s, b), M.sqr(Y0));        // s Y0^2 / 2^b, s truncated
-    Mag two = pow2(2 * E - b);
-    const bool neg = cmp(sy2, two) > 0;
-    Mag e = neg ? sub(sy2, two) : sub(two, sy2);
-    Mag corr = shr(M.mul(Y0, shr(e, c - b)), 2 * E + 1 - c);
-    return neg ? sub(Y0, corr) : add(Y0, corr);
+    N sy2 = o.mul(o.shr(s, b), o.sqr(Y0));        // s Y0^2 / 2^b, s truncated
+    N two = o.pow2(2 * E - b);
+    const bool neg = o.cmp(sy2, two) > 0;
+    N e = neg ? o.sub(sy2, two) : o.sub(two, sy2);
+    N corr = o.shr(o.mul(Y0, o.shr(e, c - b)), 2 * E + 1 - c);
+    return neg ? o.sub(Y0, corr) : o.add(Y0, corr);
 }
 
 // x0 = s Y / 2^(E + G) from Y ~ 2^(E + G) / sqrt(s) (the inverse square root
 // of s 2^(2G), G = 8 guard bits) is within about one unit; then
 // r = s - x^2 moves x to the floor root with r updated by 2x -+ 1 per step
 // (0 <= r <= 2x at the floor root), so only one square is formed.
-static Mag isqrt(const Mag& s, const Multiplier& M) {
-    const size_t L = bitlen(s);
+template <class A> static typename A::Num isqrt(A& o, const typename A::Num& s) {
+    typedef typename A::Num N;
+    const size_t L = o.bits(s);
     if (L <= 100) {
-        u128 v = to_u128(s);
+        const u128 v = o.to_u128(s);
         u128 x = (u128)std::sqrt((long double)v);
         while (x * x > v) --x;
         while ((x + 1) * (x + 1) <= v) ++x;
-        return from_u128(x);
+        return o.from_u128(x);
     }
     const size_t G = 8;
     const size_t E = L + (L & 1);
-    Mag Y = rsqrt_approx(shl(s, 2 * G), M);
+    N Y = rsqrt_approx(o, o.shl(s, 2 * G), L + 2 * G);
     const size_t b = E / 2 - 2;                   // s_lo Y / 2^(E + G) < 1/2
-    Mag x = shr(M.mul(shr(s, b), Y), E + G - b);
-    Mag x2 = M.sqr(x);
-    SInt r = cmp(s, x2) >= 0 ? SInt{sub(s, x2), 1} : SInt{sub(x2, s), -1};
-    if (r.m.empty()) r.s = 0;
+    N x = o.shr(o.mul(o.shr(s, b), Y), E + G - b);
+    N x2 = o.sqr(x);
+    const int c = o.cmp(s, x2);
+    SNum<A> r{c >= 0 ? o.sub(s, x2) : o.sub(x2, s), c};
+    const N one = o.from_u128(1);
     while (r.s < 0) {                             // s - (x-1)^2 = r + 2x - 1
-        r = sadd(r, SInt{sub(shl(x, 1), Mag{1}), 1});
-        dec(x);
+        r = sadd_t(o, r, SNum<A>{o.sub(o.shl(x, 1), one), 1});
+        x = o.dec(x);
     }
     for (;;) {                                    // s - (x+1)^2 = r - 2x - 1
-        Mag two_x = shl(x, 1);
-        if (r.s == 0 || cmp(r.m, two_x) <= 0) break;
-        r = sadd(r, SInt{add(two_x, Mag{1}), -1});
-        inc(x);
+        N two_x = o.shl(x, 1);
+        if (r.s == 0 || o.cmp(r.m, two_x) <= 0) break;
+        r = sadd_t(o, r, SNum<A>{o.add(two_x, one), -1});
+        x = o.inc(x);
     }
     return x;
 }
 
-static Mag pow10(uint64_t e, const Multiplier& M) {
-    Mag result{1}, base{10};
+template <class A> static typename A::Num pow10(A& o, uint64_t e) {
+    typename A::Num result = o.from_u128(1), base = o.from_u128(10);
     while (e) {
-        if (e & 1) result = M.mul(result, base);
+        if (e & 1) result = o.mul(result, base);
         e >>= 1;
-        if (e) base = M.sqr(base);
+        if (e) base = o.sqr(base);
     }
     return result;
 }
@@ -355,6 +409,7 @@ struct DecimalPlan {
 };
 
 static DecimalPlan decimal_plan(const Mag& x, const Multiplier& M) {
+    HostArith o{M};
     DecimalPlan p;
     const size_t need = (size_t)((double)bitlen(x) * 0.30102999566398120) + 2;
     while ((need >> p.K) > 2304) ++p.K;
@@ -368,14 +423,14 @@ static DecimalPlan decimal_plan(const Mag& x, const Multiplier& M) {
     p.Vs.resize(K);
     p.Lds.resize(K);
     for (int l = K - 1; l >= 0; --l) {
-        p.Ds[l] = l == K - 1 ? pow10(p.B, M) : M.sqr(p.Ds[l + 1]);
+        p.Ds[l] = l == K - 1 ? pow10(o, p.B) : M.sqr(p.Ds[l + 1]);
         p.Lds[l] = bitlen(p.Ds[l]);
         const Mag d32 = shl(p.Ds[l], 32);
         const size_t L = p.Lds[l] + 32;
-        if (l == K - 1) { p.Vs[l] = recip(d32, M); continue; }
+        if (l == K - 1) { p.Vs[l] = recip(o, d32); continue; }
         const size_t z = 4 * p.Lds[l + 1] + 32 - 2 * p.Lds[l];
         const size_t sh = L / 2 - 15;
-        p.Vs[l] = recip_step(d32, L, shr(M.sqr(p.Vs[l + 1]), z + sh), sh, M);
+        p.Vs[l] = recip_step(o, d32, L, shr(M.sqr(p.Vs[l + 1]), z + sh), sh);
     }
     return p;
 }
@@ -412,7 +467,8 @@ static std::string to_decimal(const Mag& x, const Multiplier& M) {
         next.reserve(2 * nodes.size());
         for (size_t i = 0; i < nodes.size(); ++i) {
             Mag q = qs[i], r;
-            fix_quotient(nodes[i], D, qds[i], q, &r);
+            HostArith o{M};
+            fix_quotient(o, nodes[i], D, qds[i], q, &r);
             next.push_back(std::move(q));
             next.push_back(std::move(r));
         }
@@ -502,6 +558,23 @@ uint64_t terms_for_digits(uint64_t digits) {              // src/chudnovsky.cpp:
     return n < 2 ? 2 : n;
 }
 
+// pi_scaled = floor(426880 Q isqrt(10005 10^(2 prec)) / T) (src/chudnovsky.cpp:
+// 130-140) on backend o; ts[0] and ts[1] stamp the ends of the 10^prec and
+// square-root phases.
+template <class A>
+static Mag pi_scaled_t(A& o, const PQT& r, uint64_t prec, std::chrono::steady_clock::time_point* ts) {
+    typedef typename A::Num N;
+    N one = pow10(o, prec);
+    N radicand = o.mul_u32(o.sqr(one), 10005u);
+    o.sync();
+    ts[0] = std::chrono::steady_clock::now();
+    N sqrtC = isqrt(o, radicand);
+    o.sync();
+    ts[1] = std::chrono::steady_clock::now();
+    N numerator = o.mul(o.mul_u32(o.up(r.Q), 426880u), sqrtC);
+    return o.down(div_floor(o, numerator, o.up(r.T.m), (N*)nullptr));   // T > 0 for the full range
+}
+
 std::string compute_pi(uint64_t digits, const Multiplier& M, double* phase_ms) {
     using clk = std::chrono::steady_clock;
     auto ms = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
@@ -510,13 +583,15 @@ std::string compute_pi(uint64_t digits, const Multiplier& M, double* phase_ms) {
     auto t0 = clk::now();
     PQT r = binary_split(N, M);
     auto t1 = clk::now();
-    Mag one = pow10(prec, M);
-    Mag radicand = mul_u32(M.sqr(one), 10005u);
-    auto t2 = clk::now();
-    Mag sqrtC = isqrt(radicand, M);
-    auto t3 = clk::now();
-    Mag numerator = M.mul(mul_u32(r.Q, 426880u), sqrtC);
-    Mag pi_scaled = div_floor(numerator, r.T.m, M, nullptr);   // T > 0 for the full range
+    clk::time_point ts[2];
+    Mag pi_scaled;
+    if (M.pi_scaled) {
+        pi_scaled = M.pi_scaled(r, prec, ts);
+    } else {
+        HostArith o{M};
+        pi_scaled = pi_scaled_t(o, r, prec, ts);
+    }
+    const clk::time_point t2 = ts[0], t3 = ts[1];
     auto t4 = clk::now();
     std::string s = to_decimal(pi_scaled, M);
     auto t5 = clk::now();
@@ -847,6 +922,151 @@ static std::string dev_to_decimal(const Mag& x, const Multiplier& M) {
     return strip_zeros(out);
 }
 
+// ---- device backend -----------------------------------------------------------------
+// Values live in device memory: a buffer of cap limbs (zero above the value's
+// length) and the normalized length as a device word. ub is a host-side upper
+// bound of that length, tight enough to size the next operation's output and
+// the multiply's transform. Buffers are carved from per-backend chunks and
+// released together when the backend ends, so the operations allocate nothing
+// and run asynchronously on one stream; the host reads lengths, signs and
+// small values only where the algorithm branches on them (bit lengths,
+// comparisons, base cases). Uploads are synchronous copies into fresh
+// buffers that no queued work uses yet.
+struct DNum {
+    uint32_t* p = nullptr;
+    uint64_t* len = nullptr;
+    uint64_t cap = 0, ub = 0;
+};
+
+class DevArith {
+public:
+    typedef DNum Num;
+    explicit DevArith(void* st) : st_(st) {
+        sign_ = (int32_t*)alloc(1).p;
+        cmp_sign_ = (int32_t*)alloc(1).p;
+        one_ = up(Mag{1});
+    }
+    ~DevArith() {
+        pdh_stream_sync(st_);
+        for (void* c : chunks_) pdh_dev_free(c);
+    }
+    DevArith(const DevArith&) = delete;
+    DevArith& operator=(const DevArith&) = delete;
+
+    void sync() { check(pdh_stream_sync(st_)); }
+    DNum up(const Mag& a) {
+        DNum x = alloc(a.size());
+        const uint64_t n = a.size();
+        if (n) check(pdh_copy_htod(x.p, a.data(), n * 4, nullptr));
+        check(pdh_copy_htod(x.len, &n, 8, nullptr));
+        x.ub = n;
+        return x;
+    }
+    Mag down(const DNum& a) {
+        Mag m(length(a));
+        if (!m.empty()) check(pdh_copy_dtoh(m.data(), a.p, m.size() * 4, nullptr));
+        return m;
+    }
+    size_t bits(const DNum& a) {
+        const uint64_t n = length(a);
+        if (!n) return 0;
+        uint32_t top = 0;
+        check(pdh_copy_dtoh(&top, a.p + n - 1, 4, nullptr));
+        return 32 * (n - 1) + (32 - __builtin_clz(top));
+    }
+    bool zero(const DNum& a) { return length(a) == 0; }
+    u128 to_u128(const DNum& a) {
+        const uint64_t n = length(a);
+        if (n > 4) throw HipError(PDH_E_ARG, "ntt-hip: to_u128 of a value above 2^128");
+        uint32_t w[4] = {0, 0, 0, 0};
+        if (n) check(pdh_copy_dtoh(w, a.p, n * 4, nullptr));
+        return ((u128)w[3] << 96) | ((u128)w[2] << 64) | ((u128)w[1] << 32) | w[0];
+    }
+    DNum from_u128(u128 v) { return up(chud::from_u128(v)); }
+    DNum shl(const DNum& a, size_t k) {
+        DNum x = alloc(a.ub + k / 32 + 1);
+        check(pdh_shift_dev(a.p, a.len, (int64_t)k, x.p, x.cap, x.len, st_));
+        return x;
+    }
+    DNum shr(const DNum& a, size_t k) {
+        const uint64_t w = k / 32;
+        DNum x = alloc(a.ub > w ? a.ub - w : 1);
+        check(pdh_shift_dev(a.p, a.len, -(int64_t)k, x.p, x.cap, x.len, st_));
+        return x;
+    }
+    DNum add(const DNum& a, const DNum& b) { return sadd(a, b, 1, std::max(a.ub, b.ub) + 1); }
+    DNum sub(const DNum& a, const DNum& b) { return sadd(a, b, -1, a.ub); }   // a >= b
+    int cmp(const DNum& a, const DNum& b) {
+        DNum t = alloc(std::max(a.ub, b.ub) + 1);
+        check(pdh_sadd_dev(a.p, a.len, 1, b.p, b.len, -1, t.p, t.cap, t.len, cmp_sign_, st_));
+        const uint64_t n = length(t);
+        if (!n) return 0;
+        int32_t sg = 0;
+        check(pdh_copy_dtoh(&sg, cmp_sign_, 4, nullptr));
+        return sg;
+    }
+    DNum pow2(size_t e) { return shl(one_, e); }
+    DNum mul(const DNum& a, const DNum& b) {
+        if (a.ub + b.ub > ((uint64_t)1 << 23)) return up(dev_mul(down(a), down(b)));   // blocked products
+        DNum x = alloc(a.ub + b.ub);
+        check(pdh_mul_dev(a.p, a.ub, b.p, b.ub, x.p, x.len, st_));
+        return x;
+    }
+    DNum sqr(const DNum& a) {
+        if (2 * a.ub > ((uint64_t)1 << 23)) return mul(a, a);
+        DNum x = alloc(2 * a.ub);
+        check(pdh_sqr_dev(a.p, a.ub, x.p, x.len, st_));
+        return x;
+    }
+    DNum inc(const DNum& a) { return add(a, one_); }
+    DNum dec(const DNum& a) { return sub(a, one_); }
+    DNum mul_u32(const DNum& a, uint32_t m) { return mul(a, up(Mag{m})); }
+
+private:
+    DNum alloc(uint64_t cap) {
+        cap = std::max<uint64_t>(cap, 1);
+        const size_t bytes = ((cap * 4 + 255) & ~(size_t)255) + 256;
+        if (bytes > left_) {
+            const size_t cb = std::max(bytes, (size_t)64 << 20);
+            void* c = nullptr;
+            check(pdh_dev_alloc(cb, &c));
+            chunks_.push_back(c);
+            cur_ = (char*)c;
+            left_ = cb;
+        }
+        DNum x;
+        x.p = (uint32_t*)cur_;
+        x.len = (uint64_t*)(cur_ + bytes - 256);
+        x.cap = x.ub = cap;
+        cur_ += bytes;
+        left_ -= bytes;
+        return x;
+    }
+    uint64_t length(const DNum& a) {
+        sync();
+        uint64_t n = 0;
+        check(pdh_copy_dtoh(&n, a.len, 8, nullptr));
+        return n;
+    }
+    DNum sadd(const DNum& a, const DNum& b, int sb, uint64_t cap) {
+        DNum x = alloc(cap);
+        check(pdh_sadd_dev(a.p, a.len, 1, b.p, b.len, sb, x.p, x.cap, x.len, sign_, st_));
+        return x;
+    }
+    void* st_;
+    std::vector<void*> chunks_;
+    char* cur_ = nullptr;
+    size_t left_ = 0;
+    int32_t* sign_ = nullptr;
+    int32_t* cmp_sign_ = nullptr;
+    DNum one_;
+};
+
+static Mag dev_pi_scaled(const PQT& r, uint64_t prec, std::chrono::steady_clock::time_point* ts) {
+    DevArith o(work_stream());
+    return pi_scaled_t(o, r, prec, ts);
+}
+
 Multiplier device_multiplier() {
     Multiplier M;
     M.mul = dev_mul;
@@ -854,6 +1074,7 @@ Multiplier device_multiplier() {
     M.mul_many = dev_mul_many;
     M.groups = dev_levels;
     M.to_decimal = dev_to_decimal;
+    M.pi_scaled = dev_pi_scaled;
     return M;
 }
 

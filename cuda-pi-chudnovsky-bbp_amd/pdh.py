@@ -39,7 +39,7 @@ EXPORTED = (
     "pdh_tune_get", "pdh_tune_set", "pdh_autotune", "pdh_dev_alloc", "pdh_dev_free", "pdh_copy_htod",
     "pdh_copy_dtoh", "pdh_stream_create", "pdh_stream_destroy", "pdh_stream_sync",
     "pdh_bbp_set_exact", "pdh_bbp_exact_count", "pdh_tune_engine", "pdh_bs_groups", "pdh_bs_merge_dev",
-    "pdh_sadd_batch_dev", "pdh_shr_batch_dev", "pdh_decimal_leaves_dev",
+    "pdh_sadd_batch_dev", "pdh_shr_batch_dev", "pdh_decimal_leaves_dev", "pdh_sadd_dev", "pdh_shift_dev",
 )
 
 
@@ -118,7 +118,11 @@ def lib() -> ctypes.CDLL:
         L.pdh_sadd_batch_dev.argtypes = [ctypes.POINTER(SaddDesc), _sz, _vp]
         L.pdh_shr_batch_dev.argtypes = [ctypes.POINTER(ShrDesc), _sz, _vp]
         L.pdh_decimal_leaves_dev.argtypes = [_vp, ctypes.c_uint64, _vp, ctypes.c_uint64, ctypes.c_uint32, _vp, _vp]
-        for name in ("pdh_sadd_batch_dev", "pdh_shr_batch_dev", "pdh_decimal_leaves_dev"):
+        L.pdh_sadd_dev.argtypes = [_vp, _vp, ctypes.c_int32, _vp, _vp, ctypes.c_int32, _vp, ctypes.c_uint64, _vp, _vp,
+                                   _vp]
+        L.pdh_shift_dev.argtypes = [_vp, _vp, ctypes.c_int64, _vp, ctypes.c_uint64, _vp, _vp]
+        for name in ("pdh_sadd_batch_dev", "pdh_shr_batch_dev", "pdh_decimal_leaves_dev", "pdh_sadd_dev",
+                     "pdh_shift_dev"):
             getattr(L, name).restype = ctypes.c_int
         L.pdh_bbp_set_exact.argtypes = [ctypes.c_int]
         L.pdh_bbp_set_exact.restype = ctypes.c_int

@@ -227,6 +227,21 @@ typedef struct pdh_shr_desc {
 } pdh_shr_desc;
 int pdh_shr_batch_dev(const pdh_shr_desc* descs, size_t n, void* stream);
 
+/* Single operations on one device magnitude each, enqueued on `stream` with
+ * no host synchronization (lengths stay on the device), for Newton
+ * iterations kept in HBM. pdh_sadd_dev: out = sa*a + sb*b as in
+ * pdh_sadd_batch_dev (out may be a; its tile statuses live in the library's
+ * scratch, so like pdh_mul_dev these calls must not overlap on the device
+ * from different streams). pdh_shift_dev: out = a * 2^bits for bits >= 0,
+ * floor(a / 2^-bits) for bits < 0; limbs below out_cap (at least 1) are
+ * written and the result must fit; *out_len receives its normalized length.
+ * No counterpart in the reference (its BigInt shifts and adds are host
+ * code, src/bignum.cpp). */
+int pdh_sadd_dev(const uint32_t* a, const uint64_t* a_len, int32_t sa, const uint32_t* b, const uint64_t* b_len,
+                 int32_t sb, uint32_t* out, uint64_t out_cap, uint64_t* out_len, int32_t* out_sign, void* stream);
+int pdh_shift_dev(const uint32_t* a, const uint64_t* a_len, int64_t bits, uint32_t* out, uint64_t out_cap,
+                  uint64_t* out_len, void* stream);
+
 /* Decimal digits of n values below 10^width (value i at d_x + i*stride limbs,
  * normalized length d_len[i]): width ASCII digits each, most significant
  * first and zero-padded, at d_out + i*width. width is a positive multiple of
