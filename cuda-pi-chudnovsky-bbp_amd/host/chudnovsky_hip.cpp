@@ -401,17 +401,18 @@ static void leaf_digits(Mag x, size_t width, char* out) {
     while (pos > 0) out[--pos] = '0';
 }
 
-struct DecimalPlan {
+template <class A> struct DecimalPlan {
     int K = 0;
     size_t B = 0, W = 0;
-    std::vector<Mag> Ds, Vs;
+    std::vector<typename A::Num> Ds, Vs;
     std::vector<size_t> Lds;
 };
 
-static DecimalPlan decimal_plan(const Mag& x, const Multiplier& M) {
-    HostArith o{M};
-    DecimalPlan p;
-    const size_t need = (size_t)((double)bitlen(x) * 0.30102999566398120) + 2;
+// Levels, leaf width and the divisors with their reciprocals for a value of
+// xbits bits, on backend o.
+template <class A> static DecimalPlan<A> decimal_plan(A& o, size_t xbits) {
+    DecimalPlan<A> p;
+    const size_t need = (size_t)((double)xbits * 0.30102999566398120) + 2;
     while ((need >> p.K) > 2304) ++p.K;
     const int K = p.K;
     p.B = ((need + ((size_t)1 << K) - 1) / ((size_t)1 << K) + 8) / 9 * 9;   // leaf width, multiple of 9
@@ -423,14 +424,14 @@ static DecimalPlan decimal_plan(const Mag& x, const Multiplier& M) {
     p.Vs.resize(K);
     p.Lds.resize(K);
     for (int l = K - 1; l >= 0; --l) {
-        p.Ds[l] = l == K - 1 ? pow10(o, p.B) : M.sqr(p.Ds[l + 1]);
-        p.Lds[l] = bitlen(p.Ds[l]);
-        const Mag d32 = shl(p.Ds[l], 32);
+        p.Ds[l] = l == K - 1 ? pow10(o, p.B) : o.sqr(p.Ds[l + 1]);
+        p.Lds[l] = o.bits(p.Ds[l]);
+        const typename A::Num d32 = o.shl(p.Ds[l], 32);
         const size_t L = p.Lds[l] + 32;
-        if (l == K - 1) { p.Vs[l] = recip(o, d32); continue; }
+        if (l == K - 1) { p.Vs[l] = recip(o, d32, L); continue; }
         const size_t z = 4 * p.Lds[l + 1] + 32 - 2 * p.Lds[l];
         const size_t sh = L / 2 - 15;
-        p.Vs[l] = recip_step(o, d32, L, shr(M.sqr(p.Vs[l + 1]), z + sh), sh);
+        p.Vs[l] = recip_step(o, d32, L, o.shr(o.sqr(p.Vs[l + 1]), z + sh), sh);
     }
     return p;
 }
@@ -443,7 +444,8 @@ static std::string strip_zeros(const std::string& out) {
 static std::string to_decimal(const Mag& x, const Multiplier& M) {
     if (x.empty()) return "0";
     if (M.to_decimal) return M.to_decimal(x, M);
-    const DecimalPlan p = decimal_plan(x, M);
+    HostArith o{M};
+    const DecimalPlan<HostArith> p = decimal_plan(o, bitlen(x));
     const int K = p.K;
     const size_t B = p.B;
     std::string out(p.W, '0');
@@ -467,7 +469,6 @@ static std::string to_decimal(const Mag& x, const Multiplier& M) {
         next.reserve(2 * nodes.size());
         for (size_t i = 0; i < nodes.size(); ++i) {
             Mag q = qs[i], r;
-            HostArith o{M};
             fix_quotient(o, nodes[i], D, qds[i], q, &r);
             next.push_back(std::move(q));
             next.push_back(std::move(r));
@@ -800,128 +801,6 @@ static std::vector<PQT> dev_levels(uint64_t N) {
     return out;
 }
 
-// Divide-and-conquer decimal conversion in device memory. Level l holds 2^l
-// nodes of width W_l = B 2^(K-l) digits at a fixed stride C_l limbs (values
-// below 10^W_l, zero-padded, device lengths). Per level, with D = 10^(W_l/2)
-// and its reciprocal V from the plan: P = X V (one batch), Q = P >> (2 Ld + 32)
-// into the next level's even slots, QD = Q D (one batch), R = X - QD into the
-// odd slots and T = |R| - D (signed sums). Q is floor(X / D) when R >= 0 and
-// T < 0; otherwise (rare: the estimate is within a unit) the node is fixed
-// by steps of D on the host. The leaves' digits come from one kernel; only
-// the lengths, signs and the final W characters cross PCIe.
-static std::string dev_to_decimal(const Mag& x, const Multiplier& M) {
-    const DecimalPlan p = decimal_plan(x, M);
-    const int K = p.K;
-    const size_t B = p.B;
-    auto cap_of = [](uint64_t w) { return (uint64_t)((double)w * 3.3219280948873623 / 32.0) + 3; };
-    auto r64 = [](uint64_t v) { return (v + 63) & ~(uint64_t)63; };
-    void *st = nullptr, *cur = nullptr, *nxt = nullptr, *aux = nullptr, *dout = nullptr;
-    auto release = [&]() { pdh_dev_free(cur); pdh_dev_free(nxt); pdh_dev_free(aux); pdh_dev_free(dout); };
-    std::string out(p.W, '0');
-    try {
-        st = work_stream();
-        // a level: n nodes x C limbs, then n lengths
-        uint64_t n = 1, C = cap_of(p.W);
-        check(pdh_dev_alloc(r64(n * C) * 4 + n * 8, &cur));
-        {
-            std::vector<uint32_t> x0(C, 0);
-            std::copy(x.begin(), x.end(), x0.begin());
-            const uint64_t lx = x.size();
-            check(pdh_copy_htod(cur, x0.data(), C * 4, st));
-            check(pdh_copy_htod((uint32_t*)cur + r64(C), &lx, 8, st));
-        }
-        for (int l = 0; l < K; ++l) {
-            uint32_t* X = (uint32_t*)cur;
-            uint64_t* XL = (uint64_t*)(X + r64(n * C));
-            const Mag& D = p.Ds[l];
-            const Mag& V = p.Vs[l];
-            const uint64_t nD = D.size(), nV = V.size(), Cn = cap_of(p.W >> (l + 1));
-            // next level
-            check(pdh_dev_alloc(r64(2 * n * Cn) * 4 + 2 * n * 8, &nxt));
-            uint32_t* Xn = (uint32_t*)nxt;
-            uint64_t* XnL = (uint64_t*)(Xn + r64(2 * n * Cn));
-            // aux: D | V | P (n x (C + nV)) | QD (n x (Cn + nD)) | T (n x Cn) | lengths | signs
-            const uint64_t oV = r64(nD), oP = oV + r64(nV), oQD = oP + r64(n * (C + nV)),
-                           oT = oQD + r64(n * (Cn + nD)), oEnd = oT + r64(n * Cn);
-            check(pdh_dev_alloc(oEnd * 4 + (3 * n + 1) * 8 + 2 * n * 4, &aux));
-            uint32_t* A = (uint32_t*)aux;
-            uint32_t *dD = A, *dV = A + oV, *dP = A + oP, *dQD = A + oQD, *dT = A + oT;
-            uint64_t* PL = (uint64_t*)(A + oEnd);
-            uint64_t *QDL = PL + n, *TL = QDL + n, *DL = TL + n;
-            int32_t* RS = (int32_t*)(DL + 1);
-            int32_t* TS = RS + n;
-            check(pdh_copy_htod(dD, D.data(), nD * 4, st));
-            check(pdh_copy_htod(dV, V.data(), nV * 4, st));
-            check(pdh_copy_htod(DL, &nD, 8, st));
-            const uint64_t sh = 2 * p.Lds[l] + 32;
-            std::vector<pdh_mul_desc> md(n);
-            for (uint64_t i = 0; i < n; ++i)
-                md[i] = pdh_mul_desc{X + i * C, C, dV, nV, dP + i * (C + nV), PL + i};
-            check(pdh_mul_batch_dev(md.data(), n, st));
-            std::vector<pdh_shr_desc> sd(n);
-            for (uint64_t i = 0; i < n; ++i)
-                sd[i] = pdh_shr_desc{dP + i * (C + nV), PL + i, Xn + 2 * i * Cn, Cn, XnL + 2 * i, sh};
-            check(pdh_shr_batch_dev(sd.data(), n, st));
-            for (uint64_t i = 0; i < n; ++i)
-                md[i] = pdh_mul_desc{Xn + 2 * i * Cn, Cn, dD, nD, dQD + i * (Cn + nD), QDL + i};
-            check(pdh_mul_batch_dev(md.data(), n, st));
-            std::vector<pdh_sadd_desc> ad(n);
-            for (uint64_t i = 0; i < n; ++i)
-                ad[i] = pdh_sadd_desc{X + i * C, XL + i, 1, dQD + i * (Cn + nD), QDL + i, -1,
-                                      Xn + (2 * i + 1) * Cn, Cn, XnL + 2 * i + 1, RS + i};
-            check(pdh_sadd_batch_dev(ad.data(), n, st));
-            for (uint64_t i = 0; i < n; ++i)
-                ad[i] = pdh_sadd_desc{Xn + (2 * i + 1) * Cn, XnL + 2 * i + 1, 1, dD, DL, -1, dT + i * Cn, Cn, TL + i,
-                                      TS + i};
-            check(pdh_sadd_batch_dev(ad.data(), n, st));
-            std::vector<int32_t> sg(2 * n);
-            std::vector<uint64_t> ln(2 * n), tl(n);
-            check(pdh_copy_dtoh(sg.data(), RS, 2 * n * 4, st));
-            check(pdh_copy_dtoh(ln.data(), XnL, 2 * n * 8, st));
-            check(pdh_copy_dtoh(tl.data(), TL, n * 8, st));
-            check(pdh_stream_sync(st));
-            for (uint64_t i = 0; i < n; ++i) {
-                // signs are meaningful only for nonzero values: R < 0, or R > 0 and |R| - D >= 0
-                const bool rnz = ln[2 * i + 1] != 0;
-                const bool neg = rnz && sg[i] < 0;
-                const bool ge = rnz && sg[i] > 0 && (tl[i] == 0 || sg[n + i] > 0);
-                if (!neg && !ge) continue;
-                Mag q(ln[2 * i]), rm(ln[2 * i + 1]);
-                check(pdh_copy_dtoh(q.data(), Xn + 2 * i * Cn, q.size() * 4, st));
-                check(pdh_copy_dtoh(rm.data(), Xn + (2 * i + 1) * Cn, rm.size() * 4, st));
-                check(pdh_stream_sync(st));
-                SInt r{rm, sg[i]};
-                while (r.s < 0) { r = sadd(r, SInt{D, 1}); dec(q); }
-                while (r.s > 0 && cmp(r.m, D) >= 0) { r = sadd(r, SInt{D, -1}); inc(q); }
-                Mag qz(Cn, 0), rz(Cn, 0);
-                std::copy(q.begin(), q.end(), qz.begin());
-                std::copy(r.m.begin(), r.m.end(), rz.begin());
-                const uint64_t lens[2] = {q.size(), r.m.size()};
-                check(pdh_copy_htod(Xn + 2 * i * Cn, qz.data(), Cn * 4, st));
-                check(pdh_copy_htod(Xn + (2 * i + 1) * Cn, rz.data(), Cn * 4, st));
-                check(pdh_copy_htod(XnL + 2 * i, lens, 16, st));
-                check(pdh_stream_sync(st));
-            }
-            pdh_dev_free(aux); aux = nullptr;
-            pdh_dev_free(cur);
-            cur = nxt; nxt = nullptr;
-            n *= 2;
-            C = Cn;
-        }
-        uint32_t* X = (uint32_t*)cur;
-        uint64_t* XL = (uint64_t*)(X + r64(n * C));
-        check(pdh_dev_alloc(p.W, &dout));
-        check(pdh_decimal_leaves_dev(X, C, XL, n, (uint32_t)B, (char*)dout, st));
-        check(pdh_copy_dtoh(&out[0], dout, p.W, st));
-        check(pdh_stream_sync(st));
-    } catch (...) {
-        release();
-        throw;
-    }
-    release();
-    return strip_zeros(out);
-}
-
 // ---- device backend -----------------------------------------------------------------
 // Values live in device memory: a buffer of cap limbs (zero above the value's
 // length) and the normalized length as a device word. ub is a host-side upper
@@ -1061,6 +940,126 @@ private:
     int32_t* cmp_sign_ = nullptr;
     DNum one_;
 };
+
+// Divide-and-conquer decimal conversion in device memory. Level l holds 2^l
+// nodes of width W_l = B 2^(K-l) digits at a fixed stride C_l limbs (values
+// below 10^W_l, zero-padded, device lengths). Per level, with D = 10^(W_l/2)
+// and its reciprocal V from the plan: P = X V (one batch), Q = P >> (2 Ld + 32)
+// into the next level's even slots, QD = Q D (one batch), R = X - QD into the
+// odd slots and T = |R| - D (signed sums). Q is floor(X / D) when R >= 0 and
+// T < 0; otherwise (rare: the estimate is within a unit) the node is fixed
+// by steps of D on the host. The leaves' digits come from one kernel; only
+// the lengths, signs and the final W characters cross PCIe.
+static std::string dev_to_decimal(const Mag& x, const Multiplier&) {
+    void* st = work_stream();
+    DevArith o(st);
+    const DecimalPlan<DevArith> p = decimal_plan(o, bitlen(x));
+    const int K = p.K;
+    const size_t B = p.B;
+    auto cap_of = [](uint64_t w) { return (uint64_t)((double)w * 3.3219280948873623 / 32.0) + 3; };
+    auto r64 = [](uint64_t v) { return (v + 63) & ~(uint64_t)63; };
+    void *cur = nullptr, *nxt = nullptr, *aux = nullptr, *dout = nullptr;
+    auto release = [&]() { pdh_dev_free(cur); pdh_dev_free(nxt); pdh_dev_free(aux); pdh_dev_free(dout); };
+    std::string out(p.W, '0');
+    try {
+        // a level: n nodes x C limbs, then n lengths
+        uint64_t n = 1, C = cap_of(p.W);
+        check(pdh_dev_alloc(r64(n * C) * 4 + n * 8, &cur));
+        {
+            std::vector<uint32_t> x0(C, 0);
+            std::copy(x.begin(), x.end(), x0.begin());
+            const uint64_t lx = x.size();
+            check(pdh_copy_htod(cur, x0.data(), C * 4, nullptr));
+            check(pdh_copy_htod((uint32_t*)cur + r64(C), &lx, 8, nullptr));
+        }
+        for (int l = 0; l < K; ++l) {
+            uint32_t* X = (uint32_t*)cur;
+            uint64_t* XL = (uint64_t*)(X + r64(n * C));
+            const DNum& D = p.Ds[l];
+            const DNum& V = p.Vs[l];
+            const uint64_t nD = D.ub, nV = V.ub, Cn = cap_of(p.W >> (l + 1));
+            // next level
+            check(pdh_dev_alloc(r64(2 * n * Cn) * 4 + 2 * n * 8, &nxt));
+            uint32_t* Xn = (uint32_t*)nxt;
+            uint64_t* XnL = (uint64_t*)(Xn + r64(2 * n * Cn));
+            // aux: P (n x (C + nV)) | QD (n x (Cn + nD)) | T (n x Cn) | lengths | signs
+            const uint64_t oQD = r64(n * (C + nV)), oT = oQD + r64(n * (Cn + nD)), oEnd = oT + r64(n * Cn);
+            check(pdh_dev_alloc(oEnd * 4 + 3 * n * 8 + 2 * n * 4, &aux));
+            uint32_t* A = (uint32_t*)aux;
+            uint32_t *dD = D.p, *dV = V.p, *dP = A, *dQD = A + oQD, *dT = A + oT;
+            uint64_t* PL = (uint64_t*)(A + oEnd);
+            uint64_t *QDL = PL + n, *TL = QDL + n;
+            const uint64_t* DL = D.len;
+            int32_t* RS = (int32_t*)(TL + n);
+            int32_t* TS = RS + n;
+            const uint64_t sh = 2 * p.Lds[l] + 32;
+            std::vector<pdh_mul_desc> md(n);
+            for (uint64_t i = 0; i < n; ++i)
+                md[i] = pdh_mul_desc{X + i * C, C, dV, nV, dP + i * (C + nV), PL + i};
+            check(pdh_mul_batch_dev(md.data(), n, st));
+            std::vector<pdh_shr_desc> sd(n);
+            for (uint64_t i = 0; i < n; ++i)
+                sd[i] = pdh_shr_desc{dP + i * (C + nV), PL + i, Xn + 2 * i * Cn, Cn, XnL + 2 * i, sh};
+            check(pdh_shr_batch_dev(sd.data(), n, st));
+            for (uint64_t i = 0; i < n; ++i)
+                md[i] = pdh_mul_desc{Xn + 2 * i * Cn, Cn, dD, nD, dQD + i * (Cn + nD), QDL + i};
+            check(pdh_mul_batch_dev(md.data(), n, st));
+            std::vector<pdh_sadd_desc> ad(n);
+            for (uint64_t i = 0; i < n; ++i)
+                ad[i] = pdh_sadd_desc{X + i * C, XL + i, 1, dQD + i * (Cn + nD), QDL + i, -1,
+                                      Xn + (2 * i + 1) * Cn, Cn, XnL + 2 * i + 1, RS + i};
+            check(pdh_sadd_batch_dev(ad.data(), n, st));
+            for (uint64_t i = 0; i < n; ++i)
+                ad[i] = pdh_sadd_desc{Xn + (2 * i + 1) * Cn, XnL + 2 * i + 1, 1, dD, DL, -1, dT + i * Cn, Cn, TL + i,
+                                      TS + i};
+            check(pdh_sadd_batch_dev(ad.data(), n, st));
+            std::vector<int32_t> sg(2 * n);
+            std::vector<uint64_t> ln(2 * n), tl(n);
+            check(pdh_copy_dtoh(sg.data(), RS, 2 * n * 4, st));
+            check(pdh_copy_dtoh(ln.data(), XnL, 2 * n * 8, st));
+            check(pdh_copy_dtoh(tl.data(), TL, n * 8, st));
+            check(pdh_stream_sync(st));
+            for (uint64_t i = 0; i < n; ++i) {
+                // signs are meaningful only for nonzero values: R < 0, or R > 0 and |R| - D >= 0
+                const bool rnz = ln[2 * i + 1] != 0;
+                const bool neg = rnz && sg[i] < 0;
+                const bool ge = rnz && sg[i] > 0 && (tl[i] == 0 || sg[n + i] > 0);
+                if (!neg && !ge) continue;
+                Mag q(ln[2 * i]), rm(ln[2 * i + 1]);
+                check(pdh_copy_dtoh(q.data(), Xn + 2 * i * Cn, q.size() * 4, st));
+                check(pdh_copy_dtoh(rm.data(), Xn + (2 * i + 1) * Cn, rm.size() * 4, st));
+                check(pdh_stream_sync(st));
+                const Mag Dh = o.down(D);
+                SInt r{rm, sg[i]};
+                while (r.s < 0) { r = sadd(r, SInt{Dh, 1}); dec(q); }
+                while (r.s > 0 && cmp(r.m, Dh) >= 0) { r = sadd(r, SInt{Dh, -1}); inc(q); }
+                Mag qz(Cn, 0), rz(Cn, 0);
+                std::copy(q.begin(), q.end(), qz.begin());
+                std::copy(r.m.begin(), r.m.end(), rz.begin());
+                const uint64_t lens[2] = {q.size(), r.m.size()};
+                check(pdh_copy_htod(Xn + 2 * i * Cn, qz.data(), Cn * 4, nullptr));
+                check(pdh_copy_htod(Xn + (2 * i + 1) * Cn, rz.data(), Cn * 4, nullptr));
+                check(pdh_copy_htod(XnL + 2 * i, lens, 16, nullptr));
+            }
+            pdh_dev_free(aux); aux = nullptr;
+            pdh_dev_free(cur);
+            cur = nxt; nxt = nullptr;
+            n *= 2;
+            C = Cn;
+        }
+        uint32_t* X = (uint32_t*)cur;
+        uint64_t* XL = (uint64_t*)(X + r64(n * C));
+        check(pdh_dev_alloc(p.W, &dout));
+        check(pdh_decimal_leaves_dev(X, C, XL, n, (uint32_t)B, (char*)dout, st));
+        check(pdh_copy_dtoh(&out[0], dout, p.W, st));
+        check(pdh_stream_sync(st));
+    } catch (...) {
+        release();
+        throw;
+    }
+    release();
+    return strip_zeros(out);
+}
 
 static Mag dev_pi_scaled(const PQT& r, uint64_t prec, std::chrono::steady_clock::time_point* ts) {
     DevArith o(work_stream());

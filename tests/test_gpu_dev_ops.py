@@ -162,3 +162,55 @@ def test_decimal_leaves(pdh, width, pad):
         assert s[i * width:(i + 1) * width] == str(v).zfill(width), i
     with pytest.raises(pdh.PdhError):
         pdh.decimal_leaves_dev(dX.data_ptr(), stride, dL.data_ptr(), 1, 10, dO.data_ptr())
+
+
+def test_single_shift_both_directions(pdh):
+    """pdh_shift_dev: a * 2^k for k >= 0 and floor(a / 2^-k) for k < 0, on one
+    stream with no host synchronization between calls."""
+    import torch
+    rng = random.Random(21)
+    dev = Dev()
+    stream = torch.cuda.current_stream().cuda_stream
+    cases = [(rng.getrandbits(32 * rng.randint(1, 5000)), rng.randint(-32 * 5000, 32 * 300)) for _ in range(40)]
+    cases += [(1, 0), (1, 31), (1, 32), (0xFFFFFFFF, 1), (0xFFFFFFFF, -31), (7, -3), (7, -64), (0, 5), (0, -5),
+              ((1 << (32 * 2048)) - 1, 33), (1 << 4000, -4000)]
+    outs = []
+    for a, k in cases:
+        A, AL = dev.mag(a)
+        want = a << k if k >= 0 else a >> -k
+        cap = max(len(limbs(a)) + (k // 32 + 1 if k >= 0 else -((-k) // 32)), 1)
+        O, OL, _ = dev.out(cap)
+        assert pdh.lib().pdh_shift_dev(A.data_ptr(), AL.data_ptr(), k, O.data_ptr(), cap, OL.data_ptr(),
+                                       stream) == 0
+        outs.append((O, OL, cap, want))
+    torch.cuda.synchronize()
+    for O, OL, cap, want in outs:
+        assert int(OL.item()) == len(limbs(want))
+        assert int.from_bytes(O[:cap].cpu().numpy().view(np.uint32).tobytes(), "little") == want
+
+
+def test_single_sadd_chain(pdh):
+    """pdh_sadd_dev calls chained on one stream: each sum feeds the next,
+    with lengths and signs only on the device until the end."""
+    import torch
+    rng = random.Random(22)
+    dev = Dev()
+    stream = torch.cuda.current_stream().cuda_stream
+    vals = [rng.getrandbits(32 * rng.randint(1, 6000)) for _ in range(12)]
+    signs = [rng.choice((-1, 1)) for _ in vals]
+    acc = vals[0]
+    A, AL = dev.mag(vals[0], cap=6100)
+    S = dev.torch.zeros(1, dtype=dev.torch.int32, device="cuda")
+    for v, sg in zip(vals[1:], signs[1:]):
+        B, BL = dev.mag(v)
+        O, OL, OS = dev.out(6100)
+        sa = 1 if acc >= 0 else -1
+        # |acc| is on the device with its sign known only to the host's model; pass it
+        assert pdh.lib().pdh_sadd_dev(A.data_ptr(), AL.data_ptr(), sa if acc else 0, B.data_ptr(), BL.data_ptr(),
+                                      sg, O.data_ptr(), 6100, OL.data_ptr(), OS.data_ptr(), stream) == 0
+        acc = acc + sg * v
+        A, AL, S = O, OL, OS
+    torch.cuda.synchronize()
+    assert Dev.value(A, AL) == abs(acc)
+    if acc:
+        assert int(S.item()) == sgn(acc)
