@@ -6,6 +6,7 @@
 //  and the decimal digits of the leaves.
 // ============================================================================
 #include "pdh_internal.h"
+#include <algorithm>
 
 namespace pdh {
 
@@ -118,7 +119,20 @@ __global__ __launch_bounds__(LV_NT) void k_decimal_leaves_lds(const uint32_t* __
     }
 }
 
+__global__ void k_fill_zero(uint32_t* __restrict__ p, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
 } // namespace
+
+hipError_t launch_fill_zero(uint32_t* p, uint64_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    const uint64_t blocks = std::min<uint64_t>((n + 1023) / 1024, 4096);
+    hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)blocks), dim3(256), 0, s, p, n);
+    g_launches.fetch_add(1, std::memory_order_relaxed);
+    return hipGetLastError();
+}
 
 hipError_t launch_shift_batch(const ShiftNode* d_nodes, const uint32_t* d_tile_node, uint32_t ntiles, hipStream_t s,
                               const ShiftNode* one) {

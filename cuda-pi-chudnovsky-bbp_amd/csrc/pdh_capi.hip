@@ -62,6 +62,7 @@ struct Ctx {
     uint64_t bbp_exact_count = 0;    // positions computed by the ordered path
     hipStream_t stream = nullptr;
     uint8_t* sadd_status = nullptr; size_t sadd_status_n = 0;   // tile statuses of pdh_sadd_dev
+    char* blk_scratch = nullptr; size_t blk_scratch_n = 0;      // block product and lengths of blocked multiplies
     std::vector<hipStream_t> sh_streams;   // internal streams of one-shot batches
     std::vector<hipEvent_t> sh_events;
 };
@@ -279,6 +280,57 @@ int ensure_ws(Ctx& c, size_t N, size_t nlimbs) {
 int log2_exact(size_t N) { int l = 0; while (((size_t)1 << l) < N) ++l; return l; }
 
 int mul_dev_locked(Ctx& c, const u32* a, size_t na, const u32* b, size_t nb, bool square,
+                   u32* out, u64* out_len, hipStream_t s);
+
+// Products beyond one transform (na + nb > 2^23 limbs): a and b in blocks of
+// 2^22 limbs, each block product one 2^24-point transform into scratch,
+// added into out at its offset by the single-node signed sum (every partial
+// sum is at most the product, so it fits from the offset to the end), out
+// zeroed first and its length normalized by a last pass. Everything is
+// enqueued on s; the scratch is the context's, as for the other
+// device-resident calls.
+int mul_blocked_locked(Ctx& c, const u32* a, size_t na, const u32* b, size_t nb, u32* out, u64* out_len,
+                       hipStream_t s) {
+    const size_t K = (size_t)1 << 22, total = na + nb;
+    const size_t ba = (na + K - 1) / K, bb = (nb + K - 1) / K;
+    std::vector<u64> wl;                               // window lengths: total - offset, then total
+    for (size_t i = 0; i < ba; ++i)
+        for (size_t j = 0; j < bb; ++j) wl.push_back(total - (i + j) * K);
+    wl.push_back(total);
+    const size_t need = ((2 * K * 4 + 255) & ~(size_t)255) + 256 + wl.size() * 8 + 64;
+    int rc = grow(&c.blk_scratch, &c.blk_scratch_n, need, 1);
+    if (rc) return rc;
+    const size_t nt = (total + BS_ADD_TILE - 1) / BS_ADD_TILE;
+    if ((rc = grow(&c.sadd_status, &c.sadd_status_n, nt, 1))) return rc;
+    char* w = (char*)c.blk_scratch;
+    u32* P = (u32*)w;
+    u64* PL = (u64*)(w + ((2 * K * 4 + 255) & ~(size_t)255));
+    u64* dwl = PL + 1;
+    u64* slen = dwl + wl.size();
+    int32_t* ssgn = (int32_t*)(slen + 1);
+    HIPCALL(hipStreamSynchronize(s));                  // the scratch may still serve earlier work on s
+    HIPCALL(hipMemcpy(dwl, wl.data(), wl.size() * 8, hipMemcpyHostToDevice));
+    hipError_t e = launch_fill_zero(out, total, s);
+    if (e != hipSuccess) return fail_hip(e, "zero fill launch");
+    size_t k = 0;
+    for (size_t i = 0; i < ba; ++i) {
+        for (size_t j = 0; j < bb; ++j, ++k) {
+            const size_t ka = std::min(K, na - i * K), kb = std::min(K, nb - j * K);
+            if ((rc = mul_dev_locked(c, a + i * K, ka, b + j * K, kb, false, P, PL, s))) return rc;
+            u32* win = out + (i + j) * K;
+            const BsAddNode node{win, win, P, dwl + k, PL, wl[k], slen, ssgn, 0u,
+                                 (uint32_t)((wl[k] + BS_ADD_TILE - 1) / BS_ADD_TILE), 1, 1};
+            e = launch_bs_signed_add(nullptr, nullptr, node.ntiles, c.sadd_status, s, &node);
+            if (e != hipSuccess) return fail_hip(e, "block sum launch");
+        }
+    }
+    const BsAddNode norm{out, out, out, dwl + k, dwl + k, total, out_len, ssgn, 0u, (uint32_t)nt, 1, 0};
+    e = launch_bs_signed_add(nullptr, nullptr, (uint32_t)nt, c.sadd_status, s, &norm);
+    if (e != hipSuccess) return fail_hip(e, "normalize launch");
+    return PDH_OK;
+}
+
+int mul_dev_locked(Ctx& c, const u32* a, size_t na, const u32* b, size_t nb, bool square,
                    u32* out, u64* out_len, hipStream_t s) {
     if (na == 0 || (!square && nb == 0)) {
         hipError_t e = launch_zero_len(out_len, s);
@@ -288,7 +340,7 @@ int mul_dev_locked(Ctx& c, const u32* a, size_t na, const u32* b, size_t nb, boo
     if (square) { b = a; nb = na; }
     const size_t N = pdh_ntt_length(na, nb);
     const int n = log2_exact(N);
-    if (n > MAX_LOG) return fail(PDH_E_TOO_LARGE, "transform length 2^" + std::to_string(n) + " exceeds 2^24");
+    if (n > MAX_LOG) return mul_blocked_locked(c, a, na, b, nb, out, out_len, s);
     int rc = ensure_ws(c, N, na + nb);
     if (rc) return rc;
     OuterTables ot{nullptr, nullptr, nullptr, nullptr, 0, nullptr, nullptr, 0};

@@ -185,6 +185,7 @@ struct ShiftNode {
 };
 hipError_t launch_shift_batch(const ShiftNode* d_nodes, const uint32_t* d_tile_node, uint32_t ntiles, hipStream_t s,
                               const ShiftNode* one = nullptr);
+hipError_t launch_fill_zero(uint32_t* p, uint64_t n, hipStream_t s);
 hipError_t launch_decimal_leaves(uint32_t* x, uint64_t stride, const uint64_t* x_len, uint64_t n, uint32_t width,
                                  char* out, hipStream_t s);
 

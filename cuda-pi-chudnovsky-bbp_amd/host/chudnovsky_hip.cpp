@@ -623,34 +623,15 @@ static void* work_stream() {
     return st;
 }
 
-// Products beyond one transform (na + nb > 2^23 limbs) are assembled from
-// blocks of at most 2^21 limbs each.
+// Any size: products beyond one transform are blocked inside the library.
 static Mag dev_mul(const Mag& a, const Mag& b) {
     if (a.empty() || b.empty()) return {};
-    const size_t LIM = (size_t)1 << 23, BLK = (size_t)1 << 21;
-    if (a.size() + b.size() <= LIM) return mul_ntt_hip(a, b);
-    Mag acc(a.size() + b.size() + 1, 0);
-    for (size_t i = 0; i < a.size(); i += BLK) {
-        Mag ai(a.begin() + i, a.begin() + std::min(a.size(), i + BLK));
-        trim(ai);
-        for (size_t j = 0; j < b.size(); j += BLK) {
-            Mag bj(b.begin() + j, b.begin() + std::min(b.size(), j + BLK));
-            trim(bj);
-            Mag p = mul_ntt_hip(ai, bj);
-            uint64_t c = 0;
-            size_t k = 0;
-            for (; k < p.size(); ++k) { c += (uint64_t)acc[i + j + k] + p[k]; acc[i + j + k] = (uint32_t)c; c >>= 32; }
-            for (size_t t = i + j + k; c; ++t) { c += acc[t]; acc[t] = (uint32_t)c; c >>= 32; }
-        }
-    }
-    trim(acc);
-    return acc;
+    return mul_ntt_hip(a, b);
 }
 
 static Mag dev_sqr(const Mag& a) {
     if (a.empty()) return {};
-    if (2 * a.size() <= ((size_t)1 << 23)) return sqr_ntt_hip(a);
-    return dev_mul(a, a);
+    return sqr_ntt_hip(a);
 }
 
 // One batched launch sequence for every pair that fits a transform.
@@ -714,8 +695,8 @@ static std::vector<Mag> dev_mul_many(const std::vector<std::pair<const Mag*, con
 // is one pdh_bs_merge_dev call (batched products and the signed T sum in HBM),
 // each level in a fresh arena, the previous one freed. Only lengths and signs
 // cross PCIe per level. Merging stops where a product would exceed one
-// transform (2^23 limbs); the remaining level is downloaded and the host loop
-// finishes it with blocked products. The returned level has one node when the
+// batched transform (2^23 limbs); the remaining level is downloaded and the
+// host loop finishes it with single products, which the library blocks. The returned level has one node when the
 // device reached the root (its P, never used, is left empty).
 static std::vector<PQT> dev_levels(uint64_t N) {
     const int G = 6;
@@ -886,13 +867,11 @@ public:
     }
     DNum pow2(size_t e) { return shl(one_, e); }
     DNum mul(const DNum& a, const DNum& b) {
-        if (a.ub + b.ub > ((uint64_t)1 << 23)) return up(dev_mul(down(a), down(b)));   // blocked products
         DNum x = alloc(a.ub + b.ub);
         check(pdh_mul_dev(a.p, a.ub, b.p, b.ub, x.p, x.len, st_));
         return x;
     }
     DNum sqr(const DNum& a) {
-        if (2 * a.ub > ((uint64_t)1 << 23)) return mul(a, a);
         DNum x = alloc(2 * a.ub);
         check(pdh_sqr_dev(a.p, a.ub, x.p, x.len, st_));
         return x;

@@ -30,7 +30,7 @@ extern "C" {
 #define PDH_E_HIP        -2   /* a HIP runtime call or kernel launch failed */
 #define PDH_E_ARG        -3   /* bad argument (null pointer, zero position) */
 #define PDH_E_CAPACITY   -4   /* output buffer smaller than required        */
-#define PDH_E_TOO_LARGE  -5   /* transform length beyond the supported 2^24 */
+#define PDH_E_TOO_LARGE  -5   /* a batch product beyond one 2^24 transform  */
 
 /* Text of the last error on this thread ("" if none). */
 const char* pdh_last_error(void);
@@ -45,7 +45,9 @@ int pdh_device_available(void);
 size_t pdh_ntt_length(size_t na, size_t nb);
 
 /* out = a * b, host buffers. out_cap >= na + nb limbs; *out_len receives the
- * normalized length. na == 0 or nb == 0 gives *out_len = 0.
+ * normalized length. na == 0 or nb == 0 gives *out_len = 0. Any size: a
+ * product of more than 2^23 limbs (beyond one 2^24 transform) is assembled
+ * on the device from 2^22-limb block products.
  * Replaces: std::vector<uint32_t> mul_ntt_cuda(const std::vector<uint32_t>&,
  * const std::vector<uint32_t>&)  (reference include/bignum.h:200-201,
  * src/ntt_cuda.cu:179-243). */

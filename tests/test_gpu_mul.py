@@ -9,6 +9,8 @@ from __future__ import annotations
 
 import hashlib
 
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -105,9 +107,42 @@ def test_edge_cases(pdh):
     assert len(pdh.mul_ntt_hip(np.zeros(5, np.uint32), a)) == 0       # zero value, not normalized
     padded = np.concatenate([a, np.zeros(37, np.uint32)])              # leading-zero limbs
     assert np.array_equal(pdh.mul_ntt_hip(padded, a), oracle.mul_ntt(a, a))
-    with pytest.raises(pdh.PdhError) as e:
-        pdh.mul_ntt_hip(np.ones(1 << 22, np.uint32), np.ones((1 << 22) + 1, np.uint32))
-    assert e.value.code == pdh.PDH_E_TOO_LARGE
+    # batch plans hold one transform per product: larger products are refused there
+    dev = [pdh.MulDesc(8, 1 << 22, 8, (1 << 22) + 1, 8, 8)]
+    plan = ctypes.c_void_p()
+    assert pdh.lib().pdh_batch_create((pdh.MulDesc * 1)(*dev), 1, ctypes.byref(plan)) == pdh.PDH_E_TOO_LARGE
+
+
+def _to_int(limbs) -> int:
+    return int.from_bytes(np.ascontiguousarray(limbs, dtype=np.uint32).tobytes(), "little")
+
+
+def _split_product(pdh, a, b, blk):
+    """a*b summed from products of blk-limb pieces, each within one transform."""
+    acc = 0
+    for i in range(0, len(a), blk):
+        for j in range(0, len(b), blk):
+            acc += _to_int(pdh.mul_ntt_hip(a[i:i + blk], b[j:j + blk])) << (32 * (i + j))
+    return acc
+
+
+def test_products_beyond_one_transform(pdh):
+    """na + nb > 2^23 limbs: the library multiplies 2^22-limb blocks and sums
+    them on the device (the reference's seam takes any size,
+    src/ntt_cuda.cu:184-185). Checked against a sum over an independent
+    2^21-limb split and, for all-ones operands (maximal carries across the
+    block seams), the closed form 2^(A+B) - 2^A - 2^B + 1."""
+    a = W.operand(901, (1 << 22) + 5000)
+    b = W.operand(902, (1 << 22) + 3000)
+    got = pdh.mul_ntt_hip(a, b)
+    assert len(got) in (len(a) + len(b), len(a) + len(b) - 1) and got[-1] != 0
+    assert _to_int(got) == _split_product(pdh, a, b, 1 << 21)
+    sq = pdh.sqr_ntt_hip(b)
+    assert _to_int(sq) == _split_product(pdh, b, b, 1 << 21)
+    n, m = (1 << 22) + 7, (1 << 22) + 1
+    ones = pdh.mul_ntt_hip(np.full(n, 0xFFFFFFFF, np.uint32), np.full(m, 0xFFFFFFFF, np.uint32))
+    A, B = 32 * n, 32 * m                       # (2^A - 1)(2^B - 1) without a big multiply
+    assert _to_int(ones) == (1 << (A + B)) - (1 << A) - (1 << B) + 1
 
 
 def test_square_matches_multiply(pdh):
