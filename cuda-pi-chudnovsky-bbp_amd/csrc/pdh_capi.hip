@@ -760,7 +760,11 @@ int batch_create_impl(const pdh_mul_desc* descs, size_t n, pdh_batch** plan, boo
         if (d.na == 0 || d.nb == 0) { p->zero_lens.push_back(d.out_len); continue; }
         if (!d.a || !d.b || !d.out) { batch_free(p); return fail(PDH_E_ARG, "null pointer in descriptor"); }
         const int ln = log2_exact(pdh_ntt_length(d.na, d.nb));
-        if (ln > MAX_LOG) { batch_free(p); return fail(PDH_E_TOO_LARGE, "transform length beyond 2^24 in batch"); }
+        if (ln > MAX_LOG) {
+            batch_free(p);
+            return fail(PDH_E_TOO_LARGE, "batch product of " + std::to_string(d.na) + " x " + std::to_string(d.nb) +
+                                             " limbs needs a transform beyond 2^24");
+        }
         by_n[ln].push_back(i);
     }
     // workspace arena: per job of class n > SMALL_MAX_LOG two spectra + carry scratch

@@ -186,8 +186,9 @@ static SInt sadd(const SInt& a, const SInt& b) {
 // ---- arithmetic backends ------------------------------------------------------------
 // The Newton algorithms below are written once over a backend A with value
 // type A::Num and the operations bits, shl, shr, add, sub (a >= b), cmp, pow2,
-// mul, sqr, zero, inc, dec, mul_u32, from_u128 / to_u128 (small values) and
-// up / down (to and from host magnitudes). HostArith keeps values on the host
+// mul, sqr, zero, inc, dec, mul_u32, from_u128 / to_u128 (small values), up /
+// down (to and from host magnitudes) and tighten (make a value's size bound
+// exact where a loose one would size later products too large). HostArith keeps values on the host
 // and multiplies through the Multiplier (the CPU tests run the algorithms
 // this way with the oracle's multiply); DevArith (further down) keeps them in
 // device memory.
@@ -212,6 +213,7 @@ struct HostArith {
     Mag up(const Mag& a) { return a; }
     Mag down(const Mag& a) { return a; }
     void sync() {}
+    void tighten(Mag&) {}
 };
 
 template <class A> struct SNum { typename A::Num m; int s = 0; };   // sign s in {-1, 0, 1}
@@ -254,7 +256,9 @@ template <class A> static typename A::Num recip(A& o, const typename A::Num& d, 
         return o.from_u128(v);
     }
     const size_t h = L / 2 + 8;
-    return recip_step(o, d, L, recip(o, o.shr(d, L - h), h), L - h);
+    typename A::Num V = recip_step(o, d, L, recip(o, o.shr(d, L - h), h), L - h);
+    o.tighten(V);
+    return V;
 }
 
 // ---- exact floor division --------------------------------------------------------
@@ -325,7 +329,9 @@ template <class A> static typename A::Num rsqrt_approx(A& o, const typename A::N
     const bool neg = o.cmp(sy2, two) > 0;
     N e = neg ? o.sub(sy2, two) : o.sub(two, sy2);
     N corr = o.shr(o.mul(Y0, o.shr(e, c - b)), 2 * E + 1 - c);
-    return neg ? o.sub(Y0, corr) : o.add(Y0, corr);
+    N Y = neg ? o.sub(Y0, corr) : o.add(Y0, corr);
+    o.tighten(Y);
+    return Y;
 }
 
 // x0 = s Y / 2^(E + G) from Y ~ 2^(E + G) / sqrt(s) (the inverse square root
@@ -364,12 +370,15 @@ template <class A> static typename A::Num isqrt(A& o, const typename A::Num& s) 
     return x;
 }
 
+// 10^e by squaring. Bounds are made exact after every step: 10^(2^k) has
+// about 3.3 bits per decimal digit, so bounds doubled from one limb would
+// overstate its size about tenfold.
 template <class A> static typename A::Num pow10(A& o, uint64_t e) {
     typename A::Num result = o.from_u128(1), base = o.from_u128(10);
     while (e) {
-        if (e & 1) result = o.mul(result, base);
+        if (e & 1) { result = o.mul(result, base); o.tighten(result); }
         e >>= 1;
-        if (e) base = o.sqr(base);
+        if (e) { base = o.sqr(base); o.tighten(base); }
     }
     return result;
 }
@@ -425,6 +434,7 @@ template <class A> static DecimalPlan<A> decimal_plan(A& o, size_t xbits) {
     p.Lds.resize(K);
     for (int l = K - 1; l >= 0; --l) {
         p.Ds[l] = l == K - 1 ? pow10(o, p.B) : o.sqr(p.Ds[l + 1]);
+        o.tighten(p.Ds[l]);
         p.Lds[l] = o.bits(p.Ds[l]);
         const typename A::Num d32 = o.shl(p.Ds[l], 32);
         const size_t L = p.Lds[l] + 32;
@@ -432,6 +442,7 @@ template <class A> static DecimalPlan<A> decimal_plan(A& o, size_t xbits) {
         const size_t z = 4 * p.Lds[l + 1] + 32 - 2 * p.Lds[l];
         const size_t sh = L / 2 - 15;
         p.Vs[l] = recip_step(o, d32, L, o.shr(o.sqr(p.Vs[l + 1]), z + sh), sh);
+        o.tighten(p.Vs[l]);
     }
     return p;
 }
@@ -567,10 +578,10 @@ static Mag pi_scaled_t(A& o, const PQT& r, uint64_t prec, std::chrono::steady_cl
     typedef typename A::Num N;
     N one = pow10(o, prec);
     N radicand = o.mul_u32(o.sqr(one), 10005u);
-    o.sync();
+    o.tighten(radicand);
     ts[0] = std::chrono::steady_clock::now();
     N sqrtC = isqrt(o, radicand);
-    o.sync();
+    o.tighten(sqrtC);
     ts[1] = std::chrono::steady_clock::now();
     N numerator = o.mul(o.mul_u32(o.up(r.Q), 426880u), sqrtC);
     return o.down(div_floor(o, numerator, o.up(r.T.m), (N*)nullptr));   // T > 0 for the full range
@@ -814,6 +825,7 @@ public:
     DevArith& operator=(const DevArith&) = delete;
 
     void sync() { check(pdh_stream_sync(st_)); }
+    void tighten(DNum& a) { a.ub = length(a); }
     DNum up(const Mag& a) {
         DNum x = alloc(a.size());
         const uint64_t n = a.size();
@@ -972,17 +984,26 @@ static std::string dev_to_decimal(const Mag& x, const Multiplier&) {
             int32_t* RS = (int32_t*)(TL + n);
             int32_t* TS = RS + n;
             const uint64_t sh = 2 * p.Lds[l] + 32;
+            // products beyond one transform (the top levels of very long
+            // conversions) go one by one through the library's blocked path
+            auto products = [&](const std::vector<pdh_mul_desc>& md) {
+                bool fits = true;
+                for (const pdh_mul_desc& d : md) fits = fits && d.na + d.nb <= ((uint64_t)1 << 23);
+                if (fits) { check(pdh_mul_batch_dev(md.data(), md.size(), st)); return; }
+                for (const pdh_mul_desc& d : md) check(pdh_mul_dev(d.a, d.na, d.b, d.nb, d.out, d.out_len, st));
+                check(pdh_stream_sync(st));
+            };
             std::vector<pdh_mul_desc> md(n);
             for (uint64_t i = 0; i < n; ++i)
                 md[i] = pdh_mul_desc{X + i * C, C, dV, nV, dP + i * (C + nV), PL + i};
-            check(pdh_mul_batch_dev(md.data(), n, st));
+            products(md);
             std::vector<pdh_shr_desc> sd(n);
             for (uint64_t i = 0; i < n; ++i)
                 sd[i] = pdh_shr_desc{dP + i * (C + nV), PL + i, Xn + 2 * i * Cn, Cn, XnL + 2 * i, sh};
             check(pdh_shr_batch_dev(sd.data(), n, st));
             for (uint64_t i = 0; i < n; ++i)
                 md[i] = pdh_mul_desc{Xn + 2 * i * Cn, Cn, dD, nD, dQD + i * (Cn + nD), QDL + i};
-            check(pdh_mul_batch_dev(md.data(), n, st));
+            products(md);
             std::vector<pdh_sadd_desc> ad(n);
             for (uint64_t i = 0; i < n; ++i)
                 ad[i] = pdh_sadd_desc{X + i * C, XL + i, 1, dQD + i * (Cn + nD), QDL + i, -1,

@@ -66,6 +66,20 @@ def test_1e7_digits_engines_agree(pdh, golden):
         assert hashlib.sha256(s1[2:].encode()).hexdigest() == ref["sha256"]
 
 
+@pytest.mark.parametrize("digits", [3 * 10 ** 7, 10 ** 8])
+def test_beyond_1e7_prefix(pdh, golden, digits):
+    """3*10^7 and 10^8 digits: products beyond one transform (the library's
+    blocked path), decimal levels and merge levels too large for a batch.
+    Pi's digits are a prefix property, so the first 10^6 and 10^7 must equal
+    the independent fixtures."""
+    s, _ = pi_dev(pdh, digits)
+    assert len(s) == digits + 2 and s[:2] == "3."
+    assert hashlib.sha256(s[2:2 + 10 ** 6].encode()).hexdigest() == golden["pi_dec_sha256"]["1000000"]["sha256"]
+    ref = golden["pi_dec_sha256"].get("10000000")
+    if ref:
+        assert hashlib.sha256(s[2:2 + 10 ** 7].encode()).hexdigest() == ref["sha256"]
+
+
 def _leaf(k):
     A, B, C = 13591409, 545140134, 10939058860032000
     if k == 0:
