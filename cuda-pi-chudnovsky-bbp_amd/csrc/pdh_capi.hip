@@ -982,14 +982,22 @@ int pdh_bs_merge_dev(const pdh_bs_node* in, size_t n_in, pdh_bs_node* out, int s
     if (pairs) {
         e = hipMemcpyAsync(ws + o_nodes, stage.data(), stage.size(), hipMemcpyHostToDevice, s);
         if (e != hipSuccess) { cleanup(); return fail_hip(e, "pdh_bs_merge_dev: node upload"); }
-        rc = batch_create_impl(jobs.data(), jobs.size(), &plan, true);
+        // products beyond one transform (the top levels of very long runs)
+        // go one by one through the blocked multiply, after the batch
+        std::vector<pdh_mul_desc> small, big;
+        for (const pdh_mul_desc& d : jobs) (d.na + d.nb > ((uint64_t)1 << 23) ? big : small).push_back(d);
+        rc = batch_create_impl(small.data(), small.size(), &plan, true);
         if (rc == PDH_OK) rc = pdh_batch_run(plan, stream);
         if (rc != PDH_OK) { std::string m = pdh_last_error(); cleanup(); return fail(rc, m); }
         {
             std::lock_guard<std::mutex> g(c.mu);
-            e = launch_bs_signed_add((const BsAddNode*)(ws + o_nodes), (const uint32_t*)(ws + o_tmap), tile,
-                                     (uint8_t*)(ws + o_stat), s);
+            for (const pdh_mul_desc& d : big)
+                if ((rc = mul_dev_locked(c, d.a, d.na, d.b, d.nb, false, d.out, d.out_len, s))) break;
+            if (rc == PDH_OK)
+                e = launch_bs_signed_add((const BsAddNode*)(ws + o_nodes), (const uint32_t*)(ws + o_tmap), tile,
+                                         (uint8_t*)(ws + o_stat), s);
         }
+        if (rc != PDH_OK) { std::string m = pdh_last_error(); cleanup(); return fail(rc, m); }
         if (e != hipSuccess) { cleanup(); return fail_hip(e, "pdh_bs_merge_dev: signed add"); }
     }
     if (n_in & 1) {

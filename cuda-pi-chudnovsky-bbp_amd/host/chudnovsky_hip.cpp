@@ -705,15 +705,12 @@ static std::vector<Mag> dev_mul_many(const std::vector<std::pair<const Mag*, con
 // kernel (pdh_bs_groups, one node per 64 consecutive terms); every level above
 // is one pdh_bs_merge_dev call (batched products and the signed T sum in HBM),
 // each level in a fresh arena, the previous one freed. Only lengths and signs
-// cross PCIe per level. Merging stops where a product would exceed one
-// batched transform (2^23 limbs); the remaining level is downloaded and the
-// host loop finishes it with single products, which the library blocks. The returned level has one node when the
-// device reached the root (its P, never used, is left empty).
+// cross PCIe per level (products beyond one transform run blocked inside the
+// merge). The returned level is the root, its P (never used) left empty.
 static std::vector<PQT> dev_levels(uint64_t N) {
     const int G = 6;
     const uint64_t groups = (N + (1u << G) - 1) >> G;
     const size_t cp = (size_t)3 << G, cq = (size_t)5 << G, ct = (size_t)6 << G;
-    const uint64_t LIM = (uint64_t)1 << 23;
     void *dP = nullptr, *dQ = nullptr, *dT = nullptr, *dL = nullptr, *dS = nullptr, *st = nullptr, *arena = nullptr;
     auto release = [&]() { pdh_dev_free(dP); pdh_dev_free(dQ); pdh_dev_free(dT); pdh_dev_free(dL);
                            pdh_dev_free(dS); pdh_dev_free(arena); };
@@ -739,7 +736,6 @@ static std::vector<PQT> dev_levels(uint64_t N) {
         while (lvl.size() > 1) {
             const size_t pairs = lvl.size() / 2, n_out = (lvl.size() + 1) / 2;
             const bool root = lvl.size() == 2;
-            bool fits = true;
             uint64_t words = 0;
             std::vector<uint64_t> cP(n_out), cQ(n_out), cT(n_out);
             for (size_t i = 0; i < pairs; ++i) {
@@ -748,9 +744,7 @@ static std::vector<PQT> dev_levels(uint64_t N) {
                 cP[i] = root ? 0 : A.nP + B.nP;
                 cQ[i] = A.nQ + B.nQ;
                 cT[i] = std::max(A.nT + B.nQ, A.nP + B.nT) + 1;
-                if (cP[i] > LIM || cQ[i] > LIM || cT[i] - 1 > LIM) fits = false;
             }
-            if (!fits) break;
             if (lvl.size() & 1) {
                 const pdh_bs_node& v = lvl.back();
                 cP[pairs] = root ? 0 : v.nP; cQ[pairs] = v.nQ; cT[pairs] = std::max<uint64_t>(v.nT, 1);

@@ -195,7 +195,7 @@ typedef struct pdh_bs_node {
  * skip_p != 0 leaves every output P unwritten (nP = 0; the root's P is never
  * used). On return (the stream is synchronized) out[i].nP, .nQ, .nT, .sT
  * hold the results. Every input needs nP > 0 and nQ > 0; nT = 0 means T = 0.
- * Products of more than 2^23 limbs return PDH_E_TOO_LARGE. */
+ * Products beyond one transform (more than 2^23 limbs) run blocked. */
 int pdh_bs_merge_dev(const pdh_bs_node* in, size_t n_in, pdh_bs_node* out, int skip_p, void* stream);
 
 /* ---- device-resident magnitudes ---------------------------------------------
