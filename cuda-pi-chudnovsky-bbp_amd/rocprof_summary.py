@@ -21,6 +21,7 @@ CALIB = "neg_kernel"
 
 
 def short(name: str) -> str:
+    name = name.replace("(anonymous namespace)::", "")
     return name.split("(")[0].replace("void ", "").replace("pdh::", "")
 
 
