@@ -77,19 +77,31 @@ Ctx& ctx() { static Ctx c; return c; }
 // as hipFree would). An allocation takes the smallest cached block of at
 // least the requested size and at most twice it; at most kCacheMax bytes are
 // kept, and blocks above kBlockMax are never cached.
+//
+// Small requests (up to kSmallMax) take power-of-two classes, and a class
+// that runs dry is refilled with kSlab blocks carved from one allocation;
+// small blocks stay with the process (never trimmed), so the many small
+// tables of a run (job tables, node and tile maps, statuses) cost no
+// allocation calls after the first of each class.
 struct BlockCache {
     std::mutex mu;
     std::multimap<size_t, void*> idle;     // size -> block
     std::map<void*, size_t> size_of;       // every block the cache handed out
-    size_t idle_bytes = 0;
+    size_t idle_bytes = 0;                 // large idle blocks only
 };
 BlockCache& bcache() { static BlockCache b; return b; }
 constexpr size_t kCacheMax = (size_t)8 << 30;
 constexpr size_t kBlockMax = (size_t)2 << 30;
+constexpr size_t kSmallMax = (size_t)1 << 20;
+constexpr int kSlab = 16;
 
-void cache_trim(BlockCache& b) {   // caller holds b.mu
-    for (auto& kv : b.idle) { b.size_of.erase(kv.second); (void)hipFree(kv.second); }
-    b.idle.clear();
+void cache_trim(BlockCache& b) {   // caller holds b.mu; small blocks stay
+    for (auto it = b.idle.begin(); it != b.idle.end();) {
+        if (it->first <= kSmallMax) { ++it; continue; }
+        b.size_of.erase(it->second);
+        (void)hipFree(it->second);
+        it = b.idle.erase(it);
+    }
     b.idle_bytes = 0;
 }
 
@@ -97,7 +109,26 @@ hipError_t cache_alloc(void** p, size_t bytes) {
     bytes = (std::max<size_t>(bytes, 1) + 255) & ~(size_t)255;
     BlockCache& b = bcache();
     std::lock_guard<std::mutex> g(b.mu);
+    if (bytes <= kSmallMax) {
+        size_t cls = 4096;
+        while (cls < bytes) cls <<= 1;
+        auto it = b.idle.find(cls);
+        if (it == b.idle.end()) {
+            char* slab = nullptr;
+            hipError_t e = hipMalloc((void**)&slab, cls * kSlab);
+            if (e != hipSuccess) return e;
+            for (int i = 0; i < kSlab; ++i) {
+                b.size_of[slab + i * cls] = cls;
+                b.idle.emplace(cls, slab + i * cls);
+            }
+            it = b.idle.find(cls);
+        }
+        *p = it->second;
+        b.idle.erase(it);
+        return hipSuccess;
+    }
     auto it = b.idle.lower_bound(bytes);
+    while (it != b.idle.end() && it->first <= kSmallMax) ++it;
     if (it != b.idle.end() && it->first <= 2 * bytes) {
         *p = it->second;
         b.idle_bytes -= it->first;
@@ -121,6 +152,10 @@ void cache_free(void* p) {
     auto it = b.size_of.find(p);
     if (it == b.size_of.end()) { (void)hipFree(p); return; }
     const size_t sz = it->second;
+    if (sz <= kSmallMax) {                 // a slab block: back to its class
+        b.idle.emplace(sz, p);
+        return;
+    }
     if (sz > kBlockMax || b.idle_bytes + sz > kCacheMax) {
         b.size_of.erase(it);
         (void)hipFree(p);
