@@ -258,15 +258,20 @@ __device__ __forceinline__ uint64_t frac64(uint32_t r, uint32_t m, double minv) 
 // bits for odd mo, so three Newton steps reach 40; and the Montgomery one
 // 2^32 mod mo is taken in the lazy range [0, 2 mo) from a 32-bit quotient
 // estimate q within one of floor(2^32 / mo): 2^32 - q mo, formed mod 2^32,
-// lies in [-mo, 2 mo), and one add of mo fixes the negative case.
-__device__ __forceinline__ uint64_t frac64_mont_wave(uint64_t e, uint32_t m, double minv) {
+// lies in [-mo, 2 mo), and one add of mo fixes the negative case. The
+// indices fit 32 bits: positions are at most 2^28, so m = 8k + j < 2^31.
+__device__ __forceinline__ uint64_t frac64_mont_wave(uint32_t e, uint32_t m) {
     const int s = __builtin_ctz(m);
     const uint32_t mo = m >> s;
-    const uint32_t E = (uint32_t)(4 * e) - (uint32_t)s + 64u;
+    const uint32_t E = 4u * e - (uint32_t)s + 64u;
     uint32_t inv = (3u * mo) ^ 2u;
     for (int i = 0; i < 3; ++i) inv *= 2u - mo * inv;
     const uint32_t ninv = 0u - inv;
-    const uint32_t q = (uint32_t)(4294967296.0 * (minv * (double)(1u << s)));
+    // q within one of floor(2^32 / mo): a single-precision reciprocal is off by
+    // at most 2^-22.4 relative, under half a unit while 2^32 / mo < 2^21;
+    // smaller moduli (the first terms of a series) divide exactly
+    const uint32_t q = mo >= 2048u ? (uint32_t)(4294967296.0f * __builtin_amdgcn_rcpf((float)mo))
+                                   : (uint32_t)(0x100000000ull / mo);
     uint32_t r = 0u - q * mo;
     if ((int32_t)r < 0) r += mo;
     const uint32_t two_mo = mo << 1;
@@ -294,21 +299,24 @@ __global__ __launch_bounds__(256) void k_bbp_fx_terms(const BbpSeries* __restric
     const unsigned s = blockIdx.x / bps;
     const uint64_t kb = blockIdx.x - (uint64_t)s * bps;
     const BbpSeries S = st[s];
-    const uint64_t base = kb * 256u * (uint64_t)KT;
-    if (!S.valid || base > S.d) return;
+    const uint64_t base64 = kb * 256u * (uint64_t)KT;
+    if (!S.valid || base64 > S.d) return;
+    // positions are at most 2^28 (the entry points check), so d, k < 2^28 and
+    // m = 8k + j < 2^31: the term indices run in 32 bits
+    const uint32_t d = (uint32_t)S.d, base = (uint32_t)base64;
     uint64_t local = 0;
     for (int i = 0; i < KT; ++i) {
-        const uint64_t k = base + (uint64_t)i * 256u + threadIdx.x;
-        if (k > S.d) break;
-        const uint64_t mi = 8ull * k + S.j;
+        const uint32_t k = base + (uint32_t)i * 256u + threadIdx.x;
+        if (k > d) break;
+        const uint32_t mi = 8u * k + S.j;
         if (mi == 1) continue;                        // 16^d mod 1 = 0
-        const double m = (double)mi;
-        const double minv = recip_f64(m);
-        if (mi < ((uint64_t)1 << 30) && k < S.d) {
-            local += frac64_mont_wave(S.d - k, (uint32_t)mi, minv);
+        if (mi < (1u << 30) && k < d) {
+            local += frac64_mont_wave(d - k, mi);
         } else {                                      // 16^0 = 1, or m >= 2^30
-            const uint32_t r = k == S.d ? 1u : (uint32_t)modpow16_exact(S.d - k, m, minv);
-            local += frac64(r, (uint32_t)mi, minv);
+            const double m = (double)mi;
+            const double minv = recip_f64(m);
+            const uint32_t r = k == d ? 1u : (uint32_t)modpow16_exact(d - k, m, minv);
+            local += frac64(r, mi, minv);
         }
     }
     for (int o = 32; o > 0; o >>= 1) local += __shfl_xor(local, o);
