@@ -368,7 +368,12 @@ def main() -> int:
     dom_B = alg_bytes(dom, na, nb, N, T, eng)
     roofline = {"bound": "hbm", "kernel": dom, "achieved": dom_B / dom_avg_s / 1e9, "peak": HBM_PEAK / 1e9,
                 "unit": "GB/s", "frac": dom_B / dom_avg_s / HBM_PEAK, "traffic": read_traffic(dom),
-                "alg_bytes_per_launch": dom_B}
+                "alg_bytes_per_launch": dom_B,
+                # the modular butterflies are integer VALU work: the kernel moves
+                # its algorithmic bytes once and waits on the VALU, not on HBM
+                "note": "integer-VALU-bound: HBM bytes equal the algorithmic bytes "
+                        "(profiles/pmc_traffic_c2_mul.txt); VALU busy 64% with ~98% of its "
+                        "instructions modular arithmetic (profiles/pmc_sq_c2_mul.txt)"}
     B_mul = 96 * N + 8 * (na + nb)
     roofline_e2e = {"bound": "hbm", "alg_bytes": B_mul, "achieved": B_mul / (ms * 1e-3) / 1e9,
                     "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": B_mul / (ms * 1e-3) / HBM_PEAK}
