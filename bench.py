@@ -438,6 +438,20 @@ def main() -> int:
         M = W.bbp_mulmod_count(np.arange(lo, hi + 1, dtype=np.uint64))
         extra["bbp_1_65536"] = {"ms": bms, "mulmod_per_s": M / (bms * 1e-3), "mulmods": M,
                                 "bit_exact": pdh.to_hex(vals.cpu().numpy()) == want}
+        # the same range through the range entry point (digit blocks)
+        rvals = torch.zeros(cnt, dtype=torch.uint8, device=dev)
+        pdh.bbp_range_dev(lo, 64, rvals.data_ptr(), sp)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        pdh.bbp_range_dev(lo, cnt, rvals.data_ptr(), sp)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        rms = e0.elapsed_time(e1)
+        extra["bbp_range_blocks_1_65536"] = {
+            "ms": rms, "entry": "pdh_bbp_range_dev", "digits_per_evaluation": 7,
+            "note": "one fixed-point evaluation per 7 positions, the rest by the per-position path; "
+                    "not the per-position workload above",
+            "bit_exact": pdh.to_hex(rvals.cpu().numpy()) == want}
         # BBP at the 64 golden positions sampled uniformly in [1e6, 1e8]
         gs = golden["bbp_sampled"]
         spos = np.array(sorted(gs["positions"]), dtype=np.uint64)

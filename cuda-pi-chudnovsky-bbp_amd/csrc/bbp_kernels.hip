@@ -207,13 +207,13 @@ __global__ __launch_bounds__(256) void k_bbp(const uint64_t* __restrict__ pos, u
 // reference's operation sequence, unchanged.
 struct BbpSeries { uint64_t d; double sum; uint32_t j; uint32_t valid; };
 
-__global__ __launch_bounds__(256) void k_bbp_init(const uint64_t* __restrict__ pos, uint64_t start, size_t count,
-                                                  BbpSeries* __restrict__ st, uint64_t* __restrict__ acc) {
+__global__ __launch_bounds__(256) void k_bbp_init(const uint64_t* __restrict__ pos, uint64_t start, uint32_t stride,
+                                                  size_t count, BbpSeries* __restrict__ st, uint64_t* __restrict__ acc) {
     const size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= 4 * count) return;
     const size_t pi = s >> 2;
     const int q = (int)(s & 3);
-    const uint64_t position = pos ? pos[pi] : start + pi;
+    const uint64_t position = pos ? pos[pi] : start + pi * stride;
     const bool valid = position >= 1 && position <= ((uint64_t)1 << 28);
     st[s] = BbpSeries{valid ? position - 1 : 0, 0.0, q == 0 ? 1u : (uint32_t)(q + 3), valid ? 1u : 0u};
     if (acc) acc[s] = 0;
@@ -327,6 +327,27 @@ __global__ __launch_bounds__(256) void k_bbp_fx_terms(const BbpSeries* __restric
         atomicAdd((unsigned long long*)&acc[s], (unsigned long long)(part[0] + part[1] + part[2] + part[3]));
 }
 
+// T = 4 F1 - 2 F4 - F5 - F6 (mod 2^64) of one position: its four head sums
+// plus the 14 tail terms floor(2^(64-4i) / m).
+__device__ __forceinline__ uint64_t bbp_fx_combine(const BbpSeries* P, const uint64_t* a, uint64_t d) {
+    uint64_t F[4];
+    for (int q = 0; q < 4; ++q) {
+        uint64_t f = a[q];
+        for (int i = 1; i <= 14; ++i) f += ((uint64_t)1 << (64 - 4 * i)) / (8ull * (d + i) + P[q].j);
+        F[q] = f;
+    }
+    return 4 * F[0] - 2 * F[1] - F[2] - F[3];
+}
+
+__device__ __forceinline__ uint64_t bbp_mu(uint64_t d) { return 24584ull * (d + 1) + 262144ull; }
+
+// Distance of a 64-bit fraction from the nearest digit boundary (a multiple of 2^60).
+__device__ __forceinline__ uint64_t bbp_edge_dist(uint64_t T) {
+    const uint64_t cell = (uint64_t)1 << 60;
+    const uint64_t x = T & (cell - 1);
+    return x < cell - x ? x : cell - x;
+}
+
 __global__ __launch_bounds__(256) void k_bbp_fx_finish(const BbpSeries* __restrict__ st, size_t count,
                                                        const uint64_t* __restrict__ acc,
                                                        uint8_t* __restrict__ out, int odd_undecided) {
@@ -335,19 +356,43 @@ __global__ __launch_bounds__(256) void k_bbp_fx_finish(const BbpSeries* __restri
     const BbpSeries* P = st + 4 * pi;
     if (!P[0].valid) { out[pi] = 0xFF; return; }
     const uint64_t d = P[0].d;
-    uint64_t F[4];
-    for (int q = 0; q < 4; ++q) {
-        uint64_t f = acc[4 * pi + q];
-        for (int i = 1; i <= 14; ++i) f += ((uint64_t)1 << (64 - 4 * i)) / (8ull * (d + i) + P[q].j);
-        F[q] = f;
-    }
-    const uint64_t T = 4 * F[0] - 2 * F[1] - F[2] - F[3];
-    const uint64_t cell = (uint64_t)1 << 60;
-    const uint64_t x = T & (cell - 1);
-    const uint64_t dist = x < cell - x ? x : cell - x;
-    const uint64_t mu = 24584ull * (d + 1) + 262144ull;
-    const bool decided = dist > 2 * mu && !(odd_undecided && (pi & 1));
+    const uint64_t T = bbp_fx_combine(P, acc + 4 * pi, d);
+    const bool decided = bbp_edge_dist(T) > 2 * bbp_mu(d) && !(odd_undecided && (pi & 1));
     out[pi] = decided ? (uint8_t)(T >> 60) : (uint8_t)0xFE;
+}
+
+// Digit blocks of a contiguous range. Anchor t sits at position start + G t
+// (d = position - 1) and gives the digits of the G positions from there on.
+// With V(d) = frac(16^d pi), V(d + i) = frac(16^i V(d)), so 16^i T mod 2^64
+// approximates 2^64 V(d + i) within 16^i e1, where e1 = 4 (d + 15) + 2
+// bounds |T - 2^64 V(d)|: the fixed-point floors lose under 4 (d + 15) units
+// after the combination and the series' terms beyond the 14 tail terms add
+// under 2 (8 sum_{i >= 15} 16^(16 - i) / (8 (d + i)) < 18 / (d + 15)). The
+// reference's FP64 value at d + i is within Mu(d + i) of the same 2^64
+// V(d + i) (Mu already holds the floors' 4 (d + 15) and these 2 units beside
+// its rounding terms). So when 16^i T mod 2^64 is farther than 16^i e1 +
+// Mu(d + i) from every digit boundary, the reference's digit at d + i is its
+// top four bits. Digit 0 keeps the per-position rule (2 Mu(d)). The host picks
+// G so that 16^(G-1) e1 stays below 2^46: e1 <= 2^31 and Mu < 2^43 keep every
+// margin below 2^60, with no overflow.
+__global__ __launch_bounds__(256) void k_bbp_fx_block(const BbpSeries* __restrict__ st, size_t anchors, uint32_t G,
+                                                      size_t nout, const uint64_t* __restrict__ acc,
+                                                      uint8_t* __restrict__ out, int odd_undecided) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= anchors) return;
+    const BbpSeries* P = st + 4 * t;
+    const uint64_t d = P[0].d;
+    const uint64_t T = P[0].valid ? bbp_fx_combine(P, acc + 4 * t, d) : 0;
+    const uint64_t e1 = 4 * (d + 15) + 2;
+    for (uint32_t i = 0; i < G; ++i) {
+        const size_t idx = t * G + i;
+        if (idx >= nout) break;
+        if (!P[0].valid) { out[idx] = 0xFF; continue; }
+        const uint64_t Ti = T << (4 * i);
+        const uint64_t margin = i == 0 ? 2 * bbp_mu(d) : (e1 << (4 * i)) + bbp_mu(d + i);
+        const bool decided = bbp_edge_dist(Ti) > margin && !(odd_undecided && (idx & 1));
+        out[idx] = decided ? (uint8_t)(Ti >> 60) : (uint8_t)0xFE;
+    }
 }
 
 __global__ __launch_bounds__(256) void k_bbp_terms(const BbpSeries* __restrict__ st, size_t nseries, uint64_t k0,
@@ -447,12 +492,13 @@ size_t bbp_few_ws_bytes(size_t count) {
 
 // dmax: the largest d = position - 1 among valid positions (known to the host).
 hipError_t launch_bbp_fx(const uint64_t* d_pos, uint64_t start, size_t count, uint64_t dmax, uint8_t* d_out,
-                         void* ws, int odd_undecided, hipStream_t s) {
+                         void* ws, int odd_undecided, hipStream_t s, uint32_t block, size_t nout) {
     const size_t S = 4 * count;
     BbpSeries* st = (BbpSeries*)ws;
     uint64_t* acc = (uint64_t*)((char*)ws + ((S * sizeof(BbpSeries) + 255) & ~(size_t)255));
-    BBP_LAUNCH("k_bbp_init", s, k_bbp_init, dim3((unsigned)((S + 255) / 256)), dim3(256), 0, s, d_pos, start, count,
-               st, acc);
+    const uint32_t stride = d_pos ? 1u : block;
+    BBP_LAUNCH("k_bbp_init", s, k_bbp_init, dim3((unsigned)((S + 255) / 256)), dim3(256), 0, s, d_pos, start, stride,
+               count, st, acc);
     // terms per lane: at least 32, more while the grid passes 2^17 blocks (and
     // a series has more than one): few long blocks beat many short ones, most
     // of them empty for the shorter series (measured on 1..65,536: 2^23-block
@@ -462,8 +508,12 @@ hipError_t launch_bbp_fx(const uint64_t* d_pos, uint64_t start, size_t count, ui
     while (bps > 1 && bps * S > ((uint64_t)1 << 17)) { KT *= 2; bps = (dmax + 1 + 256ull * KT - 1) / (256ull * KT); }
     BBP_LAUNCH("k_bbp_fx_terms", s, k_bbp_fx_terms, dim3((unsigned)(bps * S)), dim3(256), 0, s,
                (const BbpSeries*)st, (unsigned)bps, KT, acc);
-    BBP_LAUNCH("k_bbp_fx_finish", s, k_bbp_fx_finish, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s,
-               (const BbpSeries*)st, count, (const uint64_t*)acc, d_out, odd_undecided);
+    if (d_pos || block <= 1)
+        BBP_LAUNCH("k_bbp_fx_finish", s, k_bbp_fx_finish, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s,
+                   (const BbpSeries*)st, count, (const uint64_t*)acc, d_out, odd_undecided);
+    else
+        BBP_LAUNCH("k_bbp_fx_block", s, k_bbp_fx_block, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s,
+                   (const BbpSeries*)st, count, block, nout, (const uint64_t*)acc, d_out, odd_undecided);
     return hipSuccess;
 }
 
@@ -473,8 +523,8 @@ hipError_t launch_bbp_chunked(const uint64_t* d_pos, uint64_t start, size_t coun
     const int log_ch = chunk_log(S);
     BbpSeries* st = (BbpSeries*)ws;
     double* buf = (double*)((char*)ws + ((S * sizeof(BbpSeries) + 255) & ~(size_t)255));
-    BBP_LAUNCH("k_bbp_init", s, k_bbp_init, dim3((unsigned)((S + 255) / 256)), dim3(256), 0, s, d_pos, start, count,
-               st, (uint64_t*)nullptr);
+    BBP_LAUNCH("k_bbp_init", s, k_bbp_init, dim3((unsigned)((S + 255) / 256)), dim3(256), 0, s, d_pos, start, 1u,
+               count, st, (uint64_t*)nullptr);
     const uint64_t ch = (uint64_t)1 << log_ch;
     const size_t lanes = S << log_ch;
     for (uint64_t k0 = 0; k0 <= dmax; k0 += ch) {

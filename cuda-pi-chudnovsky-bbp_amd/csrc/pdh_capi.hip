@@ -58,6 +58,8 @@ struct Ctx {
     uint64_t* bbp_dmax = nullptr;
     uint64_t* fix_pos = nullptr; size_t fix_pos_n = 0;
     uint8_t* fix_val = nullptr; size_t fix_val_n = 0;
+    uint64_t* und_pos = nullptr; size_t und_pos_n = 0;     // undecided positions of a digit-block range
+    uint8_t* und_val = nullptr; size_t und_val_n = 0;
     int bbp_exact = 0;               // pdh_bbp_set_exact mode
     uint64_t bbp_exact_count = 0;    // positions computed by the ordered path
     hipStream_t stream = nullptr;
@@ -424,13 +426,67 @@ int mul_host(const u32* a, size_t na, const u32* b, size_t nb, bool square,
 
 constexpr uint64_t BBP_POS_MAX = (uint64_t)1 << 28;   // 8(pos-1)+6 < 2^31
 
+// Digits per anchor of a contiguous range whose largest d (position - 1) is
+// dmax: the largest G <= 8 with 16^(G-1) e1 <= 2^46, e1 = 4 (dmax + 15) + 2
+// (the fixed-point error bound of an anchor). The last digit of a block is
+// then undecided with probability about 2^-13; G is 7 up to d = 65,535 and
+// 5 at d = 10^8.
+uint32_t bbp_block_digits(uint64_t dmax) {
+    const uint64_t e1 = 4 * (dmax + 15) + 2;
+    uint32_t G = 1;
+    while (G < 8 && (e1 << (4 * G)) <= ((uint64_t)1 << 46)) ++G;
+    return G;
+}
+
+int bbp_few_locked(Ctx& c, const uint64_t* d_pos, const uint64_t* h_pos, uint64_t start, size_t count,
+                   uint64_t dmax, uint8_t* d_out, uint8_t* h_vals, hipStream_t st, bool blocks = false);
+
+// A contiguous range by digit blocks: one fixed-point evaluation per G
+// positions (k_bbp_fx_block), then the per-position paths for the digits
+// left undecided.
+int bbp_blocks_locked(Ctx& c, uint64_t start, size_t count, uint32_t G, uint8_t* d_out, uint8_t* h_vals,
+                      hipStream_t st) {
+    const size_t A = (count + G - 1) / G;
+    int rc = grow((char**)&c.bbp_ws, &c.bbp_ws_bytes, bbp_few_ws_bytes(A), 1);
+    if (rc) return rc;
+    hipError_t e = launch_bbp_fx(nullptr, start, A, start - 1 + (A - 1) * (uint64_t)G, d_out, c.bbp_ws,
+                                 c.bbp_exact == 2, st, G, count);
+    if (e != hipSuccess) return fail_hip(e, "bbp launch");
+    HIPCALL(hipMemcpyAsync(h_vals, d_out, count, hipMemcpyDeviceToHost, st));
+    HIPCALL(hipStreamSynchronize(st));
+    std::vector<size_t> und;
+    std::vector<uint64_t> upos;
+    for (size_t i = 0; i < count; ++i)
+        if (h_vals[i] == 0xFE) { und.push_back(i); upos.push_back(start + i); }
+    if (und.empty()) return PDH_OK;
+    const size_t nu = und.size();
+    if ((rc = grow((char**)&c.und_pos, &c.und_pos_n, nu, 8))) return rc;
+    if ((rc = grow((char**)&c.und_val, &c.und_val_n, nu, 1))) return rc;
+    HIPCALL(hipMemcpyAsync(c.und_pos, upos.data(), nu * 8, hipMemcpyHostToDevice, st));
+    std::vector<uint8_t> fixed(nu);
+    // under the test mode 2 (every odd index undecided) those digits all take the ordered path
+    const int mode = c.bbp_exact;
+    if (mode == 2) c.bbp_exact = 1;
+    rc = bbp_few_locked(c, c.und_pos, upos.data(), 0, nu, upos.back() - 1, c.und_val, fixed.data(), st);
+    c.bbp_exact = mode;
+    if (rc) return rc;
+    for (size_t i = 0; i < nu; ++i) h_vals[und[i]] = fixed[i];
+    HIPCALL(hipMemcpyAsync(d_out, h_vals, count, hipMemcpyHostToDevice, st));
+    HIPCALL(hipStreamSynchronize(st));
+    return PDH_OK;
+}
+
 // The fixed-point path, then the ordered FP64 path for every position it
 // leaves undecided (or for all of them when pdh_bbp_set_exact is 1). d_pos (or start for a contiguous
 // range) and d_out are device buffers; h_pos, when given, holds the same
 // positions on the host. The values end in d_out and in h_vals. Synchronises
-// `st`.
+// `st`. A contiguous range with `blocks` set goes by digit blocks.
 int bbp_few_locked(Ctx& c, const uint64_t* d_pos, const uint64_t* h_pos, uint64_t start, size_t count,
-                   uint64_t dmax, uint8_t* d_out, uint8_t* h_vals, hipStream_t st) {
+                   uint64_t dmax, uint8_t* d_out, uint8_t* h_vals, hipStream_t st, bool blocks) {
+    if (blocks && !d_pos && c.bbp_exact != 1 && count >= 16) {
+        const uint32_t G = bbp_block_digits(dmax);
+        if (G > 1) return bbp_blocks_locked(c, start, count, G, d_out, h_vals, st);
+    }
     int rc = grow((char**)&c.bbp_ws, &c.bbp_ws_bytes, bbp_few_ws_bytes(count), 1);
     if (rc) return rc;
     if (c.bbp_exact == 1) {
@@ -513,7 +569,7 @@ int bbp_host(const uint64_t* positions, uint64_t start, size_t count, uint8_t* v
     if (c.bbp_exact != 3) {
         const uint64_t dmax = positions ? sorted.back() - 1 : start + count - 2;
         rc = bbp_few_locked(c, positions ? c.st_pos : nullptr, positions ? sorted.data() : nullptr, start, count,
-                            dmax, c.st_val, tmp.data(), c.stream);
+                            dmax, c.st_val, tmp.data(), c.stream, !positions);
         if (rc) return rc;
     } else {
         hipError_t e = launch_bbp(positions ? c.st_pos : nullptr, start, count, c.st_val, c.bbp_next, c.stream);
@@ -617,6 +673,23 @@ int pdh_sqr_dev(const uint32_t* d_a, size_t na, uint32_t* d_out, uint64_t* d_out
 
 int pdh_bbp_range(uint64_t start, uint32_t count, uint8_t* values) {
     return bbp_host(nullptr, start, count, values);
+}
+
+int pdh_bbp_range_dev(uint64_t start, uint32_t count, uint8_t* d_values, void* stream) {
+    if (count == 0) return PDH_OK;
+    if (!d_values) return fail(PDH_E_ARG, "null output pointer");
+    if (start < 1 || start - 1 + count > BBP_POS_MAX) return fail(PDH_E_ARG, "BBP position outside [1, 2^28]");
+    Ctx& c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    int rc = init_locked(c);
+    if (rc) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    std::vector<uint8_t> tmp(count);
+    if (c.bbp_exact != 3)
+        return bbp_few_locked(c, nullptr, nullptr, start, count, start + count - 2, d_values, tmp.data(), st, true);
+    hipError_t e = launch_bbp(nullptr, start, count, d_values, c.bbp_next, st);
+    if (e != hipSuccess) return fail_hip(e, "bbp launch");
+    return PDH_OK;
 }
 
 int pdh_bbp_positions(const uint64_t* positions, size_t n, uint8_t* values) {

@@ -123,12 +123,16 @@ size_t carry_ws_bytes(size_t nlimbs);
 // reference's FP64 rounding could decide it differently; launch_bbp_chunked
 // computes terms in parallel and folds them per series in the reference's
 // FP64 order (odd_undecided, for tests, also marks every odd index
+// undecided). With block > 1 and no d_pos, launch_bbp_fx evaluates the
+// `count` anchors start + block t and writes the digits of the nout
+// positions start .. start + nout - 1 (block of them per anchor, 0xFE where
 // undecided). launch_bbp_dmax: atomic max of position - 1 into *d_dmax
 // (zeroed by the caller).
 hipError_t launch_bbp(const uint64_t* d_pos, uint64_t start, size_t count, uint8_t* d_out, uint64_t* d_next,
                       hipStream_t s);
 hipError_t launch_bbp_fx(const uint64_t* d_pos, uint64_t start, size_t count, uint64_t dmax,
-                         uint8_t* d_out, void* ws, int odd_undecided, hipStream_t s);
+                         uint8_t* d_out, void* ws, int odd_undecided, hipStream_t s, uint32_t block = 1,
+                         size_t nout = 0);
 hipError_t launch_bbp_chunked(const uint64_t* d_pos, uint64_t start, size_t count, uint64_t dmax,
                               uint8_t* d_out, void* ws, hipStream_t s);
 hipError_t launch_bbp_dmax(const uint64_t* d_pos, size_t count, uint64_t* d_dmax, hipStream_t s);

@@ -140,12 +140,8 @@ struct pdh_server {
         Job job;
         job.reqs = {r};
         job.stream = bbp_stream;
-        std::vector<uint64_t> pos(r->count);
-        for (uint32_t i = 0; i < r->count; ++i) pos[i] = r->start + i;
-        int rc = pdh_dev_alloc(8 * (size_t)r->count, &job.d_pos);
-        if (!rc) rc = pdh_dev_alloc(r->count, &job.d_val);
-        if (!rc) rc = pdh_copy_htod(job.d_pos, pos.data(), 8 * (size_t)r->count, bbp_stream);
-        if (!rc) rc = pdh_bbp_positions_dev((const uint64_t*)job.d_pos, r->count, (uint8_t*)job.d_val, bbp_stream);
+        int rc = pdh_dev_alloc(r->count, &job.d_val);
+        if (!rc) rc = pdh_bbp_range_dev(r->start, r->count, (uint8_t*)job.d_val, bbp_stream);
         if (rc) {
             pdh_dev_free(job.d_pos);
             pdh_dev_free(job.d_val);

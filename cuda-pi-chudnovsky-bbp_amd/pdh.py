@@ -33,7 +33,7 @@ PDH_E_TOO_LARGE = -5
 
 EXPORTED = (
     "pdh_last_error", "pdh_device_available", "pdh_ntt_length", "pdh_mul", "pdh_sqr",
-    "pdh_mul_dev", "pdh_sqr_dev", "pdh_bbp_range", "pdh_bbp_positions",
+    "pdh_mul_dev", "pdh_sqr_dev", "pdh_bbp_range", "pdh_bbp_range_dev", "pdh_bbp_positions",
     "pdh_bbp_positions_dev", "pdh_launch_count", "pdh_timing_enable", "pdh_timing_read", "pdh_probe_mad_rate",
     "pdh_batch_create", "pdh_batch_run", "pdh_batch_capture", "pdh_batch_destroy", "pdh_mul_batch_dev",
     "pdh_tune_get", "pdh_tune_set", "pdh_autotune", "pdh_dev_alloc", "pdh_dev_free", "pdh_copy_htod",
@@ -110,6 +110,7 @@ def lib() -> ctypes.CDLL:
         L.pdh_bbp_range.argtypes = [ctypes.c_uint64, ctypes.c_uint32, _u8p]
         L.pdh_bbp_positions.argtypes = [_u64p, _sz, _u8p]
         L.pdh_bbp_positions_dev.argtypes = [_vp, _sz, _vp, _vp]
+        L.pdh_bbp_range_dev.argtypes = [ctypes.c_uint64, ctypes.c_uint32, _vp, _vp]
         L.pdh_launch_count.restype = ctypes.c_uint64
         L.pdh_bs_groups.argtypes = [ctypes.c_uint64, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp]
         L.pdh_bs_groups.restype = ctypes.c_int
@@ -144,7 +145,7 @@ def lib() -> ctypes.CDLL:
         L.pdh_batch_capture.argtypes = [_vp, _vp]
         L.pdh_batch_destroy.argtypes = [_vp]
         L.pdh_mul_batch_dev.argtypes = [ctypes.POINTER(MulDesc), _sz, _vp]
-        for name in ("pdh_mul", "pdh_sqr", "pdh_mul_dev", "pdh_sqr_dev", "pdh_bbp_range",
+        for name in ("pdh_mul", "pdh_sqr", "pdh_mul_dev", "pdh_sqr_dev", "pdh_bbp_range", "pdh_bbp_range_dev",
                      "pdh_bbp_positions", "pdh_bbp_positions_dev", "pdh_batch_create", "pdh_batch_run",
                      "pdh_batch_capture", "pdh_batch_destroy", "pdh_mul_batch_dev"):
             getattr(L, name).restype = ctypes.c_int
@@ -298,6 +299,12 @@ def bbp_values_positions(positions) -> np.ndarray:
 
 def bbp_positions_dev(d_pos: int, n: int, d_out: int, stream: int = 0) -> None:
     _check(lib().pdh_bbp_positions_dev(d_pos, n, d_out, stream or None))
+
+
+def bbp_range_dev(start: int, count: int, d_out: int, stream: int = 0) -> None:
+    """Digit values of positions start .. start + count - 1 into device memory
+    (digit blocks: one fixed-point evaluation per few positions)."""
+    _check(lib().pdh_bbp_range_dev(start, count, d_out, stream or None))
 
 
 def bs_groups(n_terms: int, group_log: int, d_P: int, d_Q: int, d_T: int, d_len: int, d_sign: int,

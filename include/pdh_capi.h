@@ -143,8 +143,20 @@ int pdh_autotune(int n_lo, int n_hi, int reps, void* stream);
  * buffer. Position 1 is the first hex digit after the point (value 2).
  * Replaces: std::string bbp_hex_digits_cuda(uint64_t start, uint32_t count)
  * (reference include/bbp.h:34, src/bbp_cuda.cu:40-70); the caller maps
- * values to '0'-'9','A'-'F' as src/bbp_cpu.cpp:20-22 does. */
+ * values to '0'-'9','A'-'F' as src/bbp_cpu.cpp:20-22 does. A range of 16
+ * or more positions goes by digit blocks: one exact 64-bit fixed-point
+ * evaluation at every G-th position (G = 7 up to position 65,536, 5 at
+ * 10^8) decides the digits of the G positions from it on wherever the
+ * accumulated error bound keeps them clear of a digit boundary; the rest
+ * take the per-position path of pdh_bbp_positions_dev. The digits equal the
+ * reference's position for position. */
 int pdh_bbp_range(uint64_t start, uint32_t count, uint8_t* values);
+
+/* Device-output variant of pdh_bbp_range (same digit blocks): d_values is a
+ * device buffer of `count` bytes, the work is enqueued on `stream`, which
+ * the call synchronises. Replaces the same bbp_hex_digits_cuda
+ * (include/bbp.h:34) for callers that keep the digits in device memory. */
+int pdh_bbp_range_dev(uint64_t start, uint32_t count, uint8_t* d_values, void* stream);
 
 /* Digit values at arbitrary 1-based positions (host buffers). New: the
  * reference only offers contiguous blocks (bbp.h:27-34) or one position

@@ -89,3 +89,55 @@ def test_workload_mulmod_count_matches_oracle():
         assert W.bbp_mulmod_count([pos]) == oracle.bbp_mulmod_count_brute(pos) == oracle.bbp_mulmod_count(pos)
     ps = np.array([5, 99, 65536, 10**6, 10**8, 1 << 28], dtype=np.uint64)
     assert W.bbp_mulmod_count(ps) == sum(oracle.bbp_mulmod_count(int(p)) for p in ps)
+
+
+# ---- digit blocks (k_bbp_fx_block): one evaluation decides G positions ----
+
+def block_digits(dmax: int) -> int:
+    """The host's G: the largest G <= 8 with 16^(G-1) e1(dmax) <= 2^46."""
+    e1 = 4 * (dmax + 15) + 2
+    g = 1
+    while g < 8 and (e1 << (4 * g)) <= (1 << 46):
+        g += 1
+    return g
+
+
+def fx_block(anchor: int, G: int):
+    """Digits of positions anchor .. anchor + G - 1 from the anchor's T, None
+    where the margin 16^i e1 + Mu(d + i) (2 Mu(d) for i = 0) is not cleared."""
+    d = anchor - 1
+    F = [fx_series(j, d) for j in (1, 4, 5, 6)]
+    T = (4 * F[0] - 2 * F[1] - F[2] - F[3]) % M64
+    e1 = 4 * (d + 15) + 2
+    cell = 1 << 60
+    out = []
+    for i in range(G):
+        Ti = (T << (4 * i)) % M64
+        x = Ti & (cell - 1)
+        dist = min(x, cell - x)
+        margin = 2 * (24584 * (d + 1) + 262144) if i == 0 else (e1 << (4 * i)) + 24584 * (d + i + 1) + 262144
+        assert margin < cell // 2
+        out.append(Ti >> 60 if dist > margin else None)
+    return out
+
+
+def test_block_digits_choice():
+    assert block_digits(65535) == 7 and block_digits(10**8 - 1) == 5 and block_digits(0) == 8
+    assert block_digits((1 << 28) - 1) == 4
+
+
+@pytest.mark.parametrize("start,count", [(1, 700), (4090, 140), (20000, 40)])
+def test_block_digits_equal_reference(start, count):
+    """Every digit a block decides equals the reference's digit at that
+    position; nearly all are decided."""
+    G = block_digits(start + count - 2)
+    got = []
+    for a in range(start, start + count, G):
+        got += fx_block(a, G)
+    got = got[:count]
+    want = oracle.bbp_values_positions(list(range(start, start + count)))
+    decided = [i for i, g in enumerate(got) if g is not None]
+    assert len(decided) >= count - 2
+    for i in decided:
+        assert got[i] == want[i], start + i
+

@@ -92,7 +92,31 @@ def test_sampled_4096_positions(pdh):
     print("ordered fallback positions:", pdh.bbp_exact_count() - before)
 
 
+@pytest.mark.parametrize("start,count", [(1_000_000, 256), (50_000_000, 200), (100_000_000, 64),
+                                         ((1 << 28) - 99, 100)])
+def test_range_blocks_equal_per_position(pdh, golden, start, count):
+    """Digit blocks (pdh_bbp_range_dev, pdh_bbp_range: one fixed-point
+    evaluation per G positions) against the per-position path at the same
+    positions, up to the largest position the entry points take."""
+    import torch
+    want = pdh.bbp_values_positions(np.arange(start, start + count, dtype=np.uint64))
+    if start == 1_000_000:
+        assert pdh.to_hex(want[:10]) == golden["bailey_1e6"]
+    dv = torch.full((count,), 0x77, dtype=torch.uint8, device="cuda")
+    pdh.bbp_range_dev(start, count, dv.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    assert np.array_equal(dv.cpu().numpy(), want)
+    assert np.array_equal(pdh.bbp_values_range(start, count), want)
+
+
 def test_bad_positions_raise(pdh):
+    import torch
+    dv = torch.zeros(16, dtype=torch.uint8, device="cuda")
+    for start in ((1 << 28) - 5, 0):
+        with pytest.raises(pdh.PdhError, match="outside"):
+            pdh.bbp_range_dev(start, 10, dv.data_ptr())
+    with pytest.raises(pdh.PdhError, match="null"):
+        pdh.bbp_range_dev(1, 10, 0)
     with pytest.raises(pdh.PdhError):
         pdh.bbp_values_positions([0])
     with pytest.raises(pdh.PdhError):
@@ -159,6 +183,11 @@ def test_fallback_plumbing(pdh, golden, exact_mode):
     before = pdh.bbp_exact_count()
     assert pdh.bbp_hex_digits_hip(1, 64) == golden["pi_hex_64"]
     assert pdh.bbp_exact_count() == before + 32
+    dv = torch.zeros(64, dtype=torch.uint8, device="cuda")
+    pdh.bbp_range_dev(1, 64, dv.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    assert pdh.to_hex(dv.cpu().numpy()) == golden["pi_hex_64"]
+    assert pdh.bbp_exact_count() == before + 64
     rng = np.random.default_rng(17)
     pos = rng.integers(1, 200_000, size=301).astype(np.uint64)
     want = oracle.bbp_values_positions(pos, threads=8)
