@@ -242,11 +242,14 @@ __device__ __forceinline__ uint64_t frac64(uint32_t r, uint32_t m, double minv) 
 
 // floor((16^e mod m) 2^64 / m) for e >= 1 and 1 < m < 2^30, with no division.
 // With m = 2^s mo (mo odd) and r = 16^e mod m = 2^s r_o, r_o = 2^(4e - s) mod
-// mo, the fraction is floor(r_o 2^64 / mo) = F, and F mo = r_o 2^64 - rem
-// with rem = r_o 2^64 mod mo = 2^(4e - s + 64) mod mo. So F = -rem mo^-1 mod
-// 2^64 (exact: F < 2^64), and rem comes from the square-and-double ladder run
-// on E = 4e - s + 64 -- the same bit count as 4e - s but for the last few
-// terms of a series. mo^-1 mod 2^64 is one 64-bit Newton step from the 32-bit inverse.
+// mo, the fraction is F = floor(r_o 2^64 / mo), taken as two 32-bit halves:
+// r_o 2^32 = F_hi mo + rem1 and rem1 2^32 = F_lo mo + rem2, so F = F_hi 2^32 +
+// F_lo with rem1 = 2^(4e - s + 32) mod mo and rem2 = 2^(4e - s + 64) mod mo.
+// Each half is below 2^32 and exact, so F_hi = -rem1 mo^-1 and F_lo = -rem2
+// mo^-1 mod 2^32. Both remainders come from one square-and-double ladder on
+// E = 4e - s + 32 run in Montgomery form (R = 2^32): it ends holding
+// 2^E R = 2^(4e - s + 64) mod mo, which is rem2 up to one subtraction, and its
+// Montgomery reduction is 2^E mod mo = rem1.
 //
 // The ladder is modpow16_mont's, for callers whose first active lane holds
 // the wavefront's largest exponent (lanes taking consecutive terms k of one
@@ -263,7 +266,7 @@ __device__ __forceinline__ uint64_t frac64(uint32_t r, uint32_t m, double minv) 
 __device__ __forceinline__ uint64_t frac64_mont_wave(uint32_t e, uint32_t m) {
     const int s = __builtin_ctz(m);
     const uint32_t mo = m >> s;
-    const uint32_t E = 4u * e - (uint32_t)s + 64u;
+    const uint32_t E = 4u * e - (uint32_t)s + 32u;
     uint32_t inv = (3u * mo) ^ 2u;
     for (int i = 0; i < 3; ++i) inv *= 2u - mo * inv;
     const uint32_t ninv = 0u - inv;
@@ -284,11 +287,11 @@ __device__ __forceinline__ uint64_t frac64_mont_wave(uint32_t e, uint32_t m) {
         const uint32_t d = r - two_mo;
         r = d < r ? d : r;
     }
+    const uint32_t rem2 = r >= mo ? r - mo : r;
     const uint32_t mq = r * ninv;
-    uint32_t rem = (uint32_t)(((uint64_t)mq * mo + r) >> 32);
-    if (rem >= mo) rem -= mo;
-    const uint64_t inv64 = (uint64_t)inv * (2u - (uint64_t)mo * inv);
-    return (0ull - (uint64_t)rem) * inv64;
+    uint32_t rem1 = (uint32_t)(((uint64_t)mq * mo + r) >> 32);
+    if (rem1 >= mo) rem1 -= mo;
+    return ((uint64_t)(rem1 * ninv) << 32) | (uint64_t)(rem2 * ninv);
 }
 
 // Virtual block v: series v / bps, terms [kb * 256 * KT, (kb + 1) * 256 * KT).

@@ -3,7 +3,9 @@ modpow16_mont), restated step for step on Python integers with the same
 32-bit wrap-arounds, against pow(16, e, m); the asserts pin the lazy bounds
 the kernel relies on (no final correction after a Montgomery square, one
 unsigned min after a double). Moduli are the reference's 8k + j,
-j in {1, 4, 5, 6} (include/bbp_kernel.h:99-108), below 2^30."""
+j in {1, 4, 5, 6} (include/bbp_kernel.h:99-108), below 2^30. The same for
+the fixed-point path's fraction (frac64_mont_wave), against
+floor((16^e mod m) 2^64 / m) on Python integers."""
 from __future__ import annotations
 
 import random
@@ -74,3 +76,81 @@ def test_edges():
         if m < 1 << 30:
             for e in (1, 2, (1 << 28) - 1, 1 << 27):
                 assert modpow16_mont(e, m) == pow(16, e, m), (e, m)
+
+
+# ---- frac64_mont_wave: the fixed-point fraction floor((16^e mod m) 2^64 / m) ----
+
+def frac64_mont_wave(e: int, m: int, q_off: int = 0, emax: int | None = None) -> int:
+    """Step for step as the kernel: inverse from (3 mo) xor 2, the Montgomery
+    one from a quotient q within one of floor(2^32 / mo) (q_off models the
+    single-precision reciprocal's error), the ladder on E = 4e - s + 32 run
+    for the wavefront's longest bit count emax, and the two halves
+    F_hi = rem1 (-mo^-1), F_lo = rem2 (-mo^-1) mod 2^32."""
+    assert e >= 1 and 1 < m < (1 << 30)
+    s = (m & -m).bit_length() - 1
+    mo = m >> s
+    E = 4 * e - s + 32
+    inv = ((3 * mo) ^ 2) & M32
+    for _ in range(3):
+        inv = (inv * ((2 - mo * inv) & M32)) & M32
+    assert (inv * mo) & M32 == 1
+    ninv = (-inv) & M32
+    q = ((1 << 32) // mo + q_off) & M32 if mo >= 2048 else ((1 << 32) // mo) & M32
+    r = (-(q * mo)) & M32
+    if r >= 1 << 31:                       # (int32_t) r < 0
+        r = (r + mo) & M32
+    assert r < 2 * mo and r % mo == (1 << 32) % mo
+    two_mo = 2 * mo
+    bits = (emax if emax is not None else E).bit_length()
+    assert E.bit_length() <= bits
+    for b in range(bits - 1, -1, -1):
+        t = r * r
+        mq = ((t & M32) * ninv) & M32
+        r = (mq * mo + t) >> 32
+        assert r < two_mo
+        r = (r << ((E >> b) & 1)) & M32
+        d = (r - two_mo) & M32
+        r = d if d < r else r
+        assert r < two_mo
+    rem2 = r - mo if r >= mo else r
+    mq = (r * ninv) & M32
+    rem1 = (mq * mo + r) >> 32
+    assert rem1 <= mo
+    if rem1 >= mo:
+        rem1 -= mo
+    return (((rem1 * ninv) & M32) << 32) | ((rem2 * ninv) & M32)
+
+
+def frac_exact(e: int, m: int) -> int:
+    return (pow(16, e, m) << 64) // m
+
+
+def test_frac64_small_moduli_every_exponent():
+    for m in range(2, 700):
+        if m % 8 not in (1, 4, 5, 6):
+            continue
+        for e in range(1, 60):
+            assert frac64_mont_wave(e, m) == frac_exact(e, m), (e, m)
+
+
+def test_frac64_random_with_quotient_error_and_wave_padding():
+    """Random terms up to m < 2^30, e < 2^28, with the Montgomery one's
+    quotient off by -1, 0 or +1 and the ladder padded to a longer exponent of
+    the same wavefront."""
+    rng = random.Random(11)
+    for _ in range(6000):
+        m = 8 * rng.randrange(256, 1 << 27) + rng.choice((1, 4, 5, 6))
+        if m >= 1 << 30:
+            continue
+        e = rng.randrange(1, 1 << 28)
+        q_off = rng.choice((-1, 0, 1))
+        emax = 4 * (e + rng.randrange(0, 256)) + 32
+        assert frac64_mont_wave(e, m, q_off, emax) == frac_exact(e, m), (e, m, q_off)
+
+
+def test_frac64_edges():
+    for m in (4, 12, 20, 2045 * 8 + 4, 2048 * 8 + 1, 8 * ((1 << 27) - 1) + 6, (1 << 30) - 3):
+        if m % 8 not in (1, 4, 5, 6) or m >= 1 << 30:
+            continue
+        for e in (1, 2, 3, 7, 8, 9, (1 << 28) - 1):
+            assert frac64_mont_wave(e, m) == frac_exact(e, m), (e, m)
