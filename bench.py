@@ -473,13 +473,32 @@ def main() -> int:
                                    "fallback_positions": pdh.bbp_exact_count() - fb0,
                                    "extrapolated_2p20_positions_s": M20 / (Ms / (sms * 1e-3)),
                                    "bit_exact": got == want_sorted}
+        # 512 positions of the same draw, sorted: groups of 64 close enough for
+        # positions in lanes (the layout 2^20 positions would take)
+        p512 = np.sort(W.sampled_positions(gs["seed"], 512, gs["lo"], gs["hi"]))
+        d512 = torch.from_numpy(p512.view(np.int64)).to(dev)
+        v512 = torch.zeros(len(p512), dtype=torch.uint8, device=dev)
+        e0.record(stream)
+        pdh.bbp_positions_dev(d512.data_ptr(), len(p512), v512.data_ptr(), sp)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms512 = e0.elapsed_time(e1)
+        M512 = W.bbp_mulmod_count(p512)
+        with open(os.path.join(ROOT, "tests", "golden", "bbp_sampled_4096.json")) as f:
+            g4096 = json.load(f)
+        ref512 = {int(p): h for p, h in zip(g4096["positions"], g4096["hex"])}
+        h512 = pdh.to_hex(v512.cpu().numpy())
+        extra["bbp_sampled_512"] = {"ms": ms512, "mulmod_per_s": M512 / (ms512 * 1e-3), "mulmods": M512,
+                                    "positions": "512 uniform in [1e6, 1e8], seed 401, sorted",
+                                    "extrapolated_2p20_positions_s": M20 / (M512 / (ms512 * 1e-3)),
+                                    "bit_exact": all(ref512.get(int(p)) == h for p, h in zip(p512, h512))}
         # the integer-multiply ceiling: the measured v_mad_u64_u32 issue rate,
         # at four multiply-class instructions per 32-bit-modulus mulmod
         rate = pdh.probe_mad_rate()
         ceiling = rate / 4
         extra["bbp_ceiling"] = {"mad_u64_u32_lane_ops_per_s": rate, "mulmod_per_s": ceiling,
                                 "basis": "measured v_mad_u64_u32 rate / 4 multiply-class ops per mulmod"}
-        for k in ("bbp_1_65536", "bbp_sampled_64"):
+        for k in ("bbp_1_65536", "bbp_sampled_64", "bbp_sampled_512"):
             extra[k]["ceiling_frac"] = extra[k]["mulmod_per_s"] / ceiling
 
         extra["c3_batch_4096"] = bench_batch(pdh, torch, W, stream, reps=3)

@@ -330,6 +330,111 @@ __global__ __launch_bounds__(256) void k_bbp_fx_terms(const BbpSeries* __restric
         atomicAdd((unsigned long long*)&acc[s], (unsigned long long)(part[0] + part[1] + part[2] + part[3]));
 }
 
+// The same fraction when the whole wavefront shares the exponent e (lanes
+// hold different positions, so different moduli) and s (one series j per
+// wavefront): E = 4e - s + 32 is then wave-uniform, its bits are scalar, and
+// a zero bit skips the doubling by a scalar branch -- three VALU for a zero
+// bit and six for a one, against seven for every bit of the per-lane ladder.
+// Otherwise step for step frac64_mont_wave.
+__device__ __forceinline__ uint64_t frac64_mont_uniform(uint32_t E, uint32_t m, uint32_t s) {
+    const uint32_t mo = m >> s;
+    uint32_t inv = (3u * mo) ^ 2u;
+    for (int i = 0; i < 3; ++i) inv *= 2u - mo * inv;
+    const uint32_t ninv = 0u - inv;
+    const uint32_t q = mo >= 2048u ? (uint32_t)(4294967296.0f * __builtin_amdgcn_rcpf((float)mo))
+                                   : (uint32_t)(0x100000000ull / mo);
+    uint32_t r = 0u - q * mo;
+    if ((int32_t)r < 0) r += mo;
+    const uint32_t two_mo = mo << 1;
+    for (int b = 31 - __clz(E); b >= 0; --b) {
+        const uint64_t t = (uint64_t)r * r;
+        const uint32_t mq = (uint32_t)t * ninv;
+        r = (uint32_t)(((uint64_t)mq * mo + t) >> 32);
+        if ((E >> b) & 1u) {
+            // an empty volatile statement keeps this a scalar branch: if-converted
+            // (both values computed, one selected) a bit would cost eight VALU
+            __asm__ volatile("" ::: "memory");
+            r <<= 1;
+            const uint32_t d = r - two_mo;
+            r = d < r ? d : r;
+        }
+    }
+    const uint32_t rem2 = r >= mo ? r - mo : r;
+    const uint32_t mq = r * ninv;
+    uint32_t rem1 = (uint32_t)(((uint64_t)mq * mo + r) >> 32);
+    if (rem1 >= mo) rem1 -= mo;
+    return ((uint64_t)(rem1 * ninv) << 32) | (uint64_t)(rem2 * ninv);
+}
+
+// Positions in lanes: block = 64 consecutive positions (in the order given)
+// x 4 series, one wavefront per series j; block v covers group v / chunks and
+// the exponents e of chunk v % chunks, [c EK, (c + 1) EK). For each e every
+// lane takes its own position's term k = d - e (none when e > d), so the
+// wavefront's exponent is uniform. Each lane sums its terms in a register and
+// adds them to its series' accumulator once. Suited to positions that lie
+// close together within a group (a lane idles for e above its d).
+__global__ __launch_bounds__(256) void k_bbp_fx_lanes(const BbpSeries* __restrict__ st, size_t count,
+                                                      uint32_t chunks, uint32_t EK, uint64_t* __restrict__ acc) {
+    const uint32_t g = blockIdx.x / chunks;
+    const uint32_t c = blockIdx.x - g * chunks;
+    const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t lane = threadIdx.x & 63;
+    const size_t pi = (size_t)g * 64 + lane;
+    const bool in = pi < count;
+    const BbpSeries S = in ? st[4 * pi + w] : BbpSeries{0, 0.0, 1, 0};
+    const bool act = in && S.valid;
+    // positions are at most 2^28, so d, e, k < 2^28 and m = 8k + j < 2^31
+    const uint32_t d = act ? (uint32_t)S.d : 0;
+    uint32_t dm = d;
+    for (int o = 32; o > 0; o >>= 1) dm = max(dm, (uint32_t)__shfl_xor((int)dm, o));
+    dm = __builtin_amdgcn_readfirstlane(dm);
+    const uint32_t e0 = c * EK;
+    if (e0 > dm) return;                                  // the same in all four wavefronts
+    const uint32_t e1 = min(e0 + EK - 1u, dm);
+    const uint32_t j = w == 0 ? 1u : w + 3u;              // 1, 4, 5, 6
+    const uint32_t sj = j == 4 ? 2u : (j == 6 ? 1u : 0u); // 2-adic valuation of 8k + j
+    uint64_t local = 0;
+    for (uint32_t e = e0; e <= e1; ++e) {
+        if (!act || e > d) continue;
+        const uint32_t k = d - e;
+        const uint32_t mi = 8u * k + j;
+        if (mi == 1) continue;                            // 16^d mod 1 = 0
+        if (mi < (1u << 30) && e > 0) {
+            local += frac64_mont_uniform(4u * e - sj + 32u, mi, sj);
+        } else {                                          // 16^0 = 1, or m >= 2^30
+            const double m = (double)mi;
+            const double minv = recip_f64(m);
+            const uint32_t r = e == 0 ? 1u : (uint32_t)modpow16_exact(e, m, minv);
+            local += frac64(r, mi, minv);
+        }
+    }
+    if (act) atomicAdd((unsigned long long*)&acc[4 * pi + w], (unsigned long long)local);
+}
+
+// Per group of 64 consecutive entries of pos: out[0] = max(d) (atomic max),
+// out[1] += sum of d + 1 over valid positions, out[2] += 64 (max d of the
+// group + 1): out[1] / out[2] is the share of lanes k_bbp_fx_lanes keeps busy.
+__global__ void k_bbp_group_scan(const uint64_t* __restrict__ pos, size_t count,
+                                 unsigned long long* __restrict__ out) {
+    const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g * 64 >= count) return;
+    unsigned long long mx = 0, sum = 0;
+    bool any = false;
+    const size_t end = min(count, g * 64 + 64);
+    for (size_t i = g * 64; i < end; ++i) {
+        const uint64_t p = pos[i];
+        if (p >= 1 && p <= ((uint64_t)1 << 28)) {
+            any = true;
+            sum += p;
+            if (p - 1 > mx) mx = p - 1;
+        }
+    }
+    if (!any) return;
+    atomicMax(&out[0], mx);
+    atomicAdd(&out[1], sum);
+    atomicAdd(&out[2], 64ull * (mx + 1));
+}
+
 // T = 4 F1 - 2 F4 - F5 - F6 (mod 2^64) of one position: its four head sums
 // plus the 14 tail terms floor(2^(64-4i) / m).
 __device__ __forceinline__ uint64_t bbp_fx_combine(const BbpSeries* P, const uint64_t* a, uint64_t d) {
@@ -466,15 +571,6 @@ __global__ __launch_bounds__(256) void k_bbp_finish(const BbpSeries* __restrict_
     }
 }
 
-__global__ void k_bbp_dmax(const uint64_t* __restrict__ pos, size_t count, unsigned long long* __restrict__ dmax) {
-    unsigned long long m = 0;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x) {
-        const uint64_t p = pos[i];
-        if (p >= 1 && p <= ((uint64_t)1 << 28) && p - 1 > m) m = p - 1;
-    }
-    atomicMax(dmax, m);
-}
-
 static int chunk_log(size_t S) {
     int log_ch = 15;
     while (log_ch > 8 && (S << log_ch) > ((size_t)1 << 24)) --log_ch;
@@ -495,7 +591,7 @@ size_t bbp_few_ws_bytes(size_t count) {
 
 // dmax: the largest d = position - 1 among valid positions (known to the host).
 hipError_t launch_bbp_fx(const uint64_t* d_pos, uint64_t start, size_t count, uint64_t dmax, uint8_t* d_out,
-                         void* ws, int odd_undecided, hipStream_t s, uint32_t block, size_t nout) {
+                         void* ws, int odd_undecided, hipStream_t s, uint32_t block, size_t nout, bool lanes) {
     const size_t S = 4 * count;
     BbpSeries* st = (BbpSeries*)ws;
     uint64_t* acc = (uint64_t*)((char*)ws + ((S * sizeof(BbpSeries) + 255) & ~(size_t)255));
@@ -509,8 +605,21 @@ hipError_t launch_bbp_fx(const uint64_t* d_pos, uint64_t start, size_t count, ui
     int KT = 32;
     uint64_t bps = (dmax + 1 + 256ull * KT - 1) / (256ull * KT);
     while (bps > 1 && bps * S > ((uint64_t)1 << 17)) { KT *= 2; bps = (dmax + 1 + 256ull * KT - 1) / (256ull * KT); }
-    BBP_LAUNCH("k_bbp_fx_terms", s, k_bbp_fx_terms, dim3((unsigned)(bps * S)), dim3(256), 0, s,
-               (const BbpSeries*)st, (unsigned)bps, KT, acc);
+    if (lanes) {
+        // about 2^20 blocks, but at least 64 exponents per block (measured on
+        // 1..65,536: 2^14 blocks of 4,096 exponents 49 ms, 2^18 of 256 29 ms,
+        // 2^20 of 64 28 ms; every 7th position to 65,536: 10 exponents per
+        // block 8.5 ms, 37-147 4.2 ms)
+        const uint64_t groups = (count + 63) / 64;
+        const uint64_t want = std::max<uint64_t>(((uint64_t)1 << 20) / groups, 1);
+        const uint64_t EK = std::max<uint64_t>((dmax + want) / want, 64);
+        const uint64_t chunks = (dmax + EK) / EK;
+        BBP_LAUNCH("k_bbp_fx_lanes", s, k_bbp_fx_lanes, dim3((unsigned)(groups * chunks)), dim3(256), 0, s,
+                   (const BbpSeries*)st, count, (uint32_t)chunks, (uint32_t)EK, acc);
+    } else {
+        BBP_LAUNCH("k_bbp_fx_terms", s, k_bbp_fx_terms, dim3((unsigned)(bps * S)), dim3(256), 0, s,
+                   (const BbpSeries*)st, (unsigned)bps, KT, acc);
+    }
     if (d_pos || block <= 1)
         BBP_LAUNCH("k_bbp_fx_finish", s, k_bbp_fx_finish, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s,
                    (const BbpSeries*)st, count, (const uint64_t*)acc, d_out, odd_undecided);
@@ -541,10 +650,13 @@ hipError_t launch_bbp_chunked(const uint64_t* d_pos, uint64_t start, size_t coun
     return hipSuccess;
 }
 
-hipError_t launch_bbp_dmax(const uint64_t* d_pos, size_t count, uint64_t* d_dmax, hipStream_t s) {
-    BBP_LAUNCH("k_bbp_dmax", s, k_bbp_dmax, dim3(64), dim3(256), 0, s, d_pos, count, (unsigned long long*)d_dmax);
+hipError_t launch_bbp_group_scan(const uint64_t* d_pos, size_t count, uint64_t* d_out3, hipStream_t s) {
+    const size_t groups = (count + 63) / 64;
+    BBP_LAUNCH("k_bbp_group_scan", s, k_bbp_group_scan, dim3((unsigned)((groups + 255) / 256)), dim3(256), 0, s,
+               d_pos, count, (unsigned long long*)d_out3);
     return hipSuccess;
 }
+
 
 // ---- integer multiply issue-rate probe ------------------------------------------------
 // The ceiling the BBP throughput is compared with: eight independent chains of

@@ -55,7 +55,8 @@ struct Ctx {
     u64* st_len = nullptr;
     uint64_t* st_pos = nullptr; uint8_t* st_val = nullptr; size_t st_bbp = 0;
     void* bbp_ws = nullptr; size_t bbp_ws_bytes = 0;
-    uint64_t* bbp_dmax = nullptr;
+    uint64_t* bbp_scan = nullptr;    // k_bbp_group_scan's three words
+    int bbp_layout = 0;              // pdh_bbp_set_layout mode
     uint64_t* fix_pos = nullptr; size_t fix_pos_n = 0;
     uint8_t* fix_val = nullptr; size_t fix_val_n = 0;
     uint64_t* und_pos = nullptr; size_t und_pos_n = 0;     // undecided positions of a digit-block range
@@ -439,7 +440,34 @@ uint32_t bbp_block_digits(uint64_t dmax) {
 }
 
 int bbp_few_locked(Ctx& c, const uint64_t* d_pos, const uint64_t* h_pos, uint64_t start, size_t count,
-                   uint64_t dmax, uint8_t* d_out, uint8_t* h_vals, hipStream_t st, bool blocks = false);
+                   uint64_t dmax, uint8_t* d_out, uint8_t* h_vals, hipStream_t st, bool blocks = false,
+                   double share = -1.0);
+
+// Share of lanes the positions-in-lanes term kernel keeps busy: positions
+// h_pos[i] (or start + i stride) in groups of 64, sum of positions over 64 x
+// each group's largest (a lane idles for exponents above its own d).
+double lane_share(const uint64_t* h_pos, uint64_t start, uint64_t stride, size_t count) {
+    double busy = 0, total = 0;
+    for (size_t g = 0; g < count; g += 64) {
+        uint64_t mx = 0;
+        const size_t end = std::min(count, g + 64);
+        for (size_t i = g; i < end; ++i) {
+            const uint64_t p = h_pos ? h_pos[i] : start + i * stride;
+            busy += (double)p;
+            mx = std::max(mx, p);
+        }
+        total += 64.0 * (double)mx;
+    }
+    return total > 0 ? busy / total : 0.0;
+}
+
+// Positions in lanes costs about 0.65-0.8 of terms in lanes per busy lane
+// (measured: 1..65,536 28 against 36 ms, 512 sorted positions in
+// [10^6, 10^8], share 0.89, 0.52 against 0.80 s), so it pays from a share
+// of about 0.7 up (64 such positions have 0.52).
+bool bbp_use_lanes(const Ctx& c, double share) {
+    return c.bbp_layout == 2 || (c.bbp_layout == 0 && share >= 0.7);
+}
 
 // A contiguous range by digit blocks: one fixed-point evaluation per G
 // positions (k_bbp_fx_block), then the per-position paths for the digits
@@ -450,7 +478,7 @@ int bbp_blocks_locked(Ctx& c, uint64_t start, size_t count, uint32_t G, uint8_t*
     int rc = grow((char**)&c.bbp_ws, &c.bbp_ws_bytes, bbp_few_ws_bytes(A), 1);
     if (rc) return rc;
     hipError_t e = launch_bbp_fx(nullptr, start, A, start - 1 + (A - 1) * (uint64_t)G, d_out, c.bbp_ws,
-                                 c.bbp_exact == 2, st, G, count);
+                                 c.bbp_exact == 2, st, G, count, bbp_use_lanes(c, lane_share(nullptr, start, G, A)));
     if (e != hipSuccess) return fail_hip(e, "bbp launch");
     HIPCALL(hipMemcpyAsync(h_vals, d_out, count, hipMemcpyDeviceToHost, st));
     HIPCALL(hipStreamSynchronize(st));
@@ -482,7 +510,7 @@ int bbp_blocks_locked(Ctx& c, uint64_t start, size_t count, uint32_t G, uint8_t*
 // positions on the host. The values end in d_out and in h_vals. Synchronises
 // `st`. A contiguous range with `blocks` set goes by digit blocks.
 int bbp_few_locked(Ctx& c, const uint64_t* d_pos, const uint64_t* h_pos, uint64_t start, size_t count,
-                   uint64_t dmax, uint8_t* d_out, uint8_t* h_vals, hipStream_t st, bool blocks) {
+                   uint64_t dmax, uint8_t* d_out, uint8_t* h_vals, hipStream_t st, bool blocks, double share) {
     if (blocks && !d_pos && c.bbp_exact != 1 && count >= 16) {
         const uint32_t G = bbp_block_digits(dmax);
         if (G > 1) return bbp_blocks_locked(c, start, count, G, d_out, h_vals, st);
@@ -497,7 +525,9 @@ int bbp_few_locked(Ctx& c, const uint64_t* d_pos, const uint64_t* h_pos, uint64_
         c.bbp_exact_count += count;
         return PDH_OK;
     }
-    hipError_t e = launch_bbp_fx(d_pos, start, count, dmax, d_out, c.bbp_ws, c.bbp_exact == 2, st);
+    if (share < 0) share = d_pos && !h_pos ? 0.0 : lane_share(h_pos, start, 1, count);
+    hipError_t e = launch_bbp_fx(d_pos, start, count, dmax, d_out, c.bbp_ws, c.bbp_exact == 2, st, 1, 0,
+                                 bbp_use_lanes(c, share));
     if (e != hipSuccess) return fail_hip(e, "bbp launch");
     HIPCALL(hipMemcpyAsync(h_vals, d_out, count, hipMemcpyDeviceToHost, st));
     HIPCALL(hipStreamSynchronize(st));
@@ -708,15 +738,17 @@ int pdh_bbp_positions_dev(const uint64_t* d_positions, size_t n, uint8_t* d_valu
     hipError_t e;
     if (c.bbp_exact != 3) {
         // both few-position paths size their grids by the largest position
-        if (!c.bbp_dmax) HIPCALL(hipMalloc(&c.bbp_dmax, 8));
-        e = launch_zero_len(c.bbp_dmax, st);
-        if (e == hipSuccess) e = launch_bbp_dmax(d_positions, n, c.bbp_dmax, st);
-        uint64_t dmax = 0;
-        if (e == hipSuccess) e = hipMemcpyAsync(&dmax, c.bbp_dmax, 8, hipMemcpyDeviceToHost, st);
+        // and choose the term layout by how close the positions of each group of 64 lie
+        if (!c.bbp_scan) HIPCALL(hipMalloc(&c.bbp_scan, 24));
+        e = hipMemsetAsync(c.bbp_scan, 0, 24, st);
+        if (e == hipSuccess) e = launch_bbp_group_scan(d_positions, n, c.bbp_scan, st);
+        uint64_t sc[3] = {0, 0, 0};
+        if (e == hipSuccess) e = hipMemcpyAsync(sc, c.bbp_scan, 24, hipMemcpyDeviceToHost, st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);
         if (e != hipSuccess) return fail_hip(e, "bbp position scan");
         std::vector<uint8_t> tmp(n);
-        return bbp_few_locked(c, d_positions, nullptr, 0, n, dmax, d_values, tmp.data(), st);
+        return bbp_few_locked(c, d_positions, nullptr, 0, n, sc[0], d_values, tmp.data(), st, false,
+                              sc[2] ? (double)sc[1] / (double)sc[2] : 0.0);
     }
     e = launch_bbp(d_positions, 0, n, d_values, c.bbp_next, st);
     if (e != hipSuccess) return fail_hip(e, "bbp launch");
@@ -743,6 +775,14 @@ int pdh_bbp_set_exact(int mode) {
     Ctx& c = ctx();
     std::lock_guard<std::mutex> g(c.mu);
     c.bbp_exact = mode;
+    return PDH_OK;
+}
+
+int pdh_bbp_set_layout(int mode) {
+    if (mode < 0 || mode > 2) return fail(PDH_E_ARG, "pdh_bbp_set_layout: mode must be 0, 1 or 2");
+    Ctx& c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    c.bbp_layout = mode;
     return PDH_OK;
 }
 

@@ -126,16 +126,19 @@ size_t carry_ws_bytes(size_t nlimbs);
 // undecided). With block > 1 and no d_pos, launch_bbp_fx evaluates the
 // `count` anchors start + block t and writes the digits of the nout
 // positions start .. start + nout - 1 (block of them per anchor, 0xFE where
-// undecided). launch_bbp_dmax: atomic max of position - 1 into *d_dmax
-// (zeroed by the caller).
+// undecided); with `lanes` its term kernel holds positions in lanes
+// (k_bbp_fx_lanes) instead of terms. launch_bbp_group_scan: over groups of 64
+// entries of d_pos, d_out3[0] = max(position - 1), d_out3[1] = sum of
+// positions, d_out3[2] = sum of 64 x the group's largest position (the
+// caller zeroes the three words).
 hipError_t launch_bbp(const uint64_t* d_pos, uint64_t start, size_t count, uint8_t* d_out, uint64_t* d_next,
                       hipStream_t s);
 hipError_t launch_bbp_fx(const uint64_t* d_pos, uint64_t start, size_t count, uint64_t dmax,
                          uint8_t* d_out, void* ws, int odd_undecided, hipStream_t s, uint32_t block = 1,
-                         size_t nout = 0);
+                         size_t nout = 0, bool lanes = false);
 hipError_t launch_bbp_chunked(const uint64_t* d_pos, uint64_t start, size_t count, uint64_t dmax,
                               uint8_t* d_out, void* ws, hipStream_t s);
-hipError_t launch_bbp_dmax(const uint64_t* d_pos, size_t count, uint64_t* d_dmax, hipStream_t s);
+hipError_t launch_bbp_group_scan(const uint64_t* d_pos, size_t count, uint64_t* d_out3, hipStream_t s);
 // Measured issue rate of v_mad_u64_u32 (lane operations per second, whole GPU).
 hipError_t probe_mad_rate(double* lane_ops_per_s);
 size_t bbp_few_ws_bytes(size_t count);

@@ -38,7 +38,7 @@ EXPORTED = (
     "pdh_batch_create", "pdh_batch_run", "pdh_batch_capture", "pdh_batch_destroy", "pdh_mul_batch_dev",
     "pdh_tune_get", "pdh_tune_set", "pdh_autotune", "pdh_dev_alloc", "pdh_dev_free", "pdh_copy_htod",
     "pdh_copy_dtoh", "pdh_stream_create", "pdh_stream_destroy", "pdh_stream_sync",
-    "pdh_bbp_set_exact", "pdh_bbp_exact_count", "pdh_tune_engine", "pdh_bs_groups", "pdh_bs_merge_dev",
+    "pdh_bbp_set_exact", "pdh_bbp_set_layout", "pdh_bbp_exact_count", "pdh_tune_engine", "pdh_bs_groups", "pdh_bs_merge_dev",
     "pdh_sadd_batch_dev", "pdh_shr_batch_dev", "pdh_decimal_leaves_dev", "pdh_sadd_dev", "pdh_shift_dev",
 )
 
@@ -127,6 +127,8 @@ def lib() -> ctypes.CDLL:
             getattr(L, name).restype = ctypes.c_int
         L.pdh_bbp_set_exact.argtypes = [ctypes.c_int]
         L.pdh_bbp_set_exact.restype = ctypes.c_int
+        L.pdh_bbp_set_layout.argtypes = [ctypes.c_int]
+        L.pdh_bbp_set_layout.restype = ctypes.c_int
         L.pdh_bbp_exact_count.restype = ctypes.c_uint64
         L.pdh_timing_enable.argtypes = [ctypes.c_int]
         L.pdh_timing_enable.restype = ctypes.c_int
@@ -347,6 +349,12 @@ def bbp_set_exact(mode: int) -> None:
     only, 2 fixed point with odd indices sent to the fallback too (testing),
     3 one lane per series in FP64 order (work-queue kernel)."""
     _check(lib().pdh_bbp_set_exact(int(mode)))
+
+
+def bbp_set_layout(mode: int) -> None:
+    """Term kernel of the fixed-point path: 0 chosen per call (default),
+    1 terms in lanes, 2 positions in lanes (testing; same digits)."""
+    _check(lib().pdh_bbp_set_layout(int(mode)))
 
 
 def probe_mad_rate() -> float:

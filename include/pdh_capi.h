@@ -275,6 +275,13 @@ int pdh_decimal_leaves_dev(uint32_t* d_x, uint64_t stride, const uint64_t* d_len
  * process. */
 int pdh_bbp_set_exact(int mode);
 uint64_t pdh_bbp_exact_count(void);
+/* pdh_bbp_set_layout(mode): the fixed-point path's term kernel. 0 (default)
+ * chooses per call: positions in lanes (one exponent per wavefront, its bits
+ * scalar) when at least three lanes in four stay busy -- the positions of
+ * each group of 64, in the order given, lie close together -- else terms in
+ * lanes; 1 forces terms in lanes, 2 positions in lanes. The digits are the
+ * same either way. */
+int pdh_bbp_set_layout(int mode);
 /* The device's measured v_mad_u64_u32 issue rate (lane operations per
  * second, whole GPU), the integer-multiply ceiling BBP throughput is quoted
  * against. Synchronous; allocates and frees its own buffer. */

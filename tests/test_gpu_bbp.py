@@ -199,3 +199,54 @@ def test_fallback_plumbing(pdh, golden, exact_mode):
     assert np.array_equal(dval.cpu().numpy(), want)
     with pytest.raises(pdh.PdhError):
         pdh.bbp_set_exact(4)
+
+
+@pytest.fixture
+def layout(pdh):
+    yield pdh.bbp_set_layout
+    pdh.bbp_set_layout(0)
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_both_term_layouts_give_the_reference_digits(pdh, golden, layout, mode):
+    """Terms in lanes (1) and positions in lanes (2, one exponent per
+    wavefront): the contiguous range 1..65,536 by positions, the 64 sampled
+    golden positions (spread over [10^6, 10^8], so lanes idle in layout 2)
+    and digit blocks at 10^6."""
+    import torch
+    layout(mode)
+    g = golden["bbp_range"]
+    want = open(os.path.join(GOLDEN_DIR, g["file"])).read().strip()
+    pos = torch.arange(1, g["count"] + 1, dtype=torch.int64, device="cuda")
+    dv = torch.zeros(g["count"], dtype=torch.uint8, device="cuda")
+    pdh.bbp_positions_dev(pos.data_ptr(), g["count"], dv.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    got = pdh.to_hex(dv.cpu().numpy())
+    bad = [i for i in range(len(want)) if got[i] != want[i]]
+    assert not bad, f"{len(bad)} mismatches, first at position {bad[0] + 1}"
+    s = golden["bbp_sampled"]
+    assert pdh.to_hex(pdh.bbp_values_positions(np.array(s["positions"], dtype=np.uint64))) == s["hex"]
+    assert pdh.bbp_hex_digits_hip(1_000_000, 10) == golden["bailey_1e6"]
+    assert pdh.bbp_hex_digits_hip(1_000_000, 40)[:10] == golden["bailey_1e6"]
+
+
+def test_layout_choice(pdh, golden, layout):
+    """The default picks positions in lanes for close positions (a contiguous
+    range) and terms in lanes for spread ones (the 64 sampled positions)."""
+    import torch
+    layout(0)
+    s = golden["bbp_sampled"]
+    for positions, kernel in ((np.arange(1, 4097, dtype=np.uint64), "k_bbp_fx_lanes"),
+                              (np.array(s["positions"], dtype=np.uint64), "k_bbp_fx_terms")):
+        dpos = torch.from_numpy(positions.view(np.int64)).cuda()
+        dv = torch.zeros(len(positions), dtype=torch.uint8, device="cuda")
+        torch.cuda.synchronize()
+        pdh.timing_enable(True)
+        pdh.bbp_positions_dev(dpos.data_ptr(), len(positions), dv.data_ptr(),
+                              torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+        names = pdh.timing_read()
+        pdh.timing_enable(False)
+        assert kernel in names, names
+    with pytest.raises(pdh.PdhError):
+        pdh.bbp_set_layout(3)
