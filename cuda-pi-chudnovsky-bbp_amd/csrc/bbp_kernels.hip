@@ -336,6 +336,22 @@ __global__ __launch_bounds__(256) void k_bbp_fx_terms(const BbpSeries* __restric
 // a zero bit skips the doubling by a scalar branch -- three VALU for a zero
 // bit and six for a one, against seven for every bit of the per-lane ladder.
 // Otherwise step for step frac64_mont_wave.
+__device__ __forceinline__ uint32_t mont_sq_lazy(uint32_t r, uint32_t mo, uint32_t ninv) {
+    const uint64_t t = (uint64_t)r * r;
+    const uint32_t mq = (uint32_t)t * ninv;
+    return (uint32_t)(((uint64_t)mq * mo + t) >> 32);
+}
+
+__device__ __forceinline__ uint32_t dbl_lazy(uint32_t r, uint32_t two_mo) {
+    // an empty volatile statement keeps the caller's test a scalar branch:
+    // if-converted (both values computed, one selected) a bit would cost
+    // eight VALU
+    __asm__ volatile("" ::: "memory");
+    r <<= 1;
+    const uint32_t d = r - two_mo;
+    return d < r ? d : r;
+}
+
 __device__ __forceinline__ uint64_t frac64_mont_uniform(uint32_t E, uint32_t m, uint32_t s) {
     const uint32_t mo = m >> s;
     uint32_t inv = (3u * mo) ^ 2u;
@@ -346,18 +362,25 @@ __device__ __forceinline__ uint64_t frac64_mont_uniform(uint32_t E, uint32_t m, 
     uint32_t r = 0u - q * mo;
     if ((int32_t)r < 0) r += mo;
     const uint32_t two_mo = mo << 1;
-    for (int b = 31 - __clz(E); b >= 0; --b) {
-        const uint64_t t = (uint64_t)r * r;
-        const uint32_t mq = (uint32_t)t * ninv;
-        r = (uint32_t)(((uint64_t)mq * mo + t) >> 32);
-        if ((E >> b) & 1u) {
-            // an empty volatile statement keeps this a scalar branch: if-converted
-            // (both values computed, one selected) a bit would cost eight VALU
-            __asm__ volatile("" ::: "memory");
-            r <<= 1;
-            const uint32_t d = r - two_mo;
-            r = d < r ? d : r;
-        }
+    // E >= 34. Its leading one squares the Montgomery one (no change) and
+    // doubles it; the n bits below it, left-aligned in x, go four per trip
+    // (fewer scalar instructions per bit), the last n mod 4 one per trip.
+    int n = 31 - __clz(E);
+    uint32_t x = E << (32 - n);
+    r = dbl_lazy(r, two_mo);
+    for (; n >= 4; n -= 4, x <<= 4) {
+        r = mont_sq_lazy(r, mo, ninv);
+        if ((int32_t)x < 0) r = dbl_lazy(r, two_mo);
+        r = mont_sq_lazy(r, mo, ninv);
+        if ((int32_t)(x << 1) < 0) r = dbl_lazy(r, two_mo);
+        r = mont_sq_lazy(r, mo, ninv);
+        if ((int32_t)(x << 2) < 0) r = dbl_lazy(r, two_mo);
+        r = mont_sq_lazy(r, mo, ninv);
+        if ((int32_t)(x << 3) < 0) r = dbl_lazy(r, two_mo);
+    }
+    for (; n > 0; --n, x <<= 1) {
+        r = mont_sq_lazy(r, mo, ninv);
+        if ((int32_t)x < 0) r = dbl_lazy(r, two_mo);
     }
     const uint32_t rem2 = r >= mo ? r - mo : r;
     const uint32_t mq = r * ninv;

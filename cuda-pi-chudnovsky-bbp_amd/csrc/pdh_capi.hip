@@ -461,12 +461,12 @@ double lane_share(const uint64_t* h_pos, uint64_t start, uint64_t stride, size_t
     return total > 0 ? busy / total : 0.0;
 }
 
-// Positions in lanes costs about 0.65-0.8 of terms in lanes per busy lane
-// (measured: 1..65,536 28 against 36 ms, 512 sorted positions in
-// [10^6, 10^8], share 0.89, 0.52 against 0.80 s), so it pays from a share
-// of about 0.7 up (64 such positions have 0.52).
+// Positions in lanes costs about 0.5-0.75 of terms in lanes per busy lane
+// (measured: 1..65,536, share 0.99, 27 against 36 ms; 512 sorted positions
+// in [10^6, 10^8], share 0.89, 0.47 against 0.80 s), so it pays from a
+// share of about 0.6 up (64 such positions have 0.52).
 bool bbp_use_lanes(const Ctx& c, double share) {
-    return c.bbp_layout == 2 || (c.bbp_layout == 0 && share >= 0.7);
+    return c.bbp_layout == 2 || (c.bbp_layout == 0 && share >= 0.6);
 }
 
 // A contiguous range by digit blocks: one fixed-point evaluation per G

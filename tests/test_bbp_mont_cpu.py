@@ -154,3 +154,72 @@ def test_frac64_edges():
             continue
         for e in (1, 2, 3, 7, 8, 9, (1 << 28) - 1):
             assert frac64_mont_wave(e, m) == frac_exact(e, m), (e, m)
+
+
+def frac64_mont_uniform(e: int, m: int, q_off: int = 0) -> int:
+    """The positions-in-lanes form: the same set-up, the leading exponent bit
+    taken as one doubling of the Montgomery one, then the remaining bits
+    left-aligned in a 32-bit word, four per trip and the rest one by one."""
+    s = (m & -m).bit_length() - 1
+    mo = m >> s
+    E = 4 * e - s + 32
+    inv = ((3 * mo) ^ 2) & M32
+    for _ in range(3):
+        inv = (inv * ((2 - mo * inv) & M32)) & M32
+    ninv = (-inv) & M32
+    q = ((1 << 32) // mo + q_off) & M32 if mo >= 2048 else ((1 << 32) // mo) & M32
+    r = (-(q * mo)) & M32
+    if r >= 1 << 31:
+        r = (r + mo) & M32
+    two_mo = 2 * mo
+
+    def sq(r):
+        t = r * r
+        mq = ((t & M32) * ninv) & M32
+        r = (mq * mo + t) >> 32
+        assert r < two_mo
+        return r
+
+    def dbl(r):
+        r = (r << 1) & M32
+        d = (r - two_mo) & M32
+        r = d if d < r else r
+        assert r < two_mo
+        return r
+
+    n = E.bit_length() - 1
+    assert 5 <= n <= 31
+    x = (E << (32 - n)) & M32
+    r = dbl(r)
+    while n >= 4:
+        for i in range(4):
+            r = sq(r)
+            if (x >> (31 - i)) & 1:
+                r = dbl(r)
+        n -= 4
+        x = (x << 4) & M32
+    while n > 0:
+        r = sq(r)
+        if x >> 31:
+            r = dbl(r)
+        n -= 1
+        x = (x << 1) & M32
+    rem2 = r - mo if r >= mo else r
+    mq = (r * ninv) & M32
+    rem1 = (mq * mo + r) >> 32
+    if rem1 >= mo:
+        rem1 -= mo
+    return (((rem1 * ninv) & M32) << 32) | ((rem2 * ninv) & M32)
+
+
+def test_frac64_uniform_form():
+    rng = random.Random(12)
+    for m in range(2, 400):
+        if m % 8 in (1, 4, 5, 6):
+            for e in range(1, 40):
+                assert frac64_mont_uniform(e, m) == frac_exact(e, m), (e, m)
+    for _ in range(4000):
+        m = 8 * rng.randrange(256, 1 << 27) + rng.choice((1, 4, 5, 6))
+        if m < 1 << 30:
+            e = rng.randrange(1, 1 << 28)
+            assert frac64_mont_uniform(e, m, rng.choice((-1, 0, 1))) == frac_exact(e, m), (e, m)
