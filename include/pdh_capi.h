@@ -168,7 +168,9 @@ int pdh_bbp_positions(const uint64_t* positions, size_t n, uint8_t* values);
  * 64-bit fixed-point fractions, which decide the digit whenever it lies
  * provably far from a digit boundary; the rare rest (about 4e-6 per position
  * at 10^8) is recomputed with the reference's FP64 fold order. The call
- * synchronises `stream` once (to find those positions). Positions must lie in
+ * synchronises `stream` twice: after a scan of the positions (the largest,
+ * and how close each group of 64 lies, which picks the term kernel) and to
+ * find the undecided positions. Positions must lie in
  * [1, 2^28] (the kernel's exact FP64 modular arithmetic needs 8k + j <
  * 2^31); the host entry points reject others with PDH_E_ARG, and this one
  * writes 0xFF for them. */
