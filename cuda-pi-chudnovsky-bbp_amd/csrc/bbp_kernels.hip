@@ -352,6 +352,11 @@ __device__ __forceinline__ uint32_t dbl_lazy(uint32_t r, uint32_t two_mo) {
     return d < r ? d : r;
 }
 
+// SMALL: every modulus of the wavefront has mo < 2^28. A Montgomery square
+// of r < 4 mo is then below (16 mo^2 + 2^32 mo) / 2^32 < 2 mo, so a doubling
+// needs no reduction (the ladder runs on [0, 4 mo)): one shift by the scalar
+// bit, no branch. One reduction to [0, 2 mo) at the end.
+template <bool SMALL>
 __device__ __forceinline__ uint64_t frac64_mont_uniform(uint32_t E, uint32_t m, uint32_t s) {
     const uint32_t mo = m >> s;
     uint32_t inv = (3u * mo) ^ 2u;
@@ -367,20 +372,33 @@ __device__ __forceinline__ uint64_t frac64_mont_uniform(uint32_t E, uint32_t m, 
     // (fewer scalar instructions per bit), the last n mod 4 one per trip.
     int n = 31 - __clz(E);
     uint32_t x = E << (32 - n);
-    r = dbl_lazy(r, two_mo);
-    for (; n >= 4; n -= 4, x <<= 4) {
-        r = mont_sq_lazy(r, mo, ninv);
-        if ((int32_t)x < 0) r = dbl_lazy(r, two_mo);
-        r = mont_sq_lazy(r, mo, ninv);
-        if ((int32_t)(x << 1) < 0) r = dbl_lazy(r, two_mo);
-        r = mont_sq_lazy(r, mo, ninv);
-        if ((int32_t)(x << 2) < 0) r = dbl_lazy(r, two_mo);
-        r = mont_sq_lazy(r, mo, ninv);
-        if ((int32_t)(x << 3) < 0) r = dbl_lazy(r, two_mo);
-    }
-    for (; n > 0; --n, x <<= 1) {
-        r = mont_sq_lazy(r, mo, ninv);
-        if ((int32_t)x < 0) r = dbl_lazy(r, two_mo);
+    if constexpr (SMALL) {
+        r <<= 1;
+        for (; n >= 4; n -= 4, x <<= 4) {
+            r = mont_sq_lazy(r, mo, ninv) << (x >> 31);
+            r = mont_sq_lazy(r, mo, ninv) << ((x >> 30) & 1u);
+            r = mont_sq_lazy(r, mo, ninv) << ((x >> 29) & 1u);
+            r = mont_sq_lazy(r, mo, ninv) << ((x >> 28) & 1u);
+        }
+        for (; n > 0; --n, x <<= 1) r = mont_sq_lazy(r, mo, ninv) << (x >> 31);
+        const uint32_t d = r - two_mo;
+        r = d < r ? d : r;
+    } else {
+        r = dbl_lazy(r, two_mo);
+        for (; n >= 4; n -= 4, x <<= 4) {
+            r = mont_sq_lazy(r, mo, ninv);
+            if ((int32_t)x < 0) r = dbl_lazy(r, two_mo);
+            r = mont_sq_lazy(r, mo, ninv);
+            if ((int32_t)(x << 1) < 0) r = dbl_lazy(r, two_mo);
+            r = mont_sq_lazy(r, mo, ninv);
+            if ((int32_t)(x << 2) < 0) r = dbl_lazy(r, two_mo);
+            r = mont_sq_lazy(r, mo, ninv);
+            if ((int32_t)(x << 3) < 0) r = dbl_lazy(r, two_mo);
+        }
+        for (; n > 0; --n, x <<= 1) {
+            r = mont_sq_lazy(r, mo, ninv);
+            if ((int32_t)x < 0) r = dbl_lazy(r, two_mo);
+        }
     }
     const uint32_t rem2 = r >= mo ? r - mo : r;
     const uint32_t mq = r * ninv;
@@ -417,13 +435,16 @@ __global__ __launch_bounds__(256) void k_bbp_fx_lanes(const BbpSeries* __restric
     const uint32_t j = w == 0 ? 1u : w + 3u;              // 1, 4, 5, 6
     const uint32_t sj = j == 4 ? 2u : (j == 6 ? 1u : 0u); // 2-adic valuation of 8k + j
     uint64_t local = 0;
+    // every modulus of the block is 8k + j <= 8 dm + 6: below 2^28 for dm < 2^25 - 1
+    const bool small = dm < (1u << 25) - 1u;
     for (uint32_t e = e0; e <= e1; ++e) {
         if (!act || e > d) continue;
         const uint32_t k = d - e;
         const uint32_t mi = 8u * k + j;
         if (mi == 1) continue;                            // 16^d mod 1 = 0
         if (mi < (1u << 30) && e > 0) {
-            local += frac64_mont_uniform(4u * e - sj + 32u, mi, sj);
+            local += small ? frac64_mont_uniform<true>(4u * e - sj + 32u, mi, sj)
+                           : frac64_mont_uniform<false>(4u * e - sj + 32u, mi, sj);
         } else {                                          // 16^0 = 1, or m >= 2^30
             const double m = (double)mi;
             const double minv = recip_f64(m);

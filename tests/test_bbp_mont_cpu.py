@@ -156,7 +156,7 @@ def test_frac64_edges():
             assert frac64_mont_wave(e, m) == frac_exact(e, m), (e, m)
 
 
-def frac64_mont_uniform(e: int, m: int, q_off: int = 0) -> int:
+def frac64_mont_uniform(e: int, m: int, q_off: int = 0, small: bool = False) -> int:
     """The positions-in-lanes form: the same set-up, the leading exponent bit
     taken as one doubling of the Montgomery one, then the remaining bits
     left-aligned in a 32-bit word, four per trip and the rest one by one."""
@@ -190,7 +190,25 @@ def frac64_mont_uniform(e: int, m: int, q_off: int = 0) -> int:
     n = E.bit_length() - 1
     assert 5 <= n <= 31
     x = (E << (32 - n)) & M32
-    r = dbl(r)
+    if small:
+        # mo < 2^28: the ladder runs on [0, 4 mo), each doubling one shift
+        assert mo < 1 << 28
+        r = r << 1
+        while n > 0:
+            assert r < 4 * mo
+            t = r * r
+            mq = ((t & M32) * ninv) & M32
+            r = (mq * mo + t) >> 32
+            assert r < two_mo
+            r = (r << (x >> 31)) & M32
+            n -= 1
+            x = (x << 1) & M32
+        d = (r - two_mo) & M32
+        r = d if d < r else r
+        assert r < two_mo
+        n = 0
+    else:
+        r = dbl(r)
     while n >= 4:
         for i in range(4):
             r = sq(r)
@@ -223,3 +241,12 @@ def test_frac64_uniform_form():
         if m < 1 << 30:
             e = rng.randrange(1, 1 << 28)
             assert frac64_mont_uniform(e, m, rng.choice((-1, 0, 1))) == frac_exact(e, m), (e, m)
+    # the small-modulus form, up to the largest modulus it takes (8 (2^25 - 2) + 6)
+    for m in list(range(2, 300)) + [8 * ((1 << 25) - 2) + j for j in (1, 4, 5, 6)]:
+        if m % 8 in (1, 4, 5, 6):
+            for e in (1, 2, 3, 5, 8, 13, 1000, (1 << 25) - 2):
+                assert frac64_mont_uniform(e, m, small=True) == frac_exact(e, m), (e, m)
+    for _ in range(4000):
+        m = 8 * rng.randrange(256, (1 << 25) - 1) + rng.choice((1, 4, 5, 6))
+        e = rng.randrange(1, 1 << 25)
+        assert frac64_mont_uniform(e, m, rng.choice((-1, 0, 1)), small=True) == frac_exact(e, m), (e, m)
