@@ -263,6 +263,11 @@ __device__ __forceinline__ uint64_t frac64(uint32_t r, uint32_t m, double minv) 
 // estimate q within one of floor(2^32 / mo): 2^32 - q mo, formed mod 2^32,
 // lies in [-mo, 2 mo), and one add of mo fixes the negative case. The
 // indices fit 32 bits: positions are at most 2^28, so m = 8k + j < 2^31.
+// SMALL (every modulus below 2^28, as for series with d < 2^25 - 1): the
+// ladder runs on [0, 4 mo) -- a Montgomery square of r < 4 mo is below 2 mo
+// when mo < 2^28 -- so a bit is the square and one shift, five VALU, and one
+// reduction to [0, 2 mo) follows the loop.
+template <bool SMALL>
 __device__ __forceinline__ uint64_t frac64_mont_wave(uint32_t e, uint32_t m) {
     const int s = __builtin_ctz(m);
     const uint32_t mo = m >> s;
@@ -284,6 +289,12 @@ __device__ __forceinline__ uint64_t frac64_mont_wave(uint32_t e, uint32_t m) {
         const uint32_t mq = (uint32_t)t * ninv;
         r = (uint32_t)(((uint64_t)mq * mo + t) >> 32);
         r <<= (E >> b) & 1u;
+        if constexpr (!SMALL) {
+            const uint32_t d = r - two_mo;
+            r = d < r ? d : r;
+        }
+    }
+    if constexpr (SMALL) {
         const uint32_t d = r - two_mo;
         r = d < r ? d : r;
     }
@@ -307,6 +318,7 @@ __global__ __launch_bounds__(256) void k_bbp_fx_terms(const BbpSeries* __restric
     // positions are at most 2^28 (the entry points check), so d, k < 2^28 and
     // m = 8k + j < 2^31: the term indices run in 32 bits
     const uint32_t d = (uint32_t)S.d, base = (uint32_t)base64;
+    const bool small = d < (1u << 25) - 1u;           // every m = 8k + j < 2^28
     uint64_t local = 0;
     for (int i = 0; i < KT; ++i) {
         const uint32_t k = base + (uint32_t)i * 256u + threadIdx.x;
@@ -314,7 +326,7 @@ __global__ __launch_bounds__(256) void k_bbp_fx_terms(const BbpSeries* __restric
         const uint32_t mi = 8u * k + S.j;
         if (mi == 1) continue;                        // 16^d mod 1 = 0
         if (mi < (1u << 30) && k < d) {
-            local += frac64_mont_wave(d - k, mi);
+            local += small ? frac64_mont_wave<true>(d - k, mi) : frac64_mont_wave<false>(d - k, mi);
         } else {                                      // 16^0 = 1, or m >= 2^30
             const double m = (double)mi;
             const double minv = recip_f64(m);

@@ -80,7 +80,7 @@ def test_edges():
 
 # ---- frac64_mont_wave: the fixed-point fraction floor((16^e mod m) 2^64 / m) ----
 
-def frac64_mont_wave(e: int, m: int, q_off: int = 0, emax: int | None = None) -> int:
+def frac64_mont_wave(e: int, m: int, q_off: int = 0, emax: int | None = None, small: bool = False) -> int:
     """Step for step as the kernel: inverse from (3 mo) xor 2, the Montgomery
     one from a quotient q within one of floor(2^32 / mo) (q_off models the
     single-precision reciprocal's error), the ladder on E = 4e - s + 32 run
@@ -103,12 +103,20 @@ def frac64_mont_wave(e: int, m: int, q_off: int = 0, emax: int | None = None) ->
     two_mo = 2 * mo
     bits = (emax if emax is not None else E).bit_length()
     assert E.bit_length() <= bits
+    if small:
+        assert mo < 1 << 28
     for b in range(bits - 1, -1, -1):
+        assert r < (4 * mo if small else two_mo)
         t = r * r
         mq = ((t & M32) * ninv) & M32
         r = (mq * mo + t) >> 32
         assert r < two_mo
         r = (r << ((E >> b) & 1)) & M32
+        if not small:
+            d = (r - two_mo) & M32
+            r = d if d < r else r
+            assert r < two_mo
+    if small:
         d = (r - two_mo) & M32
         r = d if d < r else r
         assert r < two_mo
@@ -146,6 +154,20 @@ def test_frac64_random_with_quotient_error_and_wave_padding():
         q_off = rng.choice((-1, 0, 1))
         emax = 4 * (e + rng.randrange(0, 256)) + 32
         assert frac64_mont_wave(e, m, q_off, emax) == frac_exact(e, m), (e, m, q_off)
+
+
+def test_frac64_small_moduli_form():
+    """Series with d < 2^25 - 1 (every m below 2^28): the ladder on [0, 4 mo)."""
+    rng = random.Random(13)
+    for m in [8 * ((1 << 25) - 2) + j for j in (1, 4, 5, 6)] + list(range(2, 200)):
+        if m % 8 in (1, 4, 5, 6):
+            for e in (1, 2, 7, 64, (1 << 25) - 2):
+                assert frac64_mont_wave(e, m, small=True) == frac_exact(e, m), (e, m)
+    for _ in range(4000):
+        m = 8 * rng.randrange(256, (1 << 25) - 1) + rng.choice((1, 4, 5, 6))
+        e = rng.randrange(1, (1 << 25) - 1)
+        emax = 4 * (e + rng.randrange(0, 256)) + 32
+        assert frac64_mont_wave(e, m, rng.choice((-1, 0, 1)), emax, small=True) == frac_exact(e, m), (e, m)
 
 
 def test_frac64_edges():
