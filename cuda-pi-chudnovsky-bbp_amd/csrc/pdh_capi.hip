@@ -9,9 +9,14 @@
 //  reference allocates and frees 24N bytes on every call
 //  (src/ntt_cuda.cu:198-201,237).
 //
-//  Calls are serialised by a mutex. Device-resident calls on different
-//  streams share the one workspace, so they must not overlap in time on the
-//  device; the host entry points synchronise before returning.
+//  Calls are serialised by a mutex while they enqueue. Device-resident calls
+//  on different streams share the one workspace: the stream of the latest
+//  call that used a piece of it records an event, and a call on another
+//  stream makes its stream wait for that event before its own launches
+//  (ScratchOrder), so such calls may be issued from any thread on any
+//  stream. The host entry points synchronise before returning. BBP calls
+//  take their scratch per call from the block cache and hold the mutex only
+//  while they read the configuration, so they overlap with everything else.
 // ============================================================================
 #include "pdh_capi.h"
 #include "gl_field.h"
@@ -41,9 +46,50 @@ int fail_hip(hipError_t e, const char* what) {
 struct Outer { u64* hi = nullptr; u64* lo = nullptr; u64* hi_inv = nullptr; u64* lo_inv = nullptr; int h = 0;
                u32* o3 = nullptr; int h3 = 0; };
 
+// Reuse of one piece of the context's scratch by calls on different streams.
+// The stream of the latest call that used it records `ev` after its
+// launches; a call on another stream first makes its stream wait for that
+// event (no host synchronisation). Calls on one stream are ordered by the
+// stream itself. Streams under graph capture neither wait nor record: a
+// capturing caller orders its own work.
+struct ScratchOrder {
+    hipEvent_t ev = nullptr;
+    hipStream_t last = nullptr;
+    bool used = false;
+};
+
+bool capturing(hipStream_t s) {
+    if (!s) return false;
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &st) != hipSuccess) { (void)hipGetLastError(); return false; }
+    return st != hipStreamCaptureStatusNone;
+}
+
+int scratch_acquire(ScratchOrder& o, hipStream_t s) {
+    if (o.used && o.last != s && !capturing(s)) HIPCALL(hipStreamWaitEvent(s, o.ev, 0));
+    return PDH_OK;
+}
+
+int scratch_release(ScratchOrder& o, hipStream_t s) {
+    if (capturing(s)) return PDH_OK;
+    if (!o.ev) HIPCALL(hipEventCreateWithFlags(&o.ev, hipEventDisableTiming));
+    HIPCALL(hipEventRecord(o.ev, s));
+    o.last = s;
+    o.used = true;
+    return PDH_OK;
+}
+
+// The host waits until the latest user of the scratch is done (before a
+// synchronous host-to-device copy into it).
+int scratch_wait_host(ScratchOrder& o) {
+    if (o.used) HIPCALL(hipEventSynchronize(o.ev));
+    return PDH_OK;
+}
+
 struct Ctx {
     std::mutex mu;
     int state = 0;          // 0 unknown, 1 ready, -1 no device
+    int device = -1;        // the device current at set-up; every call must run with it current
     std::string why;
     u64* twL = nullptr;     // 2^14 entries: twL[2^L + x] = w_{2^L}^x
     u32* tw3 = nullptr;     // three primes x 2^15 entries: w_{2^L}^x R mod p
@@ -51,21 +97,14 @@ struct Ctx {
     std::map<int, Outer> outer;
     u64* wsA = nullptr; u64* wsB = nullptr; size_t ws_words = 0;
     u64* carry = nullptr; size_t carry_bytes = 0;
-    u32* st_a = nullptr; u32* st_b = nullptr; u32* st_out = nullptr; size_t st_limbs = 0;
-    u64* st_len = nullptr;
-    uint64_t* st_pos = nullptr; uint8_t* st_val = nullptr; size_t st_bbp = 0;
-    void* bbp_ws = nullptr; size_t bbp_ws_bytes = 0;
-    uint64_t* bbp_scan = nullptr;    // k_bbp_group_scan's three words
+    ScratchOrder ws_order;           // wsA, wsB, carry
     int bbp_layout = 0;              // pdh_bbp_set_layout mode
-    uint64_t* fix_pos = nullptr; size_t fix_pos_n = 0;
-    uint8_t* fix_val = nullptr; size_t fix_val_n = 0;
-    uint64_t* und_pos = nullptr; size_t und_pos_n = 0;     // undecided positions of a digit-block range
-    uint8_t* und_val = nullptr; size_t und_val_n = 0;
     int bbp_exact = 0;               // pdh_bbp_set_exact mode
-    uint64_t bbp_exact_count = 0;    // positions computed by the ordered path
-    hipStream_t stream = nullptr;
+    std::atomic<uint64_t> bbp_exact_count{0};   // positions computed by the ordered path
     uint8_t* sadd_status = nullptr; size_t sadd_status_n = 0;   // tile statuses of pdh_sadd_dev
+    ScratchOrder sadd_order;
     char* blk_scratch = nullptr; size_t blk_scratch_n = 0;      // block product and lengths of blocked multiplies
+    ScratchOrder blk_order;
     std::vector<hipStream_t> sh_streams;   // internal streams of one-shot batches
     std::vector<hipEvent_t> sh_events;
 };
@@ -75,11 +114,14 @@ Ctx& ctx() { static Ctx c; return c; }
 // Device blocks for scratch, batch workspaces and pdh_dev_alloc are kept for
 // reuse instead of being returned with hipFree, which waits for the whole
 // device and costs tens of microseconds per call. A block re-enters the cache
-// only when no work can still use it (internal users free it after
+// only when no work can still use it: internal users free it after
 // synchronizing the stream that used it; pdh_dev_free waits for the device,
-// as hipFree would). An allocation takes the smallest cached block of at
-// least the requested size and at most twice it; at most kCacheMax bytes are
-// kept, and blocks above kBlockMax are never cached.
+// as hipFree would; pdh_dev_free_async records an event on the stream of the
+// block's last use and the block turns idle once that event has completed
+// (checked without waiting by every allocation, waited for only when the
+// device runs out of memory). An allocation takes the smallest cached block
+// of at least the requested size and at most twice it; at most kCacheMax
+// bytes are kept, and blocks above kBlockMax are never cached.
 //
 // Small requests (up to kSmallMax) take power-of-two classes, and a class
 // that runs dry is refilled with kSlab blocks carved from one allocation;
@@ -91,6 +133,8 @@ struct BlockCache {
     std::multimap<size_t, void*> idle;     // size -> block
     std::map<void*, size_t> size_of;       // every block the cache handed out
     size_t idle_bytes = 0;                 // large idle blocks only
+    std::vector<std::pair<void*, hipEvent_t>> pending;   // freed once the event completes
+    std::vector<hipEvent_t> ev_pool;
 };
 BlockCache& bcache() { static BlockCache b; return b; }
 constexpr size_t kCacheMax = (size_t)8 << 30;
@@ -108,10 +152,42 @@ void cache_trim(BlockCache& b) {   // caller holds b.mu; small blocks stay
     b.idle_bytes = 0;
 }
 
+void cache_put_locked(BlockCache& b, void* p) {
+    auto it = b.size_of.find(p);
+    if (it == b.size_of.end()) { (void)hipFree(p); return; }
+    const size_t sz = it->second;
+    if (sz <= kSmallMax) {                 // a slab block: back to its class
+        b.idle.emplace(sz, p);
+        return;
+    }
+    if (sz > kBlockMax || b.idle_bytes + sz > kCacheMax) {
+        b.size_of.erase(it);
+        (void)hipFree(p);
+        return;
+    }
+    b.idle.emplace(sz, p);
+    b.idle_bytes += sz;
+}
+
+// Pending blocks whose last use has completed become idle; with `wait`, every
+// pending block is waited for. Caller holds b.mu.
+void cache_reap(BlockCache& b, bool wait) {
+    for (size_t i = 0; i < b.pending.size();) {
+        const hipError_t q = wait ? hipEventSynchronize(b.pending[i].second) : hipEventQuery(b.pending[i].second);
+        if (q == hipErrorNotReady) { (void)hipGetLastError(); ++i; continue; }
+        void* p = b.pending[i].first;
+        b.ev_pool.push_back(b.pending[i].second);
+        b.pending[i] = b.pending.back();
+        b.pending.pop_back();
+        cache_put_locked(b, p);
+    }
+}
+
 hipError_t cache_alloc(void** p, size_t bytes) {
     bytes = (std::max<size_t>(bytes, 1) + 255) & ~(size_t)255;
     BlockCache& b = bcache();
     std::lock_guard<std::mutex> g(b.mu);
+    if (!b.pending.empty()) cache_reap(b, false);
     if (bytes <= kSmallMax) {
         size_t cls = 4096;
         while (cls < bytes) cls <<= 1;
@@ -139,8 +215,9 @@ hipError_t cache_alloc(void** p, size_t bytes) {
         return hipSuccess;
     }
     hipError_t e = hipMalloc(p, bytes);
-    if (e == hipErrorOutOfMemory && !b.idle.empty()) {
+    if (e == hipErrorOutOfMemory && (!b.idle.empty() || !b.pending.empty())) {
         (void)hipGetLastError();
+        cache_reap(b, true);
         cache_trim(b);
         e = hipMalloc(p, bytes);
     }
@@ -152,21 +229,71 @@ void cache_free(void* p) {
     if (!p) return;
     BlockCache& b = bcache();
     std::lock_guard<std::mutex> g(b.mu);
-    auto it = b.size_of.find(p);
-    if (it == b.size_of.end()) { (void)hipFree(p); return; }
-    const size_t sz = it->second;
-    if (sz <= kSmallMax) {                 // a slab block: back to its class
-        b.idle.emplace(sz, p);
-        return;
-    }
-    if (sz > kBlockMax || b.idle_bytes + sz > kCacheMax) {
-        b.size_of.erase(it);
-        (void)hipFree(p);
-        return;
-    }
-    b.idle.emplace(sz, p);
-    b.idle_bytes += sz;
+    cache_put_locked(b, p);
 }
+
+// The block becomes reusable once the work enqueued on `s` so far has run.
+hipError_t cache_free_after(void* p, hipStream_t s) {
+    if (!p) return hipSuccess;
+    BlockCache& b = bcache();
+    std::lock_guard<std::mutex> g(b.mu);
+    hipEvent_t ev = nullptr;
+    if (!b.ev_pool.empty()) { ev = b.ev_pool.back(); b.ev_pool.pop_back(); }
+    else {
+        hipError_t e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+        if (e != hipSuccess) return e;
+    }
+    hipError_t e = hipEventRecord(ev, s);
+    if (e != hipSuccess) { b.ev_pool.push_back(ev); return e; }
+    b.pending.emplace_back(p, ev);
+    return hipSuccess;
+}
+
+// Device scratch of one call, from the block cache. Blocks go back when
+// the call ends, once the work enqueued on its stream so far has run.
+struct CallScratch {
+    hipStream_t st;
+    std::vector<void*> blocks;
+    explicit CallScratch(hipStream_t s) : st(s) {}
+    ~CallScratch() { for (void* p : blocks) (void)cache_free_after(p, st); }
+    CallScratch(const CallScratch&) = delete;
+    CallScratch& operator=(const CallScratch&) = delete;
+    template <class T> int get(T** p, size_t bytes) {
+        void* q = nullptr;
+        hipError_t e = cache_alloc(&q, bytes);
+        if (e != hipSuccess) return fail_hip(e, "scratch allocation");
+        blocks.push_back(q);
+        *p = (T*)q;
+        return PDH_OK;
+    }
+};
+
+// Internal streams for host-buffer calls (multiply, BBP), one per concurrent
+// call, kept for reuse.
+struct StreamPool {
+    std::mutex mu;
+    std::vector<hipStream_t> idle;
+};
+StreamPool& spool() { static StreamPool p; return p; }
+
+struct PooledStream {
+    hipStream_t s = nullptr;
+    ~PooledStream() {
+        if (!s) return;
+        StreamPool& p = spool();
+        std::lock_guard<std::mutex> g(p.mu);
+        p.idle.push_back(s);
+    }
+    int get() {
+        StreamPool& p = spool();
+        {
+            std::lock_guard<std::mutex> g(p.mu);
+            if (!p.idle.empty()) { s = p.idle.back(); p.idle.pop_back(); return PDH_OK; }
+        }
+        HIPCALL(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        return PDH_OK;
+    }
+};
 
 int init_locked(Ctx& c) {
     if (c.state == 1) return PDH_OK;
@@ -180,6 +307,7 @@ int init_locked(Ctx& c) {
     }
     int dev = 0;
     HIPCALL(hipGetDevice(&dev));
+    c.device = dev;
     hipDeviceProp_t prop;
     HIPCALL(hipGetDeviceProperties(&prop, dev));
     if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
@@ -189,7 +317,6 @@ int init_locked(Ctx& c) {
     }
     HIPCALL(init_kernel_attrs());
     HIPCALL(init_kernel_attrs3());
-    HIPCALL(hipStreamCreate(&c.stream));
     // twL[2^L + x] = w_{2^L}^x, L = 1..13
     std::vector<u64> tw((size_t)1 << 14, 0);
     for (int L = 1; L <= 13; ++L) {
@@ -294,7 +421,7 @@ int outer_for(Ctx& c, int n, OuterTables* ot) {
 template <class P>
 int grow(P** p, size_t* have, size_t need, size_t elem) {
     if (*have >= need) return PDH_OK;
-    if (*p) HIPCALL(hipFree(*p));
+    if (*p) { HIPCALL(hipDeviceSynchronize()); HIPCALL(hipFree(*p)); }   // queued work may still use it
     *p = nullptr;
     *have = 0;
     HIPCALL(hipMalloc((void**)p, need * elem));
@@ -304,6 +431,7 @@ int grow(P** p, size_t* have, size_t need, size_t elem) {
 
 int ensure_ws(Ctx& c, size_t N, size_t nlimbs) {
     if (c.ws_words < N) {
+        if (c.wsA) HIPCALL(hipDeviceSynchronize());    // queued work on any stream may still use it
         if (c.wsA) HIPCALL(hipFree(c.wsA));
         if (c.wsB) HIPCALL(hipFree(c.wsB));
         c.wsA = c.wsB = nullptr;
@@ -346,8 +474,11 @@ int mul_blocked_locked(Ctx& c, const u32* a, size_t na, const u32* b, size_t nb,
     u64* dwl = PL + 1;
     u64* slen = dwl + wl.size();
     int32_t* ssgn = (int32_t*)(slen + 1);
-    HIPCALL(hipStreamSynchronize(s));                  // the scratch may still serve earlier work on s
+    // the scratch may still serve earlier work on any stream: the host waits
+    // for it before the synchronous upload of the window lengths
+    if ((rc = scratch_wait_host(c.blk_order))) return rc;
     HIPCALL(hipMemcpy(dwl, wl.data(), wl.size() * 8, hipMemcpyHostToDevice));
+    if ((rc = scratch_acquire(c.sadd_order, s))) return rc;
     hipError_t e = launch_fill_zero(out, total, s);
     if (e != hipSuccess) return fail_hip(e, "zero fill launch");
     size_t k = 0;
@@ -365,7 +496,8 @@ int mul_blocked_locked(Ctx& c, const u32* a, size_t na, const u32* b, size_t nb,
     const BsAddNode norm{out, out, out, dwl + k, dwl + k, total, out_len, ssgn, 0u, (uint32_t)nt, 1, 0};
     e = launch_bs_signed_add(nullptr, nullptr, (uint32_t)nt, c.sadd_status, s, &norm);
     if (e != hipSuccess) return fail_hip(e, "normalize launch");
-    return PDH_OK;
+    if ((rc = scratch_release(c.sadd_order, s))) return rc;
+    return scratch_release(c.blk_order, s);
 }
 
 int mul_dev_locked(Ctx& c, const u32* a, size_t na, const u32* b, size_t nb, bool square,
@@ -384,11 +516,15 @@ int mul_dev_locked(Ctx& c, const u32* a, size_t na, const u32* b, size_t nb, boo
     OuterTables ot{nullptr, nullptr, nullptr, nullptr, 0, nullptr, nullptr, 0};
     const NttShape sh = ntt_shape(n);
     if (sh.L1 > 0 && (rc = outer_for(c, n, &ot))) return rc;
+    if ((rc = scratch_acquire(c.ws_order, s))) return rc;
     hipError_t e = launch_mul(a, na, b, nb, square, out, out_len, c.wsA, c.wsB, c.carry, c.twL, ot, sh, s);
     if (e != hipSuccess) return fail_hip(e, "multiply launch");
-    return PDH_OK;
+    return scratch_release(c.ws_order, s);
 }
 
+// Host-buffer multiply: operands staged into per-call device scratch on a
+// pooled stream; the context mutex is held only while the transforms are
+// enqueued, not while the host waits for them.
 int mul_host(const u32* a, size_t na, const u32* b, size_t nb, bool square,
              u32* out, size_t out_cap, size_t* out_len) {
     if (!out_len || (na && !a) || (!square && nb && !b)) return fail(PDH_E_ARG, "null pointer argument");
@@ -396,31 +532,43 @@ int mul_host(const u32* a, size_t na, const u32* b, size_t nb, bool square,
     if (na == 0 || nb == 0) { *out_len = 0; return PDH_OK; }
     if (!out || out_cap < na + nb) return fail(PDH_E_CAPACITY, "output capacity below na + nb limbs");
     Ctx& c = ctx();
-    std::lock_guard<std::mutex> g(c.mu);
-    int rc = init_locked(c);
-    if (rc) return rc;
-    const size_t nl = na + nb;
-    if (c.st_limbs < nl) {
-        if (c.st_a) HIPCALL(hipFree(c.st_a));
-        if (c.st_b) HIPCALL(hipFree(c.st_b));
-        if (c.st_out) HIPCALL(hipFree(c.st_out));
-        c.st_a = c.st_b = c.st_out = nullptr;
-        c.st_limbs = 0;
-        HIPCALL(hipMalloc(&c.st_a, nl * 4));
-        HIPCALL(hipMalloc(&c.st_b, nl * 4));
-        HIPCALL(hipMalloc(&c.st_out, nl * 4));
-        c.st_limbs = nl;
+    {
+        std::lock_guard<std::mutex> g(c.mu);
+        int rc = init_locked(c);
+        if (rc) return rc;
     }
-    if (!c.st_len) HIPCALL(hipMalloc(&c.st_len, 8));
-    HIPCALL(hipMemcpyAsync(c.st_a, a, na * 4, hipMemcpyHostToDevice, c.stream));
-    if (!square) HIPCALL(hipMemcpyAsync(c.st_b, b, nb * 4, hipMemcpyHostToDevice, c.stream));
-    rc = mul_dev_locked(c, c.st_a, na, square ? c.st_a : c.st_b, nb, square, c.st_out, c.st_len, c.stream);
+    PooledStream ps;
+    int rc = ps.get();
     if (rc) return rc;
+    const hipStream_t st = ps.s;
+    const size_t nl = na + nb;
     u64 len = 0;
-    HIPCALL(hipMemcpyAsync(&len, c.st_len, 8, hipMemcpyDeviceToHost, c.stream));
-    HIPCALL(hipStreamSynchronize(c.stream));
-    if (len > nl) return fail(PDH_E_HIP, "device returned an invalid product length");
-    if (len) HIPCALL(hipMemcpy(out, c.st_out, len * 4, hipMemcpyDeviceToHost));
+    {
+        CallScratch sc(st);
+        char* w = nullptr;
+        const size_t oa = 0, ob = (na * 4 + 255) & ~(size_t)255;
+        const size_t oo = ob + (square ? 0 : (nb * 4 + 255) & ~(size_t)255);
+        const size_t ol = oo + ((nl * 4 + 255) & ~(size_t)255);
+        if ((rc = sc.get(&w, ol + 8))) return rc;
+        u32* da = (u32*)(w + oa);
+        u32* db = square ? da : (u32*)(w + ob);
+        u32* dout = (u32*)(w + oo);
+        u64* dlen = (u64*)(w + ol);
+        HIPCALL(hipMemcpyAsync(da, a, na * 4, hipMemcpyHostToDevice, st));
+        if (!square) HIPCALL(hipMemcpyAsync(db, b, nb * 4, hipMemcpyHostToDevice, st));
+        {
+            std::lock_guard<std::mutex> g(c.mu);
+            rc = mul_dev_locked(c, da, na, db, nb, square, dout, dlen, st);
+        }
+        if (rc) return rc;
+        HIPCALL(hipMemcpyAsync(&len, dlen, 8, hipMemcpyDeviceToHost, st));
+        HIPCALL(hipStreamSynchronize(st));
+        if (len > nl) return fail(PDH_E_HIP, "device returned an invalid product length");
+        if (len) {
+            HIPCALL(hipMemcpyAsync(out, dout, len * 4, hipMemcpyDeviceToHost, st));
+            HIPCALL(hipStreamSynchronize(st));
+        }
+    }
     *out_len = (size_t)len;
     return PDH_OK;
 }
@@ -439,9 +587,16 @@ uint32_t bbp_block_digits(uint64_t dmax) {
     return G;
 }
 
-int bbp_few_locked(Ctx& c, const uint64_t* d_pos, const uint64_t* h_pos, uint64_t start, size_t count,
-                   uint64_t dmax, uint8_t* d_out, uint8_t* h_vals, hipStream_t st, bool blocks = false,
-                   double share = -1.0);
+// The BBP configuration a call runs with (read under the context mutex at
+// its start; the call itself holds no lock).
+struct BbpCfg {
+    int exact = 0;     // pdh_bbp_set_exact mode
+    int layout = 0;    // pdh_bbp_set_layout mode
+};
+
+int bbp_few(const BbpCfg& cfg, const uint64_t* d_pos, const uint64_t* h_pos, uint64_t start, size_t count,
+            uint64_t dmax, uint8_t* d_out, uint8_t* h_vals, hipStream_t st, bool blocks = false,
+            double share = -1.0);
 
 // Share of lanes the positions-in-lanes term kernel keeps busy: positions
 // h_pos[i] (or start + i stride) in groups of 64, sum of positions over 64 x
@@ -465,20 +620,22 @@ double lane_share(const uint64_t* h_pos, uint64_t start, uint64_t stride, size_t
 // (measured: 1..65,536, share 0.99, 27 against 36 ms; 512 sorted positions
 // in [10^6, 10^8], share 0.89, 0.47 against 0.80 s), so it pays from a
 // share of about 0.6 up (64 such positions have 0.52).
-bool bbp_use_lanes(const Ctx& c, double share) {
-    return c.bbp_layout == 2 || (c.bbp_layout == 0 && share >= 0.6);
+bool bbp_use_lanes(const BbpCfg& cfg, double share) {
+    return cfg.layout == 2 || (cfg.layout == 0 && share >= 0.6);
 }
 
 // A contiguous range by digit blocks: one fixed-point evaluation per G
 // positions (k_bbp_fx_block), then the per-position paths for the digits
 // left undecided.
-int bbp_blocks_locked(Ctx& c, uint64_t start, size_t count, uint32_t G, uint8_t* d_out, uint8_t* h_vals,
-                      hipStream_t st) {
+int bbp_blocks(const BbpCfg& cfg, uint64_t start, size_t count, uint32_t G, uint8_t* d_out, uint8_t* h_vals,
+               hipStream_t st) {
+    CallScratch sc(st);
     const size_t A = (count + G - 1) / G;
-    int rc = grow((char**)&c.bbp_ws, &c.bbp_ws_bytes, bbp_few_ws_bytes(A), 1);
+    void* ws = nullptr;
+    int rc = sc.get(&ws, bbp_few_ws_bytes(A));
     if (rc) return rc;
-    hipError_t e = launch_bbp_fx(nullptr, start, A, start - 1 + (A - 1) * (uint64_t)G, d_out, c.bbp_ws,
-                                 c.bbp_exact == 2, st, G, count, bbp_use_lanes(c, lane_share(nullptr, start, G, A)));
+    hipError_t e = launch_bbp_fx(nullptr, start, A, start - 1 + (A - 1) * (uint64_t)G, d_out, ws,
+                                 cfg.exact == 2, st, G, count, bbp_use_lanes(cfg, lane_share(nullptr, start, G, A)));
     if (e != hipSuccess) return fail_hip(e, "bbp launch");
     HIPCALL(hipMemcpyAsync(h_vals, d_out, count, hipMemcpyDeviceToHost, st));
     HIPCALL(hipStreamSynchronize(st));
@@ -488,16 +645,15 @@ int bbp_blocks_locked(Ctx& c, uint64_t start, size_t count, uint32_t G, uint8_t*
         if (h_vals[i] == 0xFE) { und.push_back(i); upos.push_back(start + i); }
     if (und.empty()) return PDH_OK;
     const size_t nu = und.size();
-    if ((rc = grow((char**)&c.und_pos, &c.und_pos_n, nu, 8))) return rc;
-    if ((rc = grow((char**)&c.und_val, &c.und_val_n, nu, 1))) return rc;
-    HIPCALL(hipMemcpyAsync(c.und_pos, upos.data(), nu * 8, hipMemcpyHostToDevice, st));
+    uint64_t* dpos = nullptr;
+    uint8_t* dval = nullptr;
+    if ((rc = sc.get(&dpos, nu * 8)) || (rc = sc.get(&dval, nu))) return rc;
+    HIPCALL(hipMemcpyAsync(dpos, upos.data(), nu * 8, hipMemcpyHostToDevice, st));
     std::vector<uint8_t> fixed(nu);
     // under the test mode 2 (every odd index undecided) those digits all take the ordered path
-    const int mode = c.bbp_exact;
-    if (mode == 2) c.bbp_exact = 1;
-    rc = bbp_few_locked(c, c.und_pos, upos.data(), 0, nu, upos.back() - 1, c.und_val, fixed.data(), st);
-    c.bbp_exact = mode;
-    if (rc) return rc;
+    BbpCfg sub = cfg;
+    if (sub.exact == 2) sub.exact = 1;
+    if ((rc = bbp_few(sub, dpos, upos.data(), 0, nu, upos.back() - 1, dval, fixed.data(), st))) return rc;
     for (size_t i = 0; i < nu; ++i) h_vals[und[i]] = fixed[i];
     HIPCALL(hipMemcpyAsync(d_out, h_vals, count, hipMemcpyHostToDevice, st));
     HIPCALL(hipStreamSynchronize(st));
@@ -509,25 +665,27 @@ int bbp_blocks_locked(Ctx& c, uint64_t start, size_t count, uint32_t G, uint8_t*
 // range) and d_out are device buffers; h_pos, when given, holds the same
 // positions on the host. The values end in d_out and in h_vals. Synchronises
 // `st`. A contiguous range with `blocks` set goes by digit blocks.
-int bbp_few_locked(Ctx& c, const uint64_t* d_pos, const uint64_t* h_pos, uint64_t start, size_t count,
-                   uint64_t dmax, uint8_t* d_out, uint8_t* h_vals, hipStream_t st, bool blocks, double share) {
-    if (blocks && !d_pos && c.bbp_exact != 1 && count >= 16) {
+int bbp_few(const BbpCfg& cfg, const uint64_t* d_pos, const uint64_t* h_pos, uint64_t start, size_t count,
+            uint64_t dmax, uint8_t* d_out, uint8_t* h_vals, hipStream_t st, bool blocks, double share) {
+    if (blocks && !d_pos && cfg.exact != 1 && count >= 16) {
         const uint32_t G = bbp_block_digits(dmax);
-        if (G > 1) return bbp_blocks_locked(c, start, count, G, d_out, h_vals, st);
+        if (G > 1) return bbp_blocks(cfg, start, count, G, d_out, h_vals, st);
     }
-    int rc = grow((char**)&c.bbp_ws, &c.bbp_ws_bytes, bbp_few_ws_bytes(count), 1);
+    CallScratch sc(st);
+    void* ws = nullptr;
+    int rc = sc.get(&ws, bbp_few_ws_bytes(count));
     if (rc) return rc;
-    if (c.bbp_exact == 1) {
-        hipError_t e = launch_bbp_chunked(d_pos, start, count, dmax, d_out, c.bbp_ws, st);
+    if (cfg.exact == 1) {
+        hipError_t e = launch_bbp_chunked(d_pos, start, count, dmax, d_out, ws, st);
         if (e != hipSuccess) return fail_hip(e, "bbp launch");
         HIPCALL(hipMemcpyAsync(h_vals, d_out, count, hipMemcpyDeviceToHost, st));
         HIPCALL(hipStreamSynchronize(st));
-        c.bbp_exact_count += count;
+        ctx().bbp_exact_count += count;
         return PDH_OK;
     }
     if (share < 0) share = d_pos && !h_pos ? 0.0 : lane_share(h_pos, start, 1, count);
-    hipError_t e = launch_bbp_fx(d_pos, start, count, dmax, d_out, c.bbp_ws, c.bbp_exact == 2, st, 1, 0,
-                                 bbp_use_lanes(c, share));
+    hipError_t e = launch_bbp_fx(d_pos, start, count, dmax, d_out, ws, cfg.exact == 2, st, 1, 0,
+                                 bbp_use_lanes(cfg, share));
     if (e != hipSuccess) return fail_hip(e, "bbp launch");
     HIPCALL(hipMemcpyAsync(h_vals, d_out, count, hipMemcpyDeviceToHost, st));
     HIPCALL(hipStreamSynchronize(st));
@@ -538,7 +696,8 @@ int bbp_few_locked(Ctx& c, const uint64_t* d_pos, const uint64_t* h_pos, uint64_
     std::vector<uint64_t> all;
     if (d_pos && !h_pos) {
         all.resize(count);
-        HIPCALL(hipMemcpy(all.data(), d_pos, count * 8, hipMemcpyDeviceToHost));
+        HIPCALL(hipMemcpyAsync(all.data(), d_pos, count * 8, hipMemcpyDeviceToHost, st));
+        HIPCALL(hipStreamSynchronize(st));
         h_pos = all.data();
     }
     std::vector<uint64_t> upos(und.size());
@@ -548,19 +707,32 @@ int bbp_few_locked(Ctx& c, const uint64_t* d_pos, const uint64_t* h_pos, uint64_
         umax = std::max(umax, upos[i] - 1);
     }
     const size_t nu = und.size();
-    if ((rc = grow((char**)&c.fix_pos, &c.fix_pos_n, nu, 8))) return rc;
-    if ((rc = grow((char**)&c.fix_val, &c.fix_val_n, nu, 1))) return rc;
-    if ((rc = grow((char**)&c.bbp_ws, &c.bbp_ws_bytes, bbp_few_ws_bytes(nu), 1))) return rc;
-    HIPCALL(hipMemcpyAsync(c.fix_pos, upos.data(), nu * 8, hipMemcpyHostToDevice, st));
-    e = launch_bbp_chunked(c.fix_pos, 0, nu, umax, c.fix_val, c.bbp_ws, st);
+    uint64_t* fpos = nullptr;
+    uint8_t* fval = nullptr;
+    void* fws = nullptr;
+    if ((rc = sc.get(&fpos, nu * 8)) || (rc = sc.get(&fval, nu)) || (rc = sc.get(&fws, bbp_few_ws_bytes(nu))))
+        return rc;
+    HIPCALL(hipMemcpyAsync(fpos, upos.data(), nu * 8, hipMemcpyHostToDevice, st));
+    e = launch_bbp_chunked(fpos, 0, nu, umax, fval, fws, st);
     if (e != hipSuccess) return fail_hip(e, "bbp launch");
     std::vector<uint8_t> fixed(nu);
-    HIPCALL(hipMemcpyAsync(fixed.data(), c.fix_val, nu, hipMemcpyDeviceToHost, st));
+    HIPCALL(hipMemcpyAsync(fixed.data(), fval, nu, hipMemcpyDeviceToHost, st));
     HIPCALL(hipStreamSynchronize(st));
     for (size_t i = 0; i < nu; ++i) h_vals[und[i]] = fixed[i];
     HIPCALL(hipMemcpyAsync(d_out, h_vals, count, hipMemcpyHostToDevice, st));
     HIPCALL(hipStreamSynchronize(st));
-    c.bbp_exact_count += nu;
+    ctx().bbp_exact_count += nu;
+    return PDH_OK;
+}
+
+// Initialises the context if needed and reads the BBP configuration.
+int bbp_begin(BbpCfg* cfg) {
+    Ctx& c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    int rc = init_locked(c);
+    if (rc) return rc;
+    cfg->exact = c.bbp_exact;
+    cfg->layout = c.bbp_layout;
     return PDH_OK;
 }
 
@@ -582,30 +754,32 @@ int bbp_host(const uint64_t* positions, uint64_t start, size_t count, uint8_t* v
     } else if (start < 1 || start - 1 + count > BBP_POS_MAX) {
         return fail(PDH_E_ARG, "BBP position outside [1, 2^28]");
     }
-    Ctx& c = ctx();
-    std::lock_guard<std::mutex> g(c.mu);
-    int rc = init_locked(c);
+    BbpCfg cfg;
+    int rc = bbp_begin(&cfg);
     if (rc) return rc;
-    if (c.st_bbp < count) {
-        if (c.st_pos) HIPCALL(hipFree(c.st_pos));
-        if (c.st_val) HIPCALL(hipFree(c.st_val));
-        c.st_pos = nullptr; c.st_val = nullptr; c.st_bbp = 0;
-        HIPCALL(hipMalloc(&c.st_pos, count * 8));
-        HIPCALL(hipMalloc(&c.st_val, count));
-        c.st_bbp = count;
-    }
-    if (positions) HIPCALL(hipMemcpyAsync(c.st_pos, sorted.data(), count * 8, hipMemcpyHostToDevice, c.stream));
+    PooledStream ps;
+    if ((rc = ps.get())) return rc;
+    const hipStream_t st = ps.s;
     std::vector<uint8_t> tmp(count);
-    if (c.bbp_exact != 3) {
-        const uint64_t dmax = positions ? sorted.back() - 1 : start + count - 2;
-        rc = bbp_few_locked(c, positions ? c.st_pos : nullptr, positions ? sorted.data() : nullptr, start, count,
-                            dmax, c.st_val, tmp.data(), c.stream, !positions);
-        if (rc) return rc;
-    } else {
-        hipError_t e = launch_bbp(positions ? c.st_pos : nullptr, start, count, c.st_val, c.bbp_next, c.stream);
-        if (e != hipSuccess) return fail_hip(e, "bbp launch");
-        HIPCALL(hipMemcpyAsync(tmp.data(), c.st_val, count, hipMemcpyDeviceToHost, c.stream));
-        HIPCALL(hipStreamSynchronize(c.stream));
+    {
+        CallScratch sc(st);
+        uint64_t* dpos = nullptr;
+        uint8_t* dval = nullptr;
+        if ((positions && (rc = sc.get(&dpos, count * 8))) || (rc = sc.get(&dval, count))) return rc;
+        if (positions) HIPCALL(hipMemcpyAsync(dpos, sorted.data(), count * 8, hipMemcpyHostToDevice, st));
+        if (cfg.exact != 3) {
+            const uint64_t dmax = positions ? sorted.back() - 1 : start + count - 2;
+            rc = bbp_few(cfg, dpos, positions ? sorted.data() : nullptr, start, count, dmax, dval, tmp.data(), st,
+                         !positions);
+            if (rc) return rc;
+        } else {
+            Ctx& c = ctx();                        // the work-queue counter is the context's
+            std::lock_guard<std::mutex> g(c.mu);
+            hipError_t e = launch_bbp(dpos, start, count, dval, c.bbp_next, st);
+            if (e != hipSuccess) return fail_hip(e, "bbp launch");
+            HIPCALL(hipMemcpyAsync(tmp.data(), dval, count, hipMemcpyDeviceToHost, st));
+            HIPCALL(hipStreamSynchronize(st));
+        }
     }
     if (positions) for (size_t i = 0; i < count; ++i) values[order[i]] = tmp[i];
     else std::memcpy(values, tmp.data(), count);
@@ -709,14 +883,15 @@ int pdh_bbp_range_dev(uint64_t start, uint32_t count, uint8_t* d_values, void* s
     if (count == 0) return PDH_OK;
     if (!d_values) return fail(PDH_E_ARG, "null output pointer");
     if (start < 1 || start - 1 + count > BBP_POS_MAX) return fail(PDH_E_ARG, "BBP position outside [1, 2^28]");
-    Ctx& c = ctx();
-    std::lock_guard<std::mutex> g(c.mu);
-    int rc = init_locked(c);
+    BbpCfg cfg;
+    int rc = bbp_begin(&cfg);
     if (rc) return rc;
     hipStream_t st = (hipStream_t)stream;
     std::vector<uint8_t> tmp(count);
-    if (c.bbp_exact != 3)
-        return bbp_few_locked(c, nullptr, nullptr, start, count, start + count - 2, d_values, tmp.data(), st, true);
+    if (cfg.exact != 3)
+        return bbp_few(cfg, nullptr, nullptr, start, count, start + count - 2, d_values, tmp.data(), st, true);
+    Ctx& c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
     hipError_t e = launch_bbp(nullptr, start, count, d_values, c.bbp_next, st);
     if (e != hipSuccess) return fail_hip(e, "bbp launch");
     return PDH_OK;
@@ -730,26 +905,31 @@ int pdh_bbp_positions(const uint64_t* positions, size_t n, uint8_t* values) {
 int pdh_bbp_positions_dev(const uint64_t* d_positions, size_t n, uint8_t* d_values, void* stream) {
     if (n == 0) return PDH_OK;
     if (!d_positions || !d_values) return fail(PDH_E_ARG, "null pointer argument");
-    Ctx& c = ctx();
-    std::lock_guard<std::mutex> g(c.mu);
-    int rc = init_locked(c);
+    BbpCfg cfg;
+    int rc = bbp_begin(&cfg);
     if (rc) return rc;
     hipStream_t st = (hipStream_t)stream;
     hipError_t e;
-    if (c.bbp_exact != 3) {
+    if (cfg.exact != 3) {
         // both few-position paths size their grids by the largest position
         // and choose the term layout by how close the positions of each group of 64 lie
-        if (!c.bbp_scan) HIPCALL(hipMalloc(&c.bbp_scan, 24));
-        e = hipMemsetAsync(c.bbp_scan, 0, 24, st);
-        if (e == hipSuccess) e = launch_bbp_group_scan(d_positions, n, c.bbp_scan, st);
         uint64_t sc[3] = {0, 0, 0};
-        if (e == hipSuccess) e = hipMemcpyAsync(sc, c.bbp_scan, 24, hipMemcpyDeviceToHost, st);
-        if (e == hipSuccess) e = hipStreamSynchronize(st);
-        if (e != hipSuccess) return fail_hip(e, "bbp position scan");
+        {
+            CallScratch scr(st);
+            uint64_t* scan = nullptr;
+            if ((rc = scr.get(&scan, 24))) return rc;
+            e = hipMemsetAsync(scan, 0, 24, st);
+            if (e == hipSuccess) e = launch_bbp_group_scan(d_positions, n, scan, st);
+            if (e == hipSuccess) e = hipMemcpyAsync(sc, scan, 24, hipMemcpyDeviceToHost, st);
+            if (e == hipSuccess) e = hipStreamSynchronize(st);
+            if (e != hipSuccess) return fail_hip(e, "bbp position scan");
+        }
         std::vector<uint8_t> tmp(n);
-        return bbp_few_locked(c, d_positions, nullptr, 0, n, sc[0], d_values, tmp.data(), st, false,
-                              sc[2] ? (double)sc[1] / (double)sc[2] : 0.0);
+        return bbp_few(cfg, d_positions, nullptr, 0, n, sc[0], d_values, tmp.data(), st, false,
+                       sc[2] ? (double)sc[1] / (double)sc[2] : 0.0);
     }
+    Ctx& c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
     e = launch_bbp(d_positions, 0, n, d_values, c.bbp_next, st);
     if (e != hipSuccess) return fail_hip(e, "bbp launch");
     return PDH_OK;
@@ -786,11 +966,7 @@ int pdh_bbp_set_layout(int mode) {
     return PDH_OK;
 }
 
-uint64_t pdh_bbp_exact_count(void) {
-    Ctx& c = ctx();
-    std::lock_guard<std::mutex> g(c.mu);
-    return c.bbp_exact_count;
-}
+uint64_t pdh_bbp_exact_count(void) { return ctx().bbp_exact_count.load(); }
 
 uint64_t pdh_launch_count(void) { return g_launches.load(); }
 
@@ -1290,8 +1466,10 @@ int pdh_sadd_dev(const uint32_t* a, const uint64_t* a_len, int32_t sa, const uin
     if (rc) return rc;
     if ((rc = grow(&c.sadd_status, &c.sadd_status_n, nt, 1))) return rc;
     const BsAddNode one{out, a, b, a_len, b_len, out_cap, out_len, out_sign, 0u, (uint32_t)nt, sa, sb};
+    if ((rc = scratch_acquire(c.sadd_order, (hipStream_t)stream))) return rc;
     hipError_t e = launch_bs_signed_add(nullptr, nullptr, (uint32_t)nt, c.sadd_status, (hipStream_t)stream, &one);
-    return e == hipSuccess ? PDH_OK : fail_hip(e, "signed sum launch");
+    if (e != hipSuccess) return fail_hip(e, "signed sum launch");
+    return scratch_release(c.sadd_order, (hipStream_t)stream);
 }
 
 int pdh_shift_dev(const uint32_t* a, const uint64_t* a_len, int64_t bits, uint32_t* out, uint64_t out_cap,
@@ -1337,6 +1515,11 @@ int pdh_dev_free(void* ptr) {
     cache_free(ptr);
     return PDH_OK;
 }
+int pdh_dev_free_async(void* ptr, void* stream) {
+    if (!ptr) return PDH_OK;
+    HIPCALL(cache_free_after(ptr, (hipStream_t)stream));
+    return PDH_OK;
+}
 int pdh_copy_htod(void* dst, const void* src, size_t bytes, void* stream) {
     if (bytes == 0) return PDH_OK;
     if (!dst || !src) return fail(PDH_E_ARG, "null pointer argument");
@@ -1367,6 +1550,12 @@ int pdh_stream_destroy(void* stream) {
 int pdh_stream_sync(void* stream) {
     HIPCALL(hipStreamSynchronize((hipStream_t)stream));
     return PDH_OK;
+}
+int pdh_stream_query(void* stream) {
+    const hipError_t e = hipStreamQuery((hipStream_t)stream);
+    if (e == hipSuccess) return 1;
+    if (e == hipErrorNotReady) { (void)hipGetLastError(); return 0; }
+    return fail_hip(e, "hipStreamQuery");
 }
 
 int pdh_tune_get(int n, int* L1, int* T, int* lr_col, int* lr_row) {

@@ -34,6 +34,7 @@
 #include "pdh_pi.h"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -51,6 +52,15 @@ typedef unsigned __int128 u128;
 
 struct SInt { Mag m; int s = 0; };            // sign s in {-1, 0, 1}, magnitude m
 struct PQT { Mag P, Q; SInt T; };
+
+// Test switch (pdh_newton_set_perturb): k != 0 moves every quotient and
+// square-root estimate by k units, and every decimal-level quotient estimate
+// by k (k > 0) or by -1 where it is at least 1 (k < 0), before the exact
+// correction, so the correction branches run on every call.
+static std::atomic<int> g_perturb{0};
+// Correction steps taken by the exact floor corrections (quotients, square
+// roots, decimal levels) in this process (pdh_newton_corrections).
+static std::atomic<uint64_t> g_corrections{0};
 
 struct Multiplier {
     std::function<Mag(const Mag&, const Mag&)> mul;
@@ -276,14 +286,23 @@ static void fix_quotient(A& o, const typename A::Num& a, const typename A::Num& 
         N deficit = o.sub(qd, a);                 // a - q d < 0
         for (;;) {
             q = o.dec(q);
+            ++g_corrections;
             if (o.cmp(deficit, d) <= 0) { r = o.sub(d, deficit); break; }
             deficit = o.sub(deficit, d);
         }
     } else {
         r = o.sub(a, qd);
-        while (o.cmp(r, d) >= 0) { r = o.sub(r, d); q = o.inc(q); }
+        while (o.cmp(r, d) >= 0) { r = o.sub(r, d); q = o.inc(q); ++g_corrections; }
     }
     if (rem) *rem = r;
+}
+
+// v moved by the test switch's k units (not below zero).
+template <class A> static typename A::Num perturbed(A& o, typename A::Num v) {
+    const int k = g_perturb.load();
+    for (int i = 0; i < k; ++i) v = o.inc(v);
+    for (int i = 0; i < -k && !o.zero(v); ++i) v = o.dec(v);
+    return v;
 }
 
 template <class A>
@@ -297,6 +316,7 @@ static typename A::Num div_floor(A& o, const typename A::Num& a, const typename 
     typename A::Num V = recip(o, dp, P);
     const size_t ta = na > k + P + 2 ? na - (k + P + 2) : 0;
     typename A::Num q = o.shr(o.mul(o.shr(a, ta), V), (size_t)((long)(2 * P) - sd - (long)ta));
+    q = perturbed(o, q);
     fix_quotient(o, a, d, o.mul(q, d), q, rem);
     return q;
 }
@@ -352,7 +372,7 @@ template <class A> static typename A::Num isqrt(A& o, const typename A::Num& s) 
     const size_t E = L + (L & 1);
     N Y = rsqrt_approx(o, o.shl(s, 2 * G), L + 2 * G);
     const size_t b = E / 2 - 2;                   // s_lo Y / 2^(E + G) < 1/2
-    N x = o.shr(o.mul(o.shr(s, b), Y), E + G - b);
+    N x = perturbed(o, o.shr(o.mul(o.shr(s, b), Y), E + G - b));
     N x2 = o.sqr(x);
     const int c = o.cmp(s, x2);
     SNum<A> r{c >= 0 ? o.sub(s, x2) : o.sub(x2, s), c};
@@ -360,12 +380,14 @@ template <class A> static typename A::Num isqrt(A& o, const typename A::Num& s) 
     while (r.s < 0) {                             // s - (x-1)^2 = r + 2x - 1
         r = sadd_t(o, r, SNum<A>{o.sub(o.shl(x, 1), one), 1});
         x = o.dec(x);
+        ++g_corrections;
     }
     for (;;) {                                    // s - (x+1)^2 = r - 2x - 1
         N two_x = o.shl(x, 1);
         if (r.s == 0 || o.cmp(r.m, two_x) <= 0) break;
         r = sadd_t(o, r, SNum<A>{o.add(two_x, one), -1});
         x = o.inc(x);
+        ++g_corrections;
     }
     return x;
 }
@@ -472,7 +494,12 @@ static std::string to_decimal(const Mag& x, const Multiplier& M) {
         for (size_t i = 0; i < nodes.size(); ++i) pairs.push_back({&xs[i], &V});
         std::vector<Mag> prods = M.mul_many(pairs);
         std::vector<Mag> qs(nodes.size());
-        for (size_t i = 0; i < nodes.size(); ++i) qs[i] = shr(prods[i], 2 * Ld + 32);
+        const int k = g_perturb.load();
+        for (size_t i = 0; i < nodes.size(); ++i) {
+            qs[i] = shr(prods[i], 2 * Ld + 32);
+            for (int j = 0; j < k; ++j) inc(qs[i]);
+            if (k < 0 && !qs[i].empty()) dec(qs[i]);
+        }
         pairs.clear();
         for (size_t i = 0; i < nodes.size(); ++i) pairs.push_back({&qs[i], &D});
         std::vector<Mag> qds = M.mul_many(pairs);
@@ -671,7 +698,7 @@ static std::vector<Mag> dev_mul_many(const std::vector<std::pair<const Mag*, con
         pa += a.size(); pb += b.size(); po += a.size() + b.size();
     }
     void *dA = nullptr, *dB = nullptr, *dO = nullptr, *dL = nullptr, *st = nullptr;
-    auto release = [&]() { pdh_dev_free(dA); pdh_dev_free(dB); pdh_dev_free(dO); pdh_dev_free(dL); };
+    auto release = [&]() { pdh_dev_free_async(dA, st); pdh_dev_free_async(dB, st); pdh_dev_free_async(dO, st); pdh_dev_free_async(dL, st); };
     try {
         st = work_stream();
         check(pdh_dev_alloc(la * 4, &dA));
@@ -712,8 +739,8 @@ static std::vector<PQT> dev_levels(uint64_t N) {
     const uint64_t groups = (N + (1u << G) - 1) >> G;
     const size_t cp = (size_t)3 << G, cq = (size_t)5 << G, ct = (size_t)6 << G;
     void *dP = nullptr, *dQ = nullptr, *dT = nullptr, *dL = nullptr, *dS = nullptr, *st = nullptr, *arena = nullptr;
-    auto release = [&]() { pdh_dev_free(dP); pdh_dev_free(dQ); pdh_dev_free(dT); pdh_dev_free(dL);
-                           pdh_dev_free(dS); pdh_dev_free(arena); };
+    auto release = [&]() { pdh_dev_free_async(dP, st); pdh_dev_free_async(dQ, st); pdh_dev_free_async(dT, st); pdh_dev_free_async(dL, st);
+                           pdh_dev_free_async(dS, st); pdh_dev_free_async(arena, st); };
     auto r4 = [](uint64_t x) { return (x + 3) & ~(uint64_t)3; };
     std::vector<PQT> out;
     try {
@@ -760,10 +787,10 @@ static std::vector<PQT> dev_levels(uint64_t N) {
                 nl[i].T = w; w += r4(cT[i]);
             }
             const int rc = pdh_bs_merge_dev(lvl.data(), lvl.size(), nl.data(), root ? 1 : 0, st);
-            if (rc != PDH_OK) { pdh_dev_free(next); check(rc); }
+            if (rc != PDH_OK) { pdh_dev_free_async(next, st); check(rc); }
             // the inputs are no longer needed: the groups' buffers or the previous arena
-            if (arena) { pdh_dev_free(arena); arena = nullptr; }
-            else { pdh_dev_free(dP); pdh_dev_free(dQ); pdh_dev_free(dT); dP = dQ = dT = nullptr; }
+            if (arena) { pdh_dev_free_async(arena, st); arena = nullptr; }
+            else { pdh_dev_free_async(dP, st); pdh_dev_free_async(dQ, st); pdh_dev_free_async(dT, st); dP = dQ = dT = nullptr; }
             arena = next;
             lvl.swap(nl);
         }
@@ -812,8 +839,7 @@ public:
         one_ = up(Mag{1});
     }
     ~DevArith() {
-        pdh_stream_sync(st_);
-        for (void* c : chunks_) pdh_dev_free(c);
+        for (void* c : chunks_) pdh_dev_free_async(c, st_);
     }
     DevArith(const DevArith&) = delete;
     DevArith& operator=(const DevArith&) = delete;
@@ -849,6 +875,24 @@ public:
         return ((u128)w[3] << 96) | ((u128)w[2] << 64) | ((u128)w[1] << 32) | w[0];
     }
     DNum from_u128(u128 v) { return up(chud::from_u128(v)); }
+    // A caller's device buffer of n limbs (the top ones may be zero) as a
+    // normalized value in this backend's memory.
+    DNum view(const uint32_t* p, uint64_t n) {
+        DNum x = alloc(1);
+        x.p = const_cast<uint32_t*>(p);
+        x.cap = x.ub = std::max<uint64_t>(n, 1);
+        check(pdh_copy_htod(x.len, &n, 8, nullptr));
+        DNum y = sadd(x, one_, 0, x.ub);
+        tighten(y);
+        return y;
+    }
+    // Writes a into a caller's device buffer of cap limbs (zero above its
+    // length) and its length into *len; PDH_E_CAPACITY if it does not fit.
+    void store(const DNum& a, uint32_t* out, uint64_t cap, uint64_t* len) {
+        if (length(a) > cap) throw HipError(PDH_E_CAPACITY, "ntt-hip: result longer than the output capacity");
+        check(pdh_shift_dev(a.p, a.len, 0, out, std::max<uint64_t>(cap, 1), len, st_));
+        sync();
+    }
     DNum shl(const DNum& a, size_t k) {
         DNum x = alloc(a.ub + k / 32 + 1);
         check(pdh_shift_dev(a.p, a.len, (int64_t)k, x.p, x.cap, x.len, st_));
@@ -926,6 +970,30 @@ private:
     DNum one_;
 };
 
+// The test switch on a decimal level's quotient estimates (even slots of Xn,
+// stride 2 Cn, device lengths XnL): + k each for k > 0, - 1 on those that are
+// at least 1 for k < 0. The fix-up on the host then corrects them.
+static void perturb_quotients(DevArith& o, uint32_t* Xn, uint64_t* XnL, uint64_t n, uint64_t Cn, void* st) {
+    const int k = g_perturb.load();
+    std::vector<uint64_t> ln(2 * n);
+    check(pdh_copy_dtoh(ln.data(), XnL, 2 * n * 8, st));
+    check(pdh_stream_sync(st));
+    const DNum kk = o.up(Mag{(uint32_t)(k > 0 ? k : 1)});
+    std::vector<int32_t> sg(n);
+    void* dsg = nullptr;
+    check(pdh_dev_alloc(n * 4, &dsg));
+    std::vector<pdh_sadd_desc> ad;
+    for (uint64_t i = 0; i < n; ++i) {
+        if (k < 0 && ln[2 * i] == 0) continue;
+        uint32_t* q = Xn + 2 * i * Cn;
+        ad.push_back(pdh_sadd_desc{q, XnL + 2 * i, 1, kk.p, kk.len, k > 0 ? 1 : -1, q, Cn, XnL + 2 * i,
+                                   (int32_t*)dsg + i});
+    }
+    const int rc = ad.empty() ? PDH_OK : pdh_sadd_batch_dev(ad.data(), ad.size(), st);
+    pdh_dev_free_async(dsg, st);
+    check(rc);
+}
+
 // Divide-and-conquer decimal conversion in device memory. Level l holds 2^l
 // nodes of width W_l = B 2^(K-l) digits at a fixed stride C_l limbs (values
 // below 10^W_l, zero-padded, device lengths). Per level, with D = 10^(W_l/2)
@@ -944,7 +1012,7 @@ static std::string dev_to_decimal(const Mag& x, const Multiplier&) {
     auto cap_of = [](uint64_t w) { return (uint64_t)((double)w * 3.3219280948873623 / 32.0) + 3; };
     auto r64 = [](uint64_t v) { return (v + 63) & ~(uint64_t)63; };
     void *cur = nullptr, *nxt = nullptr, *aux = nullptr, *dout = nullptr;
-    auto release = [&]() { pdh_dev_free(cur); pdh_dev_free(nxt); pdh_dev_free(aux); pdh_dev_free(dout); };
+    auto release = [&]() { pdh_dev_free_async(cur, st); pdh_dev_free_async(nxt, st); pdh_dev_free_async(aux, st); pdh_dev_free_async(dout, st); };
     std::string out(p.W, '0');
     try {
         // a level: n nodes x C limbs, then n lengths
@@ -995,6 +1063,7 @@ static std::string dev_to_decimal(const Mag& x, const Multiplier&) {
             for (uint64_t i = 0; i < n; ++i)
                 sd[i] = pdh_shr_desc{dP + i * (C + nV), PL + i, Xn + 2 * i * Cn, Cn, XnL + 2 * i, sh};
             check(pdh_shr_batch_dev(sd.data(), n, st));
+            if (g_perturb.load() != 0) perturb_quotients(o, Xn, XnL, n, Cn, st);
             for (uint64_t i = 0; i < n; ++i)
                 md[i] = pdh_mul_desc{Xn + 2 * i * Cn, Cn, dD, nD, dQD + i * (Cn + nD), QDL + i};
             products(md);
@@ -1025,8 +1094,8 @@ static std::string dev_to_decimal(const Mag& x, const Multiplier&) {
                 check(pdh_stream_sync(st));
                 const Mag Dh = o.down(D);
                 SInt r{rm, sg[i]};
-                while (r.s < 0) { r = sadd(r, SInt{Dh, 1}); dec(q); }
-                while (r.s > 0 && cmp(r.m, Dh) >= 0) { r = sadd(r, SInt{Dh, -1}); inc(q); }
+                while (r.s < 0) { r = sadd(r, SInt{Dh, 1}); dec(q); ++g_corrections; }
+                while (r.s > 0 && cmp(r.m, Dh) >= 0) { r = sadd(r, SInt{Dh, -1}); inc(q); ++g_corrections; }
                 Mag qz(Cn, 0), rz(Cn, 0);
                 std::copy(q.begin(), q.end(), qz.begin());
                 std::copy(r.m.begin(), r.m.end(), rz.begin());
@@ -1035,8 +1104,8 @@ static std::string dev_to_decimal(const Mag& x, const Multiplier&) {
                 check(pdh_copy_htod(Xn + (2 * i + 1) * Cn, rz.data(), Cn * 4, nullptr));
                 check(pdh_copy_htod(XnL + 2 * i, lens, 16, nullptr));
             }
-            pdh_dev_free(aux); aux = nullptr;
-            pdh_dev_free(cur);
+            pdh_dev_free_async(aux, st); aux = nullptr;
+            pdh_dev_free_async(cur, st);
             cur = nxt; nxt = nullptr;
             n *= 2;
             C = Cn;
@@ -1078,6 +1147,97 @@ std::string compute_pi_chudnovsky_hip(uint64_t digits, double* phase_ms) {
 }
 
 } // namespace pidigits
+
+// ---- exact floor division and square root on the seam ------------------------
+namespace {
+using pidigits::chud::Mag;
+
+Mag host_mag(const uint32_t* p, size_t n) {
+    Mag m(p, p + n);
+    while (!m.empty() && m.back() == 0) m.pop_back();
+    return m;
+}
+
+int put(const Mag& v, uint32_t* out, size_t cap, size_t* len) {
+    if (len) *len = v.size();
+    if (!out) return PDH_OK;                          // only the length (or nothing) wanted
+    if (v.size() > cap) return PDH_E_CAPACITY;
+    if (!v.empty()) std::memcpy(out, v.data(), v.size() * 4);
+    return PDH_OK;
+}
+
+pidigits::chud::Multiplier callback_multiplier(pdh_host_mul_fn mul) {
+    using namespace pidigits::chud;
+    Multiplier M;
+    M.mul = [mul](const Mag& a, const Mag& b) -> Mag {
+        if (a.empty() || b.empty()) return {};
+        Mag r(a.size() + b.size());
+        size_t n = 0;
+        if (mul(a.data(), a.size(), b.data(), b.size(), r.data(), r.size(), &n) != 0)
+            throw pidigits::HipError(PDH_E_HIP, "callback multiply failed");
+        r.resize(n);
+        return r;
+    };
+    M.sqr = [M](const Mag& a) { return M.mul(a, a); };
+    M.mul_many = [M](const std::vector<std::pair<const Mag*, const Mag*>>& p) {
+        std::vector<Mag> r(p.size());
+        for (size_t i = 0; i < p.size(); ++i) r[i] = M.mul(*p[i].first, *p[i].second);
+        return r;
+    };
+    return M;
+}
+
+// q, r of a / d (d != 0) and floor(sqrt(s)) on backend o.
+template <class A> void divmod_on(A& o, const Mag& a, const Mag& d, Mag* q, Mag* r) {
+    typename A::Num rem;
+    typename A::Num qq = pidigits::chud::div_floor(o, o.up(a), o.up(d), &rem);
+    *q = o.down(qq);
+    *r = o.down(rem);
+}
+
+int divmod_any(pdh_host_mul_fn mul, const uint32_t* a, size_t na, const uint32_t* d, size_t nd, uint32_t* q,
+               size_t q_cap, size_t* q_len, uint32_t* r, size_t r_cap, size_t* r_len) {
+    if ((na && !a) || (nd && !d)) return PDH_E_ARG;
+    const Mag A = host_mag(a, na), D = host_mag(d, nd);
+    if (D.empty()) return PDH_E_ARG;                  // division by zero
+    Mag Q, R;
+    try {
+        if (mul) {
+            const pidigits::chud::Multiplier M = callback_multiplier(mul);
+            pidigits::chud::HostArith o{M};
+            divmod_on(o, A, D, &Q, &R);
+        } else {
+            pidigits::chud::DevArith o(pidigits::chud::work_stream());
+            divmod_on(o, A, D, &Q, &R);
+        }
+    } catch (const pidigits::HipError& e) {
+        return e.code;
+    }
+    const int rc = put(Q, q, q_cap, q_len);
+    return rc ? rc : put(R, r, r_cap, r_len);
+}
+
+int isqrt_any(pdh_host_mul_fn mul, const uint32_t* s, size_t ns, uint32_t* x, size_t x_cap, size_t* x_len) {
+    if (ns && !s) return PDH_E_ARG;
+    const Mag S = host_mag(s, ns);
+    Mag X;
+    try {
+        if (S.empty()) {
+            X.clear();
+        } else if (mul) {
+            const pidigits::chud::Multiplier M = callback_multiplier(mul);
+            pidigits::chud::HostArith o{M};
+            X = o.down(pidigits::chud::isqrt(o, o.up(S)));
+        } else {
+            pidigits::chud::DevArith o(pidigits::chud::work_stream());
+            X = o.down(pidigits::chud::isqrt(o, o.up(S)));
+        }
+    } catch (const pidigits::HipError& e) {
+        return e.code;
+    }
+    return put(X, x, x_cap, x_len);
+}
+} // namespace
 
 // ---- C ABI ------------------------------------------------------------------------------
 extern "C" {
@@ -1121,5 +1281,65 @@ int pdh_pi_chudnovsky_cb(uint64_t digits, pdh_host_mul_fn mul, char* out, size_t
 }
 
 uint64_t pdh_chudnovsky_terms(uint64_t digits) { return pidigits::chud::terms_for_digits(digits + 16); }
+
+
+int pdh_divmod(const uint32_t* a, size_t na, const uint32_t* d, size_t nd, uint32_t* q, size_t q_cap,
+               size_t* q_len, uint32_t* r, size_t r_cap, size_t* r_len) {
+    return divmod_any(nullptr, a, na, d, nd, q, q_cap, q_len, r, r_cap, r_len);
+}
+
+int pdh_divmod_cb(pdh_host_mul_fn mul, const uint32_t* a, size_t na, const uint32_t* d, size_t nd, uint32_t* q,
+                  size_t q_cap, size_t* q_len, uint32_t* r, size_t r_cap, size_t* r_len) {
+    if (!mul) return PDH_E_ARG;
+    return divmod_any(mul, a, na, d, nd, q, q_cap, q_len, r, r_cap, r_len);
+}
+
+int pdh_isqrt(const uint32_t* s, size_t ns, uint32_t* x, size_t x_cap, size_t* x_len) {
+    return isqrt_any(nullptr, s, ns, x, x_cap, x_len);
+}
+
+int pdh_isqrt_cb(pdh_host_mul_fn mul, const uint32_t* s, size_t ns, uint32_t* x, size_t x_cap, size_t* x_len) {
+    if (!mul) return PDH_E_ARG;
+    return isqrt_any(mul, s, ns, x, x_cap, x_len);
+}
+
+int pdh_divmod_dev(const uint32_t* d_a, size_t na, const uint32_t* d_d, size_t nd, uint32_t* d_q, size_t q_cap,
+                   uint64_t* d_q_len, uint32_t* d_r, size_t r_cap, uint64_t* d_r_len, void* stream) {
+    if ((na && !d_a) || !d_d || nd == 0 || (d_q && !d_q_len) || (d_r && !d_r_len)) return PDH_E_ARG;
+    using namespace pidigits::chud;
+    try {
+        DevArith o(stream);
+        const DNum A = o.view(d_a, na), D = o.view(d_d, nd);
+        if (o.zero(D)) return PDH_E_ARG;
+        DNum R;
+        const DNum Q = div_floor(o, A, D, &R);
+        if (d_q) o.store(Q, d_q, q_cap, d_q_len);
+        if (d_r) o.store(R, d_r, r_cap, d_r_len);
+    } catch (const pidigits::HipError& e) {
+        return e.code;
+    }
+    return PDH_OK;
+}
+
+int pdh_isqrt_dev(const uint32_t* d_s, size_t ns, uint32_t* d_x, size_t x_cap, uint64_t* d_x_len, void* stream) {
+    if ((ns && !d_s) || !d_x || !d_x_len) return PDH_E_ARG;
+    using namespace pidigits::chud;
+    try {
+        DevArith o(stream);
+        const DNum S = o.view(d_s, ns);
+        o.store(o.zero(S) ? S : isqrt(o, S), d_x, x_cap, d_x_len);
+    } catch (const pidigits::HipError& e) {
+        return e.code;
+    }
+    return PDH_OK;
+}
+
+int pdh_newton_set_perturb(int k) {
+    if (k < -8 || k > 8) return PDH_E_ARG;
+    pidigits::chud::g_perturb.store(k);
+    return PDH_OK;
+}
+
+uint64_t pdh_newton_corrections(void) { return pidigits::chud::g_corrections.load(); }
 
 } // extern "C"

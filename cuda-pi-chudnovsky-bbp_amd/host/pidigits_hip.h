@@ -7,8 +7,13 @@
 //    hip_available        <-> cuda_available (include/bignum.h:205)
 //    bbp_hex_digits_hip   <-> bbp_hex_digits_cuda (include/bbp.h:34)
 //    bbp_hex_digit_hip    <-> bbp_hex_digit  (include/bbp.h:27)
-//  and MulBackend gains NttHip (include/bignum.h:77-83). The reference's
-//  bignum.cpp dispatcher routes `case MulBackend::NttHip` here.
+//  None of these names is declared by the reference's headers, so this
+//  header can be included next to its bignum.h, bbp.h and chudnovsky.h: a
+//  reference build adds NttHip to its own MulBackend (include/bignum.h:77-83)
+//  and routes `case MulBackend::NttHip` in mul_dispatch here. Alternatively
+//  seam_cuda_names.cpp defines the reference's own mul_ntt_cuda,
+//  cuda_available and bbp_hex_digits_cuda on these functions, so the
+//  reference's sources link against this library unchanged.
 //
 //  Error behaviour differs on purpose: the reference falls back to its CPU
 //  path on any device error (src/ntt_cuda.cu:182,241; src/bbp_cuda.cu:42,64).
@@ -22,8 +27,6 @@
 #include <vector>
 
 namespace pidigits {
-
-enum class MulBackend { Auto, Schoolbook, Karatsuba, NttCpu, NttCuda, NttHip };
 
 struct HipError : std::runtime_error {
     int code;

@@ -40,6 +40,7 @@ EXPORTED = (
     "pdh_copy_dtoh", "pdh_stream_create", "pdh_stream_destroy", "pdh_stream_sync",
     "pdh_bbp_set_exact", "pdh_bbp_set_layout", "pdh_bbp_exact_count", "pdh_tune_engine", "pdh_bs_groups", "pdh_bs_merge_dev",
     "pdh_sadd_batch_dev", "pdh_shr_batch_dev", "pdh_decimal_leaves_dev", "pdh_sadd_dev", "pdh_shift_dev",
+    "pdh_dev_free_async", "pdh_stream_query",
 )
 
 
@@ -141,6 +142,18 @@ def lib() -> ctypes.CDLL:
         L.pdh_autotune.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp]
         L.pdh_tune_engine.argtypes = [ctypes.c_int, ctypes.c_int, _ip]
         for name in ("pdh_tune_get", "pdh_tune_set", "pdh_autotune", "pdh_tune_engine"):
+            getattr(L, name).restype = ctypes.c_int
+        L.pdh_dev_alloc.argtypes = [_sz, ctypes.POINTER(_vp)]
+        L.pdh_dev_free.argtypes = [_vp]
+        L.pdh_dev_free_async.argtypes = [_vp, _vp]
+        L.pdh_copy_htod.argtypes = [_vp, _vp, _sz, _vp]
+        L.pdh_copy_dtoh.argtypes = [_vp, _vp, _sz, _vp]
+        L.pdh_stream_create.argtypes = [ctypes.POINTER(_vp)]
+        L.pdh_stream_destroy.argtypes = [_vp]
+        L.pdh_stream_sync.argtypes = [_vp]
+        L.pdh_stream_query.argtypes = [_vp]
+        for name in ("pdh_dev_alloc", "pdh_dev_free", "pdh_dev_free_async", "pdh_copy_htod", "pdh_copy_dtoh",
+                     "pdh_stream_create", "pdh_stream_destroy", "pdh_stream_sync", "pdh_stream_query"):
             getattr(L, name).restype = ctypes.c_int
         L.pdh_batch_create.argtypes = [ctypes.POINTER(MulDesc), _sz, ctypes.POINTER(_vp)]
         L.pdh_batch_run.argtypes = [_vp, _vp]
@@ -389,7 +402,9 @@ def bbp_hex_digit(position: int) -> str:
 PI_LIB_PATH = os.path.join(HERE, "libpidigits_hip.so")
 PI_EXPORTED = ("pdh_pi_chudnovsky", "pdh_pi_chudnovsky_cb", "pdh_chudnovsky_terms", "pdh_server_create",
                "pdh_server_submit_mul", "pdh_server_submit_pi", "pdh_server_submit_bbp", "pdh_server_wait",
-               "pdh_server_result", "pdh_server_destroy")
+               "pdh_server_result", "pdh_server_destroy", "pdh_divmod", "pdh_isqrt", "pdh_divmod_dev",
+               "pdh_isqrt_dev", "pdh_divmod_cb", "pdh_isqrt_cb", "pdh_newton_set_perturb",
+               "pdh_newton_corrections")
 _pilib = None
 
 
@@ -410,11 +425,67 @@ def pilib() -> ctypes.CDLL:
         L.pdh_server_wait.argtypes = [_vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double)]
         L.pdh_server_result.argtypes = [_vp, ctypes.c_uint64, _vp, _sz, ctypes.POINTER(_sz)]
         L.pdh_server_destroy.argtypes = [_vp]
+        _szp = ctypes.POINTER(_sz)
+        L.pdh_divmod.argtypes = [_u32p, _sz, _u32p, _sz, _u32p, _sz, _szp, _u32p, _sz, _szp]
+        L.pdh_isqrt.argtypes = [_u32p, _sz, _u32p, _sz, _szp]
+        L.pdh_divmod_cb.argtypes = [_vp, _u32p, _sz, _u32p, _sz, _u32p, _sz, _szp, _u32p, _sz, _szp]
+        L.pdh_isqrt_cb.argtypes = [_vp, _u32p, _sz, _u32p, _sz, _szp]
+        L.pdh_divmod_dev.argtypes = [_vp, _sz, _vp, _sz, _vp, _sz, _vp, _vp, _sz, _vp, _vp]
+        L.pdh_isqrt_dev.argtypes = [_vp, _sz, _vp, _sz, _vp, _vp]
+        L.pdh_newton_set_perturb.argtypes = [ctypes.c_int]
+        L.pdh_newton_corrections.restype = ctypes.c_uint64
         for name in ("pdh_pi_chudnovsky", "pdh_server_create", "pdh_server_submit_mul", "pdh_server_submit_pi",
-                     "pdh_server_submit_bbp", "pdh_server_wait", "pdh_server_result", "pdh_server_destroy"):
+                     "pdh_server_submit_bbp", "pdh_server_wait", "pdh_server_result", "pdh_server_destroy",
+                     "pdh_divmod", "pdh_isqrt", "pdh_divmod_cb", "pdh_isqrt_cb", "pdh_divmod_dev", "pdh_isqrt_dev",
+                     "pdh_newton_set_perturb"):
             getattr(L, name).restype = ctypes.c_int
         _pilib = L
     return _pilib
+
+
+def int_to_limbs(x: int) -> np.ndarray:
+    """Little-endian uint32 limbs of a non-negative Python int (empty for 0)."""
+    n = (x.bit_length() + 31) // 32
+    return np.frombuffer(x.to_bytes(4 * n, "little"), dtype=np.uint32).copy() if n else np.zeros(0, np.uint32)
+
+
+def limbs_to_int(a) -> int:
+    return int.from_bytes(np.ascontiguousarray(np.asarray(a, dtype=np.uint32)).tobytes(), "little")
+
+
+def divmod_hip(a, d, mul_fn=None):
+    """(floor(a / d), a mod d) of limb arrays, by the Newton division on the
+    device (pdh_divmod), or with every multiply done by the C function
+    pointer ``mul_fn`` (pdh_divmod_cb)."""
+    a, d = _limbs(a), _limbs(d)
+    q = np.zeros(max(len(a) + 1, 1), np.uint32)
+    r = np.zeros(max(len(d), 1), np.uint32)
+    ql, rl = _sz(), _sz()
+    args = (_ptr(a, ctypes.c_uint32), len(a), _ptr(d, ctypes.c_uint32), len(d), _ptr(q, ctypes.c_uint32), len(q),
+            ctypes.byref(ql), _ptr(r, ctypes.c_uint32), len(r), ctypes.byref(rl))
+    rc = pilib().pdh_divmod(*args) if mul_fn is None else pilib().pdh_divmod_cb(mul_fn, *args)
+    _check(rc)
+    return q[: ql.value].copy(), r[: rl.value].copy()
+
+
+def isqrt_hip(s, mul_fn=None):
+    """floor(sqrt(s)) of a limb array (pdh_isqrt, or pdh_isqrt_cb with ``mul_fn``)."""
+    s = _limbs(s)
+    x = np.zeros(max(len(s) // 2 + 2, 1), np.uint32)
+    xl = _sz()
+    args = (_ptr(s, ctypes.c_uint32), len(s), _ptr(x, ctypes.c_uint32), len(x), ctypes.byref(xl))
+    rc = pilib().pdh_isqrt(*args) if mul_fn is None else pilib().pdh_isqrt_cb(mul_fn, *args)
+    _check(rc)
+    return x[: xl.value].copy()
+
+
+def newton_set_perturb(k: int) -> None:
+    """Test switch: move every Newton estimate by k units before its exact correction."""
+    _check(pilib().pdh_newton_set_perturb(int(k)))
+
+
+def newton_corrections() -> int:
+    return int(pilib().pdh_newton_corrections())
 
 
 def pi_chudnovsky(digits: int):

@@ -61,7 +61,9 @@ int pdh_sqr(const uint32_t* a, size_t na, uint32_t* out, size_t out_cap, size_t*
 
 /* Device-resident multiply: d_a, d_b, d_out are device pointers; d_out holds
  * na + nb limbs (the top limb may be zero); *d_out_len (device uint64_t)
- * receives the normalized length. Enqueued on `stream`, asynchronous.
+ * receives the normalized length. Enqueued on `stream`, asynchronous. Calls
+ * on different streams (from any threads) share the library's workspace in
+ * turn: each waits on the device, not on the host, for the previous user.
  * Same product as pdh_mul; the operands stay in HBM (no PCIe traffic).
  * Replaces: the device part of mul_ntt_cuda (reference src/ntt_cuda.cu:
  * 198-233) without its per-call cudaMalloc/H2D/D2H/host carry. */
@@ -74,14 +76,21 @@ int pdh_sqr_dev(const uint32_t* d_a, size_t na, uint32_t* d_out, uint64_t* d_out
 
 /* ---- device memory and streams for C/C++ callers ----------------------------
  * Thin wrappers so host code that builds batches or keeps operands resident
- * never includes HIP headers. Copies with a stream are asynchronous. */
+ * never includes HIP headers. Copies with a stream are asynchronous.
+ * pdh_dev_free waits for the device, as hipFree does; pdh_dev_free_async
+ * returns at once and the block is reused only after the work enqueued on
+ * `stream` up to the call has run (the block's last use must be on that
+ * stream, or ordered before it). pdh_stream_query: 1 if every operation on
+ * the stream has finished, 0 if some are still running, < 0 on error. */
 int pdh_dev_alloc(size_t bytes, void** ptr);
 int pdh_dev_free(void* ptr);
+int pdh_dev_free_async(void* ptr, void* stream);
 int pdh_copy_htod(void* dst, const void* src, size_t bytes, void* stream);
 int pdh_copy_dtoh(void* dst, const void* src, size_t bytes, void* stream);
 int pdh_stream_create(void** stream);
 int pdh_stream_destroy(void* stream);
 int pdh_stream_sync(void* stream);
+int pdh_stream_query(void* stream);
 
 /* ---- batched multiplies ----------------------------------------------------
  * Many independent multiplies of uneven sizes (e.g. one level of the
@@ -247,8 +256,8 @@ int pdh_shr_batch_dev(const pdh_shr_desc* descs, size_t n, void* stream);
  * no host synchronization (lengths stay on the device), for Newton
  * iterations kept in HBM. pdh_sadd_dev: out = sa*a + sb*b as in
  * pdh_sadd_batch_dev (out may be a; its tile statuses live in the library's
- * scratch, so like pdh_mul_dev these calls must not overlap on the device
- * from different streams). pdh_shift_dev: out = a * 2^bits for bits >= 0,
+ * scratch, which calls on different streams take in turn, as pdh_mul_dev's
+ * workspace). pdh_shift_dev: out = a * 2^bits for bits >= 0,
  * floor(a / 2^-bits) for bits < 0; limbs below out_cap (at least 1) are
  * written and the result must fit; *out_len receives its normalized length.
  * No counterpart in the reference (its BigInt shifts and adds are host

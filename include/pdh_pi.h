@@ -28,6 +28,45 @@ int pdh_pi_chudnovsky(uint64_t digits, char* out, size_t cap, double* phase_ms);
 typedef int (*pdh_host_mul_fn)(const uint32_t* a, size_t na, const uint32_t* b, size_t nb,
                                uint32_t* out, size_t out_cap, size_t* out_len);
 int pdh_pi_chudnovsky_cb(uint64_t digits, pdh_host_mul_fn mul, char* out, size_t cap, double* phase_ms);
+int pdh_divmod_cb(pdh_host_mul_fn mul, const uint32_t* a, size_t na, const uint32_t* d, size_t nd, uint32_t* q,
+                  size_t q_cap, size_t* q_len, uint32_t* r, size_t r_cap, size_t* r_len);
+int pdh_isqrt_cb(pdh_host_mul_fn mul, const uint32_t* s, size_t ns, uint32_t* x, size_t x_cap, size_t* x_len);
+
+/* ---- exact floor division and square root -------------------------------
+ * The reference's div_fast and isqrt slots (src/bignum_div.cpp:170-178 and
+ * :189-204; include/bignum.h:220,227): q = floor(a / d), r = a - q d for
+ * d > 0, and floor(sqrt(s)), on little-endian base-2^32 magnitudes (top
+ * limbs may be zero; results normalized). Computed by Newton iterations
+ * whose multiplies run on the device (a reciprocal with exact floor
+ * correction; a division-free inverse square root, then x -> floor root by
+ * r = s - x^2), so the cost is a few multiplies, not the reference's
+ * quadratic Knuth division. PDH_E_ARG for d = 0; PDH_E_CAPACITY if a result
+ * is longer than its capacity (its length is still written). A null q or r
+ * skips that output.
+ *
+ * pdh_divmod, pdh_isqrt: host buffers. pdh_divmod_dev, pdh_isqrt_dev: device
+ * buffers on `stream` (inputs of na, nd, ns limbs; outputs zero-padded to
+ * their capacity, normalized lengths in device uint64_t words); the Newton
+ * steps read lengths and signs back, so these synchronise the stream.
+ * pdh_divmod_cb, pdh_isqrt_cb: the same algorithm with every multiply done
+ * by `mul` on host buffers (the CPU tests run it with the oracle's
+ * multiply). */
+int pdh_divmod(const uint32_t* a, size_t na, const uint32_t* d, size_t nd, uint32_t* q, size_t q_cap,
+               size_t* q_len, uint32_t* r, size_t r_cap, size_t* r_len);
+int pdh_isqrt(const uint32_t* s, size_t ns, uint32_t* x, size_t x_cap, size_t* x_len);
+int pdh_divmod_dev(const uint32_t* d_a, size_t na, const uint32_t* d_d, size_t nd, uint32_t* d_q, size_t q_cap,
+                   uint64_t* d_q_len, uint32_t* d_r, size_t r_cap, uint64_t* d_r_len, void* stream);
+int pdh_isqrt_dev(const uint32_t* d_s, size_t ns, uint32_t* d_x, size_t x_cap, uint64_t* d_x_len, void* stream);
+
+/* Test switch (no counterpart in the reference): k != 0 (|k| <= 8) moves
+ * every Newton quotient and square-root estimate by k units, and every
+ * decimal-conversion quotient estimate by k (k > 0) or by -1 where it is at
+ * least 1 (k < 0), before the exact correction, so that the correction
+ * branches run; results are unchanged. 0 (default) turns it off. */
+int pdh_newton_set_perturb(int k);
+/* Correction steps the exact floor corrections have taken in this process
+ * (quotient, square-root and decimal-level steps together). */
+uint64_t pdh_newton_corrections(void);
 
 /* Number of series terms for `digits` digits (16 guard digits included),
  * as the reference's chudnovsky_terms_for_digits(digits + 16)

@@ -92,3 +92,49 @@ def test_bs_merge_argument_errors_before_device():
     outs[0].T = None
     assert L.pdh_bs_merge_dev(ins, 2, outs, 0, None) == pdh.PDH_E_ARG          # no T buffer
     assert pdh.launch_count() == 0
+
+
+REFERENCE_DIR = "/root/reference"   # read only where the checkout exists (this container)
+
+
+def test_seam_library_exports_the_reference_symbols():
+    """libpdh_seam.so defines the reference's mul_ntt_cuda, cuda_available and
+    bbp_hex_digits_cuda (include/bignum.h:200-205, include/bbp.h:34) with the
+    reference's C++ signatures."""
+    import subprocess
+    from conftest import PKG
+    so = os.path.join(PKG, "libpdh_seam.so")
+    out = subprocess.run(["nm", "-DC", "--defined-only", so], capture_output=True, text=True, check=True).stdout
+    for sig in ("pidigits::mul_ntt_cuda(std::vector<unsigned int, std::allocator<unsigned int> > const&, "
+                "std::vector<unsigned int, std::allocator<unsigned int> > const&)",
+                "pidigits::cuda_available()",
+                "pidigits::bbp_hex_digits_cuda[abi:cxx11](unsigned long, unsigned int)"):
+        assert sig in out, sig
+
+
+@pytest.mark.skipif(not os.path.isdir(REFERENCE_DIR), reason="reference checkout not present")
+def test_mirror_header_compiles_beside_reference_headers(tmp_path):
+    """A reference translation unit can include the C++ mirror next to its own
+    bignum.h, bbp.h and chudnovsky.h (no redefinition) and catch HipError."""
+    import subprocess
+    from conftest import PKG
+    src = tmp_path / "tu.cpp"
+    src.write_text('#include "bignum.h"\n#include "bbp.h"\n#include "chudnovsky.h"\n#include "pidigits_hip.h"\n'
+                   "int f() { try { return (int)pidigits::mul_ntt_hip({1u}, {2u}).size(); }\n"
+                   "          catch (const pidigits::HipError& e) { return e.code; } }\n"
+                   "bool g() { return pidigits::hip_available() || pidigits::cuda_available(); }\n")
+    subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", os.path.join(REFERENCE_DIR, "include"),
+                    "-I", os.path.join(PKG, "host"), str(src)], check=True)
+
+
+@pytest.mark.skipif(not os.path.isdir(REFERENCE_DIR), reason="reference checkout not present")
+def test_reference_programs_link_against_the_seam():
+    """oracle/Makefile's seam target built the reference's unchanged
+    self-test and CLI against libpdh_seam.so (run on the GPU by
+    tests/test_gpu_seam.py)."""
+    import subprocess
+    for exe in ("ref_selftest_seam", "ref_pi_seam"):
+        path = os.path.join(ROOT, "oracle", "_ref", exe)
+        assert os.path.exists(path), exe
+        ldd = subprocess.run(["ldd", path], capture_output=True, text=True, check=True).stdout
+        assert "libpdh_seam.so" in ldd and "libpdh.so" in ldd and "not found" not in ldd
