@@ -1402,6 +1402,8 @@ int pdh_sadd_batch_dev(const pdh_sadd_desc* d, size_t n, void* stream) {
         if (x.sa < -1 || x.sa > 1 || x.sb < -1 || x.sb > 1 || (x.sa && (!x.a || !x.a_len)) ||
             (x.sb && (!x.b || !x.b_len)) || !x.out || x.out_cap == 0 || !x.out_len || !x.out_sign)
             return fail(PDH_E_ARG, "pdh_sadd_batch_dev: bad descriptor");
+        if ((x.sa && x.out_len == x.a_len) || (x.sb && x.out_len == x.b_len))
+            return fail(PDH_E_ARG, "pdh_sadd_batch_dev: out_len must not be an operand's length word");
         ntiles += (x.out_cap + BS_ADD_TILE - 1) / BS_ADD_TILE;
     }
     if (ntiles > 0xFFFFFFFFull) return fail(PDH_E_TOO_LARGE, "pdh_sadd_batch_dev: too many tiles");
@@ -1458,6 +1460,8 @@ int pdh_sadd_dev(const uint32_t* a, const uint64_t* a_len, int32_t sa, const uin
     if (sa < -1 || sa > 1 || sb < -1 || sb > 1 || (sa && (!a || !a_len)) || (sb && (!b || !b_len)) || !out ||
         out_cap == 0 || !out_len || !out_sign)
         return fail(PDH_E_ARG, "pdh_sadd_dev: bad argument");
+    if ((sa && out_len == a_len) || (sb && out_len == b_len))
+        return fail(PDH_E_ARG, "pdh_sadd_dev: out_len must not be an operand's length word");
     const uint64_t nt = (out_cap + BS_ADD_TILE - 1) / BS_ADD_TILE;
     if (nt > 0xFFFFFFFFull) return fail(PDH_E_TOO_LARGE, "pdh_sadd_dev: too many tiles");
     Ctx& c = ctx();

@@ -61,6 +61,15 @@ static std::atomic<int> g_perturb{0};
 // Correction steps taken by the exact floor corrections (quotients, square
 // roots, decimal levels) in this process (pdh_newton_corrections).
 static std::atomic<uint64_t> g_corrections{0};
+// The estimates are within a few units (the test switch adds at most 8), so
+// a correction that needs more steps than this means a wrong estimate: it
+// is reported instead of being walked off one unit at a time.
+static const int kMaxCorrections = 64;
+static void count_correction(int& steps) {
+    ++g_corrections;
+    if (++steps > kMaxCorrections)
+        throw HipError(PDH_E_HIP, "ntt-hip: a Newton estimate was off by more than 64 units");
+}
 
 struct Multiplier {
     std::function<Mag(const Mag&, const Mag&)> mul;
@@ -282,17 +291,18 @@ static void fix_quotient(A& o, const typename A::Num& a, const typename A::Num& 
                          typename A::Num& q, typename A::Num* rem) {
     typedef typename A::Num N;
     N r;
+    int steps = 0;
     if (o.cmp(qd, a) > 0) {
         N deficit = o.sub(qd, a);                 // a - q d < 0
         for (;;) {
             q = o.dec(q);
-            ++g_corrections;
+            count_correction(steps);
             if (o.cmp(deficit, d) <= 0) { r = o.sub(d, deficit); break; }
             deficit = o.sub(deficit, d);
         }
     } else {
         r = o.sub(a, qd);
-        while (o.cmp(r, d) >= 0) { r = o.sub(r, d); q = o.inc(q); ++g_corrections; }
+        while (o.cmp(r, d) >= 0) { r = o.sub(r, d); q = o.inc(q); count_correction(steps); }
     }
     if (rem) *rem = r;
 }
@@ -377,17 +387,18 @@ template <class A> static typename A::Num isqrt(A& o, const typename A::Num& s) 
     const int c = o.cmp(s, x2);
     SNum<A> r{c >= 0 ? o.sub(s, x2) : o.sub(x2, s), c};
     const N one = o.from_u128(1);
+    int steps = 0;
     while (r.s < 0) {                             // s - (x-1)^2 = r + 2x - 1
         r = sadd_t(o, r, SNum<A>{o.sub(o.shl(x, 1), one), 1});
         x = o.dec(x);
-        ++g_corrections;
+        count_correction(steps);
     }
     for (;;) {                                    // s - (x+1)^2 = r - 2x - 1
         N two_x = o.shl(x, 1);
         if (r.s == 0 || o.cmp(r.m, two_x) <= 0) break;
         r = sadd_t(o, r, SNum<A>{o.add(two_x, one), -1});
         x = o.inc(x);
-        ++g_corrections;
+        count_correction(steps);
     }
     return x;
 }
@@ -979,18 +990,25 @@ static void perturb_quotients(DevArith& o, uint32_t* Xn, uint64_t* XnL, uint64_t
     check(pdh_copy_dtoh(ln.data(), XnL, 2 * n * 8, st));
     check(pdh_stream_sync(st));
     const DNum kk = o.up(Mag{(uint32_t)(k > 0 ? k : 1)});
-    std::vector<int32_t> sg(n);
-    void* dsg = nullptr;
-    check(pdh_dev_alloc(n * 4, &dsg));
+    // the sums go to scratch with their own length words (a signed sum's
+    // output length must not be its operand's length word), then back
+    void* tmp = nullptr;
+    const uint64_t tl = (n * Cn + 63) & ~(uint64_t)63;
+    check(pdh_dev_alloc(tl * 4 + n * 8 + n * 4, &tmp));
+    uint32_t* T = (uint32_t*)tmp;
+    uint64_t* TL = (uint64_t*)(T + tl);
+    int32_t* TS = (int32_t*)(TL + n);
     std::vector<pdh_sadd_desc> ad;
+    std::vector<pdh_shr_desc> back;
     for (uint64_t i = 0; i < n; ++i) {
         if (k < 0 && ln[2 * i] == 0) continue;
         uint32_t* q = Xn + 2 * i * Cn;
-        ad.push_back(pdh_sadd_desc{q, XnL + 2 * i, 1, kk.p, kk.len, k > 0 ? 1 : -1, q, Cn, XnL + 2 * i,
-                                   (int32_t*)dsg + i});
+        ad.push_back(pdh_sadd_desc{q, XnL + 2 * i, 1, kk.p, kk.len, k > 0 ? 1 : -1, T + i * Cn, Cn, TL + i, TS + i});
+        back.push_back(pdh_shr_desc{T + i * Cn, TL + i, q, Cn, XnL + 2 * i, 0});
     }
-    const int rc = ad.empty() ? PDH_OK : pdh_sadd_batch_dev(ad.data(), ad.size(), st);
-    pdh_dev_free_async(dsg, st);
+    int rc = ad.empty() ? PDH_OK : pdh_sadd_batch_dev(ad.data(), ad.size(), st);
+    if (rc == PDH_OK && !back.empty()) rc = pdh_shr_batch_dev(back.data(), back.size(), st);
+    pdh_dev_free_async(tmp, st);
     check(rc);
 }
 
@@ -1094,8 +1112,9 @@ static std::string dev_to_decimal(const Mag& x, const Multiplier&) {
                 check(pdh_stream_sync(st));
                 const Mag Dh = o.down(D);
                 SInt r{rm, sg[i]};
-                while (r.s < 0) { r = sadd(r, SInt{Dh, 1}); dec(q); ++g_corrections; }
-                while (r.s > 0 && cmp(r.m, Dh) >= 0) { r = sadd(r, SInt{Dh, -1}); inc(q); ++g_corrections; }
+                int steps = 0;
+                while (r.s < 0) { r = sadd(r, SInt{Dh, 1}); dec(q); count_correction(steps); }
+                while (r.s > 0 && cmp(r.m, Dh) >= 0) { r = sadd(r, SInt{Dh, -1}); inc(q); count_correction(steps); }
                 Mag qz(Cn, 0), rz(Cn, 0);
                 std::copy(q.begin(), q.end(), qz.begin());
                 std::copy(r.m.begin(), r.m.end(), rz.begin());

@@ -229,7 +229,9 @@ int pdh_bs_merge_dev(const pdh_bs_node* in, size_t n_in, pdh_bs_node* out, int s
  * so a level's steps need no host reads of the values. */
 
 /* out = sa*a + sb*b for every descriptor (sa, sb in {-1, 0, 1}; 0 means that
- * operand is zero and is not read). out may be a itself. Limbs below out_cap
+ * operand is zero and is not read). out may be a itself, but out_len must
+ * not be a_len or b_len (the kernel clears it while other workgroups still
+ * read the operands' lengths; PDH_E_ARG if it is). Limbs below out_cap
  * are written and the result must fit in them; *out_len receives its
  * normalized length and *out_sign its sign (0 for zero). Two passes over
  * 4,096-limb tiles of every output; the stream is synchronized on return. */
