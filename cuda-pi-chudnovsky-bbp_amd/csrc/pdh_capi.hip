@@ -86,6 +86,20 @@ int scratch_wait_host(ScratchOrder& o) {
     return PDH_OK;
 }
 
+// ---- scratch of the allocation-free multiply (pdh_mul_dev_noalloc) -----------
+// Two spectra of up to 2^21 u64 words (either engine) and the carry scratch
+// of a product of up to 2^20 limbs, in a file-scope device array: every GPU
+// the library runs on holds it until the process ends. Host code never
+// passes `scratch` itself to a kernel or a copy (on the host the name stands
+// for a host-side shadow, not the device address): set-up resolves the
+// device address once with hipGetSymbolAddress and keeps it in the context.
+constexpr int NOALLOC_MAX_LOG = 21;
+constexpr size_t NOALLOC_WS_BYTES = (size_t)8 << NOALLOC_MAX_LOG;
+constexpr size_t NOALLOC_CARRY_BYTES = ((size_t)8 << 20) + ((size_t)64 << 10);
+constexpr size_t NOALLOC_SCRATCH_BYTES = 2 * NOALLOC_WS_BYTES + NOALLOC_CARRY_BYTES;
+static_assert(PDH_NOALLOC_MAX_LIMBS * 2 <= ((size_t)1 << NOALLOC_MAX_LOG), "limit and transform size disagree");
+static __device__ __align__(256) uint8_t scratch[NOALLOC_SCRATCH_BYTES];
+
 struct Ctx {
     std::mutex mu;
     int state = 0;          // 0 unknown, 1 ready, -1 no device
@@ -107,6 +121,9 @@ struct Ctx {
     ScratchOrder blk_order;
     std::vector<hipStream_t> sh_streams;   // internal streams of one-shot batches
     std::vector<hipEvent_t> sh_events;
+    char* noalloc_ptr = nullptr;           // device address of `scratch` on `device`
+    std::string noalloc_why;               // why it is unavailable, if it is
+    ScratchOrder noalloc_order;
 };
 
 Ctx& ctx() { static Ctx c; return c; }
@@ -295,6 +312,8 @@ struct PooledStream {
     }
 };
 
+void noalloc_setup(Ctx& c);
+
 int init_locked(Ctx& c) {
     if (c.state == 1) return PDH_OK;
     if (c.state == -1) return fail(PDH_E_NODEVICE, c.why);
@@ -356,6 +375,7 @@ int init_locked(Ctx& c) {
     HIPCALL(hipMalloc(&c.tw3, t3.size() * sizeof(u32)));
     HIPCALL(hipMemcpy(c.tw3, t3.data(), t3.size() * sizeof(u32), hipMemcpyHostToDevice));
     c.state = 1;
+    noalloc_setup(c);
     return PDH_OK;
 }
 
@@ -416,6 +436,35 @@ int outer_for(Ctx& c, int n, OuterTables* ot) {
     ot->hi_inv = it->second.hi_inv; ot->lo_inv = it->second.lo_inv; ot->h = it->second.h;
     ot->tw3 = c.tw3; ot->o3 = it->second.o3; ot->h3 = it->second.h3;
     return PDH_OK;
+}
+
+// Set-up of pdh_mul_dev_noalloc, with the context's device current: the
+// device address of `scratch`, the outer twiddle tables of every transform
+// size it can use and its ordering event, so that the entry point itself
+// allocates nothing. A failure here leaves the rest of the library usable
+// and makes only that entry point report it.
+void noalloc_setup(Ctx& c) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        c.noalloc_why = std::string("[hip] hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e);
+        return;
+    }
+    if (carry_ws_bytes(PDH_NOALLOC_MAX_LIMBS) > NOALLOC_CARRY_BYTES) {
+        c.noalloc_why = "allocation-free multiply: carry scratch larger than reserved";
+        return;
+    }
+    for (int n = SMALL_MAX_LOG + 1; n <= NOALLOC_MAX_LOG; ++n) {
+        OuterTables ot{};
+        if (outer_for(c, n, &ot) != PDH_OK) { c.noalloc_why = t_err; return; }
+    }
+    if (hipEventCreateWithFlags(&c.noalloc_order.ev, hipEventDisableTiming) != hipSuccess) {
+        (void)hipGetLastError();
+        c.noalloc_why = "allocation-free multiply: event creation failed";
+        return;
+    }
+    c.noalloc_ptr = (char*)p;
 }
 
 template <class P>
@@ -873,6 +922,46 @@ int pdh_sqr_dev(const uint32_t* d_a, size_t na, uint32_t* d_out, uint64_t* d_out
     int rc = init_locked(c);
     if (rc) return rc;
     return mul_dev_locked(c, d_a, na, d_a, na, true, d_out, d_out_len, (hipStream_t)stream);
+}
+
+int pdh_mul_dev_noalloc(const uint32_t* d_a, size_t na, const uint32_t* d_b, size_t nb,
+                        uint32_t* d_out, uint64_t* d_out_len, void* stream) {
+    if (!d_out_len || (na && nb && (!d_a || !d_b || !d_out))) return fail(PDH_E_ARG, "null pointer argument");
+    if (na + nb > PDH_NOALLOC_MAX_LIMBS)
+        return fail(PDH_E_TOO_LARGE, "pdh_mul_dev_noalloc: na + nb = " + std::to_string(na + nb) +
+                                         " limbs, above its limit of 2^20 (use pdh_mul_dev)");
+    Ctx& c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    if (c.state != 1)
+        return fail(PDH_E_NODEVICE, "pdh_mul_dev_noalloc: the library is not set up; call pdh_device_available() "
+                                    "first (set-up allocates, this entry point does not)");
+    if (!c.noalloc_ptr) return fail(PDH_E_HIP, c.noalloc_why);
+    int dev = -1;
+    HIPCALL(hipGetDevice(&dev));
+    if (dev != c.device) return fail(PDH_E_ARG, "pdh_mul_dev_noalloc: the current device is not the one set up");
+    const hipStream_t s = (hipStream_t)stream;
+    int rc = scratch_acquire(c.noalloc_order, s);
+    if (rc) return rc;
+    hipError_t e;
+    if (na == 0 || nb == 0) {
+        e = launch_zero_len(d_out_len, s);
+    } else {
+        const int n = log2_exact(pdh_ntt_length(na, nb));
+        const NttShape sh = ntt_shape(n);
+        OuterTables ot{};
+        if (sh.L1 > 0) {
+            auto it = c.outer.find(n);                 // built at set-up
+            if (it == c.outer.end()) return fail(PDH_E_HIP, "pdh_mul_dev_noalloc: twiddle table missing");
+            const Outer& o = it->second;
+            ot = OuterTables{o.hi, o.lo, o.hi_inv, o.lo_inv, o.h, c.tw3, o.o3, o.h3};
+        }
+        u64* wsA = (u64*)c.noalloc_ptr;
+        u64* wsB = (u64*)(c.noalloc_ptr + NOALLOC_WS_BYTES);
+        u64* carry = (u64*)(c.noalloc_ptr + 2 * NOALLOC_WS_BYTES);
+        e = launch_mul(d_a, na, d_b, nb, false, d_out, d_out_len, wsA, wsB, carry, c.twL, ot, sh, s);
+    }
+    if (e != hipSuccess) return fail_hip(e, "allocation-free multiply launch");
+    return scratch_release(c.noalloc_order, s);
 }
 
 int pdh_bbp_range(uint64_t start, uint32_t count, uint8_t* values) {

@@ -40,7 +40,7 @@ EXPORTED = (
     "pdh_copy_dtoh", "pdh_stream_create", "pdh_stream_destroy", "pdh_stream_sync",
     "pdh_bbp_set_exact", "pdh_bbp_set_layout", "pdh_bbp_exact_count", "pdh_tune_engine", "pdh_bs_groups", "pdh_bs_merge_dev",
     "pdh_sadd_batch_dev", "pdh_shr_batch_dev", "pdh_decimal_leaves_dev", "pdh_sadd_dev", "pdh_shift_dev",
-    "pdh_dev_free_async", "pdh_stream_query",
+    "pdh_dev_free_async", "pdh_stream_query", "pdh_mul_dev_noalloc",
 )
 
 
@@ -108,6 +108,8 @@ def lib() -> ctypes.CDLL:
         L.pdh_sqr.argtypes = [_u32p, _sz, _u32p, _sz, ctypes.POINTER(_sz)]
         L.pdh_mul_dev.argtypes = [_vp, _sz, _vp, _sz, _vp, _vp, _vp]
         L.pdh_sqr_dev.argtypes = [_vp, _sz, _vp, _vp, _vp]
+        L.pdh_mul_dev_noalloc.argtypes = [_vp, _sz, _vp, _sz, _vp, _vp, _vp]
+        L.pdh_mul_dev_noalloc.restype = ctypes.c_int
         L.pdh_bbp_range.argtypes = [ctypes.c_uint64, ctypes.c_uint32, _u8p]
         L.pdh_bbp_positions.argtypes = [_u64p, _sz, _u8p]
         L.pdh_bbp_positions_dev.argtypes = [_vp, _sz, _vp, _vp]
@@ -253,6 +255,14 @@ def sqr_ntt_hip(a) -> np.ndarray:
 def mul_dev(d_a: int, na: int, d_b: int, nb: int, d_out: int, d_len: int, stream: int = 0) -> None:
     """Device-resident multiply on raw device pointers (e.g. torch data_ptr())."""
     _check(lib().pdh_mul_dev(d_a, na, d_b, nb, d_out, d_len, stream or None))
+
+
+NOALLOC_MAX_LIMBS = 1 << 20
+
+
+def mul_dev_noalloc(d_a: int, na: int, d_b: int, nb: int, d_out: int, d_len: int, stream: int = 0) -> None:
+    """pdh_mul_dev without any allocation (na + nb <= 2^20; call device_available() first)."""
+    _check(lib().pdh_mul_dev_noalloc(d_a, na, d_b, nb, d_out, d_len, stream or None))
 
 
 def sqr_dev(d_a: int, na: int, d_out: int, d_len: int, stream: int = 0) -> None:

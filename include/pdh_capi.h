@@ -74,6 +74,23 @@ int pdh_mul_dev(const uint32_t* d_a, size_t na, const uint32_t* d_b, size_t nb,
 int pdh_sqr_dev(const uint32_t* d_a, size_t na, uint32_t* d_out, uint64_t* d_out_len,
                 void* stream);
 
+/* Allocation-free variant of pdh_mul_dev. No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc). Same
+ * arguments and product as pdh_mul_dev, for na + nb <= PDH_NOALLOC_MAX_LIMBS
+ * (2^20 limbs; larger products fail with PDH_E_TOO_LARGE). Its transforms
+ * run in a fixed ~40 MiB scratch array in device memory that every GPU the
+ * library runs on holds until the process ends; the call itself makes no
+ * hipMalloc, hipFree or other allocating call. The library must have been
+ * set up first: call pdh_device_available() once (set-up resolves the
+ * scratch address and builds the twiddle tables; it allocates), with the
+ * same device current as for the calls. Calls on one stream run in order;
+ * calls on different streams take the scratch in turn (each stream waits on
+ * the device for the previous user), so they do not overlap. */
+#define PDH_NOALLOC_MAX_LIMBS ((size_t)1 << 20)
+int pdh_mul_dev_noalloc(const uint32_t* d_a, size_t na, const uint32_t* d_b, size_t nb,
+                        uint32_t* d_out, uint64_t* d_out_len, void* stream);
+
 /* ---- device memory and streams for C/C++ callers ----------------------------
  * Thin wrappers so host code that builds batches or keeps operands resident
  * never includes HIP headers. Copies with a stream are asynchronous.
