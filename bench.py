@@ -223,7 +223,7 @@ def serve_trace(W, n_req, rate, seed=55):
     return reqs
 
 
-def bench_serve(pdh, W, n_req=40, rate=2.0):
+def bench_serve(pdh, W, n_req=240, rate=250.0):
     """Continuous-batching server vs a serial plain ntt-hip loop on the same
     trace; every server result must equal the serial one."""
     import threading
@@ -250,14 +250,16 @@ def bench_serve(pdh, W, n_req=40, rate=2.0):
     def run_server():
         srv = pdh.Server(200 << 30, 64)
         tickets = [None] * len(reqs)
+        late = [0.0] * len(reqs)          # submission after the arrival time (the submitting loop's own lag)
         t0 = time.perf_counter()
         for i, (arr, kind, arg) in enumerate(reqs):
             now = time.perf_counter() - t0
             if now < arr:
                 time.sleep(arr - now)
+            late[i] = max(0.0, time.perf_counter() - t0 - arr) * 1e3
             tickets[i] = (srv.submit_mul(*arg) if kind == "mul" else
                           srv.submit_pi(arg) if kind == "pi" else srv.submit_bbp(*arg))
-        lat = [srv.wait(t) for t in tickets]
+        lat = [srv.wait(t) + late[i] for i, t in enumerate(tickets)]
         span = time.perf_counter() - t0
         out = [srv.result(t) for t in tickets]
         srv.close()
@@ -526,10 +528,12 @@ def main() -> int:
             extra[f"chudnovsky_{digits}"] = rec
             del s_pi
         if not args.no_serve:
-            extra["c5_serve"] = bench_serve(pdh, W)
-            # the same mix arriving ten times faster: requests queue, the
-            # server overlaps and groups them, the serial loop cannot
-            extra["c5_serve_burst"] = bench_serve(pdh, W, n_req=40, rate=20.0)
+            # the mix arriving faster than a serial loop can serve it (its mean
+            # service time is about 5 ms): requests queue, the server overlaps
+            # and groups them, the serial loop cannot
+            extra["c5_serve"] = bench_serve(pdh, W, n_req=240, rate=250.0)
+            # a light load (arrival-bound for both)
+            extra["c5_serve_light"] = bench_serve(pdh, W, n_req=60, rate=20.0)
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(a, b, W, extras=not args.no_extra)
     line = {

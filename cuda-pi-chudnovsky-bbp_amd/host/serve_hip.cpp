@@ -2,32 +2,41 @@
 //  serve_hip.cpp -- a continuous-batching request loop on one GPU.
 //
 //  Requests (multiply, Chudnovsky pi, BBP hex digits) arrive at any time and
-//  wait in FIFO order until the HBM budget and the in-flight limit admit
-//  them; admitted requests start while earlier ones are still on the device:
+//  wait until the HBM budget and the in-flight limit admit them. Admission
+//  goes in arrival order, but a request that does not fit the budget lets
+//  later, smaller requests pass it (backfill) until it has waited
+//  kStarveMs; from then on nothing passes it, so it is admitted as soon as
+//  the requests ahead of it release enough memory. Admitted requests start
+//  while earlier ones are still on the device:
 //
-//  * multiplies admitted in one scheduler tick whose transform fits one
-//    workgroup class of moderate size (N <= 2^18) are grouped into one
-//    batched plan (pdh_batch_*), larger ones get a plan each; every plan has
-//    its own workspace and stream, so their kernels overlap on the device.
-//    Operands and products stay resident in HBM until the result is fetched.
-//  * BBP requests run in arrival order on one BBP stream (they share the
-//    library's BBP workspace), overlapping with the multiplies.
-//  * Chudnovsky requests run on a host worker thread whose multiplies and
-//    batched tree levels go to the device like everyone else's.
+//  * multiplies admitted in one scheduler tick whose transform is of
+//    moderate size (N <= 2^18) are grouped into one batched plan
+//    (pdh_batch_*), larger ones get a plan each; every plan has its own
+//    workspace and runs on its own stream from the server's pool, so their
+//    kernels overlap on the device. Operands and products stay in HBM until
+//    the result is fetched.
+//  * BBP requests go to kBbpWorkers host workers, Chudnovsky requests to
+//    kPiWorkers; each worker's device work runs on its own streams (the
+//    library orders the shared scratch between streams on the device), so
+//    they overlap with each other and with the multiplies.
 //
-//  A completion thread waits on each plan's stream and stamps the request's
-//  finish time; latency is finish - arrival. The reference has no serving
-//  layer: it computes one request per process run (src/main.cpp:93-179).
+//  A completion thread polls the running plans' streams and completes each
+//  request as soon as its own stream drains, in any order. Memory is given
+//  back with stream-ordered frees (no device-wide synchronisation). Latency
+//  is completion minus submission. The reference has no serving layer: it
+//  computes one request per process run (src/main.cpp:93-179).
 // ============================================================================
 #include "pidigits_hip.h"
 #include "pdh_capi.h"
 #include "pdh_pi.h"
 
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <list>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -42,6 +51,11 @@ using clk = std::chrono::steady_clock;
 enum Kind { MUL = 1, PI = 2, BBP = 3 };
 enum State { QUEUED = 0, RUNNING = 1, DONE = 2, FAILED = 3 };
 
+constexpr int kBbpWorkers = 2;
+constexpr int kPiWorkers = 2;
+constexpr double kStarveMs = 20.0;          // a request that waited this long stops being passed
+constexpr size_t kSmallN = (size_t)1 << 18; // multiplies up to this transform length are grouped
+
 struct Request {
     uint64_t id = 0;
     Kind kind = MUL;
@@ -49,10 +63,13 @@ struct Request {
     std::atomic<int> state{QUEUED};
     int err = 0;
     size_t bytes = 0;              // admission estimate of device memory held while running
+    size_t resident = 0;           // of which held until the result is fetched (multiply arena)
+    bool waited_mem = false;       // was passed over at least once for lack of budget
     // multiply
     std::vector<uint32_t> a, b;
     void* d_arena = nullptr;       // a | b | out | len, resident until fetched
     size_t na = 0, nb = 0;
+    bool empty_operand = false;
     // pi
     uint64_t digits = 0;
     std::string text;
@@ -62,101 +79,108 @@ struct Request {
     std::vector<uint8_t> values;
 };
 
+size_t mul_arena_bytes(size_t na, size_t nb) { return 8 * (na + nb) + 16; }
+
 size_t mul_bytes(size_t na, size_t nb) {
     const size_t N = pdh_ntt_length(na, nb);
-    return 8 * (na + nb) + 16 + 16 * N + (1u << 20);          // operands+product, two spectra, scratch
+    return mul_arena_bytes(na, nb) + 16 * N + (1u << 20);     // operands+product, two spectra, scratch
 }
 
 struct Job {
     void* stream = nullptr;
     pdh_batch* plan = nullptr;
-    void* d_pos = nullptr;          // BBP positions (freed at completion)
-    void* d_val = nullptr;
     std::vector<Request*> reqs;
-    size_t work_bytes = 0;          // released at completion
 };
+
+uint32_t* arena_out(Request* r) { return (uint32_t*)r->d_arena + r->na + r->nb; }
+uint64_t* arena_len(Request* r) {
+    return (uint64_t*)(((uintptr_t)(arena_out(r) + r->na + r->nb) + 7) & ~(uintptr_t)7);
+}
 
 } // namespace
 
 struct pdh_server {
-    size_t budget = 0, inuse = 0;
+    size_t budget = 0, inuse = 0, peak = 0;
     int max_inflight = 0, inflight = 0;
     uint64_t next_id = 1;
+    uint64_t n_submitted = 0, n_done = 0, n_waited_mem = 0, n_backfilled = 0;
     bool stop = false;
     std::mutex mu;
-    std::condition_variable cv_sched, cv_done, cv_jobs, cv_pi;
+    std::condition_variable cv_sched, cv_done, cv_jobs, cv_work;
     std::deque<Request*> pending;
     std::map<uint64_t, std::unique_ptr<Request>> all;
-    std::deque<Job> jobs;
-    std::deque<Request*> pi_queue;
-    void* bbp_stream = nullptr;
-    std::thread sched, completer, piworker;
+    std::list<Job> running;                 // multiply plans on the device
+    std::deque<Request*> pi_queue, bbp_queue;
+    std::vector<void*> stream_pool;
+    std::thread sched, completer;
+    std::vector<std::thread> workers;
+
+    // caller holds mu
+    void release_locked(size_t bytes) { inuse -= std::min(inuse, bytes); cv_sched.notify_all(); }
 
     void finish(Request* r, int state, size_t release) {
         std::lock_guard<std::mutex> g(mu);
         r->t_done = clk::now();
         r->state = state;
-        inuse -= release;
+        release_locked(release);
         --inflight;
+        ++n_done;
         cv_done.notify_all();
-        cv_sched.notify_all();
     }
 
-    int launch_muls(std::vector<Request*>& rs) {
-        if (rs.empty()) return PDH_OK;
+    void* get_stream() {
+        {
+            std::lock_guard<std::mutex> g(mu);
+            if (!stream_pool.empty()) { void* s = stream_pool.back(); stream_pool.pop_back(); return s; }
+        }
+        void* s = nullptr;
+        return pdh_stream_create(&s) == PDH_OK ? s : nullptr;
+    }
+    void put_stream(void* s) {
+        std::lock_guard<std::mutex> g(mu);
+        stream_pool.push_back(s);
+    }
+
+    // A failed multiply gives back everything it holds, its arena included.
+    void fail_mul(Request* r, int rc, void* stream) {
+        if (r->d_arena) pdh_dev_free_async(r->d_arena, stream);
+        r->d_arena = nullptr;
+        r->err = rc;
+        finish(r, FAILED, r->bytes);
+    }
+
+    void launch_muls(std::vector<Request*>& rs) {
+        if (rs.empty()) return;
         Job job;
-        job.reqs = rs;
-        int rc = pdh_stream_create(&job.stream);
-        std::vector<pdh_mul_desc> d(rs.size());
+        job.stream = get_stream();
+        int rc = job.stream ? PDH_OK : PDH_E_HIP;
+        std::vector<pdh_mul_desc> d;
         for (size_t i = 0; rc == PDH_OK && i < rs.size(); ++i) {
             Request* r = rs[i];
-            const size_t n = r->na + r->nb;
-            rc = pdh_dev_alloc(8 * n + 16, &r->d_arena);
+            rc = pdh_dev_alloc(mul_arena_bytes(r->na, r->nb), &r->d_arena);
             if (rc) break;
             uint32_t* A = (uint32_t*)r->d_arena;
             uint32_t* B = A + r->na;
-            uint32_t* O = B + r->nb;
-            uint64_t* L = (uint64_t*)(((uintptr_t)(O + n) + 7) & ~(uintptr_t)7);
             rc = pdh_copy_htod(A, r->a.data(), 4 * r->na, job.stream);
             if (!rc) rc = pdh_copy_htod(B, r->b.data(), 4 * r->nb, job.stream);
-            d[i] = pdh_mul_desc{A, r->na, B, r->nb, O, L};
-            job.work_bytes += r->bytes - (8 * n + 16);
+            d.push_back(pdh_mul_desc{A, r->na, B, r->nb, arena_out(r), arena_len(r)});
         }
         if (!rc) rc = pdh_batch_create(d.data(), d.size(), &job.plan);
         if (!rc) rc = pdh_batch_run(job.plan, job.stream);
         if (rc) {
             if (job.plan) pdh_batch_destroy(job.plan);
-            pdh_stream_destroy(job.stream);
-            for (Request* r : rs) { pdh_dev_free(r->d_arena); r->d_arena = nullptr; r->err = rc; finish(r, FAILED, r->bytes); }
-            return rc;
-        }
-        std::lock_guard<std::mutex> g(mu);
-        jobs.push_back(std::move(job));
-        cv_jobs.notify_all();
-        return PDH_OK;
-    }
-
-    void launch_bbp(Request* r) {
-        Job job;
-        job.reqs = {r};
-        job.stream = bbp_stream;
-        int rc = pdh_dev_alloc(r->count, &job.d_val);
-        if (!rc) rc = pdh_bbp_range_dev(r->start, r->count, (uint8_t*)job.d_val, bbp_stream);
-        if (rc) {
-            pdh_dev_free(job.d_pos);
-            pdh_dev_free(job.d_val);
-            r->err = rc;
-            finish(r, FAILED, r->bytes);
+            for (Request* r : rs) fail_mul(r, rc, job.stream);
+            if (job.stream) put_stream(job.stream);
             return;
         }
-        job.work_bytes = r->bytes;
+        job.reqs = rs;
         std::lock_guard<std::mutex> g(mu);
-        jobs.push_back(std::move(job));
+        running.push_back(std::move(job));
         cv_jobs.notify_all();
     }
 
-    // One scheduler tick: admit in arrival order while budget and slots allow,
-    // then start everything admitted.
+    // One scheduler tick: admit while slots and budget allow (see the file
+    // comment for the order), then start everything admitted.
     void sched_loop() {
         for (;;) {
             std::vector<Request*> adm;
@@ -164,12 +188,24 @@ struct pdh_server {
                 std::unique_lock<std::mutex> lk(mu);
                 cv_sched.wait_for(lk, std::chrono::microseconds(500), [&] { return stop || !pending.empty(); });
                 if (stop) return;
-                while (!pending.empty()) {
-                    Request* r = pending.front();
-                    if (inflight >= max_inflight || (inflight > 0 && inuse + r->bytes > budget)) break;
-                    pending.pop_front();
+                const clk::time_point now = clk::now();
+                bool blocked = false;
+                for (auto it = pending.begin(); it != pending.end() && inflight < max_inflight;) {
+                    Request* r = *it;
+                    const bool fits = inflight == 0 || inuse + r->bytes <= budget;
+                    if (!fits) {
+                        if (!r->waited_mem) { r->waited_mem = true; ++n_waited_mem; }
+                        const double waited = std::chrono::duration<double, std::milli>(now - r->t_arrival).count();
+                        if (waited >= kStarveMs) break;        // no more passing this one
+                        blocked = true;
+                        ++it;
+                        continue;
+                    }
+                    if (blocked) ++n_backfilled;
+                    it = pending.erase(it);
                     ++inflight;
                     inuse += r->bytes;
+                    peak = std::max(peak, inuse);
                     r->state = RUNNING;
                     adm.push_back(r);
                 }
@@ -177,59 +213,66 @@ struct pdh_server {
             std::vector<Request*> small;
             for (Request* r : adm) {
                 if (r->kind == MUL) {
-                    if (pdh_ntt_length(r->na, r->nb) <= ((size_t)1 << 18)) small.push_back(r);
+                    if (pdh_ntt_length(r->na, r->nb) <= kSmallN) small.push_back(r);
                     else { std::vector<Request*> one{r}; launch_muls(one); }
-                } else if (r->kind == BBP) {
-                    launch_bbp(r);
                 } else {
                     std::lock_guard<std::mutex> g(mu);
-                    pi_queue.push_back(r);
-                    cv_pi.notify_all();
+                    (r->kind == BBP ? bbp_queue : pi_queue).push_back(r);
+                    cv_work.notify_all();
                 }
             }
             launch_muls(small);
         }
     }
 
+    // Completes each plan as soon as its stream has drained, in any order.
     void complete_loop() {
         for (;;) {
-            Job job;
+            std::vector<Job> done;
             {
                 std::unique_lock<std::mutex> lk(mu);
-                cv_jobs.wait(lk, [&] { return stop || !jobs.empty(); });
-                if (jobs.empty()) return;
-                job = std::move(jobs.front());
-                jobs.pop_front();
-            }
-            int rc = pdh_stream_sync(job.stream);
-            for (Request* r : job.reqs) {
-                if (r->kind == BBP && rc == PDH_OK) {
-                    r->values.resize(r->count);
-                    rc = pdh_copy_dtoh(r->values.data(), job.d_val, r->count, nullptr);
+                cv_jobs.wait_for(lk, std::chrono::microseconds(200), [&] { return stop || !running.empty(); });
+                if (running.empty()) { if (stop) return; continue; }
+                for (auto it = running.begin(); it != running.end();) {
+                    const int q = pdh_stream_query(it->stream);
+                    if (q == 0) { ++it; continue; }
+                    done.push_back(std::move(*it));
+                    it = running.erase(it);
                 }
-                r->err = rc;
-                finish(r, rc ? FAILED : DONE, r->kind == BBP ? r->bytes : r->bytes - (8 * (r->na + r->nb) + 16));
             }
-            if (job.plan) pdh_batch_destroy(job.plan);
-            if (job.stream != bbp_stream) pdh_stream_destroy(job.stream);
-            pdh_dev_free(job.d_pos);
-            pdh_dev_free(job.d_val);
+            if (done.empty()) { std::this_thread::sleep_for(std::chrono::microseconds(20)); continue; }
+            for (Job& job : done) {
+                const int rc = pdh_stream_query(job.stream) == 1 ? PDH_OK : pdh_stream_sync(job.stream);
+                if (job.plan) pdh_batch_destroy(job.plan);
+                for (Request* r : job.reqs) {
+                    if (rc) { fail_mul(r, rc, job.stream); continue; }
+                    finish(r, DONE, r->bytes - r->resident);   // the arena stays until the result is fetched
+                }
+                put_stream(job.stream);
+            }
         }
     }
 
-    void pi_loop() {
+    void work_loop(Kind kind) {
         for (;;) {
             Request* r;
             {
                 std::unique_lock<std::mutex> lk(mu);
-                cv_pi.wait(lk, [&] { return stop || !pi_queue.empty(); });
-                if (pi_queue.empty()) return;
-                r = pi_queue.front();
-                pi_queue.pop_front();
+                std::deque<Request*>& q = kind == BBP ? bbp_queue : pi_queue;
+                cv_work.wait(lk, [&] { return stop || !q.empty(); });
+                if (q.empty()) return;
+                r = q.front();
+                q.pop_front();
             }
             int st = DONE;
             try {
-                r->text = pidigits::compute_pi_chudnovsky_hip(r->digits);
+                if (kind == BBP) {
+                    r->values.resize(r->count);
+                    const int rc = pdh_bbp_range(r->start, r->count, r->values.data());
+                    if (rc) { r->err = rc; st = FAILED; }
+                } else {
+                    r->text = pidigits::compute_pi_chudnovsky_hip(r->digits);
+                }
             } catch (const pidigits::HipError& e) {
                 r->err = e.code;
                 st = FAILED;
@@ -245,6 +288,7 @@ struct pdh_server {
         Request* raw = r.get();
         all[raw->id] = std::move(r);
         pending.push_back(raw);
+        ++n_submitted;
         cv_sched.notify_all();
         return raw->id;
     }
@@ -258,11 +302,10 @@ int pdh_server_create(uint64_t hbm_budget_bytes, int max_inflight, pdh_server** 
     pdh_server* s = new pdh_server();
     s->budget = hbm_budget_bytes;
     s->max_inflight = max_inflight;
-    int rc = pdh_stream_create(&s->bbp_stream);
-    if (rc) { delete s; return rc; }
     s->sched = std::thread([s] { s->sched_loop(); });
     s->completer = std::thread([s] { s->complete_loop(); });
-    s->piworker = std::thread([s] { s->pi_loop(); });
+    for (int i = 0; i < kBbpWorkers; ++i) s->workers.emplace_back([s] { s->work_loop(BBP); });
+    for (int i = 0; i < kPiWorkers; ++i) s->workers.emplace_back([s] { s->work_loop(PI); });
     *srv = s;
     return PDH_OK;
 }
@@ -275,10 +318,12 @@ int pdh_server_submit_mul(pdh_server* s, const uint32_t* a, size_t na, const uin
     r->a.assign(a, a + na);
     r->b.assign(b, b + nb);
     r->na = na; r->nb = nb;
-    r->bytes = mul_bytes(na, nb);
-    if (na == 0 || nb == 0) {                  // zero product: nothing to run
+    if (na == 0 || nb == 0) {                  // zero product: a one-limb stand-in, result reported empty
         r->a.assign(1, 0); r->b.assign(1, 0); r->na = r->nb = 1;
+        r->empty_operand = true;
     }
+    r->bytes = mul_bytes(r->na, r->nb);
+    r->resident = mul_arena_bytes(r->na, r->nb);
     *id = s->submit(std::move(r));
     return PDH_OK;
 }
@@ -299,7 +344,7 @@ int pdh_server_submit_bbp(pdh_server* s, uint64_t start, uint32_t count, uint64_
     r->kind = BBP;
     r->start = start;
     r->count = count;
-    r->bytes = ((size_t)1 << 28);              // the chunked BBP workspace
+    r->bytes = ((size_t)1 << 28);              // the BBP scratch of a range, bounded
     *id = s->submit(std::move(r));
     return PDH_OK;
 }
@@ -316,31 +361,30 @@ int pdh_server_wait(pdh_server* s, uint64_t id, double* latency_ms) {
 }
 
 // Copies the result (multiply: normalized u32 limbs; pi: "3.1415..."; BBP:
-// digit values) and releases the request and its resident buffers.
+// digit values) and releases the request and its resident buffers, also
+// when the request failed (its error code is returned).
 int pdh_server_result(pdh_server* s, uint64_t id, void* out, size_t cap, size_t* len) {
     if (!s || !len) return PDH_E_ARG;
-    int rc = pdh_server_wait(s, id, nullptr);
-    if (rc) return rc;
+    const int wrc = pdh_server_wait(s, id, nullptr);
     std::unique_ptr<Request> r;
     {
         std::lock_guard<std::mutex> g(s->mu);
         auto it = s->all.find(id);
+        if (it == s->all.end()) return PDH_E_ARG;
         r = std::move(it->second);
         s->all.erase(it);
     }
+    if (wrc) { *len = 0; return wrc; }
     if (r->kind == MUL) {
-        const size_t n = r->na + r->nb;
-        uint32_t* O = (uint32_t*)r->d_arena + n;
-        uint64_t* L = (uint64_t*)(((uintptr_t)(O + n) + 7) & ~(uintptr_t)7);
         uint64_t nl = 0;
-        rc = pdh_copy_dtoh(&nl, L, 8, nullptr);
-        if (r->a.size() == 1 && r->a[0] == 0) nl = 0;   // submitted with an empty operand
+        int rc = pdh_copy_dtoh(&nl, arena_len(r.get()), 8, nullptr);
+        if (r->empty_operand) nl = 0;
         *len = (size_t)nl;
         if (!rc && nl > cap) rc = PDH_E_CAPACITY;
-        if (!rc && nl) rc = pdh_copy_dtoh(out, O, 4 * nl, nullptr);
-        pdh_dev_free(r->d_arena);
+        if (!rc && nl) rc = pdh_copy_dtoh(out, arena_out(r.get()), 4 * nl, nullptr);
+        pdh_dev_free_async(r->d_arena, nullptr);       // its stream drained before the request completed
         std::lock_guard<std::mutex> g(s->mu);
-        s->inuse -= 8 * n + 16;
+        s->release_locked(r->resident);
         return rc;
     }
     const void* src = r->kind == PI ? (const void*)r->text.data() : (const void*)r->values.data();
@@ -351,20 +395,32 @@ int pdh_server_result(pdh_server* s, uint64_t id, void* out, size_t cap, size_t*
     return PDH_OK;
 }
 
+int pdh_server_stats(pdh_server* s, uint64_t* stats, int n) {
+    if (!s || !stats || n < 1) return PDH_E_ARG;
+    std::lock_guard<std::mutex> g(s->mu);
+    const uint64_t v[7] = {s->n_submitted, s->n_done, s->n_waited_mem, s->n_backfilled, (uint64_t)s->peak,
+                           (uint64_t)s->inuse, (uint64_t)s->budget};
+    for (int i = 0; i < n && i < 7; ++i) stats[i] = v[i];
+    return PDH_OK;
+}
+
 int pdh_server_destroy(pdh_server* s) {
     if (!s) return PDH_OK;
     {
-        std::lock_guard<std::mutex> g(s->mu);
+        // requests still waiting for admission are dropped; admitted ones finish
+        std::unique_lock<std::mutex> lk(s->mu);
+        s->pending.clear();
+        s->cv_done.wait(lk, [&] { return s->inflight == 0; });
         s->stop = true;
         s->cv_sched.notify_all();
         s->cv_jobs.notify_all();
-        s->cv_pi.notify_all();
+        s->cv_work.notify_all();
     }
     s->sched.join();
-    s->piworker.join();
     s->completer.join();
+    for (auto& t : s->workers) t.join();
     for (auto& kv : s->all) if (kv.second->d_arena) pdh_dev_free(kv.second->d_arena);
-    pdh_stream_destroy(s->bbp_stream);
+    for (void* st : s->stream_pool) pdh_stream_destroy(st);
     delete s;
     return PDH_OK;
 }

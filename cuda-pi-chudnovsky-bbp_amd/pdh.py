@@ -414,7 +414,7 @@ PI_EXPORTED = ("pdh_pi_chudnovsky", "pdh_pi_chudnovsky_cb", "pdh_chudnovsky_term
                "pdh_server_submit_mul", "pdh_server_submit_pi", "pdh_server_submit_bbp", "pdh_server_wait",
                "pdh_server_result", "pdh_server_destroy", "pdh_divmod", "pdh_isqrt", "pdh_divmod_dev",
                "pdh_isqrt_dev", "pdh_divmod_cb", "pdh_isqrt_cb", "pdh_newton_set_perturb",
-               "pdh_newton_corrections")
+               "pdh_newton_corrections", "pdh_server_stats")
 _pilib = None
 
 
@@ -435,6 +435,8 @@ def pilib() -> ctypes.CDLL:
         L.pdh_server_wait.argtypes = [_vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double)]
         L.pdh_server_result.argtypes = [_vp, ctypes.c_uint64, _vp, _sz, ctypes.POINTER(_sz)]
         L.pdh_server_destroy.argtypes = [_vp]
+        L.pdh_server_stats.argtypes = [_vp, _u64p, ctypes.c_int]
+        L.pdh_server_stats.restype = ctypes.c_int
         _szp = ctypes.POINTER(_sz)
         L.pdh_divmod.argtypes = [_u32p, _sz, _u32p, _sz, _u32p, _sz, _szp, _u32p, _sz, _szp]
         L.pdh_isqrt.argtypes = [_u32p, _sz, _u32p, _sz, _szp]
@@ -551,6 +553,12 @@ class Server:
         _check(pilib().pdh_server_result(self.h, ticket, ctypes.cast(buf, _vp), n, ctypes.byref(ln)))
         raw = buf.raw[: ln.value]
         return raw.decode() if kind == "pi" else np.frombuffer(raw, dtype=np.uint8).copy()
+
+    def stats(self) -> dict:
+        v = (ctypes.c_uint64 * 7)()
+        _check(pilib().pdh_server_stats(self.h, v, 7))
+        keys = ("submitted", "completed", "waited_for_memory", "backfilled", "peak_bytes", "held_bytes", "budget")
+        return dict(zip(keys, (int(x) for x in v)))
 
     def close(self) -> None:
         if self.h:

@@ -75,12 +75,15 @@ uint64_t pdh_chudnovsky_terms(uint64_t digits);
 
 /* ---- serving: a continuous-batching request loop on one GPU ----------------
  * No counterpart in the reference (one computation per process run,
- * src/main.cpp:93-179). Requests wait in arrival order until the HBM budget
- * and the in-flight limit admit them; admitted requests run while earlier
- * ones are still on the device (multiplies admitted together share one
- * batched plan; BBP requests run in order on their own stream; Chudnovsky
- * requests run on a host worker whose multiplies go to the device). Results
- * stay in device memory (multiplies) until fetched. */
+ * src/main.cpp:93-179). Requests wait until the HBM budget (bytes each
+ * request may hold on the device) and the in-flight limit admit them, in
+ * arrival order except that a request short of budget lets later ones pass
+ * for up to 20 ms; admitted requests run while earlier ones are still on
+ * the device (multiplies admitted together share one batched plan on a
+ * stream of their own; BBP and Chudnovsky requests run on two host workers
+ * each, whose device work overlaps with everything else) and complete as
+ * soon as their own work has finished. Results stay in device memory
+ * (multiplies) until fetched. */
 typedef struct pdh_server pdh_server;
 int pdh_server_create(uint64_t hbm_budget_bytes, int max_inflight, pdh_server** srv);
 int pdh_server_submit_mul(pdh_server* s, const uint32_t* a, size_t na, const uint32_t* b, size_t nb, uint64_t* id);
@@ -91,7 +94,13 @@ int pdh_server_wait(pdh_server* s, uint64_t id, double* latency_ms);
 /* Copies the result (multiply: normalized u32 limbs; pi: "3.14..." chars;
  * BBP: digit values 0..15), *len in elements, and releases the request. */
 int pdh_server_result(pdh_server* s, uint64_t id, void* out, size_t cap, size_t* len);
-/* Finishes the requests already running, then frees the server. */
+/* Counters: stats[0] requests submitted, [1] completed, [2] requests that
+ * found the budget short at least once and waited, [3] admissions that
+ * passed an earlier waiting request (backfill), [4] peak admitted bytes,
+ * [5] bytes held now, [6] the budget. Writes min(n, 7) values. */
+int pdh_server_stats(pdh_server* s, uint64_t* stats, int n);
+/* Drops requests not yet admitted, finishes the running ones, then frees
+ * the server. */
 int pdh_server_destroy(pdh_server* s);
 
 #ifdef __cplusplus

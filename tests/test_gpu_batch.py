@@ -6,6 +6,8 @@ every replay, with other device work between the replays.
 """
 from __future__ import annotations
 
+import os
+
 import numpy as np
 import pytest
 
@@ -94,4 +96,38 @@ def test_graph_replays_match_eager(pdh):
             assert np.array_equal(g, e), rep
         # unrelated eager device work between replays
         assert np.array_equal(pdh.mul_ntt_hip(other_a, other_b), oracle.mul_ntt(other_a, other_b))
+    plan.close()
+
+
+@pytest.mark.parametrize("which", ["c3_4096_pairs", "bs_levels_1e7"])
+def test_full_size_batches_vs_oracle(pdh, which):
+    """The two batched workloads at their full sizes, each run as one captured
+    plan (a hipGraph) and replayed, against the oracle's ntt-cpu product of
+    every pair computed one at a time (the reference's self-test compares
+    its device products the same way, tests/self_test.cu:64-78): the 4,096
+    pairs with limb counts log-uniform over [64, 65,536] (seed 3), and the
+    P, Q, T products of the 10^7-digit tree's depths 8-14 (src/chudnovsky.cpp:
+    64-102 level shapes; 130,048 products)."""
+    import torch
+    if which == "c3_4096_pairs":
+        pairs = W.batch_pairs(3, W.C3_PAIRS, W.C3_LO, W.C3_HI)
+        assert len(pairs) == 4096
+    else:
+        pairs = W.bs_level_pairs(7, int(pdh.pilib().pdh_chudnovsky_terms(10 ** 7)))
+        assert len(pairs) == 130_048
+    print(which, len(pairs), "pairs", flush=True)
+    db = DevBatch(pairs)
+    s = torch.cuda.Stream()
+    plan = pdh.Batch(db.descs)
+    plan.capture(s.cuda_stream)
+    for rep in range(2):
+        db.clear()
+        torch.cuda.synchronize()
+        plan.run(s.cuda_stream)
+        s.synchronize()
+        got = db.results()
+        if rep == 0:
+            want = oracle.mul_ntt_batch(pairs, threads=min(16, os.cpu_count() or 1))
+        bad = [i for i, (g, w) in enumerate(zip(got, want)) if not np.array_equal(g, w)]
+        assert not bad, (rep, bad[:5])
     plan.close()
