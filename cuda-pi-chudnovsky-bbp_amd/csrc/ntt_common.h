@@ -123,18 +123,19 @@ __device__ __forceinline__ u64 wave_max(u64 v) {
 // coefficients themselves never return to HBM. D and C determine the chunk's
 // carry function (thr from D's limbs, v0 = C, v1 = C + 1), so two scans over
 // (D, C) finish the carry: k_carry_reduce composes each workgroup's chunks
-// (k_carry_reduce_th reads the thresholds TH instead of D when the producer
-// wrote them), k_carry_blocks scans the workgroup aggregates (or, for jobs of
-// at most CARRY_NT workgroups, k_carry_apply composes them itself), and
-// k_carry_apply adds each chunk's carry-in to D in place. Limbs at or above na + nb are zero in D
-// (the product has na + nb limbs), so they are neither stored nor read.
+// into one aggregate per workgroup (k_carry_reduce_th reads the thresholds TH
+// instead of D when the producer wrote them), and k_carry_apply takes its
+// workgroup's carry-in from the aggregates below it (the nearest constant
+// one ends the look-back) and adds each chunk's carry-in to D in place.
+// Limbs at or above na + nb are zero in D (the product has na + nb limbs),
+// so they are neither stored nor read.
 //
 // Scratch per job: C[ceil(nl/2)] (enough for T >= 4), then TH[ceil(nl/2)]
 // (the chunk thresholds, when the producer writes them), then agg[nblk_max]
-// CarryFn, then cin[nblk_max].
+// CarryFn.
 struct CarryJob {
     u32* out; size_t nlimbs, nchunks, nblk;
-    u64* C; u64* TH; CarryFn* agg; u64* cin;
+    u64* C; u64* TH; CarryFn* agg;
 };
 template <int T>
 __device__ __forceinline__ CarryJob carry_job(const MulJob& J, size_t nblk_max) {
@@ -148,7 +149,6 @@ __device__ __forceinline__ CarryJob carry_job(const MulJob& J, size_t nblk_max) 
     c.C = (u64*)J.carry;
     c.TH = c.C + cap;
     c.agg = (CarryFn*)(c.TH + cap);
-    c.cin = (u64*)(c.agg + nblk_max);
     return c;
 }
 

@@ -361,44 +361,38 @@ __global__ __launch_bounds__(CARRY_NT) void k_carry_reduce_th(const MulJob* __re
     if (blk == 0 && threadIdx.x == 0) *J.out_len = 0;
 }
 
-// One workgroup of 1024 threads per job: carry-in of every carry block.
-template <int T>
-__global__ __launch_bounds__(1024) void k_carry_blocks(const MulJob* __restrict__ jobs, MulJob one,
-                                                       size_t nblk_max) {
-    __shared__ CarryFn wsum[1024 / 64];
-    __shared__ u64 seg_out[1024];
-    const MulJob J = jobs ? jobs[blockIdx.x] : one;
-    const CarryJob C = carry_job<T>(J, nblk_max);
-    // thread t takes the K contiguous aggregates [t K, t K + K): one scan over
-    // the segments' compositions, then each thread walks its own segment
-    const size_t K = (C.nblk + 1023) / 1024;
-    const size_t w0 = threadIdx.x * K;
-    CarryFn f = cf_pad();
-    bool first = true;
-    for (size_t k = 0; k < K; ++k) {
-        const size_t w = w0 + k;
-        if (w < C.nblk) {
-            f = first ? C.agg[w] : cf_compose(C.agg[w], f);
-            first = false;
-        } else if (!first) {
-            f = cf_compose(cf_pad(), f);
-        }
-    }
-    f = block_scan(f, wsum);
-    seg_out[threadIdx.x] = cf_apply(f, 0);
-    __syncthreads();
-    u64 c = threadIdx.x == 0 ? 0 : seg_out[threadIdx.x - 1];
-    for (size_t k = 0; k < K; ++k) {
-        const size_t w = w0 + k;
-        if (w >= C.nblk) break;
-        C.cin[w] = c;
-        c = cf_apply(C.agg[w], c);
+// The carry into carry workgroup blk: the composition of the aggregates of
+// workgroups blk-1, blk-2, ... down to the nearest constant one (a constant
+// function ignores its carry-in) or down to workgroup 0 (carry-in 0), taken
+// in windows of CARRY_NT aggregates, one block scan per window. For a
+// product that is not mostly all-ones limbs the aggregate of blk-1 is
+// already constant, and the caller does not come here. Every thread of the
+// workgroup calls it (it holds barriers); slot is LDS for three words.
+__device__ __forceinline__ bool cf_constant(const CarryFn& f) { return f.thr == ~(u64)0 || f.v0 == f.v1; }
+
+__device__ __noinline__ u64 carry_in_lookback(const CarryFn* __restrict__ agg, size_t blk, CarryFn* wsum,
+                                              u64* slot) {
+    CarryFn acc = cf_pad();
+    bool have = false;
+    size_t hi = blk;
+    for (;;) {
+        const size_t lo = hi > (size_t)CARRY_NT ? hi - CARRY_NT : 0;
+        const size_t t = lo + threadIdx.x;
+        const CarryFn F = block_scan(t < hi ? agg[t] : cf_pad(), wsum);   // thread 0 = workgroup lo
+        if (threadIdx.x == hi - lo - 1) { slot[0] = F.thr; slot[1] = F.v0; slot[2] = F.v1; }
+        __syncthreads();
+        const CarryFn Fw{slot[0], slot[1], slot[2]};
+        __syncthreads();
+        acc = have ? cf_compose(acc, Fw) : Fw;
+        have = true;
+        if (lo == 0 || cf_constant(Fw)) return cf_apply(acc, 0);
+        hi = lo;
     }
 }
 
 template <int T>
 __global__ __launch_bounds__(CARRY_NT) void k_carry_apply(const MulJob* __restrict__ jobs, MulJob one,
-                                                          size_t nblk_max, bool inline_cin) {
+                                                          size_t nblk_max) {
     using G = CarryGeom<T>;
     constexpr int KL = G::KL, CPT = G::CPT;
     __shared__ CarryFn wsum[CARRY_NT / 64];
@@ -413,6 +407,7 @@ __global__ __launch_bounds__(CARRY_NT) void k_carry_apply(const MulJob* __restri
     if (rb >= C.nblk) return;
     const size_t blk = C.nblk - 1 - rb;
     const size_t L0 = blk * G::LIMBS;
+    const CarryFn prev = blk > 0 ? C.agg[blk - 1] : cf_pad();      // in flight during the staging
     stage_in<T>(C, L0, sD);
     __syncthreads();
     const size_t ch0 = (blk * CARRY_NT + threadIdx.x) * CPT;
@@ -435,19 +430,9 @@ __global__ __launch_bounds__(CARRY_NT) void k_carry_apply(const MulJob* __restri
             f = cf_compose(cf_pad(), f);
         }
     }
-    // this workgroup's carry-in: from k_carry_blocks, or, when the job has at
-    // most CARRY_NT workgroups (inline_cin), by composing the aggregates of
-    // workgroups 0..blk-1 here (one scan; its entry blk-1 is their composition)
+    // this workgroup's carry-in, from the aggregates k_carry_reduce wrote
     u64 bc = 0;
-    if (!inline_cin) {
-        bc = C.cin[blk];
-    } else if (blk > 0) {
-        const CarryFn g = block_scan(threadIdx.x < blk ? C.agg[threadIdx.x] : cf_pad(), wsum);
-        if (threadIdx.x == blk - 1) outs[0] = cf_apply(g, 0);
-        __syncthreads();
-        bc = outs[0];
-        __syncthreads();
-    }
+    if (blk > 0) bc = cf_constant(prev) ? prev.v0 : carry_in_lookback(C.agg, blk, wsum, outs);
     CarryFn inc = block_scan(f, wsum);
     outs[threadIdx.x] = cf_apply(inc, bc);
     __syncthreads();
@@ -792,11 +777,7 @@ static hipError_t launch_carry_scan_T(const MulJob* d_jobs, const MulJob& one, u
     const dim3 cgrid((unsigned)(njobs * nblk_max));
     if (th) PDH_LAUNCH("k_carry_reduce_th", s, k_carry_reduce_th<T>, cgrid, dim3(CARRY_NT), 0, s, d_jobs, one, nblk_max);
     else PDH_LAUNCH("k_carry_reduce", s, k_carry_reduce<T>, cgrid, dim3(CARRY_NT), 0, s, d_jobs, one, nblk_max);
-    // a job of at most CARRY_NT carry workgroups needs no separate aggregate scan
-    const bool inline_cin = nblk_max <= (size_t)CARRY_NT;
-    if (!inline_cin)
-        PDH_LAUNCH("k_carry_blocks", s, k_carry_blocks<T>, dim3(njobs), dim3(1024), 0, s, d_jobs, one, nblk_max);
-    PDH_LAUNCH("k_carry_apply", s, k_carry_apply<T>, cgrid, dim3(CARRY_NT), 0, s, d_jobs, one, nblk_max, inline_cin);
+    PDH_LAUNCH("k_carry_apply", s, k_carry_apply<T>, cgrid, dim3(CARRY_NT), 0, s, d_jobs, one, nblk_max);
     return hipSuccess;
 }
 
@@ -825,7 +806,7 @@ size_t carry_nblk(size_t nlimbs) {
 
 size_t carry_ws_bytes(size_t nlimbs) {
     const size_t nblk = carry_nblk(nlimbs);
-    return 2 * (((nlimbs + 1) / 2 + 3) / 4 * 4) * sizeof(u64) + nblk * sizeof(CarryFn) + nblk * sizeof(u64) + 256;
+    return 2 * (((nlimbs + 1) / 2 + 3) / 4 * 4) * sizeof(u64) + nblk * sizeof(CarryFn) + 256;
 }
 
 template <bool SQ, int NT>
