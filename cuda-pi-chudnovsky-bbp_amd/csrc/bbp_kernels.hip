@@ -16,13 +16,9 @@
 //  exact method: below m = 2^30 by integer Montgomery arithmetic (see
 //  modpow16_mont), above by left-to-right square-and-multiply on FP64 values
 //  (v_fma_f64 issues at full rate on gfx950, 64-bit integer modulo does not
-//  exist in hardware): for |r| < 16m, m < 2^31,
-//      p = r*r (rounded), e = fma(r, r, -p) (exact error, r*r = p + e),
-//      q = rint(p / m),   r' = fma(-q, m, p) + e
-//  where fma(-q, m, p) is exact because p - q*m is an integer below 2^31, so
-//  r' = r*r - q*m exactly and |r'| < m. A set exponent bit multiplies by 16
-//  without reduction; the next squaring reduces. The result is brought into
-//  [0, m) at the end.
+//  exist in hardware), exact for every m < 2^48 -- the moduli of positions
+//  up to 2^45, where the reference's bbp_mulmod (include/bbp_kernel.h:70-83)
+//  is exact -- with residues kept in |r| <= 0.54 m (see modpow16_exact).
 //
 //  Layout: 4 lanes per position (one per j in {1,4,5,6}); lane 4p combines
 //  the four series through cross-lane shuffles in the reference's order.
@@ -43,15 +39,23 @@ __device__ __forceinline__ double recip_f64(double m) {
     return fma(r, e, r);
 }
 
+// 16^e mod m for 1 < m < 2^48 on FP64 values, with signed residues |r| <=
+// 0.54 m between steps. A square: p = r*r (rounded), e = fma(r, r, -p) (the
+// exact error, r^2 = p + e, |e| <= 2^43 for r^2 < 2^96), q = rint(p / m), off
+// by at most 0.54 from r^2 / m (r^2 / m <= 0.3 m < 2^47 and 1/m is correct to
+// about 2^-52); fma(-q, m, p) is the exact p - q m, an integer below 0.54 m +
+// 2^43 < 2^53, and adding e gives r^2 - q m exactly, again |r'| <= 0.54 m. A
+// set exponent bit multiplies by 16 (exact) and subtracts rint(16 r / m) m
+// the same way. The top three exponent bits start as the exact power 2^(4t).
+// The result is brought into [0, m) at the end.
 __device__ __forceinline__ double modpow16_exact(uint64_t e, double m, double minv) {
     double r = 1.0;
     if (e) {
         int bit = 63 - __clzll((long long)e);
         if (bit >= 2) {
-            // the top three exponent bits t in [4, 7]: 16^t = 2^(4t) is exact
-            const int t = (int)(e >> (bit - 2));
+            const int t = (int)(e >> (bit - 2));          // t in [4, 7]: 16^t = 2^(4t) <= 2^28
             r = ldexp(1.0, 4 * t);
-            r = fma(-floor(r * minv), m, r);      // exact: |result| < 2m
+            r = fma(-__builtin_rint(r * minv), m, r);
             bit -= 3;
         }
         for (; bit >= 0; --bit) {
@@ -59,7 +63,10 @@ __device__ __forceinline__ double modpow16_exact(uint64_t e, double m, double mi
             const double err = fma(r, r, -p);
             const double q = __builtin_rint(p * minv);
             r = fma(-q, m, p) + err;
-            if ((e >> bit) & 1) r = r * 16.0;
+            if ((e >> bit) & 1) {
+                const double v = r * 16.0;
+                r = fma(-__builtin_rint(v * minv), m, v);
+            }
         }
     }
     const double q = floor(r * minv);
@@ -107,10 +114,15 @@ __device__ __forceinline__ uint32_t modpow16_mont(uint64_t e, uint32_t m, double
     return y << s;
 }
 
-// 16^e mod m for 1 < m < 2^31: Montgomery below 2^30, the FP64 ladder above.
+// 16^e mod m for 1 < m < 2^48: Montgomery below 2^30, the FP64 ladder above.
 __device__ __forceinline__ double modpow16(uint64_t e, uint64_t mi, double m, double minv) {
     return mi < ((uint64_t)1 << 30) ? (double)modpow16_mont(e, (uint32_t)mi, minv) : modpow16_exact(e, m, minv);
 }
+
+// The largest position the kernels compute: every modulus 8k + j with
+// k <= position - 1 stays below 2^48.
+constexpr uint64_t BBP_POS_LIMIT = (uint64_t)1 << 45;
+__device__ __forceinline__ bool bbp_pos_ok(uint64_t p) { return p >= 1 && p <= BBP_POS_LIMIT; }
 
 // {16^d S_j}: bbp_kernel.h:99-122 with the integer part done exactly above.
 __device__ double bbp_series_dev(uint32_t j, uint64_t d) {
@@ -154,9 +166,9 @@ __global__ __launch_bounds__(256) void k_bbp(const uint64_t* __restrict__ pos, u
         const size_t pi = (ngroups - 1 - idx) * 16 + (size_t)(lane >> 2);
         const bool active = pi < count;
         uint64_t position = active ? (pos ? pos[pi] : start + pi) : 1;
-        // Positions outside [1, 2^28] (possible only through the device entry
+        // Positions outside [1, 2^45] (possible only through the device entry
         // point) are not computed: their value is 0xFF.
-        const bool valid = position >= 1 && position <= ((uint64_t)1 << 28);
+        const bool valid = bbp_pos_ok(position);
         if (!valid) position = 1;
         const uint32_t j = q == 0 ? 1u : (uint32_t)(q + 3);   // 1, 4, 5, 6
         const double S = bbp_series_dev(j, position - 1);
@@ -214,27 +226,28 @@ __global__ __launch_bounds__(256) void k_bbp_init(const uint64_t* __restrict__ p
     const size_t pi = s >> 2;
     const int q = (int)(s & 3);
     const uint64_t position = pos ? pos[pi] : start + pi * stride;
-    const bool valid = position >= 1 && position <= ((uint64_t)1 << 28);
+    const bool valid = bbp_pos_ok(position);
     st[s] = BbpSeries{valid ? position - 1 : 0, 0.0, q == 0 ? 1u : (uint32_t)(q + 3), valid ? 1u : 0u};
     if (acc) acc[s] = 0;
 }
 
-// floor(r 2^32 / m) and the remainder, for integers 0 <= r < m < 2^31; c32 =
-// 2^32 / m to ~2^-50 relative, so the estimate is within 2^-18 of the true
-// quotient (below 2^32 - 2) and one correction step suffices.
-__device__ __forceinline__ uint32_t div_step(uint32_t r, uint32_t m, double c32, uint32_t& rem) {
+// floor(r 2^32 / m) and the remainder, for integers 0 <= r < m < 2^48; c32 =
+// 2^32 / m to ~2^-50 relative, so the estimate (below 2^32) is within 2^-18
+// of the true quotient and one correction step suffices. r 2^32 - q m is
+// formed mod 2^64: its true value lies in (-m, 2m), so the wrapped
+// difference read as signed is exact.
+__device__ __forceinline__ uint32_t div_step(uint64_t r, uint64_t m, double c32, uint64_t& rem) {
     uint32_t q = (uint32_t)((double)r * c32);
-    const int64_t x = (int64_t)(((uint64_t)r << 32) - (uint64_t)q * m);
-    int64_t t = x;
-    if (t < 0) { --q; t += m; } else if (t >= (int64_t)m) { ++q; t -= m; }
-    rem = (uint32_t)t;
+    int64_t t = (int64_t)((r << 32) - (uint64_t)q * m);
+    if (t < 0) { --q; t += (int64_t)m; } else if (t >= (int64_t)m) { ++q; t -= (int64_t)m; }
+    rem = (uint64_t)t;
     return q;
 }
 
-// floor(r 2^64 / m) for integers 0 <= r < m < 2^31.
-__device__ __forceinline__ uint64_t frac64(uint32_t r, uint32_t m, double minv) {
+// floor(r 2^64 / m) for integers 0 <= r < m < 2^48.
+__device__ __forceinline__ uint64_t frac64(uint64_t r, uint64_t m, double minv) {
     const double c32 = minv * 4294967296.0;
-    uint32_t rem;
+    uint64_t rem;
     const uint32_t hi = div_step(r, m, c32, rem);
     const uint32_t lo = div_step(rem, m, c32, rem);
     return ((uint64_t)hi << 32) | lo;
@@ -308,6 +321,10 @@ __device__ __forceinline__ uint64_t frac64_mont_wave(uint32_t e, uint32_t m) {
 // Virtual block v: series v / bps, terms [kb * 256 * KT, (kb + 1) * 256 * KT).
 // Lane t takes k = base + i * 256 + t, so a wavefront's exponents d - k are
 // consecutive and its square-and-multiply loops have the same length.
+// WIDE (some position above 2^28, so moduli up to 2^48 and 64-bit term
+// indices): every term takes the FP64 ladder, which costs about what the
+// Montgomery one does per exponent bit.
+template <bool WIDE>
 __global__ __launch_bounds__(256) void k_bbp_fx_terms(const BbpSeries* __restrict__ st, unsigned bps, int KT,
                                                       uint64_t* __restrict__ acc) {
     const unsigned s = blockIdx.x / bps;
@@ -315,6 +332,26 @@ __global__ __launch_bounds__(256) void k_bbp_fx_terms(const BbpSeries* __restric
     const BbpSeries S = st[s];
     const uint64_t base64 = kb * 256u * (uint64_t)KT;
     if (!S.valid || base64 > S.d) return;
+    if constexpr (WIDE) {
+        uint64_t local = 0;
+        for (int i = 0; i < KT; ++i) {
+            const uint64_t k = base64 + (uint64_t)i * 256u + threadIdx.x;
+            if (k > S.d) break;
+            const uint64_t mi = 8ull * k + S.j;
+            if (mi == 1) continue;                    // 16^d mod 1 = 0
+            const double m = (double)mi;
+            const double minv = recip_f64(m);
+            const uint64_t r = k == S.d ? 1u : (uint64_t)modpow16_exact(S.d - k, m, minv);
+            local += frac64(r, mi, minv);
+        }
+        for (int o = 32; o > 0; o >>= 1) local += __shfl_xor(local, o);
+        __shared__ uint64_t wpart[4];
+        if ((threadIdx.x & 63) == 0) wpart[threadIdx.x >> 6] = local;
+        __syncthreads();
+        if (threadIdx.x == 0)
+            atomicAdd((unsigned long long*)&acc[s], (unsigned long long)(wpart[0] + wpart[1] + wpart[2] + wpart[3]));
+        return;
+    }
     // positions are at most 2^28 (the entry points check), so d, k < 2^28 and
     // m = 8k + j < 2^31: the term indices run in 32 bits
     const uint32_t d = (uint32_t)S.d, base = (uint32_t)base64;
@@ -479,7 +516,7 @@ __global__ void k_bbp_group_scan(const uint64_t* __restrict__ pos, size_t count,
     const size_t end = min(count, g * 64 + 64);
     for (size_t i = g * 64; i < end; ++i) {
         const uint64_t p = pos[i];
-        if (p >= 1 && p <= ((uint64_t)1 << 28)) {
+        if (bbp_pos_ok(p)) {
             any = true;
             sum += p;
             if (p - 1 > mx) mx = p - 1;
@@ -661,7 +698,7 @@ hipError_t launch_bbp_fx(const uint64_t* d_pos, uint64_t start, size_t count, ui
     int KT = 32;
     uint64_t bps = (dmax + 1 + 256ull * KT - 1) / (256ull * KT);
     while (bps > 1 && bps * S > ((uint64_t)1 << 17)) { KT *= 2; bps = (dmax + 1 + 256ull * KT - 1) / (256ull * KT); }
-    if (lanes) {
+    if (lanes && dmax < ((uint64_t)1 << 28)) {          // 32-bit term indices
         // about 2^20 blocks, but at least 64 exponents per block (measured on
         // 1..65,536: 2^14 blocks of 4,096 exponents 49 ms, 2^18 of 256 29 ms,
         // 2^20 of 64 28 ms; every 7th position to 65,536: 10 exponents per
@@ -672,8 +709,11 @@ hipError_t launch_bbp_fx(const uint64_t* d_pos, uint64_t start, size_t count, ui
         const uint64_t chunks = (dmax + EK) / EK;
         BBP_LAUNCH("k_bbp_fx_lanes", s, k_bbp_fx_lanes, dim3((unsigned)(groups * chunks)), dim3(256), 0, s,
                    (const BbpSeries*)st, count, (uint32_t)chunks, (uint32_t)EK, acc);
+    } else if (dmax < ((uint64_t)1 << 28)) {
+        BBP_LAUNCH("k_bbp_fx_terms", s, k_bbp_fx_terms<false>, dim3((unsigned)(bps * S)), dim3(256), 0, s,
+                   (const BbpSeries*)st, (unsigned)bps, KT, acc);
     } else {
-        BBP_LAUNCH("k_bbp_fx_terms", s, k_bbp_fx_terms, dim3((unsigned)(bps * S)), dim3(256), 0, s,
+        BBP_LAUNCH("k_bbp_fx_terms_wide", s, k_bbp_fx_terms<true>, dim3((unsigned)(bps * S)), dim3(256), 0, s,
                    (const BbpSeries*)st, (unsigned)bps, KT, acc);
     }
     if (d_pos || block <= 1)

@@ -119,18 +119,22 @@ __host__ __device__ __forceinline__ u64 gl_mulp2(u64 x) {
 // rho: primitive 2^32-th root with rho^(2^26) == 8. Found once on the host:
 // z = 7^((p-1)/2^32) is primitive of order 2^32; y = z^(2^26) is a primitive
 // 64th root and 8 = y^u for exactly one odd u < 64; rho = z^u.
-inline u64 gl_rho_host() {
-    static u64 cached = 0;
-    if (cached) return cached;
+inline u64 gl_rho_search() {
     u64 z = gl_pow(7, (GL_P - 1) >> 32);
     u64 y = gl_pow(z, 1ull << 26);
     u64 t = y;                       // y^u for odd u, stepping by y^2
     u64 y2 = gl_mul(y, y);
     for (u64 u = 1; u < 64; u += 2) {
-        if (t == 8) { cached = gl_pow(z, u); return cached; }
+        if (t == 8) return gl_pow(z, u);
         t = gl_mul(t, y2);
     }
     return 0;                        // unreachable: 8 is a primitive 64th root
+}
+// The search runs once per process; the initialisation of a function-local
+// static is thread-safe in C++11, so concurrent first calls are too.
+inline u64 gl_rho_host() {
+    static const u64 rho = gl_rho_search();
+    return rho;
 }
 // Primitive 2^k-th root of the system (k <= 32).
 inline u64 gl_root_host(int k) { return gl_pow(gl_rho_host(), 1ull << (32 - k)); }

@@ -537,6 +537,29 @@ __device__ __forceinline__ void set_step(u32* x, const u32* w, const Prime3& P) 
     }
 }
 
+// The exchange between sets with bottoms B0 and B1. A wavefront's 64
+// consecutive threads u = 64 g + lane hold, in a set of bottom B <= 6, the
+// positions ((u >> B) << (B + 4)) | (t << B) | (u mod 2^B) = 1024 g + ... :
+// exactly the aligned block [1024 g, 1024 g + 1024) of its row, whatever
+// B <= 6 is. An exchange between two such sets therefore stays inside the
+// wavefront: its LDS writes and reads touch only its own block, which no
+// other wavefront reads or writes until the next workgroup barrier, and a
+// wavefront's LDS operations execute in order, so only the compiler must be
+// kept from moving them (a wave barrier). The same holds when a whole row
+// fits in one wavefront (TPR <= 64). Other exchanges need the workgroup
+// barrier.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int L2, int B0, int B1>
+__device__ __forceinline__ void row_exchange_sync() {
+    if constexpr (RowPlan<L2>::TPR <= 64 || (B0 <= 6 && B1 <= 6)) wave_lds_sync();
+    else __syncthreads();
+}
+
 // Forward of NOP operands (1 or 2, arrays l[0], l[1]): x holds set 0 on entry,
 // the spectrum in the last set's positions on exit.
 template <int L2, int NOP, int S = 0>
@@ -552,7 +575,7 @@ __device__ __forceinline__ void row_fwd(u32 (*x)[16], const u32* w, int u, u32* 
         const int a0 = set_base<RP::bottom(S), 4, 0>(u, 0), a1 = set_base<RP::bottom(S + 1), 4, 0>(u, 0);
 #pragma unroll
         for (int k = 0; k < NOP; ++k) set_store<RP::bottom(S), 4, 0>(l[k], a0, x[k]);
-        __syncthreads();
+        row_exchange_sync<L2, RP::bottom(S), RP::bottom(S + 1)>();
 #pragma unroll
         for (int k = 0; k < NOP; ++k) set_load<RP::bottom(S + 1), 4, 0>(l[k], a1, x[k]);
         row_fwd<L2, NOP, S + 1>(x, wn, u, l, tw, P);
@@ -569,7 +592,7 @@ __device__ __forceinline__ void row_inv(u32* x, const u32* w, int u, u32* __rest
         u32 wn[16];
         set_tw<L2, S - 1, true>(wn, u, tw);
         set_store<RP::bottom(S), 4, 0>(l, set_base<RP::bottom(S), 4, 0>(u, 0), x);
-        __syncthreads();
+        row_exchange_sync<L2, RP::bottom(S), RP::bottom(S - 1)>();
         set_load<RP::bottom(S - 1), 4, 0>(l, set_base<RP::bottom(S - 1), 4, 0>(u, 0), x);
         row_inv<L2, S - 1>(x, wn, u, l, tw, P);
     }

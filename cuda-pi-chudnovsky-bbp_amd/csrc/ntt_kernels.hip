@@ -698,6 +698,8 @@ bool set_shape(const NttShape& s, bool custom) {
     return true;
 }
 
+bool shape_custom(int n) { return g_custom[n]; }
+
 // A class of many independent products already fills the GPU, so for the
 // three-prime engine at n = 14..18 short columns and wide tiles (fewer
 // levels, fewer workgroups) beat the single-product table: the 4,096-pair

@@ -19,6 +19,7 @@
 //  while they read the configuration, so they overlap with everything else.
 // ============================================================================
 #include "pdh_capi.h"
+#include "pdh_diag.h"
 #include "gl_field.h"
 #include "pdh_internal.h"
 
@@ -622,7 +623,7 @@ int mul_host(const u32* a, size_t na, const u32* b, size_t nb, bool square,
     return PDH_OK;
 }
 
-constexpr uint64_t BBP_POS_MAX = (uint64_t)1 << 28;   // 8(pos-1)+6 < 2^31
+constexpr uint64_t BBP_POS_MAX = (uint64_t)1 << 45;   // 8(pos-1)+6 < 2^48, where the reference's mulmod is exact
 
 // Digits per anchor of a contiguous range whose largest d (position - 1) is
 // dmax: the largest G <= 8 with 16^(G-1) e1 <= 2^46, e1 = 4 (dmax + 15) + 2
@@ -793,7 +794,7 @@ int bbp_host(const uint64_t* positions, uint64_t start, size_t count, uint8_t* v
     if (positions) {
         for (size_t i = 0; i < count; ++i)
             if (positions[i] < 1 || positions[i] > BBP_POS_MAX)
-                return fail(PDH_E_ARG, "BBP position outside [1, 2^28]");
+                return fail(PDH_E_ARG, "BBP position outside [1, 2^45]");
         // Similar positions share a wavefront: sort, compute, scatter back.
         order.resize(count);
         std::iota(order.begin(), order.end(), (size_t)0);
@@ -801,7 +802,7 @@ int bbp_host(const uint64_t* positions, uint64_t start, size_t count, uint8_t* v
         sorted.resize(count);
         for (size_t i = 0; i < count; ++i) sorted[i] = positions[order[i]];
     } else if (start < 1 || start - 1 + count > BBP_POS_MAX) {
-        return fail(PDH_E_ARG, "BBP position outside [1, 2^28]");
+        return fail(PDH_E_ARG, "BBP position outside [1, 2^45]");
     }
     BbpCfg cfg;
     int rc = bbp_begin(&cfg);
@@ -971,7 +972,7 @@ int pdh_bbp_range(uint64_t start, uint32_t count, uint8_t* values) {
 int pdh_bbp_range_dev(uint64_t start, uint32_t count, uint8_t* d_values, void* stream) {
     if (count == 0) return PDH_OK;
     if (!d_values) return fail(PDH_E_ARG, "null output pointer");
-    if (start < 1 || start - 1 + count > BBP_POS_MAX) return fail(PDH_E_ARG, "BBP position outside [1, 2^28]");
+    if (start < 1 || start - 1 + count > BBP_POS_MAX) return fail(PDH_E_ARG, "BBP position outside [1, 2^45]");
     BbpCfg cfg;
     int rc = bbp_begin(&cfg);
     if (rc) return rc;
@@ -1014,8 +1015,12 @@ int pdh_bbp_positions_dev(const uint64_t* d_positions, size_t n, uint8_t* d_valu
             if (e != hipSuccess) return fail_hip(e, "bbp position scan");
         }
         std::vector<uint8_t> tmp(n);
-        return bbp_few(cfg, d_positions, nullptr, 0, n, sc[0], d_values, tmp.data(), st, false,
-                       sc[2] ? (double)sc[1] / (double)sc[2] : 0.0);
+        rc = bbp_few(cfg, d_positions, nullptr, 0, n, sc[0], d_values, tmp.data(), st, false,
+                     sc[2] ? (double)sc[1] / (double)sc[2] : 0.0);
+        if (rc) return rc;
+        for (size_t i = 0; i < n; ++i)
+            if (tmp[i] == 0xFF) return fail(PDH_E_ARG, "BBP position outside [1, 2^45] (its value is 0xFF)");
+        return PDH_OK;
     }
     Ctx& c = ctx();
     std::lock_guard<std::mutex> g(c.mu);
@@ -1750,6 +1755,59 @@ int pdh_autotune(int n_lo, int n_hi, int reps, void* stream) {
         set_shape(best);
     }
     cleanup();
+    return PDH_OK;
+}
+
+// Shape files: a first line "pdh-shapes 1 gfx950", then one line per
+// transform size, "n eng L1 L2 T lr_col lr_row custom", for n = 1..MAX_LOG.
+// A file is read whole and checked (every shape must be launchable and name
+// its own n once) before any shape changes, so a bad file changes nothing.
+static const char kShapeMagic[] = "pdh-shapes 1 gfx950";
+
+int pdh_tune_save(const char* path) {
+    if (!path) return fail(PDH_E_ARG, "bad argument");
+    Ctx& c = ctx();
+    std::string text = std::string(kShapeMagic) + "\n";
+    {
+        std::lock_guard<std::mutex> g(c.mu);
+        for (int n = 1; n <= MAX_LOG; ++n) {
+            const NttShape s = ntt_shape(n);
+            char line[96];
+            std::snprintf(line, sizeof line, "%d %d %d %d %d %d %d %d\n", s.n, s.eng, s.L1, s.L2, s.T,
+                          s.lr_col, s.lr_row, shape_custom(n) ? 1 : 0);
+            text += line;
+        }
+    }
+    FILE* f = std::fopen(path, "w");
+    if (!f) return fail(PDH_E_ARG, "cannot open the shape file for writing");
+    const bool ok = std::fwrite(text.data(), 1, text.size(), f) == text.size();
+    if (std::fclose(f) != 0 || !ok) return fail(PDH_E_ARG, "cannot write the shape file");
+    return PDH_OK;
+}
+
+int pdh_tune_load(const char* path) {
+    if (!path) return fail(PDH_E_ARG, "bad argument");
+    FILE* f = std::fopen(path, "r");
+    if (!f) return fail(PDH_E_ARG, "cannot open the shape file");
+    char line[256];
+    bool ok = std::fgets(line, sizeof line, f) != nullptr && std::strncmp(line, kShapeMagic, sizeof kShapeMagic - 1) == 0 &&
+              (line[sizeof kShapeMagic - 1] == '\n' || line[sizeof kShapeMagic - 1] == 0);
+    std::vector<std::pair<NttShape, bool>> shapes;
+    bool seen[MAX_LOG + 1] = {};
+    while (ok && std::fgets(line, sizeof line, f)) {
+        NttShape s{};
+        int custom = 0, used = 0;
+        if (std::sscanf(line, "%d %d %d %d %d %d %d %d %n", &s.n, &s.eng, &s.L1, &s.L2, &s.T, &s.lr_col,
+                        &s.lr_row, &custom, &used) != 8 || line[used] != 0 || (custom != 0 && custom != 1) ||
+            !shape_valid(s) || seen[s.n]) { ok = false; break; }
+        seen[s.n] = true;
+        shapes.push_back({s, custom == 1});
+    }
+    std::fclose(f);
+    if (!ok) return fail(PDH_E_ARG, "shape file is not a pdh-shapes 1 gfx950 file of launchable shapes");
+    Ctx& c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    for (const auto& [s, custom] : shapes) set_shape(s, custom);
     return PDH_OK;
 }
 

@@ -36,6 +36,7 @@ bool shape_valid(const NttShape& s);
 // false if the shape is not launchable; custom = false marks an engine's
 // default shape (batches then keep their own shape rule)
 bool set_shape(const NttShape& s, bool custom = true);
+bool shape_custom(int n);   // set by pdh_tune_set / pdh_tune_engine / autotune / a loaded file
 NttShape default_shape_for(int n, int eng);
 // Shape for a launch of njobs independent products of size class n.
 NttShape batch_shape(int n, unsigned njobs);

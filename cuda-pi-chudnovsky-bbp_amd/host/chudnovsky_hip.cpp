@@ -32,6 +32,7 @@
 #include "pidigits_hip.h"
 #include "pdh_capi.h"
 #include "pdh_pi.h"
+#include "pdh_diag.h"
 
 #include <algorithm>
 #include <atomic>

@@ -6,6 +6,7 @@
 // ============================================================================
 #include "pidigits_hip.h"
 #include "pdh_capi.h"
+#include "pdh_diag.h"
 
 #include <cstdio>
 #include <string>

@@ -339,7 +339,7 @@ int pdh_server_submit_pi(pdh_server* s, uint64_t digits, uint64_t* id) {
 }
 
 int pdh_server_submit_bbp(pdh_server* s, uint64_t start, uint32_t count, uint64_t* id) {
-    if (!s || !id || start < 1 || count == 0 || start - 1 + count > ((uint64_t)1 << 28)) return PDH_E_ARG;
+    if (!s || !id || start < 1 || count == 0 || start - 1 + count > ((uint64_t)1 << 45)) return PDH_E_ARG;
     std::unique_ptr<Request> r(new Request());
     r->kind = BBP;
     r->start = start;

@@ -36,7 +36,7 @@ EXPORTED = (
     "pdh_mul_dev", "pdh_sqr_dev", "pdh_bbp_range", "pdh_bbp_range_dev", "pdh_bbp_positions",
     "pdh_bbp_positions_dev", "pdh_launch_count", "pdh_timing_enable", "pdh_timing_read", "pdh_probe_mad_rate",
     "pdh_batch_create", "pdh_batch_run", "pdh_batch_capture", "pdh_batch_destroy", "pdh_mul_batch_dev",
-    "pdh_tune_get", "pdh_tune_set", "pdh_autotune", "pdh_dev_alloc", "pdh_dev_free", "pdh_copy_htod",
+    "pdh_tune_get", "pdh_tune_set", "pdh_autotune", "pdh_tune_save", "pdh_tune_load", "pdh_dev_alloc", "pdh_dev_free", "pdh_copy_htod",
     "pdh_copy_dtoh", "pdh_stream_create", "pdh_stream_destroy", "pdh_stream_sync",
     "pdh_bbp_set_exact", "pdh_bbp_set_layout", "pdh_bbp_exact_count", "pdh_tune_engine", "pdh_bs_groups", "pdh_bs_merge_dev",
     "pdh_sadd_batch_dev", "pdh_shr_batch_dev", "pdh_decimal_leaves_dev", "pdh_sadd_dev", "pdh_shift_dev",
@@ -143,7 +143,9 @@ def lib() -> ctypes.CDLL:
         L.pdh_tune_set.argtypes = [ctypes.c_int] * 5
         L.pdh_autotune.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp]
         L.pdh_tune_engine.argtypes = [ctypes.c_int, ctypes.c_int, _ip]
-        for name in ("pdh_tune_get", "pdh_tune_set", "pdh_autotune", "pdh_tune_engine"):
+        L.pdh_tune_save.argtypes = [ctypes.c_char_p]
+        L.pdh_tune_load.argtypes = [ctypes.c_char_p]
+        for name in ("pdh_tune_get", "pdh_tune_set", "pdh_autotune", "pdh_tune_save", "pdh_tune_load", "pdh_tune_engine"):
             getattr(L, name).restype = ctypes.c_int
         L.pdh_dev_alloc.argtypes = [_sz, ctypes.POINTER(_vp)]
         L.pdh_dev_free.argtypes = [_vp]
@@ -219,6 +221,16 @@ def tune_set(n: int, L1: int, T: int, lr_col: int, lr_row: int) -> None:
 
 def autotune(n_lo: int, n_hi: int, reps: int = 3, stream: int = 0) -> None:
     _check(lib().pdh_autotune(n_lo, n_hi, reps, stream or None))
+
+
+def tune_save(path: str) -> None:
+    """Write every transform size's shape to `path` (pdh_tune_save)."""
+    _check(lib().pdh_tune_save(os.fsencode(path)))
+
+
+def tune_load(path: str) -> None:
+    """Read shapes written by tune_save; a bad file raises and changes nothing."""
+    _check(lib().pdh_tune_load(os.fsencode(path)))
 
 
 def ntt_length(na: int, nb: int) -> int:

@@ -164,6 +164,14 @@ int pdh_tune_engine(int n, int engine, int* current);
  * timed runs each, on seeded operands in scratch buffers) and keep the
  * fastest. Allocates and frees its scratch; the work is synchronous. */
 int pdh_autotune(int n_lo, int n_hi, int reps, void* stream);
+/* Write the shape of every transform size (defaults, set and tuned ones) to
+ * a text file, so that a later process can skip the autotuning; read such a
+ * file back. A file is checked whole before anything changes: an unreadable
+ * file, another format or an unlaunchable shape gives PDH_E_ARG and leaves
+ * every shape as it was. Sizes the file does not name keep their shapes.
+ * Neither needs a device. No counterpart in the reference. */
+int pdh_tune_save(const char* path);
+int pdh_tune_load(const char* path);
 
 /* Hex digit values (0..15) at 1-based positions start .. start+count-1, host
  * buffer. Position 1 is the first hex digit after the point (value 2).
@@ -197,9 +205,10 @@ int pdh_bbp_positions(const uint64_t* positions, size_t n, uint8_t* values);
  * synchronises `stream` twice: after a scan of the positions (the largest,
  * and how close each group of 64 lies, which picks the term kernel) and to
  * find the undecided positions. Positions must lie in
- * [1, 2^28] (the kernel's exact FP64 modular arithmetic needs 8k + j <
- * 2^31); the host entry points reject others with PDH_E_ARG, and this one
- * writes 0xFF for them. */
+ * [1, 2^45] (every modulus 8k + j below 2^48, the range where the
+ * reference's bbp_mulmod is exact, include/bbp_kernel.h:63-83); the host
+ * entry points reject others with PDH_E_ARG before any device work, and this
+ * one writes 0xFF for them and returns PDH_E_ARG. */
 int pdh_bbp_positions_dev(const uint64_t* d_positions, size_t n, uint8_t* d_values,
                           void* stream);
 
@@ -293,45 +302,8 @@ int pdh_shift_dev(const uint32_t* a, const uint64_t* a_len, int64_t bits, uint32
 int pdh_decimal_leaves_dev(uint32_t* d_x, uint64_t stride, const uint64_t* d_len, uint64_t n, uint32_t width,
                            char* d_out, void* stream);
 
-/* Testing and diagnostics (no counterpart in the reference).
- * pdh_bbp_set_exact(mode): 0 (default) the fixed-point path with the
- * FP64-order fallback; 1 the FP64-order (chunked) path for every position;
- * 2 the fixed-point path with every odd index (in sorted order) sent to the
- * fallback as well, to exercise it; 3 one lane per series in the reference's
- * FP64 order for every position (wavefronts take groups of positions from a
- * work-queue counter in the library's context, so such calls on different
- * streams must not overlap on the device). pdh_bbp_exact_count is the
- * number of positions computed by the chunked FP64-order path in this
- * process. */
-int pdh_bbp_set_exact(int mode);
-uint64_t pdh_bbp_exact_count(void);
-/* pdh_bbp_set_layout(mode): the fixed-point path's term kernel. 0 (default)
- * chooses per call: positions in lanes (one exponent per wavefront, its bits
- * scalar) when at least three lanes in four stay busy -- the positions of
- * each group of 64, in the order given, lie close together -- else terms in
- * lanes; 1 forces terms in lanes, 2 positions in lanes. The digits are the
- * same either way. */
-int pdh_bbp_set_layout(int mode);
-/* The device's measured v_mad_u64_u32 issue rate (lane operations per
- * second, whole GPU), the integer-multiply ceiling BBP throughput is quoted
- * against. Synchronous; allocates and frees its own buffer. */
-int pdh_probe_mad_rate(double* lane_ops_per_s);
-
-/* Number of kernels this library has launched in this process. Lets a test
- * assert that the device path really ran (the reference's self-test could
- * not tell its CPU fallback from the GPU, tests/self_test.cu:66). */
-uint64_t pdh_launch_count(void);
-
-/* Per-kernel device timing for benchmarks and profiling (no counterpart in
- * the reference, which times whole calls with std::chrono,
- * src/chudnovsky.cpp:108-157). pdh_timing_enable(1) clears the record and
- * brackets every later launch with hipEvents on its stream; 0 stops.
- * pdh_timing_read waits for the recorded events, writes up to max_kernels
- * kernel names ('\n'-separated, NUL-terminated) with their summed durations
- * in ms and launch counts, and clears the record. */
-int pdh_timing_enable(int on);
-int pdh_timing_read(char* names, size_t names_cap, double* total_ms, uint64_t* counts,
-                    int max_kernels, int* n_kernels);
+/* Test switches, the launch counter, per-kernel timing and a microbenchmark
+ * are declared in pdh_diag.h: they are not part of the seam. */
 
 #ifdef __cplusplus
 }

@@ -58,16 +58,6 @@ int pdh_divmod_dev(const uint32_t* d_a, size_t na, const uint32_t* d_d, size_t n
                    uint64_t* d_q_len, uint32_t* d_r, size_t r_cap, uint64_t* d_r_len, void* stream);
 int pdh_isqrt_dev(const uint32_t* d_s, size_t ns, uint32_t* d_x, size_t x_cap, uint64_t* d_x_len, void* stream);
 
-/* Test switch (no counterpart in the reference): k != 0 (|k| <= 8) moves
- * every Newton quotient and square-root estimate by k units, and every
- * decimal-conversion quotient estimate by k (k > 0) or by -1 where it is at
- * least 1 (k < 0), before the exact correction, so that the correction
- * branches run; results are unchanged. 0 (default) turns it off. */
-int pdh_newton_set_perturb(int k);
-/* Correction steps the exact floor corrections have taken in this process
- * (quotient, square-root and decimal-level steps together). */
-uint64_t pdh_newton_corrections(void);
-
 /* Number of series terms for `digits` digits (16 guard digits included),
  * as the reference's chudnovsky_terms_for_digits(digits + 16)
  * (src/chudnovsky.cpp:54-59, 118-120). */

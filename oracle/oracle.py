@@ -84,6 +84,8 @@ def rlib() -> ctypes.CDLL:
         L.ref_mul_ntt_cpu_batch.restype = ctypes.c_int
         L.ref_bbp_hex_range.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_char_p, ctypes.c_int]
         L.ref_bbp_positions.argtypes = [_u64p, _sz, ctypes.c_char_p, ctypes.c_int]
+        L.ref_bbp_modpow16.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
+        L.ref_bbp_modpow16.restype = ctypes.c_uint64
         _r = L
     return _r
 

@@ -5,14 +5,15 @@
 //  CPU functions. oracle/Makefile compiles it together with the reference's
 //  src/ntt_cpu.cpp and src/bbp_cpu.cpp, read in place from the reference
 //  checkout, into oracle/_ref/libref.so. Nothing of the reference is copied
-//  here; only its public declarations (bignum.h:194-195, bbp.h:27-30) are
-//  included and called.
+//  here; only its public declarations (bignum.h:194-195, bbp.h:27-30) and
+//  the inline bbp_modpow16 of bbp_kernel.h are included and called.
 //
 //  Used to pin the restatement in cpu_ref.cpp and as the "reference" CPU
 //  baseline in bench.py when the library has been built.
 // ============================================================================
 #include "bignum.h"
 #include "bbp.h"
+#include "bbp_kernel.h"
 
 #include <cstdint>
 #include <cstring>
@@ -77,5 +78,9 @@ void ref_bbp_hex_range(uint64_t start, uint32_t count, char* out, int threads) {
 void ref_bbp_positions(const uint64_t* pos, size_t n, char* out, int threads) {
     parallel_for(n, threads, [&](size_t i) { out[i] = pidigits::bbp_hex_digit(pos[i]); });
 }
+
+// The reference's 16^e mod m (include/bbp_kernel.h:86-96, with its 128-bit
+// bbp_mulmod branch for m >= 2^32, :70-83).
+uint64_t ref_bbp_modpow16(uint64_t e, uint64_t m) { return pidigits::bbp_modpow16(e, m); }
 
 } // extern "C"

@@ -8,6 +8,7 @@
 //  own self-test cannot, tests/self_test.cu:66).
 // ============================================================================
 #include "pdh_capi.h"
+#include "pdh_diag.h"
 
 #include <cstdio>
 

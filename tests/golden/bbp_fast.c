@@ -14,7 +14,9 @@
  *   otherwise m = 2^s mo with mo odd, and for 4e >= s (always, s <= 2)
  *   16^e mod m = 2^s (2^(4e - s) mod mo), with 2^E mod mo by a left-to-right
  *   square-and-double ladder in Montgomery form (R = 2^32, mo < 2^30),
- *   eight terms per AVX-512 vector and four vectors interleaved.
+ *   eight terms per AVX-512 vector and four vectors interleaved; moduli from
+ *   2^30 up to 2^48 (positions up to 2^45) take the same ladder with R = 2^64
+ *   one term at a time.
  *
  * Checked against the reference's own bbp_cpu (compiled from source) on
  * positions 1..65,536 and on the committed sampled positions before use. */
@@ -68,8 +70,33 @@ static void pow2_mod(const uint64_t* E, const uint64_t* mo, const uint64_t* mi, 
     }
 }
 
+/* 2^E mod mo for odd mo in [2^30, 2^48) (moduli of positions above 2^27),
+ * one term at a time: the same square-and-double ladder in Montgomery form
+ * with R = 2^64 on 128-bit products. For x < 2 mo, REDC(x^2) < (4 mo^2 +
+ * 2^64 mo) / 2^64 < 2 mo; a doubling and two conditional subtractions keep
+ * x in [0, mo) between steps. */
+static uint64_t pow2_mod64(uint64_t E, uint64_t mo) {
+    uint64_t x = mo;                            /* mo^-1 mod 2^64: 3 -> 96 bits */
+    for (int i = 0; i < 5; ++i) x *= 2u - mo * x;
+    const uint64_t ninv = 0u - x;
+    const uint64_t one = (uint64_t)(((unsigned __int128)1 << 64) % mo);
+    uint64_t r = one;
+    for (int b = E ? 63 - __builtin_clzll(E) : -1; b >= 0; --b) {
+        const unsigned __int128 t = (unsigned __int128)r * r;
+        const uint64_t m = (uint64_t)t * ninv;
+        r = (uint64_t)((t + (unsigned __int128)m * mo) >> 64);
+        if ((E >> b) & 1) r <<= 1;
+        if (r >= mo) r -= mo;
+        if (r >= mo) r -= mo;
+    }
+    const uint64_t m = r * ninv;
+    r = (uint64_t)(((unsigned __int128)r + (unsigned __int128)m * mo) >> 64);
+    return r >= mo ? r - mo : r;
+}
+
 typedef struct {
     uint64_t E[BLK], mo[BLK], mi[BLK], one[BLK], r[BLK];
+    uint8_t big[BLK];
     double q[4][BLK];
 } Work;
 
@@ -95,6 +122,11 @@ static void block_terms(Work* w, uint64_t d, uint64_t k0, int len) {
             const uint64_t k = k0 + (uint64_t)i;
             const uint64_t e = k <= d ? d - k : 0;
             uint64_t mo = (8 * k + j) >> s;
+            w->big[i] = mo >= (1u << 30);
+            if (w->big[i]) {                         /* the 64-bit ladder below; a dummy lane here */
+                w->r[i] = e ? pow2_mod64(4 * e - (uint64_t)s, mo) : 1;
+                mo = 3;
+            }
             mo = mo < 3 ? 3 : mo;
             uint32_t x = (uint32_t)mo;              /* -mo^-1 mod 2^32 by Newton */
             for (int it = 0; it < 5; ++it) x *= 2u - (uint32_t)mo * x;
@@ -107,7 +139,10 @@ static void block_terms(Work* w, uint64_t d, uint64_t k0, int len) {
             w->mi[i] = (uint32_t)(0u - x);
             w->one[i] = (uint64_t)one;
         }
+        uint64_t keep[BLK];
+        for (int i = 0; i < n; ++i) keep[i] = w->r[i];
         pow2_mod(w->E, w->mo, w->mi, w->one, w->r, n);
+        for (int i = 0; i < n; ++i) if (w->big[i]) w->r[i] = keep[i];
         for (int i = 0; i < len; ++i) {
             const uint64_t m = 8 * (k0 + (uint64_t)i) + j;
             w->q[jj][i] = (double)(w->r[i] << s) / (double)m;
@@ -176,10 +211,10 @@ static void* worker(void* arg) {
     return NULL;
 }
 
-/* Digit values (0..15) at 1-based positions, positions <= 2^27 (m < 2^30). */
+/* Digit values (0..15) at 1-based positions, positions <= 2^45 (m < 2^48). */
 int bbp_fast_positions(const uint64_t* pos, size_t n, uint8_t* vals, int threads) {
     for (size_t i = 0; i < n; ++i)
-        if (pos[i] == 0 || pos[i] > (1ull << 27)) return -1;
+        if (pos[i] == 0 || pos[i] > (1ull << 45)) return -1;
     if (threads < 1) threads = 1;
     Job job = {pos, vals, n, 0, PTHREAD_MUTEX_INITIALIZER};
     memset(vals, 0xFF, n);
@@ -199,6 +234,7 @@ uint64_t bbp_fast_modpow16(uint64_t e, uint64_t m) {
     const int s = __builtin_ctzll(m);
     const uint64_t mo = m >> s;
     if (mo == 1) return 0;
+    if (mo >= (1u << 30)) return pow2_mod64(4 * e - (uint64_t)s, mo) << s;
     uint64_t E[32], MO[32], MI[32], ONE[32], R[32];
     for (int i = 0; i < 32; ++i) {
         E[i] = 4 * e - (uint64_t)s; MO[i] = mo; MI[i] = neg_inv32((uint32_t)mo); ONE[i] = (1ull << 32) % mo;

@@ -18,6 +18,16 @@ the reference's order, with the exact residues 16^(d-k) mod m from a
 vectorized Montgomery ladder. It is first checked against the reference's
 own digits for positions 1..65,536 and the 64 sampled positions.
     python tests/golden/make_golden.py --part pi-data
+    python tests/golden/make_golden.py --part bbp-high     # ~15 min, 8 threads
+    python tests/golden/make_golden.py --part bbp-ref-high # ~30 min, 2 threads
+
+bbp-high: digits above position 2^27, where moduli pass 2^30 (and, above
+2^29, 2^32, the reference's 128-bit bbp_mulmod branch, exact below 2^48).
+Eight positions in (2^27, 2^28] and two in (2^28, 2^29] come from the
+reference's own bbp_cpu, and bbp_fast.c must agree with them; two positions
+whose moduli pass 2^32 (the reference's 128-bit branch takes hours per
+digit there) come from bbp_fast.c alone, after its residues are checked
+against the reference's bbp_modpow16 on moduli up to 2^48.
 """
 from __future__ import annotations
 
@@ -139,6 +149,53 @@ def part_bbp_sampled_4096(g: dict, threads: int) -> None:
     os.remove(partial)
 
 
+def part_bbp_high(g: dict, threads: int) -> None:
+    import ctypes
+    import random
+    import tempfile
+    rng = random.Random(2028)
+    ref_pos = sorted(rng.randint((1 << 27) + 1, 1 << 28) for _ in range(8))
+    ref_pos += sorted(rng.randint((1 << 28) + 1, 1 << 29) for _ in range(2))
+    fast_pos = [600_000_007, 1_000_000_000]
+    with tempfile.TemporaryDirectory() as tmp:
+        fast = build_bbp_fast(tmp)
+        L = ctypes.CDLL(os.path.join(tmp, "libbbpfast.so"))
+        L.bbp_fast_modpow16.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
+        L.bbp_fast_modpow16.restype = ctypes.c_uint64
+        R = oracle.rlib()
+        for _ in range(20000):                         # BBP moduli 8k + j, up to 2^48
+            bits = rng.randint(30, 48)
+            m = 8 * (rng.getrandbits(bits - 3) | (1 << (bits - 4))) + rng.choice((1, 4, 5, 6))
+            e = rng.getrandbits(rng.randint(1, 45))
+            assert L.bbp_fast_modpow16(e, m) == R.ref_bbp_modpow16(e, m) == pow(16, e, m), (e, m)
+        print("bbp_fast residues equal the reference's up to 2^48", flush=True)
+        t = time.time()
+        want = oracle.ref_bbp_hex_positions(np.array(ref_pos, dtype=np.uint64), threads)
+        print("reference digits", want, f"{time.time() - t:.0f}s", flush=True)
+        t = time.time()
+        got = fast(ref_pos, threads)
+        assert got == want, (got, want)
+        print("bbp_fast agrees", f"{time.time() - t:.0f}s", flush=True)
+        t = time.time()
+        high = fast(fast_pos, threads)
+        print("bbp_fast above 2^29", high, f"{time.time() - t:.0f}s", flush=True)
+    g["bbp_high"] = {"positions": ref_pos + fast_pos, "hex": want + high,
+                     "from_reference": len(ref_pos), "generator": "reference bbp_cpu, then bbp_fast.c"}
+
+
+def part_bbp_ref_high(g: dict, threads: int) -> None:
+    """The reference's own digits at 3*10^8 and 10^9 (moduli past 2^31 and
+    2^32), one position per thread: about half an hour. The 10^9 digit must
+    equal the one bbp_fast.c gave for "bbp_high"."""
+    pos = [300_000_000, 1_000_000_000]
+    t = time.time()
+    want = oracle.ref_bbp_hex_positions(np.array(pos, dtype=np.uint64), min(threads, len(pos)))
+    print("reference digits", want, f"{time.time() - t:.0f}s", flush=True)
+    hi = g["bbp_high"]
+    assert want[1] == hi["hex"][hi["positions"].index(1_000_000_000)]
+    g["bbp_ref_high"] = {"positions": pos, "hex": want, "generator": "reference bbp_cpu"}
+
+
 def part_pi_data(g: dict) -> None:
     text = open(os.path.join(REFERENCE_DIR, "include", "pi_reference.h")).read()
 
@@ -155,7 +212,9 @@ def part_pi_data(g: dict) -> None:
 
 def main() -> None:
     ap = argparse.ArgumentParser()
-    ap.add_argument("--part", required=True, choices=["mul", "bbp-range", "bbp-sampled", "bbp-sampled-4096", "pi-data"])
+    ap.add_argument("--part", required=True,
+                    choices=["mul", "bbp-range", "bbp-sampled", "bbp-sampled-4096", "pi-data", "bbp-high",
+                             "bbp-ref-high"])
     ap.add_argument("--count", type=int, default=64)
     ap.add_argument("--threads", type=int, default=os.cpu_count() or 1)
     args = ap.parse_args()
@@ -170,6 +229,10 @@ def main() -> None:
         part_bbp_range(g, args.threads)
     elif args.part == "bbp-sampled":
         part_bbp_sampled(g, args.count, args.threads)
+    elif args.part == "bbp-high":
+        part_bbp_high(g, args.threads)
+    elif args.part == "bbp-ref-high":
+        part_bbp_ref_high(g, args.threads)
     else:
         part_pi_data(g)
     # re-read before writing so parts run in parallel do not drop each other's keys

@@ -13,31 +13,44 @@ from conftest import ROOT
 
 HEADER = os.path.join(ROOT, "include", "pdh_capi.h")
 PI_HEADER = os.path.join(ROOT, "include", "pdh_pi.h")
+DIAG_HEADER = os.path.join(ROOT, "include", "pdh_diag.h")
 
 
-def declared(header=HEADER):
-    text = open(header).read()
+def declared(header=HEADER, text=None):
+    text = open(header).read() if text is None else text
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return sorted(set(re.findall(r"\b(pdh_[a-z_0-9]+)\s*\(", text)))
+
+
+def declared_diag():
+    """The test and diagnostic entry points of pdh_diag.h, split by the
+    library that exports them."""
+    text = open(DIAG_HEADER).read()
+    core, pi = text.split("exported by libpidigits_hip.so")
+    return declared(text=core), declared(text=pi)
 
 
 def test_header_declarations_are_exported():
     import pdh
     names = declared()
     assert "pdh_mul" in names and "pdh_bbp_positions" in names
+    diag = declared_diag()[0]
+    assert "pdh_bbp_set_exact" in diag and "pdh_bbp_set_exact" not in names
     L = pdh.lib()
-    for n in names:
+    for n in names + diag:
         assert hasattr(L, n), n
-    assert set(names) == set(pdh.EXPORTED)
+    assert set(names) | set(diag) == set(pdh.EXPORTED)
 
 
 def test_pi_header_declarations_are_exported():
     import pdh
     names = [n for n in declared(PI_HEADER) if n != "pdh_host_mul_fn"]
+    diag = declared_diag()[1]
+    assert diag == ["pdh_newton_corrections", "pdh_newton_set_perturb"]
     L = pdh.pilib()
-    for n in names:
+    for n in names + diag:
         assert hasattr(L, n), n
-    assert set(names) == set(pdh.PI_EXPORTED)
+    assert set(names) | set(diag) == set(pdh.PI_EXPORTED)
 
 
 def test_ntt_length_is_the_reference_rule():
@@ -138,3 +151,41 @@ def test_reference_programs_link_against_the_seam():
         assert os.path.exists(path), exe
         ldd = subprocess.run(["ldd", path], capture_output=True, text=True, check=True).stdout
         assert "libpdh_seam.so" in ldd and "libpdh.so" in ldd and "not found" not in ldd
+
+
+def test_shape_file_round_trip_and_rejects_bad_files(tmp_path):
+    """pdh_tune_save / pdh_tune_load need no device: shapes written are read
+    back, and a malformed file, an unlaunchable shape or a repeated size is
+    refused with nothing changed."""
+    import pdh
+    before = {n: pdh.tune_get(n) for n in range(1, 25)}
+    path = tmp_path / "shapes.txt"
+    pdh.tune_save(str(path))
+    lines = path.read_text().splitlines()
+    assert lines[0] == "pdh-shapes 1 gfx950" and len(lines) == 25
+    orig = path.read_text()
+    try:
+        pdh.tune_set(16, 5, 16, 3, 4)
+        assert pdh.tune_get(16) == (5, 16, 3, 4)
+        pdh.tune_load(str(path))
+        assert {n: pdh.tune_get(n) for n in range(1, 25)} == before
+        pdh.tune_set(16, 5, 16, 3, 4)
+        bad = {
+            "magic": orig.replace("pdh-shapes 1 gfx950", "pdh-shapes 2 gfx950"),
+            "unlaunchable": orig.replace(lines[16], "16 1 5 10 7 3 4 1"),
+            "repeated": orig + lines[16] + "\n",
+            "trailing": orig.replace(lines[16], lines[16] + " 9"),
+            "custom flag": orig.replace(lines[16], lines[16][:-1] + "2"),
+        }
+        for why, text in bad.items():
+            p = tmp_path / "bad.txt"
+            p.write_text(text)
+            with pytest.raises(pdh.PdhError) as e:
+                pdh.tune_load(str(p))
+            assert e.value.code == pdh.PDH_E_ARG, why
+            assert pdh.tune_get(16) == (5, 16, 3, 4), why
+        with pytest.raises(pdh.PdhError):
+            pdh.tune_load(str(tmp_path / "missing.txt"))
+    finally:
+        pdh.tune_load(str(path))
+    assert {n: pdh.tune_get(n) for n in range(1, 25)} == before

@@ -110,9 +110,13 @@ def test_range_blocks_equal_per_position(pdh, golden, start, count):
 
 
 def test_bad_positions_raise(pdh):
+    """Positions outside [1, 2^45] (moduli from 2^48 on, where the
+    reference's bbp_mulmod stops being exact): the host entry points reject
+    them before any device work; the device-positions entry point computes
+    the valid ones, writes 0xFF for the others and reports PDH_E_ARG."""
     import torch
     dv = torch.zeros(16, dtype=torch.uint8, device="cuda")
-    for start in ((1 << 28) - 5, 0):
+    for start in ((1 << 45) - 5, 0):
         with pytest.raises(pdh.PdhError, match="outside"):
             pdh.bbp_range_dev(start, 10, dv.data_ptr())
     with pytest.raises(pdh.PdhError, match="null"):
@@ -122,7 +126,13 @@ def test_bad_positions_raise(pdh):
     with pytest.raises(pdh.PdhError):
         pdh.bbp_values_range(0, 4)
     with pytest.raises(pdh.PdhError):
-        pdh.bbp_values_positions([(1 << 28) + 1])
+        pdh.bbp_values_positions([(1 << 45) + 1])
+    pos = torch.tensor([1, 0, 2, (1 << 45) + 1], dtype=torch.int64, device="cuda")
+    out = torch.zeros(4, dtype=torch.uint8, device="cuda")
+    with pytest.raises(pdh.PdhError, match="outside"):
+        pdh.bbp_positions_dev(pos.data_ptr(), 4, out.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    assert out.cpu().tolist() == [2, 0xFF, 4, 0xFF]
 
 
 @pytest.fixture

@@ -49,13 +49,13 @@ def shapes(n, eng):
 
 
 @pytest.fixture
-def restore_shapes(pdh):
-    saved = {n: (pdh.tune_engine(n), pdh.tune_get(n)) for n in range(2, 25)}
+def restore_shapes(pdh, tmp_path):
+    """Every shape and whether it was set by hand (a hand-set shape is also
+    used for batches) as they were before the test."""
+    path = str(tmp_path / "shapes.txt")
+    pdh.tune_save(path)
     yield
-    for n, (eng, (L1, T, lc, lr)) in saved.items():
-        if n > 13:
-            pdh.tune_engine(n, eng)
-        pdh.tune_set(n, L1, T, lc, lr)
+    pdh.tune_load(path)
 
 
 CASES = [(5, 0), (12, 0), (13, 0), (14, 0), (14, 1), (15, 1), (17, 0), (17, 1), (19, 0), (19, 1)]
@@ -122,8 +122,15 @@ def test_engines_agree(pdh, restore_shapes):
             assert np.array_equal(outs[1][0], oracle.mul_ntt(a, b)), n
 
 
-def test_autotune_keeps_parity(pdh, restore_shapes):
+def test_autotune_keeps_parity(pdh, restore_shapes, tmp_path):
     pdh.autotune(13, 16, 1)
+    tuned = {n: pdh.tune_get(n) for n in range(2, 25)}
+    path = str(tmp_path / "tuned.txt")
+    pdh.tune_save(path)                                 # a later process loads the tuned shapes
+    pdh.tune_set(14, *pdh.tune_get(14))
+    pdh.tune_set(16, 8, 4, 3, 3)
+    pdh.tune_load(path)
+    assert {n: pdh.tune_get(n) for n in range(2, 25)} == tuned
     for n in (13, 14, 16):
         N = 1 << n
         a, b = W.operand(700 + n, N // 4), W.operand(800 + n, N // 4)
