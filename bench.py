@@ -97,7 +97,9 @@ def cpu_baseline(a, b, W=None, extras=True):
     fn(a, b)
     dt = time.perf_counter() - t
     out = {"value": 1.0 / dt, "unit": "mul/s", "cores": 1, "kind": kind,
-           "sample": f"one 2^22 x 2^22-limb multiply, single-threaded ntt-cpu, {dt:.2f} s on {cpu_model()}"}
+           "sample": f"one 2^22 x 2^22-limb multiply, single-threaded ntt-cpu, {dt:.2f} s on {cpu_model()}",
+           "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+           "threads_used_by_throughput_legs": min(CPU_THREADS, os.cpu_count() or 1)}
     if not extras or W is None:
         return out
     import numpy as np
@@ -142,6 +144,175 @@ def read_traffic(kernel: str):
             except ValueError:
                 return None
     return None
+
+
+def read_pmc(profile: str, kernel: str, counters):
+    """Per-dispatch values of `counters` for `kernel` from a committed
+    rocprofv3 --pmc summary in profiles/ (None where absent)."""
+    path = os.path.join(ROOT, "profiles", profile)
+    if not os.path.exists(path):
+        return None
+    vals, cur = {}, None
+    for line in open(path):
+        if line.startswith("#") or not line.strip():
+            continue
+        if not line.startswith(" "):
+            cur = line.strip()
+            continue
+        parts = line.split()
+        if cur and cur.startswith(kernel) and len(parts) == 2 and parts[0] in counters:
+            vals[parts[0]] = float(parts[1])
+    return vals if all(c in vals for c in counters) else None
+
+
+_T0 = time.perf_counter()
+
+
+def progress(leg: str) -> None:
+    """One line on stderr as each leg ends (the JSON line stays alone on stdout)."""
+    print(f"bench: {leg} done at {time.perf_counter() - _T0:.1f} s", file=sys.stderr, flush=True)
+
+
+# VALU issue peak of MI355X: 256 CUs x 4 SIMDs x 16 lanes at 2.4 GHz (a wave64
+# VALU instruction occupies a SIMD for four cycles), in lane operations/s
+VALU_PEAK = 256 * 4 * 16 * 2.4e9
+
+
+def bench_bbp(pdh, torch, W, golden, stream, dev, big=True, small_only=False):
+    """The BBP legs (C4): positions 1..65,536 position by position and by
+    digit blocks, the 64 golden sampled positions, 512 and 16,384 sorted
+    positions of the same draw (seed 401, uniform in [1e6, 1e8]), with the
+    reference's bbp_mulmod count as the work unit. small_only: just the 64
+    and 512 sampled positions (one dispatch of each term kernel, for a
+    counter profile)."""
+    import numpy as np
+    out = {}
+    sp = stream.cuda_stream
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    if not small_only:
+        bbp_range_legs(pdh, torch, W, golden, stream, dev, out)
+    bbp_sampled_legs(pdh, torch, W, golden, stream, dev, out, big and not small_only)
+    return out
+
+
+def bbp_range_legs(pdh, torch, W, golden, stream, dev, out):
+    import numpy as np
+    sp = stream.cuda_stream
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    # BBP positions 1..65536
+    lo, hi = W.C4_RANGE
+    cnt = hi - lo + 1
+    vals = torch.zeros(cnt, dtype=torch.uint8, device=dev)
+    pos = torch.arange(lo, hi + 1, dtype=torch.int64, device=dev)
+    pdh.bbp_positions_dev(pos.data_ptr(), 64, vals.data_ptr(), sp)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    pdh.bbp_positions_dev(pos.data_ptr(), cnt, vals.data_ptr(), sp)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    bms = e0.elapsed_time(e1)
+    want = open(os.path.join(ROOT, "tests", "golden", golden["bbp_range"]["file"])).read().strip()
+    M = W.bbp_mulmod_count(np.arange(lo, hi + 1, dtype=np.uint64))
+    out["bbp_1_65536"] = {"ms": bms, "mulmod_per_s": M / (bms * 1e-3), "mulmods": M,
+                            "bit_exact": pdh.to_hex(vals.cpu().numpy()) == want}
+    # the same range through the range entry point (digit blocks)
+    rvals = torch.zeros(cnt, dtype=torch.uint8, device=dev)
+    pdh.bbp_range_dev(lo, 64, rvals.data_ptr(), sp)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    pdh.bbp_range_dev(lo, cnt, rvals.data_ptr(), sp)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    rms = e0.elapsed_time(e1)
+    out["bbp_range_blocks_1_65536"] = {
+        "ms": rms, "entry": "pdh_bbp_range_dev", "digits_per_evaluation": 7,
+        "note": "one fixed-point evaluation per 7 positions, the rest by the per-position path; "
+                "not the per-position workload above",
+        "bit_exact": pdh.to_hex(rvals.cpu().numpy()) == want}
+
+
+def bbp_sampled_legs(pdh, torch, W, golden, stream, dev, out, big):
+    import numpy as np
+    sp = stream.cuda_stream
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    # BBP at the 64 golden positions sampled uniformly in [1e6, 1e8]
+    gs = golden["bbp_sampled"]
+    spos = np.array(sorted(gs["positions"]), dtype=np.uint64)
+    order = np.argsort(np.array(gs["positions"], dtype=np.uint64), kind="stable")
+    dpos = torch.from_numpy(spos.view(np.int64)).to(dev)
+    svals = torch.zeros(len(spos), dtype=torch.uint8, device=dev)
+    fb0 = pdh.bbp_exact_count()
+    e0.record(stream)
+    pdh.bbp_positions_dev(dpos.data_ptr(), len(spos), svals.data_ptr(), sp)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    sms = e0.elapsed_time(e1)
+    got = pdh.to_hex(svals.cpu().numpy())
+    want_sorted = "".join(gs["hex"][i] for i in order)
+    Ms = W.bbp_mulmod_count(spos)
+    M20 = W.bbp_mulmod_count(W.sampled_positions(gs["seed"], 1 << 20, gs["lo"], gs["hi"]))
+    out["bbp_sampled_64"] = {"ms": sms, "mulmod_per_s": Ms / (sms * 1e-3), "mulmods": Ms,
+                               "positions": "64 uniform in [1e6, 1e8], seed 401",
+                               "fallback_positions": pdh.bbp_exact_count() - fb0,
+                               "extrapolated_2p20_positions_s": M20 / (Ms / (sms * 1e-3)),
+                               "bit_exact": got == want_sorted}
+    # 512 positions of the same draw, sorted: groups of 64 close enough for
+    # positions in lanes (the layout 2^20 positions would take)
+    p512 = np.sort(W.sampled_positions(gs["seed"], 512, gs["lo"], gs["hi"]))
+    d512 = torch.from_numpy(p512.view(np.int64)).to(dev)
+    v512 = torch.zeros(len(p512), dtype=torch.uint8, device=dev)
+    e0.record(stream)
+    pdh.bbp_positions_dev(d512.data_ptr(), len(p512), v512.data_ptr(), sp)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms512 = e0.elapsed_time(e1)
+    M512 = W.bbp_mulmod_count(p512)
+    with open(os.path.join(ROOT, "tests", "golden", "bbp_sampled_4096.json")) as f:
+        g4096 = json.load(f)
+    ref512 = {int(p): h for p, h in zip(g4096["positions"], g4096["hex"])}
+    h512 = pdh.to_hex(v512.cpu().numpy())
+    out["bbp_sampled_512"] = {"ms": ms512, "mulmod_per_s": M512 / (ms512 * 1e-3), "mulmods": M512,
+                                "positions": "512 uniform in [1e6, 1e8], seed 401, sorted",
+                                "extrapolated_2p20_positions_s": M20 / (M512 / (ms512 * 1e-3)),
+                                "bit_exact": all(ref512.get(int(p)) == h for p, h in zip(p512, h512))}
+    # 16,384 sorted positions of the same draw (about 15 s): checks the
+    # extrapolation to 2^20 positions on a larger sample; the 4,096 that are
+    # in the fixture are checked digit for digit
+    if big:
+        p16k = np.sort(W.sampled_positions(gs["seed"], 16384, gs["lo"], gs["hi"]))
+        d16k = torch.from_numpy(p16k.view(np.int64)).to(dev)
+        v16k = torch.zeros(len(p16k), dtype=torch.uint8, device=dev)
+        e0.record(stream)
+        pdh.bbp_positions_dev(d16k.data_ptr(), len(p16k), v16k.data_ptr(), sp)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms16k = e0.elapsed_time(e1)
+        M16k = W.bbp_mulmod_count(p16k)
+        h16k = pdh.to_hex(v16k.cpu().numpy())
+        ref4k = {int(p): h for p, h in zip(g4096["positions"], g4096["hex"])}
+        checked = [(ref4k[int(p)], h) for p, h in zip(p16k, h16k) if int(p) in ref4k]
+        out["bbp_sampled_16384"] = {"ms": ms16k, "mulmod_per_s": M16k / (ms16k * 1e-3), "mulmods": M16k,
+                                    "positions": "16,384 uniform in [1e6, 1e8], seed 401, sorted",
+                                    "extrapolated_2p20_positions_s": M20 / (M16k / (ms16k * 1e-3)),
+                                    "checked_vs_fixture": len(checked),
+                                    "bit_exact": all(a == b for a, b in checked)}
+    # the bound is VALU issue: each term kernel's VALU wave-instructions per
+    # dispatch (rocprofv3 SQ counters over this same workload: the 64 and the
+    # 512 sampled positions, profiles/pmc_sq_bbp.txt), 64 lane operations
+    # each, over the leg's measured time, against the VALU issue peak
+    for kern, leg in (("k_bbp_fx_terms", "bbp_sampled_64"), ("k_bbp_fx_lanes", "bbp_sampled_512")):
+        v = read_pmc("pmc_sq_bbp.txt", kern, ("SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_WAVE_CYCLES", "SQ_WAIT_ANY"))
+        if v and leg in out:
+            lane_ops = v["SQ_INSTS_VALU"] * 64
+            t = out[leg]["ms"] * 1e-3
+            out["bbp_valu_" + kern] = {
+                "bound": "valu", "workload": leg, "achieved": lane_ops / t / 1e12, "peak": VALU_PEAK / 1e12,
+                "unit": "T lane-op/s", "frac": lane_ops / t / VALU_PEAK,
+                "valu_instr_per_mulmod": lane_ops / out[leg]["mulmods"],
+                "salu_per_valu": v["SQ_INSTS_SALU"] / v["SQ_INSTS_VALU"],
+                "wait_frac": v["SQ_WAIT_ANY"] / v["SQ_WAVE_CYCLES"],
+                "source": "profiles/pmc_sq_bbp.txt"}
+    return out
 
 
 def bench_batch(pdh, torch, W, stream, reps=3, pairs=None, reps_plain=None):
@@ -286,6 +457,9 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving traces (about a minute)")
+    ap.add_argument("--bbp-only", action="store_true",
+                    help="run only the 64 and 512 sampled BBP positions (one dispatch of each term kernel) and "
+                         "print their record: the workload of the BBP counter profile")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 128 MiB-in, 128 MiB-out negate kernel (torch) so a --pmc profile can "
                          "scale FETCH_SIZE/WRITE_SIZE by a kernel of known traffic")
@@ -317,6 +491,10 @@ def main() -> int:
     ln = torch.zeros(1, dtype=torch.int64, device=dev)
     stream = torch.cuda.current_stream()
     sp = stream.cuda_stream
+    if args.bbp_only:
+        rec = bench_bbp(pdh, torch, W, golden, stream, dev, small_only=True)
+        print(json.dumps({"bbp_only": rec}))
+        return 0 if all(v.get("bit_exact", True) for v in rec.values()) else 3
 
     def step():
         pdh.mul_dev(da.data_ptr(), na, db.data_ptr(), nb, out.data_ptr(), ln.data_ptr(), sp)
@@ -346,6 +524,7 @@ def main() -> int:
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     total_ms = e0.elapsed_time(e1)
+    progress("timed multiplies")
     ms = total_ms / args.steps
 
     n = int(ln.item())
@@ -372,10 +551,18 @@ def main() -> int:
                 "unit": "GB/s", "frac": dom_B / dom_avg_s / HBM_PEAK, "traffic": read_traffic(dom),
                 "alg_bytes_per_launch": dom_B,
                 # the modular butterflies are integer VALU work: the kernel moves
-                # its algorithmic bytes once and waits on the VALU, not on HBM
-                "note": "integer-VALU-bound: HBM bytes equal the algorithmic bytes "
-                        "(profiles/pmc_traffic_c2_mul.txt); VALU busy 64% with ~98% of its "
-                        "instructions modular arithmetic (profiles/pmc_sq_c2_mul.txt)"}
+                # its algorithmic bytes about once and waits on the VALU (roofline_valu)
+                "note": "integer-VALU-bound; traffic is the measured HBM bytes per launch "
+                        "(profiles/pmc_traffic_c2_mul.txt), the VALU figure is roofline_valu"}
+    # the same kernel against the VALU issue peak: its wave instructions per
+    # launch from the committed SQ counter profile, 64 lane operations each
+    pv = read_pmc("pmc_sq_c2_mul.txt", dom, ("SQ_INSTS_VALU",))
+    roofline_valu = None
+    if pv:
+        lane_ops = pv["SQ_INSTS_VALU"] * 64
+        roofline_valu = {"bound": "valu", "kernel": dom, "achieved": lane_ops / dom_avg_s / 1e12, "peak": VALU_PEAK / 1e12,
+                         "unit": "T lane-op/s", "frac": lane_ops / dom_avg_s / VALU_PEAK,
+                         "valu_instr_per_launch": pv["SQ_INSTS_VALU"], "source": "profiles/pmc_sq_c2_mul.txt"}
     B_mul = 96 * N + 8 * (na + nb)
     roofline_e2e = {"bound": "hbm", "alg_bytes": B_mul, "achieved": B_mul / (ms * 1e-3) / 1e9,
                     "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": B_mul / (ms * 1e-3) / HBM_PEAK}
@@ -399,6 +586,7 @@ def main() -> int:
         nsq = int(ln.item())
         gsq = osq[:nsq].cpu().numpy().view(np.uint32)
         B_sqr = 64 * N + 12 * len(asq)
+        progress("square")
         extra["c2_square"] = {"ms": ms_sq, "sqr_per_s": 1e3 / ms_sq, "alg_bytes": B_sqr,
                               "roofline_frac": B_sqr / (ms_sq * 1e-3) / HBM_PEAK,
                               "bit_exact": nsq == gs["len"] and hashlib.sha256(
@@ -421,94 +609,20 @@ def main() -> int:
         torch.cuda.synchronize()
         n1 = int(ln.item())
         g1 = o1[:n1].cpu().numpy().view(np.uint32)
+        progress("60,000-limb multiply")
         extra["c1_60000_limbs"] = {
             "ms_per_mul": e0.elapsed_time(e1) / reps,
             "bit_exact": n1 == c1["len"] and hashlib.sha256(g1.astype("<u4").tobytes()).hexdigest() == c1["sha256"]}
-        # BBP positions 1..65536
-        lo, hi = W.C4_RANGE
-        cnt = hi - lo + 1
-        vals = torch.zeros(cnt, dtype=torch.uint8, device=dev)
-        pos = torch.arange(lo, hi + 1, dtype=torch.int64, device=dev)
-        pdh.bbp_positions_dev(pos.data_ptr(), 64, vals.data_ptr(), sp)
-        torch.cuda.synchronize()
-        e0.record(stream)
-        pdh.bbp_positions_dev(pos.data_ptr(), cnt, vals.data_ptr(), sp)
-        e1.record(stream)
-        torch.cuda.synchronize()
-        bms = e0.elapsed_time(e1)
-        want = open(os.path.join(ROOT, "tests", "golden", golden["bbp_range"]["file"])).read().strip()
-        M = W.bbp_mulmod_count(np.arange(lo, hi + 1, dtype=np.uint64))
-        extra["bbp_1_65536"] = {"ms": bms, "mulmod_per_s": M / (bms * 1e-3), "mulmods": M,
-                                "bit_exact": pdh.to_hex(vals.cpu().numpy()) == want}
-        # the same range through the range entry point (digit blocks)
-        rvals = torch.zeros(cnt, dtype=torch.uint8, device=dev)
-        pdh.bbp_range_dev(lo, 64, rvals.data_ptr(), sp)
-        torch.cuda.synchronize()
-        e0.record(stream)
-        pdh.bbp_range_dev(lo, cnt, rvals.data_ptr(), sp)
-        e1.record(stream)
-        torch.cuda.synchronize()
-        rms = e0.elapsed_time(e1)
-        extra["bbp_range_blocks_1_65536"] = {
-            "ms": rms, "entry": "pdh_bbp_range_dev", "digits_per_evaluation": 7,
-            "note": "one fixed-point evaluation per 7 positions, the rest by the per-position path; "
-                    "not the per-position workload above",
-            "bit_exact": pdh.to_hex(rvals.cpu().numpy()) == want}
-        # BBP at the 64 golden positions sampled uniformly in [1e6, 1e8]
-        gs = golden["bbp_sampled"]
-        spos = np.array(sorted(gs["positions"]), dtype=np.uint64)
-        order = np.argsort(np.array(gs["positions"], dtype=np.uint64), kind="stable")
-        dpos = torch.from_numpy(spos.view(np.int64)).to(dev)
-        svals = torch.zeros(len(spos), dtype=torch.uint8, device=dev)
-        fb0 = pdh.bbp_exact_count()
-        e0.record(stream)
-        pdh.bbp_positions_dev(dpos.data_ptr(), len(spos), svals.data_ptr(), sp)
-        e1.record(stream)
-        torch.cuda.synchronize()
-        sms = e0.elapsed_time(e1)
-        got = pdh.to_hex(svals.cpu().numpy())
-        want_sorted = "".join(gs["hex"][i] for i in order)
-        Ms = W.bbp_mulmod_count(spos)
-        M20 = W.bbp_mulmod_count(W.sampled_positions(gs["seed"], 1 << 20, gs["lo"], gs["hi"]))
-        extra["bbp_sampled_64"] = {"ms": sms, "mulmod_per_s": Ms / (sms * 1e-3), "mulmods": Ms,
-                                   "positions": "64 uniform in [1e6, 1e8], seed 401",
-                                   "fallback_positions": pdh.bbp_exact_count() - fb0,
-                                   "extrapolated_2p20_positions_s": M20 / (Ms / (sms * 1e-3)),
-                                   "bit_exact": got == want_sorted}
-        # 512 positions of the same draw, sorted: groups of 64 close enough for
-        # positions in lanes (the layout 2^20 positions would take)
-        p512 = np.sort(W.sampled_positions(gs["seed"], 512, gs["lo"], gs["hi"]))
-        d512 = torch.from_numpy(p512.view(np.int64)).to(dev)
-        v512 = torch.zeros(len(p512), dtype=torch.uint8, device=dev)
-        e0.record(stream)
-        pdh.bbp_positions_dev(d512.data_ptr(), len(p512), v512.data_ptr(), sp)
-        e1.record(stream)
-        torch.cuda.synchronize()
-        ms512 = e0.elapsed_time(e1)
-        M512 = W.bbp_mulmod_count(p512)
-        with open(os.path.join(ROOT, "tests", "golden", "bbp_sampled_4096.json")) as f:
-            g4096 = json.load(f)
-        ref512 = {int(p): h for p, h in zip(g4096["positions"], g4096["hex"])}
-        h512 = pdh.to_hex(v512.cpu().numpy())
-        extra["bbp_sampled_512"] = {"ms": ms512, "mulmod_per_s": M512 / (ms512 * 1e-3), "mulmods": M512,
-                                    "positions": "512 uniform in [1e6, 1e8], seed 401, sorted",
-                                    "extrapolated_2p20_positions_s": M20 / (M512 / (ms512 * 1e-3)),
-                                    "bit_exact": all(ref512.get(int(p)) == h for p, h in zip(p512, h512))}
-        # the integer-multiply ceiling: the measured v_mad_u64_u32 issue rate,
-        # at four multiply-class instructions per 32-bit-modulus mulmod
-        rate = pdh.probe_mad_rate()
-        ceiling = rate / 4
-        extra["bbp_ceiling"] = {"mad_u64_u32_lane_ops_per_s": rate, "mulmod_per_s": ceiling,
-                                "basis": "measured v_mad_u64_u32 rate / 4 multiply-class ops per mulmod"}
-        for k in ("bbp_1_65536", "bbp_sampled_64", "bbp_sampled_512"):
-            extra[k]["ceiling_frac"] = extra[k]["mulmod_per_s"] / ceiling
-
+        extra.update(bench_bbp(pdh, torch, W, golden, stream, dev))
+        progress("BBP")
         extra["c3_batch_4096"] = bench_batch(pdh, torch, W, stream, reps=3)
+        progress("4,096-pair batch")
         # the binary-splitting products of a 10^7-digit run at tree depths 8..14
         bs = W.bs_level_pairs(7, int(pdh.pilib().pdh_chudnovsky_terms(10**7)))
         r = bench_batch(pdh, torch, W, stream, reps=3, pairs=bs, reps_plain=1)
         r["levels"] = "depths 8-14 of the 10^7-digit tree, 130,048 products of 10..4,862 limbs"
         extra["c3_bs_levels_1e7"] = r
+        progress("binary-splitting levels")
         # Chudnovsky with every multiply on the device
         g7 = golden.get("pi_dec_sha256", {})
         # (10^8: beyond the fixtures, checked by its 10^6 and 10^7 prefixes --
@@ -527,15 +641,18 @@ def main() -> int:
                         hashlib.sha256(s_pi[2:2 + p].encode()).hexdigest() == g7[str(p)]["sha256"])
             extra[f"chudnovsky_{digits}"] = rec
             del s_pi
+            progress(f"Chudnovsky {digits} digits")
         if not args.no_serve:
             # the mix arriving faster than a serial loop can serve it (its mean
             # service time is about 5 ms): requests queue, the server overlaps
             # and groups them, the serial loop cannot
             extra["c5_serve"] = bench_serve(pdh, W, n_req=240, rate=250.0)
+            progress("serving trace")
             # a light load (arrival-bound for both)
             extra["c5_serve_light"] = bench_serve(pdh, W, n_req=60, rate=20.0)
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(a, b, W, extras=not args.no_extra)
+    progress("CPU baseline")
     line = {
         "metric": METRIC, "value": 1e3 / ms, "unit": "mul/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
@@ -545,7 +662,7 @@ def main() -> int:
                    "engine": "three primes, 2^23-point transforms" if eng == 1 else "Goldilocks, 2^24-point transforms",
                    "shape": dict(zip(("L1", "T", "lr_col", "lr_row"), pdh.tune_get(N.bit_length() - 1)))},
         "bit_exact": exact, "wall_s_timed": wall,
-        "roofline": roofline, "roofline_e2e": roofline_e2e, "kernels": kernels,
+        "roofline": roofline, "roofline_valu": roofline_valu, "roofline_e2e": roofline_e2e, "kernels": kernels,
         "cpu_baseline": cpu, "extra": extra,
     }
     print(json.dumps(line))
