@@ -483,6 +483,7 @@ struct RowPlan {
     static constexpr int WG = TPR < 256 ? 256 : TPR;    // threads per workgroup
     static constexpr int G = WG / TPR;                  // rows per workgroup
     static constexpr int REM = L2 % 4;                  // levels of the last, partial set
+    static constexpr int WORDS = (1 << L2) + (1 << (L2 - 4));   // one pad word per 16 (RowLay)
     static constexpr int NSET = L2 / 4 + (REM ? 1 : 0);
     // bottom bit of set s: L2 - 4 (s + 1) for the full sets, 0 for the partial one
     static constexpr int bottom(int s) { return (REM && s == NSET - 1) ? 0 : L2 - 4 * (s + 1); }
@@ -492,22 +493,48 @@ struct RowPlan {
 // Positions of set B of radix 2^LQ in a sub-transform whose element i sits at
 // LDS word lpad((i << LOGT) | col): i(t) = ((u >> B) << (B + LQ)) | (t << B) |
 // (u mod 2^B). The offsets of t > 0 from t = 0 are compile-time constants.
-template <int B, int LQ, int LOGT>
+// LDS layouts: word e at e + K (e >> A), K pad words per 2^A words. The map
+// is additive over bit-disjoint parts, so a set's offsets from its first
+// word are compile-time constants. (A, K) = (3, 1) is the layout of the
+// column kernels.
+template <int A, int K> __device__ __forceinline__ int padak(int e) { return e + K * (e >> A); }
+
+template <int B, int LQ, int LOGT, int A = 3, int K = 1>
 __device__ __forceinline__ int set_base(int u, int col) {
-    return lpad(((((u >> B) << (B + LQ)) | (u & ((1 << B) - 1))) << LOGT) | col);
+    return padak<A, K>(((((u >> B) << (B + LQ)) | (u & ((1 << B) - 1))) << LOGT) | col);
 }
 
-template <int B, int LQ, int LOGT>
+template <int B, int LQ, int LOGT, int A = 3, int K = 1>
 __device__ __forceinline__ void set_store(u32* __restrict__ l, int a0, const u32* x) {
 #pragma unroll
-    for (int t = 0; t < (1 << LQ); ++t) l[a0 + pad_step((t << B) << LOGT)] = x[t];
+    for (int t = 0; t < (1 << LQ); ++t) l[a0 + padak<A, K>((t << B) << LOGT)] = x[t];
 }
 
-template <int B, int LQ, int LOGT>
+template <int B, int LQ, int LOGT, int A = 3, int K = 1>
 __device__ __forceinline__ void set_load(const u32* __restrict__ l, int a0, u32* x) {
 #pragma unroll
-    for (int t = 0; t < (1 << LQ); ++t) x[t] = l[a0 + pad_step((t << B) << LOGT)];
+    for (int t = 0; t < (1 << LQ); ++t) x[t] = l[a0 + padak<A, K>((t << B) << LOGT)];
 }
+
+// The row kernel's layout for the exchange between sets with bottoms B0 and
+// B1 (either order). Every layout has one pad word per 16 on average, so a
+// wavefront's block of 1,024 points always occupies the same 1,088 words
+// (the wavefront-local exchanges rely on that); within the block the pads
+// sit where they keep both sides of the exchange free of bank conflicts: a
+// set's access for one t touches 64 words per wavefront, and with the
+// exchange's layout they fall in 64 distinct banks (and each half-wavefront's
+// 32 in 32 distinct banks). Lane bits map to point bits 0-5 for bottoms >= 6,
+// to 0..B-1 and B+4..9 for bottoms B < 6; four pad words per 64 suit every
+// pair whose lower bottom is at least 2, two per 32 a lower bottom of 1, one
+// per 16 a lower bottom of 0 (the partial set). A single layout cannot serve
+// all pairs: one pad per 8 words, the column kernels' layout, leaves every
+// row access 2-way conflicted.
+template <int B0, int B1>
+struct RowLay {
+    static constexpr int LO = B0 < B1 ? B0 : B1;
+    static constexpr int A = LO == 0 ? 4 : LO == 1 ? 5 : 6;
+    static constexpr int K = 1 << (A - 4);
+};
 
 // Twiddles of set S's radix-16 step: w_M^(j c), M = 2^(B + 4), j = u mod 2^B.
 template <int L2, int S, bool INV>
@@ -572,12 +599,14 @@ __device__ __forceinline__ void row_fwd(u32 (*x)[16], const u32* w, int u, u32* 
         // the next set's twiddles are in flight during the exchange
         u32 wn[16];
         set_tw<L2, S + 1, false>(wn, u, tw);
-        const int a0 = set_base<RP::bottom(S), 4, 0>(u, 0), a1 = set_base<RP::bottom(S + 1), 4, 0>(u, 0);
+        using XL = RowLay<RP::bottom(S), RP::bottom(S + 1)>;
+        const int a0 = set_base<RP::bottom(S), 4, 0, XL::A, XL::K>(u, 0);
+        const int a1 = set_base<RP::bottom(S + 1), 4, 0, XL::A, XL::K>(u, 0);
 #pragma unroll
-        for (int k = 0; k < NOP; ++k) set_store<RP::bottom(S), 4, 0>(l[k], a0, x[k]);
+        for (int k = 0; k < NOP; ++k) set_store<RP::bottom(S), 4, 0, XL::A, XL::K>(l[k], a0, x[k]);
         row_exchange_sync<L2, RP::bottom(S), RP::bottom(S + 1)>();
 #pragma unroll
-        for (int k = 0; k < NOP; ++k) set_load<RP::bottom(S + 1), 4, 0>(l[k], a1, x[k]);
+        for (int k = 0; k < NOP; ++k) set_load<RP::bottom(S + 1), 4, 0, XL::A, XL::K>(l[k], a1, x[k]);
         row_fwd<L2, NOP, S + 1>(x, wn, u, l, tw, P);
     }
 }
@@ -591,9 +620,10 @@ __device__ __forceinline__ void row_inv(u32* x, const u32* w, int u, u32* __rest
     if constexpr (S > 0) {
         u32 wn[16];
         set_tw<L2, S - 1, true>(wn, u, tw);
-        set_store<RP::bottom(S), 4, 0>(l, set_base<RP::bottom(S), 4, 0>(u, 0), x);
+        using XL = RowLay<RP::bottom(S), RP::bottom(S - 1)>;
+        set_store<RP::bottom(S), 4, 0, XL::A, XL::K>(l, set_base<RP::bottom(S), 4, 0, XL::A, XL::K>(u, 0), x);
         row_exchange_sync<L2, RP::bottom(S), RP::bottom(S - 1)>();
-        set_load<RP::bottom(S - 1), 4, 0>(l, set_base<RP::bottom(S - 1), 4, 0>(u, 0), x);
+        set_load<RP::bottom(S - 1), 4, 0, XL::A, XL::K>(l, set_base<RP::bottom(S - 1), 4, 0, XL::A, XL::K>(u, 0), x);
         row_inv<L2, S - 1>(x, wn, u, l, tw, P);
     }
 }
@@ -610,7 +640,7 @@ __global__ __launch_bounds__(RowPlan<L2>::WG) void k3_row_reg(const MulJob* __re
     const int n3 = L1 + L2;
     const int g = threadIdx.x / TPR;
     const int u = threadIdx.x % TPR;
-    u32* l[2] = {lds3 + g * lds_words(R), lds3 + (RP::G + g) * lds_words(R)};
+    u32* l[2] = {lds3 + g * RP::WORDS, lds3 + (RP::G + g) * RP::WORDS};
     const unsigned wgs = (1u << L1) / RP::G;                       // workgroups per prime
     const unsigned job = blockIdx.x / (3u * wgs);
     const int pi = (int)((blockIdx.x / wgs) % 3u);
@@ -747,6 +777,12 @@ struct ColPlan {
     static constexpr int NSET = L1 / 3 + (REM ? 1 : 0);
     static constexpr bool partial(int s) { return REM && s == 0; }
     static constexpr int bottom(int s) { return REM ? (s == 0 ? L1 - 3 : L1 - REM - 3 * s) : L1 - 3 * (s + 1); }
+    // LDS layout of the tile (element (i, col) is word (i << LOGT) | col):
+    // one pad word per 8. (Eight per 64 make every set access free of bank
+    // conflicts, but the inverse pass's carry, one row segment of T words
+    // per thread, then conflicts 8-way: 79 against 71 us at 2^9 x 16, and
+    // the forward pass does not get faster.)
+    static constexpr int PA = 3, PK = 1;
 };
 
 // Tiles the register column kernels are built for: at least one wave, the
@@ -796,12 +832,13 @@ __device__ __forceinline__ void col_fwd3(u32 (*x)[8], int u, int col, u32* const
     for (int p = 0; p < 3; ++p) col_step<L1, T, S, false>(x[p], u, tw0 + (size_t)p * TW3_STRIDE, PS.p[p]);
     if constexpr (S + 1 < CP::NSET) {
         constexpr int B0 = CP::bottom(S), B1 = CP::bottom(S + 1);
-        const int a0 = set_base<B0, 3, CP::LOGT>(u, col), a1 = set_base<B1, 3, CP::LOGT>(u, col);
+        const int a0 = set_base<B0, 3, CP::LOGT, CP::PA, CP::PK>(u, col);
+        const int a1 = set_base<B1, 3, CP::LOGT, CP::PA, CP::PK>(u, col);
 #pragma unroll
-        for (int p = 0; p < 3; ++p) set_store<B0, 3, CP::LOGT>(l[p], a0, x[p]);
+        for (int p = 0; p < 3; ++p) set_store<B0, 3, CP::LOGT, CP::PA, CP::PK>(l[p], a0, x[p]);
         __syncthreads();
 #pragma unroll
-        for (int p = 0; p < 3; ++p) set_load<B1, 3, CP::LOGT>(l[p], a1, x[p]);
+        for (int p = 0; p < 3; ++p) set_load<B1, 3, CP::LOGT, CP::PA, CP::PK>(l[p], a1, x[p]);
         col_fwd3<L1, T, S + 1>(x, u, col, l, tw0, PS);
     }
 }
@@ -814,12 +851,13 @@ __device__ __forceinline__ void col_inv3(u32 (*x)[8], int u, int col, u32* const
     for (int p = 0; p < 3; ++p) col_step<L1, T, S, true>(x[p], u, tw0 + (size_t)p * TW3_STRIDE, PS.p[p]);
     if constexpr (S > 0) {
         constexpr int B0 = CP::bottom(S), B1 = CP::bottom(S - 1);
-        const int a0 = set_base<B0, 3, CP::LOGT>(u, col), a1 = set_base<B1, 3, CP::LOGT>(u, col);
+        const int a0 = set_base<B0, 3, CP::LOGT, CP::PA, CP::PK>(u, col);
+        const int a1 = set_base<B1, 3, CP::LOGT, CP::PA, CP::PK>(u, col);
 #pragma unroll
-        for (int p = 0; p < 3; ++p) set_store<B0, 3, CP::LOGT>(l[p], a0, x[p]);
+        for (int p = 0; p < 3; ++p) set_store<B0, 3, CP::LOGT, CP::PA, CP::PK>(l[p], a0, x[p]);
         __syncthreads();
 #pragma unroll
-        for (int p = 0; p < 3; ++p) set_load<B1, 3, CP::LOGT>(l[p], a1, x[p]);
+        for (int p = 0; p < 3; ++p) set_load<B1, 3, CP::LOGT, CP::PA, CP::PK>(l[p], a1, x[p]);
         col_inv3<L1, T, S - 1>(x, u, col, l, tw0, PS);
     }
 }
@@ -889,10 +927,10 @@ __global__ __launch_bounds__((ColPlan<L1, T>::WG)) void k3_col_inv_reg(const Mul
     // at the words this thread read last
     {
         constexpr int B0 = CP::bottom(0);
-        const int a0 = set_base<B0, 3, CP::LOGT>(u, col);
+        const int a0 = set_base<B0, 3, CP::LOGT, CP::PA, CP::PK>(u, col);
 #pragma unroll
         for (int t = 0; t < 8; ++t) {
-            const int e = a0 + pad_step((t << B0) << CP::LOGT);
+            const int e = a0 + padak<CP::PA, CP::PK>((t << B0) << CP::LOGT);
             u32 lo32; u64 up;
             crt3(umin32(x[0][t], x[0][t] - PS.p[0].p), umin32(x[1][t], x[1][t] - PS.p[1].p),
                  umin32(x[2][t], x[2][t] - PS.p[2].p), PS, lo32, up);
@@ -914,7 +952,7 @@ __global__ __launch_bounds__((ColPlan<L1, T>::WG)) void k3_col_inv_reg(const Mul
         bool ones = true;
 #pragma unroll
         for (int q = 0; q < T; ++q) {
-            const int e = lpad(i * T + q);
+            const int e = padak<CP::PA, CP::PK>(i * T + q);
             const u64 up = (u64)l[1][e] | ((u64)l[2][e] << 32);
             const u64 s = (carry & 0xFFFFFFFFull) + l[0][e];
             l[0][e] = (u32)s;
@@ -933,7 +971,7 @@ __global__ __launch_bounds__((ColPlan<L1, T>::WG)) void k3_col_inv_reg(const Mul
     for (int k = 0; k < 8; ++k) {
         const int e = k * CP::WG + threadIdx.x;
         const size_t g = (size_t)(e / T) * S + b0 + (e & (T - 1));
-        if (g < C.nlimbs) C.out[g] = l[0][lpad(e)];
+        if (g < C.nlimbs) C.out[g] = l[0][padak<CP::PA, CP::PK>(e)];
     }
 }
 
@@ -943,7 +981,7 @@ __global__ __launch_bounds__((ColPlan<L1, T>::WG)) void k3_col_inv_reg(const Mul
 #define PDH3_FOR_COL_L1(X) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11)
 static size_t row_reg_lds(int L2, bool square) {
     const int tpr = 1 << (L2 - 4), wg = tpr < 256 ? 256 : tpr;
-    return (size_t)(square ? 1 : 2) * (wg / tpr) * lds_words(1 << L2) * 4;
+    return (size_t)(square ? 1 : 2) * (wg / tpr) * ((1 << L2) + (1 << (L2 - 4))) * 4;   // RowPlan::WORDS
 }
 
 template <class K>
