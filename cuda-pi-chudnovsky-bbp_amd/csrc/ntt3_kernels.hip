@@ -658,14 +658,28 @@ __global__ __launch_bounds__(RowPlan<L2>::WG) void k3_row_reg(const MulJob* __re
     const u64 c = freq_rt(row, L1, lr_col);
     const u64 N = (u64)1 << n3;
     u32 x[NOP][16];
-    // set 0 from global memory with the outer twiddle w^(b c), b = t R/16 + u
+    // Every load of the prologue is issued before any arithmetic, in the order
+    // the values are used (loads complete in issue order, so each use waits
+    // only for the loads issued before it): the outer-twiddle table entries,
+    // the row's points (set 0, b = t R/16 + u), set 0's level twiddles and
+    // the inverse's outer-twiddle entries, which would otherwise be fetched
+    // after the last exchange with nothing left to overlap them.
+    const u64 e0 = (c * (u64)u) & (N - 1), lmask = ((u64)1 << h) - 1;
+    const u32 fhi = gld(hi + (e0 >> h)), flo = gld(lo + (e0 & lmask));
+    sched_fence();
 #pragma unroll
     for (int t = 0; t < 16; ++t) {
-        x[0][t] = A[t * TPR + u];
-        if constexpr (NOP == 2) x[1][t] = Bv[t * TPR + u];
+        x[0][t] = gld(A + t * TPR + u);
+        if constexpr (NOP == 2) x[1][t] = gld(Bv + t * TPR + u);
     }
+    sched_fence();
+    u32 w0[16];
+    set_tw<L2, 0, false>(w0, u, tw);
+    const u32 ihi = gld(hi_inv + (e0 >> h)), ilo = gld(lo_inv + (e0 & lmask));
+    sched_fence();
     {
-        u32 w = r_tw(hi, lo, h, (c * (u64)u) & (N - 1), P);
+        // outer twiddle w^(b c) on the loaded points
+        u32 w = mm(fhi, flo, P.p, P.ninv);
         const u32 step = r_tw(hi, lo, h, (c * (u64)TPR) & (N - 1), P);
 #pragma unroll
         for (int t = 0; t < 16; ++t) {
@@ -674,11 +688,8 @@ __global__ __launch_bounds__(RowPlan<L2>::WG) void k3_row_reg(const MulJob* __re
             w = mml(w, step, P.p, P.ninv);
         }
     }
-    {
-        u32 w[16];
-        set_tw<L2, 0, false>(w, u, tw);
-        row_fwd<L2, NOP>(x, w, u, l, tw, P);
-    }
+    const u32 winv = mm(ihi, ilo, P.p, P.ninv);
+    row_fwd<L2, NOP>(x, w0, u, l, tw, P);
 #pragma unroll
     for (int t = 0; t < 16; ++t) x[0][t] = mml(x[0][t], x[NOP - 1][t], P.p, P.ninv);
     // the inverse's first exchange writes the positions this thread read last
@@ -687,11 +698,11 @@ __global__ __launch_bounds__(RowPlan<L2>::WG) void k3_row_reg(const MulJob* __re
         set_tw<L2, RP::NSET - 1, true>(w, u, tw);
         row_inv<L2>(x[0], w, u, l[0], tw, P);
     }
-    u32 w = r_tw(hi_inv, lo_inv, h, (c * (u64)u) & (N - 1), P);
+    u32 w = winv;
     const u32 step = r_tw(hi, lo, h, (N - c * (u64)TPR) & (N - 1), P);
 #pragma unroll
     for (int t = 0; t < 16; ++t) {
-        A[t * TPR + u] = mml(x[0][t], w, P.p, P.ninv);
+        gst(A + t * TPR + u, mml(x[0][t], w, P.p, P.ninv));
         w = mml(w, step, P.p, P.ninv);
     }
 }
@@ -792,8 +803,21 @@ constexpr bool col_reg_pair(int L1, int T) {
     return L1 >= 3 && L1 + lt >= 9 && L1 + lt <= 13;
 }
 
+// Level twiddles of a full radix-8 set S with bottom B > 0: w_M^(j c),
+// M = 2^(B + 3), j = u mod 2^B, c = 1..7 (the partial set loads its own).
 template <int L1, int T, int S, bool INV>
-__device__ __forceinline__ void col_step(u32* x, int u, const u32* __restrict__ tw, const Prime3& P) {
+__device__ __forceinline__ void col_tw(u32* w, int u, const u32* __restrict__ tw) {
+    using CP = ColPlan<L1, T>;
+    if constexpr (!CP::partial(S) && CP::bottom(S) > 0) {
+        constexpr int B = CP::bottom(S);
+        const uint4* tl = (const uint4*)(tw_level<8, INV>(tw, B + 3) + (u & ((1 << B) - 1)) * 8);
+        const uint4 v0 = tl[0], v1 = tl[1];
+        w[1] = v0.y; w[2] = v0.z; w[3] = v0.w; w[4] = v1.x; w[5] = v1.y; w[6] = v1.z; w[7] = v1.w;
+    }
+}
+
+template <int L1, int T, int S, bool INV>
+__device__ __forceinline__ void col_step(u32* x, const u32* w, int u, const u32* __restrict__ tw, const Prime3& P) {
     using CP = ColPlan<L1, T>;
     if constexpr (CP::partial(S)) {
         // radix Q = 2^REM over the top bits: t = q 2^(3 - REM) + tlo, so the
@@ -813,24 +837,22 @@ __device__ __forceinline__ void col_step(u32* x, int u, const u32* __restrict__ 
             for (int q = 0; q < Q; ++q) x[q * TL + tlo] = y[q];
         }
     } else {
-        constexpr int B = CP::bottom(S);
-        u32 w[8];
-        if constexpr (B > 0) {
-            const uint4* tl = (const uint4*)(tw_level<8, INV>(tw, B + 3) + (u & ((1 << B) - 1)) * 8);
-            const uint4 v0 = tl[0], v1 = tl[1];
-            w[1] = v0.y; w[2] = v0.z; w[3] = v0.w; w[4] = v1.x; w[5] = v1.y; w[6] = v1.z; w[7] = v1.w;
-        }
-        lazy_butterfly<8, INV>(x, w, B > 0, P);
+        lazy_butterfly<8, INV>(x, w, CP::bottom(S) > 0, P);
     }
 }
 
+// w holds set S's level twiddles per prime; the next set's are loaded
+// before the exchange, so their latency overlaps it.
 template <int L1, int T, int S = 0>
-__device__ __forceinline__ void col_fwd3(u32 (*x)[8], int u, int col, u32* const* l, const u32* __restrict__ tw0,
-                                         const Primes3& PS) {
+__device__ __forceinline__ void col_fwd3(u32 (*x)[8], const u32 (*w)[8], int u, int col, u32* const* l,
+                                         const u32* __restrict__ tw0, const Primes3& PS) {
     using CP = ColPlan<L1, T>;
 #pragma unroll
-    for (int p = 0; p < 3; ++p) col_step<L1, T, S, false>(x[p], u, tw0 + (size_t)p * TW3_STRIDE, PS.p[p]);
+    for (int p = 0; p < 3; ++p) col_step<L1, T, S, false>(x[p], w[p], u, tw0 + (size_t)p * TW3_STRIDE, PS.p[p]);
     if constexpr (S + 1 < CP::NSET) {
+        u32 wn[3][8];
+#pragma unroll
+        for (int p = 0; p < 3; ++p) col_tw<L1, T, S + 1, false>(wn[p], u, tw0 + (size_t)p * TW3_STRIDE);
         constexpr int B0 = CP::bottom(S), B1 = CP::bottom(S + 1);
         const int a0 = set_base<B0, 3, CP::LOGT, CP::PA, CP::PK>(u, col);
         const int a1 = set_base<B1, 3, CP::LOGT, CP::PA, CP::PK>(u, col);
@@ -839,17 +861,20 @@ __device__ __forceinline__ void col_fwd3(u32 (*x)[8], int u, int col, u32* const
         __syncthreads();
 #pragma unroll
         for (int p = 0; p < 3; ++p) set_load<B1, 3, CP::LOGT, CP::PA, CP::PK>(l[p], a1, x[p]);
-        col_fwd3<L1, T, S + 1>(x, u, col, l, tw0, PS);
+        col_fwd3<L1, T, S + 1>(x, wn, u, col, l, tw0, PS);
     }
 }
 
 template <int L1, int T, int S = ColPlan<L1, T>::NSET - 1>
-__device__ __forceinline__ void col_inv3(u32 (*x)[8], int u, int col, u32* const* l, const u32* __restrict__ tw0,
-                                         const Primes3& PS) {
+__device__ __forceinline__ void col_inv3(u32 (*x)[8], const u32 (*w)[8], int u, int col, u32* const* l,
+                                         const u32* __restrict__ tw0, const Primes3& PS) {
     using CP = ColPlan<L1, T>;
 #pragma unroll
-    for (int p = 0; p < 3; ++p) col_step<L1, T, S, true>(x[p], u, tw0 + (size_t)p * TW3_STRIDE, PS.p[p]);
+    for (int p = 0; p < 3; ++p) col_step<L1, T, S, true>(x[p], w[p], u, tw0 + (size_t)p * TW3_STRIDE, PS.p[p]);
     if constexpr (S > 0) {
+        u32 wn[3][8];
+#pragma unroll
+        for (int p = 0; p < 3; ++p) col_tw<L1, T, S - 1, true>(wn[p], u, tw0 + (size_t)p * TW3_STRIDE);
         constexpr int B0 = CP::bottom(S), B1 = CP::bottom(S - 1);
         const int a0 = set_base<B0, 3, CP::LOGT, CP::PA, CP::PK>(u, col);
         const int a1 = set_base<B1, 3, CP::LOGT, CP::PA, CP::PK>(u, col);
@@ -858,7 +883,7 @@ __device__ __forceinline__ void col_inv3(u32 (*x)[8], int u, int col, u32* const
         __syncthreads();
 #pragma unroll
         for (int p = 0; p < 3; ++p) set_load<B1, 3, CP::LOGT, CP::PA, CP::PK>(l[p], a1, x[p]);
-        col_inv3<L1, T, S - 1>(x, u, col, l, tw0, PS);
+        col_inv3<L1, T, S - 1>(x, wn, u, col, l, tw0, PS);
     }
 }
 
@@ -882,21 +907,31 @@ __global__ __launch_bounds__((ColPlan<L1, T>::WG)) void k3_col_fwd_reg(const Mul
     const size_t nl = op ? J.nb : J.na;
     u32* __restrict__ dst = (u32*)(op ? J.wsB : J.wsA);
     const int col = threadIdx.x & (T - 1), u = threadIdx.x >> CP::LOGT;
-    u32 x[3][8];
+    // the limbs, then set 0's twiddles, all issued before the arithmetic
+    u32 v[8];
 #pragma unroll
     for (int t = 0; t < 8; ++t) {
         const size_t g = (size_t)(t * (R / 8) + u) * S + b0 + col;
-        const u32 v = g < nl ? src[g] : 0u;
+        v[t] = g < nl ? gld(src + g) : 0u;
+    }
+    sched_fence();
+    u32 w[3][8];
 #pragma unroll
-        for (int p = 0; p < 3; ++p) x[p][t] = r_reduce_limb(v, PS.p[p].p);
+    for (int p = 0; p < 3; ++p) col_tw<L1, T, 0, false>(w[p], u, tw3 + (size_t)p * TW3_STRIDE);
+    sched_fence();
+    u32 x[3][8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+#pragma unroll
+        for (int p = 0; p < 3; ++p) x[p][t] = r_reduce_limb(v[t], PS.p[p].p);
     }
     u32* l[3] = {lds3, lds3 + words, lds3 + 2 * words};
-    col_fwd3<L1, T>(x, u, col, l, tw3, PS);
+    col_fwd3<L1, T>(x, w, u, col, l, tw3, PS);
 #pragma unroll
     for (int t = 0; t < 8; ++t) {
         const size_t g = (size_t)((u << 3) | t) * S + b0 + col;
 #pragma unroll
-        for (int p = 0; p < 3; ++p) dst[((size_t)p << n3) + g] = x[p][t];
+        for (int p = 0; p < 3; ++p) gst(dst + ((size_t)p << n3) + g, x[p][t]);
     }
 }
 
@@ -919,10 +954,15 @@ __global__ __launch_bounds__((ColPlan<L1, T>::WG)) void k3_col_inv_reg(const Mul
     for (int t = 0; t < 8; ++t) {
         const size_t g = (size_t)((u << 3) | t) * S + b0 + col;
 #pragma unroll
-        for (int p = 0; p < 3; ++p) x[p][t] = ((const u32*)J.wsA)[((size_t)p << n3) + g];
+        for (int p = 0; p < 3; ++p) x[p][t] = gld((const u32*)J.wsA + ((size_t)p << n3) + g);
     }
+    sched_fence();
+    u32 w[3][8];
+#pragma unroll
+    for (int p = 0; p < 3; ++p) col_tw<L1, T, CP::NSET - 1, true>(w[p], u, tw3 + (size_t)p * TW3_STRIDE);
+    sched_fence();
     u32* l[3] = {lds3, lds3 + words, lds3 + 2 * words};
-    col_inv3<L1, T>(x, u, col, l, tw3, PS);
+    col_inv3<L1, T>(x, w, u, col, l, tw3, PS);
     // CRT of each coefficient of set 0 into (lo32, up) over the three arrays
     // at the words this thread read last
     {
@@ -971,7 +1011,7 @@ __global__ __launch_bounds__((ColPlan<L1, T>::WG)) void k3_col_inv_reg(const Mul
     for (int k = 0; k < 8; ++k) {
         const int e = k * CP::WG + threadIdx.x;
         const size_t g = (size_t)(e / T) * S + b0 + (e & (T - 1));
-        if (g < C.nlimbs) C.out[g] = l[0][padak<CP::PA, CP::PK>(e)];
+        if (g < C.nlimbs) gst(C.out + g, l[0][padak<CP::PA, CP::PK>(e)]);
     }
 }
 

@@ -10,6 +10,18 @@
 
 namespace pdh {
 
+// Global-memory accesses through pointers read from job descriptors. Such
+// pointers are generic, and generic (flat) loads count on the LDS counter as
+// well as the vector-memory one, so every wait for an LDS exchange would also
+// wait for each global load still in flight; the casts let the compiler emit
+// global loads and stores instead.
+typedef __attribute__((address_space(1))) u32 g_u32;
+__device__ __forceinline__ u32 gld(const u32* p) { return *(const g_u32*)p; }
+__device__ __forceinline__ void gst(u32* p, u32 v) { *(g_u32*)p = v; }
+// Keeps the machine scheduler from moving instructions across this point
+// (used to issue a group of loads before the arithmetic that consumes them).
+__device__ __forceinline__ void sched_fence() { __builtin_amdgcn_sched_barrier(0); }
+
 __device__ __forceinline__ int lpad(int e) { return e + (e >> 3); }
 __host__ __device__ constexpr int lds_words(int e) { return e + (e >> 3); }
 
