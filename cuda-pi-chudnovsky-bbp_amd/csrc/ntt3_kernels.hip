@@ -188,7 +188,12 @@ __device__ __forceinline__ void l_dftq(u32* x, const Prime3& P) {
 // (outputs in [0, 4p)), then each output c >= 1 times its twiddle (mml gives
 // [0, 2p)) and output 0 reduced; without twiddles every output is reduced.
 // Inverse: inputs c >= 1 times their twiddles first, then DFT_Q and reduce.
-template <int Q, bool INV>
+// Two inverse options let consecutive levels skip reductions: ROUT = false
+// leaves the outputs in [0, 4p), and RIN0 reduces input 0 of a twiddled
+// butterfly, so a level with RIN0 takes inputs in [0, 4p) (a twiddle product
+// of a < 4p and a canonical w < p is (a w + m p) / 2^32 < p (4p / 2^32 + 1)
+// < 2p since 4p < 2^32). A level without twiddles needs inputs in [0, 2p).
+template <int Q, bool INV, bool RIN0 = false, bool ROUT = true>
 __device__ __forceinline__ void lazy_butterfly(u32* x, const u32* w, bool has_tw, const Prime3& P) {
     const u32 p2 = P.p << 1;
     if (!INV) {
@@ -203,12 +208,15 @@ __device__ __forceinline__ void lazy_butterfly(u32* x, const u32* w, bool has_tw
         }
     } else {
         if (has_tw) {
+            if constexpr (RIN0) x[0] = red2(x[0], p2);
 #pragma unroll
             for (int c = 1; c < Q; ++c) x[c] = mml(x[c], w[c], P.p, P.ninv);
         }
         l_dftq<Q, true>(x, P);
+        if constexpr (ROUT) {
 #pragma unroll
-        for (int c = 0; c < Q; ++c) x[c] = red2(x[c], p2);
+            for (int c = 0; c < Q; ++c) x[c] = red2(x[c], p2);
+        }
     }
 }
 
@@ -552,15 +560,21 @@ __device__ __forceinline__ void set_tw(u32* w, int u, const u32* __restrict__ tw
 
 // Set S's step: the radix-16 butterfly, or for the partial set the
 // 2^(4 - REM) butterflies of radix 2^REM over the low bits.
+// The inverse runs its sets from the last one down to set 0: every set but
+// set 0 leaves its outputs in [0, 4p) for the next set's twiddle products and
+// input-0 reduction; set 0 reduces its outputs for the output twiddle, whose
+// running power is only in [0, 2p). (The partial set is the inverse's first,
+// so its inputs, the pointwise products, are in [0, 2p).)
 template <int L2, int S, bool INV>
 __device__ __forceinline__ void set_step(u32* x, const u32* w, const Prime3& P) {
     using RP = RowPlan<L2>;
+    constexpr bool ROUT = !INV || S == 0;
     if constexpr (RP::partial(S)) {
         constexpr int Q = 1 << RP::REM;
 #pragma unroll
-        for (int hq = 0; hq < 16 / Q; ++hq) lazy_butterfly<Q, INV>(x + hq * Q, w, false, P);
+        for (int hq = 0; hq < 16 / Q; ++hq) lazy_butterfly<Q, INV, INV, ROUT>(x + hq * Q, w, false, P);
     } else {
-        lazy_butterfly<16, INV>(x, w, RP::bottom(S) > 0, P);
+        lazy_butterfly<16, INV, INV, ROUT>(x, w, RP::bottom(S) > 0, P);
     }
 }
 
@@ -589,12 +603,15 @@ __device__ __forceinline__ void row_exchange_sync() {
 
 // Forward of NOP operands (1 or 2, arrays l[0], l[1]): x holds set 0 on entry,
 // the spectrum in the last set's positions on exit.
-template <int L2, int NOP, int S = 0>
+// STEP = false: the caller has already applied set S's step.
+template <int L2, int NOP, int S = 0, bool STEP = true>
 __device__ __forceinline__ void row_fwd(u32 (*x)[16], const u32* w, int u, u32* const* l, const u32* __restrict__ tw,
                                         const Prime3& P) {
     using RP = RowPlan<L2>;
+    if constexpr (STEP) {
 #pragma unroll
-    for (int k = 0; k < NOP; ++k) set_step<L2, S, false>(x[k], w, P);
+        for (int k = 0; k < NOP; ++k) set_step<L2, S, false>(x[k], w, P);
+    }
     if constexpr (S + 1 < RP::NSET) {
         // the next set's twiddles are in flight during the exchange
         u32 wn[16];
@@ -660,36 +677,42 @@ __global__ __launch_bounds__(RowPlan<L2>::WG) void k3_row_reg(const MulJob* __re
     u32 x[NOP][16];
     // Every load of the prologue is issued before any arithmetic, in the order
     // the values are used (loads complete in issue order, so each use waits
-    // only for the loads issued before it): the outer-twiddle table entries,
-    // the row's points (set 0, b = t R/16 + u), set 0's level twiddles and
-    // the inverse's outer-twiddle entries, which would otherwise be fetched
-    // after the last exchange with nothing left to overlap them.
+    // only for the loads issued before it): the outer-twiddle table entries
+    // and set 0's level twiddles, A's points (set 0, b = t R/16 + u), B's
+    // points, and the inverse's outer-twiddle entries, which would otherwise
+    // be fetched after the last exchange with nothing left to overlap them.
+    // A's outer twiddle and set-0 step then run while B's points arrive.
     const u64 e0 = (c * (u64)u) & (N - 1), lmask = ((u64)1 << h) - 1;
     const u32 fhi = gld(hi + (e0 >> h)), flo = gld(lo + (e0 & lmask));
-    sched_fence();
-#pragma unroll
-    for (int t = 0; t < 16; ++t) {
-        x[0][t] = gld(A + t * TPR + u);
-        if constexpr (NOP == 2) x[1][t] = gld(Bv + t * TPR + u);
-    }
-    sched_fence();
     u32 w0[16];
     set_tw<L2, 0, false>(w0, u, tw);
+    sched_fence();
+#pragma unroll
+    for (int t = 0; t < 16; ++t) x[0][t] = gld(A + t * TPR + u);
+    sched_fence();
+    if constexpr (NOP == 2) {
+#pragma unroll
+        for (int t = 0; t < 16; ++t) x[1][t] = gld(Bv + t * TPR + u);
+    }
     const u32 ihi = gld(hi_inv + (e0 >> h)), ilo = gld(lo_inv + (e0 & lmask));
     sched_fence();
+    // outer twiddles w^(b c)
+    u32 wt[16];
     {
-        // outer twiddle w^(b c) on the loaded points
         u32 w = mm(fhi, flo, P.p, P.ninv);
         const u32 step = r_tw(hi, lo, h, (c * (u64)TPR) & (N - 1), P);
 #pragma unroll
-        for (int t = 0; t < 16; ++t) {
+        for (int t = 0; t < 16; ++t) { wt[t] = w; w = mml(w, step, P.p, P.ninv); }
+    }
 #pragma unroll
-            for (int k = 0; k < NOP; ++k) x[k][t] = mml(x[k][t], w, P.p, P.ninv);
-            w = mml(w, step, P.p, P.ninv);
-        }
+    for (int k = 0; k < NOP; ++k) {
+#pragma unroll
+        for (int t = 0; t < 16; ++t) x[k][t] = mml(x[k][t], wt[t], P.p, P.ninv);
+        set_step<L2, 0, false>(x[k], w0, P);
+        sched_fence();
     }
     const u32 winv = mm(ihi, ilo, P.p, P.ninv);
-    row_fwd<L2, NOP>(x, w0, u, l, tw, P);
+    row_fwd<L2, NOP, 0, false>(x, w0, u, l, tw, P);
 #pragma unroll
     for (int t = 0; t < 16; ++t) x[0][t] = mml(x[0][t], x[NOP - 1][t], P.p, P.ninv);
     // the inverse's first exchange writes the positions this thread read last
@@ -832,12 +855,15 @@ __device__ __forceinline__ void col_step(u32* x, const u32* w, int u, const u32*
             for (int q = 0; q < Q; ++q) y[q] = x[q * TL + tlo];
 #pragma unroll
             for (int c = 1; c < Q; ++c) w[c] = tl[c];
-            lazy_butterfly<Q, INV>(y, w, true, P);
+            lazy_butterfly<Q, INV, INV, true>(y, w, true, P);   // partial set: the inverse's last
 #pragma unroll
             for (int q = 0; q < Q; ++q) x[q * TL + tlo] = y[q];
         }
     } else {
-        lazy_butterfly<8, INV>(x, w, CP::bottom(S) > 0, P);
+        // inverse: sets before set 0 leave outputs in [0, 4p) for the next
+        // set's twiddle products and input-0 reduction; set 0 (the CRT's
+        // input) reduces them
+        lazy_butterfly<8, INV, INV, !INV || S == 0>(x, w, CP::bottom(S) > 0, P);
     }
 }
 
