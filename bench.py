@@ -63,8 +63,8 @@ def alg_bytes(kernel: str, na: int, nb: int, N: int, T: int, eng: int) -> float:
         "k3_row_reg": 3 * 12 * N3,                # the same, register-resident radix-16 rows
         "k3_row_reg_sqr": 3 * 8 * N3,
         "k3_col_inv": 3 * 4 * N3 + 4 * nl + cb,   # three spectra in; digits and carry-outs out
-        "k3_col_fwd_reg": 4 * nl + 2 * 3 * 4 * N3,    # the same, register-resident radix-8 columns
-        "k3_col_inv_reg": 3 * 4 * N3 + 4 * nl + cb,
+        "k3_col_fwd_seq": 4 * nl + 2 * 3 * 4 * N3,    # the same, register-resident radix-8 columns
+        "k3_col_inv_seq": 3 * 4 * N3 + 4 * nl + cb,
         "k_carry_reduce": 4 * nl + cb,        # digits and carry-outs in
         "k_carry_reduce_th": 2 * cb,          # chunk thresholds and carry-outs in
         "k_carry_apply": 8 * nl + cb,         # digits and carry-outs in, limbs out

@@ -377,7 +377,7 @@ __global__ __launch_bounds__(1024) void k3_col_fwd(const MulJob* __restrict__ jo
     const unsigned rem = bid % ((unsigned)ops * tiles);
     const int op = (int)(rem / tiles);
     const size_t b0 = (size_t)(rem % tiles) * T;
-    const MulJob J = jobs ? jobs[job] : one;
+    const MulJob J = job_of(jobs, job, one);
     const u32* __restrict__ src = op ? J.b : J.a;
     const size_t nl = op ? J.nb : J.na;
     u32* __restrict__ dst = (u32*)(op ? J.wsB : J.wsA);
@@ -425,7 +425,7 @@ __global__ __launch_bounds__(1024) void k3_row_mul(const MulJob* __restrict__ jo
     const unsigned job = blockIdx.x / (3u * rows);
     const int pi = (int)((blockIdx.x / rows) % 3u);
     const int row = (int)(blockIdx.x % rows);
-    const MulJob J = jobs ? jobs[job] : one;
+    const MulJob J = job_of(jobs, job, one);
     const Prime3 P = pick3(PS, pi);
     u32* __restrict__ A = (u32*)J.wsA + ((size_t)pi << n3);
     const u32* __restrict__ B = (const u32*)J.wsB + ((size_t)pi << n3);
@@ -662,7 +662,7 @@ __global__ __launch_bounds__(RowPlan<L2>::WG) void k3_row_reg(const MulJob* __re
     const unsigned job = blockIdx.x / (3u * wgs);
     const int pi = (int)((blockIdx.x / wgs) % 3u);
     const int row = (int)(blockIdx.x % wgs) * RP::G + g;
-    const MulJob J = jobs ? jobs[job] : one;
+    const MulJob J = job_of(jobs, job, one);
     const Prime3 P = pick3(PS, pi);
     u32* __restrict__ A = (u32*)J.wsA + ((size_t)pi << n3) + (size_t)row * R;
     const u32* __restrict__ Bv = (const u32*)J.wsB + ((size_t)pi << n3) + (size_t)row * R;
@@ -755,7 +755,7 @@ __global__ __launch_bounds__(1024) void k3_col_inv(const MulJob* __restrict__ jo
     const int R = 1 << L1;
     const unsigned tiles = (unsigned)(S / T);
     const unsigned bid = xcd_tile_order(blockIdx.x, gridDim.x);
-    const MulJob J = jobs ? jobs[bid / tiles] : one;
+    const MulJob J = job_of(jobs, bid / tiles, one);
     const size_t b0 = (size_t)(bid % tiles) * T;
     const int words = lds_words(R * T);
     for (int pi = 0; pi < 3; ++pi) {
@@ -792,8 +792,8 @@ __global__ __launch_bounds__(1024) void k3_col_inv(const MulJob* __restrict__ jo
 }
 
 // ---- register-resident column passes (radix-8 sets) ---------------------------------
-// A tile of T columns x R = 2^L1 rows of all three primes; a thread holds 8
-// rows of one column for each prime (24 residues), WG = R/8 * T threads. Sets
+// A tile of T columns x R = 2^L1 rows, the three primes in turn through one
+// LDS tile; a thread holds 8 rows of one column, WG = R/8 * T threads. Sets
 // as in the row kernel but of radix 8 with the partial set first: set 0 holds
 // rows t R/8 + u and applies the top L1 mod 3 levels (radix 2 or 4, twiddled)
 // when L1 is not a multiple of 3, then radix-8 sets at bottoms
@@ -819,8 +819,8 @@ struct ColPlan {
     static constexpr int PA = 3, PK = 1;
 };
 
-// Tiles the register column kernels are built for: at least one wave, the
-// three residue tiles within the LDS limit.
+// Tiles the register column kernels are built for: at least one wave and at
+// most 1,024 threads (R T / 8 of them).
 constexpr bool col_reg_pair(int L1, int T) {
     const int lt = T == 4 ? 2 : T == 8 ? 3 : T == 16 ? 4 : T == 32 ? 5 : 99;
     return L1 >= 3 && L1 + lt >= 9 && L1 + lt <= 13;
@@ -867,60 +867,39 @@ __device__ __forceinline__ void col_step(u32* x, const u32* w, int u, const u32*
     }
 }
 
-// w holds set S's level twiddles per prime; the next set's are loaded
-// before the exchange, so their latency overlaps it.
+// One prime's forward column sets: x holds set 0 (rows t R/8 + u) on entry,
+// the last set (rows 8u + t) on exit; l is the tile's single LDS array.
 template <int L1, int T, int S = 0>
-__device__ __forceinline__ void col_fwd3(u32 (*x)[8], const u32 (*w)[8], int u, int col, u32* const* l,
-                                         const u32* __restrict__ tw0, const Primes3& PS) {
+__device__ __forceinline__ void col_fwd1(u32* x, const u32* w, int u, int col, u32* __restrict__ l,
+                                         const u32* __restrict__ tw, const Prime3& P) {
     using CP = ColPlan<L1, T>;
-#pragma unroll
-    for (int p = 0; p < 3; ++p) col_step<L1, T, S, false>(x[p], w[p], u, tw0 + (size_t)p * TW3_STRIDE, PS.p[p]);
+    col_step<L1, T, S, false>(x, w, u, tw, P);
     if constexpr (S + 1 < CP::NSET) {
-        u32 wn[3][8];
-#pragma unroll
-        for (int p = 0; p < 3; ++p) col_tw<L1, T, S + 1, false>(wn[p], u, tw0 + (size_t)p * TW3_STRIDE);
+        u32 wn[8];
+        col_tw<L1, T, S + 1, false>(wn, u, tw);
         constexpr int B0 = CP::bottom(S), B1 = CP::bottom(S + 1);
-        const int a0 = set_base<B0, 3, CP::LOGT, CP::PA, CP::PK>(u, col);
-        const int a1 = set_base<B1, 3, CP::LOGT, CP::PA, CP::PK>(u, col);
-#pragma unroll
-        for (int p = 0; p < 3; ++p) set_store<B0, 3, CP::LOGT, CP::PA, CP::PK>(l[p], a0, x[p]);
+        set_store<B0, 3, CP::LOGT, CP::PA, CP::PK>(l, set_base<B0, 3, CP::LOGT, CP::PA, CP::PK>(u, col), x);
         __syncthreads();
-#pragma unroll
-        for (int p = 0; p < 3; ++p) set_load<B1, 3, CP::LOGT, CP::PA, CP::PK>(l[p], a1, x[p]);
-        col_fwd3<L1, T, S + 1>(x, wn, u, col, l, tw0, PS);
+        set_load<B1, 3, CP::LOGT, CP::PA, CP::PK>(l, set_base<B1, 3, CP::LOGT, CP::PA, CP::PK>(u, col), x);
+        col_fwd1<L1, T, S + 1>(x, wn, u, col, l, tw, P);
     }
 }
 
-template <int L1, int T, int S = ColPlan<L1, T>::NSET - 1>
-__device__ __forceinline__ void col_inv3(u32 (*x)[8], const u32 (*w)[8], int u, int col, u32* const* l,
-                                         const u32* __restrict__ tw0, const Primes3& PS) {
-    using CP = ColPlan<L1, T>;
-#pragma unroll
-    for (int p = 0; p < 3; ++p) col_step<L1, T, S, true>(x[p], w[p], u, tw0 + (size_t)p * TW3_STRIDE, PS.p[p]);
-    if constexpr (S > 0) {
-        u32 wn[3][8];
-#pragma unroll
-        for (int p = 0; p < 3; ++p) col_tw<L1, T, S - 1, true>(wn[p], u, tw0 + (size_t)p * TW3_STRIDE);
-        constexpr int B0 = CP::bottom(S), B1 = CP::bottom(S - 1);
-        const int a0 = set_base<B0, 3, CP::LOGT, CP::PA, CP::PK>(u, col);
-        const int a1 = set_base<B1, 3, CP::LOGT, CP::PA, CP::PK>(u, col);
-#pragma unroll
-        for (int p = 0; p < 3; ++p) set_store<B0, 3, CP::LOGT, CP::PA, CP::PK>(l[p], a0, x[p]);
-        __syncthreads();
-#pragma unroll
-        for (int p = 0; p < 3; ++p) set_load<B1, 3, CP::LOGT, CP::PA, CP::PK>(l[p], a1, x[p]);
-        col_inv3<L1, T, S - 1>(x, wn, u, col, l, tw0, PS);
-    }
-}
+// A limb below 2^32 as a residue in [0, 2p) by one Montgomery product with
+// R mod p (w8[0] is w8^0 R): v (R mod p) < 2^32 p, so the product is below 2p.
+__device__ __forceinline__ u32 r_limb_mont(u32 v, const Prime3& P) { return mml(v, P.w8[0], P.p, P.ninv); }
 
+// Forward column pass with the primes in sequence through one LDS tile: a
+// third of the LDS of the three tiles at once, so two workgroups share a CU (at most
+// 64 VGPRs) and one's loads, stores and barriers overlap the other's
+// arithmetic. The limbs are loaded once and reduced per prime.
 template <int L1, int T>
-__global__ __launch_bounds__((ColPlan<L1, T>::WG)) void k3_col_fwd_reg(const MulJob* __restrict__ jobs, MulJob one,
-                                                                    int ops, int n3, const u32* __restrict__ tw3,
-                                                                    Primes3 PS) {
+__global__ __launch_bounds__((ColPlan<L1, T>::WG), (ColPlan<L1, T>::WG >= 1024 ? 8 : 4))
+void k3_col_fwd_seq(const MulJob* __restrict__ jobs, MulJob one, int ops, int n3, const u32* __restrict__ tw3,
+                    Primes3 PS) {
     using CP = ColPlan<L1, T>;
     extern __shared__ u32 lds3[];
     constexpr int R = 1 << L1;
-    constexpr int words = lds_words(R * T);
     const size_t S = (size_t)1 << (n3 - L1);
     const unsigned tiles = (unsigned)(S / T);
     const unsigned bid = xcd_tile_order(blockIdx.x, gridDim.x);
@@ -928,12 +907,11 @@ __global__ __launch_bounds__((ColPlan<L1, T>::WG)) void k3_col_fwd_reg(const Mul
     const unsigned rem = bid % ((unsigned)ops * tiles);
     const int op = (int)(rem / tiles);
     const size_t b0 = (size_t)(rem % tiles) * T;
-    const MulJob J = jobs ? jobs[job] : one;
+    const MulJob J = job_of(jobs, job, one);
     const u32* __restrict__ src = op ? J.b : J.a;
     const size_t nl = op ? J.nb : J.na;
     u32* __restrict__ dst = (u32*)(op ? J.wsB : J.wsA);
     const int col = threadIdx.x & (T - 1), u = threadIdx.x >> CP::LOGT;
-    // the limbs, then set 0's twiddles, all issued before the arithmetic
     u32 v[8];
 #pragma unroll
     for (int t = 0; t < 8; ++t) {
@@ -941,103 +919,146 @@ __global__ __launch_bounds__((ColPlan<L1, T>::WG)) void k3_col_fwd_reg(const Mul
         v[t] = g < nl ? gld(src + g) : 0u;
     }
     sched_fence();
-    u32 w[3][8];
-#pragma unroll
-    for (int p = 0; p < 3; ++p) col_tw<L1, T, 0, false>(w[p], u, tw3 + (size_t)p * TW3_STRIDE);
+    u32 w[8];
+    col_tw<L1, T, 0, false>(w, u, tw3);
     sched_fence();
-    u32 x[3][8];
 #pragma unroll
-    for (int t = 0; t < 8; ++t) {
+    for (int p = 0; p < 3; ++p) {
+        const Prime3& P = PS.p[p];
+        const u32* tw = tw3 + (size_t)p * TW3_STRIDE;
+        u32 x[8];
 #pragma unroll
-        for (int p = 0; p < 3; ++p) x[p][t] = r_reduce_limb(v[t], PS.p[p].p);
-    }
-    u32* l[3] = {lds3, lds3 + words, lds3 + 2 * words};
-    col_fwd3<L1, T>(x, w, u, col, l, tw3, PS);
+        for (int t = 0; t < 8; ++t) x[t] = r_limb_mont(v[t], P);
+        // the previous prime's last exchange has been read everywhere before
+        // this prime's first exchange writes the tile
+        if (p > 0) __syncthreads();
+        col_fwd1<L1, T>(x, w, u, col, lds3, tw, P);
+        if (p < 2) col_tw<L1, T, 0, false>(w, u, tw3 + (size_t)(p + 1) * TW3_STRIDE);
 #pragma unroll
-    for (int t = 0; t < 8; ++t) {
-        const size_t g = (size_t)((u << 3) | t) * S + b0 + col;
-#pragma unroll
-        for (int p = 0; p < 3; ++p) gst(dst + ((size_t)p << n3) + g, x[p][t]);
+        for (int t = 0; t < 8; ++t) {
+            const size_t g = (size_t)((u << 3) | t) * S + b0 + col;
+            gst(dst + ((size_t)p << n3) + g, x[t]);
+        }
     }
 }
 
+// One prime's inverse column sets: x holds the last set (rows 8u + t) on
+// entry, set 0 (rows t R/8 + u) on exit; l is the tile's single LDS array.
+template <int L1, int T, int S = ColPlan<L1, T>::NSET - 1>
+__device__ __forceinline__ void col_inv1(u32* x, const u32* w, int u, int col, u32* __restrict__ l,
+                                         const u32* __restrict__ tw, const Prime3& P) {
+    using CP = ColPlan<L1, T>;
+    col_step<L1, T, S, true>(x, w, u, tw, P);
+    if constexpr (S > 0) {
+        u32 wn[8];
+        col_tw<L1, T, S - 1, true>(wn, u, tw);
+        constexpr int B0 = CP::bottom(S), B1 = CP::bottom(S - 1);
+        set_store<B0, 3, CP::LOGT, CP::PA, CP::PK>(l, set_base<B0, 3, CP::LOGT, CP::PA, CP::PK>(u, col), x);
+        __syncthreads();
+        set_load<B1, 3, CP::LOGT, CP::PA, CP::PK>(l, set_base<B1, 3, CP::LOGT, CP::PA, CP::PK>(u, col), x);
+        col_inv1<L1, T, S - 1>(x, wn, u, col, l, tw, P);
+    }
+}
+
+// LDS words of k3_col_inv_seq: the exchange tile, or the carry's three
+// arrays over half the rows, whichever is larger.
+__host__ __device__ constexpr int col_inv_seq_words(int L1, int T) {
+    return lds_words((1 << L1) * T) > 3 * lds_words((1 << (L1 - 1)) * T) ? lds_words((1 << L1) * T)
+                                                                            : 3 * lds_words((1 << (L1 - 1)) * T);
+}
+
+// Inverse column pass with the primes in sequence through one LDS tile (as
+// k3_col_fwd_seq), each prime's set 0 kept in registers; then the CRT in
+// registers and each row segment's carry through LDS in two halves of the
+// rows (rows t R/8 + u with t < 4, then t >= 4), so that the LDS is about
+// half of what the three residue tiles at once would take, and two
+// workgroups share a CU.
 template <int L1, int T>
-__global__ __launch_bounds__((ColPlan<L1, T>::WG)) void k3_col_inv_reg(const MulJob* __restrict__ jobs, MulJob one,
-                                                                    int n3, const u32* __restrict__ tw3, Primes3 PS,
-                                                                    size_t nblk_max) {
+__global__ __launch_bounds__((ColPlan<L1, T>::WG), (ColPlan<L1, T>::WG >= 1024 ? 8 : 4))
+void k3_col_inv_seq(const MulJob* __restrict__ jobs, MulJob one, int n3, const u32* __restrict__ tw3, Primes3 PS,
+                    size_t nblk_max) {
     using CP = ColPlan<L1, T>;
     extern __shared__ u32 lds3[];
-    constexpr int R = 1 << L1;
-    constexpr int words = lds_words(R * T);
+    constexpr int R = 1 << L1, RH = R / 2;
+    constexpr int hwords = lds_words(RH * T);
     const size_t S = (size_t)1 << (n3 - L1);
     const unsigned tiles = (unsigned)(S / T);
     const unsigned bid = xcd_tile_order(blockIdx.x, gridDim.x);
-    const MulJob J = jobs ? jobs[bid / tiles] : one;
+    const MulJob J = job_of(jobs, bid / tiles, one);
     const size_t b0 = (size_t)(bid % tiles) * T;
     const int col = threadIdx.x & (T - 1), u = threadIdx.x >> CP::LOGT;
+    // each prime's residues are loaded when its turn comes (all three up
+    // front would not fit the register budget of two workgroups per CU)
     u32 x[3][8];
+    u32 w[8];
 #pragma unroll
-    for (int t = 0; t < 8; ++t) {
-        const size_t g = (size_t)((u << 3) | t) * S + b0 + col;
-#pragma unroll
-        for (int p = 0; p < 3; ++p) x[p][t] = gld((const u32*)J.wsA + ((size_t)p << n3) + g);
-    }
-    sched_fence();
-    u32 w[3][8];
-#pragma unroll
-    for (int p = 0; p < 3; ++p) col_tw<L1, T, CP::NSET - 1, true>(w[p], u, tw3 + (size_t)p * TW3_STRIDE);
-    sched_fence();
-    u32* l[3] = {lds3, lds3 + words, lds3 + 2 * words};
-    col_inv3<L1, T>(x, w, u, col, l, tw3, PS);
-    // CRT of each coefficient of set 0 into (lo32, up) over the three arrays
-    // at the words this thread read last
-    {
-        constexpr int B0 = CP::bottom(0);
-        const int a0 = set_base<B0, 3, CP::LOGT, CP::PA, CP::PK>(u, col);
+    for (int p = 0; p < 3; ++p) {
 #pragma unroll
         for (int t = 0; t < 8; ++t) {
-            const int e = a0 + padak<CP::PA, CP::PK>((t << B0) << CP::LOGT);
-            u32 lo32; u64 up;
-            crt3(umin32(x[0][t], x[0][t] - PS.p[0].p), umin32(x[1][t], x[1][t] - PS.p[1].p),
-                 umin32(x[2][t], x[2][t] - PS.p[2].p), PS, lo32, up);
-            l[0][e] = lo32; l[1][e] = (u32)up; l[2][e] = (u32)(up >> 32);
+            const size_t g = (size_t)((u << 3) | t) * S + b0 + col;
+            x[p][t] = gld((const u32*)J.wsA + ((size_t)p << n3) + g);
         }
+        col_tw<L1, T, CP::NSET - 1, true>(w, u, tw3 + (size_t)p * TW3_STRIDE);
+        sched_fence();
+        // the previous prime's last exchange has been read everywhere before
+        // this prime's first exchange writes the tile
+        if (p > 0) __syncthreads();
+        col_inv1<L1, T>(x[p], w, u, col, lds3, tw3 + (size_t)p * TW3_STRIDE, PS.p[p]);
     }
-    __syncthreads();
+    // CRT of each coefficient (row t R/8 + u, column col) into (lo32, up)
+    u32 lo[8];
+    u64 up[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t)
+        crt3(umin32(x[0][t], x[0][t] - PS.p[0].p), umin32(x[1][t], x[1][t] - PS.p[1].p),
+             umin32(x[2][t], x[2][t] - PS.p[2].p), PS, lo[t], up[t]);
     // chunk = the T coefficients (= limbs) of one row segment, carry-in 0;
     // the chunk's limbs go back to array 0 in place. With the carry-out C the
     // chunk's threshold goes out too (2^(32T) - D when limbs 2..T-1 of D are
     // all ones, else never; limbs at or above na + nb count as zero), so the
     // carry scan reads (TH, C) and not the digits.
     const CarryJob C = carry_job<2 * T>(J, nblk_max);
-    for (int i = threadIdx.x; i < R; i += blockDim.x) {
-        const size_t k0 = (size_t)i * S + b0;
-        const size_t ch = k0 / T;
-        if (ch >= C.nchunks) continue;
-        u64 carry = 0, lo = 0;
-        bool ones = true;
+    u32* l0 = lds3;
+    u32* l1 = lds3 + hwords;
+    u32* l2 = lds3 + 2 * hwords;
 #pragma unroll
-        for (int q = 0; q < T; ++q) {
-            const int e = padak<CP::PA, CP::PK>(i * T + q);
-            const u64 up = (u64)l[1][e] | ((u64)l[2][e] << 32);
-            const u64 s = (carry & 0xFFFFFFFFull) + l[0][e];
-            l[0][e] = (u32)s;
-            const bool in = k0 + q < C.nlimbs;
-            if (q < 2) lo |= (in ? (u64)(u32)s : 0) << (32 * q);
-            else ones &= in && (u32)s == 0xFFFFFFFFu;
-            carry = (s >> 32) + (carry >> 32) + up;
+    for (int h = 0; h < 2; ++h) {
+        __syncthreads();                        // the tile (or the previous half) is no longer read
+#pragma unroll
+        for (int t = 4 * h; t < 4 * h + 4; ++t) {
+            const int e = padak<CP::PA, CP::PK>((((t - 4 * h) * (R / 8) + u) << CP::LOGT) | col);
+            l0[e] = lo[t]; l1[e] = (u32)up[t]; l2[e] = (u32)(up[t] >> 32);
         }
-        C.C[ch] = carry;
-        C.TH[ch] = (ones && lo != 0) ? (u64)(0 - lo) : ~(u64)0;
-    }
-    __syncthreads();
-    // limbs out with consecutive lanes on consecutive columns of a row
-    // (T-limb runs), not one row per lane
+        __syncthreads();
+        for (int i = threadIdx.x; i < RH; i += blockDim.x) {
+            const size_t k0 = (size_t)(i + h * RH) * S + b0;
+            const size_t ch = k0 / T;
+            if (ch >= C.nchunks) continue;
+            u64 carry = 0, lw = 0;
+            bool ones = true;
+#pragma unroll 2
+            for (int q = 0; q < T; ++q) {
+                const int e = padak<CP::PA, CP::PK>(i * T + q);
+                const u64 hi = (u64)l1[e] | ((u64)l2[e] << 32);
+                const u64 sum = (carry & 0xFFFFFFFFull) + l0[e];
+                l0[e] = (u32)sum;
+                const bool in = k0 + q < C.nlimbs;
+                if (q < 2) lw |= (in ? (u64)(u32)sum : 0) << (32 * q);
+                else ones &= in && (u32)sum == 0xFFFFFFFFu;
+                carry = (sum >> 32) + (carry >> 32) + hi;
+            }
+            C.C[ch] = carry;
+            C.TH[ch] = (ones && lw != 0) ? (u64)(0 - lw) : ~(u64)0;
+        }
+        __syncthreads();
+        // limbs out with consecutive lanes on consecutive columns of a row
+        // (T-limb runs), not one row per lane
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        const int e = k * CP::WG + threadIdx.x;
-        const size_t g = (size_t)(e / T) * S + b0 + (e & (T - 1));
-        if (g < C.nlimbs) gst(C.out + g, l[0][padak<CP::PA, CP::PK>(e)]);
+        for (int k = 0; k < RH * T / CP::WG; ++k) {
+            const int e = k * CP::WG + threadIdx.x;
+            const size_t g = (size_t)(e / T + h * RH) * S + b0 + (e & (T - 1));
+            if (g < C.nlimbs) gst(C.out + g, l0[padak<CP::PA, CP::PK>(e)]);
+        }
     }
 }
 
@@ -1061,8 +1082,8 @@ static hipError_t set_col_reg_attrs() {
     if constexpr (LL <= 11) {
         if constexpr (col_reg_pair(LL, T)) {
             hipError_t e;
-            if ((e = set_lds3(k3_col_fwd_reg<LL, T>, MAXLDS)) != hipSuccess) return e;
-            if ((e = set_lds3(k3_col_inv_reg<LL, T>, MAXLDS)) != hipSuccess) return e;
+            if ((e = set_lds3(k3_col_fwd_seq<LL, T>, MAXLDS / 3)) != hipSuccess) return e;
+            if ((e = set_lds3(k3_col_inv_seq<LL, T>, (size_t)col_inv_seq_words(LL, T) * 4)) != hipSuccess) return e;
         }
         return set_col_reg_attrs<T, LL + 1>();
     }
@@ -1108,10 +1129,10 @@ static hipError_t launch_cols3(const MulJob* d_jobs, const MulJob& one, unsigned
         case LL: \
             if constexpr (col_reg_pair(LL, T)) { \
                 constexpr int WG = ColPlan<LL, T>::WG; \
-                if (!inverse) PDH3_LAUNCH("k3_col_fwd_reg", s, (k3_col_fwd_reg<LL, T>), dim3(njobs * ops * tiles), \
-                                          dim3(WG), (size_t)3 * words * 4, s, d_jobs, one, ops, n3, ot.tw3, PS); \
-                else PDH3_LAUNCH("k3_col_inv_reg", s, (k3_col_inv_reg<LL, T>), dim3(njobs * tiles), dim3(WG), \
-                                 (size_t)3 * words * 4, s, d_jobs, one, n3, ot.tw3, PS, nblk_max); \
+                if (!inverse) PDH3_LAUNCH("k3_col_fwd_seq", s, (k3_col_fwd_seq<LL, T>), dim3(njobs * ops * tiles), \
+                                          dim3(WG), (size_t)words * 4, s, d_jobs, one, ops, n3, ot.tw3, PS); \
+                else PDH3_LAUNCH("k3_col_inv_seq", s, (k3_col_inv_seq<LL, T>), dim3(njobs * tiles), dim3(WG), \
+                                 (size_t)col_inv_seq_words(LL, T) * 4, s, d_jobs, one, n3, ot.tw3, PS, nblk_max); \
                 done = true; \
             } \
             break;

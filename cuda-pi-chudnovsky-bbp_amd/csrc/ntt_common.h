@@ -18,9 +18,32 @@ namespace pdh {
 typedef __attribute__((address_space(1))) u32 g_u32;
 __device__ __forceinline__ u32 gld(const u32* p) { return *(const g_u32*)p; }
 __device__ __forceinline__ void gst(u32* p, u32 v) { *(g_u32*)p = v; }
+typedef u32 v4u __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) v4u g_v4u;
+__device__ __forceinline__ uint4 gld4(const uint4* p) {
+    const v4u v = *(const g_v4u*)p;
+    return uint4{v.x, v.y, v.z, v.w};
+}
+__device__ __forceinline__ void gst4(uint4* p, uint4 v) { *(g_v4u*)p = v4u{v.x, v.y, v.z, v.w}; }
+typedef __attribute__((address_space(1))) u64 g_u64;
+__device__ __forceinline__ u64 gld64(const u64* p) { return *(const g_u64*)p; }
 // Keeps the machine scheduler from moving instructions across this point
 // (used to issue a group of loads before the arithmetic that consumes them).
 __device__ __forceinline__ void sched_fence() { __builtin_amdgcn_sched_barrier(0); }
+
+// The job descriptor of a workgroup, loaded through the constant address
+// space (scalar loads into SGPRs): from the job array when one is given, else
+// the single job passed as a kernel argument. A plain `jobs ? jobs[i] : one`
+// selects between two generic pointers and loads the descriptor into VGPRs.
+__device__ __forceinline__ MulJob job_of(const MulJob* jobs, size_t i, const MulJob& one) {
+    typedef __attribute__((address_space(4))) const MulJob c_job;
+    if (!jobs) return one;
+    const c_job* p = (const c_job*)jobs + i;
+    MulJob J;
+    J.a = p->a; J.b = p->b; J.out = p->out; J.out_len = p->out_len; J.na = p->na; J.nb = p->nb;
+    J.wsA = p->wsA; J.wsB = p->wsB; J.carry = p->carry;
+    return J;
+}
 
 __device__ __forceinline__ int lpad(int e) { return e + (e >> 3); }
 __host__ __device__ constexpr int lds_words(int e) { return e + (e >> 3); }
@@ -165,15 +188,22 @@ __device__ __forceinline__ CarryJob carry_job(const MulJob& J, size_t nblk_max) 
 }
 
 // Coalesced copy of the workgroup's limb range [L0, L0 + LIMBS) into LDS
-// (zeros at and above nlimbs), and back.
+// (zeros at and above nlimbs), and back. A whole, aligned range moves as
+// 16-byte vectors, all of a thread's loads issued before the first LDS write.
 template <int T>
 __device__ __forceinline__ void stage_in(const CarryJob& c, size_t L0, u32* sD) {
     constexpr int LB = CarryGeom<T>::LIMBS;
+    constexpr int NV = LB / 4 / CARRY_NT;                       // vectors per thread
+    static_assert(LB % (4 * CARRY_NT) == 0, "whole vectors per thread");
     const u32* src = c.out + L0;
     if (L0 + LB <= c.nlimbs && ((uintptr_t)src & 15) == 0) {
-        for (int v = threadIdx.x; v < LB / 4; v += CARRY_NT) {
-            const uint4 q = ((const uint4*)src)[v];
-            sD[cpad(4 * v)] = q.x; sD[cpad(4 * v + 1)] = q.y; sD[cpad(4 * v + 2)] = q.z; sD[cpad(4 * v + 3)] = q.w;
+        uint4 q[NV];
+#pragma unroll
+        for (int k = 0; k < NV; ++k) q[k] = gld4((const uint4*)src + k * CARRY_NT + threadIdx.x);
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            const int v = k * CARRY_NT + threadIdx.x;
+            sD[cpad(4 * v)] = q[k].x; sD[cpad(4 * v + 1)] = q[k].y; sD[cpad(4 * v + 2)] = q[k].z; sD[cpad(4 * v + 3)] = q[k].w;
         }
     } else {
         for (int w = threadIdx.x; w < LB; w += CARRY_NT) sD[cpad(w)] = (L0 + w < c.nlimbs) ? src[w] : 0u;
@@ -182,10 +212,14 @@ __device__ __forceinline__ void stage_in(const CarryJob& c, size_t L0, u32* sD) 
 template <int T>
 __device__ __forceinline__ void stage_out(const CarryJob& c, size_t L0, const u32* sD) {
     constexpr int LB = CarryGeom<T>::LIMBS;
+    constexpr int NV = LB / 4 / CARRY_NT;
     u32* dst = c.out + L0;
     if (L0 + LB <= c.nlimbs && ((uintptr_t)dst & 15) == 0) {
-        for (int v = threadIdx.x; v < LB / 4; v += CARRY_NT)
-            ((uint4*)dst)[v] = uint4{sD[cpad(4 * v)], sD[cpad(4 * v + 1)], sD[cpad(4 * v + 2)], sD[cpad(4 * v + 3)]};
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            const int v = k * CARRY_NT + threadIdx.x;
+            gst4((uint4*)dst + v, uint4{sD[cpad(4 * v)], sD[cpad(4 * v + 1)], sD[cpad(4 * v + 2)], sD[cpad(4 * v + 3)]});
+        }
     } else {
         for (int w = threadIdx.x; w < LB; w += CARRY_NT)
             if (L0 + w < c.nlimbs) dst[w] = sD[cpad(w)];

@@ -212,7 +212,7 @@ __global__ __launch_bounds__(1024) void k_col_fwd(const MulJob* __restrict__ job
     const unsigned rem = bid % (ops * tiles);
     const int op = (int)(rem / tiles);
     const size_t b0 = (size_t)(rem % tiles) * T;
-    const MulJob J = jobs ? jobs[job] : one;
+    const MulJob J = job_of(jobs, job, one);
     const u32* __restrict__ src = op ? J.b : J.a;
     const size_t nl = op ? J.nb : J.na;
     u64* __restrict__ dst = op ? J.wsB : J.wsA;
@@ -252,7 +252,7 @@ __global__ __launch_bounds__(1024) void k_row_mul(const MulJob* __restrict__ job
     const int R = 1 << L2;
     u64* la = lds;
     u64* lb = lds + lds_words(R);
-    const MulJob J = jobs ? jobs[blockIdx.x >> L1] : one;
+    const MulJob J = job_of(jobs, blockIdx.x >> L1, one);
     u64* __restrict__ A = J.wsA;
     const u64* __restrict__ B = J.wsB;
     const int row = (int)(blockIdx.x & ((1u << L1) - 1));
@@ -299,7 +299,7 @@ __global__ __launch_bounds__(CARRY_NT) void k_carry_reduce(const MulJob* __restr
     constexpr int KL = G::KL, CPT = G::CPT;
     __shared__ CarryFn wsum[CARRY_NT / 64];
     __shared__ u32 sD[G::LIMBS + G::LIMBS / 16];
-    const MulJob J = jobs ? jobs[blockIdx.x / nblk_max] : one;
+    const MulJob J = job_of(jobs, blockIdx.x / nblk_max, one);
     const CarryJob C = carry_job<T>(J, nblk_max);
     const size_t blk = blockIdx.x % nblk_max;
     if (blk >= C.nblk) return;                       // whole workgroup leaves together
@@ -337,7 +337,7 @@ __global__ __launch_bounds__(CARRY_NT) void k_carry_reduce_th(const MulJob* __re
     using G = CarryGeom<T>;
     constexpr int CPT = G::CPT;
     __shared__ CarryFn wsum[CARRY_NT / 64];
-    const MulJob J = jobs ? jobs[blockIdx.x / nblk_max] : one;
+    const MulJob J = job_of(jobs, blockIdx.x / nblk_max, one);
     const CarryJob C = carry_job<T>(J, nblk_max);
     const size_t blk = blockIdx.x % nblk_max;
     if (blk >= C.nblk) return;                       // whole workgroup leaves together
@@ -399,7 +399,7 @@ __global__ __launch_bounds__(CARRY_NT) void k_carry_apply(const MulJob* __restri
     __shared__ u64 outs[CARRY_NT];
     __shared__ u64 wmax[CARRY_NT / 64];
     __shared__ u32 sD[G::LIMBS + G::LIMBS / 16];
-    const MulJob J = jobs ? jobs[blockIdx.x / nblk_max] : one;
+    const MulJob J = job_of(jobs, blockIdx.x / nblk_max, one);
     const CarryJob C = carry_job<T>(J, nblk_max);
     // Highest limbs first: the block holding the top nonzero limb usually
     // runs early, and later blocks then skip the atomic on out_len.
@@ -407,22 +407,31 @@ __global__ __launch_bounds__(CARRY_NT) void k_carry_apply(const MulJob* __restri
     if (rb >= C.nblk) return;
     const size_t blk = C.nblk - 1 - rb;
     const size_t L0 = blk * G::LIMBS;
-    const CarryFn prev = blk > 0 ? C.agg[blk - 1] : cf_pad();      // in flight during the staging
-    stage_in<T>(C, L0, sD);
-    __syncthreads();
+    // the aggregate below and the carry-outs are loaded unconditionally (at
+    // clamped indices) so that they are in flight with the staging loads
+    const u64* pa = (const u64*)(C.agg + (blk > 0 ? blk - 1 : 0));
+    const CarryFn pload{gld64(pa), gld64(pa + 1), gld64(pa + 2)};
     const size_t ch0 = (blk * CARRY_NT + threadIdx.x) * CPT;
     u64 Cc[CPT];
+#pragma unroll
+    for (int u = 0; u < CPT; ++u) {
+        const size_t ch = ch0 + u;
+        Cc[u] = gld64(C.C + (ch < C.nchunks ? ch : C.nchunks - 1));
+    }
+    stage_in<T>(C, L0, sD);
+    const CarryFn prev = blk > 0 ? pload : cf_pad();
+#pragma unroll
+    for (int u = 0; u < CPT; ++u) if (ch0 + u >= C.nchunks) Cc[u] = 0;
+    __syncthreads();
     CarryFn f = cf_pad();
     bool first = true;
 #pragma unroll
     for (int u = 0; u < CPT; ++u) {
         const size_t ch = ch0 + u;
-        Cc[u] = 0;
         if (ch < C.nchunks) {
             u32 D[KL];
 #pragma unroll
             for (int q = 0; q < KL; ++q) D[q] = sD[cpad((threadIdx.x * CPT + u) * KL + q)];
-            Cc[u] = C.C[ch];
             const CarryFn g = chunk_fn<KL>(D, Cc[u]);
             f = first ? g : cf_compose(g, f);
             first = false;
@@ -488,7 +497,7 @@ __global__ __launch_bounds__(1024) void k_col_inv(const MulJob* __restrict__ job
     const int R = 1 << L1;
     const unsigned tiles = (unsigned)(S / T);
     const unsigned bid = xcd_tile_order(blockIdx.x, gridDim.x);
-    const MulJob J = jobs ? jobs[bid / tiles] : one;
+    const MulJob J = job_of(jobs, bid / tiles, one);
     const u64* __restrict__ buf = J.wsA;
     const size_t b0 = (size_t)(bid % tiles) * T;
     for (int e = threadIdx.x; e < R * T; e += blockDim.x) {
@@ -596,7 +605,7 @@ __global__ __launch_bounds__(NT) void k_small_mul(const MulJob* __restrict__ job
     extern __shared__ u64 lds[];
     __shared__ CarryFn wsum[NT / 64];
     __shared__ u64 outs[NT];
-    const MulJob J = jobs ? jobs[blockIdx.x] : one;
+    const MulJob J = job_of(jobs, blockIdx.x, one);
     const int R = 1 << n;
     u64* la = lds;
     u64* lb = lds + lds_words(R);
