@@ -109,16 +109,19 @@ __device__ __forceinline__ CarryFn cf_compose(const CarryFn& later, const CarryF
 __device__ __forceinline__ CarryFn cf_pad() { return CarryFn{~(u64)0, 0, 0}; }
 
 constexpr int CARRY_NT = 256;
-// Multi-workgroup carry: each thread takes CPT = max(1, 16 / KL) consecutive
-// chunks (KL limbs each), a workgroup CARRY_NT * CPT chunks.
+// Multi-workgroup carry: each thread takes CPT = max(1, 32 / KL) consecutive
+// chunks (KL limbs each, 32 limbs for every KL <= 32), a workgroup
+// CARRY_NT * CPT chunks. (Against 16 limbs per thread, twice as many
+// workgroups: 2^23 limbs took 9.0 + 21.3 us for the reduce and the apply
+// against 7.9 + 19.5 us; 64 limbs per thread did not gain further.)
 template <int T> struct CarryGeom {
     static constexpr int KL = T / 2;
-    static constexpr int CPT = KL >= 16 ? 1 : 16 / KL;
+    static constexpr int CPT = KL >= 32 ? 1 : 32 / KL;
     static constexpr int LIMBS = CARRY_NT * CPT * KL;          // limbs per workgroup
 };
-// LDS index of a workgroup's limb w: one pad word per 16 keeps the per-thread
-// (16-limb stride) accesses on distinct banks.
-__device__ __forceinline__ int cpad(int w) { return w + (w >> 4); }
+// LDS index of a workgroup's limb w: one pad word per 32 keeps the per-thread
+// (32-limb stride) accesses on distinct banks.
+__device__ __forceinline__ int cpad(int w) { return w + (w >> 5); }
 
 __device__ __forceinline__ CarryFn shfl_up_fn(const CarryFn& f, int d) {
     return CarryFn{__shfl_up(f.thr, d), __shfl_up(f.v0, d), __shfl_up(f.v1, d)};

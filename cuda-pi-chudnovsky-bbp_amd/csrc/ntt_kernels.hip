@@ -298,7 +298,7 @@ __global__ __launch_bounds__(CARRY_NT) void k_carry_reduce(const MulJob* __restr
     using G = CarryGeom<T>;
     constexpr int KL = G::KL, CPT = G::CPT;
     __shared__ CarryFn wsum[CARRY_NT / 64];
-    __shared__ u32 sD[G::LIMBS + G::LIMBS / 16];
+    __shared__ u32 sD[G::LIMBS + G::LIMBS / 32];
     const MulJob J = job_of(jobs, blockIdx.x / nblk_max, one);
     const CarryJob C = carry_job<T>(J, nblk_max);
     const size_t blk = blockIdx.x % nblk_max;
@@ -398,7 +398,7 @@ __global__ __launch_bounds__(CARRY_NT) void k_carry_apply(const MulJob* __restri
     __shared__ CarryFn wsum[CARRY_NT / 64];
     __shared__ u64 outs[CARRY_NT];
     __shared__ u64 wmax[CARRY_NT / 64];
-    __shared__ u32 sD[G::LIMBS + G::LIMBS / 16];
+    __shared__ u32 sD[G::LIMBS + G::LIMBS / 32];
     const MulJob J = job_of(jobs, blockIdx.x / nblk_max, one);
     const CarryJob C = carry_job<T>(J, nblk_max);
     // Highest limbs first: the block holding the top nonzero limb usually
@@ -810,9 +810,9 @@ hipError_t launch_zero_len(uint64_t* out_len, hipStream_t s) {
     return hipSuccess;
 }
 
-// Carry blocks: every shape's workgroup covers at least CARRY_NT * 16 limbs.
+// Carry blocks: every shape's workgroup covers CARRY_NT * 32 limbs.
 size_t carry_nblk(size_t nlimbs) {
-    return (nlimbs + CARRY_NT * 16 - 1) / (CARRY_NT * 16);
+    return (nlimbs + CARRY_NT * 32 - 1) / (CARRY_NT * 32);   // CarryGeom::LIMBS
 }
 
 size_t carry_ws_bytes(size_t nlimbs) {
