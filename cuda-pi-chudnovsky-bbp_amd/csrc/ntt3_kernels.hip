@@ -1062,6 +1062,118 @@ void k3_col_inv_seq(const MulJob* __restrict__ jobs, MulJob one, int n3, const u
     }
 }
 
+// ---- whole multiply in one workgroup, three primes ------------------------------------
+// Products with N = 2^n <= 2^13 (the reference's 16-bit digit count): the
+// 32-bit limbs over N' = 2^(n-1) points modulo each prime in six LDS arrays
+// (A's three residues, then B's), level-at-a-time radix-8 transforms (DIF
+// forward, DIT inverse, the j-major level twiddles), the pointwise product,
+// one Montgomery product by 2^-(n-1) R^2 (undoing the pointwise R^-1 and the
+// transform's factor N', with canonical residues out), the CRT into
+// (lo32, up) over the three arrays and the carry as a block scan.
+
+// Carry of one workgroup's coefficients (lo32 in a[0], up in a[1] | a[2] <<
+// 32, up < 2^54): each thread takes k >= 2 consecutive coefficients, so that
+// a chunk's carry-out with carry-in c < 2^57 is C or C + 1 (the function of
+// (TH, C, C + 1) that the multi-workgroup carry composes); then a block scan
+// gives every chunk its carry-in, and the chunks' limbs go out with the
+// normalized length.
+template <int NT>
+__device__ __forceinline__ void block_carry3(u32* const* a, size_t nlimbs, u32* __restrict__ out,
+                                             u64* __restrict__ out_len, CarryFn* wsum, u64* outs) {
+    size_t k = (nlimbs + NT - 1) / NT;
+    if (k < 2) k = 2;
+    const size_t l0 = (size_t)threadIdx.x * k;
+    const size_t l1 = l0 + k < nlimbs ? l0 + k : nlimbs;
+    CarryFn f = cf_pad();
+    if (l0 < nlimbs) {
+        u64 carry = 0, lo = 0;
+        bool ones = true;
+        for (size_t l = l0; l < l1; ++l) {
+            const int e = lpad((int)l);
+            const u64 up = (u64)a[1][e] | ((u64)a[2][e] << 32);
+            const u64 sum = (carry & 0xFFFFFFFFull) + a[0][e];
+            const u32 limb = (u32)sum;
+            carry = (sum >> 32) + (carry >> 32) + up;
+            if (l - l0 < 2) lo |= (u64)limb << (32 * (l - l0));
+            else ones &= limb == 0xFFFFFFFFu;
+        }
+        // a one-limb chunk can only be the last one, whose function only
+        // reaches the padding threads after it
+        f = CarryFn{(l1 - l0 >= 2 && ones && lo != 0) ? (u64)(0 - lo) : ~(u64)0, carry, carry + 1};
+    }
+    const CarryFn inc = block_scan(f, wsum);
+    outs[threadIdx.x] = cf_apply(inc, 0);
+    __syncthreads();
+    u64 carry = threadIdx.x == 0 ? 0 : outs[threadIdx.x - 1];
+    u64 top = 0;
+    for (size_t l = l0; l < l1; ++l) {
+        const int e = lpad((int)l);
+        const u64 up = (u64)a[1][e] | ((u64)a[2][e] << 32);
+        const u64 sum = (carry & 0xFFFFFFFFull) + a[0][e];
+        const u32 limb = (u32)sum;
+        carry = (sum >> 32) + (carry >> 32) + up;
+        gst(out + l, limb);
+        if (limb) top = l + 1;
+    }
+    top = wave_max(top);
+    __syncthreads();                                  // outs[] is reused below
+    if ((threadIdx.x & 63) == 0) outs[threadIdx.x >> 6] = top;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        u64 m = 0;
+        for (int w = 0; w < NT / 64; ++w) m = outs[w] > m ? outs[w] : m;
+        *out_len = m;
+    }
+}
+
+template <bool SQUARE, int NT>
+__global__ __launch_bounds__(NT) void k3_small_mul(const MulJob* __restrict__ jobs, MulJob one, int n3,
+                                                   const u32* __restrict__ tw3, Primes3 PS, u32 sc0, u32 sc1,
+                                                   u32 sc2) {
+    extern __shared__ u32 lds3[];
+    __shared__ CarryFn wsum[NT / 64];
+    __shared__ u64 outs[NT];
+    const MulJob J = job_of(jobs, blockIdx.x, one);
+    const int R = 1 << n3;
+    const int words = lds_words(R);
+    u32* a[3] = {lds3, lds3 + words, lds3 + 2 * words};
+    u32* b[3] = {lds3 + 3 * words, lds3 + 4 * words, lds3 + 5 * words};
+    for (int e = threadIdx.x; e < R; e += NT) {
+        const int i = lpad(e);
+        const u32 va = (size_t)e < J.na ? gld(J.a + e) : 0u;
+#pragma unroll
+        for (int p = 0; p < 3; ++p) a[p][i] = r_limb_mont(va, PS.p[p]);
+        if constexpr (!SQUARE) {
+            const u32 vb = (size_t)e < J.nb ? gld(J.b + e) : 0u;
+#pragma unroll
+            for (int p = 0; p < 3; ++p) b[p][i] = r_limb_mont(vb, PS.p[p]);
+        }
+    }
+    __syncthreads();
+    r_lds_dft3<false>(a, n3, 0, tw3, PS);
+    if constexpr (!SQUARE) r_lds_dft3<false>(b, n3, 0, tw3, PS);
+    for (int e = threadIdx.x; e < R; e += NT) {
+        const int i = lpad(e);
+#pragma unroll
+        for (int p = 0; p < 3; ++p) {
+            const u32 x = a[p][i];
+            a[p][i] = mml(x, SQUARE ? x : b[p][i], PS.p[p].p, PS.p[p].ninv);
+        }
+    }
+    __syncthreads();
+    r_lds_dft3<true>(a, n3, 0, tw3, PS);
+    for (int e = threadIdx.x; e < R; e += NT) {
+        const int i = lpad(e);
+        u32 lo32;
+        u64 up;
+        crt3(mm(a[0][i], sc0, PS.p[0].p, PS.p[0].ninv), mm(a[1][i], sc1, PS.p[1].p, PS.p[1].ninv),
+             mm(a[2][i], sc2, PS.p[2].p, PS.p[2].ninv), PS, lo32, up);
+        a[0][i] = lo32; a[1][i] = (u32)up; a[2][i] = (u32)(up >> 32);
+    }
+    __syncthreads();
+    block_carry3<NT>(a, J.na + J.nb, J.out, J.out_len, wsum, outs);
+}
+
 // ---- launcher -----------------------------------------------------------------------
 // Row lengths of the register-resident row kernel and its LDS per workgroup.
 #define PDH3_FOR_ROW(X) X(8) X(9) X(10) X(11) X(12) X(13) X(14)
@@ -1105,6 +1217,10 @@ hipError_t init_kernel_attrs3() {
     if ((e = set_lds3(k3_row_reg<LL, false>, row_reg_lds(LL, false))) != hipSuccess) return e;
     PDH3_FOR_ROW(PDH3_SET_ROW)
 #undef PDH3_SET_ROW
+    if ((e = set_lds3(k3_small_mul<false, 1024>, MAXLDS)) != hipSuccess) return e;
+    if ((e = set_lds3(k3_small_mul<true, 1024>, MAXLDS)) != hipSuccess) return e;
+    if ((e = set_lds3(k3_small_mul<false, 512>, MAXLDS)) != hipSuccess) return e;
+    if ((e = set_lds3(k3_small_mul<true, 512>, MAXLDS)) != hipSuccess) return e;
     if ((e = set_col_reg_attrs<4>()) != hipSuccess) return e;
     if ((e = set_col_reg_attrs<8>()) != hipSuccess) return e;
     if ((e = set_col_reg_attrs<16>()) != hipSuccess) return e;
@@ -1162,6 +1278,40 @@ static hipError_t launch_cols3_T(const MulJob* d_jobs, const MulJob& one, unsign
     case 32: return launch_cols3<32>(d_jobs, one, njobs, ops, n3, sh, ot, s, inverse, nblk_max);
     default: return hipErrorInvalidValue;
     }
+}
+
+// One workgroup per product (N = 2^n <= 2^13): the scale 2^-(n-1) R^2 per
+// prime (as Montgomery constants).
+hipError_t launch_small3(const MulJob* d_jobs, const MulJob& one, unsigned njobs, int n, bool square,
+                         const u32* tw3, hipStream_t s) {
+    const int n3 = n - 1;
+    const Primes3& PS = primes3();
+    u32 sc[3];
+    for (int pi = 0; pi < 3; ++pi) {
+        const u32 p = PS.p[pi].p;
+        const u32 inv = r3_pow(r3_pow(2, (uint64_t)n3, p), p - 2, p);   // 2^-n3
+        sc[pi] = r3_to_mont(r3_to_mont(inv, p), p);
+    }
+    const size_t lds = (size_t)(square ? 3 : 6) * lds_words(1 << n3) * 4;
+    const dim3 g(njobs);
+#define PDH3_SMALL(NT)                                                                                              \
+    do {                                                                                                            \
+        if (square) PDH3_LAUNCH("k3_small_mul", s, (k3_small_mul<true, NT>), g, dim3(NT), lds, s, d_jobs, one, n3,  \
+                                tw3, PS, sc[0], sc[1], sc[2]);                                                      \
+        else PDH3_LAUNCH("k3_small_mul", s, (k3_small_mul<false, NT>), g, dim3(NT), lds, s, d_jobs, one, n3, tw3,   \
+                         PS, sc[0], sc[1], sc[2]);                                                                  \
+    } while (0)
+    // N'/4 threads (at least a wave, at most 1,024), measured per size on
+    // batches of equal products: against N'/8 and N'/2, 2^10..2^13-point
+    // products (16-bit digit count) ran at 73.9, 32.9, 15.4, 6.7 M/s against
+    // 69.5, 33.0, 14.2, 6.6 and 67.2, 25.7, 10.0, 6.8 M/s
+    if (n3 >= 12) PDH3_SMALL(1024);
+    else if (n3 == 11) PDH3_SMALL(512);
+    else if (n3 == 10) PDH3_SMALL(256);
+    else if (n3 == 9) PDH3_SMALL(128);
+    else PDH3_SMALL(64);
+#undef PDH3_SMALL
+    return hipSuccess;
 }
 
 hipError_t launch_class3(const MulJob* d_jobs, const MulJob& one, unsigned njobs, const NttShape& sh, bool square,

@@ -639,7 +639,12 @@ __global__ void k_zero_len(u64* out_len) { *out_len = 0; }
 // space): {L1, T} for n = 14..24; Goldilocks radix 8 for both passes.
 static NttShape default_shape_eng(int n, int eng) {
     NttShape s{n, 0, n, 1, 3, 3, 0};
-    if (n <= SMALL_MAX_LOG) return s;
+    if (n <= SMALL_MAX_LOG) {
+        // one workgroup: Goldilocks over 2^n points, or the three primes over
+        // 2^(n-1) points (radix-8 levels through LDS)
+        if (eng == 1) { s.eng = 1; s.L2 = n - 1; }
+        return s;
+    }
     if (eng == 0) {
         static const int tab[MAX_LOG + 1][2] = {
             {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0},
@@ -663,7 +668,10 @@ static NttShape default_shape_eng(int n, int eng) {
     s.T = tab3[n][1];
     return s;
 }
-static NttShape default_shape(int n) { return default_shape_eng(n, n > SMALL_MAX_LOG ? 1 : 0); }
+// Engine 1 everywhere from N = 4: on batches of equal products the
+// one-workgroup three-prime kernel ran 1.6-1.9x the Goldilocks one at every
+// N = 2^4..2^13 (2^9: 137 against 81 M/s; 2^13: 6.7 against 4.1 M/s).
+static NttShape default_shape(int n) { return default_shape_eng(n, n >= 2 ? 1 : 0); }
 
 static NttShape g_shapes[MAX_LOG + 1];
 static bool g_shapes_init = false;
@@ -679,7 +687,9 @@ NttShape ntt_shape(int n) {
 bool shape_valid(const NttShape& s) {
     if (s.n < 1 || s.n > MAX_LOG) return false;
     if (s.lr_col < 3 || s.lr_col > 4 || s.lr_row < 3 || s.lr_row > 4) return false;
-    if (s.L1 == 0) return s.n <= SMALL_MAX_LOG && s.L2 == s.n && s.eng == 0;
+    if (s.L1 == 0)
+        return s.n <= SMALL_MAX_LOG && s.T == 1 && ((s.eng == 0 && s.L2 == s.n) ||
+                                                   (s.eng == 1 && s.n >= 2 && s.L2 == s.n - 1 && s.lr_col == 3 && s.lr_row == 3));
     if (s.n <= SMALL_MAX_LOG) return false;
     if (s.T != 4 && s.T != 8 && s.T != 16 && s.T != 32 && s.T != 64) return false;
     if (s.T > (1 << s.L2) || s.L1 < 1 || s.L2 < 3) return false;
@@ -695,7 +705,8 @@ bool shape_valid(const NttShape& s) {
     return s.L1 + s.L2 == s.n - 1 && (size_t)3 * lds_words((1 << s.L1) * s.T) * 4 <= 147456;
 }
 
-NttShape default_shape_for(int n, int eng) { return default_shape_eng(n, eng); }
+// eng = -1: the default engine for n
+NttShape default_shape_for(int n, int eng) { return eng < 0 ? default_shape(n) : default_shape_eng(n, eng); }
 
 static bool g_custom[MAX_LOG + 1];   // shape set by pdh_tune_set / pdh_tune_engine / autotune
 
@@ -717,7 +728,7 @@ bool shape_custom(int n) { return g_custom[n]; }
 // used as it is.
 NttShape batch_shape(int n, unsigned njobs) {
     const NttShape s = ntt_shape(n);
-    if (g_custom[n] || njobs < 16 || s.eng != 1 || n > 18) return s;
+    if (g_custom[n] || njobs < 16 || s.eng != 1 || s.L1 == 0 || n > 18) return s;
     NttShape t = s;
     t.L1 = 4; t.L2 = n - 1 - 4; t.T = 32;
     return shape_valid(t) ? t : s;
@@ -836,6 +847,7 @@ hipError_t launch_class(const MulJob* d_jobs, const MulJob& one, unsigned njobs,
                         const NttShape* shape) {
     const NttShape sh = shape ? *shape : ntt_shape(n);
     const size_t N = (size_t)1 << n;
+    if (sh.L1 == 0 && sh.eng == 1) return launch_small3(d_jobs, one, njobs, n, square, ot.tw3, s);
     if (sh.L1 == 0) {
         const size_t lds = (size_t)(square ? 1 : 2) * lds_words((int)N) * 8;
         const u64 ninv = gl_inv((u64)N);

@@ -563,7 +563,7 @@ int mul_dev_locked(Ctx& c, const u32* a, size_t na, const u32* b, size_t nb, boo
     if (n > MAX_LOG) return mul_blocked_locked(c, a, na, b, nb, out, out_len, s);
     int rc = ensure_ws(c, N, na + nb);
     if (rc) return rc;
-    OuterTables ot{nullptr, nullptr, nullptr, nullptr, 0, nullptr, nullptr, 0};
+    OuterTables ot{nullptr, nullptr, nullptr, nullptr, 0, c.tw3, nullptr, 0};   // tw3: the small three-prime path
     const NttShape sh = ntt_shape(n);
     if (sh.L1 > 0 && (rc = outer_for(c, n, &ot))) return rc;
     if ((rc = scratch_acquire(c.ws_order, s))) return rc;
@@ -950,6 +950,7 @@ int pdh_mul_dev_noalloc(const uint32_t* d_a, size_t na, const uint32_t* d_b, siz
         const int n = log2_exact(pdh_ntt_length(na, nb));
         const NttShape sh = ntt_shape(n);
         OuterTables ot{};
+        ot.tw3 = c.tw3;
         if (sh.L1 > 0) {
             auto it = c.outer.find(n);                 // built at set-up
             if (it == c.outer.end()) return fail(PDH_E_HIP, "pdh_mul_dev_noalloc: twiddle table missing");
@@ -1226,7 +1227,7 @@ int batch_create_impl(const pdh_mul_desc* descs, size_t n, pdh_batch** plan, boo
         cl.d_jobs = p->d_jobs + first;
         cl.nblk_max = carry_nblk((size_t)1 << (kv.first - 1));
         cl.shape = batch_shape(kv.first, cl.njobs);
-        cl.ot = OuterTables{nullptr, nullptr, nullptr, nullptr, 0, nullptr, nullptr, 0};
+        cl.ot = OuterTables{nullptr, nullptr, nullptr, nullptr, 0, c.tw3, nullptr, 0};
         if (kv.first > SMALL_MAX_LOG && (rc = outer_for(c, kv.first, &cl.ot))) { batch_free(p); return rc; }
         first += kv.second.size();
         p->classes.push_back(cl);
@@ -1671,7 +1672,7 @@ int pdh_tune_set(int n, int L1, int T, int lr_col, int lr_row) {
     std::lock_guard<std::mutex> g(c.mu);
     const int eng = ntt_shape(n).eng;
     NttShape sh{n, L1, n - eng - L1, T, lr_col, lr_row, eng};
-    if (L1 == 0) { sh.L2 = n; sh.T = 1; sh.lr_col = 3; }
+    if (L1 == 0) { sh.L2 = n - eng; sh.T = 1; sh.lr_col = 3; }
     if (!set_shape(sh)) return fail(PDH_E_ARG, "shape cannot be launched");
     return PDH_OK;
 }
@@ -1681,8 +1682,8 @@ int pdh_tune_engine(int n, int engine, int* current) {
     Ctx& c = ctx();
     std::lock_guard<std::mutex> g(c.mu);
     if (engine >= 0 && engine != ntt_shape(n).eng) {
-        if (n <= SMALL_MAX_LOG) return fail(PDH_E_ARG, "products with N <= 2^13 run on one workgroup (engine 0)");
-        set_shape(default_shape_for(n, engine), engine != 1);   // engine 1 is the default above 2^13
+        if (n < 2) return fail(PDH_E_ARG, "no product has N = 2");
+        set_shape(default_shape_for(n, engine), engine != default_shape_for(n, -1).eng);
     }
     if (current) *current = ntt_shape(n).eng;
     return PDH_OK;
@@ -1726,7 +1727,9 @@ int pdh_autotune(int n_lo, int n_hi, int reps, void* stream) {
         std::vector<NttShape> cands;
         const NttShape def = ntt_shape(n);
         if (n <= SMALL_MAX_LOG) {
-            for (int lr = 3; lr <= 4; ++lr) cands.push_back(NttShape{n, 0, n, 1, 3, lr});
+            // one workgroup: Goldilocks at either radix, or the three primes
+            for (int lr = 3; lr <= 4; ++lr) cands.push_back(NttShape{n, 0, n, 1, 3, lr, 0});
+            cands.push_back(default_shape_for(n, 1));
         } else {
             for (int L1 = def.L1 - 2; L1 <= def.L1 + 2; ++L1)
                 for (int T : {4, 8, 16, 32, 64})

@@ -97,6 +97,10 @@ hipError_t init_kernel_attrs();
 hipError_t init_kernel_attrs3();
 hipError_t launch_class3(const MulJob* d_jobs, const MulJob& one, unsigned njobs, const NttShape& sh, bool square,
                          size_t nblk_max, const OuterTables& ot, hipStream_t s);
+// One workgroup per product on the three primes (N = 2^n <= 2^13; tw3 = the
+// level twiddle table).
+hipError_t launch_small3(const MulJob* d_jobs, const MulJob& one, unsigned njobs, int n, bool square,
+                         const uint32_t* tw3, hipStream_t s);
 // The carry scan over chunks of `kl` limbs (digits and carry-outs already in
 // the product buffer and the carry scratch; with `th`, also the chunk
 // thresholds, so the reduce reads no digits).

@@ -26,7 +26,8 @@ def lds_words(e):
 def shapes(n, eng):
     out = []
     if n <= 13:
-        return [(0, 1, 3, lr) for lr in (3, 4)]
+        # one workgroup: Goldilocks at radix 8 or 16, or the three primes
+        return [(0, 1, 3, lr) for lr in (3, 4)] if eng == 0 else [(0, 1, 3, 3)]
     pts = n - eng
     for L1 in range(1, pts - 2):
         L2 = pts - L1
@@ -58,7 +59,8 @@ def restore_shapes(pdh, tmp_path):
     pdh.tune_load(path)
 
 
-CASES = [(5, 0), (12, 0), (13, 0), (14, 0), (14, 1), (15, 1), (17, 0), (17, 1), (19, 0), (19, 1)]
+CASES = [(5, 0), (5, 1), (12, 0), (12, 1), (13, 0), (13, 1), (14, 0), (14, 1), (15, 1), (17, 0), (17, 1), (19, 0),
+         (19, 1)]
 
 
 @pytest.mark.parametrize("n,eng", CASES)
@@ -67,8 +69,7 @@ def test_every_shape_is_exact(pdh, restore_shapes, n, eng):
     a, b = W.operand(500 + n, N // 4), W.operand(600 + n, N // 4 - 1)
     want = oracle.mul_ntt(a, b)
     want_sq = oracle.mul_ntt(a, a)
-    if n > 13:
-        assert pdh.tune_engine(n, eng) == eng
+    assert pdh.tune_engine(n, eng) == eng
     for sh in shapes(n, eng):
         pdh.tune_set(n, *sh)
         assert np.array_equal(pdh.mul_ntt_hip(a, b), want), (n, eng, sh)
@@ -102,13 +103,16 @@ def test_invalid_shape_rejected(pdh, restore_shapes):
         pdh.tune_set(24, 8, 8, 3, 3)          # engine 1 rows longer than 2^14
     with pytest.raises(pdh.PdhError):
         pdh.tune_set(14, 6, 4, 3, 4)          # radix-16 row sets need rows of 2^8 or more
+    pdh.tune_engine(13, 1)                    # one workgroup on the three primes
     with pytest.raises(pdh.PdhError):
-        pdh.tune_engine(13, 1)                # one-workgroup sizes stay on engine 0
+        pdh.tune_set(13, 0, 1, 3, 4)          # which runs radix-8 levels only
+    with pytest.raises(pdh.PdhError):
+        pdh.tune_engine(1, 1)                 # no product has N = 2
 
 
 def test_engines_agree(pdh, restore_shapes):
     rng = np.random.default_rng(31)
-    for n in (14, 16, 18, 20, 22):
+    for n in (8, 12, 14, 16, 18, 20, 22):
         N = 1 << n
         na = int(rng.integers(N // 8, N // 4))
         a, b = W.operand(900 + n, na), W.operand(950 + n, N // 2 - na)
