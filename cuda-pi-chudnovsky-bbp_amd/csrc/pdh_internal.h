@@ -12,7 +12,8 @@ namespace pdh {
 
 // How a product with N = 2^n (16-bit digit count, include/ntt.h:47-50) is
 // transformed. n <= SMALL_MAX_LOG runs as one workgroup per multiply
-// (Goldilocks). Above that, engine 0 is the Goldilocks four-step over 2^n
+// (L1 = 0): engine 0 over 2^n Goldilocks points (L2 = n), engine 1 over
+// 2^(n-1) points of the three primes (L2 = n - 1, the default). Above that, engine 0 is the Goldilocks four-step over 2^n
 // points (L1 + L2 = n) and engine 1 the three-prime four-step over 32-bit
 // digits, 2^(n-1) points (L1 + L2 = n - 1): a column pass over the top L1
 // index bits (T columns per workgroup) and a row pass over the low L2 bits.

@@ -144,18 +144,19 @@ int pdh_mul_batch_dev(const pdh_mul_desc* descs, size_t n, void* stream);
  * A 2^n-point transform (n > 13) runs as a column pass over the top L1 index
  * bits (T adjacent columns per workgroup, radix 2^lr_col butterflies) and a
  * fused row pass over the low n - L1 bits (radix 2^lr_row); n <= 13 runs in
- * one workgroup (radix 2^lr_row). On the three-prime engine lr_col is 3 and
- * lr_row selects the kernels: 4 (default, rows of 2^8 points or more)
- * register-resident radix-16 rows and radix-8 columns, 3 radix-8 levels
- * through LDS. Every valid shape gives the same product; only speed differs.
- * No counterpart in the reference (its radix-2 transform has no parameters,
- * src/ntt_cuda.cu:136-159). */
+ * one workgroup (L1 = 0; radix 2^lr_row on engine 0, radix 8 on engine 1).
+ * On the three-prime engine lr_col is 3 and lr_row selects the kernels: 4
+ * (default, rows of 2^8 points or more) register-resident radix-16 rows and
+ * radix-8 columns, 3 radix-8 levels through LDS. Every valid shape gives the
+ * same product; only speed differs. No counterpart in the reference (its
+ * radix-2 transform has no parameters, src/ntt_cuda.cu:136-159). */
 int pdh_tune_get(int n, int* L1, int* T, int* lr_col, int* lr_row);
 /* PDH_E_ARG if the shape cannot be launched. */
 int pdh_tune_set(int n, int L1, int T, int lr_col, int lr_row);
-/* Engine for products with N = 2^n > 2^13: 0 = Goldilocks over 2^n points
- * (16-bit digits, the reference's field), 1 = three primes below 2^31 over
- * 2^(n-1) points (32-bit digits, CRT); the default is 1. engine = -1 only
+/* Engine for products with N = 2^n (n >= 2): 0 = Goldilocks over 2^n
+ * points (16-bit digits, the reference's field), 1 = three primes below 2^31
+ * over 2^(n-1) points (32-bit digits, CRT); the default is 1, for the
+ * one-workgroup sizes (N <= 2^13) as for the four-step ones. engine = -1 only
  * reads. Switching resets that n to the engine's default shape, after which
  * pdh_tune_get/set use that engine's split (L1 + L2 = n - engine). Every
  * engine and shape gives the same product bits. */
