@@ -1174,6 +1174,88 @@ __global__ __launch_bounds__(NT) void k3_small_mul(const MulJob* __restrict__ jo
     block_carry3<NT>(a, J.na + J.nb, J.out, J.out_len, wsum, outs);
 }
 
+// Register-resident one-workgroup product for N' = 2^L2 = 2^8..2^12 points:
+// the three primes as three "rows" of the row kernel's register sets (TPR =
+// N'/16 threads per prime, 16 points of A and of B each, radix-16 sets and
+// the per-exchange LDS layouts), with no outer twiddle (the whole transform
+// is one row); the scale 2^-L2 R^2 per prime, the residues through LDS to the
+// CRT and the block carry as in k3_small_mul. The workgroup is 3 TPR threads
+// rounded up to whole waves; at L2 = 8 and 9 the primes share waves (their
+// exchanges are wave-local, so the padding lanes take no barrier).
+template <int L2>
+struct SmallPlan {
+    static constexpr int TPR = RowPlan<L2>::TPR;
+    static constexpr int WG = (3 * TPR + 63) / 64 * 64;
+    static constexpr int WORDS = RowPlan<L2>::WORDS;
+};
+template <int L2, bool SQUARE>
+constexpr size_t small_reg_lds() {
+    constexpr size_t ex = (size_t)(SQUARE ? 3 : 6) * RowPlan<L2>::WORDS;
+    constexpr size_t crt = (size_t)3 * lds_words(1 << L2);
+    return (ex > crt ? ex : crt) * 4;
+}
+
+template <int L2, bool SQUARE>
+__global__ __launch_bounds__(SmallPlan<L2>::WG) void k3_small_reg(const MulJob* __restrict__ jobs, MulJob one,
+                                                                  const u32* __restrict__ tw3, Primes3 PS,
+                                                                  u32 sc0, u32 sc1, u32 sc2) {
+    using RP = RowPlan<L2>;
+    using SP = SmallPlan<L2>;
+    static_assert(L2 >= 8 && L2 <= 12, "register sets of 16 points per thread");
+    static_assert(SP::TPR <= 64 || 3 * SP::TPR == SP::WG, "padding lanes only where exchanges are wave-local");
+    constexpr int NOP = SQUARE ? 1 : 2;
+    constexpr int TPR = SP::TPR;
+    constexpr int R = 1 << L2;
+    extern __shared__ u32 lds3[];
+    __shared__ CarryFn wsum[SP::WG / 64];
+    __shared__ u64 outs[SP::WG];
+    const MulJob J = job_of(jobs, blockIdx.x, one);
+    const int g = threadIdx.x / TPR;                 // the prime of this thread's row
+    const int u = threadIdx.x % TPR;
+    u32 x[NOP][16];
+    if (g < 3) {
+        const Prime3 P = pick3(PS, g);
+        const u32* tw = tw3 + (size_t)g * TW3_STRIDE;
+        u32* l[2] = {lds3 + g * RP::WORDS, lds3 + (3 + g) * RP::WORDS};
+        u32 w0[16];
+        set_tw<L2, 0, false>(w0, u, tw);
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+            const size_t b = (size_t)t * TPR + u;
+            x[0][t] = r_limb_mont(b < J.na ? gld(J.a + b) : 0u, P);
+            if constexpr (NOP == 2) x[1][t] = r_limb_mont(b < J.nb ? gld(J.b + b) : 0u, P);
+        }
+        row_fwd<L2, NOP>(x, w0, u, l, tw, P);
+#pragma unroll
+        for (int t = 0; t < 16; ++t) x[0][t] = mml(x[0][t], x[NOP - 1][t], P.p, P.ninv);
+        {
+            u32 w[16];
+            set_tw<L2, RP::NSET - 1, true>(w, u, tw);
+            row_inv<L2>(x[0], w, u, l[0], tw, P);
+        }
+        const u32 sc = g == 0 ? sc0 : (g == 1 ? sc1 : sc2);
+#pragma unroll
+        for (int t = 0; t < 16; ++t) x[0][t] = mm(x[0][t], sc, P.p, P.ninv);
+    }
+    __syncthreads();                                  // every exchange has been read
+    if (g < 3) {
+        u32* rp = lds3 + g * lds_words(R);
+#pragma unroll
+        for (int t = 0; t < 16; ++t) rp[lpad(t * TPR + u)] = x[0][t];
+    }
+    __syncthreads();
+    u32* a[3] = {lds3, lds3 + lds_words(R), lds3 + 2 * lds_words(R)};
+    for (int b = threadIdx.x; b < R; b += SP::WG) {
+        const int i = lpad(b);
+        u32 lo32;
+        u64 up;
+        crt3(a[0][i], a[1][i], a[2][i], PS, lo32, up);
+        a[0][i] = lo32; a[1][i] = (u32)up; a[2][i] = (u32)(up >> 32);
+    }
+    __syncthreads();
+    block_carry3<SP::WG>(a, J.na + J.nb, J.out, J.out_len, wsum, outs);
+}
+
 // ---- launcher -----------------------------------------------------------------------
 // Row lengths of the register-resident row kernel and its LDS per workgroup.
 #define PDH3_FOR_ROW(X) X(8) X(9) X(10) X(11) X(12) X(13) X(14)
@@ -1217,6 +1299,11 @@ hipError_t init_kernel_attrs3() {
     if ((e = set_lds3(k3_row_reg<LL, false>, row_reg_lds(LL, false))) != hipSuccess) return e;
     PDH3_FOR_ROW(PDH3_SET_ROW)
 #undef PDH3_SET_ROW
+#define PDH3_SET_SMALL(LL) \
+    if ((e = set_lds3(k3_small_reg<LL, false>, small_reg_lds<LL, false>())) != hipSuccess) return e; \
+    if ((e = set_lds3(k3_small_reg<LL, true>, small_reg_lds<LL, true>())) != hipSuccess) return e;
+    PDH3_SET_SMALL(8) PDH3_SET_SMALL(9) PDH3_SET_SMALL(10) PDH3_SET_SMALL(11) PDH3_SET_SMALL(12)
+#undef PDH3_SET_SMALL
     if ((e = set_lds3(k3_small_mul<false, 1024>, MAXLDS)) != hipSuccess) return e;
     if ((e = set_lds3(k3_small_mul<true, 1024>, MAXLDS)) != hipSuccess) return e;
     if ((e = set_lds3(k3_small_mul<false, 512>, MAXLDS)) != hipSuccess) return e;
@@ -1292,8 +1379,23 @@ hipError_t launch_small3(const MulJob* d_jobs, const MulJob& one, unsigned njobs
         const u32 inv = r3_pow(r3_pow(2, (uint64_t)n3, p), p - 2, p);   // 2^-n3
         sc[pi] = r3_to_mont(r3_to_mont(inv, p), p);
     }
-    const size_t lds = (size_t)(square ? 3 : 6) * lds_words(1 << n3) * 4;
     const dim3 g(njobs);
+    if (n3 >= 8 && n3 <= 12) {
+        // register-resident sets (k3_small_reg)
+        switch (n3) {
+#define PDH3_SMALL_REG(LL)                                                                                         \
+        case LL:                                                                                                   \
+            if (square) PDH3_LAUNCH("k3_small_reg", s, (k3_small_reg<LL, true>), g, dim3(SmallPlan<LL>::WG),       \
+                                    (small_reg_lds<LL, true>()), s, d_jobs, one, tw3, PS, sc[0], sc[1], sc[2]);   \
+            else PDH3_LAUNCH("k3_small_reg", s, (k3_small_reg<LL, false>), g, dim3(SmallPlan<LL>::WG),             \
+                             (small_reg_lds<LL, false>()), s, d_jobs, one, tw3, PS, sc[0], sc[1], sc[2]);         \
+            return hipSuccess;
+        PDH3_SMALL_REG(8) PDH3_SMALL_REG(9) PDH3_SMALL_REG(10) PDH3_SMALL_REG(11) PDH3_SMALL_REG(12)
+#undef PDH3_SMALL_REG
+        default: break;
+        }
+    }
+    const size_t lds = (size_t)(square ? 3 : 6) * lds_words(1 << n3) * 4;
 #define PDH3_SMALL(NT)                                                                                              \
     do {                                                                                                            \
         if (square) PDH3_LAUNCH("k3_small_mul", s, (k3_small_mul<true, NT>), g, dim3(NT), lds, s, d_jobs, one, n3,  \
