@@ -1304,10 +1304,6 @@ hipError_t init_kernel_attrs3() {
     if ((e = set_lds3(k3_small_reg<LL, true>, small_reg_lds<LL, true>())) != hipSuccess) return e;
     PDH3_SET_SMALL(8) PDH3_SET_SMALL(9) PDH3_SET_SMALL(10) PDH3_SET_SMALL(11) PDH3_SET_SMALL(12)
 #undef PDH3_SET_SMALL
-    if ((e = set_lds3(k3_small_mul<false, 1024>, MAXLDS)) != hipSuccess) return e;
-    if ((e = set_lds3(k3_small_mul<true, 1024>, MAXLDS)) != hipSuccess) return e;
-    if ((e = set_lds3(k3_small_mul<false, 512>, MAXLDS)) != hipSuccess) return e;
-    if ((e = set_lds3(k3_small_mul<true, 512>, MAXLDS)) != hipSuccess) return e;
     if ((e = set_col_reg_attrs<4>()) != hipSuccess) return e;
     if ((e = set_col_reg_attrs<8>()) != hipSuccess) return e;
     if ((e = set_col_reg_attrs<16>()) != hipSuccess) return e;
@@ -1395,24 +1391,14 @@ hipError_t launch_small3(const MulJob* d_jobs, const MulJob& one, unsigned njobs
         default: break;
         }
     }
+    // up to 2^7 points: radix-8 levels through LDS, one wave (at the sizes
+    // the register sets take over, N'/4 threads had been the best of N'/8,
+    // N'/4 and N'/2, and the sets ran 1.1-1.5x that)
     const size_t lds = (size_t)(square ? 3 : 6) * lds_words(1 << n3) * 4;
-#define PDH3_SMALL(NT)                                                                                              \
-    do {                                                                                                            \
-        if (square) PDH3_LAUNCH("k3_small_mul", s, (k3_small_mul<true, NT>), g, dim3(NT), lds, s, d_jobs, one, n3,  \
-                                tw3, PS, sc[0], sc[1], sc[2]);                                                      \
-        else PDH3_LAUNCH("k3_small_mul", s, (k3_small_mul<false, NT>), g, dim3(NT), lds, s, d_jobs, one, n3, tw3,   \
-                         PS, sc[0], sc[1], sc[2]);                                                                  \
-    } while (0)
-    // N'/4 threads (at least a wave, at most 1,024), measured per size on
-    // batches of equal products: against N'/8 and N'/2, 2^10..2^13-point
-    // products (16-bit digit count) ran at 73.9, 32.9, 15.4, 6.7 M/s against
-    // 69.5, 33.0, 14.2, 6.6 and 67.2, 25.7, 10.0, 6.8 M/s
-    if (n3 >= 12) PDH3_SMALL(1024);
-    else if (n3 == 11) PDH3_SMALL(512);
-    else if (n3 == 10) PDH3_SMALL(256);
-    else if (n3 == 9) PDH3_SMALL(128);
-    else PDH3_SMALL(64);
-#undef PDH3_SMALL
+    if (square) PDH3_LAUNCH("k3_small_mul", s, (k3_small_mul<true, 64>), g, dim3(64), lds, s, d_jobs, one, n3, tw3, PS,
+                            sc[0], sc[1], sc[2]);
+    else PDH3_LAUNCH("k3_small_mul", s, (k3_small_mul<false, 64>), g, dim3(64), lds, s, d_jobs, one, n3, tw3, PS,
+                     sc[0], sc[1], sc[2]);
     return hipSuccess;
 }
 
