@@ -369,17 +369,22 @@ def bench_batch(pdh, torch, W, stream, reps=3, pairs=None, reps_plain=None):
     same = bool(torch.equal(out, ref_out) and torch.equal(lens, ref_lens))
     plan.close()
     n = len(pairs)
-    # the survey's per-product model B_mul = 96 N + 8 (na + nb), summed over
-    # the batch (N = pdh_ntt_length), and the limbs the batch must move at
-    # least (operands in, products out)
-    sum_bmul = float(sum(96 * pdh.ntt_length(int(x), int(y)) + 8 * (int(x) + int(y)) for x, y in zip(na, nb)))
+    # the per-product model B_mul = 96 N + 8 (na + nb) (8 (na + nb) when
+    # N <= 2^13: both spectra fit one CU's LDS), summed over the batch
+    # (N = pdh_ntt_length), and the limbs the batch must move at least
+    # (operands in, products out)
+    def bmul(x, y):
+        N = pdh.ntt_length(x, y)
+        # products with N <= 2^13 stay in one workgroup's LDS: limbs in and out
+        return 8 * (x + y) + (96 * N if N > (1 << 13) else 0)
+    sum_bmul = float(sum(bmul(int(x), int(y)) for x, y in zip(na, nb)))
     limb_bytes = float(8 * (na + nb).sum())
     return {"pairs": n, "plain_ms": t_plain, "batched_ms": t_batch, "plain_mul_per_s": n / (t_plain * 1e-3),
             "batched_mul_per_s": n / (t_batch * 1e-3), "speedup": t_plain / t_batch,
             "batched_equals_plain": same,
             "roofline": {"bound": "hbm", "alg_bytes": sum_bmul, "achieved": sum_bmul / (t_batch * 1e-3) / 1e9,
                          "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": sum_bmul / (t_batch * 1e-3) / HBM_PEAK,
-                         "model": "sum over the products of B_mul = 96 N + 8 (na + nb)",
+                         "model": "sum over the products of B_mul = 96 N + 8 (na + nb), 8 (na + nb) for N <= 2^13",
                          "limb_bytes": limb_bytes, "limb_frac": limb_bytes / (t_batch * 1e-3) / HBM_PEAK}}
 
 
