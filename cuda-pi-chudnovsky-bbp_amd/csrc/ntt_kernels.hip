@@ -655,12 +655,13 @@ static NttShape default_shape_eng(int n, int eng) {
         return s;
     }
     // three primes over 2^(n-1) points: {L1, T} for n = 14..24, measured on
-    // MI355X with the shape sweep (rows of 2^13..2^14 points at the top);
+    // MI355X with the shape sweep and the autotuner on distinct operands
+    // (n = 23: 154.5 us at {9, 16} against 165.2 us at {8, 16});
     // lr_row 4: rows as register-resident radix-16 sets and, where the tile
     // has a register kernel, columns as register-resident radix-8 sets
     static const int tab3[MAX_LOG + 1][2] = {
         {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0},
-        {4, 8}, {5, 8}, {6, 8}, {6, 8}, {5, 16}, {6, 16}, {6, 16}, {8, 16}, {8, 16}, {8, 16}, {9, 16}};
+        {4, 8}, {5, 8}, {6, 8}, {6, 8}, {5, 16}, {6, 16}, {6, 8}, {8, 8}, {8, 16}, {9, 16}, {9, 16}};
     s.eng = 1;
     s.lr_row = 4;
     s.L1 = tab3[n][0];

@@ -1715,10 +1715,13 @@ int pdh_autotune(int n_lo, int n_hi, int reps, void* stream) {
     if (e == hipSuccess) e = hipEventCreate(&e0);
     if (e == hipSuccess) e = hipEventCreate(&e1);
     if (e == hipSuccess) {
+        // distinct operands: with a == b the column pass reads every limb
+        // twice from cache and favours other shapes than a real product
         std::vector<u32> h(maxlimbs);
         uint64_t x = 0x9e3779b97f4a7c15ULL;
         for (auto& v : h) { x ^= x << 13; x ^= x >> 7; x ^= x << 17; v = (u32)x | 1u; }
         e = hipMemcpy(a, h.data(), maxlimbs * 4, hipMemcpyHostToDevice);
+        for (auto& v : h) { x ^= x << 13; x ^= x >> 7; x ^= x << 17; v = (u32)x | 1u; }
         if (e == hipSuccess) e = hipMemcpy(b, h.data(), maxlimbs * 4, hipMemcpyHostToDevice);
     }
     if (e != hipSuccess) { cleanup(); return fail_hip(e, "autotune setup"); }
