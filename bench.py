@@ -315,11 +315,12 @@ def bbp_sampled_legs(pdh, torch, W, golden, stream, dev, out, big):
     return out
 
 
-def bench_batch(pdh, torch, W, stream, reps=3, pairs=None, reps_plain=None):
+def bench_batch(pdh, torch, W, stream, reps=3, pairs=None, reps_plain=None, plain_path=True):
     """Independent pairs (default: 4,096 with limb counts log-uniform over
     [64, 65536]): batched plan replayed as one hipGraph vs the plain path (one
     pdh_mul_dev launch sequence per multiply, serial on one stream); outputs
-    compared."""
+    compared. plain_path=False runs the batched plan alone (nothing compared:
+    the kernel trace of the plan by itself)."""
     import numpy as np
     if pairs is None:
         pairs = W.batch_pairs(3, W.C3_PAIRS, W.C3_LO, W.C3_HI)
@@ -343,15 +344,17 @@ def bench_batch(pdh, torch, W, stream, reps=3, pairs=None, reps_plain=None):
     def plain():
         for d in descs:
             pdh.mul_dev(*d, sp)
-    plain()
-    torch.cuda.synchronize()
-    e0.record(stream)
-    for _ in range(reps_plain):
+    t_plain = None
+    if plain_path:
         plain()
-    e1.record(stream)
-    torch.cuda.synchronize()
-    t_plain = e0.elapsed_time(e1) / reps_plain
-    ref_out, ref_lens = out.clone(), lens.clone()
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(reps_plain):
+            plain()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        t_plain = e0.elapsed_time(e1) / reps_plain
+        ref_out, ref_lens = out.clone(), lens.clone()
     plan = pdh.Batch(descs)
     plan.run(sp)
     plan.capture(sp)
@@ -366,7 +369,7 @@ def bench_batch(pdh, torch, W, stream, reps=3, pairs=None, reps_plain=None):
     e1.record(stream)
     torch.cuda.synchronize()
     t_batch = e0.elapsed_time(e1) / reps
-    same = bool(torch.equal(out, ref_out) and torch.equal(lens, ref_lens))
+    same = bool(torch.equal(out, ref_out) and torch.equal(lens, ref_lens)) if plain_path else None
     plan.close()
     n = len(pairs)
     # the per-product model B_mul = 96 N + 8 (na + nb) (8 (na + nb) when
@@ -379,8 +382,9 @@ def bench_batch(pdh, torch, W, stream, reps=3, pairs=None, reps_plain=None):
         return 8 * (x + y) + (96 * N if N > (1 << 13) else 0)
     sum_bmul = float(sum(bmul(int(x), int(y)) for x, y in zip(na, nb)))
     limb_bytes = float(8 * (na + nb).sum())
-    return {"pairs": n, "plain_ms": t_plain, "batched_ms": t_batch, "plain_mul_per_s": n / (t_plain * 1e-3),
-            "batched_mul_per_s": n / (t_batch * 1e-3), "speedup": t_plain / t_batch,
+    return {"pairs": n, "plain_ms": t_plain, "batched_ms": t_batch,
+            "plain_mul_per_s": n / (t_plain * 1e-3) if plain_path else None,
+            "batched_mul_per_s": n / (t_batch * 1e-3), "speedup": t_plain / t_batch if plain_path else None,
             "batched_equals_plain": same,
             "roofline": {"bound": "hbm", "alg_bytes": sum_bmul, "achieved": sum_bmul / (t_batch * 1e-3) / 1e9,
                          "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": sum_bmul / (t_batch * 1e-3) / HBM_PEAK,
@@ -474,6 +478,10 @@ def main() -> int:
     ap.add_argument("--bbp-only", action="store_true",
                     help="run only the 64 and 512 sampled BBP positions (one dispatch of each term kernel) and "
                          "print their record: the workload of the BBP counter profile")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="run only the batched plans of the 4,096-pair batch and of the 10^7 binary-splitting "
+                         "levels (no plain path, nothing compared) and print their record: the workload of "
+                         "the batch kernel-trace profile")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 128 MiB-in, 128 MiB-out negate kernel (torch) so a --pmc profile can "
                          "scale FETCH_SIZE/WRITE_SIZE by a kernel of known traffic")
@@ -509,6 +517,14 @@ def main() -> int:
         rec = bench_bbp(pdh, torch, W, golden, stream, dev, small_only=True)
         print(json.dumps({"bbp_only": rec}))
         return 0 if all(v.get("bit_exact", True) for v in rec.values()) else 3
+    if args.batch_only:
+        rec = {"c3_batch_4096": bench_batch(pdh, torch, W, stream, reps=3, plain_path=False)}
+        progress("4,096-pair batch")
+        bs = W.bs_level_pairs(7, int(pdh.pilib().pdh_chudnovsky_terms(10**7)))
+        rec["c3_bs_levels_1e7"] = bench_batch(pdh, torch, W, stream, reps=3, pairs=bs, plain_path=False)
+        progress("binary-splitting levels")
+        print(json.dumps({"batch_only": rec}))
+        return 0
 
     def step():
         pdh.mul_dev(da.data_ptr(), na, db.data_ptr(), nb, out.data_ptr(), ln.data_ptr(), sp)

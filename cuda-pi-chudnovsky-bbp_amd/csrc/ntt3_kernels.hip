@@ -1187,16 +1187,24 @@ struct SmallPlan {
     static constexpr int TPR = RowPlan<L2>::TPR;
     static constexpr int WG = (3 * TPR + 63) / 64 * 64;
     static constexpr int WORDS = RowPlan<L2>::WORDS;
+    // LDS words of the CRT arrays; the carry's wave sums and thread outputs
+    // follow them, in LDS the exchanges no longer use
+    static constexpr int CRTW = 3 * lds_words(1 << L2);
+    static constexpr size_t CARRY_BYTES = (size_t)(WG / 64) * sizeof(CarryFn) + (size_t)WG * 8;
+    // waves per SIMD the register budget is set for: at 2^11 points, five
+    // (at most 96 VGPRs), so that three workgroups of six waves share a CU
+    // (their LDS, 52 KB each, fits three); elsewhere the compiler's choice
+    static constexpr int MINW = L2 == 11 ? 5 : 1;
 };
 template <int L2, bool SQUARE>
 constexpr size_t small_reg_lds() {
-    constexpr size_t ex = (size_t)(SQUARE ? 3 : 6) * RowPlan<L2>::WORDS;
-    constexpr size_t crt = (size_t)3 * lds_words(1 << L2);
-    return (ex > crt ? ex : crt) * 4;
+    constexpr size_t ex = (size_t)(SQUARE ? 3 : 6) * RowPlan<L2>::WORDS * 4;
+    constexpr size_t crt = (size_t)SmallPlan<L2>::CRTW * 4 + SmallPlan<L2>::CARRY_BYTES;
+    return ex > crt ? ex : crt;
 }
 
 template <int L2, bool SQUARE>
-__global__ __launch_bounds__(SmallPlan<L2>::WG) void k3_small_reg(const MulJob* __restrict__ jobs, MulJob one,
+__global__ __launch_bounds__(SmallPlan<L2>::WG, SmallPlan<L2>::MINW) void k3_small_reg(const MulJob* __restrict__ jobs, MulJob one,
                                                                   const u32* __restrict__ tw3, Primes3 PS,
                                                                   u32 sc0, u32 sc1, u32 sc2) {
     using RP = RowPlan<L2>;
@@ -1207,8 +1215,9 @@ __global__ __launch_bounds__(SmallPlan<L2>::WG) void k3_small_reg(const MulJob* 
     constexpr int TPR = SP::TPR;
     constexpr int R = 1 << L2;
     extern __shared__ u32 lds3[];
-    __shared__ CarryFn wsum[SP::WG / 64];
-    __shared__ u64 outs[SP::WG];
+    static_assert(SP::CRTW % 2 == 0, "8-byte alignment of the carry's LDS");
+    CarryFn* wsum = (CarryFn*)(lds3 + SP::CRTW);
+    u64* outs = (u64*)(wsum + SP::WG / 64);
     const MulJob J = job_of(jobs, blockIdx.x, one);
     const int g = threadIdx.x / TPR;                 // the prime of this thread's row
     const int u = threadIdx.x % TPR;
