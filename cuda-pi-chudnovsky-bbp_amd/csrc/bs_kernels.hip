@@ -284,14 +284,34 @@ constexpr int SA_K = BS_ADD_TILE / SA_NT;
 static_assert(SA_NT * SA_K == BS_ADD_TILE, "tile size");
 constexpr uint32_t ST_KILL = 0, ST_PROP = 1, ST_GEN = 2;
 
+typedef uint32_t sa_v4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) sa_v4 sa_gv4;
+typedef __attribute__((address_space(1))) uint32_t sa_gu32;
+
+// A thread's SA_K limbs of one operand from base (a multiple of SA_K): as
+// 16-byte vectors when they all lie below the length and the operand is
+// 16-byte aligned, else limb by limb with zeros from the length on.
+__device__ __forceinline__ void sa_load1(const uint32_t* a, uint64_t la, uint64_t base, uint32_t* x) {
+    if (base + SA_K <= la && ((uintptr_t)a & 15) == 0) {
+        const sa_gv4* p = (const sa_gv4*)(a + base);
+#pragma unroll
+        for (int q = 0; q < SA_K / 4; ++q) {
+            const sa_v4 v = p[q];
+            x[4 * q] = v.x; x[4 * q + 1] = v.y; x[4 * q + 2] = v.z; x[4 * q + 3] = v.w;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < SA_K; ++k) {
+            const uint64_t i = base + k;
+            x[k] = i < la ? ((const sa_gu32*)a)[i] : 0u;
+        }
+    }
+}
+
 __device__ __forceinline__ void sa_load(const uint32_t* a, uint64_t la, const uint32_t* b, uint64_t lb,
                                         uint64_t base, uint32_t* x, uint32_t* y) {
-#pragma unroll
-    for (int k = 0; k < SA_K; ++k) {
-        const uint64_t i = base + k;
-        x[k] = i < la ? a[i] : 0u;
-        y[k] = i < lb ? b[i] : 0u;
-    }
+    sa_load1(a, la, base, x);
+    sa_load1(b, lb, base, y);
 }
 
 __device__ __forceinline__ uint32_t seg_status(const uint32_t* x, const uint32_t* y, bool sub) {
@@ -416,23 +436,36 @@ __global__ __launch_bounds__(SA_NT) void k_bs_add_apply(const BsAddNode* __restr
     const uint32_t in = block_status_in(seg_status(x, y, v.sub), wv, &agg);
     const uint32_t st_in = in != ST_PROP ? in : tin;
     uint64_t c = st_in == ST_GEN ? 1u : 0u;
-    uint32_t top = 0;
+    uint32_t r[SA_K];
 #pragma unroll
     for (int k = 0; k < SA_K; ++k) {
-        uint32_t r;
         if (!v.sub) {
             c += (uint64_t)x[k] + y[k];
-            r = (uint32_t)c;
+            r[k] = (uint32_t)c;
             c >>= 32;
         } else {
             const uint64_t d = (uint64_t)x[k] - y[k] - c;
-            r = (uint32_t)d;
+            r[k] = (uint32_t)d;
             c = (d >> 32) & 1u;
         }
-        const uint64_t i = base + k;
-        if (i < N.cap) {
-            N.T[i] = r;
-            if (r) top = (uint32_t)(k + 1);
+    }
+    // T as 16-byte vectors where the segment lies below the capacity and T
+    // is 16-byte aligned, else limb by limb up to the capacity
+    uint32_t top = 0;
+    if (base + SA_K <= N.cap && ((uintptr_t)N.T & 15) == 0) {
+        sa_gv4* p = (sa_gv4*)(N.T + base);
+#pragma unroll
+        for (int q = 0; q < SA_K / 4; ++q) p[q] = sa_v4{r[4 * q], r[4 * q + 1], r[4 * q + 2], r[4 * q + 3]};
+#pragma unroll
+        for (int k = 0; k < SA_K; ++k)
+            if (r[k]) top = (uint32_t)(k + 1);
+    } else {
+#pragma unroll
+        for (int k = 0; k < SA_K; ++k) {
+            if (base + k < N.cap) {
+                ((sa_gu32*)N.T)[base + k] = r[k];
+                if (r[k]) top = (uint32_t)(k + 1);
+            }
         }
     }
     uint64_t hi = top ? base + top : 0;
