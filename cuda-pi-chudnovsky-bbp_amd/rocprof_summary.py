@@ -4,7 +4,12 @@ profiles/ directory.
     python cuda-pi-chudnovsky-bbp_amd/rocprof_summary.py trace DB TITLE COMMAND
         per-kernel calls / total / average duration (top_kernels view)
     python cuda-pi-chudnovsky-bbp_amd/rocprof_summary.py pmc DB [DB ...]
-        per-kernel mean per dispatch of every counter collected
+        per-kernel mean per dispatch of every counter collected, and, where
+        SQ_INSTS_VALU and GRBM_GUI_ACTIVE are among them, derived lines:
+        the dispatch's shader cycles per XCD (GRBM_GUI_ACTIVE is summed over
+        MI355X's 8 XCDs), the VALU issue fraction (SQ_INSTS_VALU counts wave
+        instructions; each takes 4 cycles of one of the 1,024 SIMDs), LDS
+        bank-conflict cycles per LDS instruction and the waves' wait fraction
     python cuda-pi-chudnovsky-bbp_amd/rocprof_summary.py traffic FETCH_DB WRITE_DB CALIB_BYTES COMMAND
         HBM bytes per dispatch from FETCH_SIZE and WRITE_SIZE (KiB), each
         scaled so that the calibration kernel (torch's elementwise negate,
@@ -18,6 +23,7 @@ import sqlite3
 import sys
 
 CALIB = "neg_kernel"
+XCDS, SIMDS = 8, 1024          # MI355X: 8 XCDs, 256 CUs of 4 SIMDs
 
 
 def short(name: str) -> str:
@@ -57,8 +63,17 @@ def pmc(dbs: list[str]) -> None:
             out[k].update(d)
     for k in sorted(out):
         print(k)
-        for cn, v in sorted(out[k].items()):
+        c = out[k]
+        for cn, v in sorted(c.items()):
             print(f"    {cn:<28} {v:>16.1f}")
+        if c.get("GRBM_GUI_ACTIVE") and "SQ_INSTS_VALU" in c:
+            cyc = c["GRBM_GUI_ACTIVE"] / XCDS
+            print(f"    {'= cycles per XCD':<28} {cyc:>16.1f}")
+            print(f"    {'= VALU issue fraction':<28} {c['SQ_INSTS_VALU'] * 4 / SIMDS / cyc:>16.3f}")
+            if c.get("SQ_INSTS_LDS"):
+                print(f"    {'= LDS conflict cyc/instr':<28} {c.get('SQ_LDS_BANK_CONFLICT', 0) / c['SQ_INSTS_LDS']:>16.2f}")
+            if c.get("SQ_WAVE_CYCLES"):
+                print(f"    {'= wait fraction':<28} {c.get('SQ_WAIT_ANY', 0) / c['SQ_WAVE_CYCLES']:>16.3f}")
 
 
 def traffic(fetch_db: str, write_db: str, calib_bytes: float, cmd: str) -> None:
