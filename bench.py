@@ -195,6 +195,46 @@ def bench_bbp(pdh, torch, W, golden, stream, dev, big=True, small_only=False):
     return out
 
 
+def bench_bbp_2p20(pdh, torch, W, golden, stream, dev, part):
+    """The C4 sampled workload at full size: 2^20 positions uniform in
+    [1e6, 1e8] (seed 401), sorted, in 64 chunks of 16,384; `part` 0 or 1
+    runs the chunks of the first or second half of the reference's mulmod
+    count (each about 7 minutes), 2 runs all of them. Positions in the
+    4,096-position fixture are checked digit for digit."""
+    import numpy as np
+    gs = golden["bbp_sampled"]
+    pos = np.sort(W.sampled_positions(gs["seed"], 1 << 20, gs["lo"], gs["hi"]))
+    chunks = np.split(pos, 64)
+    mm = [W.bbp_mulmod_count(c) for c in chunks]
+    half = np.searchsorted(np.cumsum(mm), sum(mm) / 2) + 1
+    sel = range(0, half) if part == 0 else range(half, 64) if part == 1 else range(64)
+    with open(os.path.join(ROOT, "tests", "golden", "bbp_sampled_4096.json")) as f:
+        g4096 = json.load(f)
+    ref = {int(p): h for p, h in zip(g4096["positions"], g4096["hex"])}
+    sp = stream.cuda_stream
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    total_ms, total_mm, checked, same = 0.0, 0, 0, True
+    for i in sel:
+        d = torch.from_numpy(chunks[i].view(np.int64)).to(dev)
+        v = torch.zeros(len(chunks[i]), dtype=torch.uint8, device=dev)
+        e0.record(stream)
+        pdh.bbp_positions_dev(d.data_ptr(), len(chunks[i]), v.data_ptr(), sp)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1)
+        total_ms += ms
+        total_mm += mm[i]
+        h = pdh.to_hex(v.cpu().numpy())
+        for p, x in zip(chunks[i], h):
+            if int(p) in ref:
+                checked += 1
+                same &= ref[int(p)] == x
+        progress(f"chunk {i} (positions {int(chunks[i][0])}..{int(chunks[i][-1])}, {ms / 1e3:.1f} s)")
+    return {"part": part, "chunks": [sel.start, sel.stop], "positions": 16384 * len(sel), "ms": total_ms,
+            "mulmods": total_mm, "mulmod_per_s": total_mm / (total_ms * 1e-3), "all_mulmods": int(sum(mm)),
+            "checked_vs_fixture": checked, "bit_exact": bool(same)}
+
+
 def bbp_range_legs(pdh, torch, W, golden, stream, dev, out):
     import numpy as np
     sp = stream.cuda_stream
@@ -482,6 +522,9 @@ def main() -> int:
                     help="run only the batched plans of the 4,096-pair batch and of the 10^7 binary-splitting "
                          "levels (no plain path, nothing compared) and print their record: the workload of "
                          "the batch kernel-trace profile")
+    ap.add_argument("--bbp-2p20", type=int, choices=(0, 1, 2), default=None,
+                    help="run the C4 sampled workload at full size (2^20 sorted positions in 64 chunks): "
+                         "0 or 1 the first or second half of its mulmods (each about 7 minutes), 2 all")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 128 MiB-in, 128 MiB-out negate kernel (torch) so a --pmc profile can "
                          "scale FETCH_SIZE/WRITE_SIZE by a kernel of known traffic")
@@ -517,6 +560,10 @@ def main() -> int:
         rec = bench_bbp(pdh, torch, W, golden, stream, dev, small_only=True)
         print(json.dumps({"bbp_only": rec}))
         return 0 if all(v.get("bit_exact", True) for v in rec.values()) else 3
+    if args.bbp_2p20 is not None:
+        rec = bench_bbp_2p20(pdh, torch, W, golden, stream, dev, args.bbp_2p20)
+        print(json.dumps({"bbp_sampled_2p20": rec}))
+        return 0 if rec["bit_exact"] else 3
     if args.batch_only:
         rec = {"c3_batch_4096": bench_batch(pdh, torch, W, stream, reps=3, plain_path=False)}
         progress("4,096-pair batch")
