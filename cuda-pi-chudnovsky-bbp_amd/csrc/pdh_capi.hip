@@ -267,6 +267,66 @@ hipError_t cache_free_after(void* p, hipStream_t s) {
     return hipSuccess;
 }
 
+// Page-locked host blocks for staging descriptor tables: an async copy from
+// pageable memory runs synchronously, so the tables of calls that return
+// before their work has run go through these. A block is reused once the
+// copy that read it has run (an event per block); classes are powers of two
+// from 4 KB, and blocks stay allocated until the process ends.
+struct PinCache {
+    std::mutex mu;
+    std::multimap<size_t, void*> idle;
+    std::map<void*, size_t> size_of;
+    std::vector<std::pair<void*, hipEvent_t>> pending;
+    std::vector<hipEvent_t> ev_pool;
+};
+PinCache& pcache() { static PinCache b; return b; }
+
+void pin_reap(PinCache& b) {   // caller holds b.mu
+    for (size_t i = 0; i < b.pending.size();) {
+        const hipError_t q = hipEventQuery(b.pending[i].second);
+        if (q == hipErrorNotReady) { (void)hipGetLastError(); ++i; continue; }
+        b.idle.emplace(b.size_of[b.pending[i].first], b.pending[i].first);
+        b.ev_pool.push_back(b.pending[i].second);
+        b.pending[i] = b.pending.back();
+        b.pending.pop_back();
+    }
+}
+
+hipError_t pin_alloc(void** p, size_t bytes) {
+    size_t cls = 4096;
+    while (cls < bytes) cls <<= 1;
+    PinCache& b = pcache();
+    std::lock_guard<std::mutex> g(b.mu);
+    if (!b.pending.empty()) pin_reap(b);
+    auto it = b.idle.find(cls);
+    if (it != b.idle.end()) {
+        *p = it->second;
+        b.idle.erase(it);
+        return hipSuccess;
+    }
+    hipError_t e = hipHostMalloc(p, cls, hipHostMallocDefault);
+    if (e == hipSuccess) b.size_of[*p] = cls;
+    return e;
+}
+
+// The staging block becomes reusable once the work enqueued on `s` so far
+// (its copy among it) has run.
+hipError_t pin_free_after(void* p, hipStream_t s) {
+    if (!p) return hipSuccess;
+    PinCache& b = pcache();
+    std::lock_guard<std::mutex> g(b.mu);
+    hipEvent_t ev = nullptr;
+    if (!b.ev_pool.empty()) { ev = b.ev_pool.back(); b.ev_pool.pop_back(); }
+    else {
+        hipError_t e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+        if (e != hipSuccess) return e;
+    }
+    hipError_t e = hipEventRecord(ev, s);
+    if (e != hipSuccess) { b.ev_pool.push_back(ev); return e; }
+    b.pending.emplace_back(p, ev);
+    return hipSuccess;
+}
+
 // Device scratch of one call, from the block cache. Blocks go back when
 // the call ends, once the work enqueued on its stream so far has run.
 struct CallScratch {
@@ -1088,6 +1148,7 @@ struct pdh_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipStream_t last = nullptr;           // stream of the latest run
+    hipStream_t up = nullptr;             // stream of the job table's async upload (one-shot batches)
     bool ran = false;
     bool shared = false;                  // streams and events are the context's (one-shot batches)
 };
@@ -1127,14 +1188,30 @@ void batch_free(pdh_batch* p) {
         for (auto st : p->streams) (void)hipStreamDestroy(st);
         for (auto ev : p->events) (void)hipEventDestroy(ev);
     }
-    cache_free(p->arena);
-    cache_free(p->d_jobs);
+    if (p->up) {                          // an upload may still be in flight (error paths)
+        (void)cache_free_after(p->arena, p->up);
+        (void)cache_free_after(p->d_jobs, p->up);
+    } else {
+        cache_free(p->arena);
+        cache_free(p->d_jobs);
+    }
     delete p;
 }
 
-// One-shot batches (created, run once, waited for and destroyed while the
-// caller holds no other plan) share the context's streams and events: their
-// enqueues are serialised by the context mutex, so reuse is safe.
+// A one-shot plan after its run: its blocks return to the cache once the
+// caller's stream (which joins every branch of the run) has passed this
+// point; the host does not wait.
+void batch_release_after(pdh_batch* p, hipStream_t s) {
+    (void)cache_free_after(p->arena, s);
+    (void)cache_free_after(p->d_jobs, s);
+    delete p;
+}
+
+// One-shot batches (created, run once and released) share the context's
+// streams and events: their enqueues are serialised by the context mutex, a
+// shared stream runs its work in order, and a stream's wait on an event takes
+// the event's latest record at the time of the wait, so a later batch's
+// records do not move an earlier batch's waits.
 int shared_streams(Ctx& c, size_t n, pdh_batch* p) {
     while (c.sh_streams.size() < 4) {
         hipStream_t st;
@@ -1152,7 +1229,8 @@ int shared_streams(Ctx& c, size_t n, pdh_batch* p) {
     return PDH_OK;
 }
 
-int batch_create_impl(const pdh_mul_desc* descs, size_t n, pdh_batch** plan, bool shared);
+int batch_create_impl(const pdh_mul_desc* descs, size_t n, pdh_batch** plan, bool shared,
+                      hipStream_t up = nullptr);
 
 } // namespace
 
@@ -1164,7 +1242,9 @@ int pdh_batch_create(const pdh_mul_desc* descs, size_t n, pdh_batch** plan) {
 
 namespace {
 
-int batch_create_impl(const pdh_mul_desc* descs, size_t n, pdh_batch** plan, bool shared) {
+// up: upload the job table asynchronously on this stream through a pinned
+// staging block (the one-shot path); else a synchronous copy.
+int batch_create_impl(const pdh_mul_desc* descs, size_t n, pdh_batch** plan, bool shared, hipStream_t up) {
     if (!plan || (n && !descs)) return fail(PDH_E_ARG, "null argument");
     *plan = nullptr;
     Ctx& c = ctx();
@@ -1215,8 +1295,21 @@ int batch_create_impl(const pdh_mul_desc* descs, size_t n, pdh_batch** plan, boo
         }
     }
     if (njobs_total) {
-        hipError_t e = cache_alloc((void**)&p->d_jobs, njobs_total * sizeof(MulJob));
-        if (e == hipSuccess) e = hipMemcpy(p->d_jobs, jobs.data(), njobs_total * sizeof(MulJob), hipMemcpyHostToDevice);
+        const size_t jb = njobs_total * sizeof(MulJob);
+        hipError_t e = cache_alloc((void**)&p->d_jobs, jb);
+        if (e == hipSuccess && up) {
+            void* pin = nullptr;
+            p->up = up;
+            e = pin_alloc(&pin, jb);
+            if (e == hipSuccess) {
+                std::memcpy(pin, jobs.data(), jb);
+                e = hipMemcpyAsync(p->d_jobs, pin, jb, hipMemcpyHostToDevice, up);
+                const hipError_t e2 = pin_free_after(pin, up);
+                if (e == hipSuccess) e = e2;
+            }
+        } else if (e == hipSuccess) {
+            e = hipMemcpy(p->d_jobs, jobs.data(), jb, hipMemcpyHostToDevice);
+        }
         if (e != hipSuccess) { batch_free(p); return fail_hip(e, "batch job table upload"); }
     }
     size_t first = 0;
@@ -1302,14 +1395,13 @@ int pdh_batch_destroy(pdh_batch* p) {
 
 int pdh_mul_batch_dev(const pdh_mul_desc* descs, size_t n, void* stream) {
     pdh_batch* p = nullptr;
-    int rc = batch_create_impl(descs, n, &p, true);
+    int rc = batch_create_impl(descs, n, &p, true, (hipStream_t)stream);
     if (rc) return rc;
     rc = pdh_batch_run(p, stream);
-    if (rc == PDH_OK) {
-        hipError_t e = hipStreamSynchronize((hipStream_t)stream);
-        if (e != hipSuccess) rc = fail_hip(e, "batch synchronize");
-    }
-    pdh_batch_destroy(p);
+    Ctx& c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    if (rc == PDH_OK) batch_release_after(p, (hipStream_t)stream);
+    else batch_free(p);
     return rc;
 }
 
@@ -1453,28 +1545,31 @@ int pdh_bs_merge_dev(const pdh_bs_node* in, size_t n_in, pdh_bs_node* out, int s
 namespace {
 
 // Uploads a node table (nodes, then the tile -> node map) into one device
-// allocation, runs `launch` on it, waits for the stream and frees it.
+// allocation through a pinned staging block, runs `launch` on it, and hands
+// both blocks back once the stream has passed the launches (no host wait).
 template <class Node, class Launch>
 int run_tiled(const std::vector<Node>& nodes, const std::vector<uint32_t>& tmap, size_t extra, hipStream_t s,
               const char* what, Launch launch) {
     auto al256 = [](size_t x) { return (x + 255) & ~(size_t)255; };
     const size_t o_tmap = al256(nodes.size() * sizeof(Node));
     const size_t o_extra = o_tmap + al256(tmap.size() * 4);
-    std::vector<char> stage(o_extra);
-    std::memcpy(stage.data(), nodes.data(), nodes.size() * sizeof(Node));
-    std::memcpy(stage.data() + o_tmap, tmap.data(), tmap.size() * 4);
     char* ws = nullptr;
+    void* pin = nullptr;
     hipError_t e = cache_alloc((void**)&ws, o_extra + al256(extra));
     if (e != hipSuccess) return fail_hip(e, what);
-    e = hipMemcpyAsync(ws, stage.data(), stage.size(), hipMemcpyHostToDevice, s);
+    e = pin_alloc(&pin, o_extra);
+    if (e != hipSuccess) { cache_free(ws); return fail_hip(e, what); }
+    std::memcpy(pin, nodes.data(), nodes.size() * sizeof(Node));
+    std::memcpy((char*)pin + o_tmap, tmap.data(), tmap.size() * 4);
+    e = hipMemcpyAsync(ws, pin, o_extra, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) {
         Ctx& c = ctx();
         std::lock_guard<std::mutex> g(c.mu);
         e = launch((const Node*)ws, (const uint32_t*)(ws + o_tmap), (uint32_t)tmap.size(), ws + o_extra);
     }
-    hipError_t e2 = hipStreamSynchronize(s);
-    cache_free(ws);
-    if (e == hipSuccess) e = e2;
+    const hipError_t e2 = pin_free_after(pin, s);
+    const hipError_t e3 = cache_free_after(ws, s);
+    if (e == hipSuccess) e = e2 != hipSuccess ? e2 : e3;
     return e == hipSuccess ? PDH_OK : fail_hip(e, what);
 }
 

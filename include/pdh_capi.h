@@ -137,7 +137,10 @@ int pdh_batch_run(pdh_batch* plan, void* stream);
 int pdh_batch_capture(pdh_batch* plan, void* stream);
 /* Waits for the plan's work on the device, then frees it. */
 int pdh_batch_destroy(pdh_batch* plan);
-/* One-shot: create, run on `stream`, wait, destroy. */
+/* One-shot: create, run on `stream` and release the plan. Returns without
+ * waiting for the device: the products and lengths are complete once
+ * `stream` has passed this call (stream order); the descriptors are copied
+ * before the call returns. */
 int pdh_mul_batch_dev(const pdh_mul_desc* descs, size_t n, void* stream);
 
 /* ---- transform shape tuning ----------------------------------------------
@@ -261,7 +264,8 @@ int pdh_bs_merge_dev(const pdh_bs_node* in, size_t n_in, pdh_bs_node* out, int s
  * read the operands' lengths; PDH_E_ARG if it is). Limbs below out_cap
  * are written and the result must fit in them; *out_len receives its
  * normalized length and *out_sign its sign (0 for zero). Two passes over
- * 4,096-limb tiles of every output; the stream is synchronized on return. */
+ * 4,096-limb tiles of every output, enqueued on `stream`; the call returns
+ * without waiting for the device (results in stream order). */
 typedef struct pdh_sadd_desc {
     const uint32_t* a; const uint64_t* a_len; int32_t sa;
     const uint32_t* b; const uint64_t* b_len; int32_t sb;
@@ -272,7 +276,8 @@ int pdh_sadd_batch_dev(const pdh_sadd_desc* descs, size_t n, void* stream);
 
 /* out = floor(a / 2^bits) for every descriptor; *a_len is a's normalized
  * length (device). Limbs below out_cap (at least 1) are written; the result
- * must fit. *out_len receives the normalized length. Synchronizes the stream. */
+ * must fit. *out_len receives the normalized length. Enqueued on `stream`;
+ * returns without waiting for the device (results in stream order). */
 typedef struct pdh_shr_desc {
     const uint32_t* a; const uint64_t* a_len;
     uint32_t* out; uint64_t out_cap;

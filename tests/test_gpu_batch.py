@@ -131,3 +131,43 @@ def test_full_size_batches_vs_oracle(pdh, which):
         bad = [i for i, (g, w) in enumerate(zip(got, want)) if not np.array_equal(g, w)]
         assert not bad, (rep, bad[:5])
     plan.close()
+
+
+def test_chained_one_shot_batches_are_stream_ordered(pdh):
+    """pdh_mul_batch_dev returns without waiting: three batches chained on
+    one side stream (each multiplies the previous batch's products by new
+    operands, read from the device before the host has waited for
+    anything) must give the oracle's products once the stream is synchronized."""
+    import torch
+    rng = np.random.default_rng(404)
+    k = 48
+    sizes = [(int(x), int(y)) for x, y in zip(rng.integers(1, 3000, k), rng.integers(1, 3000, k))]
+    ops = [[W.operand(40000 + 3 * i + j, sizes[i][j % 2] if j < 2 else int(rng.integers(1, 5000)))
+            for j in range(4)] for i in range(k)]
+    # stage 0: a*b, stage 1: (a*b)*c, stage 2: ((a*b)*c)*d
+    caps = [[len(o[0]) + len(o[1])] for o in ops]
+    for i, o in enumerate(ops):
+        caps[i].append(caps[i][0] + len(o[2]))
+        caps[i].append(caps[i][1] + len(o[3]))
+    dev = [[torch.from_numpy(x.view(np.int32)).cuda() for x in o] for o in ops]
+    outs = [[torch.zeros(c, dtype=torch.int32, device="cuda") for c in cs] for cs in caps]
+    lens = torch.zeros((3, k), dtype=torch.int64, device="cuda")
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    for st in range(3):
+        descs = []
+        for i in range(k):
+            a = dev[i][0] if st == 0 else outs[i][st - 1]
+            na = len(ops[i][0]) if st == 0 else caps[i][st - 1]
+            b = dev[i][st + 1]
+            descs.append((a.data_ptr(), na, b.data_ptr(), len(ops[i][st + 1]), outs[i][st].data_ptr(),
+                          lens[st].data_ptr() + 8 * i))
+        pdh.mul_batch_dev(descs, s.cuda_stream)
+    s.synchronize()
+    L = lens.cpu().numpy()
+    for i, o in enumerate(ops):
+        want = o[0]
+        for st in range(3):
+            want = oracle.mul_ntt(want, o[st + 1])
+            got = outs[i][st].cpu().numpy().view(np.uint32)[: L[st, i]]
+            assert L[st, i] == len(want) and np.array_equal(got, want), (i, st)
