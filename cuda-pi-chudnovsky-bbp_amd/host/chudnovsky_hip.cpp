@@ -40,6 +40,8 @@
 #include <cmath>
 #include <cstring>
 #include <functional>
+#include <future>
+#include <memory>
 #include <string>
 #include <thread>
 #include <utility>
@@ -81,8 +83,13 @@ struct Multiplier {
     // P may be empty); empty = from the leaves
     std::function<std::vector<PQT>(uint64_t N)> groups;
     // optional: the decimal string of x > 0 (the levels run elsewhere); empty =
-    // the host loop below
-    std::function<std::string(const std::vector<uint32_t>& x, const Multiplier& M)> to_decimal;
+    // the host loop below. `ahead` is plan_decimal's handle or null.
+    std::function<std::string(const std::vector<uint32_t>& x, const Multiplier& M,
+                              const std::shared_ptr<void>& ahead)> to_decimal;
+    // optional: start the divisors and reciprocals of to_decimal for a value
+    // of xbits bits beside the caller's work (they depend only on the size);
+    // the handle goes to to_decimal
+    std::function<std::shared_ptr<void>(size_t xbits)> plan_decimal;
     // optional: floor(426880 Q sqrt(10005 10^(2 prec)) / T) computed elsewhere
     // (the square root and the division kept in device memory); empty = the
     // host backend below
@@ -486,9 +493,9 @@ static std::string strip_zeros(const std::string& out) {
     return nz == std::string::npos ? "0" : out.substr(nz);
 }
 
-static std::string to_decimal(const Mag& x, const Multiplier& M) {
+static std::string to_decimal(const Mag& x, const Multiplier& M, const std::shared_ptr<void>& ahead = nullptr) {
     if (x.empty()) return "0";
-    if (M.to_decimal) return M.to_decimal(x, M);
+    if (M.to_decimal) return M.to_decimal(x, M, ahead);
     HostArith o{M};
     const DecimalPlan<HostArith> p = decimal_plan(o, bitlen(x));
     const int K = p.K;
@@ -632,6 +639,12 @@ std::string compute_pi(uint64_t digits, const Multiplier& M, double* phase_ms) {
     const uint64_t guard = 16, prec = digits + guard;
     const uint64_t N = terms_for_digits(prec);
     auto t0 = clk::now();
+    // pi_scaled is about pi 10^prec: its bit length is floor(log2 pi + prec
+    // log2 10) + 1 unless that is within rounding of an integer, and a plan
+    // made for another length is not used, so the decimal plan can start now
+    std::shared_ptr<void> ahead;
+    if (M.plan_decimal)
+        ahead = M.plan_decimal((size_t)std::floor(1.6514961294723187 + (double)prec * 3.3219280948873623) + 1);
     PQT r = binary_split(N, M);
     auto t1 = clk::now();
     clk::time_point ts[2];
@@ -644,7 +657,7 @@ std::string compute_pi(uint64_t digits, const Multiplier& M, double* phase_ms) {
     }
     const clk::time_point t2 = ts[0], t3 = ts[1];
     auto t4 = clk::now();
-    std::string s = to_decimal(pi_scaled, M);
+    std::string s = to_decimal(pi_scaled, M, ahead);
     auto t5 = clk::now();
     if (phase_ms) {
         phase_ms[0] = ms(t0, t1); phase_ms[1] = ms(t1, t2); phase_ms[2] = ms(t2, t3);
@@ -1022,10 +1035,46 @@ static void perturb_quotients(DevArith& o, uint32_t* Xn, uint64_t* XnL, uint64_t
 // T < 0; otherwise (rare: the estimate is within a unit) the node is fixed
 // by steps of D on the host. The leaves' digits come from one kernel; only
 // the lengths, signs and the final W characters cross PCIe.
-static std::string dev_to_decimal(const Mag& x, const Multiplier&) {
+// A decimal plan made ahead on its own stream and host thread. Members are
+// destroyed in reverse order: the task is waited for before its backend and
+// plan go; the backend's buffers are released on the plan stream, so the
+// user of the plan synchronizes its own stream first.
+struct DevPlanAhead {
+    size_t xbits = 0;
+    std::unique_ptr<DevArith> o;
+    DecimalPlan<DevArith> p;
+    std::future<void> done;
+};
+
+// One plan stream per calling host thread, kept (as work_stream).
+static void* plan_stream() {
+    static thread_local void* st = nullptr;
+    if (!st) check(pdh_stream_create(&st));
+    return st;
+}
+
+static std::shared_ptr<void> dev_plan_decimal(size_t xbits) {
+    auto h = std::make_shared<DevPlanAhead>();
+    h->xbits = xbits;
+    void* st = plan_stream();
+    DevPlanAhead* raw = h.get();
+    h->done = std::async(std::launch::async, [raw, st]() {
+        raw->o.reset(new DevArith(st));
+        raw->p = decimal_plan(*raw->o, raw->xbits);
+        raw->o->sync();
+    });
+    return h;
+}
+
+static std::string dev_to_decimal(const Mag& x, const Multiplier&, const std::shared_ptr<void>& ahead) {
     void* st = work_stream();
     DevArith o(st);
-    const DecimalPlan<DevArith> p = decimal_plan(o, bitlen(x));
+    DevPlanAhead* pa = static_cast<DevPlanAhead*>(ahead.get());
+    if (pa) pa->done.get();                      // the plan's work has run (or its error is rethrown)
+    const size_t xb = bitlen(x);
+    DecimalPlan<DevArith> own;
+    if (!pa || pa->xbits != xb) own = decimal_plan(o, xb);
+    const DecimalPlan<DevArith>& p = (pa && pa->xbits == xb) ? pa->p : own;
     const int K = p.K;
     const size_t B = p.B;
     auto cap_of = [](uint64_t w) { return (uint64_t)((double)w * 3.3219280948873623 / 32.0) + 3; };
@@ -1138,6 +1187,7 @@ static std::string dev_to_decimal(const Mag& x, const Multiplier&) {
         check(pdh_stream_sync(st));
     } catch (...) {
         release();
+        (void)pdh_stream_sync(st);               // nothing queued still reads the plan when it is freed
         throw;
     }
     release();
@@ -1156,6 +1206,7 @@ Multiplier device_multiplier() {
     M.mul_many = dev_mul_many;
     M.groups = dev_levels;
     M.to_decimal = dev_to_decimal;
+    M.plan_decimal = dev_plan_decimal;
     M.pi_scaled = dev_pi_scaled;
     return M;
 }
