@@ -365,6 +365,28 @@ __device__ __forceinline__ uint32_t block_max_u32(uint32_t v, uint32_t* red) {
     return r;
 }
 
+// The key (tile + 1) * 4 + status of the highest non-PROP tile among the
+// node's tiles [0, hi) (0 if every one is PROP), looking down from hi - 1 a
+// workgroup's width at a time and stopping at the first chunk that holds a
+// non-PROP tile. A PROP tile has every limb of its sum 0xFFFFFFFF (or, for a
+// difference, every limb equal), so the first chunk ends nearly every search
+// and the cost no longer grows with the node's tile count.
+__device__ __forceinline__ uint32_t top_non_prop(const uint8_t* __restrict__ status, uint32_t tile0, uint32_t hi,
+                                                 uint32_t* red) {
+    for (int64_t top = (int64_t)hi - 1; top >= 0; top -= SA_NT) {
+        const int64_t i = top - (int64_t)threadIdx.x;
+        uint32_t key = 0;
+        if (i >= 0) {
+            const uint32_t st = status[tile0 + i];
+            if (st != ST_PROP) key = (uint32_t)(i + 1) * 4 + st;
+        }
+        key = block_max_u32(key, red);
+        __syncthreads();                        // red is written again by the next chunk
+        if (key) return key;
+    }
+    return 0;
+}
+
 struct SaView {
     uint32_t nd, j;
     uint64_t la, lb;
@@ -406,25 +428,18 @@ __global__ __launch_bounds__(SA_NT) void k_bs_add_apply(const BsAddNode* __restr
                                                         const uint8_t* __restrict__ status) {
     __shared__ uint32_t wv[SA_NT / 64];
     __shared__ uint32_t red[2][SA_NT / 64];
-    const uint32_t t = blockIdx.x;
+    // highest tiles first (the last node's top tile first): the tile that
+    // holds a node's top nonzero limb usually runs before the lower ones,
+    // which then find the length already above theirs and skip the atomic
+    const uint32_t t = gridDim.x - 1 - blockIdx.x;
     const uint32_t nd = nodes ? tile_node[t] : 0u;
     const BsAddNode N = nodes ? nodes[nd] : one;
     const SaView v = sa_view(N, nd, t);
     // nearest lower non-PROP tile (carry into this tile) and, for a
     // difference, the highest non-PROP tile (the order of |A| and |B|);
     // keys are (tile + 1) * 4 + status, 0 for none
-    uint32_t kin = 0, kall = 0;
-    const uint32_t lim = v.sub ? N.ntiles : v.j;
-    for (uint32_t i = threadIdx.x; i < lim; i += SA_NT) {
-        const uint32_t s = status[N.tile0 + i];
-        if (s != ST_PROP) {
-            const uint32_t key = (i + 1) * 4 + s;
-            if (i < v.j) kin = key > kin ? key : kin;
-            kall = key > kall ? key : kall;
-        }
-    }
-    kin = block_max_u32(kin, red[0]);
-    kall = v.sub ? block_max_u32(kall, red[1]) : 0;
+    const uint32_t kin = top_non_prop(status, N.tile0, v.j, red[0]);
+    const uint32_t kall = v.sub ? top_non_prop(status, N.tile0, N.ntiles, red[1]) : 0;
     const bool swap = v.sub && (kall & 3) == ST_GEN;          // |A| < |B|: T = sb (B - A)
     uint32_t tin = kin ? (kin & 3) : ST_PROP;
     if (swap && tin != ST_PROP) tin = ST_GEN + ST_KILL - tin;
@@ -474,7 +489,8 @@ __global__ __launch_bounds__(SA_NT) void k_bs_add_apply(const BsAddNode* __restr
         const uint64_t u = __shfl_xor(hi, o);
         hi = u > hi ? u : hi;
     }
-    if ((threadIdx.x & 63) == 0 && hi) atomicMax((unsigned long long*)N.len, (unsigned long long)hi);
+    if ((threadIdx.x & 63) == 0 && hi > __hip_atomic_load(N.len, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        atomicMax((unsigned long long*)N.len, (unsigned long long)hi);
     if (v.j == 0 && threadIdx.x == 0) *N.sign = !v.sub ? (N.sa ? N.sa : N.sb) : (swap ? N.sb : N.sa);
 }
 
