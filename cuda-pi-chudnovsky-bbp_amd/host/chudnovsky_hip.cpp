@@ -54,7 +54,9 @@ typedef std::vector<uint32_t> Mag;
 typedef unsigned __int128 u128;
 
 struct SInt { Mag m; int s = 0; };            // sign s in {-1, 0, 1}, magnitude m
-struct PQT { Mag P, Q; SInt T; };
+// dev: when set, Q and T (sign T.s) are in device memory (a DevRoot of the
+// device backend) and the host magnitudes are empty
+struct PQT { Mag P, Q; SInt T; std::shared_ptr<void> dev; };
 
 // Test switch (pdh_newton_set_perturb): k != 0 moves every quotient and
 // square-root estimate by k units, and every decimal-level quotient estimate
@@ -94,6 +96,11 @@ struct Multiplier {
     // (the square root and the division kept in device memory); empty = the
     // host backend below
     std::function<Mag(const PQT& r, uint64_t prec, std::chrono::steady_clock::time_point* ts)> pi_scaled;
+    // optional: that value's decimal string computed without it leaving the
+    // backend (ts[0..2] stamp the ends of 10^prec, the square root and the
+    // division); replaces pi_scaled and to_decimal
+    std::function<std::string(const PQT& r, uint64_t prec, std::chrono::steady_clock::time_point* ts,
+                              const std::shared_ptr<void>& ahead)> pi_decimal;
 };
 
 // ---- magnitude helpers (little-endian base 2^32, normalized) -----------------
@@ -239,6 +246,8 @@ struct HostArith {
     u128 to_u128(const Mag& a) { return chud::to_u128(a); }
     Mag up(const Mag& a) { return a; }
     Mag down(const Mag& a) { return a; }
+    Mag pqt_q(const PQT& r) { return r.Q; }
+    Mag pqt_t(const PQT& r) { return r.T.m; }
     void sync() {}
     void tighten(Mag&) {}
 };
@@ -617,10 +626,10 @@ uint64_t terms_for_digits(uint64_t digits) {              // src/chudnovsky.cpp:
 }
 
 // pi_scaled = floor(426880 Q isqrt(10005 10^(2 prec)) / T) (src/chudnovsky.cpp:
-// 130-140) on backend o; ts[0] and ts[1] stamp the ends of the 10^prec and
-// square-root phases.
+// 130-140) on backend o, as the backend's number; ts[0] and ts[1] stamp the
+// ends of the 10^prec and square-root phases.
 template <class A>
-static Mag pi_scaled_t(A& o, const PQT& r, uint64_t prec, std::chrono::steady_clock::time_point* ts) {
+static typename A::Num pi_scaled_num(A& o, const PQT& r, uint64_t prec, std::chrono::steady_clock::time_point* ts) {
     typedef typename A::Num N;
     N one = pow10(o, prec);
     N radicand = o.mul_u32(o.sqr(one), 10005u);
@@ -629,8 +638,13 @@ static Mag pi_scaled_t(A& o, const PQT& r, uint64_t prec, std::chrono::steady_cl
     N sqrtC = isqrt(o, radicand);
     o.tighten(sqrtC);
     ts[1] = std::chrono::steady_clock::now();
-    N numerator = o.mul(o.mul_u32(o.up(r.Q), 426880u), sqrtC);
-    return o.down(div_floor(o, numerator, o.up(r.T.m), (N*)nullptr));   // T > 0 for the full range
+    N numerator = o.mul(o.mul_u32(o.pqt_q(r), 426880u), sqrtC);
+    return div_floor(o, numerator, o.pqt_t(r), (N*)nullptr);   // T > 0 for the full range
+}
+
+template <class A>
+static Mag pi_scaled_t(A& o, const PQT& r, uint64_t prec, std::chrono::steady_clock::time_point* ts) {
+    return o.down(pi_scaled_num(o, r, prec, ts));
 }
 
 std::string compute_pi(uint64_t digits, const Multiplier& M, double* phase_ms) {
@@ -647,17 +661,22 @@ std::string compute_pi(uint64_t digits, const Multiplier& M, double* phase_ms) {
         ahead = M.plan_decimal((size_t)std::floor(1.6514961294723187 + (double)prec * 3.3219280948873623) + 1);
     PQT r = binary_split(N, M);
     auto t1 = clk::now();
-    clk::time_point ts[2];
-    Mag pi_scaled;
-    if (M.pi_scaled) {
-        pi_scaled = M.pi_scaled(r, prec, ts);
+    clk::time_point ts[3];
+    std::string s;
+    if (M.pi_decimal) {
+        s = M.pi_decimal(r, prec, ts, ahead);
     } else {
-        HostArith o{M};
-        pi_scaled = pi_scaled_t(o, r, prec, ts);
+        Mag pi_scaled;
+        if (M.pi_scaled) {
+            pi_scaled = M.pi_scaled(r, prec, ts);
+        } else {
+            HostArith o{M};
+            pi_scaled = pi_scaled_t(o, r, prec, ts);
+        }
+        ts[2] = clk::now();
+        s = to_decimal(pi_scaled, M, ahead);
     }
-    const clk::time_point t2 = ts[0], t3 = ts[1];
-    auto t4 = clk::now();
-    std::string s = to_decimal(pi_scaled, M, ahead);
+    const clk::time_point t2 = ts[0], t3 = ts[1], t4 = ts[2];
     auto t5 = clk::now();
     if (phase_ms) {
         phase_ms[0] = ms(t0, t1); phase_ms[1] = ms(t1, t2); phase_ms[2] = ms(t2, t3);
@@ -753,6 +772,18 @@ static std::vector<Mag> dev_mul_many(const std::vector<std::pair<const Mag*, con
     return out;
 }
 
+// The root of a device tree left in HBM: Q and T in blocks this handle owns,
+// freed in stream order on `st` (the stream of the thread that built the
+// tree, which is also the one that reads them) when the handle goes.
+struct DevRoot {
+    std::vector<void*> blocks;
+    void* st = nullptr;
+    const uint32_t* Q = nullptr;
+    const uint32_t* T = nullptr;
+    uint64_t nQ = 0, nT = 0;
+    ~DevRoot() { for (void* b : blocks) (void)pdh_dev_free_async(b, st); }
+};
+
 // The tree on the device. The leaves and the first six levels come from one
 // kernel (pdh_bs_groups, one node per 64 consecutive terms); every level above
 // is one pdh_bs_merge_dev call (batched products and the signed T sum in HBM),
@@ -819,18 +850,18 @@ static std::vector<PQT> dev_levels(uint64_t N) {
             arena = next;
             lvl.swap(nl);
         }
-        out.resize(lvl.size());
-        for (size_t i = 0; i < lvl.size(); ++i) {
-            const pdh_bs_node& v = lvl[i];
-            out[i].P.resize(v.nP);
-            out[i].Q.resize(v.nQ);
-            out[i].T.m.resize(v.nT);
-            out[i].T.s = v.nT ? v.sT : 0;
-            check(pdh_copy_dtoh(out[i].P.data(), v.P, v.nP * 4, st));
-            check(pdh_copy_dtoh(out[i].Q.data(), v.Q, v.nQ * 4, st));
-            check(pdh_copy_dtoh(out[i].T.m.data(), v.T, v.nT * 4, st));
-        }
-        check(pdh_stream_sync(st));
+        // the root stays in HBM: its blocks pass to a handle, and the tail
+        // of the computation reads Q and T on this thread's stream
+        const pdh_bs_node& v = lvl[0];
+        auto root = std::make_shared<DevRoot>();
+        root->st = st;
+        if (arena) { root->blocks.push_back(arena); arena = nullptr; }
+        else { root->blocks = {dP, dQ, dT}; dP = dQ = dT = nullptr; }
+        root->Q = v.Q; root->nQ = v.nQ;
+        root->T = v.T; root->nT = v.nT;
+        out.resize(1);
+        out[0].T.s = v.nT ? v.sT : 0;
+        out[0].dev = root;
     } catch (...) {
         release();
         throw;
@@ -870,6 +901,7 @@ public:
     DevArith& operator=(const DevArith&) = delete;
 
     void sync() { check(pdh_stream_sync(st_)); }
+    void* stream() const { return st_; }
     void tighten(DNum& a) { a.ub = length(a); }
     DNum up(const Mag& a) {
         DNum x = alloc(a.size());
@@ -941,6 +973,16 @@ public:
         return sg;
     }
     DNum pow2(size_t e) { return shl(one_, e); }
+    // Q and T of a binary-splitting root: copies of the device tree's
+    // buffers when it was left in HBM, else uploads
+    DNum pqt_q(const PQT& r) {
+        const DevRoot* d = (const DevRoot*)r.dev.get();
+        return d ? view(d->Q, d->nQ) : up(r.Q);
+    }
+    DNum pqt_t(const PQT& r) {
+        const DevRoot* d = (const DevRoot*)r.dev.get();
+        return d ? view(d->T, d->nT) : up(r.T.m);
+    }
     DNum mul(const DNum& a, const DNum& b) {
         DNum x = alloc(a.ub + b.ub);
         check(pdh_mul_dev(a.p, a.ub, b.p, b.ub, x.p, x.len, st_));
@@ -1066,12 +1108,11 @@ static std::shared_ptr<void> dev_plan_decimal(size_t xbits) {
     return h;
 }
 
-static std::string dev_to_decimal(const Mag& x, const Multiplier&, const std::shared_ptr<void>& ahead) {
-    void* st = work_stream();
-    DevArith o(st);
+static std::string dev_decimal(DevArith& o, const DNum& x, const std::shared_ptr<void>& ahead) {
+    void* st = o.stream();
     DevPlanAhead* pa = static_cast<DevPlanAhead*>(ahead.get());
     if (pa) pa->done.get();                      // the plan's work has run (or its error is rethrown)
-    const size_t xb = bitlen(x);
+    const size_t xb = o.bits(x);
     DecimalPlan<DevArith> own;
     if (!pa || pa->xbits != xb) own = decimal_plan(o, xb);
     const DecimalPlan<DevArith>& p = (pa && pa->xbits == xb) ? pa->p : own;
@@ -1086,13 +1127,7 @@ static std::string dev_to_decimal(const Mag& x, const Multiplier&, const std::sh
         // a level: n nodes x C limbs, then n lengths
         uint64_t n = 1, C = cap_of(p.W);
         check(pdh_dev_alloc(r64(n * C) * 4 + n * 8, &cur));
-        {
-            std::vector<uint32_t> x0(C, 0);
-            std::copy(x.begin(), x.end(), x0.begin());
-            const uint64_t lx = x.size();
-            check(pdh_copy_htod(cur, x0.data(), C * 4, nullptr));
-            check(pdh_copy_htod((uint32_t*)cur + r64(C), &lx, 8, nullptr));
-        }
+        o.store(x, (uint32_t*)cur, C, (uint64_t*)((uint32_t*)cur + r64(C)));   // zero above x's length
         for (int l = 0; l < K; ++l) {
             uint32_t* X = (uint32_t*)cur;
             uint64_t* XL = (uint64_t*)(X + r64(n * C));
@@ -1194,9 +1229,31 @@ static std::string dev_to_decimal(const Mag& x, const Multiplier&, const std::sh
     return strip_zeros(out);
 }
 
+static std::string dev_to_decimal(const Mag& x, const Multiplier&, const std::shared_ptr<void>& ahead) {
+    DevArith o(work_stream());
+    return dev_decimal(o, o.up(x), ahead);
+}
+
 static Mag dev_pi_scaled(const PQT& r, uint64_t prec, std::chrono::steady_clock::time_point* ts) {
     DevArith o(work_stream());
     return pi_scaled_t(o, r, prec, ts);
+}
+
+// The square root, the division and the decimal conversion with every value
+// in HBM (the root's Q and T from the device tree, pi_scaled handed over to
+// the conversion without a host copy). The division's memory goes (in stream
+// order) once its result is copied into the conversion's backend.
+static std::string dev_pi_decimal(const PQT& r, uint64_t prec, std::chrono::steady_clock::time_point* ts,
+                                  const std::shared_ptr<void>& ahead) {
+    void* st = work_stream();
+    std::unique_ptr<DevArith> od(new DevArith(st));
+    DNum xd = pi_scaled_num(*od, r, prec, ts);
+    od->tighten(xd);
+    ts[2] = std::chrono::steady_clock::now();
+    DevArith o(st);
+    const DNum x = o.view(xd.p, xd.ub);
+    od.reset();
+    return dev_decimal(o, x, ahead);
 }
 
 Multiplier device_multiplier() {
@@ -1208,6 +1265,7 @@ Multiplier device_multiplier() {
     M.to_decimal = dev_to_decimal;
     M.plan_decimal = dev_plan_decimal;
     M.pi_scaled = dev_pi_scaled;
+    M.pi_decimal = dev_pi_decimal;
     return M;
 }
 
