@@ -96,10 +96,11 @@ struct Multiplier {
     // (the square root and the division kept in device memory); empty = the
     // host backend below
     std::function<Mag(const PQT& r, uint64_t prec, std::chrono::steady_clock::time_point* ts)> pi_scaled;
-    // optional: that value's decimal string computed without it leaving the
-    // backend (ts[0..2] stamp the ends of 10^prec, the square root and the
-    // division); replaces pi_scaled and to_decimal
-    std::function<std::string(const PQT& r, uint64_t prec, std::chrono::steady_clock::time_point* ts,
+    // optional: that value as "3." and `digits` decimals ("3" for none),
+    // computed without it leaving the backend (ts[0..2] stamp the ends of
+    // 10^prec, the square root and the division); replaces pi_scaled and
+    // to_decimal
+    std::function<std::string(const PQT& r, uint64_t prec, uint64_t digits, std::chrono::steady_clock::time_point* ts,
                               const std::shared_ptr<void>& ahead)> pi_decimal;
 };
 
@@ -497,9 +498,12 @@ template <class A> static DecimalPlan<A> decimal_plan(A& o, size_t xbits) {
     return p;
 }
 
-static std::string strip_zeros(const std::string& out) {
-    size_t nz = out.find_first_not_of('0');
-    return nz == std::string::npos ? "0" : out.substr(nz);
+// In place (a 10^8-digit string is not copied into a second buffer).
+static std::string strip_zeros(std::string&& out) {
+    const size_t nz = out.find_first_not_of('0');
+    if (nz == std::string::npos) return "0";
+    out.erase(0, nz);
+    return std::move(out);
 }
 
 static std::string to_decimal(const Mag& x, const Multiplier& M, const std::shared_ptr<void>& ahead = nullptr) {
@@ -550,7 +554,7 @@ static std::string to_decimal(const Mag& x, const Multiplier& M, const std::shar
             for (size_t i = t; i < nodes.size(); i += nth) leaf_digits(nodes[i], B, &out[i * B]);
         });
     for (auto& th : pool) th.join();
-    return strip_zeros(out);
+    return strip_zeros(std::move(out));
 }
 
 // ---- binary splitting, bottom-up with batched levels ---------------------------------
@@ -663,8 +667,9 @@ std::string compute_pi(uint64_t digits, const Multiplier& M, double* phase_ms) {
     auto t1 = clk::now();
     clk::time_point ts[3];
     std::string s;
-    if (M.pi_decimal) {
-        s = M.pi_decimal(r, prec, ts, ahead);
+    const bool formed = (bool)M.pi_decimal;
+    if (formed) {
+        s = M.pi_decimal(r, prec, digits, ts, ahead);
     } else {
         Mag pi_scaled;
         if (M.pi_scaled) {
@@ -682,13 +687,12 @@ std::string compute_pi(uint64_t digits, const Multiplier& M, double* phase_ms) {
         phase_ms[0] = ms(t0, t1); phase_ms[1] = ms(t1, t2); phase_ms[2] = ms(t2, t3);
         phase_ms[3] = ms(t3, t4); phase_ms[4] = ms(t4, t5); phase_ms[5] = (double)N;
     }
-    std::string out;
+    if (formed) return s;
     if (digits == 0) return "3";
-    out.reserve(digits + 2);
-    out.push_back(s[0]);
-    out.push_back('.');
-    out.append(s, 1, (size_t)digits);
-    return out;
+    // "3." and the first `digits` decimals, in place in s (prec + 1 digits)
+    if (s.size() > (size_t)digits + 1) s.resize((size_t)digits + 1);
+    s.insert(s.begin() + 1, '.');
+    return s;
 }
 
 // ---- the device multiplier -------------------------------------------------------------
@@ -1108,7 +1112,15 @@ static std::shared_ptr<void> dev_plan_decimal(size_t xbits) {
     return h;
 }
 
-static std::string dev_decimal(DevArith& o, const DNum& x, const std::shared_ptr<void>& ahead) {
+// A value's known digit count and the form wanted: its first digit, then (for
+// keep > 0) a point and the next keep digits.
+struct DecForm { uint64_t ndig, keep; };
+
+// The decimal digits of x (leading zeros stripped), or with `form` (and the
+// first digit where form says) the formed string, *formed then set.
+static std::string dev_decimal(DevArith& o, const DNum& x, const std::shared_ptr<void>& ahead,
+                               const DecForm* form = nullptr, bool* formed = nullptr) {
+    if (formed) *formed = false;
     void* st = o.stream();
     DevPlanAhead* pa = static_cast<DevPlanAhead*>(ahead.get());
     if (pa) pa->done.get();                      // the plan's work has run (or its error is rethrown)
@@ -1122,7 +1134,7 @@ static std::string dev_decimal(DevArith& o, const DNum& x, const std::shared_ptr
     auto r64 = [](uint64_t v) { return (v + 63) & ~(uint64_t)63; };
     void *cur = nullptr, *nxt = nullptr, *aux = nullptr, *dout = nullptr;
     auto release = [&]() { pdh_dev_free_async(cur, st); pdh_dev_free_async(nxt, st); pdh_dev_free_async(aux, st); pdh_dev_free_async(dout, st); };
-    std::string out(p.W, '0');
+    std::string out;
     try {
         // a level: n nodes x C limbs, then n lengths
         uint64_t n = 1, C = cap_of(p.W);
@@ -1218,15 +1230,35 @@ static std::string dev_decimal(DevArith& o, const DNum& x, const std::shared_ptr
         uint64_t* XL = (uint64_t*)(X + r64(n * C));
         check(pdh_dev_alloc(p.W, &dout));
         check(pdh_decimal_leaves_dev(X, C, XL, n, (uint32_t)B, (char*)dout, st));
-        check(pdh_copy_dtoh(&out[0], dout, p.W, st));
-        check(pdh_stream_sync(st));
+        const char* dd = (const char*)dout;
+        if (form && formed && form->ndig >= 1 && form->ndig <= p.W && form->keep < form->ndig) {
+            // only the digits wanted cross to the host, once the digit
+            // before the first (if any) is seen to be 0 and the first not
+            const size_t nz = p.W - form->ndig;
+            char edge[2] = {'0', '0'};
+            check(pdh_copy_dtoh(nz ? edge : edge + 1, dd + (nz ? nz - 1 : 0), nz ? 2 : 1, st));
+            check(pdh_stream_sync(st));
+            if (edge[0] == '0' && edge[1] != '0') {
+                out.assign(form->keep ? form->keep + 2 : 1, '.');
+                out[0] = edge[1];
+                if (form->keep) check(pdh_copy_dtoh(&out[2], dd + nz + 1, form->keep, st));
+                check(pdh_stream_sync(st));
+                if (formed) *formed = true;
+            }
+        }
+        if (!formed || !*formed) {
+            out.assign(p.W, '0');
+            check(pdh_copy_dtoh(&out[0], dout, p.W, st));
+            check(pdh_stream_sync(st));
+        }
     } catch (...) {
         release();
         (void)pdh_stream_sync(st);               // nothing queued still reads the plan when it is freed
         throw;
     }
     release();
-    return strip_zeros(out);
+    if (formed && *formed) return out;
+    return strip_zeros(std::move(out));
 }
 
 static std::string dev_to_decimal(const Mag& x, const Multiplier&, const std::shared_ptr<void>& ahead) {
@@ -1243,8 +1275,8 @@ static Mag dev_pi_scaled(const PQT& r, uint64_t prec, std::chrono::steady_clock:
 // in HBM (the root's Q and T from the device tree, pi_scaled handed over to
 // the conversion without a host copy). The division's memory goes (in stream
 // order) once its result is copied into the conversion's backend.
-static std::string dev_pi_decimal(const PQT& r, uint64_t prec, std::chrono::steady_clock::time_point* ts,
-                                  const std::shared_ptr<void>& ahead) {
+static std::string dev_pi_decimal(const PQT& r, uint64_t prec, uint64_t digits,
+                                  std::chrono::steady_clock::time_point* ts, const std::shared_ptr<void>& ahead) {
     void* st = work_stream();
     std::unique_ptr<DevArith> od(new DevArith(st));
     DNum xd = pi_scaled_num(*od, r, prec, ts);
@@ -1253,7 +1285,15 @@ static std::string dev_pi_decimal(const PQT& r, uint64_t prec, std::chrono::stea
     DevArith o(st);
     const DNum x = o.view(xd.p, xd.ub);
     od.reset();
-    return dev_decimal(o, x, ahead);
+    // floor(pi 10^prec) lies in [3 10^prec, 4 10^prec): prec + 1 digits
+    const DecForm f{prec + 1, digits};
+    bool formed = false;
+    std::string s = dev_decimal(o, x, ahead, &f, &formed);
+    if (formed) return s;
+    if (digits == 0) return s.substr(0, 1);
+    if (s.size() > (size_t)digits + 1) s.resize((size_t)digits + 1);
+    s.insert(s.begin() + 1, '.');
+    return s;
 }
 
 Multiplier device_multiplier() {

@@ -6,8 +6,12 @@
 //  goes in arrival order, but a request that does not fit the budget lets
 //  later, smaller requests pass it (backfill) until it has waited
 //  kStarveMs; from then on nothing passes it, so it is admitted as soon as
-//  the requests ahead of it release enough memory. Admitted requests start
-//  while earlier ones are still on the device:
+//  the requests ahead of it release enough memory. With nothing running, a
+//  request is admitted over the budget (so that one larger than the budget,
+//  or a client that waits on it while holding unfetched results, cannot
+//  stall the server) at once if nothing is held, else after kIdleOverMs, the
+//  time the client has to fetch the results whose products it still holds.
+//  Admitted requests start while earlier ones are still on the device:
 //
 //  * multiplies admitted in one scheduler tick whose transform is of
 //    moderate size (N <= 2^18) are grouped into one batched plan
@@ -54,6 +58,7 @@ enum State { QUEUED = 0, RUNNING = 1, DONE = 2, FAILED = 3 };
 constexpr int kBbpWorkers = 2;
 constexpr int kPiWorkers = 2;
 constexpr double kStarveMs = 20.0;          // a request that waited this long stops being passed
+constexpr double kIdleOverMs = 50.0;        // nothing running, results held: wait this long before going over budget
 constexpr size_t kSmallN = (size_t)1 << 18; // multiplies up to this transform length are grouped
 
 struct Request {
@@ -102,6 +107,7 @@ uint64_t* arena_len(Request* r) {
 struct pdh_server {
     size_t budget = 0, inuse = 0, peak = 0;
     int max_inflight = 0, inflight = 0;
+    clk::time_point idle_since = clk::now();   // when inflight last fell to 0
     uint64_t next_id = 1;
     uint64_t n_submitted = 0, n_done = 0, n_waited_mem = 0, n_backfilled = 0;
     bool stop = false;
@@ -123,7 +129,7 @@ struct pdh_server {
         r->t_done = clk::now();
         r->state = state;
         release_locked(release);
-        --inflight;
+        if (--inflight == 0) idle_since = r->t_done;
         ++n_done;
         cv_done.notify_all();
     }
@@ -192,10 +198,12 @@ struct pdh_server {
                 bool blocked = false;
                 for (auto it = pending.begin(); it != pending.end() && inflight < max_inflight;) {
                     Request* r = *it;
-                    const bool fits = inflight == 0 || inuse + r->bytes <= budget;
+                    const double waited = std::chrono::duration<double, std::milli>(now - r->t_arrival).count();
+                    const double idle = std::chrono::duration<double, std::milli>(now - idle_since).count();
+                    const bool fits = inuse + r->bytes <= budget ||
+                                      (inflight == 0 && (inuse == 0 || idle >= kIdleOverMs));
                     if (!fits) {
                         if (!r->waited_mem) { r->waited_mem = true; ++n_waited_mem; }
-                        const double waited = std::chrono::duration<double, std::milli>(now - r->t_arrival).count();
                         if (waited >= kStarveMs) break;        // no more passing this one
                         blocked = true;
                         ++it;

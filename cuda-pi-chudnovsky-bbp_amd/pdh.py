@@ -514,10 +514,13 @@ def newton_corrections() -> int:
 
 def pi_chudnovsky(digits: int):
     """("3.1415...", phase ms [split, 10^prec, sqrt, divide, format, terms])."""
-    buf = ctypes.create_string_buffer(digits + 8)
+    n = digits + 8
+    raw = bytearray(n)                          # decoded in place below (no bytes copy of 10^8 digits)
+    buf = (ctypes.c_char * n).from_buffer(raw)
     ph = (ctypes.c_double * 6)()
-    _check(pilib().pdh_pi_chudnovsky(digits, buf, len(buf), ph))
-    return buf.value.decode(), list(ph)
+    _check(pilib().pdh_pi_chudnovsky(digits, buf, n, ph))
+    del buf
+    return str(memoryview(raw)[: raw.index(0)], "ascii"), list(ph)
 
 
 class Server:

