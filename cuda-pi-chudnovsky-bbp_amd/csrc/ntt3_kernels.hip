@@ -106,17 +106,59 @@ __device__ __forceinline__ u32 mml(u32 a, u32 b, u32 p, u32 ninv) {
     const u32 m = (u32)t * ninv;
     return (u32)(((u64)m * p + t) >> 32);
 }
+// (a - b) w R^-1 mod p as (a w + b (p - w)) R^-1: two multiply-adds take the
+// place of the subtraction and its 2p offset. For a, b < 4p and a canonical
+// twiddle w the sum is at most max(a, b) p < 4p^2 < p 2^32, so the result is
+// in [0, 2p) as mml's.
+__device__ __forceinline__ u32 mmd(u32 a, u32 b, u32 w, u32 p, u32 ninv) {
+    const u64 t = (u64)a * w + (u64)b * (p - w);
+    const u32 m = (u32)t * ninv;
+    return (u32)(((u64)m * p + t) >> 32);
+}
 
 // ---- radix-q DFTs on lazy residues, natural order in and out ----------------------
+// Inputs in [0, 2p). The DFT4 is two radix-2 halves: c0 = x0 + x2,
+// c1 = x1 + x3, d0 = x0 - x2 and d1 = (x1 - x3) w4, each in [0, 2p); its
+// outputs are Y0 = c0 + c1, Y1 = d0 + d1, Y2 = c0 - c1, Y3 = d0 - d1.
+template <bool INV>
+__device__ __forceinline__ void l_half4(u32 x0, u32 x1, u32 x2, u32 x3, u32& c0, u32& c1, u32& d0, u32& d1,
+                                        const Prime3& P) {
+    const u32 p2 = P.p << 1;
+    c0 = red2(x0 + x2, p2); d0 = red2(x0 + p2 - x2, p2);
+    c1 = red2(x1 + x3, p2);
+    d1 = mmd(x1, x3, INV ? P.w8i[2] : P.w8[2], P.p, P.ninv);   // w4^{-1}, w4
+}
+
+// Output forms of a DFT's last radix-4 round: RAW leaves the outputs in
+// [0, 4p); RED reduces each to [0, 2p); TW reduces output 0 and multiplies
+// output k >= 1 by the twiddle w[k] (result in [0, 2p)), the differences
+// by mmd, so the subtraction costs nothing extra.
+enum : int { OUT_RAW = 0, OUT_RED = 1, OUT_TW = 2 };
+
+template <int OUT>
+__device__ __forceinline__ void out4(u32 c0, u32 c1, u32 d0, u32 d1, u32& y0, u32& y1, u32& y2, u32& y3,
+                                     const u32* w, int k0, int ks, const Prime3& P) {
+    const u32 p2 = P.p << 1;
+    if constexpr (OUT == OUT_RAW) {
+        y0 = c0 + c1; y2 = c0 + p2 - c1;
+        y1 = d0 + d1; y3 = d0 + p2 - d1;
+    } else if constexpr (OUT == OUT_RED) {
+        y0 = red2(c0 + c1, p2); y2 = red2(c0 + p2 - c1, p2);
+        y1 = red2(d0 + d1, p2); y3 = red2(d0 + p2 - d1, p2);
+    } else {
+        y0 = k0 == 0 ? red2(c0 + c1, p2) : mml(c0 + c1, w[k0], P.p, P.ninv);
+        y1 = mml(d0 + d1, w[k0 + ks], P.p, P.ninv);
+        y2 = mmd(c0, c1, w[k0 + 2 * ks], P.p, P.ninv);
+        y3 = mmd(d0, d1, w[k0 + 3 * ks], P.p, P.ninv);
+    }
+}
+
 // Inputs in [0, 2p), outputs in [0, 4p).
 template <bool INV>
 __device__ __forceinline__ void l_dft4(u32& x0, u32& x1, u32& x2, u32& x3, const Prime3& P) {
-    const u32 p2 = P.p << 1;
-    const u32 c0 = red2(x0 + x2, p2), d0 = red2(x0 + p2 - x2, p2);
-    const u32 c1 = red2(x1 + x3, p2);
-    const u32 d1 = mml(x1 + p2 - x3, INV ? P.w8i[2] : P.w8[2], P.p, P.ninv);   // w4^{-1}, w4
-    x0 = c0 + c1; x2 = c0 + p2 - c1;
-    x1 = d0 + d1; x3 = d0 + p2 - d1;
+    u32 c0, c1, d0, d1;
+    l_half4<INV>(x0, x1, x2, x3, c0, c1, d0, d1, P);
+    out4<OUT_RAW>(c0, c1, d0, d1, x0, x1, x2, x3, nullptr, 0, 0, P);
 }
 
 // Y_{2k} = DFT4(x_t + x_{t+4})_k, Y_{2k+1} = DFT4((x_t - x_{t+4}) w8^t)_k.
@@ -125,9 +167,9 @@ __device__ __forceinline__ void l_dft8(u32* x, const Prime3& P) {
     const u32* w = INV ? P.w8i : P.w8;
     const u32 p2 = P.p << 1;
     u32 a0 = red2(x[0] + x[4], p2), b0 = red2(x[0] + p2 - x[4], p2);
-    u32 a1 = red2(x[1] + x[5], p2), b1 = mml(x[1] + p2 - x[5], w[1], P.p, P.ninv);
-    u32 a2 = red2(x[2] + x[6], p2), b2 = mml(x[2] + p2 - x[6], w[2], P.p, P.ninv);
-    u32 a3 = red2(x[3] + x[7], p2), b3 = mml(x[3] + p2 - x[7], w[3], P.p, P.ninv);
+    u32 a1 = red2(x[1] + x[5], p2), b1 = mmd(x[1], x[5], w[1], P.p, P.ninv);
+    u32 a2 = red2(x[2] + x[6], p2), b2 = mmd(x[2], x[6], w[2], P.p, P.ninv);
+    u32 a3 = red2(x[3] + x[7], p2), b3 = mmd(x[3], x[7], w[3], P.p, P.ninv);
     l_dft4<INV>(a0, a1, a2, a3, P);
     l_dft4<INV>(b0, b1, b2, b3, P);
     x[0] = a0; x[2] = a1; x[4] = a2; x[6] = a3;
@@ -138,35 +180,40 @@ __device__ __forceinline__ void l_dft8(u32* x, const Prime3& P) {
 // Y_k = sum_t2 w4^(t2 k2) [w16^(t2 k1) sum_t1 w4^(t1 k1) x_{4 t1 + t2}]:
 // four DFT4 over t1, nine twiddles w16^(t2 k1) (t2, k1 >= 1), four DFT4
 // over t2. The middle values are reduced to [0, 2p) by the twiddle product or
-// red2, so each DFT4 sees inputs in [0, 2p).
-template <bool INV>
-__device__ __forceinline__ void l_dft16(u32* x, const Prime3& P) {
+// red2, so each DFT4 sees inputs in [0, 2p); the middle differences with a
+// twiddle go through mmd. OUT selects the outputs' form (out4), with the
+// twiddles w[k] in natural order for OUT_TW.
+template <bool INV, int OUT = OUT_RAW>
+__device__ __forceinline__ void l_dft16(u32* x, const Prime3& P, const u32* w = nullptr) {
     const u32 p2 = P.p << 1;
     const u32* w8 = INV ? P.w8i : P.w8;
     const u32* w16 = INV ? P.w16i : P.w16;
-#pragma unroll
-    for (int t2 = 0; t2 < 4; ++t2) l_dft4<INV>(x[t2], x[t2 + 4], x[t2 + 8], x[t2 + 12], P);
-    // x[t2 + 4 k1] now holds the inner DFT at (t2, k1); exponents t2 k1:
+    // inner twiddles w16^(t2 k1) for t2, k1 = 1..3 (row t2 - 1):
     // 1 2 3 / 2 4 6 / 3 6 9 -> w16, w8, w16^3 / w8, w4, w8^3 / w16^3, w8^3, w16^9
-    x[0] = red2(x[0], p2); x[1] = red2(x[1], p2); x[2] = red2(x[2], p2); x[3] = red2(x[3], p2);
-    x[4] = red2(x[4], p2); x[8] = red2(x[8], p2); x[12] = red2(x[12], p2);
-    x[5] = mml(x[5], w16[0], P.p, P.ninv);
-    x[6] = mml(x[6], w8[1], P.p, P.ninv);
-    x[7] = mml(x[7], w16[1], P.p, P.ninv);
-    x[9] = mml(x[9], w8[1], P.p, P.ninv);
-    x[10] = mml(x[10], w8[2], P.p, P.ninv);
-    x[11] = mml(x[11], w8[3], P.p, P.ninv);
-    x[13] = mml(x[13], w16[1], P.p, P.ninv);
-    x[14] = mml(x[14], w8[3], P.p, P.ninv);
-    x[15] = mml(x[15], w16[2], P.p, P.ninv);
+    const u32 wi[3][3] = {{w16[0], w8[1], w16[1]}, {w8[1], w8[2], w8[3]}, {w16[1], w8[3], w16[2]}};
+    u32 m[16];   // m[t2 + 4 k1]: the inner DFT at (t2, k1), in [0, 2p)
 #pragma unroll
-    for (int k1 = 0; k1 < 4; ++k1) l_dft4<INV>(x[4 * k1], x[4 * k1 + 1], x[4 * k1 + 2], x[4 * k1 + 3], P);
-    // x[4 k1 + k2] = Y_{k1 + 4 k2}: transpose to natural order
-    u32 y[16];
+    for (int t2 = 0; t2 < 4; ++t2) {
+        u32 c0, c1, d0, d1;
+        l_half4<INV>(x[t2], x[t2 + 4], x[t2 + 8], x[t2 + 12], c0, c1, d0, d1, P);
+        m[t2] = red2(c0 + c1, p2);
+        if (t2 == 0) {
+            m[4] = red2(d0 + d1, p2);
+            m[8] = red2(c0 + p2 - c1, p2);
+            m[12] = red2(d0 + p2 - d1, p2);
+        } else {
+            m[t2 + 4] = mml(d0 + d1, wi[t2 - 1][0], P.p, P.ninv);
+            m[t2 + 8] = mmd(c0, c1, wi[t2 - 1][1], P.p, P.ninv);
+            m[t2 + 12] = mmd(d0, d1, wi[t2 - 1][2], P.p, P.ninv);
+        }
+    }
+    // the outer DFT4 over t2 of row k1 gives Y_{k1 + 4 k2}, k2 = 0..3
 #pragma unroll
-    for (int k = 0; k < 16; ++k) y[k] = x[4 * (k & 3) + (k >> 2)];
-#pragma unroll
-    for (int k = 0; k < 16; ++k) x[k] = y[k];
+    for (int k1 = 0; k1 < 4; ++k1) {
+        u32 c0, c1, d0, d1;
+        l_half4<INV>(m[4 * k1], m[4 * k1 + 1], m[4 * k1 + 2], m[4 * k1 + 3], c0, c1, d0, d1, P);
+        out4<OUT>(c0, c1, d0, d1, x[k1], x[k1 + 4], x[k1 + 8], x[k1 + 12], w, k1, 4, P);
+    }
 }
 
 template <int Q, bool INV>
@@ -196,7 +243,20 @@ __device__ __forceinline__ void l_dftq(u32* x, const Prime3& P) {
 template <int Q, bool INV, bool RIN0 = false, bool ROUT = true>
 __device__ __forceinline__ void lazy_butterfly(u32* x, const u32* w, bool has_tw, const Prime3& P) {
     const u32 p2 = P.p << 1;
-    if (!INV) {
+    if constexpr (Q == 16) {
+        // the outputs' reduction or twiddle product fused into the last round
+        if (!INV) {
+            if (has_tw) l_dft16<false, OUT_TW>(x, P, w);
+            else l_dft16<false, OUT_RED>(x, P);
+        } else {
+            if (has_tw) {
+                if constexpr (RIN0) x[0] = red2(x[0], p2);
+#pragma unroll
+                for (int c = 1; c < Q; ++c) x[c] = mml(x[c], w[c], P.p, P.ninv);
+            }
+            l_dft16<true, ROUT ? OUT_RED : OUT_RAW>(x, P);
+        }
+    } else if (!INV) {
         l_dftq<Q, false>(x, P);
         if (has_tw) {
             x[0] = red2(x[0], p2);
@@ -645,8 +705,11 @@ __device__ __forceinline__ void row_inv(u32* x, const u32* w, int u, u32* __rest
     }
 }
 
+// The register budget is set for four waves per SIMD (128 VGPRs): at
+// 2^13 points a row's LDS lets two 512-thread workgroups share a CU, which
+// more registers would prevent.
 template <int L2, bool SQUARE>
-__global__ __launch_bounds__(RowPlan<L2>::WG) void k3_row_reg(const MulJob* __restrict__ jobs, MulJob one, int L1,
+__global__ __launch_bounds__(RowPlan<L2>::WG, 4) void k3_row_reg(const MulJob* __restrict__ jobs, MulJob one, int L1,
                                                              int lr_col, const u32* __restrict__ tw3,
                                                              const u32* __restrict__ o3, int h, Primes3 PS) {
     using RP = RowPlan<L2>;
