@@ -253,6 +253,14 @@ __device__ __forceinline__ uint64_t frac64(uint64_t r, uint64_t m, double minv) 
     return ((uint64_t)hi << 32) | lo;
 }
 
+// r a 2^-32 mod mo, lazy: for r, a < 2^32 with r a < 2^32 mo the result is
+// below 2 mo.
+__device__ __forceinline__ uint32_t mont_mul_lazy(uint32_t r, uint32_t a, uint32_t mo, uint32_t ninv) {
+    const uint64_t t = (uint64_t)r * a;
+    const uint32_t mq = (uint32_t)t * ninv;
+    return (uint32_t)(((uint64_t)mq * mo + t) >> 32);
+}
+
 // floor((16^e mod m) 2^64 / m) for e >= 1 and 1 < m < 2^30, with no division.
 // With m = 2^s mo (mo odd) and r = 16^e mod m = 2^s r_o, r_o = 2^(4e - s) mod
 // mo, the fraction is F = floor(r_o 2^64 / mo), taken as two 32-bit halves:
@@ -276,11 +284,11 @@ __device__ __forceinline__ uint64_t frac64(uint64_t r, uint64_t m, double minv) 
 // estimate q within one of floor(2^32 / mo): 2^32 - q mo, formed mod 2^32,
 // lies in [-mo, 2 mo), and one add of mo fixes the negative case. The
 // indices fit 32 bits: positions are at most 2^28, so m = 8k + j < 2^31.
-// SMALL (every modulus below 2^28, as for series with d < 2^25 - 1): the
-// ladder runs on [0, 4 mo) -- a Montgomery square of r < 4 mo is below 2 mo
-// when mo < 2^28 -- so a bit is the square and one shift, five VALU, and one
-// reduction to [0, 2 mo) follows the loop.
-template <bool SMALL>
+// FOLD (every modulus below 2^29): each bit's doubling goes into its square,
+// r <- (r * (r << bit)) 2^-32 (mont_mul_lazy): with r < 2 mo the product is
+// below 8 mo^2 < 2^32 mo, so r stays below 2 mo -- five VALU per bit (the bit,
+// the shift, the product and its reduction) against seven.
+template <bool FOLD>
 __device__ __forceinline__ uint64_t frac64_mont_wave(uint32_t e, uint32_t m) {
     const int s = __builtin_ctz(m);
     const uint32_t mo = m >> s;
@@ -298,18 +306,13 @@ __device__ __forceinline__ uint64_t frac64_mont_wave(uint32_t e, uint32_t m) {
     const uint32_t two_mo = mo << 1;
     const uint32_t Emax = __builtin_amdgcn_readfirstlane(E);
     for (int b = 31 - __clz(Emax); b >= 0; --b) {
-        const uint64_t t = (uint64_t)r * r;
-        const uint32_t mq = (uint32_t)t * ninv;
-        r = (uint32_t)(((uint64_t)mq * mo + t) >> 32);
-        r <<= (E >> b) & 1u;
-        if constexpr (!SMALL) {
+        if constexpr (FOLD) {
+            r = mont_mul_lazy(r, r << ((E >> b) & 1u), mo, ninv);
+        } else {
+            r = mont_mul_lazy(r, r, mo, ninv) << ((E >> b) & 1u);
             const uint32_t d = r - two_mo;
             r = d < r ? d : r;
         }
-    }
-    if constexpr (SMALL) {
-        const uint32_t d = r - two_mo;
-        r = d < r ? d : r;
     }
     const uint32_t rem2 = r >= mo ? r - mo : r;
     const uint32_t mq = r * ninv;
@@ -355,15 +358,17 @@ __global__ __launch_bounds__(256) void k_bbp_fx_terms(const BbpSeries* __restric
     // positions are at most 2^28 (the entry points check), so d, k < 2^28 and
     // m = 8k + j < 2^31: the term indices run in 32 bits
     const uint32_t d = (uint32_t)S.d, base = (uint32_t)base64;
-    const bool small = d < (1u << 25) - 1u;           // every m = 8k + j < 2^28
     uint64_t local = 0;
     for (int i = 0; i < KT; ++i) {
         const uint32_t k = base + (uint32_t)i * 256u + threadIdx.x;
         if (k > d) break;
+        // every modulus of this step, 8k + j <= 8 (kb 256 KT + 256 i + 255) + 7,
+        // is below 2^29 (the FOLD ladder) when that k is at most 2^26 - 2
+        const bool fold = base + (uint32_t)i * 256u + 255u <= (1u << 26) - 2u;
         const uint32_t mi = 8u * k + S.j;
         if (mi == 1) continue;                        // 16^d mod 1 = 0
         if (mi < (1u << 30) && k < d) {
-            local += small ? frac64_mont_wave<true>(d - k, mi) : frac64_mont_wave<false>(d - k, mi);
+            local += fold ? frac64_mont_wave<true>(d - k, mi) : frac64_mont_wave<false>(d - k, mi);
         } else {                                      // 16^0 = 1, or m >= 2^30
             const double m = (double)mi;
             const double minv = recip_f64(m);
@@ -401,11 +406,12 @@ __device__ __forceinline__ uint32_t dbl_lazy(uint32_t r, uint32_t two_mo) {
     return d < r ? d : r;
 }
 
-// SMALL: every modulus of the wavefront has mo < 2^28. A Montgomery square
-// of r < 4 mo is then below (16 mo^2 + 2^32 mo) / 2^32 < 2 mo, so a doubling
-// needs no reduction (the ladder runs on [0, 4 mo)): one shift by the scalar
-// bit, no branch. One reduction to [0, 2 mo) at the end.
-template <bool SMALL>
+// FOLD: every modulus of the wavefront has mo < 2^29. Each bit's doubling
+// then goes into its square, r <- (r * (r << b)) 2^-32: with r < 2 mo the
+// product is below 8 mo^2 < 2^32 mo, so the result stays below 2 mo, at four
+// VALU per bit (shift by the scalar bit, multiply, Montgomery reduction) and
+// no branch.
+template <bool FOLD>
 __device__ __forceinline__ uint64_t frac64_mont_uniform(uint32_t E, uint32_t m, uint32_t s) {
     const uint32_t mo = m >> s;
     uint32_t inv = (3u * mo) ^ 2u;
@@ -421,17 +427,17 @@ __device__ __forceinline__ uint64_t frac64_mont_uniform(uint32_t E, uint32_t m, 
     // (fewer scalar instructions per bit), the last n mod 4 one per trip.
     int n = 31 - __clz(E);
     uint32_t x = E << (32 - n);
-    if constexpr (SMALL) {
-        r <<= 1;
-        for (; n >= 4; n -= 4, x <<= 4) {
-            r = mont_sq_lazy(r, mo, ninv) << (x >> 31);
-            r = mont_sq_lazy(r, mo, ninv) << ((x >> 30) & 1u);
-            r = mont_sq_lazy(r, mo, ninv) << ((x >> 29) & 1u);
-            r = mont_sq_lazy(r, mo, ninv) << ((x >> 28) & 1u);
-        }
-        for (; n > 0; --n, x <<= 1) r = mont_sq_lazy(r, mo, ninv) << (x >> 31);
+    if constexpr (FOLD) {
+        r <<= 1;                                          // the leading one, reduced to [0, 2 mo)
         const uint32_t d = r - two_mo;
         r = d < r ? d : r;
+        for (; n >= 4; n -= 4, x <<= 4) {
+            r = mont_mul_lazy(r, r << (x >> 31), mo, ninv);
+            r = mont_mul_lazy(r, r << ((x >> 30) & 1u), mo, ninv);
+            r = mont_mul_lazy(r, r << ((x >> 29) & 1u), mo, ninv);
+            r = mont_mul_lazy(r, r << ((x >> 28) & 1u), mo, ninv);
+        }
+        for (; n > 0; --n, x <<= 1) r = mont_mul_lazy(r, r << (x >> 31), mo, ninv);
     } else {
         r = dbl_lazy(r, two_mo);
         for (; n >= 4; n -= 4, x <<= 4) {
@@ -484,21 +490,35 @@ __global__ __launch_bounds__(256) void k_bbp_fx_lanes(const BbpSeries* __restric
     const uint32_t j = w == 0 ? 1u : w + 3u;              // 1, 4, 5, 6
     const uint32_t sj = j == 4 ? 2u : (j == 6 ? 1u : 0u); // 2-adic valuation of 8k + j
     uint64_t local = 0;
-    // every modulus of the block is 8k + j <= 8 dm + 6: below 2^28 for dm < 2^25 - 1
-    const bool small = dm < (1u << 25) - 1u;
-    for (uint32_t e = e0; e <= e1; ++e) {
+    // At exponent e every modulus of the wavefront is 8k + j <= 8 (dm - e) + 6:
+    // below 2^29 (the FOLD ladder) from e_fold = dm - (2^26 - 2) on. The two
+    // ranges of e are wave-uniform, so each loop runs one ladder.
+    const uint32_t e_fold = dm > (1u << 26) - 2u ? dm - ((1u << 26) - 2u) : 0u;
+    const uint32_t e_mid = min(max(e_fold, e0), e1 + 1u);
+    for (uint32_t e = e0; e < e_mid; ++e) {
         if (!act || e > d) continue;
         const uint32_t k = d - e;
         const uint32_t mi = 8u * k + j;
         if (mi == 1) continue;                            // 16^d mod 1 = 0
         if (mi < (1u << 30) && e > 0) {
-            local += small ? frac64_mont_uniform<true>(4u * e - sj + 32u, mi, sj)
-                           : frac64_mont_uniform<false>(4u * e - sj + 32u, mi, sj);
+            local += frac64_mont_uniform<false>(4u * e - sj + 32u, mi, sj);
         } else {                                          // 16^0 = 1, or m >= 2^30
             const double m = (double)mi;
             const double minv = recip_f64(m);
             const uint32_t r = e == 0 ? 1u : (uint32_t)modpow16_exact(e, m, minv);
             local += frac64(r, mi, minv);
+        }
+    }
+    for (uint32_t e = e_mid; e <= e1; ++e) {
+        if (!act || e > d) continue;
+        const uint32_t k = d - e;
+        const uint32_t mi = 8u * k + j;
+        if (mi == 1) continue;                            // 16^d mod 1 = 0
+        if (e > 0) {
+            local += frac64_mont_uniform<true>(4u * e - sj + 32u, mi, sj);
+        } else {                                          // 16^0 = 1
+            const double m = (double)mi;
+            local += frac64(1u, mi, recip_f64(m));
         }
     }
     if (act) atomicAdd((unsigned long long*)&acc[4 * pi + w], (unsigned long long)local);

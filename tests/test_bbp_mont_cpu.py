@@ -80,7 +80,7 @@ def test_edges():
 
 # ---- frac64_mont_wave: the fixed-point fraction floor((16^e mod m) 2^64 / m) ----
 
-def frac64_mont_wave(e: int, m: int, q_off: int = 0, emax: int | None = None, small: bool = False) -> int:
+def frac64_mont_wave(e: int, m: int, q_off: int = 0, emax: int | None = None, fold: bool = False) -> int:
     """Step for step as the kernel: inverse from (3 mo) xor 2, the Montgomery
     one from a quotient q within one of floor(2^32 / mo) (q_off models the
     single-precision reciprocal's error), the ladder on E = 4e - s + 32 run
@@ -103,23 +103,28 @@ def frac64_mont_wave(e: int, m: int, q_off: int = 0, emax: int | None = None, sm
     two_mo = 2 * mo
     bits = (emax if emax is not None else E).bit_length()
     assert E.bit_length() <= bits
-    if small:
-        assert mo < 1 << 28
+    if fold:
+        assert mo < 1 << 29
     for b in range(bits - 1, -1, -1):
-        assert r < (4 * mo if small else two_mo)
-        t = r * r
-        mq = ((t & M32) * ninv) & M32
-        r = (mq * mo + t) >> 32
         assert r < two_mo
-        r = (r << ((E >> b) & 1)) & M32
-        if not small:
+        if fold:
+            # the doubling folded into the square: r (r << bit) < 8 mo^2 < 2^32 mo
+            a = r << ((E >> b) & 1)
+            assert a < 1 << 32
+            t = r * a
+            assert t < mo << 32
+            mq = ((t & M32) * ninv) & M32
+            r = (mq * mo + t) >> 32
+            assert r < two_mo
+        else:
+            t = r * r
+            mq = ((t & M32) * ninv) & M32
+            r = (mq * mo + t) >> 32
+            assert r < two_mo
+            r = (r << ((E >> b) & 1)) & M32
             d = (r - two_mo) & M32
             r = d if d < r else r
             assert r < two_mo
-    if small:
-        d = (r - two_mo) & M32
-        r = d if d < r else r
-        assert r < two_mo
     rem2 = r - mo if r >= mo else r
     mq = (r * ninv) & M32
     rem1 = (mq * mo + r) >> 32
@@ -156,18 +161,20 @@ def test_frac64_random_with_quotient_error_and_wave_padding():
         assert frac64_mont_wave(e, m, q_off, emax) == frac_exact(e, m), (e, m, q_off)
 
 
-def test_frac64_small_moduli_form():
-    """Series with d < 2^25 - 1 (every m below 2^28): the ladder on [0, 4 mo)."""
+def test_frac64_fold_form():
+    """Steps whose terms all have k <= 2^26 - 2 (every m below 2^29): each
+    doubling folded into the next square, up to the largest modulus the
+    kernel gives this form, 8 (2^26 - 2) + 6."""
     rng = random.Random(13)
-    for m in [8 * ((1 << 25) - 2) + j for j in (1, 4, 5, 6)] + list(range(2, 200)):
+    for m in [8 * ((1 << 26) - 2) + j for j in (1, 4, 5, 6)] + list(range(2, 200)):
         if m % 8 in (1, 4, 5, 6):
-            for e in (1, 2, 7, 64, (1 << 25) - 2):
-                assert frac64_mont_wave(e, m, small=True) == frac_exact(e, m), (e, m)
+            for e in (1, 2, 7, 64, (1 << 25) - 2, (1 << 28) - 1):
+                assert frac64_mont_wave(e, m, fold=True) == frac_exact(e, m), (e, m)
     for _ in range(4000):
-        m = 8 * rng.randrange(256, (1 << 25) - 1) + rng.choice((1, 4, 5, 6))
-        e = rng.randrange(1, (1 << 25) - 1)
+        m = 8 * rng.randrange(256, (1 << 26) - 1) + rng.choice((1, 4, 5, 6))
+        e = rng.randrange(1, 1 << 28)
         emax = 4 * (e + rng.randrange(0, 256)) + 32
-        assert frac64_mont_wave(e, m, rng.choice((-1, 0, 1)), emax, small=True) == frac_exact(e, m), (e, m)
+        assert frac64_mont_wave(e, m, rng.choice((-1, 0, 1)), emax, fold=True) == frac_exact(e, m), (e, m)
 
 
 def test_frac64_edges():
@@ -178,7 +185,7 @@ def test_frac64_edges():
             assert frac64_mont_wave(e, m) == frac_exact(e, m), (e, m)
 
 
-def frac64_mont_uniform(e: int, m: int, q_off: int = 0, small: bool = False) -> int:
+def frac64_mont_uniform(e: int, m: int, q_off: int = 0, fold: bool = False) -> int:
     """The positions-in-lanes form: the same set-up, the leading exponent bit
     taken as one doubling of the Montgomery one, then the remaining bits
     left-aligned in a 32-bit word, four per trip and the rest one by one."""
@@ -212,23 +219,22 @@ def frac64_mont_uniform(e: int, m: int, q_off: int = 0, small: bool = False) -> 
     n = E.bit_length() - 1
     assert 5 <= n <= 31
     x = (E << (32 - n)) & M32
-    if small:
-        # mo < 2^28: the ladder runs on [0, 4 mo), each doubling one shift
-        assert mo < 1 << 28
-        r = r << 1
+    if fold:
+        # mo < 2^29: the leading one doubled and reduced, then each bit's
+        # doubling folded into its square, r (r << bit) < 8 mo^2 < 2^32 mo
+        assert mo < 1 << 29
+        r = dbl(r)
         while n > 0:
-            assert r < 4 * mo
-            t = r * r
+            assert r < two_mo
+            a = r << (x >> 31)
+            assert a < 1 << 32
+            t = r * a
+            assert t < mo << 32
             mq = ((t & M32) * ninv) & M32
             r = (mq * mo + t) >> 32
             assert r < two_mo
-            r = (r << (x >> 31)) & M32
             n -= 1
             x = (x << 1) & M32
-        d = (r - two_mo) & M32
-        r = d if d < r else r
-        assert r < two_mo
-        n = 0
     else:
         r = dbl(r)
     while n >= 4:
@@ -263,12 +269,13 @@ def test_frac64_uniform_form():
         if m < 1 << 30:
             e = rng.randrange(1, 1 << 28)
             assert frac64_mont_uniform(e, m, rng.choice((-1, 0, 1))) == frac_exact(e, m), (e, m)
-    # the small-modulus form, up to the largest modulus it takes (8 (2^25 - 2) + 6)
-    for m in list(range(2, 300)) + [8 * ((1 << 25) - 2) + j for j in (1, 4, 5, 6)]:
+    # the folded form, up to the largest modulus it takes (8 (2^26 - 2) + 6),
+    # with exponents up to the largest position in this kernel (2^28)
+    for m in list(range(2, 300)) + [8 * ((1 << 26) - 2) + j for j in (1, 4, 5, 6)]:
         if m % 8 in (1, 4, 5, 6):
-            for e in (1, 2, 3, 5, 8, 13, 1000, (1 << 25) - 2):
-                assert frac64_mont_uniform(e, m, small=True) == frac_exact(e, m), (e, m)
+            for e in (1, 2, 3, 5, 8, 13, 1000, (1 << 25) - 2, (1 << 28) - 1):
+                assert frac64_mont_uniform(e, m, fold=True) == frac_exact(e, m), (e, m)
     for _ in range(4000):
-        m = 8 * rng.randrange(256, (1 << 25) - 1) + rng.choice((1, 4, 5, 6))
-        e = rng.randrange(1, 1 << 25)
-        assert frac64_mont_uniform(e, m, rng.choice((-1, 0, 1)), small=True) == frac_exact(e, m), (e, m)
+        m = 8 * rng.randrange(256, (1 << 26) - 1) + rng.choice((1, 4, 5, 6))
+        e = rng.randrange(1, 1 << 28)
+        assert frac64_mont_uniform(e, m, rng.choice((-1, 0, 1)), fold=True) == frac_exact(e, m), (e, m)
