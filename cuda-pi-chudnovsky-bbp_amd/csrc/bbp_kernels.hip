@@ -524,6 +524,253 @@ __global__ __launch_bounds__(256) void k_bbp_fx_lanes(const BbpSeries* __restric
     if (act) atomicAdd((unsigned long long*)&acc[4 * pi + w], (unsigned long long)local);
 }
 
+// ---- moduli in lanes -----------------------------------------------------------------
+// A wavefront holds 64 consecutive terms k = k0 + lane of one series j, that
+// is 64 fixed moduli m = 8k + j, and steps through the positions of its
+// group of 64 in ascending order. With w the Montgomery form of
+// 16^e 2^-s mod mo (e = d - k, m = 2^s mo), w is rem1 = 2^(4e - s + 32) mod mo
+// of frac64_mont_wave, and Montgomery multiplying it by the form of 2^32
+// (16^8, table entry 3) gives rem2; the fraction is then four VALU. Going
+// from position d to the next, d', multiplies w by 16^(d' - d): the gap is
+// wave-uniform, so below 2^16 its set bits select products with the lane's
+// table T_t = form of 16^(2^t) (scalar branches; one test skips the upper
+// eight for gaps below 2^8), and larger gaps take a uniform ladder. Each lane's own ladder (the long one, e ~ d) runs once,
+// at the first position it reaches, and the set-up once per 64 terms. The
+// term with e = 0 (k = d) is the one integer 16^0 = 1 that the 2^-s form
+// does not give, and goes through frac64. Moduli stay below 2^30 (positions
+// up to 2^27 - 1), so every lazy product of residues below 2 mo is below
+// 4 mo^2 < 2^32 mo and reduces to [0, 2 mo).
+constexpr uint32_t BBP_MODULI_DMAX = (uint32_t)BBP_MODULI_DMAX_HOST;   // largest d = position - 1 this layout takes
+
+__device__ __forceinline__ uint32_t dbl_red(uint32_t r, uint32_t two_mo) {
+    r <<= 1;
+    const uint32_t d = r - two_mo;
+    return d < r ? d : r;
+}
+
+// x / 2 mod mo for x in [0, 2 mo), mo odd: the result is below 3 mo / 2.
+__device__ __forceinline__ uint32_t half_mod(uint32_t x, uint32_t mo) { return (x & 1u) ? (x + mo) >> 1 : x >> 1; }
+
+// Montgomery form of 2^E (lazy) for a wave-uniform E >= 2 from the form of
+// one, r1 in [0, 2 mo): scalar bits; FOLD needs mo < 2^29.
+template <bool FOLD>
+__device__ __forceinline__ uint32_t pow2_uniform(uint32_t E, uint32_t r1, uint32_t mo, uint32_t ninv, uint32_t two_mo) {
+    int n = 31 - __clz(E);
+    uint32_t x = E << (32 - n);
+    uint32_t r = dbl_red(r1, two_mo);
+    for (; n > 0; --n, x <<= 1) {
+        if constexpr (FOLD) {
+            r = mont_mul_lazy(r, r << (x >> 31), mo, ninv);
+        } else {
+            r = mont_mul_lazy(r, r, mo, ninv);
+            if ((int32_t)x < 0) r = dbl_lazy(r, two_mo);
+        }
+    }
+    return r;
+}
+
+// Montgomery form of 2^E for a per-lane E whose largest value sits in the
+// first active lane (lanes with a shorter E square the form of one through
+// their leading zero bits, which keeps it).
+template <bool FOLD>
+__device__ __forceinline__ uint32_t pow2_lane(uint32_t E, uint32_t r1, uint32_t mo, uint32_t ninv, uint32_t two_mo) {
+    const uint32_t Emax = __builtin_amdgcn_readfirstlane(E);
+    uint32_t r = r1;
+    for (int b = 31 - __clz(Emax); b >= 0; --b) {
+        if constexpr (FOLD) {
+            r = mont_mul_lazy(r, r << ((E >> b) & 1u), mo, ninv);
+        } else {
+            r = mont_mul_lazy(r, r, mo, ninv) << ((E >> b) & 1u);
+            const uint32_t d = r - two_mo;
+            r = d < r ? d : r;
+        }
+    }
+    return r;
+}
+
+__device__ __forceinline__ void bbp_wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Sum the batch of eight staged slots over the 64 moduli lanes: lane (r, part)
+// reads slot r at lanes part + 8 t (rows 72 words apart put the 64 lanes'
+// reads in 64 distinct banks), three shuffles finish the sum, and sorted
+// position 8 b + r (batch b) is kept by lane 8 r + b.
+__device__ __forceinline__ void bbp_moduli_flush(const uint32_t* __restrict__ lo, const uint32_t* __restrict__ hi,
+                                                 uint32_t batch, uint32_t lane, uint64_t& acc_l) {
+    bbp_wave_sync();
+    const uint32_t r = lane >> 3, part = lane & 7u;
+    uint64_t sum = 0;
+#pragma unroll
+    for (int t = 0; t < 8; ++t)
+        sum += (uint64_t)lo[r * 72u + part + 8u * t] | ((uint64_t)hi[r * 72u + part + 8u * t] << 32);
+    sum += __shfl_xor(sum, 1);
+    sum += __shfl_xor(sum, 2);
+    sum += __shfl_xor(sum, 4);
+    if (part == batch) acc_l += sum;
+    bbp_wave_sync();
+}
+
+// w *= form of 16^gp for a wave-uniform gap gp > 0.
+template <bool FOLD>
+__device__ __forceinline__ uint32_t bbp_gap_mul(uint32_t w, uint32_t gp, const uint32_t* T, uint32_t r1, uint32_t mo,
+                                                uint32_t ninv, uint32_t two_mo) {
+    if (gp < 65536u) {
+#pragma unroll
+        for (int t = 0; t < 8; ++t)
+            if (gp & (1u << t)) w = mont_mul_lazy(w, T[t], mo, ninv);
+        if (gp >> 8) {
+#pragma unroll
+            for (int t = 8; t < 16; ++t)
+                if (gp & (1u << t)) w = mont_mul_lazy(w, T[t], mo, ninv);
+        }
+        return w;
+    }
+    return mont_mul_lazy(w, pow2_uniform<FOLD>(4u * gp, r1, mo, ninv, two_mo), mo, ninv);
+}
+
+// F = floor((16^e mod m) 2^64 / m) for e >= 1 from w (0 for w = 0).
+__device__ __forceinline__ uint64_t bbp_moduli_frac(uint32_t w, uint32_t T3, uint32_t mo, uint32_t ninv) {
+    const uint32_t rem1 = w >= mo ? w - mo : w;
+    uint32_t rem2 = mont_mul_lazy(w, T3, mo, ninv);
+    if (rem2 >= mo) rem2 -= mo;
+    return ((uint64_t)(rem1 * ninv) << 32) | (uint64_t)(rem2 * ninv);
+}
+
+// One k-block k0 .. k0 + 63 of a wavefront over its sorted positions
+// key[0 .. nv) (key held by lane i), with FOLD for every modulus below 2^29.
+// Positions up to the block's largest live k (the triangle, only in blocks
+// that reach into the group) start lanes one by one and hold the e = 0
+// terms; from the first position above it every live lane has started and
+// e >= 1, and a lane with no terms keeps w = 0 (so F = 0) through the gap
+// products. Each position's sum over the lanes goes to the lane
+// p(l) = 8 (l & 7) + (l >> 3) holds (acc_l).
+template <bool FOLD>
+__device__ __forceinline__ void bbp_moduli_kblock(uint32_t k0, uint32_t j, uint32_t sj, uint32_t key, uint32_t nv,
+                                                  uint32_t dmax, uint32_t lane, uint32_t* __restrict__ lo,
+                                                  uint32_t* __restrict__ hi, uint64_t& acc_l) {
+    const uint32_t k = k0 + lane;
+    const uint32_t mi = 8u * k + j;
+    const bool live = k <= dmax && mi != 1u;
+    const uint32_t mo = live ? mi >> sj : 3u;
+    uint32_t inv = (3u * mo) ^ 2u;
+    for (int i = 0; i < 3; ++i) inv *= 2u - mo * inv;
+    const uint32_t ninv = 0u - inv;
+    const uint32_t q = mo >= 2048u ? (uint32_t)(4294967296.0f * __builtin_amdgcn_rcpf((float)mo))
+                                   : (uint32_t)(0x100000000ull / mo);
+    uint32_t r1 = 0u - q * mo;                       // 2^32 mod mo in [0, 2 mo)
+    if ((int32_t)r1 < 0) r1 += mo;
+    const uint32_t two_mo = mo << 1;
+    uint32_t T[16];                                  // forms of 16^(2^t)
+    T[0] = dbl_red(dbl_red(dbl_red(dbl_red(r1, two_mo), two_mo), two_mo), two_mo);
+#pragma unroll
+    for (int t = 1; t < 16; ++t) T[t] = mont_mul_lazy(T[t - 1], T[t - 1], mo, ninv);
+    uint32_t h = r1;                                 // form of 2^-s
+    if (sj >= 1u) h = half_mod(h, mo);
+    if (sj >= 2u) h = half_mod(h, mo);
+    // the first position this block reaches (the smallest d >= k0), and the
+    // first above its largest live k
+    const uint32_t kmax = min(k0 + 63u, dmax);
+    const uint32_t first = (uint32_t)__builtin_ctzll(__ballot(lane < nv && key >= k0));
+    const uint64_t above = __ballot(lane < nv && key > kmax);
+    const uint32_t i1 = above ? (uint32_t)__builtin_ctzll(above) : nv;
+    // slots of the first batch before it hold zero
+    for (uint32_t sl = first & ~7u; sl < first; ++sl) { lo[(sl & 7u) * 72u + lane] = 0u; hi[(sl & 7u) * 72u + lane] = 0u; }
+    uint32_t w = 0;
+    bool started = false;
+    uint32_t dprev = 0;
+    uint32_t i = first;
+    for (; i < i1; ++i) {                            // the triangle
+        const uint32_t di = (uint32_t)__builtin_amdgcn_readlane((int)key, (int)i);
+        if (i > first && di != dprev) w = bbp_gap_mul<FOLD>(w, di - dprev, T, r1, mo, ninv, two_mo);
+        dprev = di;
+        if (live && !started && k <= di) {
+            const uint32_t e = di - k;
+            w = e == 0u ? h : mont_mul_lazy(pow2_lane<FOLD>(4u * e, r1, mo, ninv, two_mo), h, mo, ninv);
+            started = true;
+        }
+        uint64_t F = started ? bbp_moduli_frac(w, T[3], mo, ninv) : 0;
+        if (started && k == di) F = frac64(1u, mi, recip_f64((double)mi));
+        lo[(i & 7u) * 72u + lane] = (uint32_t)F;
+        hi[(i & 7u) * 72u + lane] = (uint32_t)(F >> 32);
+        if ((i & 7u) == 7u) bbp_moduli_flush(lo, hi, i >> 3, lane, acc_l);
+    }
+    if (i < nv) {
+        // position i1: the lanes started in the triangle take the gap, the
+        // others start here (e = d - k >= 1)
+        const uint32_t di = (uint32_t)__builtin_amdgcn_readlane((int)key, (int)i);
+        if (i > first && di != dprev) w = bbp_gap_mul<FOLD>(w, di - dprev, T, r1, mo, ninv, two_mo);
+        if (live && !started) w = mont_mul_lazy(pow2_lane<FOLD>(4u * (di - k), r1, mo, ninv, two_mo), h, mo, ninv);
+        dprev = di;
+        for (;;) {
+            const uint64_t F = bbp_moduli_frac(w, T[3], mo, ninv);
+            lo[(i & 7u) * 72u + lane] = (uint32_t)F;
+            hi[(i & 7u) * 72u + lane] = (uint32_t)(F >> 32);
+            if ((i & 7u) == 7u) bbp_moduli_flush(lo, hi, i >> 3, lane, acc_l);
+            if (++i >= nv) break;
+            const uint32_t dn = (uint32_t)__builtin_amdgcn_readlane((int)key, (int)i);
+            if (dn != dprev) w = bbp_gap_mul<FOLD>(w, dn - dprev, T, r1, mo, ninv, two_mo);
+            dprev = dn;
+        }
+    }
+    // the last batch, when it is partial
+    if (nv & 7u) {
+        for (uint32_t z = nv & 7u; z < 8u; ++z) { lo[z * 72u + lane] = 0u; hi[z * 72u + lane] = 0u; }
+        bbp_moduli_flush(lo, hi, (nv - 1u) >> 3, lane, acc_l);
+    }
+}
+
+// Block = group g of 64 consecutive positions (in the order given) x 4
+// series, one wavefront per series j; chunk c of the block takes the
+// k-blocks [c KB, (c + 1) KB). Each wavefront sorts its group's positions
+// in registers (bitonic, ascending, invalid ones last) and adds each
+// position's sum over its k-blocks with one atomic.
+__global__ __launch_bounds__(256) void k_bbp_fx_moduli(const BbpSeries* __restrict__ st, size_t count,
+                                                       uint32_t chunks, uint32_t KB, uint64_t* __restrict__ acc) {
+    __shared__ uint32_t sbuf[4][2][8 * 72];
+    const uint32_t g = blockIdx.x / chunks;
+    const uint32_t c = blockIdx.x - g * chunks;
+    const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t lane = threadIdx.x & 63;
+    const size_t pi = (size_t)g * 64 + lane;
+    const bool in = pi < count;
+    const BbpSeries S = in ? st[4 * pi + w] : BbpSeries{0, 0.0, 1, 0};
+    const bool act = in && S.valid;
+    uint32_t key = act ? (uint32_t)S.d : 0xFFFFFFFFu;    // d <= 2^27 - 2 here
+    uint32_t idx = lane;
+    for (uint32_t kk = 2; kk <= 64; kk <<= 1) {
+        for (uint32_t jj = kk >> 1; jj > 0; jj >>= 1) {
+            const uint32_t ok = (uint32_t)__shfl_xor((int)key, (int)jj), oi = (uint32_t)__shfl_xor((int)idx, (int)jj);
+            const bool up = (lane & kk) == 0, low = (lane & jj) == 0;
+            const bool less = ok < key || (ok == key && oi < idx);
+            const bool more = ok > key || (ok == key && oi > idx);
+            if (low == up ? less : more) { key = ok; idx = oi; }
+        }
+    }
+    const uint32_t nv = (uint32_t)__popcll(__ballot(act));
+    if (nv == 0) return;                                  // the same in all four wavefronts
+    const uint32_t dmax = (uint32_t)__builtin_amdgcn_readlane((int)key, (int)(nv - 1));
+    const uint32_t j = w == 0 ? 1u : w + 3u;              // 1, 4, 5, 6
+    const uint32_t sj = j == 4 ? 2u : (j == 6 ? 1u : 0u);
+    uint32_t* lo = sbuf[w][0];
+    uint32_t* hi = sbuf[w][1];
+    uint64_t acc_l = 0;
+    const uint32_t kb_end = dmax / 64u + 1u;
+    for (uint32_t kb = c * KB; kb < min((c + 1u) * KB, kb_end); ++kb) {
+        const uint32_t k0 = kb * 64u;
+        // the block's largest modulus, 8 min(k0 + 63, dmax) + 6, below 2^29
+        if (min(k0 + 63u, dmax) <= (1u << 26) - 2u)
+            bbp_moduli_kblock<true>(k0, j, sj, key, nv, dmax, lane, lo, hi, acc_l);
+        else
+            bbp_moduli_kblock<false>(k0, j, sj, key, nv, dmax, lane, lo, hi, acc_l);
+    }
+    const uint32_t p = 8u * (lane & 7u) + (lane >> 3);
+    const uint32_t orig = (uint32_t)__shfl((int)idx, (int)p);
+    if (p < nv && acc_l) atomicAdd((unsigned long long*)&acc[4 * ((size_t)g * 64 + orig) + w], (unsigned long long)acc_l);
+}
+
 // Per group of 64 consecutive entries of pos: out[0] = max(d) (atomic max),
 // out[1] += sum of d + 1 over valid positions, out[2] += 64 (max d of the
 // group + 1): out[1] / out[2] is the share of lanes k_bbp_fx_lanes keeps busy.
@@ -704,7 +951,7 @@ size_t bbp_few_ws_bytes(size_t count) {
 
 // dmax: the largest d = position - 1 among valid positions (known to the host).
 hipError_t launch_bbp_fx(const uint64_t* d_pos, uint64_t start, size_t count, uint64_t dmax, uint8_t* d_out,
-                         void* ws, int odd_undecided, hipStream_t s, uint32_t block, size_t nout, bool lanes) {
+                         void* ws, int odd_undecided, hipStream_t s, uint32_t block, size_t nout, int tk) {
     const size_t S = 4 * count;
     BbpSeries* st = (BbpSeries*)ws;
     uint64_t* acc = (uint64_t*)((char*)ws + ((S * sizeof(BbpSeries) + 255) & ~(size_t)255));
@@ -718,7 +965,17 @@ hipError_t launch_bbp_fx(const uint64_t* d_pos, uint64_t start, size_t count, ui
     int KT = 32;
     uint64_t bps = (dmax + 1 + 256ull * KT - 1) / (256ull * KT);
     while (bps > 1 && bps * S > ((uint64_t)1 << 17)) { KT *= 2; bps = (dmax + 1 + 256ull * KT - 1) / (256ull * KT); }
-    if (lanes && dmax < ((uint64_t)1 << 28)) {          // 32-bit term indices
+    if (tk == BBP_TK_MODULI && dmax <= BBP_MODULI_DMAX) {
+        // about 2^20 blocks: each wavefront takes KB consecutive blocks of 64
+        // terms, its set-up and sort shared by their 64 x 64 terms
+        const uint64_t groups = (count + 63) / 64;
+        const uint64_t kblocks = dmax / 64 + 1;
+        const uint64_t want = std::max<uint64_t>(((uint64_t)1 << 20) / groups, 1);
+        const uint64_t chunks = std::min<uint64_t>(want, kblocks);
+        const uint64_t KB = (kblocks + chunks - 1) / chunks;
+        BBP_LAUNCH("k_bbp_fx_moduli", s, k_bbp_fx_moduli, dim3((unsigned)(groups * chunks)), dim3(256), 0, s,
+                   (const BbpSeries*)st, count, (uint32_t)chunks, (uint32_t)KB, acc);
+    } else if (tk != BBP_TK_TERMS && dmax < ((uint64_t)1 << 28)) {          // 32-bit term indices
         // about 2^20 blocks, but at least 64 exponents per block (measured on
         // 1..65,536: 2^14 blocks of 4,096 exponents 49 ms, 2^18 of 256 29 ms,
         // 2^20 of 64 28 ms; every 7th position to 65,536: 10 exponents per

@@ -730,8 +730,16 @@ double lane_share(const uint64_t* h_pos, uint64_t start, uint64_t stride, size_t
 // (measured: 1..65,536, share 0.99, 27 against 36 ms; 512 sorted positions
 // in [10^6, 10^8], share 0.89, 0.47 against 0.80 s), so it pays from a
 // share of about 0.6 up (64 such positions have 0.52).
-bool bbp_use_lanes(const BbpCfg& cfg, double share) {
-    return cfg.layout == 2 || (cfg.layout == 0 && share >= 0.6);
+// Moduli in lanes, where it applies (largest position up to 2^27 - 1), is
+// the default: measured against the better of the other two it takes 12.7
+// against 23.6 ms on 1..65,536, 0.33 against 0.44 s on 512 sorted positions
+// in [10^6, 10^8], 3.0 against 4.8 s on 4,096 unsorted ones and the same
+// 62-66 ms on 64 unsorted ones. pdh_bbp_set_layout 1 and 2 force the
+// others, 3 forces it (the default rule where it does not apply).
+int bbp_term_kernel(const BbpCfg& cfg, double share, uint64_t dmax) {
+    if ((cfg.layout == 0 || cfg.layout == 3) && dmax <= BBP_MODULI_DMAX_HOST) return BBP_TK_MODULI;
+    if (cfg.layout == 1) return BBP_TK_TERMS;
+    return cfg.layout == 2 || share >= 0.6 ? BBP_TK_LANES : BBP_TK_TERMS;
 }
 
 // A contiguous range by digit blocks: one fixed-point evaluation per G
@@ -745,7 +753,8 @@ int bbp_blocks(const BbpCfg& cfg, uint64_t start, size_t count, uint32_t G, uint
     int rc = sc.get(&ws, bbp_few_ws_bytes(A));
     if (rc) return rc;
     hipError_t e = launch_bbp_fx(nullptr, start, A, start - 1 + (A - 1) * (uint64_t)G, d_out, ws,
-                                 cfg.exact == 2, st, G, count, bbp_use_lanes(cfg, lane_share(nullptr, start, G, A)));
+                                 cfg.exact == 2, st, G, count,
+                                 bbp_term_kernel(cfg, lane_share(nullptr, start, G, A), start - 1 + (A - 1) * (uint64_t)G));
     if (e != hipSuccess) return fail_hip(e, "bbp launch");
     HIPCALL(hipMemcpyAsync(h_vals, d_out, count, hipMemcpyDeviceToHost, st));
     HIPCALL(hipStreamSynchronize(st));
@@ -795,7 +804,7 @@ int bbp_few(const BbpCfg& cfg, const uint64_t* d_pos, const uint64_t* h_pos, uin
     }
     if (share < 0) share = d_pos && !h_pos ? 0.0 : lane_share(h_pos, start, 1, count);
     hipError_t e = launch_bbp_fx(d_pos, start, count, dmax, d_out, ws, cfg.exact == 2, st, 1, 0,
-                                 bbp_use_lanes(cfg, share));
+                                 bbp_term_kernel(cfg, share, dmax));
     if (e != hipSuccess) return fail_hip(e, "bbp launch");
     HIPCALL(hipMemcpyAsync(h_vals, d_out, count, hipMemcpyDeviceToHost, st));
     HIPCALL(hipStreamSynchronize(st));
@@ -1114,7 +1123,7 @@ int pdh_bbp_set_exact(int mode) {
 }
 
 int pdh_bbp_set_layout(int mode) {
-    if (mode < 0 || mode > 2) return fail(PDH_E_ARG, "pdh_bbp_set_layout: mode must be 0, 1 or 2");
+    if (mode < 0 || mode > 3) return fail(PDH_E_ARG, "pdh_bbp_set_layout: mode must be 0, 1, 2 or 3");
     Ctx& c = ctx();
     std::lock_guard<std::mutex> g(c.mu);
     c.bbp_layout = mode;

@@ -132,8 +132,9 @@ size_t carry_ws_bytes(size_t nlimbs);
 // undecided). With block > 1 and no d_pos, launch_bbp_fx evaluates the
 // `count` anchors start + block t and writes the digits of the nout
 // positions start .. start + nout - 1 (block of them per anchor, 0xFE where
-// undecided); with `lanes` its term kernel holds positions in lanes
-// (k_bbp_fx_lanes) instead of terms. launch_bbp_group_scan: over groups of 64
+// undecided); `tk` picks its term kernel: terms in lanes (k_bbp_fx_terms),
+// positions in lanes (k_bbp_fx_lanes) or moduli in lanes (k_bbp_fx_moduli,
+// for dmax <= BBP_MODULI_DMAX). launch_bbp_group_scan: over groups of 64
 // entries of d_pos, d_out3[0] = max(position - 1), d_out3[1] = sum of
 // positions, d_out3[2] = sum of 64 x the group's largest position (the
 // caller zeroes the three words).
@@ -141,7 +142,9 @@ hipError_t launch_bbp(const uint64_t* d_pos, uint64_t start, size_t count, uint8
                       hipStream_t s);
 hipError_t launch_bbp_fx(const uint64_t* d_pos, uint64_t start, size_t count, uint64_t dmax,
                          uint8_t* d_out, void* ws, int odd_undecided, hipStream_t s, uint32_t block = 1,
-                         size_t nout = 0, bool lanes = false);
+                         size_t nout = 0, int tk = 0);
+enum : int { BBP_TK_TERMS = 0, BBP_TK_LANES = 1, BBP_TK_MODULI = 2 };
+constexpr uint64_t BBP_MODULI_DMAX_HOST = ((uint64_t)1 << 27) - 2;   // largest d the moduli kernel takes
 hipError_t launch_bbp_chunked(const uint64_t* d_pos, uint64_t start, size_t count, uint64_t dmax,
                               uint8_t* d_out, void* ws, hipStream_t s);
 hipError_t launch_bbp_group_scan(const uint64_t* d_pos, size_t count, uint64_t* d_out3, hipStream_t s);

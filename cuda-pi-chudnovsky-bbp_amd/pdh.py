@@ -388,7 +388,8 @@ def bbp_set_exact(mode: int) -> None:
 
 def bbp_set_layout(mode: int) -> None:
     """Term kernel of the fixed-point path: 0 chosen per call (default),
-    1 terms in lanes, 2 positions in lanes (testing; same digits)."""
+    1 terms in lanes, 2 positions in lanes, 3 moduli in lanes (testing; same
+    digits)."""
     _check(lib().pdh_bbp_set_layout(int(mode)))
 
 

@@ -30,11 +30,14 @@ extern "C" {
 int pdh_bbp_set_exact(int mode);
 uint64_t pdh_bbp_exact_count(void);
 /* pdh_bbp_set_layout(mode): the fixed-point path's term kernel. 0 (default)
- * chooses per call: positions in lanes (one exponent per wavefront, its bits
- * scalar) when at least three lanes in four stay busy -- the positions of
- * each group of 64, in the order given, lie close together -- else terms in
- * lanes; 1 forces terms in lanes, 2 positions in lanes. The digits are the
- * same either way. */
+ * chooses per call: moduli in lanes (64 terms k of one series per
+ * wavefront, stepping through each group of 64 positions in sorted order)
+ * when the largest position is at most 2^27 - 1; above, positions in lanes
+ * (one exponent per wavefront, its bits scalar) when at least three lanes
+ * in four stay busy -- the positions of each group of 64, in the order
+ * given, lie close together -- else terms in lanes. 1 forces terms in
+ * lanes, 2 positions in lanes, 3 moduli in lanes (where it applies, else as
+ * 0). The digits are the same either way. */
 int pdh_bbp_set_layout(int mode);
 /* The device's measured v_mad_u64_u32 issue rate (lane operations per
  * second, whole GPU), the integer-multiply ceiling BBP throughput is quoted

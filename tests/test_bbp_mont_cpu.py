@@ -279,3 +279,132 @@ def test_frac64_uniform_form():
         m = 8 * rng.randrange(256, (1 << 26) - 1) + rng.choice((1, 4, 5, 6))
         e = rng.randrange(1, 1 << 28)
         assert frac64_mont_uniform(e, m, rng.choice((-1, 0, 1)), fold=True) == frac_exact(e, m), (e, m)
+
+
+# ---- moduli in lanes: 64 terms k per wavefront stepping through sorted positions ----
+
+def moduli_block_sums(keys, j: int, k0: int, dmax: int, r1_high: bool = False) -> list:
+    """k_bbp_fx_moduli's k-block k0 .. k0 + 63 of series j over the sorted
+    positions - 1 `keys`, lane by lane as the kernel: the Montgomery set-up,
+    the table T_t = form of 16^(2^t), the form h of 2^-s, each lane started at
+    its first position by its own ladder (folded below 2^29), gaps below 2^16
+    by table products and larger ones by a uniform ladder, the fraction from
+    rem1 = w and rem2 = w T_3 (frac64 for e = 0), and the sum over the 64
+    lanes of each position mod 2^64. Every lazy value's bound is asserted.
+    r1_high takes the Montgomery one as 2^32 mod mo + mo, what a quotient
+    estimate one too low gives."""
+    s = 2 if j == 4 else (1 if j == 6 else 0)
+    fold = min(k0 + 63, dmax) <= (1 << 26) - 2
+    nv = len(keys)
+    first = next(i for i in range(nv) if keys[i] >= k0)
+    sums = [0] * nv
+    for lane in range(64):
+        k = k0 + lane
+        mi = 8 * k + j
+        if not (k <= dmax and mi != 1):
+            continue
+        mo = mi >> s
+        assert mo < (1 << 29 if fold else 1 << 30)
+        inv = ((3 * mo) ^ 2) & M32
+        for _ in range(3):
+            inv = (inv * ((2 - mo * inv) & M32)) & M32
+        ninv = (-inv) & M32
+        two_mo = 2 * mo
+        r1 = (-(((1 << 32) // mo) * mo)) & M32
+        r1 = r1 if r1 < 1 << 31 else (r1 + mo) & M32
+        if r1_high:
+            r1 += mo
+        assert r1 < two_mo and r1 % mo == (1 << 32) % mo
+
+        def mont(a, b):
+            assert a < two_mo and b < 1 << 32 and a * b < mo << 32
+            t = a * b
+            r = (((t & M32) * ninv & M32) * mo + t) >> 32
+            assert r < two_mo
+            return r
+
+        def dbl(r):
+            r <<= 1
+            assert r < 1 << 32
+            return r - two_mo if r >= two_mo else r
+
+        T = [dbl(dbl(dbl(dbl(r1))))]
+        for _ in range(15):
+            T.append(mont(T[-1], T[-1]))
+        h = r1
+        for _ in range(s):
+            h = (h + mo) >> 1 if h & 1 else h >> 1
+
+        def ladder(E):                       # form of 2^E, E >= 1
+            r = r1
+            for b in range(E.bit_length() - 1, -1, -1):
+                bit = (E >> b) & 1
+                if fold:
+                    r = mont(r, r << bit)
+                else:
+                    r = dbl(mont(r, r)) if bit else mont(r, r)
+            return r
+
+        started, w, dprev = False, 0, 0
+        for i in range(first, nv):
+            di = keys[i]
+            if i > first and di != dprev:
+                gp = di - dprev
+                if gp < 1 << 16:
+                    for t in range(16):
+                        if gp >> t & 1 and started:
+                            w = mont(w, T[t])
+                elif started:
+                    w = mont(w, ladder(4 * gp))
+            dprev = di
+            if not started and k <= di:
+                e = di - k
+                w = h if e == 0 else mont(ladder(4 * e), h)
+                started = True
+            if started:
+                if k == di:
+                    F = (1 << 64) // mi
+                else:
+                    rem1 = w - mo if w >= mo else w
+                    rem2 = mont(w, T[3])
+                    rem2 = rem2 - mo if rem2 >= mo else rem2
+                    F = (((rem1 * ninv) & M32) << 32) | ((rem2 * ninv) & M32)
+                sums[i] = (sums[i] + F) & ((1 << 64) - 1)
+    return sums
+
+
+def moduli_block_direct(keys, j: int, k0: int, dmax: int) -> list:
+    out = []
+    for d in keys:
+        t = 0
+        for k in range(k0, min(k0 + 63, dmax, d) + 1):
+            m = 8 * k + j
+            if m != 1:
+                t += (pow(16, d - k, m) << 64) // m
+        out.append(t & ((1 << 64) - 1))
+    return out
+
+
+def test_moduli_layout_restated():
+    """Small positions (k = 0 and e = 0 terms, m = 4), blocks that start
+    inside the group (lanes started late), gaps below 2^8, below 2^16 and
+    above, repeats, and k-blocks on both sides of the folded ladder's bound (2^26 - 2)
+    and at the layout's largest position - 1 (2^27 - 2)."""
+    cases = [
+        (list(range(0, 70)), 0),
+        (list(range(0, 70)), 64),
+        ([5, 5, 9, 300, 301, 900, 4000, 4000, 4255, 70000], 0),
+        ([5, 5, 9, 300, 301, 900, 4000, 4000, 4255, 70000], 4032),
+        ([(1 << 26) - 40 + 37 * i for i in range(12)], (1 << 26) - 64),
+        ([(1 << 26) - 40 + 37 * i for i in range(12)], (1 << 26) - 40),
+        ([(1 << 26) + 900, (1 << 26) + 1300, (1 << 26) + 1301], 1 << 25),
+        ([(1 << 27) - 1000, (1 << 27) - 301, (1 << 27) - 300, (1 << 27) - 2], (1 << 27) - 64),
+        ([(1 << 27) - 1000, (1 << 27) - 301, (1 << 27) - 300, (1 << 27) - 2], (1 << 27) - 1024),
+    ]
+    for keys, k0 in cases:
+        keys = sorted(keys)
+        dmax = keys[-1]
+        for j in (1, 4, 5, 6):
+            want = moduli_block_direct(keys, j, k0, dmax)
+            for high in (False, True):
+                assert moduli_block_sums(keys, j, k0, dmax, high) == want, (keys[:3], j, k0, high)

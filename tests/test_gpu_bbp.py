@@ -217,12 +217,14 @@ def layout(pdh):
     pdh.bbp_set_layout(0)
 
 
-@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("mode", [1, 2, 3])
 def test_both_term_layouts_give_the_reference_digits(pdh, golden, layout, mode):
-    """Terms in lanes (1) and positions in lanes (2, one exponent per
-    wavefront): the contiguous range 1..65,536 by positions, the 64 sampled
-    golden positions (spread over [10^6, 10^8], so lanes idle in layout 2)
-    and digit blocks at 10^6."""
+    """Terms in lanes (1), positions in lanes (2, one exponent per
+    wavefront) and moduli in lanes (3, 64 terms k per wavefront stepping
+    through the sorted positions): the contiguous range 1..65,536 by
+    positions, the 64 sampled golden positions (spread over [10^6, 10^8], so
+    lanes idle in layout 2 and the gaps pass 2^8 in layout 3) and digit
+    blocks at 10^6."""
     import torch
     layout(mode)
     g = golden["bbp_range"]
@@ -241,13 +243,18 @@ def test_both_term_layouts_give_the_reference_digits(pdh, golden, layout, mode):
 
 
 def test_layout_choice(pdh, golden, layout):
-    """The default picks positions in lanes for close positions (a contiguous
-    range) and terms in lanes for spread ones (the 64 sampled positions)."""
+    """The default takes moduli in lanes up to position 2^27 - 1 (a
+    contiguous range, the 64 sampled positions); above it, positions in lanes
+    for close positions and terms in lanes for spread ones."""
     import torch
     layout(0)
     s = golden["bbp_sampled"]
-    for positions, kernel in ((np.arange(1, 4097, dtype=np.uint64), "k_bbp_fx_lanes"),
-                              (np.array(s["positions"], dtype=np.uint64), "k_bbp_fx_terms")):
+    above = np.uint64((1 << 27) + 1)
+    for positions, kernel in ((np.arange(1, 4097, dtype=np.uint64), "k_bbp_fx_moduli"),
+                              (np.array(s["positions"], dtype=np.uint64), "k_bbp_fx_moduli"),
+                              (above + np.arange(0, 256, dtype=np.uint64), "k_bbp_fx_lanes"),
+                              (np.uint64(1) + np.arange(0, 64, dtype=np.uint64) * np.uint64(1 << 22),
+                               "k_bbp_fx_terms")):
         dpos = torch.from_numpy(positions.view(np.int64)).cuda()
         dv = torch.zeros(len(positions), dtype=torch.uint8, device="cuda")
         torch.cuda.synchronize()
@@ -259,4 +266,49 @@ def test_layout_choice(pdh, golden, layout):
         pdh.timing_enable(False)
         assert kernel in names, names
     with pytest.raises(pdh.PdhError):
-        pdh.bbp_set_layout(3)
+        pdh.bbp_set_layout(4)
+
+
+def test_moduli_layout_edges(pdh, golden, layout):
+    """Moduli in lanes against terms in lanes, position by position, where
+    its cases differ: groups in no order with repeats (the kernel sorts each
+    group of 64), positions 1 and 2 (the terms k = 0 and e = 0, m = 4 among
+    them), ragged group and batch sizes, gaps below and above 2^8, moduli on
+    both sides of 2^29 (the folded ladder's bound) and the largest position
+    the layout takes, 2^27 - 1, beside one it hands to the default."""
+    import torch
+    rng = np.random.default_rng(77)
+    s = np.array(golden["bbp_sampled"]["positions"], dtype=np.uint64)
+    sets = {
+        "sampled, shuffled, repeats": np.concatenate([rng.permutation(s), s[:5], s[3:4]]),
+        "1..70 reversed": np.arange(70, 0, -1, dtype=np.uint64),
+        "around 2^26 by 37": (np.uint64(1 << 26) - np.uint64(600) + np.arange(0, 37 * 45, 37, dtype=np.uint64)),
+        "near 2^27 - 1": np.array([(1 << 27) - 1, (1 << 27) - 1000, (1 << 27) - 300, 5, (1 << 27) - 301],
+                                  dtype=np.uint64),
+        "past the layout": np.array([(1 << 27) + 5, 1000], dtype=np.uint64),
+    }
+    for name, positions in sets.items():
+        want = None
+        for mode in (1, 3):
+            layout(mode)
+            dpos = torch.from_numpy(positions.astype(np.uint64).view(np.int64)).cuda()
+            dv = torch.zeros(len(positions), dtype=torch.uint8, device="cuda")
+            torch.cuda.synchronize()
+            pdh.timing_enable(True)
+            pdh.bbp_positions_dev(dpos.data_ptr(), len(positions), dv.data_ptr(),
+                                  torch.cuda.current_stream().cuda_stream)
+            torch.cuda.synchronize()
+            names = pdh.timing_read()
+            pdh.timing_enable(False)
+            got = dv.cpu().numpy()
+            if mode == 1:
+                want = got
+            else:
+                assert ("k_bbp_fx_moduli" in names) == (name != "past the layout"), (name, names)
+                bad = np.nonzero(got != want)[0]
+                assert not len(bad), f"{name}: {len(bad)} differ, first position {positions[bad[0]]}"
+        print(f"{name}: {len(positions)} positions equal", flush=True)
+    s64 = golden["bbp_sampled"]
+    layout(3)
+    assert pdh.to_hex(pdh.bbp_values_positions(np.array(s64["positions"], dtype=np.uint64)[::-1].copy())) == \
+        s64["hex"][::-1]

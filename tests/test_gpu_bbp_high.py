@@ -44,7 +44,7 @@ def test_fixture_shape(high):
     assert all(8 * (int(p) - 1) + 6 >= (1 << 32) for p in pos[10:])
 
 
-@pytest.mark.parametrize("layout", [0, 1, 2])
+@pytest.mark.parametrize("layout", [0, 1, 2, 3])
 def test_fixed_point_layouts(modes, high, layout):
     pdh = modes
     pos, hexd, _ = high
