@@ -336,17 +336,38 @@ def bbp_sampled_legs(pdh, torch, W, golden, stream, dev, out, big):
                                     "extrapolated_2p20_positions_s": M20 / (M16k / (ms16k * 1e-3)),
                                     "checked_vs_fixture": len(checked),
                                     "bit_exact": all(a == b for a, b in checked)}
-    # the bound is VALU issue: each term kernel's VALU wave-instructions per
-    # dispatch (rocprofv3 SQ counters over this same workload: the 64 and the
-    # 512 sampled positions, profiles/pmc_sq_bbp.txt), 64 lane operations
-    # each, over the leg's measured time, against the VALU issue peak
-    for kern, leg in (("k_bbp_fx_terms", "bbp_sampled_64"), ("k_bbp_fx_lanes", "bbp_sampled_512")):
-        v = read_pmc("pmc_sq_bbp.txt", kern, ("SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_WAVE_CYCLES", "SQ_WAIT_ANY"))
+        # one chunk of the full workload as --bbp-2p20 runs it: the 2^20
+        # positions sorted and cut in 64, chunk 32 (16,384 consecutive sorted
+        # positions, so gaps ~100 apart, where the sample above has ~6,000)
+        chunk = np.split(np.sort(W.sampled_positions(gs["seed"], 1 << 20, gs["lo"], gs["hi"])), 64)[32]
+        dch = torch.from_numpy(chunk.view(np.int64)).to(dev)
+        vch = torch.zeros(len(chunk), dtype=torch.uint8, device=dev)
+        e0.record(stream)
+        pdh.bbp_positions_dev(dch.data_ptr(), len(chunk), vch.data_ptr(), sp)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        msch = e0.elapsed_time(e1)
+        Mch = W.bbp_mulmod_count(chunk)
+        hch = pdh.to_hex(vch.cpu().numpy())
+        cch = [(ref4k[int(p)], h) for p, h in zip(chunk, hch) if int(p) in ref4k]
+        out["bbp_c4_chunk32"] = {"ms": msch, "mulmod_per_s": Mch / (msch * 1e-3), "mulmods": Mch,
+                                 "positions": f"chunk 32 of 64 of the 2^20 sorted draw ({int(chunk[0])}..{int(chunk[-1])})",
+                                 "extrapolated_2p20_positions_s": M20 / (Mch / (msch * 1e-3)),
+                                 "checked_vs_fixture": len(cch),
+                                 "bit_exact": all(a == b for a, b in cch)}
+    # the bound is VALU issue: the term kernel's VALU wave-instructions per
+    # dispatch (rocprofv3 SQ counters over this same workload: its first
+    # dispatch is the 64 sampled positions, its second the 512,
+    # profiles/pmc_sq_bbp.txt), 64 lane operations each, over the leg's
+    # measured time, against the VALU issue peak
+    for entry, leg in (("k_bbp_fx_moduli [1]", "bbp_sampled_64"), ("k_bbp_fx_moduli [2]", "bbp_sampled_512")):
+        v = read_pmc("pmc_sq_bbp.txt", entry, ("SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_WAVE_CYCLES", "SQ_WAIT_ANY"))
         if v and leg in out:
             lane_ops = v["SQ_INSTS_VALU"] * 64
             t = out[leg]["ms"] * 1e-3
-            out["bbp_valu_" + kern] = {
-                "bound": "valu", "workload": leg, "achieved": lane_ops / t / 1e12, "peak": VALU_PEAK / 1e12,
+            out["bbp_valu_" + leg[4:]] = {
+                "bound": "valu", "kernel": entry, "workload": leg, "achieved": lane_ops / t / 1e12,
+                "peak": VALU_PEAK / 1e12,
                 "unit": "T lane-op/s", "frac": lane_ops / t / VALU_PEAK,
                 "valu_instr_per_mulmod": lane_ops / out[leg]["mulmods"],
                 "salu_per_valu": v["SQ_INSTS_SALU"] / v["SQ_INSTS_VALU"],

@@ -3,8 +3,10 @@ profiles/ directory.
 
     python cuda-pi-chudnovsky-bbp_amd/rocprof_summary.py trace DB TITLE COMMAND
         per-kernel calls / total / average duration (top_kernels view)
-    python cuda-pi-chudnovsky-bbp_amd/rocprof_summary.py pmc DB [DB ...]
-        per-kernel mean per dispatch of every counter collected, and, where
+    python cuda-pi-chudnovsky-bbp_amd/rocprof_summary.py pmc [--split KERNEL] DB [DB ...]
+        per-kernel mean per dispatch of every counter collected (with
+        --split, KERNEL's dispatches one by one, as "KERNEL [n]" in dispatch
+        order, for a kernel that serves several workloads), and, where
         SQ_INSTS_VALU and GRBM_GUI_ACTIVE are among them, derived lines:
         the dispatch's shader cycles per XCD (GRBM_GUI_ACTIVE is summed over
         MI355X's 8 XCDs), the VALU issue fraction (SQ_INSTS_VALU counts wave
@@ -31,7 +33,7 @@ def short(name: str) -> str:
     return name.split("(")[0].replace("void ", "").replace("pdh::", "")
 
 
-def counters(db: str) -> dict:
+def counters(db: str, split: str | None = None) -> dict:
     c = sqlite3.connect(db)
     per = collections.defaultdict(float)
     names = {}
@@ -39,9 +41,11 @@ def counters(db: str) -> dict:
             "select dispatch_id, kernel_name, counter_name, value from counters_collection"):
         per[(disp, cname)] += val
         names[disp] = kname
+    order = {d: n + 1 for n, d in enumerate(sorted(d for d in names if short(names[d]) == split))}
     agg = collections.defaultdict(lambda: collections.defaultdict(list))
     for (disp, cname), v in per.items():
-        agg[short(names[disp])][cname].append(v)
+        k = short(names[disp])
+        agg[f"{k} [{order[disp]}]" if k == split else k][cname].append(v)
     return {k: {cn: sum(v) / len(v) for cn, v in d.items()} for k, d in agg.items()}
 
 
@@ -56,10 +60,10 @@ def trace(db: str, title: str, cmd: str) -> None:
         print(f"{short(name)[:44]:<44} {calls:>6} {tot:>11.1f} {avg:>9.2f} {pct:>6.2f}")
 
 
-def pmc(dbs: list[str]) -> None:
+def pmc(dbs: list[str], split: str | None = None) -> None:
     out = collections.defaultdict(dict)
     for db in dbs:
-        for k, d in counters(db).items():
+        for k, d in counters(db, split).items():
             out[k].update(d)
     for k in sorted(out):
         print(k)
@@ -97,7 +101,10 @@ if __name__ == "__main__":
     if mode == "trace":
         trace(sys.argv[2], sys.argv[3], sys.argv[4])
     elif mode == "pmc":
-        pmc(sys.argv[2:])
+        if sys.argv[2] == "--split":
+            pmc(sys.argv[4:], sys.argv[3])
+        else:
+            pmc(sys.argv[2:])
     elif mode == "traffic":
         traffic(sys.argv[2], sys.argv[3], float(sys.argv[4]), sys.argv[5])
     else:
