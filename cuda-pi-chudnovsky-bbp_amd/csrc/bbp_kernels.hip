@@ -530,12 +530,13 @@ __global__ __launch_bounds__(256) void k_bbp_fx_lanes(const BbpSeries* __restric
 // group of 64 in ascending order. With w the Montgomery form of
 // 16^e 2^-s mod mo (e = d - k, m = 2^s mo), w is rem1 = 2^(4e - s + 32) mod mo
 // of frac64_mont_wave, and Montgomery multiplying it by the form of 2^32
-// (16^8, table entry 3) gives rem2; the fraction is then four VALU. Going
-// from position d to the next, d', multiplies w by 16^(d' - d): the gap is
-// wave-uniform, so below 2^16 its set bits select products with the lane's
-// table T_t = form of 16^(2^t) (scalar branches; one test skips the upper
-// eight for gaps below 2^8), and larger gaps take a uniform ladder. Each lane's own ladder (the long one, e ~ d) runs once,
-// at the first position it reaches, and the set-up once per 64 terms. The
+// (16^8) gives rem2; the fraction is then four VALU. Going from position d
+// to the next, d', multiplies w by 16^(d' - d): the gap is wave-uniform, so
+// its two low nibbles index the lane's tables of 16^n and 16^(16 n)
+// (n < 16) -- at most two products below a gap of 2^8 -- and its higher
+// part takes a uniform power of 16^256. Each lane's own ladder (the long
+// one, e ~ d) runs once, at the first position it reaches, and the set-up
+// once per 64 terms. The
 // term with e = 0 (k = d) is the one integer 16^0 = 1 that the 2^-s form
 // does not give, and goes through frac64. Moduli stay below 2^30 (positions
 // up to 2^27 - 1), so every lazy product of residues below 2 mo is below
@@ -550,24 +551,6 @@ __device__ __forceinline__ uint32_t dbl_red(uint32_t r, uint32_t two_mo) {
 
 // x / 2 mod mo for x in [0, 2 mo), mo odd: the result is below 3 mo / 2.
 __device__ __forceinline__ uint32_t half_mod(uint32_t x, uint32_t mo) { return (x & 1u) ? (x + mo) >> 1 : x >> 1; }
-
-// Montgomery form of 2^E (lazy) for a wave-uniform E >= 2 from the form of
-// one, r1 in [0, 2 mo): scalar bits; FOLD needs mo < 2^29.
-template <bool FOLD>
-__device__ __forceinline__ uint32_t pow2_uniform(uint32_t E, uint32_t r1, uint32_t mo, uint32_t ninv, uint32_t two_mo) {
-    int n = 31 - __clz(E);
-    uint32_t x = E << (32 - n);
-    uint32_t r = dbl_red(r1, two_mo);
-    for (; n > 0; --n, x <<= 1) {
-        if constexpr (FOLD) {
-            r = mont_mul_lazy(r, r << (x >> 31), mo, ninv);
-        } else {
-            r = mont_mul_lazy(r, r, mo, ninv);
-            if ((int32_t)x < 0) r = dbl_lazy(r, two_mo);
-        }
-    }
-    return r;
-}
 
 // Montgomery form of 2^E for a per-lane E whose largest value sits in the
 // first active lane (lanes with a shorter E square the form of one through
@@ -595,17 +578,16 @@ __device__ __forceinline__ void bbp_wave_sync() {
 }
 
 // Sum the batch of eight staged slots over the 64 moduli lanes: lane (r, part)
-// reads slot r at lanes part + 8 t (rows 72 words apart put the 64 lanes'
-// reads in 64 distinct banks), three shuffles finish the sum, and sorted
+// reads slot r at lanes part + 8 t (rows of 72 u64 put each half-wavefront's
+// 32 reads in 64 distinct banks), three shuffles finish the sum, and sorted
 // position 8 b + r (batch b) is kept by lane 8 r + b.
-__device__ __forceinline__ void bbp_moduli_flush(const uint32_t* __restrict__ lo, const uint32_t* __restrict__ hi,
-                                                 uint32_t batch, uint32_t lane, uint64_t& acc_l) {
+__device__ __forceinline__ void bbp_moduli_flush(const uint64_t* __restrict__ buf, uint32_t batch, uint32_t lane,
+                                                 uint64_t& acc_l) {
     bbp_wave_sync();
     const uint32_t r = lane >> 3, part = lane & 7u;
     uint64_t sum = 0;
 #pragma unroll
-    for (int t = 0; t < 8; ++t)
-        sum += (uint64_t)lo[r * 72u + part + 8u * t] | ((uint64_t)hi[r * 72u + part + 8u * t] << 32);
+    for (int t = 0; t < 8; ++t) sum += buf[r * 72u + part + 8u * t];
     sum += __shfl_xor(sum, 1);
     sum += __shfl_xor(sum, 2);
     sum += __shfl_xor(sum, 4);
@@ -613,22 +595,24 @@ __device__ __forceinline__ void bbp_moduli_flush(const uint32_t* __restrict__ lo
     bbp_wave_sync();
 }
 
-// w *= form of 16^gp for a wave-uniform gap gp > 0.
-template <bool FOLD>
-__device__ __forceinline__ uint32_t bbp_gap_mul(uint32_t w, uint32_t gp, const uint32_t* T, uint32_t r1, uint32_t mo,
-                                                uint32_t ninv, uint32_t two_mo) {
-    if (gp < 65536u) {
-#pragma unroll
-        for (int t = 0; t < 8; ++t)
-            if (gp & (1u << t)) w = mont_mul_lazy(w, T[t], mo, ninv);
-        if (gp >> 8) {
-#pragma unroll
-            for (int t = 8; t < 16; ++t)
-                if (gp & (1u << t)) w = mont_mul_lazy(w, T[t], mo, ninv);
+// w *= form of 16^gp for a wave-uniform gap gp > 0: gp = n0 + 16 n1 + 256 r
+// takes the lane's entries U0[n0] (form of 16^n0) and U1[n1] (16^(16 n1)),
+// read by the uniform index, and V^r (V = form of 16^256) by a uniform
+// square-and-multiply.
+__device__ __forceinline__ uint32_t bbp_gap_mul(uint32_t w, uint32_t gp, const uint32_t* U0, const uint32_t* U1,
+                                                uint32_t V, uint32_t mo, uint32_t ninv) {
+    const uint32_t n0 = gp & 15u, n1 = (gp >> 4) & 15u, r = gp >> 8;
+    if (n0) w = mont_mul_lazy(w, U0[n0], mo, ninv);
+    if (n1) w = mont_mul_lazy(w, U1[n1], mo, ninv);
+    if (r) {
+        uint32_t p = V;
+        for (int b = 30 - __clz(r); b >= 0; --b) {
+            p = mont_mul_lazy(p, p, mo, ninv);
+            if ((r >> b) & 1u) p = mont_mul_lazy(p, V, mo, ninv);
         }
-        return w;
+        w = mont_mul_lazy(w, p, mo, ninv);
     }
-    return mont_mul_lazy(w, pow2_uniform<FOLD>(4u * gp, r1, mo, ninv, two_mo), mo, ninv);
+    return w;
 }
 
 // F = floor((16^e mod m) 2^64 / m) for e >= 1 from w (0 for w = 0).
@@ -649,8 +633,8 @@ __device__ __forceinline__ uint64_t bbp_moduli_frac(uint32_t w, uint32_t T3, uin
 // p(l) = 8 (l & 7) + (l >> 3) holds (acc_l).
 template <bool FOLD>
 __device__ __forceinline__ void bbp_moduli_kblock(uint32_t k0, uint32_t j, uint32_t sj, uint32_t key, uint32_t nv,
-                                                  uint32_t dmax, uint32_t lane, uint32_t* __restrict__ lo,
-                                                  uint32_t* __restrict__ hi, uint64_t& acc_l) {
+                                                  uint32_t dmax, uint32_t lane, uint64_t* __restrict__ buf,
+                                                  uint64_t& acc_l) {
     const uint32_t k = k0 + lane;
     const uint32_t mi = 8u * k + j;
     const bool live = k <= dmax && mi != 1u;
@@ -663,10 +647,17 @@ __device__ __forceinline__ void bbp_moduli_kblock(uint32_t k0, uint32_t j, uint3
     uint32_t r1 = 0u - q * mo;                       // 2^32 mod mo in [0, 2 mo)
     if ((int32_t)r1 < 0) r1 += mo;
     const uint32_t two_mo = mo << 1;
-    uint32_t T[16];                                  // forms of 16^(2^t)
-    T[0] = dbl_red(dbl_red(dbl_red(dbl_red(r1, two_mo), two_mo), two_mo), two_mo);
+    uint32_t U0[16], U1[16];                         // forms of 16^n and 16^(16 n)
+    U0[0] = r1;
+    U0[1] = dbl_red(dbl_red(dbl_red(dbl_red(r1, two_mo), two_mo), two_mo), two_mo);
 #pragma unroll
-    for (int t = 1; t < 16; ++t) T[t] = mont_mul_lazy(T[t - 1], T[t - 1], mo, ninv);
+    for (int t = 2; t < 16; ++t) U0[t] = mont_mul_lazy(U0[t - 1], U0[1], mo, ninv);
+    U1[0] = r1;
+    U1[1] = mont_mul_lazy(U0[15], U0[1], mo, ninv);
+#pragma unroll
+    for (int t = 2; t < 16; ++t) U1[t] = mont_mul_lazy(U1[t - 1], U1[1], mo, ninv);
+    const uint32_t V = mont_mul_lazy(U1[15], U1[1], mo, ninv);     // 16^256
+    const uint32_t T8 = U0[8];                       // 16^8 = 2^32: rem2 from rem1
     uint32_t h = r1;                                 // form of 2^-s
     if (sj >= 1u) h = half_mod(h, mo);
     if (sj >= 2u) h = half_mod(h, mo);
@@ -677,48 +668,45 @@ __device__ __forceinline__ void bbp_moduli_kblock(uint32_t k0, uint32_t j, uint3
     const uint64_t above = __ballot(lane < nv && key > kmax);
     const uint32_t i1 = above ? (uint32_t)__builtin_ctzll(above) : nv;
     // slots of the first batch before it hold zero
-    for (uint32_t sl = first & ~7u; sl < first; ++sl) { lo[(sl & 7u) * 72u + lane] = 0u; hi[(sl & 7u) * 72u + lane] = 0u; }
+    for (uint32_t sl = first & ~7u; sl < first; ++sl) buf[(sl & 7u) * 72u + lane] = 0;
     uint32_t w = 0;
     bool started = false;
     uint32_t dprev = 0;
     uint32_t i = first;
     for (; i < i1; ++i) {                            // the triangle
         const uint32_t di = (uint32_t)__builtin_amdgcn_readlane((int)key, (int)i);
-        if (i > first && di != dprev) w = bbp_gap_mul<FOLD>(w, di - dprev, T, r1, mo, ninv, two_mo);
+        if (i > first && di != dprev) w = bbp_gap_mul(w, di - dprev, U0, U1, V, mo, ninv);
         dprev = di;
         if (live && !started && k <= di) {
             const uint32_t e = di - k;
             w = e == 0u ? h : mont_mul_lazy(pow2_lane<FOLD>(4u * e, r1, mo, ninv, two_mo), h, mo, ninv);
             started = true;
         }
-        uint64_t F = started ? bbp_moduli_frac(w, T[3], mo, ninv) : 0;
+        uint64_t F = started ? bbp_moduli_frac(w, T8, mo, ninv) : 0;
         if (started && k == di) F = frac64(1u, mi, recip_f64((double)mi));
-        lo[(i & 7u) * 72u + lane] = (uint32_t)F;
-        hi[(i & 7u) * 72u + lane] = (uint32_t)(F >> 32);
-        if ((i & 7u) == 7u) bbp_moduli_flush(lo, hi, i >> 3, lane, acc_l);
+        buf[(i & 7u) * 72u + lane] = F;
+        if ((i & 7u) == 7u) bbp_moduli_flush(buf, i >> 3, lane, acc_l);
     }
     if (i < nv) {
         // position i1: the lanes started in the triangle take the gap, the
         // others start here (e = d - k >= 1)
         const uint32_t di = (uint32_t)__builtin_amdgcn_readlane((int)key, (int)i);
-        if (i > first && di != dprev) w = bbp_gap_mul<FOLD>(w, di - dprev, T, r1, mo, ninv, two_mo);
+        if (i > first && di != dprev) w = bbp_gap_mul(w, di - dprev, U0, U1, V, mo, ninv);
         if (live && !started) w = mont_mul_lazy(pow2_lane<FOLD>(4u * (di - k), r1, mo, ninv, two_mo), h, mo, ninv);
         dprev = di;
         for (;;) {
-            const uint64_t F = bbp_moduli_frac(w, T[3], mo, ninv);
-            lo[(i & 7u) * 72u + lane] = (uint32_t)F;
-            hi[(i & 7u) * 72u + lane] = (uint32_t)(F >> 32);
-            if ((i & 7u) == 7u) bbp_moduli_flush(lo, hi, i >> 3, lane, acc_l);
+            buf[(i & 7u) * 72u + lane] = bbp_moduli_frac(w, T8, mo, ninv);
+            if ((i & 7u) == 7u) bbp_moduli_flush(buf, i >> 3, lane, acc_l);
             if (++i >= nv) break;
             const uint32_t dn = (uint32_t)__builtin_amdgcn_readlane((int)key, (int)i);
-            if (dn != dprev) w = bbp_gap_mul<FOLD>(w, dn - dprev, T, r1, mo, ninv, two_mo);
+            if (dn != dprev) w = bbp_gap_mul(w, dn - dprev, U0, U1, V, mo, ninv);
             dprev = dn;
         }
     }
     // the last batch, when it is partial
     if (nv & 7u) {
-        for (uint32_t z = nv & 7u; z < 8u; ++z) { lo[z * 72u + lane] = 0u; hi[z * 72u + lane] = 0u; }
-        bbp_moduli_flush(lo, hi, (nv - 1u) >> 3, lane, acc_l);
+        for (uint32_t z = nv & 7u; z < 8u; ++z) buf[z * 72u + lane] = 0;
+        bbp_moduli_flush(buf, (nv - 1u) >> 3, lane, acc_l);
     }
 }
 
@@ -729,7 +717,7 @@ __device__ __forceinline__ void bbp_moduli_kblock(uint32_t k0, uint32_t j, uint3
 // position's sum over its k-blocks with one atomic.
 __global__ __launch_bounds__(256) void k_bbp_fx_moduli(const BbpSeries* __restrict__ st, size_t count,
                                                        uint32_t chunks, uint32_t KB, uint64_t* __restrict__ acc) {
-    __shared__ uint32_t sbuf[4][2][8 * 72];
+    __shared__ uint64_t sbuf[4][8 * 72];
     const uint32_t g = blockIdx.x / chunks;
     const uint32_t c = blockIdx.x - g * chunks;
     const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -754,17 +742,16 @@ __global__ __launch_bounds__(256) void k_bbp_fx_moduli(const BbpSeries* __restri
     const uint32_t dmax = (uint32_t)__builtin_amdgcn_readlane((int)key, (int)(nv - 1));
     const uint32_t j = w == 0 ? 1u : w + 3u;              // 1, 4, 5, 6
     const uint32_t sj = j == 4 ? 2u : (j == 6 ? 1u : 0u);
-    uint32_t* lo = sbuf[w][0];
-    uint32_t* hi = sbuf[w][1];
+    uint64_t* buf = sbuf[w];
     uint64_t acc_l = 0;
     const uint32_t kb_end = dmax / 64u + 1u;
     for (uint32_t kb = c * KB; kb < min((c + 1u) * KB, kb_end); ++kb) {
         const uint32_t k0 = kb * 64u;
         // the block's largest modulus, 8 min(k0 + 63, dmax) + 6, below 2^29
         if (min(k0 + 63u, dmax) <= (1u << 26) - 2u)
-            bbp_moduli_kblock<true>(k0, j, sj, key, nv, dmax, lane, lo, hi, acc_l);
+            bbp_moduli_kblock<true>(k0, j, sj, key, nv, dmax, lane, buf, acc_l);
         else
-            bbp_moduli_kblock<false>(k0, j, sj, key, nv, dmax, lane, lo, hi, acc_l);
+            bbp_moduli_kblock<false>(k0, j, sj, key, nv, dmax, lane, buf, acc_l);
     }
     const uint32_t p = 8u * (lane & 7u) + (lane >> 3);
     const uint32_t orig = (uint32_t)__shfl((int)idx, (int)p);

@@ -286,10 +286,11 @@ def test_frac64_uniform_form():
 def moduli_block_sums(keys, j: int, k0: int, dmax: int, r1_high: bool = False) -> list:
     """k_bbp_fx_moduli's k-block k0 .. k0 + 63 of series j over the sorted
     positions - 1 `keys`, lane by lane as the kernel: the Montgomery set-up,
-    the table T_t = form of 16^(2^t), the form h of 2^-s, each lane started at
-    its first position by its own ladder (folded below 2^29), gaps below 2^16
-    by table products and larger ones by a uniform ladder, the fraction from
-    rem1 = w and rem2 = w T_3 (frac64 for e = 0), and the sum over the 64
+    the tables U0[n] = form of 16^n and U1[n] = form of 16^(16 n) and V = form
+    of 16^256, the form h of 2^-s, each lane started at its first position by
+    its own ladder (folded below 2^29), a gap n0 + 16 n1 + 256 r by U0[n0],
+    U1[n1] and V^r (square-and-multiply), the fraction from rem1 = w and
+    rem2 = w U0[8] (frac64 for e = 0), and the sum over the 64
     lanes of each position mod 2^64. Every lazy value's bound is asserted.
     r1_high takes the Montgomery one as 2^32 mod mo + mo, what a quotient
     estimate one too low gives."""
@@ -328,9 +329,28 @@ def moduli_block_sums(keys, j: int, k0: int, dmax: int, r1_high: bool = False) -
             assert r < 1 << 32
             return r - two_mo if r >= two_mo else r
 
-        T = [dbl(dbl(dbl(dbl(r1))))]
-        for _ in range(15):
-            T.append(mont(T[-1], T[-1]))
+        U0 = [r1, dbl(dbl(dbl(dbl(r1))))]
+        for _ in range(14):
+            U0.append(mont(U0[-1], U0[1]))
+        U1 = [r1, mont(U0[15], U0[1])]
+        for _ in range(14):
+            U1.append(mont(U1[-1], U1[1]))
+        V = mont(U1[15], U1[1])
+
+        def gap_mul(w, gp):
+            n0, n1, r = gp & 15, (gp >> 4) & 15, gp >> 8
+            if n0:
+                w = mont(w, U0[n0])
+            if n1:
+                w = mont(w, U1[n1])
+            if r:
+                p = V
+                for b in range(r.bit_length() - 2, -1, -1):
+                    p = mont(p, p)
+                    if (r >> b) & 1:
+                        p = mont(p, V)
+                w = mont(w, p)
+            return w
         h = r1
         for _ in range(s):
             h = (h + mo) >> 1 if h & 1 else h >> 1
@@ -348,14 +368,8 @@ def moduli_block_sums(keys, j: int, k0: int, dmax: int, r1_high: bool = False) -
         started, w, dprev = False, 0, 0
         for i in range(first, nv):
             di = keys[i]
-            if i > first and di != dprev:
-                gp = di - dprev
-                if gp < 1 << 16:
-                    for t in range(16):
-                        if gp >> t & 1 and started:
-                            w = mont(w, T[t])
-                elif started:
-                    w = mont(w, ladder(4 * gp))
+            if i > first and di != dprev and started:
+                w = gap_mul(w, di - dprev)
             dprev = di
             if not started and k <= di:
                 e = di - k
@@ -366,7 +380,7 @@ def moduli_block_sums(keys, j: int, k0: int, dmax: int, r1_high: bool = False) -
                     F = (1 << 64) // mi
                 else:
                     rem1 = w - mo if w >= mo else w
-                    rem2 = mont(w, T[3])
+                    rem2 = mont(w, U0[8])
                     rem2 = rem2 - mo if rem2 >= mo else rem2
                     F = (((rem1 * ninv) & M32) << 32) | ((rem2 * ninv) & M32)
                 sums[i] = (sums[i] + F) & ((1 << 64) - 1)
@@ -387,13 +401,13 @@ def moduli_block_direct(keys, j: int, k0: int, dmax: int) -> list:
 
 def test_moduli_layout_restated():
     """Small positions (k = 0 and e = 0 terms, m = 4), blocks that start
-    inside the group (lanes started late), gaps below 2^8, below 2^16 and
-    above, repeats, and k-blocks on both sides of the folded ladder's bound (2^26 - 2)
+    inside the group (lanes started late), gaps of one and two nibbles and
+    longer ones (the power of 16^256), repeats, and k-blocks on both sides of the folded ladder's bound (2^26 - 2)
     and at the layout's largest position - 1 (2^27 - 2)."""
     cases = [
         (list(range(0, 70)), 0),
         (list(range(0, 70)), 64),
-        ([5, 5, 9, 300, 301, 900, 4000, 4000, 4255, 70000], 0),
+        ([5, 5, 9, 300, 301, 900, 4000, 4000, 4255, 70000, 70256, 70272, 70288 + 3 * 2 ** 20], 0),
         ([5, 5, 9, 300, 301, 900, 4000, 4000, 4255, 70000], 4032),
         ([(1 << 26) - 40 + 37 * i for i in range(12)], (1 << 26) - 64),
         ([(1 << 26) - 40 + 37 * i for i in range(12)], (1 << 26) - 40),
