@@ -595,15 +595,16 @@ __device__ __forceinline__ void bbp_moduli_flush(const uint64_t* __restrict__ bu
     bbp_wave_sync();
 }
 
-// w *= form of 16^gp for a wave-uniform gap gp > 0: gp = n0 + 16 n1 + 256 r
+// w *= form of 16^gp for a wave-uniform gap gp: gp = n0 + 16 n1 + 256 r
 // takes the lane's entries U0[n0] (form of 16^n0) and U1[n1] (16^(16 n1)),
-// read by the uniform index, and V^r (V = form of 16^256) by a uniform
+// read by the uniform index (entry 0 is the form of one: no test, most gaps
+// have both nibbles), and V^r (V = form of 16^256) by a uniform
 // square-and-multiply.
 __device__ __forceinline__ uint32_t bbp_gap_mul(uint32_t w, uint32_t gp, const uint32_t* U0, const uint32_t* U1,
                                                 uint32_t V, uint32_t mo, uint32_t ninv) {
     const uint32_t n0 = gp & 15u, n1 = (gp >> 4) & 15u, r = gp >> 8;
-    if (n0) w = mont_mul_lazy(w, U0[n0], mo, ninv);
-    if (n1) w = mont_mul_lazy(w, U1[n1], mo, ninv);
+    w = mont_mul_lazy(w, U0[n0], mo, ninv);
+    w = mont_mul_lazy(w, U1[n1], mo, ninv);
     if (r) {
         uint32_t p = V;
         for (int b = 30 - __clz(r); b >= 0; --b) {
@@ -699,7 +700,7 @@ __device__ __forceinline__ void bbp_moduli_kblock(uint32_t k0, uint32_t j, uint3
             if ((i & 7u) == 7u) bbp_moduli_flush(buf, i >> 3, lane, acc_l);
             if (++i >= nv) break;
             const uint32_t dn = (uint32_t)__builtin_amdgcn_readlane((int)key, (int)i);
-            if (dn != dprev) w = bbp_gap_mul(w, dn - dprev, U0, U1, V, mo, ninv);
+            w = bbp_gap_mul(w, dn - dprev, U0, U1, V, mo, ninv);
             dprev = dn;
         }
     }

@@ -339,10 +339,7 @@ def moduli_block_sums(keys, j: int, k0: int, dmax: int, r1_high: bool = False) -
 
         def gap_mul(w, gp):
             n0, n1, r = gp & 15, (gp >> 4) & 15, gp >> 8
-            if n0:
-                w = mont(w, U0[n0])
-            if n1:
-                w = mont(w, U1[n1])
+            w = mont(mont(w, U0[n0]), U1[n1])       # entry 0 is the form of one
             if r:
                 p = V
                 for b in range(r.bit_length() - 2, -1, -1):
@@ -368,7 +365,7 @@ def moduli_block_sums(keys, j: int, k0: int, dmax: int, r1_high: bool = False) -
         started, w, dprev = False, 0, 0
         for i in range(first, nv):
             di = keys[i]
-            if i > first and di != dprev and started:
+            if i > first and started:
                 w = gap_mul(w, di - dprev)
             dprev = di
             if not started and k <= di:
