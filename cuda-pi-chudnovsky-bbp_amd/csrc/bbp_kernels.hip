@@ -542,6 +542,11 @@ __global__ __launch_bounds__(256) void k_bbp_fx_lanes(const BbpSeries* __restric
 // up to 2^27 - 1), so every lazy product of residues below 2 mo is below
 // 4 mo^2 < 2^32 mo and reduces to [0, 2 mo).
 constexpr uint32_t BBP_MODULI_DMAX = (uint32_t)BBP_MODULI_DMAX_HOST;   // largest d = position - 1 this layout takes
+// Residues (blocks of 64 terms) per lane. Two share each position step's
+// scalar work and staging but take 152 VGPRs (three waves per SIMD): C4
+// chunk 32 took 2.94 s against 2.61 s with one (5.30 s held to four waves,
+// where the tables spill).
+constexpr uint32_t BBP_MODULI_NR = 1;
 
 __device__ __forceinline__ uint32_t dbl_red(uint32_t r, uint32_t two_mo) {
     r <<= 1;
@@ -616,91 +621,134 @@ __device__ __forceinline__ uint32_t bbp_gap_mul(uint32_t w, uint32_t gp, const u
     return w;
 }
 
-// F = floor((16^e mod m) 2^64 / m) for e >= 1 from w (0 for w = 0).
+// F = floor((16^e mod m) 2^64 / m) for e >= 1 from w (0 for w = 0). For
+// x < 2 mo, min(x, x - mo) (unsigned) is x mod mo: two VALU.
 __device__ __forceinline__ uint64_t bbp_moduli_frac(uint32_t w, uint32_t T3, uint32_t mo, uint32_t ninv) {
-    const uint32_t rem1 = w >= mo ? w - mo : w;
-    uint32_t rem2 = mont_mul_lazy(w, T3, mo, ninv);
-    if (rem2 >= mo) rem2 -= mo;
+    const uint32_t rem1 = min(w, w - mo);
+    const uint32_t t = mont_mul_lazy(w, T3, mo, ninv);
+    const uint32_t rem2 = min(t, t - mo);
     return ((uint64_t)(rem1 * ninv) << 32) | (uint64_t)(rem2 * ninv);
 }
 
-// One k-block k0 .. k0 + 63 of a wavefront over its sorted positions
-// key[0 .. nv) (key held by lane i), with FOLD for every modulus below 2^29.
-// Positions up to the block's largest live k (the triangle, only in blocks
-// that reach into the group) start lanes one by one and hold the e = 0
-// terms; from the first position above it every live lane has started and
-// e >= 1, and a lane with no terms keeps w = 0 (so F = 0) through the gap
-// products. Each position's sum over the lanes goes to the lane
-// p(l) = 8 (l & 7) + (l >> 3) holds (acc_l).
-template <bool FOLD>
-__device__ __forceinline__ void bbp_moduli_kblock(uint32_t k0, uint32_t j, uint32_t sj, uint32_t key, uint32_t nv,
-                                                  uint32_t dmax, uint32_t lane, uint64_t* __restrict__ buf,
-                                                  uint64_t& acc_l) {
-    const uint32_t k = k0 + lane;
-    const uint32_t mi = 8u * k + j;
-    const bool live = k <= dmax && mi != 1u;
-    const uint32_t mo = live ? mi >> sj : 3u;
+// One modulus of a lane: its set-up and running residue. Its tables U0[n]
+// (form of 16^n) and U1[n] (16^(16 n)) are plain local arrays beside it: an
+// array member read at a run-time index would keep the whole struct in
+// scratch memory.
+struct ModLane {
+    uint32_t k, mi, mo, ninv, r1, two_mo, V, T8, h, w;
+    bool live, started;
+};
+
+__device__ __forceinline__ void mod_lane_init(ModLane& L, uint32_t* U0, uint32_t* U1, uint32_t k, uint32_t j,
+                                              uint32_t sj, uint32_t dmax) {
+    L.k = k;
+    L.mi = 8u * k + j;
+    L.live = k <= dmax && L.mi != 1u;
+    const uint32_t mo = L.live ? L.mi >> sj : 3u;
+    L.mo = mo;
     uint32_t inv = (3u * mo) ^ 2u;
     for (int i = 0; i < 3; ++i) inv *= 2u - mo * inv;
-    const uint32_t ninv = 0u - inv;
+    L.ninv = 0u - inv;
     const uint32_t q = mo >= 2048u ? (uint32_t)(4294967296.0f * __builtin_amdgcn_rcpf((float)mo))
                                    : (uint32_t)(0x100000000ull / mo);
     uint32_t r1 = 0u - q * mo;                       // 2^32 mod mo in [0, 2 mo)
     if ((int32_t)r1 < 0) r1 += mo;
-    const uint32_t two_mo = mo << 1;
-    uint32_t U0[16], U1[16];                         // forms of 16^n and 16^(16 n)
+    L.r1 = r1;
+    L.two_mo = mo << 1;
     U0[0] = r1;
-    U0[1] = dbl_red(dbl_red(dbl_red(dbl_red(r1, two_mo), two_mo), two_mo), two_mo);
+    U0[1] = dbl_red(dbl_red(dbl_red(dbl_red(r1, L.two_mo), L.two_mo), L.two_mo), L.two_mo);
 #pragma unroll
-    for (int t = 2; t < 16; ++t) U0[t] = mont_mul_lazy(U0[t - 1], U0[1], mo, ninv);
+    for (int t = 2; t < 16; ++t) U0[t] = mont_mul_lazy(U0[t - 1], U0[1], mo, L.ninv);
     U1[0] = r1;
-    U1[1] = mont_mul_lazy(U0[15], U0[1], mo, ninv);
+    U1[1] = mont_mul_lazy(U0[15], U0[1], mo, L.ninv);
 #pragma unroll
-    for (int t = 2; t < 16; ++t) U1[t] = mont_mul_lazy(U1[t - 1], U1[1], mo, ninv);
-    const uint32_t V = mont_mul_lazy(U1[15], U1[1], mo, ninv);     // 16^256
-    const uint32_t T8 = U0[8];                       // 16^8 = 2^32: rem2 from rem1
+    for (int t = 2; t < 16; ++t) U1[t] = mont_mul_lazy(U1[t - 1], U1[1], mo, L.ninv);
+    L.V = mont_mul_lazy(U1[15], U1[1], mo, L.ninv);  // 16^256
+    L.T8 = U0[8];                                    // 16^8 = 2^32: rem2 from rem1
     uint32_t h = r1;                                 // form of 2^-s
     if (sj >= 1u) h = half_mod(h, mo);
     if (sj >= 2u) h = half_mod(h, mo);
-    // the first position this block reaches (the smallest d >= k0), and the
-    // first above its largest live k
-    const uint32_t kmax = min(k0 + 63u, dmax);
+    L.h = h;
+    L.w = 0;
+    L.started = false;
+}
+
+// w = form of 16^(d - k) 2^-s for a lane reaching position d >= k.
+template <bool FOLD>
+__device__ __forceinline__ void mod_lane_start(ModLane& L, uint32_t d) {
+    const uint32_t e = d - L.k;
+    L.w = e == 0u ? L.h : mont_mul_lazy(pow2_lane<FOLD>(4u * e, L.r1, L.mo, L.ninv, L.two_mo), L.h, L.mo, L.ninv);
+    L.started = true;
+}
+
+// NR blocks of 64 terms at once, k0 .. k0 + 64 NR - 1 (lane l holds
+// k0 + l + 64 r, r < NR), of a wavefront over its sorted positions
+// key[0 .. nv) (key held by lane i), with FOLD for every modulus below 2^29.
+// The NR residues of a lane share each position step (the gap, its scalar
+// work, the staging and the sum), and their fractions are added before the
+// staging. Positions up to the largest live k (the triangle, only in blocks
+// that reach into the group) start residues one by one and hold the e = 0
+// terms; from the first position above it every live residue has started
+// and e >= 1, and a residue with no terms keeps w = 0 (so F = 0) through the
+// gap products. Each position's sum over the lanes goes to the lane
+// p(l) = 8 (l & 7) + (l >> 3) holds (acc_l).
+template <bool FOLD, int NR>
+__device__ __forceinline__ void bbp_moduli_kblock(uint32_t k0, uint32_t j, uint32_t sj, uint32_t key, uint32_t nv,
+                                                  uint32_t dmax, uint32_t lane, uint64_t* __restrict__ buf,
+                                                  uint64_t& acc_l) {
+    static_assert(NR == 1 || NR == 2, "one or two residues per lane");
+    // two named lanes (not an array of them), so that each table stays a
+    // register array read at a uniform index
+    ModLane La, Lb;
+    uint32_t U0a[16], U1a[16], U0b[16], U1b[16];
+    auto each = [&](auto&& f) {
+        f(La, U0a, U1a);
+        if constexpr (NR == 2) f(Lb, U0b, U1b);
+    };
+    mod_lane_init(La, U0a, U1a, k0 + lane, j, sj, dmax);
+    if constexpr (NR == 2) mod_lane_init(Lb, U0b, U1b, k0 + lane + 64u, j, sj, dmax);
+    // the first position these blocks reach (the smallest d >= k0), and the
+    // first above their largest live k
+    const uint32_t kmax = min(k0 + 64u * NR - 1u, dmax);
     const uint32_t first = (uint32_t)__builtin_ctzll(__ballot(lane < nv && key >= k0));
     const uint64_t above = __ballot(lane < nv && key > kmax);
     const uint32_t i1 = above ? (uint32_t)__builtin_ctzll(above) : nv;
     // slots of the first batch before it hold zero
     for (uint32_t sl = first & ~7u; sl < first; ++sl) buf[(sl & 7u) * 72u + lane] = 0;
-    uint32_t w = 0;
-    bool started = false;
     uint32_t dprev = 0;
     uint32_t i = first;
     for (; i < i1; ++i) {                            // the triangle
         const uint32_t di = (uint32_t)__builtin_amdgcn_readlane((int)key, (int)i);
-        if (i > first && di != dprev) w = bbp_gap_mul(w, di - dprev, U0, U1, V, mo, ninv);
+        uint64_t F = 0;
+        each([&](ModLane& M, const uint32_t* U0, const uint32_t* U1) {
+            if (i > first && di != dprev) M.w = bbp_gap_mul(M.w, di - dprev, U0, U1, M.V, M.mo, M.ninv);
+            if (M.live && !M.started && M.k <= di) mod_lane_start<FOLD>(M, di);
+            if (M.started)
+                F += M.k == di ? frac64(1u, M.mi, recip_f64((double)M.mi)) : bbp_moduli_frac(M.w, M.T8, M.mo, M.ninv);
+        });
         dprev = di;
-        if (live && !started && k <= di) {
-            const uint32_t e = di - k;
-            w = e == 0u ? h : mont_mul_lazy(pow2_lane<FOLD>(4u * e, r1, mo, ninv, two_mo), h, mo, ninv);
-            started = true;
-        }
-        uint64_t F = started ? bbp_moduli_frac(w, T8, mo, ninv) : 0;
-        if (started && k == di) F = frac64(1u, mi, recip_f64((double)mi));
         buf[(i & 7u) * 72u + lane] = F;
         if ((i & 7u) == 7u) bbp_moduli_flush(buf, i >> 3, lane, acc_l);
     }
     if (i < nv) {
-        // position i1: the lanes started in the triangle take the gap, the
-        // others start here (e = d - k >= 1)
+        // position i1: the residues started in the triangle take the gap,
+        // the others start here (e = d - k >= 1)
         const uint32_t di = (uint32_t)__builtin_amdgcn_readlane((int)key, (int)i);
-        if (i > first && di != dprev) w = bbp_gap_mul(w, di - dprev, U0, U1, V, mo, ninv);
-        if (live && !started) w = mont_mul_lazy(pow2_lane<FOLD>(4u * (di - k), r1, mo, ninv, two_mo), h, mo, ninv);
+        each([&](ModLane& M, const uint32_t* U0, const uint32_t* U1) {
+            if (i > first && di != dprev) M.w = bbp_gap_mul(M.w, di - dprev, U0, U1, M.V, M.mo, M.ninv);
+            if (M.live && !M.started) mod_lane_start<FOLD>(M, di);
+        });
         dprev = di;
         for (;;) {
-            buf[(i & 7u) * 72u + lane] = bbp_moduli_frac(w, T8, mo, ninv);
+            uint64_t F = 0;
+            each([&](ModLane& M, const uint32_t*, const uint32_t*) { F += bbp_moduli_frac(M.w, M.T8, M.mo, M.ninv); });
+            buf[(i & 7u) * 72u + lane] = F;
             if ((i & 7u) == 7u) bbp_moduli_flush(buf, i >> 3, lane, acc_l);
             if (++i >= nv) break;
             const uint32_t dn = (uint32_t)__builtin_amdgcn_readlane((int)key, (int)i);
-            w = bbp_gap_mul(w, dn - dprev, U0, U1, V, mo, ninv);
+            each([&](ModLane& M, const uint32_t* U0, const uint32_t* U1) {
+                M.w = bbp_gap_mul(M.w, dn - dprev, U0, U1, M.V, M.mo, M.ninv);
+            });
             dprev = dn;
         }
     }
@@ -745,14 +793,15 @@ __global__ __launch_bounds__(256) void k_bbp_fx_moduli(const BbpSeries* __restri
     const uint32_t sj = j == 4 ? 2u : (j == 6 ? 1u : 0u);
     uint64_t* buf = sbuf[w];
     uint64_t acc_l = 0;
+    // KB is a multiple of BBP_MODULI_NR: each step takes NR blocks of the chunk
     const uint32_t kb_end = dmax / 64u + 1u;
-    for (uint32_t kb = c * KB; kb < min((c + 1u) * KB, kb_end); ++kb) {
+    for (uint32_t kb = c * KB; kb < min((c + 1u) * KB, kb_end); kb += BBP_MODULI_NR) {
         const uint32_t k0 = kb * 64u;
-        // the block's largest modulus, 8 min(k0 + 63, dmax) + 6, below 2^29
-        if (min(k0 + 63u, dmax) <= (1u << 26) - 2u)
-            bbp_moduli_kblock<true>(k0, j, sj, key, nv, dmax, lane, buf, acc_l);
+        // the largest modulus, 8 min(k0 + 64 NR - 1, dmax) + 6, below 2^29
+        if (min(k0 + 64u * BBP_MODULI_NR - 1u, dmax) <= (1u << 26) - 2u)
+            bbp_moduli_kblock<true, BBP_MODULI_NR>(k0, j, sj, key, nv, dmax, lane, buf, acc_l);
         else
-            bbp_moduli_kblock<false>(k0, j, sj, key, nv, dmax, lane, buf, acc_l);
+            bbp_moduli_kblock<false, BBP_MODULI_NR>(k0, j, sj, key, nv, dmax, lane, buf, acc_l);
     }
     const uint32_t p = 8u * (lane & 7u) + (lane >> 3);
     const uint32_t orig = (uint32_t)__shfl((int)idx, (int)p);
@@ -960,7 +1009,7 @@ hipError_t launch_bbp_fx(const uint64_t* d_pos, uint64_t start, size_t count, ui
         const uint64_t kblocks = dmax / 64 + 1;
         const uint64_t want = std::max<uint64_t>(((uint64_t)1 << 20) / groups, 1);
         const uint64_t chunks = std::min<uint64_t>(want, kblocks);
-        const uint64_t KB = (kblocks + chunks - 1) / chunks;
+        const uint64_t KB = ((kblocks + chunks - 1) / chunks + BBP_MODULI_NR - 1) / BBP_MODULI_NR * BBP_MODULI_NR;
         BBP_LAUNCH("k_bbp_fx_moduli", s, k_bbp_fx_moduli, dim3((unsigned)(groups * chunks)), dim3(256), 0, s,
                    (const BbpSeries*)st, count, (uint32_t)chunks, (uint32_t)KB, acc);
     } else if (tk != BBP_TK_TERMS && dmax < ((uint64_t)1 << 28)) {          // 32-bit term indices
