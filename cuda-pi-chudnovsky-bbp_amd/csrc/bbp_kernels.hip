@@ -530,15 +530,15 @@ __global__ __launch_bounds__(256) void k_bbp_fx_lanes(const BbpSeries* __restric
 // group of 64 in ascending order. With w the Montgomery form of
 // 16^e 2^-s mod mo (e = d - k, m = 2^s mo), w is rem1 = 2^(4e - s + 32) mod mo
 // of frac64_mont_wave, and Montgomery multiplying it by the form of 2^32
-// (16^8) gives rem2; the fraction is then four VALU. Going from position d
-// to the next, d', multiplies w by 16^(d' - d): the gap is wave-uniform, so
-// its two low nibbles index the lane's tables of 16^n and 16^(16 n)
-// (n < 16) -- at most two products below a gap of 2^8 -- and its higher
-// part takes a uniform power of 16^256. Each lane's own ladder (the long
-// one, e ~ d) runs once, at the first position it reaches, and the set-up
-// once per 64 terms. The
-// term with e = 0 (k = d) is the one integer 16^0 = 1 that the 2^-s form
-// does not give, and goes through frac64. Moduli stay below 2^30 (positions
+// (16^8) gives rem2; the fraction then costs that product, two unsigned
+// minima and two multiplies (nine VALU). Going from position d to the next,
+// d', multiplies w by 16^(d' - d): the gap is wave-uniform, so its two low
+// nibbles index the lane's tables of 16^n and 16^(16 n) (n < 16) -- two
+// products below a gap of 2^8 -- and its higher part takes a uniform power
+// of 16^256. Each lane's own ladder (the long one, e ~ d) runs once, at the
+// first position it reaches, and the set-up once per 64 terms. The term
+// with e = 0 (k = d) is the one integer 16^0 = 1 that the 2^-s form does
+// not give, and goes through frac64. Moduli stay below 2^30 (positions
 // up to 2^27 - 1), so every lazy product of residues below 2 mo is below
 // 4 mo^2 < 2^32 mo and reduces to [0, 2 mo).
 constexpr uint32_t BBP_MODULI_DMAX = (uint32_t)BBP_MODULI_DMAX_HOST;   // largest d = position - 1 this layout takes
