@@ -26,6 +26,8 @@
 #include "pdh_internal.h"
 #include <algorithm>
 #include <cmath>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/iterator/counting_iterator.hpp>
 
 namespace pdh {
 
@@ -1057,6 +1059,28 @@ hipError_t launch_bbp_chunked(const uint64_t* d_pos, uint64_t start, size_t coun
     }
     BBP_LAUNCH("k_bbp_finish", s, k_bbp_finish, dim3((unsigned)((S + 255) / 256)), dim3(256), 0, s,
                (const BbpSeries*)st, count, d_out);
+    return hipSuccess;
+}
+
+// Positions and their indices sorted by the low 27 bits of the position
+// (every position the moduli kernel takes lies below 2^27; others land
+// anywhere, which changes the cost and never a digit). With temp null,
+// *temp_bytes receives the scratch size rocPRIM's radix sort needs.
+hipError_t bbp_sort_positions(const uint64_t* d_pos, size_t n, uint64_t* d_sorted, uint32_t* d_perm, void* temp,
+                              size_t* temp_bytes, hipStream_t s) {
+    return rocprim::radix_sort_pairs(temp, *temp_bytes, d_pos, d_sorted, rocprim::counting_iterator<uint32_t>(0u),
+                                     d_perm, (unsigned)n, 0u, 27u, s);
+}
+
+__global__ __launch_bounds__(256) void k_bbp_scatter(const uint8_t* __restrict__ in, const uint32_t* __restrict__ perm,
+                                                     size_t n, uint8_t* __restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[perm[i]] = in[i];
+}
+
+hipError_t launch_bbp_scatter(const uint8_t* d_in, const uint32_t* d_perm, size_t n, uint8_t* d_out, hipStream_t s) {
+    BBP_LAUNCH("k_bbp_scatter", s, k_bbp_scatter, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d_in, d_perm, n,
+               d_out);
     return hipSuccess;
 }
 

@@ -1084,10 +1084,32 @@ int pdh_bbp_positions_dev(const uint64_t* d_positions, size_t n, uint8_t* d_valu
             if (e == hipSuccess) e = hipStreamSynchronize(st);
             if (e != hipSuccess) return fail_hip(e, "bbp position scan");
         }
+        const double share = sc[2] ? (double)sc[1] / (double)sc[2] : 0.0;
         std::vector<uint8_t> tmp(n);
-        rc = bbp_few(cfg, d_positions, nullptr, 0, n, sc[0], d_values, tmp.data(), st, false,
-                     sc[2] ? (double)sc[1] / (double)sc[2] : 0.0);
-        if (rc) return rc;
+        if (n > 64 && cfg.exact != 1 && bbp_term_kernel(cfg, share, sc[0]) == BBP_TK_MODULI) {
+            // moduli in lanes steps a lane from one sorted position of its
+            // group to the next at a cost that grows with the gap: sort the
+            // positions on the device, compute in that order, scatter back
+            CallScratch scr(st);
+            size_t tb = 0;
+            if ((e = bbp_sort_positions(d_positions, n, nullptr, nullptr, nullptr, &tb, st)) != hipSuccess)
+                return fail_hip(e, "bbp position sort");
+            uint64_t* dsorted = nullptr;
+            uint32_t* dperm = nullptr;
+            uint8_t* dvals = nullptr;
+            void* dtemp = nullptr;
+            if ((rc = scr.get(&dsorted, n * 8)) || (rc = scr.get(&dperm, n * 4)) || (rc = scr.get(&dvals, n)) ||
+                (rc = scr.get(&dtemp, tb ? tb : 1)))
+                return rc;
+            if ((e = bbp_sort_positions(d_positions, n, dsorted, dperm, dtemp, &tb, st)) != hipSuccess)
+                return fail_hip(e, "bbp position sort");
+            if ((rc = bbp_few(cfg, dsorted, nullptr, 0, n, sc[0], dvals, tmp.data(), st, false, share))) return rc;
+            if ((e = launch_bbp_scatter(dvals, dperm, n, d_values, st)) != hipSuccess)
+                return fail_hip(e, "bbp scatter");
+        } else {
+            if ((rc = bbp_few(cfg, d_positions, nullptr, 0, n, sc[0], d_values, tmp.data(), st, false, share)))
+                return rc;
+        }
         for (size_t i = 0; i < n; ++i)
             if (tmp[i] == 0xFF) return fail(PDH_E_ARG, "BBP position outside [1, 2^45] (its value is 0xFF)");
         return PDH_OK;

@@ -148,6 +148,13 @@ constexpr uint64_t BBP_MODULI_DMAX_HOST = ((uint64_t)1 << 27) - 2;   // largest 
 hipError_t launch_bbp_chunked(const uint64_t* d_pos, uint64_t start, size_t count, uint64_t dmax,
                               uint8_t* d_out, void* ws, hipStream_t s);
 hipError_t launch_bbp_group_scan(const uint64_t* d_pos, size_t count, uint64_t* d_out3, hipStream_t s);
+// Device sort of positions (by their low 27 bits) with their indices, and
+// the scatter of values computed in that order back to the caller's order
+// (out[perm[i]] = in[i]). bbp_sort_positions with temp null only sets
+// *temp_bytes.
+hipError_t bbp_sort_positions(const uint64_t* d_pos, size_t n, uint64_t* d_sorted, uint32_t* d_perm, void* temp,
+                              size_t* temp_bytes, hipStream_t s);
+hipError_t launch_bbp_scatter(const uint8_t* d_in, const uint32_t* d_perm, size_t n, uint8_t* d_out, hipStream_t s);
 // Measured issue rate of v_mad_u64_u32 (lane operations per second, whole GPU).
 hipError_t probe_mad_rate(double* lane_ops_per_s);
 size_t bbp_few_ws_bytes(size_t count);

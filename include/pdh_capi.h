@@ -208,11 +208,11 @@ int pdh_bbp_positions(const uint64_t* positions, size_t n, uint8_t* values);
  * at 10^8) is recomputed with the reference's FP64 fold order. The call
  * synchronises `stream` twice: after a scan of the positions (the largest,
  * and how close each group of 64 lies, which picks the term kernel) and to
- * find the undecided positions. Positions are taken in groups of 64
- * consecutive entries (each group sorted on the device), and up to 2^27 - 1
- * a group's cost falls with the gaps between its sorted positions: callers
- * with many positions should pass them sorted (pdh_bbp_positions sorts host
- * positions itself); any order gives the same digits. Positions must lie in
+ * find the undecided positions. Up to position 2^27 - 1 more than 64
+ * positions are sorted on the device first (rocPRIM radix sort) and the
+ * digits scattered back to the caller's order, since the term kernel's cost
+ * falls with the gaps between the sorted positions of each group of 64; any
+ * order gives the same digits at the same cost. Positions must lie in
  * [1, 2^45] (every modulus 8k + j below 2^48, the range where the
  * reference's bbp_mulmod is exact, include/bbp_kernel.h:63-83); the host
  * entry points reject others with PDH_E_ARG before any device work, and this

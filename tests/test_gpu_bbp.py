@@ -271,7 +271,8 @@ def test_layout_choice(pdh, golden, layout):
 
 def test_moduli_layout_edges(pdh, golden, layout):
     """Moduli in lanes against terms in lanes, position by position, where
-    its cases differ: groups in no order with repeats (the kernel sorts each
+    its cases differ: positions in no order with repeats (more than 64 are
+    sorted on the device and the digits scattered back; the kernel sorts each
     group of 64), positions 1 and 2 (the terms k = 0 and e = 0, m = 4 among
     them), ragged group and batch sizes, gaps below and above 2^8, moduli on
     both sides of 2^29 (the folded ladder's bound) and the largest position
@@ -305,6 +306,9 @@ def test_moduli_layout_edges(pdh, golden, layout):
                 want = got
             else:
                 assert ("k_bbp_fx_moduli" in names) == (name != "past the layout"), (name, names)
+                # more than one group of 64 in moduli in lanes: sorted on the device, scattered back
+                sorted_path = len(positions) > 64 and name != "past the layout"
+                assert ("k_bbp_scatter" in names) == sorted_path, (name, names)
                 bad = np.nonzero(got != want)[0]
                 assert not len(bad), f"{name}: {len(bad)} differ, first position {positions[bad[0]]}"
         print(f"{name}: {len(positions)} positions equal", flush=True)
