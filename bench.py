@@ -356,11 +356,11 @@ def bbp_sampled_legs(pdh, torch, W, golden, stream, dev, out, big):
                                  "checked_vs_fixture": len(cch),
                                  "bit_exact": all(a == b for a, b in cch)}
     # the bound is VALU issue: the term kernel's VALU wave-instructions per
-    # dispatch (rocprofv3 SQ counters over this same workload: its first
-    # dispatch is the 64 sampled positions, its second the 512,
+    # dispatch (rocprofv3 SQ counters over this same workload: terms in lanes
+    # for the 64 sampled positions, moduli in lanes for the 512,
     # profiles/pmc_sq_bbp.txt), 64 lane operations each, over the leg's
     # measured time, against the VALU issue peak
-    for entry, leg in (("k_bbp_fx_moduli [1]", "bbp_sampled_64"), ("k_bbp_fx_moduli [2]", "bbp_sampled_512")):
+    for entry, leg in (("k_bbp_fx_terms", "bbp_sampled_64"), ("k_bbp_fx_moduli", "bbp_sampled_512")):
         v = read_pmc("pmc_sq_bbp.txt", entry, ("SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_WAVE_CYCLES", "SQ_WAIT_ANY"))
         if v and leg in out:
             lane_ops = v["SQ_INSTS_VALU"] * 64

@@ -731,13 +731,18 @@ double lane_share(const uint64_t* h_pos, uint64_t start, uint64_t stride, size_t
 // in [10^6, 10^8], share 0.89, 0.47 against 0.80 s), so it pays from a
 // share of about 0.6 up (64 such positions have 0.52).
 // Moduli in lanes, where it applies (largest position up to 2^27 - 1), is
-// the default: measured against the better of the other two it takes 12.7
-// against 23.6 ms on 1..65,536, 0.33 against 0.44 s on 512 sorted positions
-// in [10^6, 10^8], 3.0 against 4.8 s on 4,096 unsorted ones and the same
-// 62-66 ms on 64 unsorted ones. pdh_bbp_set_layout 1 and 2 force the
-// others, 3 forces it (the default rule where it does not apply).
-int bbp_term_kernel(const BbpCfg& cfg, double share, uint64_t dmax) {
-    if ((cfg.layout == 0 || cfg.layout == 3) && dmax <= BBP_MODULI_DMAX_HOST) return BBP_TK_MODULI;
+// the default for more than 64 positions: measured against the better of
+// the other two it takes 10.5 against 23.6 ms on 1..65,536, 0.28 against
+// 0.44 s on 512 sorted positions in [10^6, 10^8] and 1.7 against 4.7 s on
+// 4,096 unsorted ones (sorted on the device first). Its cost has a part set
+// by the largest position alone (every block of 64 terms sets up its lanes
+// and runs their long ladders however few positions its group holds), so
+// for up to 64 positions the other two serve: 1 position near 5*10^7 takes
+// 1.0 against 33 ms, 64 spread ones 62 against 70 ms. pdh_bbp_set_layout 1
+// and 2 force the others, 3 forces it (the default rule where it does not
+// apply).
+int bbp_term_kernel(const BbpCfg& cfg, double share, uint64_t dmax, size_t count) {
+    if (dmax <= BBP_MODULI_DMAX_HOST && (cfg.layout == 3 || (cfg.layout == 0 && count > 64))) return BBP_TK_MODULI;
     if (cfg.layout == 1) return BBP_TK_TERMS;
     return cfg.layout == 2 || share >= 0.6 ? BBP_TK_LANES : BBP_TK_TERMS;
 }
@@ -754,7 +759,7 @@ int bbp_blocks(const BbpCfg& cfg, uint64_t start, size_t count, uint32_t G, uint
     if (rc) return rc;
     hipError_t e = launch_bbp_fx(nullptr, start, A, start - 1 + (A - 1) * (uint64_t)G, d_out, ws,
                                  cfg.exact == 2, st, G, count,
-                                 bbp_term_kernel(cfg, lane_share(nullptr, start, G, A), start - 1 + (A - 1) * (uint64_t)G));
+                                 bbp_term_kernel(cfg, lane_share(nullptr, start, G, A), start - 1 + (A - 1) * (uint64_t)G, A));
     if (e != hipSuccess) return fail_hip(e, "bbp launch");
     HIPCALL(hipMemcpyAsync(h_vals, d_out, count, hipMemcpyDeviceToHost, st));
     HIPCALL(hipStreamSynchronize(st));
@@ -804,7 +809,7 @@ int bbp_few(const BbpCfg& cfg, const uint64_t* d_pos, const uint64_t* h_pos, uin
     }
     if (share < 0) share = d_pos && !h_pos ? 0.0 : lane_share(h_pos, start, 1, count);
     hipError_t e = launch_bbp_fx(d_pos, start, count, dmax, d_out, ws, cfg.exact == 2, st, 1, 0,
-                                 bbp_term_kernel(cfg, share, dmax));
+                                 bbp_term_kernel(cfg, share, dmax, count));
     if (e != hipSuccess) return fail_hip(e, "bbp launch");
     HIPCALL(hipMemcpyAsync(h_vals, d_out, count, hipMemcpyDeviceToHost, st));
     HIPCALL(hipStreamSynchronize(st));
@@ -1086,7 +1091,7 @@ int pdh_bbp_positions_dev(const uint64_t* d_positions, size_t n, uint8_t* d_valu
         }
         const double share = sc[2] ? (double)sc[1] / (double)sc[2] : 0.0;
         std::vector<uint8_t> tmp(n);
-        if (n > 64 && cfg.exact != 1 && bbp_term_kernel(cfg, share, sc[0]) == BBP_TK_MODULI) {
+        if (n > 64 && cfg.exact != 1 && bbp_term_kernel(cfg, share, sc[0], n) == BBP_TK_MODULI) {
             // moduli in lanes steps a lane from one sorted position of its
             // group to the next at a cost that grows with the gap: sort the
             // positions on the device, compute in that order, scatter back

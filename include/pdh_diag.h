@@ -32,7 +32,8 @@ uint64_t pdh_bbp_exact_count(void);
 /* pdh_bbp_set_layout(mode): the fixed-point path's term kernel. 0 (default)
  * chooses per call: moduli in lanes (64 terms k of one series per
  * wavefront, stepping through each group of 64 positions in sorted order)
- * when the largest position is at most 2^27 - 1; above, positions in lanes
+ * for more than 64 positions when the largest is at most 2^27 - 1; else
+ * positions in lanes
  * (one exponent per wavefront, its bits scalar) when at least three lanes
  * in four stay busy -- the positions of each group of 64, in the order
  * given, lie close together -- else terms in lanes. 1 forces terms in

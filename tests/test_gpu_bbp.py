@@ -243,15 +243,16 @@ def test_both_term_layouts_give_the_reference_digits(pdh, golden, layout, mode):
 
 
 def test_layout_choice(pdh, golden, layout):
-    """The default takes moduli in lanes up to position 2^27 - 1 (a
-    contiguous range, the 64 sampled positions); above it, positions in lanes
-    for close positions and terms in lanes for spread ones."""
+    """The default takes moduli in lanes for more than 64 positions up to
+    position 2^27 - 1 (a contiguous range); otherwise positions in lanes for
+    close positions and terms in lanes for spread ones (the 64 sampled
+    positions, and positions above 2^27)."""
     import torch
     layout(0)
     s = golden["bbp_sampled"]
     above = np.uint64((1 << 27) + 1)
     for positions, kernel in ((np.arange(1, 4097, dtype=np.uint64), "k_bbp_fx_moduli"),
-                              (np.array(s["positions"], dtype=np.uint64), "k_bbp_fx_moduli"),
+                              (np.array(s["positions"], dtype=np.uint64), "k_bbp_fx_terms"),
                               (above + np.arange(0, 256, dtype=np.uint64), "k_bbp_fx_lanes"),
                               (np.uint64(1) + np.arange(0, 64, dtype=np.uint64) * np.uint64(1 << 22),
                                "k_bbp_fx_terms")):
