@@ -135,6 +135,33 @@ def test_bad_positions_raise(pdh):
     assert out.cpu().tolist() == [2, 0xFF, 4, 0xFF]
 
 
+def test_bad_positions_in_a_sorted_batch(pdh, golden):
+    """More than 64 positions take the device sort and the scatter back:
+    invalid positions among them still come back as 0xFF at their own index,
+    with PDH_E_ARG, and the valid ones equal the reference's digits."""
+    import torch
+    rng = np.random.default_rng(5)
+    vals = np.concatenate([np.arange(1, 99, dtype=np.int64), np.array([0, (1 << 45) + 1], dtype=np.int64)])
+    perm = rng.permutation(len(vals))
+    pos = torch.from_numpy(vals[perm]).cuda()
+    out = torch.zeros(len(vals), dtype=torch.uint8, device="cuda")
+    torch.cuda.synchronize()
+    pdh.timing_enable(True)
+    with pytest.raises(pdh.PdhError, match="outside"):
+        pdh.bbp_positions_dev(pos.data_ptr(), len(vals), out.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    names = pdh.timing_read()
+    pdh.timing_enable(False)
+    assert "k_bbp_scatter" in names, names
+    want = open(os.path.join(GOLDEN_DIR, golden["bbp_range"]["file"])).read().strip()
+    got = out.cpu().numpy()
+    for i, p in enumerate(vals[perm]):
+        if 1 <= p <= 98:
+            assert got[i] == int(want[p - 1], 16), (i, p)
+        else:
+            assert got[i] == 0xFF, (i, p)
+
+
 @pytest.fixture
 def exact_mode(pdh):
     yield pdh.bbp_set_exact
